@@ -1,0 +1,970 @@
+// oracle.cpp -- CPU restatement of the reference Riemersma hot path.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).  The product never links this.
+//
+// Built with g++ -O2 -ffp-contract=off (oracle/Makefile): every double and
+// float operation below rounds exactly once, in the order written, which is
+// the order the reference writes them in.
+//
+// How each part is pinned (tests/test_oracle.py, tests/test_golden.py):
+//   * weights        -- exact IEEE hex of all 16 values (closed form 16^(-i/15))
+//                       and the reference's table; against oracle/_ref.
+//   * curve          -- closed form == explicit recursion for levels 0..9
+//                       (brute force); endpoint (2^L-1,0); permutation of the
+//                       grid; block-curve length pixels+1; unvisited-pixel rule;
+//                       against oracle/_ref BuildCurveIndex.
+//   * scale_char     -- exhaustive equality with a long-double reformulation
+//                       on every float in [0, 65536) (tests/test_oracle.py).
+//   * octree         -- reference KAT (tests/in_gray_alpha.png at 2 colours:
+//                       52 nodes, 1 colour 32612/32768); golden fixtures from
+//                       oracle/_ref ColorTree::Build on seeded images.
+//   * sequential     -- golden fixtures from oracle/_ref RiemersmaWalkCpu; a
+//                       hand-computed 1x1 walk; memo first-touch property.
+//   * blocks         -- golden fixtures from oracle/_ref RiemersmaBlocksCpu;
+//                       block >= n equals the cache-free sequential walk on
+//                       the block curve.
+//   * inverse filter -- triangular Toeplitz solve of (I-H) g = delta (numpy).
+//   * Jacobi         -- PARITY UNPINNED as a whole: the reference has no CPU
+//                       Jacobi and its CUDA engine cannot be built here.  Its
+//                       parts are pinned: the inverse filter (above), the lookup
+//                       (shared with the octree), the fixed-point property
+//                       (a sweep applied to the cache-free walk's indices with a
+//                       long filter returns those indices, up to clamping).
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr double kRange = 65535.0;       // QuantumRange
+constexpr double kScale = 1.0 / 65535.0; // QuantumScale
+constexpr double kEps = 1.0e-12;         // MagickEpsilon
+constexpr int kQueue = 16;               // ErrorQueueLength
+constexpr int kMaxDepth = 8;             // MaxTreeDepth
+constexpr int kMaxColormap = 65536;      // MaxColormapSize
+constexpr size_t kMaxQNodes = 266817;    // MaxQNodes
+
+struct D4 {
+  double c[4] = {0.0, 0.0, 0.0, 0.0};
+};
+
+// rd_types.h:30-33 MagickSafeReciprocal.
+double safe_recip(double x) {
+  const double sign = x < 0.0 ? -1.0 : 1.0;
+  return (sign * x) >= kEps ? 1.0 / x : sign * (1.0 / kEps);
+}
+
+// rd_types.h:39-43 ClampPixel -> float.
+float clamp_px(double v) {
+  if (v < 0.0) return 0.0f;
+  if (v >= kRange) return (float)kRange;
+  return (float)v;
+}
+
+// rd_types.h:133-138 ScaleQuantumToChar: float divide, float add, truncate.
+unsigned scale_char(float q) {
+  if (std::isnan(q) || q <= 0.0f) return 0;
+  const float s = q / 257.0f;
+  if (s >= 255.0f) return 255;
+  return (unsigned)(unsigned char)(s + 0.5f);
+}
+
+// rd_types.h:141-153 CacheOffset: 6 bits per channel.
+int memo_key(bool assoc, const D4& p) {
+  int k = (int)(scale_char(clamp_px(p.c[0])) >> 2) |
+          ((int)(scale_char(clamp_px(p.c[1])) >> 2) << 6) |
+          ((int)(scale_char(clamp_px(p.c[2])) >> 2) << 12);
+  if (assoc) k |= (int)(scale_char(clamp_px(p.c[3])) >> 2) << 18;
+  return k;
+}
+
+// rd_types.h:169-176: weight *= exp(log(16)/15), reciprocal stored.
+void raw_weights(double w[kQueue]) {
+  double weight = 1.0;
+  for (int i = 0; i < kQueue; ++i) {
+    w[i] = safe_recip(weight);
+    weight *= std::exp(std::log(1.0 / (1.0 / 16.0)) / (kQueue - 1.0));
+  }
+}
+
+// rd_riemersma_cpu.cpp:370-373: (1/16 * diffusion) * weights[k].
+void scaled_weights(double diffusion, double sw[kQueue]) {
+  double w[kQueue];
+  raw_weights(w);
+  for (int k = 0; k < kQueue; ++k) sw[k] = (1.0 / 16.0) * diffusion * w[k];
+}
+
+// rd_types.h:179-192 AssociateAlphaPixel (float pixel -> double).
+D4 assoc_px(bool assoc, const float* px) {
+  D4 o;
+  o.c[3] = (double)px[3];
+  if (!assoc || px[3] == (float)kRange) {
+    o.c[0] = (double)px[0];
+    o.c[1] = (double)px[1];
+    o.c[2] = (double)px[2];
+    return o;
+  }
+  const double a = kScale * (double)px[3];
+  o.c[0] = a * (double)px[0];
+  o.c[1] = a * (double)px[1];
+  o.c[2] = a * (double)px[2];
+  return o;
+}
+
+// rd_types.h:195-208 AssociateAlphaPixelInfo (palette entry).
+D4 assoc_info(bool assoc, const double* e) {
+  D4 o;
+  o.c[3] = e[3];
+  if (!assoc || e[3] == kRange) {
+    o.c[0] = e[0];
+    o.c[1] = e[1];
+    o.c[2] = e[2];
+    return o;
+  }
+  const double a = kScale * e[3];
+  o.c[0] = a * e[0];
+  o.c[1] = a * e[1];
+  o.c[2] = a * e[2];
+  return o;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Octree.  Restates src/rd_octree.cpp:39-409 (itself quantize.c's
+// Classify/Prune/Reduce/DefineImageColormap/ClosestColor).
+// ===========================================================================
+struct ONode {
+  int32_t child[16];
+  int32_t parent;
+  double total[4];
+  double qe;
+  uint64_t nu;
+  uint32_t color_number;
+  uint32_t id;
+  uint32_t level;
+};
+
+struct OrcTree {
+  std::vector<ONode> nodes;
+  std::vector<double> cmap;  // 4 doubles per entry
+  bool assoc = false;
+  int maxc = 0;
+  int depth = 3;
+  int count = 0;  // colour count (classification counter, then colormap size)
+  size_t live = 0;
+  double prune_thr = 0.0;
+  double next_thr = 0.0;
+
+  int kids() const { return assoc ? 16 : 8; }
+
+  int32_t add_node(uint32_t id, uint32_t level, int32_t parent) {
+    ONode n;
+    for (int i = 0; i < 16; ++i) n.child[i] = -1;
+    n.parent = parent;
+    for (int c = 0; c < 4; ++c) n.total[c] = 0.0;
+    n.qe = 0.0;
+    n.nu = 0;
+    n.color_number = 0;
+    n.id = id;
+    n.level = level;
+    nodes.push_back(n);
+    return (int32_t)nodes.size() - 1;
+  }
+
+  // rd_octree.cpp:50-67 PackChannels / NodeId.
+  void pack(const D4& p, unsigned c[4]) const {
+    for (int k = 0; k < 4; ++k) c[k] = scale_char(clamp_px(p.c[k]));
+  }
+  int node_id(const unsigned c[4], int index) const {
+    int id = (int)((c[0] >> index) & 1u);
+    id |= (int)((c[1] >> index) & 1u) << 1;
+    id |= (int)((c[2] >> index) & 1u) << 2;
+    if (assoc) id |= (int)((c[3] >> index) & 1u) << 3;
+    return id;
+  }
+
+  // rd_octree.cpp:69-93 PruneChild: gate on the live counter.
+  void prune_child(int32_t n) {
+    for (int i = 0; i < kids(); ++i) {
+      const int32_t ch = nodes[n].child[i];
+      if (ch >= 0) prune_child(ch);
+    }
+    if (live > (size_t)maxc) {
+      const int32_t p = nodes[n].parent;
+      if (p >= 0) {
+        nodes[p].nu += nodes[n].nu;
+        for (int c = 0; c < 4; ++c) nodes[p].total[c] += nodes[n].total[c];
+        nodes[p].child[nodes[n].id] = -1;
+      }
+      --live;
+    }
+  }
+  // rd_octree.cpp:95-102.
+  void prune_level(int32_t n) {
+    for (int i = 0; i < kids(); ++i) {
+      const int32_t ch = nodes[n].child[i];
+      if (ch >= 0) prune_level(ch);
+    }
+    if ((int)nodes[n].level == depth) prune_child(n);
+  }
+  // rd_octree.cpp:104-111.
+  void prune_to_cube(int32_t n) {
+    for (int i = 0; i < kids(); ++i) {
+      const int32_t ch = nodes[n].child[i];
+      if (ch >= 0) prune_to_cube(ch);
+    }
+    if ((int)nodes[n].level > depth) prune_child(n);
+  }
+  // rd_octree.cpp:113-127.
+  void reduce(int32_t n) {
+    for (int i = 0; i < kids(); ++i) {
+      const int32_t ch = nodes[n].child[i];
+      if (ch >= 0) reduce(ch);
+    }
+    if (nodes[n].qe <= prune_thr) {
+      prune_child(n);
+    } else {
+      if (nodes[n].nu > 0) ++count;
+      if (nodes[n].qe < next_thr) next_thr = nodes[n].qe;
+    }
+  }
+  // rd_octree.cpp:129-140: pre-order, cut off at the live counter.
+  size_t flatten(int32_t n, size_t off, std::vector<double>& out) const {
+    if (off >= live) return 0;
+    out[off] = nodes[n].qe;
+    size_t k = 1;
+    for (int i = 0; i < kids(); ++i) {
+      const int32_t ch = nodes[n].child[i];
+      if (ch >= 0) k += flatten(ch, off + k, out);
+    }
+    return k;
+  }
+  // rd_octree.cpp:142-176.
+  void define_colormap(int32_t n) {
+    for (int i = 0; i < kids(); ++i) {
+      const int32_t ch = nodes[n].child[i];
+      if (ch >= 0) define_colormap(ch);
+    }
+    ONode& q = nodes[n];
+    if (q.nu == 0) return;
+    if ((size_t)count >= cmap.size() / 4) cmap.resize((cmap.size() / 4 * 2 + 1) * 4, 0.0);
+    double* e = &cmap[4 * (size_t)count];
+    const double alpha = safe_recip((double)q.nu);
+    if (!assoc) {
+      e[0] = (double)(float)(alpha * kRange * q.total[0]);
+      e[1] = (double)(float)(alpha * kRange * q.total[1]);
+      e[2] = (double)(float)(alpha * kRange * q.total[2]);
+      e[3] = kRange;
+    } else {
+      const double opacity = alpha * kRange * q.total[3];
+      e[3] = (double)(float)opacity;
+      if (e[3] == kRange) {
+        e[0] = (double)(float)(alpha * kRange * q.total[0]);
+        e[1] = (double)(float)(alpha * kRange * q.total[1]);
+        e[2] = (double)(float)(alpha * kRange * q.total[2]);
+      } else {
+        const double gamma = safe_recip(kScale * e[3]);
+        e[0] = (double)(float)(alpha * gamma * kRange * q.total[0]);
+        e[1] = (double)(float)(alpha * gamma * kRange * q.total[1]);
+        e[2] = (double)(float)(alpha * gamma * kRange * q.total[2]);
+      }
+    }
+    q.color_number = (uint32_t)count;
+    ++count;
+  }
+  // rd_octree.cpp:178-217: post-order scan with per-channel early exits.
+  void closest(int32_t n, const D4& t, double* dist, int* cn) const {
+    for (int i = 0; i < kids(); ++i) {
+      const int32_t ch = nodes[n].child[i];
+      if (ch >= 0) closest(ch, t, dist, cn);
+    }
+    const ONode& q = nodes[n];
+    if (q.nu == 0) return;
+    const double* p = &cmap[4 * (size_t)q.color_number];
+    double alpha = 1.0, beta = 1.0;
+    if (assoc) {
+      alpha = kScale * p[3];
+      beta = kScale * t.c[3];
+    }
+    double v = alpha * p[0] - beta * t.c[0];
+    double d = v * v;
+    if (d <= *dist) {
+      v = alpha * p[1] - beta * t.c[1];
+      d += v * v;
+      if (d <= *dist) {
+        v = alpha * p[2] - beta * t.c[2];
+        d += v * v;
+        if (d <= *dist) {
+          if (assoc) {
+            v = p[3] - t.c[3];
+            d += v * v;
+          }
+          if (d <= *dist) {
+            *dist = d;
+            *cn = (int)q.color_number;
+          }
+        }
+      }
+    }
+  }
+  // rd_riemersma_cpu.cpp:82-92: descend over bit 7..1, search parent's subtree.
+  int lookup(const D4& t) const {
+    unsigned c[4];
+    pack(t, c);
+    int32_t n = 0;
+    for (int i = kMaxDepth - 1; i > 0; --i) {
+      const int32_t ch = nodes[n].child[node_id(c, i)];
+      if (ch < 0) break;
+      n = ch;
+    }
+    double dist = 4.0 * ((kRange + 1.0) * (kRange + 1.0)) + 1.0;
+    int idx = 0;
+    closest(nodes[n].parent, t, &dist, &idx);
+    return idx;
+  }
+
+  // rd_octree.cpp:305-362: one scanline, run-length encoded.
+  void classify_row(const float* row, int64_t width, int level_limit, int colors_level) {
+    const double midpoint = kRange / 2.0;
+    int64_t x = 0;
+    while (x < width) {
+      int64_t run = 1;
+      while (x + run < width && row[4 * x] == row[4 * (x + run)] &&
+             row[4 * x + 1] == row[4 * (x + run) + 1] &&
+             row[4 * x + 2] == row[4 * (x + run) + 2] &&
+             row[4 * x + 3] == row[4 * (x + run) + 3])
+        ++run;
+      const D4 px = assoc_px(assoc, row + 4 * x);
+      unsigned c[4];
+      pack(px, c);
+      int index = kMaxDepth - 1;
+      double bisect = (kRange + 1.0) / 2.0;
+      double mid[4] = {midpoint, midpoint, midpoint, midpoint};
+      int32_t n = 0;
+      for (int level = 1; level <= level_limit; ++level) {
+        bisect *= 0.5;
+        const int id = node_id(c, index);
+        for (int k = 0; k < 4; ++k) mid[k] += (id & (1 << k)) ? bisect : -bisect;
+        if (nodes[n].child[id] < 0) {
+          const int32_t made = add_node((uint32_t)id, (uint32_t)level, n);
+          nodes[n].child[id] = made;
+          ++live;
+          if (level == colors_level) ++count;
+        }
+        n = nodes[n].child[id];
+        const double e0 = kScale * (px.c[0] - mid[0]);
+        const double e1 = kScale * (px.c[1] - mid[1]);
+        const double e2 = kScale * (px.c[2] - mid[2]);
+        const double e3 = assoc ? kScale * (px.c[3] - mid[3]) : 0.0;
+        double dist = e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
+        if (std::isnan(dist)) dist = 0.0;
+        nodes[n].qe += (double)run * std::sqrt(dist);
+        nodes[0].qe += nodes[n].qe;  // IM's double count into the root
+        --index;
+      }
+      ONode& leaf = nodes[n];
+      leaf.nu += (uint64_t)run;
+      leaf.total[0] += (double)run * kScale * (double)clamp_px(px.c[0]);
+      leaf.total[1] += (double)run * kScale * (double)clamp_px(px.c[1]);
+      leaf.total[2] += (double)run * kScale * (double)clamp_px(px.c[2]);
+      leaf.total[3] += (double)run * kScale * (double)clamp_px(assoc ? px.c[3] : kRange);
+      x += run;
+    }
+  }
+};
+
+namespace {
+
+int quantize_error_cmp(const void* a, const void* b) {  // rd_octree.cpp:23-29
+  const double p = *(const double*)a;
+  const double q = *(const double*)b;
+  if (p > q) return 1;
+  if (std::fabs(q - p) <= kEps) return 0;
+  return -1;
+}
+
+}  // namespace
+
+extern "C" OrcTree* orc_tree_build(const float* rgba, int64_t w, int64_t h,
+                                   int max_colors, int assoc, int gray) {
+  OrcTree* t = new OrcTree();
+  t->cmap.assign((size_t)kMaxColormap * 4, 0.0);
+  for (size_t i = 0; i < (size_t)kMaxColormap; ++i) t->cmap[4 * i + 3] = kRange;
+  t->assoc = assoc != 0;
+  t->maxc = max_colors <= 0 ? kMaxColormap : std::min(max_colors, kMaxColormap);
+  // rd_octree.cpp:255-275: depth = Log4(colours)+2, adjusted.
+  int depth = 1;
+  for (size_t c = (size_t)t->maxc; c != 0; ++depth) c >>= 2;
+  if (depth > 2) --depth;
+  if (t->assoc && depth > 5) --depth;
+  if (gray) depth = kMaxDepth;
+  if (depth > kMaxDepth) depth = kMaxDepth;
+  if (depth < 2) depth = 2;
+  t->depth = depth;
+  t->add_node(0, 0, -1);
+  t->live = 1;
+  t->nodes[0].parent = 0;  // root is its own parent (rd_octree.cpp:285)
+
+  int64_t y = 0;
+  for (; y < h; ++y) {  // loop 1: full depth, colours counted at level 8
+    if (t->live > kMaxQNodes) {
+      t->prune_level(0);
+      --t->depth;
+    }
+    t->classify_row(rgba + 4 * y * w, w, kMaxDepth, kMaxDepth);
+    if (t->count > t->maxc) {
+      t->prune_to_cube(0);
+      break;
+    }
+  }
+  ++y;
+  for (; y < h; ++y) {  // loop 2: cube depth
+    if (t->live > kMaxQNodes) {
+      t->prune_level(0);
+      --t->depth;
+    }
+    t->classify_row(rgba + 4 * y * w, w, t->depth, t->depth);
+  }
+
+  // rd_octree.cpp:388-404 ReduceImageColors.
+  if (t->count > t->maxc) {
+    t->next_thr = 0.0;
+    if (t->live > 0) {
+      std::vector<double> err(t->live, 0.0);
+      (void)t->flatten(0, 0, err);
+      const size_t n = t->live;
+      std::qsort(err.data(), n, sizeof(double), quantize_error_cmp);
+      const size_t aggressive = (110u * ((size_t)t->maxc + 1u)) / 100u;
+      if (n > aggressive) t->next_thr = err[n - aggressive];
+    }
+  }
+  while (t->count > t->maxc) {
+    t->prune_thr = t->next_thr;
+    t->next_thr = t->nodes[0].qe - 1.0;
+    t->count = 0;
+    t->reduce(0);
+  }
+  t->count = 0;
+  t->define_colormap(0);
+  return t;
+}
+
+extern "C" void orc_tree_free(OrcTree* t) { delete t; }
+extern "C" int orc_tree_color_count(const OrcTree* t) { return t->count; }
+extern "C" int64_t orc_tree_node_count(const OrcTree* t) { return (int64_t)t->nodes.size(); }
+extern "C" int orc_tree_depth(const OrcTree* t) { return t->depth; }
+extern "C" int orc_tree_assoc(const OrcTree* t) { return t->assoc ? 1 : 0; }
+
+extern "C" void orc_tree_colormap(const OrcTree* t, double* out) {
+  std::memcpy(out, t->cmap.data(), sizeof(double) * 4 * (size_t)t->count);
+}
+
+extern "C" void orc_tree_nodes(const OrcTree* t, int32_t* child16, int32_t* parent,
+                               uint32_t* color_number, uint64_t* number_unique,
+                               double* total4, double* qe, uint32_t* id,
+                               uint32_t* level) {
+  for (size_t i = 0; i < t->nodes.size(); ++i) {
+    const ONode& n = t->nodes[i];
+    if (child16) for (int c = 0; c < 16; ++c) child16[16 * i + c] = n.child[c];
+    if (parent) parent[i] = n.parent;
+    if (color_number) color_number[i] = n.color_number;
+    if (number_unique) number_unique[i] = n.nu;
+    if (total4) for (int c = 0; c < 4; ++c) total4[4 * i + c] = n.total[c];
+    if (qe) qe[i] = n.qe;
+    if (id) id[i] = n.id;
+    if (level) level[i] = n.level;
+  }
+}
+
+extern "C" int orc_tree_lookup(const OrcTree* t, const double target[4]) {
+  D4 d;
+  for (int c = 0; c < 4; ++c) d.c[c] = target[c];
+  return t->lookup(d);
+}
+
+// ===========================================================================
+// Numeric contract exports.
+// ===========================================================================
+extern "C" void orc_error_weights(double out[16]) { raw_weights(out); }
+extern "C" void orc_scaled_weights(double diffusion, double out[16]) {
+  scaled_weights(diffusion, out);
+}
+extern "C" unsigned orc_scale_char(float q) { return scale_char(q); }
+extern "C" float orc_clamp_pixel(double v) { return clamp_px(v); }
+
+// ===========================================================================
+// Curve.
+// ===========================================================================
+// rd_riemersma.h:88-94: level = ceil(log2(max(w,h))), via log2 on the host.
+extern "C" int orc_curve_level(int64_t w, int64_t h) {
+  const int64_t extent = w > h ? w : h;
+  if (extent <= 0) return 0;
+  int level = (int)std::log2((double)extent);
+  if (((int64_t)1 << level) < extent) ++level;
+  return level;
+}
+
+// rd_riemersma_cpu.cpp:167-189: textbook d2xy (the reference's orientation).
+extern "C" void orc_hilbert_point(int64_t n, int level, int* ox, int* oy) {
+  int64_t x = 0, y = 0;
+  const int64_t span = (int64_t)1 << level;
+  for (int64_t s = 1; s < span; s <<= 1) {
+    const int64_t rx = 1 & (n / 2);
+    const int64_t ry = 1 & (n ^ rx);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = s - 1 - x;
+        y = s - 1 - y;
+      }
+      std::swap(x, y);
+    }
+    x += s * rx;
+    y += s * ry;
+    n /= 4;
+  }
+  *ox = (int)x;
+  *oy = (int)y;
+}
+
+namespace {
+
+// IM's Riemersma() recursion, as the sequence of moves per (level, heading).
+// Directions: 0=W 1=E 2=N 3=S.  At level 1 a heading expands into three
+// visits (each followed by a move); above level 1 into
+// R(a) D(m1) R(b) D(m2) R(c) D(m3) R(d).  Same schedule as the reference's
+// tables (rd_riemersma.h:103-114), written as strings here.
+const char* kLeafMoves[4] = {"ESW", "WNE", "SEN", "NWS"};
+const char* kNodeSteps[4] = {"NeWsWwS", "SwEnEeN", "WsNeNnE", "EnSwSsW"};
+
+int dir_of(char c) {
+  switch (c) {
+    case 'W': case 'w': return 0;
+    case 'E': case 'e': return 1;
+    case 'N': case 'n': return 2;
+    default: return 3;
+  }
+}
+
+template <typename Fn>
+void recurse(int level, int heading, int* x, int* y, Fn&& visit) {
+  auto move = [&](int d) {
+    if (d == 0) --*x;
+    else if (d == 1) ++*x;
+    else if (d == 2) --*y;
+    else ++*y;
+  };
+  if (level == 1) {
+    for (int k = 0; k < 3; ++k) {
+      visit(*x, *y);
+      move(dir_of(kLeafMoves[heading][k]));
+    }
+    return;
+  }
+  const char* s = kNodeSteps[heading];
+  for (int k = 0; k < 7; ++k) {
+    const char c = s[k];
+    if (c >= 'A' && c <= 'Z') {
+      recurse(level - 1, dir_of(c), x, y, visit);
+    } else {
+      visit(*x, *y);
+      move(dir_of(c));
+    }
+  }
+}
+
+// Visit sequence of the sequential walk: the recursion started heading North,
+// then the trailing ForgetGravity visit at the resting cursor
+// (rd_riemersma_cpu.cpp:403-416).
+template <typename Fn>
+void walk_order(int level, Fn&& visit) {
+  int x = 0, y = 0;
+  if (level > 0) recurse(level, 2, &x, &y, visit);
+  visit(x, y);
+}
+
+}  // namespace
+
+extern "C" int64_t orc_recursive_walk(int level, int* xs, int* ys) {
+  int64_t k = 0;
+  walk_order(level, [&](int x, int y) {
+    xs[k] = x;
+    ys[k] = y;
+    ++k;
+  });
+  return k;
+}
+
+extern "C" int64_t orc_seq_order(int64_t w, int64_t h, int32_t* order) {
+  const int level = orc_curve_level(w, h);
+  int64_t k = 0;
+  walk_order(level, [&](int x, int y) {
+    if (x >= 0 && y >= 0 && x < w && y < h) order[k++] = (int32_t)(y * w + x);
+  });
+  return k;
+}
+
+// rd_riemersma_cpu.cpp:269-356: in-bounds of [0, 4^L-1) in ascending n, then
+// the origin, owner = last writer.
+extern "C" int64_t orc_block_order(int64_t w, int64_t h, int32_t* order, int32_t* owner) {
+  const int level = orc_curve_level(w, h);
+  for (int64_t p = 0; p < w * h; ++p) owner[p] = -1;
+  if (level <= 0) return 0;
+  const int64_t leaves = ((int64_t)1 << (2 * level)) - 1;
+  int64_t k = 0;
+  for (int64_t n = 0; n < leaves; ++n) {
+    int x, y;
+    orc_hilbert_point(n, level, &x, &y);
+    if (x < w && y < h) order[k++] = (int32_t)(y * w + x);
+  }
+  order[k++] = 0;
+  for (int64_t i = 0; i < k; ++i) owner[order[i]] = (int32_t)i;
+  return k;
+}
+
+// ===========================================================================
+// Sequential walk.  rd_riemersma_cpu.cpp:39-113 (Visit) and :359-417.
+// ===========================================================================
+namespace {
+
+struct SeqState {
+  const OrcTree* t;
+  double sw[kQueue];
+  D4 err[kQueue];
+  std::vector<int> memo;
+  bool use_cache;
+};
+
+int visit(SeqState& s, float* px) {
+  const OrcTree& t = *s.t;
+  const bool assoc = t.assoc;
+  D4 p = assoc_px(assoc, px);
+  for (int i = 0; i < kQueue; ++i) {
+    const double w = s.sw[i];
+    p.c[0] += w * s.err[i].c[0];
+    p.c[1] += w * s.err[i].c[1];
+    p.c[2] += w * s.err[i].c[2];
+    if (assoc) p.c[3] += w * s.err[i].c[3];
+  }
+  p.c[0] = (double)clamp_px(p.c[0]);
+  p.c[1] = (double)clamp_px(p.c[1]);
+  p.c[2] = (double)clamp_px(p.c[2]);
+  if (assoc) p.c[3] = (double)clamp_px(p.c[3]);
+  const int key = memo_key(assoc, p);
+  int idx = s.use_cache ? s.memo[(size_t)key] : -1;
+  if (idx < 0) {
+    idx = t.lookup(p);
+    if (s.use_cache) s.memo[(size_t)key] = idx;
+  }
+  const double* e = &t.cmap[4 * (size_t)idx];
+  px[0] = (float)e[0];
+  px[1] = (float)e[1];
+  px[2] = (float)e[2];
+  if (assoc) px[3] = (float)e[3];
+  for (int i = 0; i < kQueue - 1; ++i) s.err[i] = s.err[i + 1];
+  const D4 col = assoc_info(assoc, e);
+  s.err[kQueue - 1].c[0] = p.c[0] - col.c[0];
+  s.err[kQueue - 1].c[1] = p.c[1] - col.c[1];
+  s.err[kQueue - 1].c[2] = p.c[2] - col.c[2];
+  if (assoc) s.err[kQueue - 1].c[3] = p.c[3] - col.c[3];
+  return idx;
+}
+
+void walk_seq_impl(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                   double diffusion, int use_cache, int32_t* index_out) {
+  SeqState s;
+  s.t = t;
+  scaled_weights(diffusion, s.sw);
+  s.use_cache = use_cache != 0;
+  if (s.use_cache) s.memo.assign(t->assoc ? ((size_t)1 << 24) : ((size_t)1 << 18), -1);
+  const int level = orc_curve_level(w, h);
+  int64_t k = 0;
+  walk_order(level, [&](int x, int y) {
+    if (x < 0 || y < 0 || x >= w || y >= h) return;
+    const int idx = visit(s, rgba + 4 * ((int64_t)y * w + x));
+    if (index_out) index_out[k] = idx;
+    ++k;
+  });
+}
+
+}  // namespace
+
+extern "C" void orc_walk_seq(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                             double diffusion, int use_cache) {
+  walk_seq_impl(t, rgba, w, h, diffusion, use_cache, nullptr);
+}
+
+extern "C" void orc_walk_seq_index(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                                   double diffusion, int use_cache, int32_t* index_out) {
+  walk_seq_impl(t, rgba, w, h, diffusion, use_cache, index_out);
+}
+
+extern "C" int orc_walk_seq_many(OrcTree* const* trees, float* const* images,
+                                 const int64_t* ws, const int64_t* hs, int n,
+                                 double diffusion, int threads) {
+  std::vector<int> ord((size_t)n);
+  for (int i = 0; i < n; ++i) ord[(size_t)i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    return ws[a] * hs[a] > ws[b] * hs[b];
+  });
+  std::vector<std::thread> pool;
+  std::vector<int> next(1, 0);
+  std::mutex* mu = new std::mutex();
+  if (threads < 1) threads = 1;
+  for (int k = 0; k < threads; ++k) {
+    pool.emplace_back([&]() {
+      for (;;) {
+        int j;
+        {
+          std::lock_guard<std::mutex> g(*mu);
+          if (next[0] >= n) return;
+          j = ord[(size_t)next[0]++];
+        }
+        walk_seq_impl(trees[j], images[j], ws[j], hs[j], diffusion, 1, nullptr);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  delete mu;
+  return 0;
+}
+
+// ===========================================================================
+// Block engine.  rd_riemersma_cpu.cpp:477-805 (unfused path and the rgba64
+// fused scatter).  CPU semantics for associated alpha (one association).
+// ===========================================================================
+namespace {
+
+int blocks_indices(const OrcTree* t, const float* batch, int64_t w, int64_t h,
+                   int frames, int block, double diffusion,
+                   std::vector<int32_t>& curve, std::vector<int32_t>& owner,
+                   std::vector<uint8_t>& chosen) {
+  if (t->count > 256) return -1;
+  if (frames < 1 || block < kQueue) return -1;
+  double sw[kQueue];
+  scaled_weights(diffusion, sw);
+  curve.assign((size_t)(w * h + 1), 0);
+  owner.assign((size_t)(w * h), -1);
+  const int64_t n = orc_block_order(w, h, curve.data(), owner.data());
+  curve.resize((size_t)n);
+  chosen.assign((size_t)n * (size_t)frames, 0);
+  const bool assoc = t->assoc;
+  const int64_t pixels = w * h;
+  const int64_t nblocks = (n + block - 1) / block;
+  for (int f = 0; f < frames; ++f) {
+    const float* frame = batch + 4 * (size_t)f * (size_t)pixels;
+    for (int64_t b = 0; b < nblocks; ++b) {
+      D4 err[kQueue];
+      const int64_t begin = b * block;
+      const int64_t end = std::min(begin + (int64_t)block, n);
+      for (int64_t i = begin; i < end; ++i) {
+        D4 p = assoc_px(assoc, frame + 4 * (size_t)curve[(size_t)i]);
+        for (int k = 0; k < kQueue; ++k) {
+          const double wk = sw[k];
+          p.c[0] += wk * err[k].c[0];
+          p.c[1] += wk * err[k].c[1];
+          p.c[2] += wk * err[k].c[2];
+          if (assoc) p.c[3] += wk * err[k].c[3];
+        }
+        p.c[0] = (double)clamp_px(p.c[0]);
+        p.c[1] = (double)clamp_px(p.c[1]);
+        p.c[2] = (double)clamp_px(p.c[2]);
+        if (assoc) p.c[3] = (double)clamp_px(p.c[3]);
+        const int idx = t->lookup(p);
+        chosen[(size_t)f * (size_t)n + (size_t)i] = (uint8_t)idx;
+        for (int k = 0; k < kQueue - 1; ++k) err[k] = err[k + 1];
+        const D4 col = assoc_info(assoc, &t->cmap[4 * (size_t)idx]);
+        err[kQueue - 1].c[0] = p.c[0] - col.c[0];
+        err[kQueue - 1].c[1] = p.c[1] - col.c[1];
+        err[kQueue - 1].c[2] = p.c[2] - col.c[2];
+        if (assoc) err[kQueue - 1].c[3] = p.c[3] - col.c[3];
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int orc_walk_blocks(const OrcTree* t, float* batch, int64_t w, int64_t h,
+                               int frames, int block, double diffusion,
+                               uint8_t* index_out) {
+  std::vector<int32_t> curve, owner;
+  std::vector<uint8_t> chosen;
+  if (blocks_indices(t, batch, w, h, frames, block, diffusion, curve, owner, chosen) != 0)
+    return -1;
+  const int64_t n = (int64_t)curve.size();
+  const int64_t pixels = w * h;
+  for (int f = 0; f < frames; ++f) {
+    float* frame = batch + 4 * (size_t)f * (size_t)pixels;
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t pix = curve[(size_t)i];
+      if (owner[(size_t)pix] != i) continue;
+      const double* e = &t->cmap[4 * (size_t)chosen[(size_t)f * (size_t)n + (size_t)i]];
+      frame[4 * pix + 0] = (float)e[0];
+      frame[4 * pix + 1] = (float)e[1];
+      frame[4 * pix + 2] = (float)e[2];
+      if (t->assoc) frame[4 * pix + 3] = (float)e[3];
+    }
+  }
+  if (index_out) std::memcpy(index_out, chosen.data(), chosen.size());
+  return 0;
+}
+
+extern "C" int orc_walk_blocks_u16(const OrcTree* t, const float* batch, int64_t w,
+                                   int64_t h, int frames, int block, double diffusion,
+                                   uint16_t* out) {
+  std::vector<int32_t> curve, owner;
+  std::vector<uint8_t> chosen;
+  if (blocks_indices(t, batch, w, h, frames, block, diffusion, curve, owner, chosen) != 0)
+    return -1;
+  const int64_t n = (int64_t)curve.size();
+  const int64_t pixels = w * h;
+  for (int f = 0; f < frames; ++f) {
+    const float* src = batch + 4 * (size_t)f * (size_t)pixels;
+    uint16_t* u = out + 4 * (size_t)f * (size_t)pixels;
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t pix = curve[(size_t)i];
+      if (owner[(size_t)pix] != i) continue;
+      const double* e = &t->cmap[4 * (size_t)chosen[(size_t)f * (size_t)n + (size_t)i]];
+      u[4 * pix + 0] = (uint16_t)(float)e[0];
+      u[4 * pix + 1] = (uint16_t)(float)e[1];
+      u[4 * pix + 2] = (uint16_t)(float)e[2];
+      u[4 * pix + 3] = t->assoc ? (uint16_t)(float)e[3] : (uint16_t)src[4 * pix + 3];
+    }
+    for (int64_t pix = 0; pix < pixels; ++pix) {  // unvisited: source value
+      if (owner[(size_t)pix] >= 0) continue;
+      for (int c = 0; c < 4; ++c) u[4 * pix + c] = (uint16_t)src[4 * pix + c];
+    }
+  }
+  return 0;
+}
+
+// ===========================================================================
+// Jacobi approximation.  src/rd_approx_cuda.cu:63-384 restated on the CPU.
+// ===========================================================================
+extern "C" void orc_inverse_filter(int taps, double diffusion, double* g) {
+  double w[kQueue];
+  raw_weights(w);
+  g[0] = 1.0;
+  for (int d = 1; d <= taps; ++d) {
+    double acc = 0.0;
+    const int kmax = d < kQueue ? d : kQueue;
+    for (int k = 1; k <= kmax; ++k) acc += (diffusion / 16.0) * w[k - 1] * g[d - k];
+    g[d] = acc;
+  }
+}
+
+namespace {
+
+// rd_cuda_common.cuh:57-72: the clamped double target, associated from double.
+D4 assoc_from_double(bool assoc, double r, double g, double b, double a) {
+  D4 o;
+  const double ca = (double)clamp_px(a);
+  o.c[3] = ca;
+  if (!assoc || a == kRange) {
+    o.c[0] = (double)clamp_px(r);
+    o.c[1] = (double)clamp_px(g);
+    o.c[2] = (double)clamp_px(b);
+    return o;
+  }
+  const double alpha = kScale * ca;
+  o.c[0] = alpha * (double)clamp_px(r) + 0.0;
+  o.c[1] = alpha * (double)clamp_px(g) + 0.0;
+  o.c[2] = alpha * (double)clamp_px(b) + 0.0;
+  return o;
+}
+
+// rd_cuda_common.cuh:451-476: plain ascending scan, last exact tie wins.
+int nearest_linear(const OrcTree* t, const D4& tg) {
+  double dist = 4.0 * ((kRange + 1.0) * (kRange + 1.0)) + 1.0;
+  int best = 0;
+  const double beta = kScale * tg.c[3];
+  for (int i = 0; i < t->count; ++i) {
+    const double* e = &t->cmap[4 * (size_t)i];
+    const double alpha = t->assoc ? kScale * e[3] : 1.0;
+    const double dr = alpha * e[0] + -beta * tg.c[0];
+    const double dg = alpha * e[1] + -beta * tg.c[1];
+    const double db = alpha * e[2] + -beta * tg.c[2];
+    double d = dr * dr + (dg * dg + db * db);
+    if (t->assoc) {
+      const double da = e[3] + -tg.c[3];
+      d = d + da * da;
+    }
+    if (d <= dist) {
+      dist = d;
+      best = i;
+    }
+  }
+  return best;
+}
+
+template <typename T>
+int jacobi_impl(const OrcTree* t, float* rgba, int64_t w, int64_t h, int iterations,
+                int taps, int use_tree, double diffusion, uint8_t* q_out) {
+  if (t->count < 1 || t->count > 256) return -1;
+  if (taps < 1 || taps > 4096) return -1;
+  const bool assoc = t->assoc;
+  std::vector<int32_t> curve((size_t)(w * h + 1)), owner((size_t)(w * h));
+  const int64_t n = orc_block_order(w, h, curve.data(), owner.data());
+  std::vector<double> gd((size_t)taps + 1);
+  orc_inverse_filter(taps, diffusion, gd.data());
+  std::vector<T> g((size_t)taps + 1);
+  for (size_t i = 0; i < g.size(); ++i) g[i] = (T)gd[i];
+  std::vector<T> cx((size_t)n * 4);
+  for (int64_t i = 0; i < n; ++i)  // GatherKernelT
+    for (int c = 0; c < 4; ++c) cx[4 * (size_t)i + c] = (T)rgba[4 * (size_t)curve[(size_t)i] + c];
+  std::vector<uint8_t> q0((size_t)n), q1((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {  // InitIndexKernel
+    float v[4];
+    for (int c = 0; c < 4; ++c) v[c] = (float)cx[4 * (size_t)i + c];
+    const D4 tg = assoc_px(assoc, v);
+    q0[(size_t)i] = (uint8_t)(use_tree ? t->lookup(tg) : nearest_linear(t, tg));
+  }
+  for (int it = 0; it < iterations; ++it) {  // JacobiStepKernel
+    for (int64_t i = 0; i < n; ++i) {
+      T acc[4] = {T(0), T(0), T(0), T(0)};
+      const int64_t dmax = taps < i ? taps : i;
+      for (int64_t d = 0; d <= dmax; ++d) {
+        const T* xs = &cx[4 * (size_t)(i - d)];
+        const double* p = &t->cmap[4 * (size_t)q0[(size_t)(i - d)]];
+        const T gk = g[(size_t)d];
+        for (int c = 0; c < 4; ++c) acc[c] += gk * (xs[c] - (T)p[c]);
+      }
+      const double* p0 = &t->cmap[4 * (size_t)q0[(size_t)i]];
+      const double vr = p0[0] + (double)acc[0];
+      const double vg = p0[1] + (double)acc[1];
+      const double vb = p0[2] + (double)acc[2];
+      const double va = assoc ? p0[3] + (double)acc[3] : kRange;
+      const D4 tg = assoc_from_double(assoc, vr, vg, vb, va);
+      q1[(size_t)i] = (uint8_t)(use_tree ? t->lookup(tg) : nearest_linear(t, tg));
+    }
+    std::swap(q0, q1);
+  }
+  for (int64_t i = 0; i < n; ++i) {  // ScatterKernel (owner-filtered)
+    const int32_t pix = curve[(size_t)i];
+    if (owner[(size_t)pix] != i) continue;
+    const double* p = &t->cmap[4 * (size_t)q0[(size_t)i]];
+    rgba[4 * pix + 0] = (float)p[0];
+    rgba[4 * pix + 1] = (float)p[1];
+    rgba[4 * pix + 2] = (float)p[2];
+    rgba[4 * pix + 3] = assoc ? (float)p[3] : (float)65535.0;
+  }
+  if (q_out) std::memcpy(q_out, q0.data(), (size_t)n);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int orc_jacobi(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                          int iterations, int taps, int fp64, int use_tree,
+                          double diffusion, uint8_t* q_out) {
+  if (fp64) return jacobi_impl<double>(t, rgba, w, h, iterations, taps, use_tree, diffusion, q_out);
+  return jacobi_impl<float>(t, rgba, w, h, iterations, taps, use_tree, diffusion, q_out);
+}

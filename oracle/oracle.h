@@ -1,0 +1,109 @@
+/* oracle.h -- CPU restatement of the reference Riemersma hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (stipple_amd/) links,
+ * loads or calls this code.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, and only as the checker or as the
+ * timed CPU baseline -- never as the thing measured or shipped.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * the reference repository root).  Arithmetic order and float width follow
+ * the reference exactly; the library is built with -O2 -ffp-contract=off and
+ * no fast-math so every double/float operation rounds once, as there.
+ *
+ * Parity status of each function (how it is pinned) is listed in oracle.cpp's
+ * header comment.
+ */
+#ifndef STIPPLE_ORACLE_H_
+#define STIPPLE_ORACLE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- numeric contract (include/rd_types.h) ---------------------------- */
+void orc_error_weights(double out[16]);                   /* rd_types.h:169-176 */
+void orc_scaled_weights(double diffusion, double out[16]); /* rd_riemersma_cpu.cpp:370-373 */
+unsigned orc_scale_char(float q);                         /* rd_types.h:133-138 */
+float orc_clamp_pixel(double v);                          /* rd_types.h:39-43 */
+
+/* ---- curve (include/rd_riemersma.h:88-157, src/rd_riemersma_cpu.cpp) ---- */
+int orc_curve_level(int64_t w, int64_t h);                /* rd_riemersma.h:88-94 */
+void orc_hilbert_point(int64_t n, int level, int* x, int* y); /* rd_riemersma_cpu.cpp:167-189 */
+/* The explicit recursion of IM's Riemersma() (rd_riemersma_cpu.cpp:126-153):
+ * writes the 4^level leaf positions followed by the trailing ForgetGravity
+ * visit.  Returns the number of entries (4^level, or 1 at level 0). */
+int64_t orc_recursive_walk(int level, int* xs, int* ys);
+/* In-bounds pixel ids of the sequential walk in visit order (w*h of them). */
+int64_t orc_seq_order(int64_t w, int64_t h, int32_t* order);
+/* Block-engine curve: in-bounds of n in [0,4^L-1) plus a trailing origin, and
+ * the last-writer owner map (rd_riemersma_cpu.cpp:269-356).  order must hold
+ * w*h+1 entries, owner w*h.  Returns the curve length. */
+int64_t orc_block_order(int64_t w, int64_t h, int32_t* order, int32_t* owner);
+
+/* ---- colour octree (src/rd_octree.cpp) -------------------------------- */
+typedef struct OrcTree OrcTree;
+/* rgba: w*h*4 floats, row-major, Q16.  rd_octree.cpp:219-409 */
+OrcTree* orc_tree_build(const float* rgba, int64_t w, int64_t h, int max_colors,
+                        int assoc, int gray);
+void orc_tree_free(OrcTree* t);
+int orc_tree_color_count(const OrcTree* t);
+int64_t orc_tree_node_count(const OrcTree* t);
+int orc_tree_depth(const OrcTree* t);
+int orc_tree_assoc(const OrcTree* t);
+/* colormap as count*4 doubles (r,g,b,a). */
+void orc_tree_colormap(const OrcTree* t, double* out);
+/* node arrays, each node_count long (child: 16 per node). */
+void orc_tree_nodes(const OrcTree* t, int32_t* child16, int32_t* parent,
+                    uint32_t* color_number, uint64_t* number_unique,
+                    double* total4, double* qe, uint32_t* id, uint32_t* level);
+/* Descent + ClosestColor for one clamped target (rd_riemersma_cpu.cpp:82-92). */
+int orc_tree_lookup(const OrcTree* t, const double target[4]);
+
+/* ---- engines ---------------------------------------------------------- */
+/* RiemersmaWalkCpu (rd_riemersma_cpu.cpp:359-417), palette = tree colormap.
+ * rgba is dithered in place. */
+void orc_walk_seq(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                  double diffusion, int use_cache);
+/* Same walk, but also records the palette index chosen at every visit in
+ * visit order (w*h entries).  Used to check index outputs. */
+void orc_walk_seq_index(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                        double diffusion, int use_cache, int32_t* index_out);
+/* RiemersmaBlocksCpu unfused path (rd_riemersma_cpu.cpp:477-805).  batch is
+ * frames*w*h*4 floats dithered in place; index_out (frames*n bytes, n = block
+ * curve length) receives the chosen index per curve position when non-null.
+ * Returns 0, or -1 for the reference's refusals (count>256, frames<1, B<16). */
+int orc_walk_blocks(const OrcTree* t, float* batch, int64_t w, int64_t h,
+                    int frames, int block, double diffusion,
+                    uint8_t* index_out);
+/* rgba64le output of the fused scatter (rd_riemersma_cpu.cpp:739-752, 794-801):
+ * frames*w*h*4 uint16 from the same walk as orc_walk_blocks.  batch is read
+ * only. */
+int orc_walk_blocks_u16(const OrcTree* t, const float* batch, int64_t w,
+                        int64_t h, int frames, int block, double diffusion,
+                        uint16_t* out_u16);
+
+/* Inverse filter g[0..taps] (src/rd_approx_cuda.cu:73-88). */
+void orc_inverse_filter(int taps, double diffusion, double* g);
+/* The Jacobi engine restated on the CPU (src/rd_approx_cuda.cu:90-384):
+ * gather, InitIndex, `iterations` sweeps, owner-filtered scatter into rgba.
+ * fp64 selects the double field; use_tree the octree lookup.  q_out (n bytes,
+ * n = block-curve length) receives the final index per curve position when
+ * non-null.  Returns 0 or -1 (count>256, taps outside 1..4096). */
+int orc_jacobi(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+               int iterations, int taps, int fp64, int use_tree,
+               double diffusion, uint8_t* q_out);
+
+/* Thread-pool CPU baseline: walks `n` independent images (each with its own
+ * tree) with `threads` std::threads, largest first.  Returns 0. */
+int orc_walk_seq_many(OrcTree* const* trees, float* const* images,
+                      const int64_t* ws, const int64_t* hs, int n,
+                      double diffusion, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* STIPPLE_ORACLE_H_ */

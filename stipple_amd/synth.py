@@ -1,0 +1,127 @@
+"""Seeded synthetic inputs for the five measured workloads.
+
+Every generator returns 8- or 16-bit integer samples (the wire format a
+caller hands over) and never depends on anything outside numpy.  8-bit
+samples widen to Q16 by x257 and 16-bit samples are used as-is, exactly as an
+8-bit or 16-bit PNG decodes into a Q16 float image (widen_q16).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def widen_q16(img: np.ndarray) -> np.ndarray:
+    """uint8/uint16 (H, W, 3|4) -> float32 (H, W, 4) Q16; alpha 65535 if absent."""
+    h, w, c = img.shape
+    out = np.empty((h, w, 4), np.float32)
+    scale = 257.0 if img.dtype == np.uint8 else 1.0
+    out[..., :c] = img.astype(np.float32) * np.float32(scale)
+    if c == 3:
+        out[..., 3] = 65535.0
+    return out
+
+
+def is_gray(img: np.ndarray) -> bool:
+    """The grayscale test rd_cli applies before ColorTree::Build (r == g == b)."""
+    return bool(np.all(img[..., 0] == img[..., 1]) and np.all(img[..., 1] == img[..., 2]))
+
+
+# -- C1: one 640x480 RGB8 ramp/sine image -----------------------------------
+def ramp_image(w=640, h=480, sigma=8.0, seed=1) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    x = np.arange(w, dtype=np.float64)[None, :]
+    y = np.arange(h, dtype=np.float64)[:, None]
+    r = np.round(255.0 * x / max(w - 1, 1)) + 0 * y
+    g = np.round(255.0 * np.sin(math.pi * y / max(h - 1, 1))) + 0 * x
+    b = np.floor((r + g) / 2.0)
+    img = np.stack([r, g, b], axis=-1)
+    img = img + rng.normal(0.0, sigma, img.shape)
+    return np.clip(np.round(img), 0, 255).astype(np.uint8)
+
+
+# -- C2: a batch of uneven RGBA8 stills --------------------------------------
+def sinusoid_image(w, h, rng, channels=4, sigma=6.0, rects=True) -> np.ndarray:
+    x = np.arange(w, dtype=np.float64)[None, :]
+    y = np.arange(h, dtype=np.float64)[:, None]
+    planes = []
+    for _ in range(3):
+        acc = np.zeros((h, w))
+        for _ in range(3):
+            fx, fy = rng.uniform(-0.03, 0.03, 2)
+            ph = rng.uniform(0, 2 * math.pi)
+            acc += np.sin(2 * math.pi * (fx * x + fy * y) + ph)
+        planes.append(127.5 + 127.5 * acc / 3.0)
+    rgb = np.stack(planes, -1) + rng.normal(0.0, sigma, (h, w, 3))
+    rgb = np.clip(np.round(rgb), 0, 255).astype(np.uint8)
+    if channels == 3:
+        return rgb
+    alpha = np.full((h, w), 255, np.uint8)
+    if rects:
+        for _ in range(int(rng.integers(1, 9))):
+            x0, x1 = np.sort(rng.integers(0, w + 1, 2))
+            y0, y1 = np.sort(rng.integers(0, h + 1, 2))
+            alpha[y0:y1, x0:x1] = 255 if rng.random() < 0.5 else 0
+    return np.concatenate([rgb, alpha[..., None]], -1)
+
+
+def still_batch(n=64, seed=2, max_side=2048):
+    """[(img uint8 HxWx4, colours)] with w, h uniform in [1, max_side],
+    including 1x1, 3x5 and 17x13; colours log-uniform on {2..256}."""
+    rng = np.random.default_rng(seed)
+    sizes = [(1, 1), (3, 5), (17, 13)]
+    while len(sizes) < n:
+        sizes.append((int(rng.integers(1, max_side + 1)), int(rng.integers(1, max_side + 1))))
+    out = []
+    for (w, h) in sizes[:n]:
+        colours = int(round(math.exp(rng.uniform(math.log(2), math.log(256)))))
+        colours = min(max(colours, 2), 256)
+        out.append((sinusoid_image(w, h, rng), colours))
+    return out
+
+
+# -- C3: plasma video -------------------------------------------------------
+def plasma_params(seed=3):
+    rng = np.random.default_rng(seed)
+    return {
+        "freq": rng.uniform(0.002, 0.02, (3, 4, 2)),
+        "phase": rng.uniform(0, 2 * math.pi, (3, 4)),
+        "drift": 0.05,
+    }
+
+
+def plasma_frame(t, w=1920, h=1080, params=None, sigma=4.0, seed=3) -> np.ndarray:
+    p = params or plasma_params(seed)
+    rng = np.random.default_rng(seed * 100003 + t)
+    x = np.arange(w, dtype=np.float64)[None, :]
+    y = np.arange(h, dtype=np.float64)[:, None]
+    planes = []
+    for c in range(3):
+        acc = np.zeros((h, w))
+        for k in range(4):
+            fx, fy = p["freq"][c, k]
+            acc += np.sin(2 * math.pi * (fx * x + fy * y) + p["phase"][c, k] + p["drift"] * t)
+        planes.append(127.5 + 127.5 * acc / 4.0)
+    rgb = np.stack(planes, -1) + rng.normal(0.0, sigma, (h, w, 3))
+    return np.clip(np.round(rgb), 0, 255).astype(np.uint8)
+
+
+# -- C4: RGB16 radial gradients with salt and pepper -------------------------
+def radial_frame16(t, w=3840, h=2160, seed=4, salt=0.05) -> np.ndarray:
+    rng = np.random.default_rng(seed * 7919 + t)
+    base = np.random.default_rng(seed)
+    centres = base.uniform(0, 1, (3, 2)) * [w, h]
+    x = np.arange(w, dtype=np.float64)[None, :]
+    y = np.arange(h, dtype=np.float64)[:, None]
+    diag = math.hypot(w, h)
+    planes = []
+    for c in range(3):
+        cx, cy = centres[c] + 3.0 * t
+        r = np.sqrt((x - cx) ** 2 + (y - cy) ** 2) / diag
+        planes.append(65535.0 * np.clip(1.0 - r, 0, 1))
+    img = np.stack(planes, -1)
+    mask = rng.random((h, w)) < salt
+    vals = np.where(rng.random((h, w)) < 0.5, 0.0, 65535.0)
+    img[mask] = vals[mask][:, None]
+    return np.clip(np.round(img), 0, 65535).astype(np.uint16)
