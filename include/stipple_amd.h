@@ -1,0 +1,129 @@
+/* stipple_amd.h -- the C-ABI of the MI355X Riemersma hot path.
+ *
+ * Plain C, plain pointers and sizes; no C++ or torch types cross this line.
+ * Every function returns 0 (SA_OK) on success or a nonzero SA_E* code, and
+ * writes a human-readable message into `err` (when non-null) on failure only
+ * -- the same success-is-empty convention as the reference's engines
+ * (src/rd_cuda_stub.cpp:60-74: an engine never reports success silently).
+ *
+ * Device pointers are HIP device pointers on the current device.  Calls that
+ * take a `stream` (a hipStream_t, or NULL for the null stream) are
+ * stream-ordered: they enqueue work and return; results are ready when the
+ * stream reaches that point.  Calls without a stream are synchronous.
+ *
+ * The C++ drop-in layer (include/rd_riemersma.h, include/rd_octree.h,
+ * include/rd_plugin.h in this repository) is built on these entry points; each
+ * entry point names the reference interface it replaces.
+ */
+#ifndef STIPPLE_AMD_H_
+#define STIPPLE_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SA_OK = 0,
+  SA_EINVAL = 1,       /* bad argument */
+  SA_EREFUSED = 2,     /* an input the reference engines also refuse */
+  SA_EDEVICE = 3,      /* HIP runtime error, or no device */
+  SA_ENOMEM = 4,       /* device allocation failed */
+  SA_EUNSUPPORTED = 5  /* valid input this build does not handle */
+};
+
+/* Pixel formats: interleaved, row-major, dense (width*height pixels). */
+enum {
+  SA_PIX_RGBAF32 = 0, /* RgbaF: 4 x float Q16, the reference's in-memory layout */
+  SA_PIX_RGB8 = 1,    /* 8-bit samples widen x257 on input */
+  SA_PIX_RGBA8 = 2,
+  SA_PIX_RGB16 = 3,   /* 16-bit samples are Q16 as-is */
+  SA_PIX_RGBA16 = 4,  /* rgba64le */
+  SA_PIX_INDEX8 = 5,  /* output only: palette index per pixel */
+  SA_PIX_INDEX16 = 6
+};
+
+typedef struct sa_palette_s* sa_palette;
+
+/* ---- runtime ----------------------------------------------------------- */
+
+/* Replaces rd::CudaAvailable (include/rd_riemersma.h:284): 1 when a gfx950
+ * device is usable. */
+int sa_available(void);
+
+/* Selects the device and warms the runtime.  Replaces the implicit
+ * cudaGetDevice of src/rd_riemersma_cuda.cu:561-562. */
+int sa_init(int device, char* err, size_t errlen);
+
+/* Device name (the `device_name` out-parameter of the reference engines,
+ * include/rd_riemersma.h:214-217). */
+int sa_device_name(char* out, size_t len);
+
+/* ---- palette + octree --------------------------------------------------- */
+
+/* Uploads a palette and the octree that its lookups search.  Replaces the
+ * host-side UploadTree / UploadPalette / BuildFlatSearch of
+ * include/rd_cuda_common.cuh:290-360,479-522.  `entries` holds count*4
+ * doubles (r,g,b,a: rd::Palette::entries); the tree arrays are rd::QNode
+ * fields, `nodes` long (child: 16 per node). */
+int sa_palette_create(const double* entries, int count, int assoc, const int32_t* child16,
+                      const int32_t* parent, const uint32_t* color_number,
+                      const uint64_t* number_unique, int64_t nodes, sa_palette* out,
+                      char* err, size_t errlen);
+int sa_palette_destroy(sa_palette p);
+int sa_palette_count(sa_palette p);
+
+/* ---- curve --------------------------------------------------------------- */
+
+/* Hilbert visit order of a w x h image, generated on the device: the
+ * w*h in-bounds pixel ids in the order the sequential walk visits them.
+ * Replaces rd::BuildCurveIndex / BlockCurveOrder (include/rd_riemersma.h:
+ * 178-179, 278-279); the block curve is this order without its last entry
+ * when w is a power of two >= h, plus the origin.  `order` is a device array
+ * of w*h int32. */
+int sa_curve_order(int64_t w, int64_t h, int32_t* order, void* stream, char* err,
+                   size_t errlen);
+
+/* ---- engines -------------------------------------------------------------- */
+
+typedef struct {
+  const void* src; /* device pixels, src_fmt */
+  void* dst;       /* device pixels, dst_fmt; may equal src for the sequential engine */
+  int64_t width;
+  int64_t height;
+  int32_t src_fmt;
+  int32_t dst_fmt;
+  sa_palette palette;
+} sa_image;
+
+typedef struct {
+  double diffusion; /* dither:diffusion-amount (rd::DitherParams::diffusion), 1.0 */
+  int32_t use_cache;/* quantize.c's memo; 1 reproduces ImageMagick */
+  int32_t memo_bits;/* LDS memo slots per wavefront = 2^memo_bits, 0 = auto */
+  int32_t waves;    /* persistent wavefronts, 0 = auto */
+  int32_t reserved;
+} sa_seq_options;
+
+/* The exact sequential walk over a batch of independent images, one
+ * wavefront per image, pulled from a device queue largest-first.  Replaces
+ * rd::RiemersmaWalkCuda (include/rd_riemersma.h:214-217, whose `frames`
+ * copies become arbitrary (image, palette) items).  Bit-exact per image with
+ * rd::RiemersmaWalkCpu. */
+int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_options* opt, void* stream,
+                      char* err, size_t errlen);
+
+/* The block-parallel walk over `n` frames of one geometry (each frame may
+ * carry its own palette).  Replaces rd::RiemersmaBlocksCuda
+ * (include/rd_riemersma.h:450-464); bit-exact with rd::RiemersmaBlocksCpu at
+ * the same `block`.  dst must not alias src.  Refuses block < 16 and palettes
+ * above 256 entries, as the reference does. */
+int sa_walk_blocks(const sa_image* frames, int n, int block, double diffusion, void* stream,
+                   char* err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STIPPLE_AMD_H_ */

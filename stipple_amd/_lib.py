@@ -1,0 +1,193 @@
+"""ctypes binding of include/stipple_amd.h.
+
+Loads the in-tree libstipple_amd.so; a missing or unloadable library is an
+error (ImportError at first use), never a silent fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "libstipple_amd.so")
+
+PIX_RGBAF32, PIX_RGB8, PIX_RGBA8, PIX_RGB16, PIX_RGBA16, PIX_INDEX8, PIX_INDEX16 = range(7)
+_PIX_CHANNELS = {PIX_RGBAF32: 4, PIX_RGB8: 3, PIX_RGBA8: 4, PIX_RGB16: 3, PIX_RGBA16: 4,
+                 PIX_INDEX8: 1, PIX_INDEX16: 1}
+
+
+class EngineError(RuntimeError):
+    """An engine refused its input or failed on the device (code, message)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class _Image(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("width", C.c_int64),
+                ("height", C.c_int64), ("src_fmt", C.c_int32), ("dst_fmt", C.c_int32),
+                ("palette", C.c_void_p)]
+
+
+class SeqOptions(C.Structure):
+    _fields_ = [("diffusion", C.c_double), ("use_cache", C.c_int32),
+                ("memo_bits", C.c_int32), ("waves", C.c_int32), ("reserved", C.c_int32)]
+
+    def __init__(self, diffusion=1.0, use_cache=True, memo_bits=0, waves=0):
+        super().__init__(diffusion, int(use_cache), memo_bits, waves, 0)
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def library_path():
+    return _SO
+
+
+def lib():
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(_SO):
+                raise ImportError(
+                    f"{_SO} is not built; run __graft_entry__.build() (make -C stipple_amd/csrc)")
+            L = C.CDLL(_SO)
+            vp, i32, i64, dbl, cp, sz = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_char_p, C.c_size_t
+            sig = {
+                "sa_available": (i32, []),
+                "sa_init": (i32, [i32, cp, sz]),
+                "sa_device_name": (i32, [cp, sz]),
+                "sa_palette_create": (i32, [vp, i32, i32, vp, vp, vp, vp, i64, C.POINTER(vp), cp, sz]),
+                "sa_palette_destroy": (i32, [vp]),
+                "sa_palette_count": (i32, [vp]),
+                "sa_curve_order": (i32, [i64, i64, vp, vp, cp, sz]),
+                "sa_walk_seq_batch": (i32, [C.POINTER(_Image), i32, C.POINTER(SeqOptions), vp, cp, sz]),
+                "sa_walk_blocks": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
+            }
+            for name, (res, args) in sig.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def _check(rc, buf):
+    if rc != 0:
+        raise EngineError(rc, buf.value.decode(errors="replace"))
+
+
+def _err():
+    return C.create_string_buffer(1024)
+
+
+def available():
+    return bool(lib().sa_available())
+
+
+def init(device=0):
+    e = _err()
+    _check(lib().sa_init(device, e, len(e)), e)
+
+
+def device_name():
+    b = C.create_string_buffer(256)
+    _check(lib().sa_device_name(b, len(b)), b)
+    return b.value.decode()
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Palette:
+    """A palette and the octree its lookups search, resident on the device.
+
+    Mirrors the (Palette, ColorTree) pair the reference engines take
+    (include/rd_riemersma.h:63-67, include/rd_octree.h:54-123)."""
+
+    def __init__(self, entries, assoc, child16, parent, color_number, number_unique):
+        entries = np.ascontiguousarray(entries, np.float64).reshape(-1, 4)
+        child16 = np.ascontiguousarray(child16, np.int32).reshape(-1, 16)
+        parent = np.ascontiguousarray(parent, np.int32)
+        color_number = np.ascontiguousarray(color_number, np.uint32)
+        number_unique = np.ascontiguousarray(number_unique, np.uint64)
+        h = C.c_void_p()
+        e = _err()
+        rc = lib().sa_palette_create(_np_ptr(entries), entries.shape[0], int(bool(assoc)),
+                                     _np_ptr(child16), _np_ptr(parent), _np_ptr(color_number),
+                                     _np_ptr(number_unique), child16.shape[0], C.byref(h), e, len(e))
+        _check(rc, e)
+        self.handle = h.value
+        self.count = entries.shape[0]
+        self.assoc = bool(assoc)
+        self.entries = entries
+
+    @classmethod
+    def from_tree(cls, tree):
+        """From any object with colormap/child/parent/color_number/number_unique/assoc
+        arrays (e.g. a TreeData)."""
+        return cls(tree.colormap, tree.assoc, tree.child, tree.parent, tree.color_number,
+                   tree.number_unique)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sa_palette_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dev_ptr(t):
+    import torch
+    assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous(), "device tensor required"
+    return t.data_ptr()
+
+
+def image_spec(src, dst, width, height, src_fmt, dst_fmt, palette):
+    """One image for the engines; src/dst are contiguous torch device tensors."""
+    return _Image(_dev_ptr(src), _dev_ptr(dst), width, height, src_fmt, dst_fmt, palette.handle)
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream
+
+
+def walk_seq_batch(images, options=None, stream=None):
+    """The exact sequential walk over a batch (sa_walk_seq_batch)."""
+    arr = (_Image * len(images))(*images)
+    opt = options or SeqOptions()
+    e = _err()
+    rc = lib().sa_walk_seq_batch(arr, len(images), C.byref(opt), _stream_ptr(stream), e, len(e))
+    _check(rc, e)
+
+
+def walk_blocks(frames, block=64, diffusion=1.0, stream=None):
+    """The block-parallel walk over frames of one geometry (sa_walk_blocks)."""
+    arr = (_Image * len(frames))(*frames)
+    e = _err()
+    rc = lib().sa_walk_blocks(arr, len(frames), block, diffusion, _stream_ptr(stream), e, len(e))
+    _check(rc, e)
+
+
+def curve_order(width, height, stream=None):
+    """Device Hilbert visit order (int32 torch tensor of width*height)."""
+    import torch
+    out = torch.empty(width * height, dtype=torch.int32, device="cuda")
+    e = _err()
+    rc = lib().sa_curve_order(width, height, out.data_ptr(), _stream_ptr(stream), e, len(e))
+    _check(rc, e)
+    return out

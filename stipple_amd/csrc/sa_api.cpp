@@ -1,0 +1,562 @@
+// sa_api.cpp -- the C-ABI (include/stipple_amd.h) over the HIP kernels.
+//
+// Host-side responsibilities only: argument checks with the reference's
+// refusals, the palette search image, the per-geometry curve cache, the error
+// weights (computed with the host libm, exactly as the reference computes
+// them, never on the device), stream-ordered scratch, and launches.
+#include "../../include/stipple_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sa_internal.h"
+#include "sa_math.h"
+#include "sa_palette.h"
+
+struct sa_palette_s {
+  sa::PaletteDev hdr;        // host copy of the header
+  sa::PaletteDev* dev_hdr = nullptr;
+  void* dev_image = nullptr;
+  int count = 0;
+  int assoc = 0;
+};
+
+namespace {
+
+using sa::PaletteDev;
+
+std::mutex g_mu;
+int g_device = -1;
+int g_cus = 256;
+size_t g_lds_per_cu = 160 * 1024;
+char g_name[256] = "";
+
+int fail(char* err, size_t n, int code, const char* fmt, ...) {
+  if (err && n) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, n, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(err, errlen, SA_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure_init(char* err, size_t errlen) {
+  std::lock_guard<std::mutex> g(g_mu);
+  if (g_device >= 0) return SA_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(err, errlen, SA_EDEVICE, "no HIP device available");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  g_cus = prop.multiProcessorCount;
+  snprintf(g_name, sizeof(g_name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName, g_cus);
+  g_device = dev;
+  return SA_OK;
+}
+
+// Scaled weights (1/16 * diffusion) * 16^(-k/15), via the host libm: the
+// reference computes them on the host and uploads them (include/rd_types.h:
+// 169-176, src/rd_riemersma_cpu.cpp:370-373, include/rd_cuda_common.cuh:4-10).
+void scaled_weights(double diffusion, double sw[16]) {
+  double weight = 1.0;
+  for (int i = 0; i < 16; ++i) {
+    const double w = sa::safe_recip(weight);
+    sw[i] = (1.0 / 16.0) * diffusion * w;
+    weight *= std::exp(std::log(1.0 / (1.0 / 16.0)) / (16 - 1.0));
+  }
+}
+
+// ---- pinned staging for small host->device argument arrays ----------------
+struct Staging {
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t done = nullptr;
+};
+std::mutex g_stage_mu;
+std::vector<Staging> g_stage;
+
+// Copies `bytes` from `src` to device `dst` on `s` through a pinned buffer
+// that is not reused until the copy has executed.
+hipError_t staged_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_stage_mu);
+  Staging* pick = nullptr;
+  for (Staging& st : g_stage) {
+    if (st.cap >= bytes && hipEventQuery(st.done) == hipSuccess) {
+      pick = &st;
+      break;
+    }
+  }
+  if (!pick) {
+    Staging st;
+    st.cap = std::max<size_t>(bytes, 64 * 1024);
+    hipError_t e = hipHostMalloc(&st.host, st.cap, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    g_stage.push_back(st);
+    pick = &g_stage.back();
+  }
+  std::memcpy(pick->host, src, bytes);
+  hipError_t e = hipMemcpyAsync(dst, pick->host, bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipEventRecord(pick->done, s);
+}
+
+// ---- curve cache ---------------------------------------------------------
+struct CurveEntry {
+  int64_t w, h;
+  int32_t* order;  // w*h
+  size_t bytes;
+};
+std::mutex g_curve_mu;
+std::list<CurveEntry> g_curves;  // most recently used first
+size_t g_curve_bytes = 0;
+const size_t kCurveBudget = (size_t)8 << 30;
+
+int curve_level(int64_t w, int64_t h) {  // include/rd_riemersma.h:88-94, on the host
+  const int64_t extent = w > h ? w : h;
+  if (extent <= 0) return 0;
+  int level = (int)std::log2((double)extent);
+  if (((int64_t)1 << level) < extent) ++level;
+  return level;
+}
+
+int build_curve(int64_t w, int64_t h, int32_t* order, hipStream_t s, char* err, size_t errlen) {
+  const int level = curve_level(w, h);
+  if (level > 15) return fail(err, errlen, SA_EUNSUPPORTED, "image side above 32768");
+  const int64_t chunks = sa::curve_chunks(level);
+  int32_t* counts = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&counts, sizeof(int32_t) * chunks, s));
+  HIP_TRY(sa::launch_curve_count(level, w, h, counts, chunks, s));
+  HIP_TRY(sa::launch_curve_scan(counts, chunks, s));
+  HIP_TRY(sa::launch_curve_emit(level, w, h, counts, chunks, order, s));
+  HIP_TRY(hipFreeAsync(counts, s));
+  return SA_OK;
+}
+
+// Returns a cached device curve for (w, h), building it on `s` if needed.
+int get_curve(int64_t w, int64_t h, hipStream_t s, const int32_t** out, char* err,
+              size_t errlen) {
+  std::lock_guard<std::mutex> g(g_curve_mu);
+  for (auto it = g_curves.begin(); it != g_curves.end(); ++it) {
+    if (it->w == w && it->h == h) {
+      g_curves.splice(g_curves.begin(), g_curves, it);
+      *out = g_curves.front().order;
+      return SA_OK;
+    }
+  }
+  CurveEntry e;
+  e.w = w;
+  e.h = h;
+  e.bytes = sizeof(int32_t) * (size_t)(w * h);
+  // Evict least recently used entries beyond the budget.  The device must be
+  // done with them: synchronise before freeing.
+  if (g_curve_bytes + e.bytes > kCurveBudget && !g_curves.empty()) {
+    HIP_TRY(hipDeviceSynchronize());
+    while (g_curve_bytes + e.bytes > kCurveBudget && !g_curves.empty()) {
+      HIP_TRY(hipFree(g_curves.back().order));
+      g_curve_bytes -= g_curves.back().bytes;
+      g_curves.pop_back();
+    }
+  }
+  HIP_TRY(hipMalloc((void**)&e.order, e.bytes));
+  const int rc = build_curve(w, h, e.order, s, err, errlen);
+  if (rc != SA_OK) {
+    hipFree(e.order);
+    return rc;
+  }
+  g_curves.push_front(e);
+  g_curve_bytes += e.bytes;
+  *out = e.order;
+  return SA_OK;
+}
+
+// ---- weights cache (per diffusion value) ---------------------------------
+std::mutex g_w_mu;
+std::map<double, double*> g_weights;
+
+int get_weights(double diffusion, hipStream_t s, const double** out, char* err, size_t errlen) {
+  std::lock_guard<std::mutex> g(g_w_mu);
+  auto it = g_weights.find(diffusion);
+  if (it != g_weights.end()) {
+    *out = it->second;
+    return SA_OK;
+  }
+  double sw[16];
+  scaled_weights(diffusion, sw);
+  double* d = nullptr;
+  HIP_TRY(hipMalloc((void**)&d, sizeof(sw)));
+  HIP_TRY(hipMemcpy(d, sw, sizeof(sw), hipMemcpyHostToDevice));
+  (void)s;
+  g_weights[diffusion] = d;
+  *out = d;
+  return SA_OK;
+}
+
+bool valid_in_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kRgba16; }
+bool valid_out_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kIndex16; }
+
+}  // namespace
+
+// ===========================================================================
+extern "C" int sa_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0 ? 1 : 0;
+}
+
+extern "C" int sa_init(int device, char* err, size_t errlen) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(err, errlen, SA_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(err, errlen, SA_EINVAL, "device %d of %d", device, n);
+  HIP_TRY(hipSetDevice(device));
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    g_device = -1;
+  }
+  return ensure_init(err, errlen);
+}
+
+extern "C" int sa_device_name(char* out, size_t len) {
+  char err[256];
+  if (ensure_init(err, sizeof(err)) != SA_OK) return SA_EDEVICE;
+  snprintf(out, len, "%s", g_name);
+  return SA_OK;
+}
+
+// ---- palette ----------------------------------------------------------------
+extern "C" int sa_palette_create(const double* entries, int count, int assoc,
+                                 const int32_t* child16, const int32_t* parent,
+                                 const uint32_t* color_number, const uint64_t* number_unique,
+                                 int64_t nodes, sa_palette* out, char* err, size_t errlen) {
+  (void)parent;
+  if (!out) return fail(err, errlen, SA_EINVAL, "null output handle");
+  *out = nullptr;
+  if (count < 1) return fail(err, errlen, SA_EREFUSED, "empty palette");
+  if (count > 65536) return fail(err, errlen, SA_EINVAL, "palette above 65536 entries");
+  if (nodes < 1 || !child16 || !color_number || !number_unique || !entries)
+    return fail(err, errlen, SA_EINVAL, "missing tree arrays");
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  const int kids = assoc ? 16 : 8;
+  // Reachable nodes from the root, pre-order; internal ones get rows.
+  std::vector<int32_t> row_of((size_t)nodes, -1);
+  std::vector<int32_t> rows;  // node id per row
+  std::vector<int32_t> stack{0};
+  std::vector<char> seen((size_t)nodes, 0);
+  while (!stack.empty()) {
+    const int32_t n = stack.back();
+    stack.pop_back();
+    if (n < 0 || n >= nodes || seen[(size_t)n]) return fail(err, errlen, SA_EINVAL, "malformed tree");
+    seen[(size_t)n] = 1;
+    bool internal = false;
+    for (int c = 0; c < kids; ++c) internal |= child16[16 * n + c] >= 0;
+    if (internal || n == 0) {
+      row_of[(size_t)n] = (int32_t)rows.size();
+      rows.push_back(n);
+    }
+    for (int c = kids - 1; c >= 0; --c)
+      if (child16[16 * n + c] >= 0) stack.push_back(child16[16 * n + c]);
+  }
+  if (rows.size() > 32767) return fail(err, errlen, SA_EUNSUPPORTED, "tree above 32767 internal nodes");
+  // Post-order candidate list and the range of every internal node's subtree.
+  std::vector<uint16_t> cand;
+  std::vector<int32_t> sub_off(rows.size(), 0), sub_len(rows.size(), 0);
+  struct Fr {
+    int32_t n;
+    int next;
+    int32_t first;
+  };
+  std::vector<Fr> st{{0, 0, 0}};
+  while (!st.empty()) {
+    Fr& f = st.back();
+    if (f.next < kids) {
+      const int32_t c = child16[16 * f.n + f.next];
+      ++f.next;
+      if (c >= 0) st.push_back({c, 0, (int32_t)cand.size()});
+      continue;
+    }
+    if (number_unique[f.n] != 0) {
+      const uint32_t cn = color_number[f.n];
+      if ((int)cn >= count) return fail(err, errlen, SA_EINVAL, "colour number out of range");
+      cand.push_back((uint16_t)cn);
+    }
+    const int32_t r = row_of[(size_t)f.n];
+    if (r >= 0) {
+      sub_off[(size_t)r] = f.first;
+      sub_len[(size_t)r] = (int32_t)cand.size() - f.first;
+    }
+    st.pop_back();
+  }
+  // Search image.
+  const sa::PaletteLayout L = sa::palette_layout(count, (int)rows.size(), kids);
+  std::vector<unsigned char> img((size_t)L.bytes, 0);
+  double* col = (double*)(img.data() + L.off_col);
+  float* ent = (float*)(img.data() + L.off_ent);
+  int32_t* sub = (int32_t*)(img.data() + L.off_sub);
+  int16_t* child = (int16_t*)(img.data() + L.off_child);
+  uint16_t* cnd = (uint16_t*)(img.data() + L.off_cand);
+  for (int i = 0; i < count; ++i) {
+    const double* e = entries + 4 * i;
+    for (int c = 0; c < 4; ++c) {
+      if ((double)(float)e[c] != e[c])
+        return fail(err, errlen, SA_EUNSUPPORTED, "palette entry %d is not a Quantum (float) value", i);
+      ent[4 * i + c] = (float)e[c];
+    }
+    sa::assoc_entry(assoc != 0, e, col + 4 * i);
+  }
+  for (size_t r = 0; r < rows.size(); ++r) {
+    sub[2 * r] = sub_off[r];
+    sub[2 * r + 1] = sub_len[r];
+    const int32_t n = rows[r];
+    for (int c = 0; c < kids; ++c) {
+      const int32_t ch = child16[16 * n + c];
+      int16_t v = sa::kNoChild;
+      if (ch >= 0) v = row_of[(size_t)ch] >= 0 ? (int16_t)row_of[(size_t)ch] : sa::kLeaf;
+      child[r * kids + c] = v;
+    }
+  }
+  for (size_t i = 0; i < cand.size() && i < (size_t)count; ++i) cnd[i] = cand[i];
+  sa_palette p = new sa_palette_s();
+  p->count = count;
+  p->assoc = assoc ? 1 : 0;
+  PaletteDev& h = p->hdr;
+  h.count = count;
+  h.assoc = p->assoc;
+  h.kids = kids;
+  h.rows = (int32_t)rows.size();
+  h.bytes = L.bytes;
+  h.off_ent = L.off_ent;
+  h.off_sub = L.off_sub;
+  h.off_child = L.off_child;
+  h.off_cand = L.off_cand;
+  h.pad_ = 0;
+  hipError_t e = hipMalloc(&p->dev_image, (size_t)L.bytes);
+  if (e == hipSuccess) e = hipMemcpy(p->dev_image, img.data(), (size_t)L.bytes, hipMemcpyHostToDevice);
+  h.image = p->dev_image;
+  if (e == hipSuccess) e = hipMalloc((void**)&p->dev_hdr, sizeof(PaletteDev));
+  if (e == hipSuccess) e = hipMemcpy(p->dev_hdr, &h, sizeof(PaletteDev), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    sa_palette_destroy(p);
+    return fail(err, errlen, SA_EDEVICE, "palette upload: %s", hipGetErrorString(e));
+  }
+  *out = p;
+  return SA_OK;
+}
+
+extern "C" int sa_palette_destroy(sa_palette p) {
+  if (!p) return SA_OK;
+  hipDeviceSynchronize();
+  if (p->dev_image) hipFree(p->dev_image);
+  if (p->dev_hdr) hipFree(p->dev_hdr);
+  delete p;
+  return SA_OK;
+}
+
+extern "C" int sa_palette_count(sa_palette p) { return p ? p->count : 0; }
+
+// ---- curve -------------------------------------------------------------------
+extern "C" int sa_curve_order(int64_t w, int64_t h, int32_t* order, void* stream, char* err,
+                              size_t errlen) {
+  if (w < 1 || h < 1 || !order) return fail(err, errlen, SA_EINVAL, "bad geometry");
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  return build_curve(w, h, order, (hipStream_t)stream, err, errlen);
+}
+
+// ---- sequential engine -----------------------------------------------------
+extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_options* opt,
+                                 void* stream, char* err, size_t errlen) {
+  if (n < 0 || (n > 0 && !images)) return fail(err, errlen, SA_EINVAL, "bad image list");
+  if (n == 0) return SA_OK;
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  sa_seq_options o{1.0, 1, 0, 0, 0};
+  if (opt) o = *opt;
+  for (int i = 0; i < n; ++i) {
+    const sa_image& im = images[i];
+    if (!im.src || !im.dst || !im.palette || im.width < 1 || im.height < 1)
+      return fail(err, errlen, SA_EINVAL, "image %d: null pointer or empty geometry", i);
+    if (!valid_in_fmt(im.src_fmt) || !valid_out_fmt(im.dst_fmt))
+      return fail(err, errlen, SA_EINVAL, "image %d: bad pixel format", i);
+    if (im.dst == im.src && (im.src_fmt != im.dst_fmt))
+      return fail(err, errlen, SA_EINVAL, "image %d: in-place needs the same format", i);
+    if (im.palette->hdr.bytes > 96 * 1024)
+      return fail(err, errlen, SA_EUNSUPPORTED, "image %d: palette search image above 96 KiB", i);
+  }
+  const double* sw = nullptr;
+  rc = get_weights(o.diffusion, s, &sw, err, errlen);
+  if (rc) return rc;
+  // Two groups by alpha association (different tree arity and key width).
+  for (int assoc = 0; assoc < 2; ++assoc) {
+    std::vector<int> sel;
+    for (int i = 0; i < n; ++i)
+      if (images[i].palette->assoc == assoc) sel.push_back(i);
+    if (sel.empty()) continue;
+    // Largest first: the makespan is set by the biggest walk.
+    std::stable_sort(sel.begin(), sel.end(), [&](int a, int b) {
+      return images[a].width * images[a].height > images[b].width * images[b].height;
+    });
+    int max_img = 0;
+    int64_t max_n = 0, total = 0;
+    for (int i : sel) {
+      max_img = std::max(max_img, images[i].palette->hdr.bytes);
+      const int64_t px = images[i].width * images[i].height;
+      max_n = std::max(max_n, px);
+      total += px;
+    }
+    const int K = assoc ? 24 : 18;
+    int S = o.memo_bits;
+    if (S <= 0) {
+      S = 15;
+      while (S > 10 && (4u << S) + (size_t)max_img > 150 * 1024) --S;
+    }
+    if (S < 8 || S > 16) return fail(err, errlen, SA_EINVAL, "memo_bits %d outside 8..16", S);
+    const size_t lds = (o.use_cache ? (4u << S) : 0u) + (size_t)max_img;
+    if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "LDS %zu too large", lds);
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
+    int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
+    waves = std::min<int>(waves, (int)sel.size());
+    // Scratch: curves, gathered inputs, indices, memo spill tables.
+    std::vector<sa::SeqItem> items(sel.size());
+    size_t xs_bytes = 0, idx_bytes = 0, spill_bytes = 0;
+    for (size_t k = 0; k < sel.size(); ++k) {
+      const int64_t px = images[sel[k]].width * images[sel[k]].height;
+      xs_bytes += 32 * (size_t)px;
+      idx_bytes += (2 * (size_t)px + 255) & ~(size_t)255;
+      if (o.use_cache) spill_bytes += (size_t)4 << K;
+    }
+    unsigned char* scratch = nullptr;
+    const size_t items_bytes = (sizeof(sa::SeqItem) * sel.size() + 255) & ~(size_t)255;
+    const size_t total_bytes = items_bytes + 256 + xs_bytes + idx_bytes + spill_bytes;
+    HIP_TRY(hipMallocAsync((void**)&scratch, total_bytes, s));
+    unsigned char* p = scratch + items_bytes + 256;
+    for (size_t k = 0; k < sel.size(); ++k) {
+      const sa_image& im = images[sel[k]];
+      const int64_t px = im.width * im.height;
+      sa::SeqItem& it = items[k];
+      const int32_t* curve = nullptr;
+      rc = get_curve(im.width, im.height, s, &curve, err, errlen);
+      if (rc) {
+        hipFreeAsync(scratch, s);
+        return rc;
+      }
+      it.src = im.src;
+      it.dst = im.dst;
+      it.curve = curve;
+      it.xs = (double*)p;
+      p += 32 * (size_t)px;
+      it.idx = (uint16_t*)p;
+      p += (2 * (size_t)px + 255) & ~(size_t)255;
+      it.spill = nullptr;
+      if (o.use_cache) {
+        it.spill = (int32_t*)p;
+        p += (size_t)4 << K;
+      }
+      it.pal = im.palette->dev_hdr;
+      it.n = px;
+      it.src_fmt = im.src_fmt;
+      it.dst_fmt = im.dst_fmt;
+    }
+    sa::SeqItem* d_items = (sa::SeqItem*)scratch;
+    int32_t* d_queue = (int32_t*)(scratch + items_bytes);
+    HIP_TRY(staged_copy(d_items, items.data(), sizeof(sa::SeqItem) * items.size(), s));
+    HIP_TRY(hipMemsetAsync(d_queue, 0, sizeof(int32_t), s));
+    HIP_TRY(sa::launch_seq_gather(d_items, (int)items.size(), max_n, s));
+    HIP_TRY(sa::launch_seq_walk(d_items, (int)items.size(), d_queue, sw, S, o.use_cache, max_img,
+                                waves, assoc, s));
+    HIP_TRY(sa::launch_seq_pack(d_items, (int)items.size(), max_n, s));
+    HIP_TRY(hipFreeAsync(scratch, s));
+    (void)total;
+  }
+  return SA_OK;
+}
+
+// ---- block engine ----------------------------------------------------------
+extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double diffusion,
+                              void* stream, char* err, size_t errlen) {
+  if (n < 1 || !frames)
+    return fail(err, errlen, SA_EREFUSED,
+                "invalid dither batch: frames must be >= 1 and block must be >= 16 (got frames=%d, block=%d)",
+                n, block);
+  if (block < 16)
+    return fail(err, errlen, SA_EREFUSED,
+                "invalid dither batch: frames must be >= 1 and block must be >= 16 (got frames=%d, block=%d)",
+                n, block);
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t w = frames[0].width, h = frames[0].height;
+  int assoc = -1, max_img = 0;
+  for (int i = 0; i < n; ++i) {
+    const sa_image& f = frames[i];
+    if (!f.src || !f.dst || !f.palette) return fail(err, errlen, SA_EINVAL, "frame %d: null pointer", i);
+    if (f.width != w || f.height != h) return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
+    if (f.dst == f.src) return fail(err, errlen, SA_EINVAL, "frame %d: dst aliases src", i);
+    if (!valid_in_fmt(f.src_fmt) || !valid_out_fmt(f.dst_fmt))
+      return fail(err, errlen, SA_EINVAL, "frame %d: bad pixel format", i);
+    if (f.palette->count > 256)
+      return fail(err, errlen, SA_EREFUSED,
+                  "the blocks engine carries one palette index per byte, so the palette would alias "
+                  "onto the first 256 (%d colours requested)", f.palette->count);
+    if (assoc < 0) assoc = f.palette->assoc;
+    if (assoc != f.palette->assoc)
+      return fail(err, errlen, SA_EUNSUPPORTED, "frames mix associated and plain alpha");
+    max_img = std::max(max_img, f.palette->hdr.bytes);
+  }
+  if (w < 1 || h < 1) return fail(err, errlen, SA_EINVAL, "bad geometry");
+  const double* sw = nullptr;
+  rc = get_weights(diffusion, s, &sw, err, errlen);
+  if (rc) return rc;
+  const int level = curve_level(w, h);
+  const int32_t* curve = nullptr;
+  rc = get_curve(w, h, s, &curve, err, errlen);
+  if (rc) return rc;
+  // Block curve length: in-bounds of [0, 4^L - 1) plus the origin; the gap
+  // cell (2^L-1, 0) is in bounds exactly when w == 2^L.
+  int64_t nb = 0, gap = -1;
+  if (level > 0) {
+    const bool gap_in = w == ((int64_t)1 << level);
+    nb = w * h - (gap_in ? 1 : 0) + 1;
+    if (gap_in) gap = w - 1;
+  } else {
+    gap = 0;  // 1x1: the block curve is empty (src/rd_riemersma_cpu.cpp:275)
+  }
+  std::vector<sa::BlockFrame> fr((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    fr[(size_t)i].src = frames[i].src;
+    fr[(size_t)i].dst = frames[i].dst;
+    fr[(size_t)i].pal = frames[i].palette->dev_hdr;
+    fr[(size_t)i].src_fmt = frames[i].src_fmt;
+    fr[(size_t)i].dst_fmt = frames[i].dst_fmt;
+  }
+  sa::BlockFrame* d_fr = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::BlockFrame) * fr.size(), s));
+  HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::BlockFrame) * fr.size(), s));
+  HIP_TRY(sa::launch_blocks(d_fr, n, curve, w, h, nb, block, sw, assoc, max_img, s));
+  if (gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, n, gap, s));
+  HIP_TRY(hipFreeAsync(d_fr, s));
+  return SA_OK;
+}

@@ -1,0 +1,117 @@
+// sa_internal.h -- types shared by the HIP kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sa_palette.h"
+
+namespace sa {
+
+// Pixel formats (values match SA_PIX_* in include/stipple_amd.h).
+enum PixFmt : int32_t {
+  kRgbaF32 = 0,
+  kRgb8 = 1,
+  kRgba8 = 2,
+  kRgb16 = 3,
+  kRgba16 = 4,
+  kIndex8 = 5,
+  kIndex16 = 6,
+};
+
+inline int pix_bytes(int fmt) {
+  switch (fmt) {
+    case kRgbaF32: return 16;
+    case kRgb8: return 3;
+    case kRgba8: return 4;
+    case kRgb16: return 6;
+    case kRgba16: return 8;
+    case kIndex8: return 1;
+    case kIndex16: return 2;
+  }
+  return 0;
+}
+
+// Loads one source pixel as Q16 floats: 8-bit samples widen x257 (exact),
+// 16-bit samples as-is, float as-is; alpha 65535 when the format has none.
+__device__ __forceinline__ float4 load_pixel(int fmt, const void* base, int64_t p) {
+  float4 o;
+  switch (fmt) {
+    case kRgb8: {
+      const uint8_t* s = (const uint8_t*)base + 3 * p;
+      o = make_float4((float)s[0] * 257.0f, (float)s[1] * 257.0f, (float)s[2] * 257.0f, 65535.0f);
+      break;
+    }
+    case kRgba8: {
+      const uint32_t v = ((const uint32_t*)base)[p];
+      o = make_float4((float)(v & 255u) * 257.0f, (float)((v >> 8) & 255u) * 257.0f,
+                      (float)((v >> 16) & 255u) * 257.0f, (float)(v >> 24) * 257.0f);
+      break;
+    }
+    case kRgb16: {
+      const uint16_t* s = (const uint16_t*)base + 3 * p;
+      o = make_float4((float)s[0], (float)s[1], (float)s[2], 65535.0f);
+      break;
+    }
+    case kRgba16: {
+      const uint2 v = ((const uint2*)base)[p];
+      o = make_float4((float)(v.x & 0xFFFFu), (float)(v.x >> 16), (float)(v.y & 0xFFFFu),
+                      (float)(v.y >> 16));
+      break;
+    }
+    default:
+      o = ((const float4*)base)[p];
+      break;
+  }
+  return o;
+}
+
+// One image of the sequential engine (device array of these).
+struct SeqItem {
+  const void* src;        // source pixels (raster)
+  void* dst;              // output (raster), may alias src for kRgbaF32
+  const int32_t* curve;   // visit order: w*h pixel ids
+  double* xs;             // scratch: 4 doubles per visit (associated input)
+  uint16_t* idx;          // scratch: palette index per visit
+  int32_t* spill;         // scratch: 1<<K memo backing (K = 18, or 24 with alpha)
+  const PaletteDev* pal;  // device palette header
+  int64_t n;              // visits == w*h
+  int32_t src_fmt;
+  int32_t dst_fmt;
+};
+
+// One frame of the block engine.
+struct BlockFrame {
+  const void* src;
+  void* dst;
+  const PaletteDev* pal;
+  int32_t src_fmt;
+  int32_t dst_fmt;
+};
+
+struct Launch {
+  hipStream_t stream;
+  int cus;  // compute units on the device
+};
+
+// ---- launchers (return hipError_t) ----------------------------------------
+hipError_t launch_curve_count(int level, int64_t w, int64_t h, int32_t* chunk_counts,
+                              int64_t chunks, hipStream_t s);
+hipError_t launch_curve_scan(int32_t* chunk_counts, int64_t chunks, hipStream_t s);
+hipError_t launch_curve_emit(int level, int64_t w, int64_t h, const int32_t* chunk_offsets,
+                             int64_t chunks, int32_t* order, hipStream_t s);
+int64_t curve_chunks(int level);
+
+hipError_t launch_seq_gather(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
+hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, const double* sw,
+                           int memo_bits, int use_cache, int lds_image_bytes, int waves,
+                           int assoc_any, hipStream_t s);
+hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
+
+hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,
+                         int64_t h, int64_t nb, int block, const double* sw, int assoc,
+                         int lds_image_bytes, hipStream_t s);
+hipError_t launch_blocks_fill(const BlockFrame* frames, int nframes, int64_t gap_pixel,
+                              hipStream_t s);
+
+}  // namespace sa

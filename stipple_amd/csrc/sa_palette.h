@@ -1,0 +1,63 @@
+// sa_palette.h -- the device-resident palette + octree search structure.
+//
+// One contiguous "search image" per palette, laid out so an engine can copy it
+// into LDS verbatim:
+//
+//   double4  col[count]    associated colour of each entry, r,g,b,a (the value
+//                          the residual subtracts: AssociateAlphaPixelInfo)
+//   float4   ent[count]    the entry itself (Quantum == float, exact)
+//   int2     sub[rows]     candidate range [off, off+len) of each internal node
+//   int16    child[rows*kids]  -1 none, kLeaf a child that has no children,
+//                          else the child's row
+//   uint16   cand[count]   palette indices of the colour-bearing nodes in
+//                          post-order
+//
+// Why this is the reference's search, exactly: quantize.c (restated in
+// src/rd_octree.cpp:178-217) scans the subtree of parent(deepest node) in
+// post-order and keeps the LAST candidate whose distance is <= the running
+// minimum.  A subtree's post-order is a contiguous range of the root's
+// post-order, so every candidate list is a range of `cand`.  The descent
+// (src/rd_riemersma_cpu.cpp:82-89) only needs to know, per child, whether it
+// exists and whether it has children of its own; the node the descent stops
+// below ("prev") then names the range.  Only internal nodes get a row.
+#pragma once
+
+#include <stdint.h>
+
+namespace sa {
+
+constexpr int16_t kNoChild = -1;
+constexpr int16_t kLeaf = -2;
+
+struct PaletteDev {   // header, in device memory
+  int32_t count;      // palette entries (<= 65536)
+  int32_t assoc;      // alpha associated (16-ary tree, 4-channel distance)
+  int32_t kids;       // 8 or 16
+  int32_t rows;       // internal nodes (row 0 is the root)
+  int32_t bytes;      // size of the search image
+  int32_t off_ent;    // byte offsets inside the search image
+  int32_t off_sub;
+  int32_t off_child;
+  int32_t off_cand;
+  int32_t pad_;
+  const void* image;  // device pointer to the search image
+};
+
+// Byte layout of a search image with the given counts (16-byte aligned parts).
+struct PaletteLayout {
+  int32_t off_col, off_ent, off_sub, off_child, off_cand, bytes;
+};
+
+inline PaletteLayout palette_layout(int count, int rows, int kids) {
+  auto up = [](int32_t v) { return (v + 15) & ~15; };
+  PaletteLayout L;
+  L.off_col = 0;
+  L.off_ent = up(L.off_col + 32 * count);
+  L.off_sub = up(L.off_ent + 16 * count);
+  L.off_child = up(L.off_sub + 8 * rows);
+  L.off_cand = up(L.off_child + 2 * rows * kids);
+  L.bytes = up(L.off_cand + 2 * count);
+  return L;
+}
+
+}  // namespace sa

@@ -1,0 +1,186 @@
+"""Parity of the HIP engines with the oracle, through the C-ABI.
+
+Bit-exact for every integer, index and float output.  Each case prints its
+name and shape before touching the device.
+"""
+import numpy as np
+import pytest
+
+from stipple_amd import synth
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+_KEEP = []  # device buffers referenced by pending specs stay alive until the module ends
+
+GEOMS = [(1, 1), (3, 5), (17, 13), (2, 1), (1, 2), (16, 16), (64, 64), (100, 37), (256, 192),
+         (33, 200), (640, 480), (512, 480), (1000, 999), (2048, 7), (5, 2048)]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("w,h", GEOMS)
+def test_curve_order(gpu, oracle, w, h):
+    say("curve", w, h)
+    got = gpu.curve_order(w, h).cpu().numpy()
+    assert np.array_equal(got, oracle.seq_order(w, h))
+
+
+def _seq_case(gpu, oracle, img, colors, use_cache=True, dst_fmt=None, diffusion=1.0):
+    tree, f = H.oracle_tree(oracle, img, colors)
+    pal = gpu.Palette.from_tree(tree.data())
+    h, w = img.shape[:2]
+    src = H.to_device(img)
+    fmt = H.fmt_of(img)
+    dst_fmt = gpu.PIX_RGBAF32 if dst_fmt is None else dst_fmt
+    dst = H.out_alloc((h, w), dst_fmt)
+    spec = gpu.image_spec(src, dst, w, h, fmt, dst_fmt, pal)
+    _KEEP.append((src, pal, tree))
+    return tree, f, pal, spec, dst
+
+
+def _expect_seq(oracle, tree, f, use_cache, diffusion=1.0):
+    return oracle.walk_seq(tree, f, diffusion=diffusion, use_cache=use_cache)
+
+
+@pytest.mark.parametrize("use_cache", [True, False])
+@pytest.mark.parametrize("assoc", [False, True])
+def test_seq_small_batch(gpu, oracle, use_cache, assoc):
+    rng = np.random.default_rng(11 + assoc)
+    sizes = [(1, 1), (3, 5), (17, 13), (64, 48), (96, 64), (128, 128), (200, 150), (31, 257)]
+    colours = [2, 4, 16, 16, 64, 256, 16, 8]
+    cases, specs = [], []
+    for (w, h), col in zip(sizes, colours):
+        img = synth.sinusoid_image(w, h, rng, channels=4 if assoc else 3)
+        say("seq case", w, h, col, "assoc", assoc, "cache", use_cache)
+        tree, f, pal, spec, dst = _seq_case(gpu, oracle, img, col, use_cache)
+        cases.append((tree, f, pal, dst, (h, w)))
+        specs.append(spec)
+    gpu.walk_seq_batch(specs, gpu.SeqOptions(use_cache=use_cache))
+    import torch
+    torch.cuda.synchronize()
+    for tree, f, pal, dst, hw in cases:
+        got = H.from_device(dst, (hw[0], hw[1], 4), np.float32)
+        exp = _expect_seq(oracle, tree, f, use_cache)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), f"mismatch at {hw}"
+
+
+@pytest.mark.parametrize("dst_name", ["RGBA8", "RGB8", "RGBA16", "INDEX16"])
+def test_seq_output_formats(gpu, oracle, dst_name):
+    rng = np.random.default_rng(5)
+    img = synth.sinusoid_image(120, 90, rng, channels=4)
+    dst_fmt = getattr(gpu, "PIX_" + dst_name)
+    say("seq format", dst_name)
+    tree, f, pal, spec, dst = _seq_case(gpu, oracle, img, 16, True, dst_fmt)
+    gpu.walk_seq_batch([spec])
+    import torch
+    torch.cuda.synchronize()
+    exp, idx = oracle.walk_seq(tree, f, want_index=True)
+    h, w = img.shape[:2]
+    if dst_name == "INDEX16":
+        got = H.from_device(dst, (h * w,), np.uint16)
+        order = oracle.seq_order(w, h)
+        want = np.zeros(h * w, np.uint16)
+        want[order] = idx
+        assert np.array_equal(got, want)
+    elif dst_name == "RGBA16":
+        got = H.from_device(dst, (h, w, 4), np.uint16)
+        assert np.array_equal(got, exp.astype(np.uint16))
+    else:
+        c = 4 if dst_name == "RGBA8" else 3
+        got = H.from_device(dst, (h, w, c), np.uint8)
+        assert np.array_equal(got, H.scale_char_np(exp[..., :c]))
+
+
+def test_seq_ramp_640x480(gpu, oracle):
+    img = synth.ramp_image()
+    say("seq ramp 640x480 16 colours")
+    tree, f, pal, spec, dst = _seq_case(gpu, oracle, img, 16)
+    gpu.walk_seq_batch([spec])
+    got = H.from_device(dst, (480, 640, 4), np.float32)
+    exp = oracle.walk_seq(tree, f)
+    assert np.array_equal(got, exp)
+
+
+def test_seq_small_memo_spills(gpu, oracle):
+    """A tiny LDS memo forces evictions into the global table; first-touch
+    results must not change."""
+    rng = np.random.default_rng(9)
+    img = synth.sinusoid_image(300, 200, rng, channels=3, sigma=30.0)
+    say("seq spill 300x200 memo_bits=8")
+    tree, f, pal, spec, dst = _seq_case(gpu, oracle, img, 64)
+    gpu.walk_seq_batch([spec], gpu.SeqOptions(memo_bits=8))
+    got = H.from_device(dst, (200, 300, 4), np.float32)
+    assert np.array_equal(got, oracle.walk_seq(tree, f))
+
+
+def test_seq_diffusion(gpu, oracle):
+    rng = np.random.default_rng(4)
+    img = synth.sinusoid_image(80, 60, rng, channels=3)
+    say("seq diffusion 0.5")
+    tree, f, pal, spec, dst = _seq_case(gpu, oracle, img, 16)
+    gpu.walk_seq_batch([spec], gpu.SeqOptions(diffusion=0.5))
+    got = H.from_device(dst, (60, 80, 4), np.float32)
+    assert np.array_equal(got, oracle.walk_seq(tree, f, diffusion=0.5))
+
+
+def _blocks_case(gpu, oracle, imgs, colors, block, dst_fmt):
+    import torch
+    trees, specs, dsts, fs = [], [], [], []
+    for img in imgs:
+        tree, f = H.oracle_tree(oracle, img, colors)
+        pal = gpu.Palette.from_tree(tree.data())
+        h, w = img.shape[:2]
+        src = H.to_device(img)
+        dst = H.out_alloc((h, w), dst_fmt)
+        specs.append(gpu.image_spec(src, dst, w, h, H.fmt_of(img), dst_fmt, pal))
+        trees.append((tree, pal, src))
+        dsts.append(dst)
+        fs.append(f)
+    gpu.walk_blocks(specs, block=block)
+    torch.cuda.synchronize()
+    return trees, dsts, fs
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (3, 5), (17, 13), (64, 64), (200, 150), (256, 192),
+                                 (97, 301)])
+@pytest.mark.parametrize("block", [16, 64, 100])
+def test_blocks_float(gpu, oracle, w, h, block):
+    rng = np.random.default_rng(w * 7 + h)
+    imgs = [synth.sinusoid_image(w, h, rng, channels=3) for _ in range(2)]
+    say("blocks", w, h, "B", block)
+    trees, dsts, fs = _blocks_case(gpu, oracle, imgs, 16, block, gpu.PIX_RGBAF32)
+    for (tree, _, _), dst, f in zip(trees, dsts, fs):
+        got = H.from_device(dst, (h, w, 4), np.float32)
+        exp = oracle.walk_blocks(tree, f, block)[0]
+        assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("assoc", [False, True])
+def test_blocks_u16(gpu, oracle, assoc):
+    rng = np.random.default_rng(21)
+    w, h = 128, 96
+    imgs = [synth.sinusoid_image(w, h, rng, channels=4 if assoc else 3) for _ in range(3)]
+    say("blocks u16 assoc", assoc)
+    trees, dsts, fs = _blocks_case(gpu, oracle, imgs, 64 if assoc else 16, 64, gpu.PIX_RGBA16)
+    for (tree, _, _), dst, f in zip(trees, dsts, fs):
+        got = H.from_device(dst, (h, w, 4), np.uint16)
+        exp = oracle.walk_blocks_u16(tree, f, 64)[0]
+        assert np.array_equal(got, exp)
+
+
+def test_blocks_refusals(gpu, oracle):
+    rng = np.random.default_rng(1)
+    img = synth.sinusoid_image(32, 32, rng, channels=3)
+    tree, f = H.oracle_tree(oracle, img, 16)
+    pal = gpu.Palette.from_tree(tree.data())
+    src = H.to_device(img)
+    dst = H.out_alloc((32, 32), gpu.PIX_RGBAF32)
+    spec = gpu.image_spec(src, dst, 32, 32, gpu.PIX_RGB8, gpu.PIX_RGBAF32, pal)
+    with pytest.raises(gpu.EngineError) as ei:
+        gpu.walk_blocks([spec], block=15)
+    assert ei.value.code == 2
+    with pytest.raises(gpu.EngineError):
+        gpu.walk_blocks([], block=64)

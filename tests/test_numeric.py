@@ -1,0 +1,45 @@
+"""Host checks of the numeric contract used by the device code."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_scale_char_reciprocal_form_is_exact_on_every_float():
+    """sa::scale_char (one double multiply) == ScaleQuantumToChar's correctly
+    rounded float division on all 1.2e9 floats in [0, 65535]."""
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "numeric_check")
+        subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-pthread",
+                        os.path.join(HERE, "native", "numeric_check.cpp"), "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    assert "mismatches 0" in out, out
+
+
+def test_weights_hex(oracle):
+    """16^(-i/15) with the host libm, exactly the reference's doubles."""
+    want = ["3ff0000000000000", "3fea9980376421bc", "3fe61c50ae161fd6", "3fe2611186bae675",
+            "3fde8e0ffab4f93f", "3fd965fea53d6e3e", "3fd51cb453b9536e", "3fd18c98599396ab",
+            "3fcd2cd4a3ec542f", "3fc8406003b2ae5e", "3fc428a2f98d728c", "3fc0c1b77b6cebee",
+            "3fbbdb8cdadbe121", "3fb72803c9b50335", "3fb33f972e23dede", "3fb0000000000000"]
+    w = oracle.error_weights()
+    assert [f"{v.view(np.uint64):016x}" for v in w] == want
+    # Closed form within an ulp, and the endpoints exact.
+    assert w[0] == 1.0 and w[15] == 1.0 / 16.0
+    closed = 16.0 ** (-np.arange(16) / 15.0)
+    assert np.max(np.abs(w - closed) / closed) < 1e-15
+    # The reference's rounded table (docs/DESIGN.md:1151-1176).
+    for i, v in [(0, 1.0), (1, 0.831238), (2, 0.690956), (8, 0.227931), (14, 0.075189), (15, 0.0625)]:
+        assert abs(w[i] - v) < 5e-7
+
+
+def test_scaled_weights_order(oracle):
+    """(1/16 * diffusion) * w[k], left to right."""
+    w = oracle.error_weights()
+    for diff in (1.0, 0.5, 0.75):
+        sw = oracle.scaled_weights(diff)
+        assert np.array_equal(sw, (np.float64(1 / 16) * np.float64(diff)) * w)

@@ -1,0 +1,249 @@
+"""Pins the CPU restatement (oracle/) before it is trusted as the checker.
+
+* golden vectors written by the reference's own CPU sources (tests/golden);
+* the reference's recorded octree result for tests/in_gray_alpha.png;
+* what the mathematics fixes: the closed-form curve against the explicit
+  recursion (brute force), the curve endpoint, permutation and length rules,
+  the inverse filter against a triangular Toeplitz solve;
+* cross-checks between independent oracle paths (block engine with one block
+  == cache-free sequential walk on the same positions);
+* against the reference checker (oracle/_ref) directly when it was built.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Ref
+from stipple_amd import synth
+from tests.pngread import read_png, to_q16_rgbaf
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "golden.npz")
+
+
+def d2xy_np(n, level):
+    """Textbook Hilbert d2xy, vectorised (a third, independent statement)."""
+    n = np.asarray(n, np.int64).copy()
+    x = np.zeros_like(n)
+    y = np.zeros_like(n)
+    s = 1
+    while s < (1 << level):
+        rx = 1 & (n // 2)
+        ry = 1 & (n ^ rx)
+        flip = (ry == 0) & (rx == 1)
+        x = np.where(flip, s - 1 - x, x)
+        y = np.where(flip, s - 1 - y, y)
+        swap = ry == 0
+        x, y = np.where(swap, y, x), np.where(swap, x, y)
+        x = x + s * rx
+        y = y + s * ry
+        n = n // 4
+        s *= 2
+    return x, y
+
+
+# ---- curve ------------------------------------------------------------------
+@pytest.mark.parametrize("level", range(0, 10))
+def test_closed_form_equals_recursion(oracle, level):
+    xs, ys = oracle.recursive_walk(level)
+    total = 1 << (2 * level)
+    assert len(xs) == total
+    cx, cy = d2xy_np(np.arange(total), level)
+    assert np.array_equal(xs, cx) and np.array_equal(ys, cy)
+    # The grid is covered exactly once: leaves + the trailing visit.
+    assert len(set(zip(xs.tolist(), ys.tolist()))) == total
+
+
+@pytest.mark.parametrize("level", range(1, 12))
+def test_curve_endpoint(oracle, level):
+    assert oracle.hilbert_point((1 << (2 * level)) - 1, level) == ((1 << level) - 1, 0)
+    assert oracle.hilbert_point(0, level) == (0, 0)
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (3, 5), (17, 13), (640, 480), (1920, 1080), (256, 192),
+                                 (512, 480), (480, 512), (1024, 768), (320, 240), (2048, 3), (7, 900)])
+def test_curve_lengths_and_gap(oracle, w, h):
+    seq = oracle.seq_order(w, h)
+    assert len(seq) == w * h and np.array_equal(np.sort(seq), np.arange(w * h))
+    order, owner = oracle.block_order(w, h)
+    level = oracle.curve_level(w, h)
+    if level == 0:
+        assert len(order) == 0
+        return
+    gap_in = w == (1 << level)  # src/rd_riemersma_cpu.cpp:439-443: power of two and w >= h
+    assert len(order) == w * h + (0 if gap_in else 1)
+    assert order[-1] == 0 and order[0] == 0
+    unvisited = np.flatnonzero(owner < 0)
+    if gap_in:
+        assert unvisited.tolist() == [w - 1]
+    else:
+        assert len(unvisited) == 0
+    assert owner[0] == len(order) - 1
+
+
+def test_curve_level(oracle):
+    assert oracle.curve_level(640, 480) == 10
+    assert oracle.curve_level(1920, 1080) == 11
+    assert oracle.curve_level(3840, 2160) == 12
+    assert oracle.curve_level(1, 1) == 0
+    assert oracle.curve_level(1024, 1) == 10 and oracle.curve_level(1025, 1) == 11
+
+
+# ---- numeric ----------------------------------------------------------------
+def test_scale_char_spot(oracle):
+    for q, want in [(0.0, 0), (-1.0, 0), (128.4, 0), (128.5, 1), (257.0, 1), (65535.0, 255),
+                    (65407.0, 255), (float("nan"), 0)]:
+        assert oracle.scale_char(q) == want, q
+
+
+def test_inverse_filter_solves_toeplitz(oracle):
+    """g = (I - H)^-1 delta, H the 16-tap feedback filter h_k = w_{k-1}/16."""
+    taps = 96
+    g = oracle.inverse_filter(taps)
+    w = oracle.error_weights()
+    h = np.zeros(taps + 1)
+    h[1:17] = w / 16.0
+    A = np.eye(taps + 1)
+    for i in range(taps + 1):
+        for k in range(1, min(i, 16) + 1):
+            A[i, i - k] -= h[k]
+    e0 = np.zeros(taps + 1)
+    e0[0] = 1.0
+    sol = np.linalg.solve(A, e0)
+    assert np.allclose(g, sol, rtol=1e-12, atol=1e-15)
+    # DC gain of F: 1/(1-G), G = sum h = 0.351104 (docs/DESIGN.md:1178-1196).
+    G = h.sum()
+    assert abs(G - 0.351104) < 1e-6
+    assert abs(oracle.inverse_filter(4000).sum() - 1.0 / (1.0 - G)) < 1e-9
+
+
+# ---- octree KAT -------------------------------------------------------------------
+def test_octree_kat_gray_alpha(oracle):
+    """src/rd_im.cpp:195-201: in_gray_alpha.png at 2 colours, associated alpha:
+    52 nodes, one colour (32612, 32612, 32612, 32768)."""
+    s, d, c = read_png(os.path.join(HERE, "golden", "in_gray_alpha.png"))
+    f, has_alpha = to_q16_rgbaf(s, d, c)
+    assert has_alpha
+    t = oracle.tree_build(f, 2, True, synth.is_gray(f))
+    assert t.node_count == 52 and t.count == 1
+    assert np.array_equal(np.round(t.colormap()), [[32612, 32612, 32612, 32768]])
+
+
+# ---- golden vectors from the reference's own code ---------------------------------------
+def _golden():
+    return np.load(GOLDEN)
+
+
+def _golden_names():
+    return [str(n) for n in _golden()["names"]]
+
+
+@pytest.mark.parametrize("name", _golden_names())
+def test_golden_tree(oracle, name):
+    g = _golden()
+    p = name + "/"
+    f = g[p + "input"]
+    colours, assoc, gray, depth, nodes = g[p + "meta"].tolist()
+    t = oracle.tree_build(f, colours, bool(assoc), bool(gray))
+    d = t.data()
+    assert t.depth == depth and t.node_count == nodes
+    assert np.array_equal(d.colormap, g[p + "colormap"])
+    assert np.array_equal(d.child, g[p + "child"])
+    assert np.array_equal(d.qe.view(np.uint64), g[p + "qe"].view(np.uint64))
+    assert np.array_equal(d.total.view(np.uint64), g[p + "total"].view(np.uint64))
+    assert np.array_equal(d.number_unique, g[p + "number_unique"])
+    assert np.array_equal(d.color_number, g[p + "color_number"])
+
+
+@pytest.mark.parametrize("name", _golden_names())
+def test_golden_walks(oracle, name):
+    g = _golden()
+    p = name + "/"
+    f = g[p + "input"]
+    colours, assoc, gray, depth, nodes = g[p + "meta"].tolist()
+    t = oracle.tree_build(f, colours, bool(assoc), bool(gray))
+    assert np.array_equal(oracle.walk_seq(t, f).view(np.uint32), g[p + "seq"].view(np.uint32))
+    assert np.array_equal(oracle.walk_seq(t, f, use_cache=False).view(np.uint32),
+                          g[p + "seq_nocache"].view(np.uint32))
+    if p + "blocks16" in g:
+        assert np.array_equal(oracle.walk_blocks(t, f, 16)[0], g[p + "blocks16"])
+        assert np.array_equal(oracle.walk_blocks(t, f, 64)[0], g[p + "blocks64"])
+        assert np.array_equal(oracle.walk_blocks_u16(t, f, 64)[0], g[p + "blocks64_u16"])
+    order, owner = oracle.block_order(f.shape[1], f.shape[0])
+    assert np.array_equal(order, g[p + "block_order"])
+    assert np.array_equal(owner, g[p + "owner"])
+
+
+def test_golden_weights(oracle):
+    assert np.array_equal(oracle.error_weights(), _golden()["weights"])
+
+
+# ---- independent oracle paths agree ----------------------------------------------------
+@pytest.mark.parametrize("w,h", [(17, 13), (100, 37), (63, 64)])
+def test_one_block_equals_cachefree_walk(oracle, w, h):
+    """With B >= curve length the block engine is the cache-free sequential
+    walk over the block curve (which, for w not a power of two, is the
+    sequential order plus one more visit of the origin)."""
+    rng = np.random.default_rng(w + h)
+    img = synth.sinusoid_image(w, h, rng, channels=3)
+    f = synth.widen_q16(img)
+    t = oracle.tree_build(f, 16, False)
+    order, _ = oracle.block_order(w, h)
+    _, bidx = oracle.walk_blocks(t, f, len(order) + 16, want_index=True)
+    _, sidx = oracle.walk_seq(t, f, use_cache=False, want_index=True)
+    n = w * h
+    assert np.array_equal(order[:n], oracle.seq_order(w, h))
+    assert np.array_equal(bidx[0][:n], sidx[:n].astype(np.uint8))
+
+
+def test_one_pixel_walk(oracle):
+    """1x1: level 0, the trailing ForgetGravity visit is the only visit; the
+    one-colour palette is the pixel itself (rounded through total/count)."""
+    f = np.array([[[1000.0, 20000.0, 65535.0, 65535.0]]], np.float32)
+    t = oracle.tree_build(f, 16, False)
+    assert t.count == 1
+    out = oracle.walk_seq(t, f)
+    assert np.array_equal(out[0, 0, :3], t.colormap()[0, :3].astype(np.float32))
+
+
+def test_memo_first_touch_is_observable(oracle):
+    """The memo changes some outputs (quantize.c's 6-bit key), so a walk
+    without it is a different walk: the checker must keep it."""
+    rng = np.random.default_rng(3)
+    img = synth.sinusoid_image(160, 120, rng, channels=3, sigma=12)
+    f = synth.widen_q16(img)
+    t = oracle.tree_build(f, 64, False)
+    a = oracle.walk_seq(t, f)
+    b = oracle.walk_seq(t, f, use_cache=False)
+    assert not np.array_equal(a, b)
+
+
+# ---- the reference checker directly, where it was built ---------------------------------
+needs_ref = pytest.mark.skipif(not Ref.available(), reason="oracle/_ref not built (no reference checkout)")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(4))
+def test_against_reference_checker(oracle, seed):
+    ref = Ref()
+    rng = np.random.default_rng(100 + seed)
+    for (w, h, col, ch) in [(1, 1, 4, 3), (45, 33, 16, 3), (80, 81, 64, 4), (150, 20, 256, 3)]:
+        img = synth.sinusoid_image(w, h, rng, channels=ch)
+        f = synth.widen_q16(img)
+        to = oracle.tree_build(f, col, ch == 4)
+        tr = ref.tree_build(f, col, ch == 4)
+        assert np.array_equal(to.colormap(), tr.colormap())
+        assert np.array_equal(oracle.walk_seq(to, f), ref.walk_seq(tr, f))
+        assert np.array_equal(oracle.walk_blocks(to, f, 32)[0], ref.walk_blocks(tr, f, 32)[0])
+        oo, ow = oracle.block_order(w, h)
+        ro, rw = ref.build_curve_index(w, h)
+        assert np.array_equal(oo, ro) and np.array_equal(ow, rw)
+
+
+@needs_ref
+def test_reference_curve_self_test():
+    """The reference's own --self-test property (src/rd_cli.cpp:1589-1632)."""
+    ref = Ref()
+    for level in range(1, 9):
+        assert ref.curve_mismatches(level, 0) == 0
