@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Throughput of the exact sequential Riemersma walk on one MI355X.
+
+Workload (the metric's batched configuration): 64 seeded RGBA8 stills of
+uneven size (w, h uniform in [1, 2048], including 1x1, 3x5 and 17x13), alpha
+from random opaque/transparent rectangles, RGB from seeded sinusoids plus
+noise; palette size log-uniform on {2..256}; associated alpha.
+
+One step = the whole hot path over the batch, inputs already in HBM:
+octree classify -> reduce -> colormap on the device for every image, then the
+batched exact walk and the pack into RGBA8 outputs.
+
+Reported beside it, in the same run:
+  * plain: the same work one image per launch, synchronised per image (the
+    shape of the reference's RiemersmaWalkCuda calls);
+  * walk_only: the engine alone with resident palettes;
+  * roofline of the walk kernel (algorithmic 8 B/visit: RGBA8 in + out),
+    timed with HIP events on its own stream;
+  * cpu_baseline: the reference's own CPU engine (oracle/_ref, built from the
+    reference sources) -- or the restatement when that library is absent --
+    on the host's cores, palette build + walk, one image per thread.
+
+Prints ONE JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpixel/s per engine on 1 MI355X, batched vs plain; p50/p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_batch(n, seed):
+    from stipple_amd import synth
+    return synth.still_batch(n, seed=seed)
+
+
+def cpu_baseline(batch, threads_list):
+    """Reference CPU engine (palette build + walk), one image per thread."""
+    from oracle.oracle import Ref, Oracle
+    from stipple_amd import synth
+    use_ref = Ref.available()
+    lib = Ref() if use_ref else Oracle()
+    imgs = [synth.widen_q16(img) for img, _ in batch]
+    pixels = sum(im.shape[0] * im.shape[1] for im in imgs)
+    out = {}
+    for threads in threads_list:
+        work = [im.copy() for im in imgs]
+        t0 = time.perf_counter()
+        trees = [lib.tree_build(im, col, True, False) for im, (_, col) in zip(work, batch)]
+        t1 = time.perf_counter()
+        lib.walk_seq_many(trees, work, threads=threads)
+        t2 = time.perf_counter()
+        out[threads] = {"walk_s": t2 - t1, "palette_s": t1 - t0, "total_s": t2 - t0}
+        del trees
+    return ("reference" if use_ref else "port"), pixels, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--images", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-image-per-launch leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the sequential walk shards by image with no "
+              f"exchange, and this build runs and reports a single MI355X (got --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import stipple_amd as sa
+    from stipple_amd import synth
+
+    sa.init(0)
+    dev = sa.device_name()
+    log("device:", dev)
+    t0 = time.perf_counter()
+    batch = make_batch(args.images, args.seed)
+    log(f"generated {len(batch)} images in {time.perf_counter() - t0:.1f}s")
+    pixels = sum(img.shape[0] * img.shape[1] for img, _ in batch)
+
+    srcs, dsts = [], []
+    for img, _ in batch:
+        srcs.append(torch.from_numpy(img.reshape(-1).view(np.uint8).copy()).cuda())
+        dsts.append(torch.empty(img.shape[0] * img.shape[1] * 4, dtype=torch.uint8, device="cuda"))
+    torch.cuda.synchronize()
+
+    def build_palettes(stream=None):
+        return [sa.Palette.build(src, img.shape[1], img.shape[0], sa.PIX_RGBA8, col, True,
+                                 synth.is_gray(img), stream=stream)
+                for src, (img, col) in zip(srcs, batch)]
+
+    def specs_for(pals):
+        return [sa.image_spec(src, dst, img.shape[1], img.shape[0], sa.PIX_RGBA8, sa.PIX_RGBA8, p)
+                for src, dst, (img, _), p in zip(srcs, dsts, batch, pals)]
+
+    def step():
+        pals = build_palettes()
+        sa.walk_seq_batch(specs_for(pals))
+        torch.cuda.synchronize()
+        return pals
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pals = step()
+    torch.cuda.synchronize()
+    t_batched = (time.perf_counter() - t0) / args.steps
+    log(f"batched chain: {t_batched * 1e3:.1f} ms/step  {pixels / t_batched / 1e6:.1f} Mpx/s")
+
+    # Engine alone, palettes resident, with the walk kernel timed by events.
+    specs = specs_for(pals)
+    sa.timing_enable(True)
+    sa.timing_collect()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sa.walk_seq_batch(specs)
+    torch.cuda.synchronize()
+    t_walk = (time.perf_counter() - t0) / args.steps
+    walk_ms, walk_launches = sa.timing_collect("seq_walk")
+    gather_ms, _ = sa.timing_collect("seq_gather")
+    pack_ms, _ = sa.timing_collect("seq_pack")
+    sa.timing_collect()
+    sa.timing_enable(False)
+    walk_ms /= max(walk_launches, 1)
+    gather_ms /= max(walk_launches, 1)
+    pack_ms /= max(walk_launches, 1)
+    log(f"walk only: {t_walk * 1e3:.1f} ms  kernel {walk_ms:.1f} ms  gather {gather_ms:.2f} pack {pack_ms:.2f}")
+
+    # Palette build alone.
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    build_palettes()
+    torch.cuda.synchronize()
+    t_pal = time.perf_counter() - t0
+
+    plain = None
+    if not args.no_plain:
+        # One image per launch, synchronised each time.
+        torch.cuda.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for k, (src, dst, (img, col)) in enumerate(zip(srcs, dsts, batch)):
+            a = time.perf_counter()
+            p = sa.Palette.build(src, img.shape[1], img.shape[0], sa.PIX_RGBA8, col, True,
+                                 synth.is_gray(img))
+            sa.walk_seq_batch([sa.image_spec(src, dst, img.shape[1], img.shape[0], sa.PIX_RGBA8,
+                                             sa.PIX_RGBA8, p)])
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - a)
+        t_plain = time.perf_counter() - t0
+        plain = {"value": round(pixels / t_plain / 1e6, 3), "ms": round(t_plain * 1e3, 2),
+                 "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)}}
+        log(f"plain: {t_plain * 1e3:.1f} ms  {pixels / t_plain / 1e6:.1f} Mpx/s")
+
+    cpu = None
+    if not args.no_cpu:
+        nproc = min(os.cpu_count() or 1, 16)
+        kind, cpx, res = cpu_baseline(batch, [1, nproc])
+        one = res[1]
+        allc = res[nproc]
+        cpu = {"value": round(cpx / one["total_s"] / 1e6, 3), "unit": "Mpixel/s", "cores": 1,
+               "kind": kind,
+               "sample": f"the same {len(batch)} images ({cpx} px): ColorTree::Build + RiemersmaWalkCpu, one thread",
+               "walk_only_value": round(cpx / one["walk_s"] / 1e6, 3),
+               "all_cores": {"cores": nproc, "value": round(cpx / allc["total_s"] / 1e6, 3),
+                             "walk_only_value": round(cpx / allc["walk_s"] / 1e6, 3)}}
+        log(f"cpu ({kind}): 1 core {cpu['value']} Mpx/s, {nproc} cores {cpu['all_cores']['value']} Mpx/s")
+
+    algo_bytes = 8.0 * pixels  # RGBA8 in + RGBA8 out per visit
+    achieved = algo_bytes / (walk_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(pixels / t_batched / 1e6, 3),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_batched * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(batch)} seeded RGBA8 stills, w,h uniform in [1,2048] "
+                               "(incl. 1x1, 3x5, 17x13), octree palette 2..256 colours built on "
+                               "the device, exact sequential Riemersma walk, RGBA8 out",
+                   "images": len(batch), "pixels": pixels, "engine": "sequential",
+                   "form": "batched", "device": dev},
+        "walk_only": {"value": round(pixels / t_walk / 1e6, 3), "ms": round(t_walk * 1e3, 3),
+                      "kernel_ms": round(walk_ms, 3), "gather_ms": round(gather_ms, 3),
+                      "pack_ms": round(pack_ms, 3)},
+        "palette_build_ms": round(t_pal * 1e3, 3),
+        "plain": plain,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "kernel": "seq_walk_kernel", "note": "latency-bound serial chain; bytes are "
+                     "algorithmic (8 B/visit), per launch over the whole batch"},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_vs_cpu_1core"] = round(line["value"] / cpu["value"], 2)
+    if plain:
+        line["batched_vs_plain"] = round(line["value"] / plain["value"], 2)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

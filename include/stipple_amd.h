@@ -75,6 +75,33 @@ int sa_palette_create(const double* entries, int count, int assoc, const int32_t
 int sa_palette_destroy(sa_palette p);
 int sa_palette_count(sa_palette p);
 
+/* ColorTree::Build on the device (classify, reduce, colormap).  Replaces
+ * rd::ColorTree::Build (include/rd_octree.h:62-63, src/rd_octree.cpp:219-409);
+ * bit-exact with it: same colormap, same node numbering, the same doubles in
+ * every node.  `src` is a device image in src_fmt.  Synchronises `stream`
+ * (the build reads back a few scalars).  With keep_tree the node arrays stay
+ * on the device for sa_palette_tree.  Trees that would exceed MaxQNodes
+ * during classification (quantize.c then prunes whole levels mid-image)
+ * return SA_EUNSUPPORTED. */
+int sa_palette_build(const void* src, int src_fmt, int64_t width, int64_t height, int colors,
+                     int assoc, int gray, int keep_tree, void* stream, sa_palette* out,
+                     char* err, size_t errlen);
+
+/* Palette facts: colours, association, tree depth, node count (ColorTree::
+ * color_count / depth / node_count, include/rd_octree.h:65-99) and whether the
+ * root's quantize_error was summed serially (1) or bounded (0). */
+int sa_palette_info(sa_palette p, int* count, int* assoc, int* depth, int64_t* nodes,
+                    int* exact_root);
+
+/* The colormap, count*4 doubles (ColorTree::colormap, include/rd_octree.h:66). */
+int sa_palette_entries(sa_palette p, double* out, char* err, size_t errlen);
+
+/* The node arrays of a tree built with keep_tree (ColorTree::nodes,
+ * include/rd_octree.h:67); each array `nodes` long, child 16 per node. */
+int sa_palette_tree(sa_palette p, int32_t* child16, int32_t* parent, uint32_t* color_number,
+                    uint64_t* number_unique, double* total4, double* qe, uint32_t* id,
+                    uint32_t* level, char* err, size_t errlen);
+
 /* ---- curve --------------------------------------------------------------- */
 
 /* Hilbert visit order of a w x h image, generated on the device: the
@@ -121,6 +148,28 @@ int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_options* opt, 
  * above 256 entries, as the reference does. */
 int sa_walk_blocks(const sa_image* frames, int n, int block, double diffusion, void* stream,
                    char* err, size_t errlen);
+
+/* The Jacobi approximation: `iterations` fixed-point sweeps of
+ * t = pal[q] + F (x - pal[q]) along the block curve, F = (I-H)^-1 truncated
+ * to `taps`+1 terms, then an owner-filtered scatter.  Replaces
+ * rd::RiemersmaApproxCuda (include/rd_riemersma.h:296-307; ApproxOptions
+ * iterations / taps / fp64 / use_tree).  Frames share one geometry and may
+ * carry their own palettes; dst must not alias src.  Refuses palettes above
+ * 256 entries and taps outside 1..4096, as the reference does. */
+int sa_walk_approx(const sa_image* frames, int n, int iterations, int taps, int fp64,
+                   int use_tree, double diffusion, void* stream, char* err, size_t errlen);
+
+/* ---- kernel timing --------------------------------------------------------- */
+
+/* Brackets every engine kernel launch with HIP events on its own stream while
+ * enabled -- the RD_KERNEL_TIMING diagnostic of the reference
+ * (src/rd_blocks_cuda.cu:820-837).  Kernel names: seq_gather, seq_walk,
+ * seq_pack, blocks_walk, approx_sweep, octree. */
+int sa_timing_enable(int on);
+
+/* Sums the recorded durations of kernel `name` (all kernels when NULL),
+ * waiting for them to finish, and forgets them. */
+int sa_timing_collect(const char* name, double* total_ms, int* launches, char* err, size_t errlen);
 
 #ifdef __cplusplus
 }

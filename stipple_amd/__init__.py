@@ -30,7 +30,10 @@ from ._lib import (  # noqa: F401
     image_spec,
     init,
     lib,
+    timing_collect,
+    timing_enable,
     library_path,
+    walk_approx,
     walk_blocks,
     walk_seq_batch,
 )
@@ -45,7 +48,10 @@ __all__ = [
     "image_spec",
     "init",
     "lib",
+    "timing_collect",
+    "timing_enable",
     "library_path",
+    "walk_approx",
     "walk_blocks",
     "walk_seq_batch",
 ]

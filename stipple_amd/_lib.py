@@ -67,8 +67,17 @@ def lib():
                 "sa_palette_destroy": (i32, [vp]),
                 "sa_palette_count": (i32, [vp]),
                 "sa_curve_order": (i32, [i64, i64, vp, vp, cp, sz]),
+                "sa_palette_build": (i32, [vp, i32, i64, i64, i32, i32, i32, i32, vp,
+                                           C.POINTER(vp), cp, sz]),
+                "sa_palette_info": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                          C.POINTER(i64), C.POINTER(i32)]),
+                "sa_palette_entries": (i32, [vp, vp, cp, sz]),
+                "sa_palette_tree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, cp, sz]),
                 "sa_walk_seq_batch": (i32, [C.POINTER(_Image), i32, C.POINTER(SeqOptions), vp, cp, sz]),
                 "sa_walk_blocks": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
+                "sa_walk_approx": (i32, [C.POINTER(_Image), i32, i32, i32, i32, i32, dbl, vp, cp, sz]),
+                "sa_timing_enable": (i32, [i32]),
+                "sa_timing_collect": (i32, [cp, C.POINTER(dbl), C.POINTER(i32), cp, sz]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -130,6 +139,56 @@ class Palette:
         self.entries = entries
 
     @classmethod
+    def build(cls, src, width, height, src_fmt, colors, assoc, gray=False, keep_tree=False,
+              stream=None):
+        """ColorTree::Build on the device from a device image (sa_palette_build)."""
+        h = C.c_void_p()
+        e = _err()
+        rc = lib().sa_palette_build(_dev_ptr(src), src_fmt, width, height, colors, int(bool(assoc)),
+                                    int(bool(gray)), int(bool(keep_tree)), _stream_ptr(stream),
+                                    C.byref(h), e, len(e))
+        _check(rc, e)
+        self = cls.__new__(cls)
+        self.handle = h.value
+        self.assoc = bool(assoc)
+        info = self.info()
+        self.count = info["count"]
+        self.entries = self.colormap()
+        return self
+
+    def info(self):
+        c, a, d, ex = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        n = C.c_int64()
+        lib().sa_palette_info(self.handle, C.byref(c), C.byref(a), C.byref(d), C.byref(n),
+                              C.byref(ex))
+        return {"count": c.value, "assoc": bool(a.value), "depth": d.value, "nodes": n.value,
+                "exact_root": bool(ex.value)}
+
+    def colormap(self):
+        out = np.zeros((self.info()["count"], 4), np.float64)
+        e = _err()
+        _check(lib().sa_palette_entries(self.handle, _np_ptr(out), e, len(e)), e)
+        return out
+
+    def tree(self):
+        """Node arrays (needs keep_tree=True at build)."""
+        n = self.info()["nodes"]
+        child = np.zeros((n, 16), np.int32)
+        parent = np.zeros(n, np.int32)
+        cn = np.zeros(n, np.uint32)
+        nu = np.zeros(n, np.uint64)
+        tot = np.zeros((n, 4), np.float64)
+        qe = np.zeros(n, np.float64)
+        ids = np.zeros(n, np.uint32)
+        lev = np.zeros(n, np.uint32)
+        e = _err()
+        _check(lib().sa_palette_tree(self.handle, _np_ptr(child), _np_ptr(parent), _np_ptr(cn),
+                                     _np_ptr(nu), _np_ptr(tot), _np_ptr(qe), _np_ptr(ids),
+                                     _np_ptr(lev), e, len(e)), e)
+        return {"child": child, "parent": parent, "color_number": cn, "number_unique": nu,
+                "total": tot, "qe": qe, "id": ids, "level": lev}
+
+    @classmethod
     def from_tree(cls, tree):
         """From any object with colormap/child/parent/color_number/number_unique/assoc
         arrays (e.g. a TreeData)."""
@@ -183,6 +242,16 @@ def walk_blocks(frames, block=64, diffusion=1.0, stream=None):
     _check(rc, e)
 
 
+def walk_approx(frames, iterations=20, taps=96, fp64=False, use_tree=True, diffusion=1.0,
+                stream=None):
+    """The Jacobi approximation over frames of one geometry (sa_walk_approx)."""
+    arr = (_Image * len(frames))(*frames)
+    e = _err()
+    rc = lib().sa_walk_approx(arr, len(frames), iterations, taps, int(bool(fp64)),
+                              int(bool(use_tree)), diffusion, _stream_ptr(stream), e, len(e))
+    _check(rc, e)
+
+
 def curve_order(width, height, stream=None):
     """Device Hilbert visit order (int32 torch tensor of width*height)."""
     import torch
@@ -191,3 +260,18 @@ def curve_order(width, height, stream=None):
     rc = lib().sa_curve_order(width, height, out.data_ptr(), _stream_ptr(stream), e, len(e))
     _check(rc, e)
     return out
+
+
+def timing_enable(on=True):
+    """Bracket engine kernels with HIP events (sa_timing_enable)."""
+    lib().sa_timing_enable(int(bool(on)))
+
+
+def timing_collect(name=None):
+    """(total_ms, launches) of kernel `name` since the last collect."""
+    ms = C.c_double()
+    n = C.c_int()
+    e = _err()
+    _check(lib().sa_timing_collect(name.encode() if name else None, C.byref(ms), C.byref(n), e,
+                                   len(e)), e)
+    return ms.value, n.value

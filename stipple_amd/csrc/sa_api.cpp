@@ -21,6 +21,7 @@
 
 #include "sa_internal.h"
 #include "sa_math.h"
+#include "sa_octree.h"
 #include "sa_palette.h"
 
 struct sa_palette_s {
@@ -29,6 +30,13 @@ struct sa_palette_s {
   void* dev_image = nullptr;
   int count = 0;
   int assoc = 0;
+  // Set when the palette was built on the device.
+  double* dev_entries = nullptr;
+  int depth = 0;
+  int64_t nodes = 0;
+  int exact_root = 0;
+  bool has_tree = false;
+  sa::TreeDev tree{};
 };
 
 namespace {
@@ -213,6 +221,32 @@ int get_weights(double diffusion, hipStream_t s, const double** out, char* err, 
   return SA_OK;
 }
 
+// ---- kernel timing (the RD_KERNEL_TIMING of src/rd_blocks_cuda.cu:820-837) --
+struct Timed {
+  const char* name;
+  hipEvent_t a, b;
+};
+std::mutex g_t_mu;
+bool g_timing = false;
+std::vector<Timed> g_timed;
+
+// Brackets one launch with events on its stream when timing is enabled.
+template <typename Fn>
+hipError_t timed(const char* name, hipStream_t s, Fn&& launch) {
+  if (!g_timing) return launch();
+  Timed t{name, nullptr, nullptr};
+  hipError_t e = hipEventCreate(&t.a);
+  if (e == hipSuccess) e = hipEventCreate(&t.b);
+  if (e == hipSuccess) e = hipEventRecord(t.a, s);
+  if (e != hipSuccess) return e;
+  e = launch();
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(t.b, s);
+  std::lock_guard<std::mutex> g(g_t_mu);
+  g_timed.push_back(t);
+  return e;
+}
+
 bool valid_in_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kRgba16; }
 bool valid_out_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kIndex16; }
 
@@ -367,10 +401,104 @@ extern "C" int sa_palette_create(const double* entries, int count, int assoc,
 
 extern "C" int sa_palette_destroy(sa_palette p) {
   if (!p) return SA_OK;
-  hipDeviceSynchronize();
-  if (p->dev_image) hipFree(p->dev_image);
-  if (p->dev_hdr) hipFree(p->dev_hdr);
+  (void)hipDeviceSynchronize();
+  if (p->dev_image) (void)hipFree(p->dev_image);
+  if (p->dev_hdr) (void)hipFree(p->dev_hdr);
+  if (p->dev_entries) (void)hipFree(p->dev_entries);
+  if (p->has_tree) {
+    sa::TreeDev& t = p->tree;
+    for (void* q : {(void*)t.key, (void*)t.child, (void*)t.parent, (void*)t.id, (void*)t.level,
+                    (void*)t.qe, (void*)t.share, (void*)t.total, (void*)t.nu,
+                    (void*)t.color_number, (void*)t.row})
+      if (q) (void)hipFreeAsync(q, nullptr);
+  }
   delete p;
+  return SA_OK;
+}
+
+extern "C" int sa_palette_build(const void* src, int src_fmt, int64_t w, int64_t h, int colors,
+                                int assoc, int gray, int keep_tree, void* stream, sa_palette* out,
+                                char* err, size_t errlen) {
+  if (!out) return fail(err, errlen, SA_EINVAL, "null output handle");
+  *out = nullptr;
+  if (!src || w < 1 || h < 1) return fail(err, errlen, SA_EINVAL, "bad image");
+  if (!valid_in_fmt(src_fmt)) return fail(err, errlen, SA_EINVAL, "bad pixel format");
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  sa::OctreeOut o;
+  o.keep_tree = keep_tree ? 1 : 0;
+  const sa::OctreeResult r = sa::build_octree(src, src_fmt, w, h, colors, assoc, gray, s, &o);
+  if (r.status != 0) {
+    if (o.image) (void)hipFree(o.image);
+    if (o.hdr) (void)hipFree(o.hdr);
+    if (o.entries) (void)hipFree(o.entries);
+    return fail(err, errlen, r.status, "%s", r.msg);
+  }
+  sa_palette p = new sa_palette_s();
+  p->dev_image = o.image;
+  p->dev_hdr = o.hdr;
+  p->dev_entries = o.entries;
+  p->count = r.count;
+  p->assoc = assoc ? 1 : 0;
+  p->depth = r.depth;
+  p->nodes = r.nodes;
+  p->exact_root = r.exact_root;
+  p->has_tree = o.keep_tree != 0;
+  p->tree = o.tree;
+  hipError_t e = hipMemcpyAsync(&p->hdr, o.hdr, sizeof(sa::PaletteDev), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    sa_palette_destroy(p);
+    return fail(err, errlen, SA_EDEVICE, "palette header: %s", hipGetErrorString(e));
+  }
+  *out = p;
+  return SA_OK;
+}
+
+extern "C" int sa_palette_info(sa_palette p, int* count, int* assoc, int* depth, int64_t* nodes,
+                               int* exact_root) {
+  if (!p) return SA_EINVAL;
+  if (count) *count = p->count;
+  if (assoc) *assoc = p->assoc;
+  if (depth) *depth = p->depth;
+  if (nodes) *nodes = p->nodes;
+  if (exact_root) *exact_root = p->exact_root;
+  return SA_OK;
+}
+
+extern "C" int sa_palette_entries(sa_palette p, double* out, char* err, size_t errlen) {
+  if (!p || !out) return fail(err, errlen, SA_EINVAL, "null argument");
+  if (p->dev_entries) {
+    HIP_TRY(hipMemcpy(out, p->dev_entries, sizeof(double) * 4 * (size_t)p->count,
+                      hipMemcpyDeviceToHost));
+    return SA_OK;
+  }
+  // Uploaded palettes: the search image holds the entries as floats.
+  std::vector<float> ent(4 * (size_t)p->count);
+  HIP_TRY(hipMemcpy(ent.data(), (const unsigned char*)p->dev_image + p->hdr.off_ent,
+                    sizeof(float) * ent.size(), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < ent.size(); ++i) out[i] = (double)ent[i];
+  return SA_OK;
+}
+
+extern "C" int sa_palette_tree(sa_palette p, int32_t* child16, int32_t* parent,
+                               uint32_t* color_number, uint64_t* number_unique, double* total4,
+                               double* qe, uint32_t* id, uint32_t* level, char* err,
+                               size_t errlen) {
+  if (!p) return fail(err, errlen, SA_EINVAL, "null palette");
+  if (!p->has_tree) return fail(err, errlen, SA_EINVAL, "palette was built without keep_tree");
+  const size_t n = (size_t)p->nodes;
+  const sa::TreeDev& t = p->tree;
+  HIP_TRY(hipDeviceSynchronize());
+  if (child16) HIP_TRY(hipMemcpy(child16, t.child, 64 * n, hipMemcpyDeviceToHost));
+  if (parent) HIP_TRY(hipMemcpy(parent, t.parent, 4 * n, hipMemcpyDeviceToHost));
+  if (color_number) HIP_TRY(hipMemcpy(color_number, t.color_number, 4 * n, hipMemcpyDeviceToHost));
+  if (number_unique) HIP_TRY(hipMemcpy(number_unique, t.nu, 8 * n, hipMemcpyDeviceToHost));
+  if (total4) HIP_TRY(hipMemcpy(total4, t.total, 32 * n, hipMemcpyDeviceToHost));
+  if (qe) HIP_TRY(hipMemcpy(qe, t.qe, 8 * n, hipMemcpyDeviceToHost));
+  if (id) HIP_TRY(hipMemcpy(id, t.id, 4 * n, hipMemcpyDeviceToHost));
+  if (level) HIP_TRY(hipMemcpy(level, t.level, 4 * n, hipMemcpyDeviceToHost));
   return SA_OK;
 }
 
@@ -484,10 +612,12 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     int32_t* d_queue = (int32_t*)(scratch + items_bytes);
     HIP_TRY(staged_copy(d_items, items.data(), sizeof(sa::SeqItem) * items.size(), s));
     HIP_TRY(hipMemsetAsync(d_queue, 0, sizeof(int32_t), s));
-    HIP_TRY(sa::launch_seq_gather(d_items, (int)items.size(), max_n, s));
-    HIP_TRY(sa::launch_seq_walk(d_items, (int)items.size(), d_queue, sw, S, o.use_cache, max_img,
-                                waves, assoc, s));
-    HIP_TRY(sa::launch_seq_pack(d_items, (int)items.size(), max_n, s));
+    const int ni = (int)items.size();
+    HIP_TRY(timed("seq_gather", s, [&] { return sa::launch_seq_gather(d_items, ni, max_n, s); }));
+    HIP_TRY(timed("seq_walk", s, [&] {
+      return sa::launch_seq_walk(d_items, ni, d_queue, sw, S, o.use_cache, max_img, waves, assoc, s);
+    }));
+    HIP_TRY(timed("seq_pack", s, [&] { return sa::launch_seq_pack(d_items, ni, max_n, s); }));
     HIP_TRY(hipFreeAsync(scratch, s));
     (void)total;
   }
@@ -555,8 +685,137 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
   sa::BlockFrame* d_fr = nullptr;
   HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::BlockFrame) * fr.size(), s));
   HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::BlockFrame) * fr.size(), s));
-  HIP_TRY(sa::launch_blocks(d_fr, n, curve, w, h, nb, block, sw, assoc, max_img, s));
+  HIP_TRY(timed("blocks_walk", s, [&] {
+    return sa::launch_blocks(d_fr, n, curve, w, h, nb, block, sw, assoc, max_img, s);
+  }));
   if (gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, n, gap, s));
   HIP_TRY(hipFreeAsync(d_fr, s));
+  return SA_OK;
+}
+
+// ---- Jacobi approximation ----------------------------------------------------------
+// BuildInverseFilter (src/rd_approx_cuda.cu:73-88) with the host libm weights.
+static void inverse_filter(int taps, double diffusion, std::vector<double>& g) {
+  double w[16];
+  double weight = 1.0;
+  for (int i = 0; i < 16; ++i) {
+    w[i] = sa::safe_recip(weight);
+    weight *= std::exp(std::log(1.0 / (1.0 / 16.0)) / (16 - 1.0));
+  }
+  g.assign((size_t)taps + 1, 0.0);
+  g[0] = 1.0;
+  for (int d = 1; d <= taps; ++d) {
+    double acc = 0.0;
+    const int kmax = d < 16 ? d : 16;
+    for (int k = 1; k <= kmax; ++k) acc += (diffusion / 16.0) * w[k - 1] * g[(size_t)(d - k)];
+    g[(size_t)d] = acc;
+  }
+}
+
+extern "C" int sa_walk_approx(const sa_image* frames, int n, int iterations, int taps, int fp64,
+                              int use_tree, double diffusion, void* stream, char* err,
+                              size_t errlen) {
+  if (n < 1 || !frames) return fail(err, errlen, SA_EINVAL, "no frames");
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t w = frames[0].width, h = frames[0].height;
+  int assoc = -1, max_img = 0;
+  for (int i = 0; i < n; ++i) {
+    const sa_image& f = frames[i];
+    if (!f.src || !f.dst || !f.palette) return fail(err, errlen, SA_EINVAL, "frame %d: null pointer", i);
+    if (f.width != w || f.height != h) return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
+    if (f.dst == f.src) return fail(err, errlen, SA_EINVAL, "frame %d: dst aliases src", i);
+    if (!valid_in_fmt(f.src_fmt) || !valid_out_fmt(f.dst_fmt))
+      return fail(err, errlen, SA_EINVAL, "frame %d: bad pixel format", i);
+    if (f.palette->count < 1) return fail(err, errlen, SA_EREFUSED, "empty palette");
+    if (f.palette->count > 256)
+      return fail(err, errlen, SA_EREFUSED,
+                  "the approx engine carries one palette index per byte, so %d colours would alias "
+                  "onto the first 256", f.palette->count);
+    if (assoc < 0) assoc = f.palette->assoc;
+    if (assoc != f.palette->assoc)
+      return fail(err, errlen, SA_EUNSUPPORTED, "frames mix associated and plain alpha");
+    max_img = std::max(max_img, f.palette->hdr.bytes);
+  }
+  if (taps < 1 || taps > 4096) return fail(err, errlen, SA_EREFUSED, "--approx-taps out of range");
+  if (iterations < 0) return fail(err, errlen, SA_EINVAL, "negative iterations");
+  const int level = curve_level(w, h);
+  const bool gap_in = level > 0 && w == ((int64_t)1 << level);
+  const int64_t nb = level > 0 ? w * h - (gap_in ? 1 : 0) + 1 : 0;
+  if (nb < w * h)  // src/rd_approx_cuda.cu:271-274 (a 1x1 image has an empty curve)
+    return fail(err, errlen, SA_EREFUSED,
+                "the compacted curve is shorter than the image; the Hilbert transcription is wrong");
+  const size_t lds = sa::approx_lds_bytes(max_img, taps, fp64, assoc);
+  if (lds > g_lds_per_cu)
+    return fail(err, errlen, SA_EUNSUPPORTED, "taps %d with this palette need %zu B of LDS", taps, lds);
+  const int32_t* curve = nullptr;
+  rc = get_curve(w, h, s, &curve, err, errlen);
+  if (rc) return rc;
+  std::vector<double> gd;
+  inverse_filter(taps, diffusion, gd);
+  std::vector<float> gf(gd.size());
+  for (size_t i = 0; i < gd.size(); ++i) gf[i] = (float)gd[i];
+  const size_t gbytes = fp64 ? 8 * gd.size() : 4 * gf.size();
+  const size_t elem = fp64 ? 8 : 4;
+  std::vector<sa::BlockFrame> fr((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    fr[(size_t)i].src = frames[i].src;
+    fr[(size_t)i].dst = frames[i].dst;
+    fr[(size_t)i].pal = frames[i].palette->dev_hdr;
+    fr[(size_t)i].src_fmt = frames[i].src_fmt;
+    fr[(size_t)i].dst_fmt = frames[i].dst_fmt;
+  }
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t b_fr = up(sizeof(sa::BlockFrame) * fr.size()), b_g = up(gbytes);
+  const size_t b_cx = up(elem * 4 * (size_t)nb * (size_t)n), b_q = up((size_t)nb * (size_t)n);
+  unsigned char* scratch = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&scratch, b_fr + b_g + b_cx + 2 * b_q, s));
+  sa::BlockFrame* d_fr = (sa::BlockFrame*)scratch;
+  void* d_g = scratch + b_fr;
+  void* d_cx = scratch + b_fr + b_g;
+  uint8_t* q0 = scratch + b_fr + b_g + b_cx;
+  uint8_t* q1 = q0 + b_q;
+  HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::BlockFrame) * fr.size(), s));
+  HIP_TRY(staged_copy(d_g, fp64 ? (const void*)gd.data() : (const void*)gf.data(), gbytes, s));
+  uint8_t* result = nullptr;
+  HIP_TRY(timed("approx", s, [&] {
+    return sa::launch_approx(d_fr, n, curve, nb, iterations, taps, d_g, fp64, use_tree, assoc,
+                             max_img, d_cx, q0, q1, &result, s);
+  }));
+  if (gap_in) HIP_TRY(sa::launch_blocks_fill(d_fr, n, w - 1, s));
+  HIP_TRY(hipFreeAsync(scratch, s));
+  return SA_OK;
+}
+
+// ---- timing ---------------------------------------------------------------------
+extern "C" int sa_timing_enable(int on) {
+  std::lock_guard<std::mutex> g(g_t_mu);
+  g_timing = on != 0;
+  return SA_OK;
+}
+
+extern "C" int sa_timing_collect(const char* name, double* total_ms, int* launches, char* err,
+                                 size_t errlen) {
+  std::lock_guard<std::mutex> g(g_t_mu);
+  double sum = 0.0;
+  int n = 0;
+  std::vector<Timed> keep;
+  for (Timed& t : g_timed) {
+    if (name && std::strcmp(name, t.name) != 0) {
+      keep.push_back(t);
+      continue;
+    }
+    HIP_TRY(hipEventSynchronize(t.b));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+    sum += ms;
+    ++n;
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  g_timed.swap(keep);
+  if (total_ms) *total_ms = sum;
+  if (launches) *launches = n;
   return SA_OK;
 }

@@ -24,6 +24,13 @@
 
 #include <stdint.h>
 
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define SA_PAL_HD __host__ __device__
+#else
+#define SA_PAL_HD
+#endif
+
 namespace sa {
 
 constexpr int16_t kNoChild = -1;
@@ -48,15 +55,14 @@ struct PaletteLayout {
   int32_t off_col, off_ent, off_sub, off_child, off_cand, bytes;
 };
 
-inline PaletteLayout palette_layout(int count, int rows, int kids) {
-  auto up = [](int32_t v) { return (v + 15) & ~15; };
+SA_PAL_HD inline PaletteLayout palette_layout(int count, int rows, int kids) {
   PaletteLayout L;
   L.off_col = 0;
-  L.off_ent = up(L.off_col + 32 * count);
-  L.off_sub = up(L.off_ent + 16 * count);
-  L.off_child = up(L.off_sub + 8 * rows);
-  L.off_cand = up(L.off_child + 2 * rows * kids);
-  L.bytes = up(L.off_cand + 2 * count);
+  L.off_ent = (L.off_col + 32 * count + 15) & ~15;
+  L.off_sub = (L.off_ent + 16 * count + 15) & ~15;
+  L.off_child = (L.off_sub + 8 * rows + 15) & ~15;
+  L.off_cand = (L.off_child + 2 * rows * kids + 15) & ~15;
+  L.bytes = (L.off_cand + 2 * count + 15) & ~15;
   return L;
 }
 

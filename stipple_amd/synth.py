@@ -43,18 +43,24 @@ def ramp_image(w=640, h=480, sigma=8.0, seed=1) -> np.ndarray:
 
 # -- C2: a batch of uneven RGBA8 stills --------------------------------------
 def sinusoid_image(w, h, rng, channels=4, sigma=6.0, rects=True) -> np.ndarray:
-    x = np.arange(w, dtype=np.float64)[None, :]
-    y = np.arange(h, dtype=np.float64)[:, None]
-    planes = []
-    for _ in range(3):
-        acc = np.zeros((h, w))
+    """RGB = sum of 3 random-phase 2-D sinusoids per channel + N(0, sigma);
+    alpha from 1-8 random opaque/transparent rectangles (when 4 channels).
+    sin(ax + by + c) is evaluated separably as sin(ax+c)cos(by) + cos(ax+c)sin(by)."""
+    x = np.arange(w, dtype=np.float64)
+    y = np.arange(h, dtype=np.float64)
+    planes = np.empty((h, w, 3), np.float32)
+    for c in range(3):
+        acc = np.zeros((h, w), np.float32)
         for _ in range(3):
             fx, fy = rng.uniform(-0.03, 0.03, 2)
             ph = rng.uniform(0, 2 * math.pi)
-            acc += np.sin(2 * math.pi * (fx * x + fy * y) + ph)
-        planes.append(127.5 + 127.5 * acc / 3.0)
-    rgb = np.stack(planes, -1) + rng.normal(0.0, sigma, (h, w, 3))
-    rgb = np.clip(np.round(rgb), 0, 255).astype(np.uint8)
+            ax = 2 * math.pi * fx * x + ph
+            by = 2 * math.pi * fy * y
+            acc += np.outer(np.cos(by), np.sin(ax)).astype(np.float32)
+            acc += np.outer(np.sin(by), np.cos(ax)).astype(np.float32)
+        planes[..., c] = 127.5 + 127.5 * acc / 3.0
+    planes += rng.standard_normal((h, w, 3), dtype=np.float32) * np.float32(sigma)
+    rgb = np.clip(np.rint(planes), 0, 255).astype(np.uint8)
     if channels == 3:
         return rgb
     alpha = np.full((h, w), 255, np.uint8)
