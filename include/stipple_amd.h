@@ -128,7 +128,7 @@ typedef struct {
 typedef struct {
   double diffusion; /* dither:diffusion-amount (rd::DitherParams::diffusion), 1.0 */
   int32_t use_cache;/* quantize.c's memo; 1 reproduces ImageMagick */
-  int32_t memo_bits;/* LDS memo slots per wavefront = 2^memo_bits, 0 = auto */
+  int32_t memo_bits;/* LDS memo line slots per wavefront = 2^memo_bits (4..10), 0 = auto */
   int32_t waves;    /* persistent wavefronts, 0 = auto */
   int32_t reserved;
 } sa_seq_options;
@@ -140,6 +140,12 @@ typedef struct {
  * rd::RiemersmaWalkCpu. */
 int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_options* opt, void* stream,
                       char* err, size_t errlen);
+
+/* Counters of the sequential walks since the last call, then reset:
+ * out[0] memo misses (full octree searches), out[1] LDS memo line fills,
+ * out[2] line fills that read HBM, out[3] reserved.  The analogue of the
+ * reference's RD_CC_STATS (include/rd_cuda_common.cuh:210-229). */
+int sa_seq_stats(uint64_t* out, char* err, size_t errlen);
 
 /* The block-parallel walk over `n` frames of one geometry (each frame may
  * carry its own palette).  Replaces rd::RiemersmaBlocksCuda

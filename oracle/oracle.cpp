@@ -639,6 +639,7 @@ struct SeqState {
   D4 err[kQueue];
   std::vector<int> memo;
   bool use_cache;
+  int64_t misses = 0;
 };
 
 int visit(SeqState& s, float* px) {
@@ -660,6 +661,7 @@ int visit(SeqState& s, float* px) {
   int idx = s.use_cache ? s.memo[(size_t)key] : -1;
   if (idx < 0) {
     idx = t.lookup(p);
+    ++s.misses;
     if (s.use_cache) s.memo[(size_t)key] = idx;
   }
   const double* e = &t.cmap[4 * (size_t)idx];
@@ -676,8 +678,8 @@ int visit(SeqState& s, float* px) {
   return idx;
 }
 
-void walk_seq_impl(const OrcTree* t, float* rgba, int64_t w, int64_t h,
-                   double diffusion, int use_cache, int32_t* index_out) {
+int64_t walk_seq_impl(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                      double diffusion, int use_cache, int32_t* index_out) {
   SeqState s;
   s.t = t;
   scaled_weights(diffusion, s.sw);
@@ -691,6 +693,7 @@ void walk_seq_impl(const OrcTree* t, float* rgba, int64_t w, int64_t h,
     if (index_out) index_out[k] = idx;
     ++k;
   });
+  return s.misses;
 }
 
 }  // namespace
@@ -703,6 +706,12 @@ extern "C" void orc_walk_seq(const OrcTree* t, float* rgba, int64_t w, int64_t h
 extern "C" void orc_walk_seq_index(const OrcTree* t, float* rgba, int64_t w, int64_t h,
                                    double diffusion, int use_cache, int32_t* index_out) {
   walk_seq_impl(t, rgba, w, h, diffusion, use_cache, index_out);
+}
+
+// Number of memo misses (full octree searches) of one walk.
+extern "C" int64_t orc_walk_seq_misses(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                                       double diffusion) {
+  return walk_seq_impl(t, rgba, w, h, diffusion, 1, nullptr);
 }
 
 extern "C" int orc_walk_seq_many(OrcTree* const* trees, float* const* images,

@@ -71,6 +71,9 @@ void orc_walk_seq(const OrcTree* t, float* rgba, int64_t w, int64_t h,
  * visit order (w*h entries).  Used to check index outputs. */
 void orc_walk_seq_index(const OrcTree* t, float* rgba, int64_t w, int64_t h,
                         double diffusion, int use_cache, int32_t* index_out);
+/* Memo misses (full octree searches) of one cached walk (rgba dithered in place). */
+int64_t orc_walk_seq_misses(const OrcTree* t, float* rgba, int64_t w, int64_t h,
+                            double diffusion);
 /* RiemersmaBlocksCpu unfused path (rd_riemersma_cpu.cpp:477-805).  batch is
  * frames*w*h*4 floats dithered in place; index_out (frames*n bytes, n = block
  * curve length) receives the chosen index per curve position when non-null.

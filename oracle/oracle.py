@@ -103,6 +103,12 @@ class Oracle(_Lib):
         self._inv = f("inverse_filter", None, _i32, _dbl, _p)
         self._jac = f("jacobi", _i32, _p, _p, _i64, _i64, _i32, _i32, _i32, _i32, _dbl, _p)
         self._many = f("walk_seq_many", _i32, _p, _p, _p, _p, _i32, _dbl, _i32)
+        self._misses = f("walk_seq_misses", _i64, _p, _p, _i64, _i64, _dbl)
+
+    def walk_seq_misses(self, tree, img, diffusion=1.0):
+        a = _rgba_f32(img).copy()
+        h, w = a.shape[:2]
+        return int(self._misses(tree.h, _ptr(a), w, h, diffusion))
 
     # -- numeric -------------------------------------------------------
     def error_weights(self):

@@ -30,6 +30,7 @@ from ._lib import (  # noqa: F401
     image_spec,
     init,
     lib,
+    seq_stats,
     timing_collect,
     timing_enable,
     library_path,
@@ -48,6 +49,7 @@ __all__ = [
     "image_spec",
     "init",
     "lib",
+    "seq_stats",
     "timing_collect",
     "timing_enable",
     "library_path",

@@ -76,6 +76,7 @@ def lib():
                 "sa_walk_seq_batch": (i32, [C.POINTER(_Image), i32, C.POINTER(SeqOptions), vp, cp, sz]),
                 "sa_walk_blocks": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_walk_approx": (i32, [C.POINTER(_Image), i32, i32, i32, i32, i32, dbl, vp, cp, sz]),
+                "sa_seq_stats": (i32, [vp, cp, sz]),
                 "sa_timing_enable": (i32, [i32]),
                 "sa_timing_collect": (i32, [cp, C.POINTER(dbl), C.POINTER(i32), cp, sz]),
             }
@@ -275,3 +276,11 @@ def timing_collect(name=None):
     _check(lib().sa_timing_collect(name.encode() if name else None, C.byref(ms), C.byref(n), e,
                                    len(e)), e)
     return ms.value, n.value
+
+
+def seq_stats():
+    """Walk counters since the last call: misses, line fills, HBM line loads."""
+    out = np.zeros(4, np.uint64)
+    e = _err()
+    _check(lib().sa_seq_stats(_np_ptr(out), e, len(e)), e)
+    return {"misses": int(out[0]), "fills": int(out[1]), "loads": int(out[2])}

@@ -45,6 +45,7 @@ using sa::PaletteDev;
 
 std::mutex g_mu;
 int g_device = -1;
+uint32_t* g_stats = nullptr;  // walk counters: misses, line fills, line loads from HBM
 int g_cus = 256;
 size_t g_lds_per_cu = 160 * 1024;
 char g_name[256] = "";
@@ -77,6 +78,10 @@ int ensure_init(char* err, size_t errlen) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, dev));
   g_cus = prop.multiProcessorCount;
+  if (!g_stats) {
+    HIP_TRY(hipMalloc((void**)&g_stats, 16));
+    HIP_TRY(hipMemset(g_stats, 0, 16));
+  }
   snprintf(g_name, sizeof(g_name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName, g_cus);
   g_device = dev;
   return SA_OK;
@@ -348,7 +353,9 @@ extern "C" int sa_palette_create(const double* entries, int count, int assoc,
   std::vector<unsigned char> img((size_t)L.bytes, 0);
   double* col = (double*)(img.data() + L.off_col);
   float* ent = (float*)(img.data() + L.off_ent);
+  float* cent = (float*)(img.data() + L.off_cent);
   int32_t* sub = (int32_t*)(img.data() + L.off_sub);
+  uint32_t* desc = (uint32_t*)(img.data() + L.off_desc);
   int16_t* child = (int16_t*)(img.data() + L.off_child);
   uint16_t* cnd = (uint16_t*)(img.data() + L.off_cand);
   for (int i = 0; i < count; ++i) {
@@ -371,7 +378,11 @@ extern "C" int sa_palette_create(const double* entries, int count, int assoc,
       child[r * kids + c] = v;
     }
   }
-  for (size_t i = 0; i < cand.size() && i < (size_t)count; ++i) cnd[i] = cand[i];
+  for (size_t i = 0; i < cand.size() && i < (size_t)count; ++i) {
+    cnd[i] = cand[i];
+    for (int c = 0; c < 4; ++c) cent[4 * i + c] = ent[4 * cand[i] + c];
+  }
+  for (int e = 0; e < sa::desc_entries(kids); ++e) desc[e] = sa::descent3(child, kids, (uint32_t)e);
   sa_palette p = new sa_palette_s();
   p->count = count;
   p->assoc = assoc ? 1 : 0;
@@ -382,7 +393,9 @@ extern "C" int sa_palette_create(const double* entries, int count, int assoc,
   h.rows = (int32_t)rows.size();
   h.bytes = L.bytes;
   h.off_ent = L.off_ent;
+  h.off_cent = L.off_cent;
   h.off_sub = L.off_sub;
+  h.off_desc = L.off_desc;
   h.off_child = L.off_child;
   h.off_cand = L.off_cand;
   h.pad_ = 0;
@@ -533,6 +546,8 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       return fail(err, errlen, SA_EINVAL, "image %d: in-place needs the same format", i);
     if (im.palette->hdr.bytes > 96 * 1024)
       return fail(err, errlen, SA_EUNSUPPORTED, "image %d: palette search image above 96 KiB", i);
+    if (im.palette->count > 65535)
+      return fail(err, errlen, SA_EUNSUPPORTED, "image %d: the walk's memo holds indices below 65535", i);
   }
   const double* sw = nullptr;
   rc = get_weights(o.diffusion, s, &sw, err, errlen);
@@ -556,25 +571,26 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       total += px;
     }
     const int K = assoc ? 24 : 18;
+    // memo_bits: log2 of the LDS line slots (132 B each: tag + 64 entries).
     int S = o.memo_bits;
     if (S <= 0) {
-      S = 15;
-      while (S > 10 && (4u << S) + (size_t)max_img > 150 * 1024) --S;
+      S = 10;
+      while (S > 4 && (132u << S) + (size_t)max_img > 158 * 1024) --S;
     }
-    if (S < 8 || S > 16) return fail(err, errlen, SA_EINVAL, "memo_bits %d outside 8..16", S);
-    const size_t lds = (o.use_cache ? (4u << S) : 0u) + (size_t)max_img;
+    if (S < 4 || S > 10) return fail(err, errlen, SA_EINVAL, "memo_bits %d outside 4..10", S);
+    const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)max_img;
     if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "LDS %zu too large", lds);
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
     int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
     waves = std::min<int>(waves, (int)sel.size());
-    // Scratch: curves, gathered inputs, indices, memo spill tables.
+    // Scratch: gathered inputs, indices, HBM memo tables.
     std::vector<sa::SeqItem> items(sel.size());
     size_t xs_bytes = 0, idx_bytes = 0, spill_bytes = 0;
     for (size_t k = 0; k < sel.size(); ++k) {
       const int64_t px = images[sel[k]].width * images[sel[k]].height;
       xs_bytes += 32 * (size_t)px;
       idx_bytes += (2 * (size_t)px + 255) & ~(size_t)255;
-      if (o.use_cache) spill_bytes += (size_t)4 << K;
+      if (o.use_cache) spill_bytes += (size_t)2 << K;
     }
     unsigned char* scratch = nullptr;
     const size_t items_bytes = (sizeof(sa::SeqItem) * sel.size() + 255) & ~(size_t)255;
@@ -598,10 +614,10 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       p += 32 * (size_t)px;
       it.idx = (uint16_t*)p;
       p += (2 * (size_t)px + 255) & ~(size_t)255;
-      it.spill = nullptr;
+      it.memo = nullptr;
       if (o.use_cache) {
-        it.spill = (int32_t*)p;
-        p += (size_t)4 << K;
+        it.memo = (uint16_t*)p;
+        p += (size_t)2 << K;
       }
       it.pal = im.palette->dev_hdr;
       it.n = px;
@@ -615,7 +631,8 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     const int ni = (int)items.size();
     HIP_TRY(timed("seq_gather", s, [&] { return sa::launch_seq_gather(d_items, ni, max_n, s); }));
     HIP_TRY(timed("seq_walk", s, [&] {
-      return sa::launch_seq_walk(d_items, ni, d_queue, sw, S, o.use_cache, max_img, waves, assoc, s);
+      return sa::launch_seq_walk(d_items, ni, d_queue, sw, S, o.use_cache, max_img, waves, assoc,
+                                 g_stats, s);
     }));
     HIP_TRY(timed("seq_pack", s, [&] { return sa::launch_seq_pack(d_items, ni, max_n, s); }));
     HIP_TRY(hipFreeAsync(scratch, s));
@@ -817,5 +834,16 @@ extern "C" int sa_timing_collect(const char* name, double* total_ms, int* launch
   g_timed.swap(keep);
   if (total_ms) *total_ms = sum;
   if (launches) *launches = n;
+  return SA_OK;
+}
+
+extern "C" int sa_seq_stats(uint64_t* out, char* err, size_t errlen) {
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  uint32_t h[4] = {0, 0, 0, 0};
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(h, g_stats, 16, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemset(g_stats, 0, 16));
+  for (int i = 0; i < 4; ++i) out[i] = h[i];
   return SA_OK;
 }

@@ -73,7 +73,7 @@ struct SeqItem {
   const int32_t* curve;   // visit order: w*h pixel ids
   double* xs;             // scratch: 4 doubles per visit (associated input)
   uint16_t* idx;          // scratch: palette index per visit
-  int32_t* spill;         // scratch: 1<<K memo backing (K = 18, or 24 with alpha)
+  uint16_t* memo;         // scratch: 1<<K memo entries (K = 18, or 24 with alpha)
   const PaletteDev* pal;  // device palette header
   int64_t n;              // visits == w*h
   int32_t src_fmt;
@@ -105,7 +105,7 @@ int64_t curve_chunks(int level);
 hipError_t launch_seq_gather(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
 hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, const double* sw,
                            int memo_bits, int use_cache, int lds_image_bytes, int waves,
-                           int assoc_any, hipStream_t s);
+                           int assoc_any, uint32_t* stats, hipStream_t s);
 hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
 
 hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,

@@ -35,9 +35,11 @@ constexpr double kInv257 = 1.0 / 257.0;
 
 // include/rd_types.h:39-43 ClampPixel (narrows through float).
 SA_HD float clamp_pixel(double v) {
-  if (v < 0.0) return 0.0f;
-  if (v >= kRange) return 65535.0f;
-  return (float)v;
+  // Selects rather than branches: the same three cases, no divergent flow.
+  float f = (float)v;
+  f = (v >= kRange) ? 65535.0f : f;
+  f = (v < 0.0) ? 0.0f : f;
+  return f;
 }
 
 // include/rd_types.h:133-138 ScaleQuantumToChar, with the float division.

@@ -688,6 +688,7 @@ __global__ void k_reduce(TreeDev t, int kids, int64_t maxc, int depth, int64_t M
   unsigned char* img = out.image;
   double* col = (double*)(img + L.off_col);
   float* ent = (float*)(img + L.off_ent);
+  float* cent = (float*)(img + L.off_cent);
   int32_t* sub = (int32_t*)(img + L.off_sub);
   int16_t* child = (int16_t*)(img + L.off_child);
   uint16_t* cand = (uint16_t*)(img + L.off_cand);
@@ -717,7 +718,11 @@ __global__ void k_reduce(TreeDev t, int kids, int64_t maxc, int depth, int64_t M
         }
         continue;
       }
-      if (t.nu[n] != 0) cand[ncand++] = (uint16_t)t.color_number[n];
+      if (t.nu[n] != 0) {
+        const uint32_t cn = t.color_number[n];
+        for (int k = 0; k < 4; ++k) cent[4 * ncand + k] = ent[4 * cn + k];
+        cand[ncand++] = (uint16_t)cn;
+      }
       const int32_t r = t.row[n];
       if (r >= 0) {
         sub[2 * r] = first[sp];
@@ -730,6 +735,7 @@ __global__ void k_reduce(TreeDev t, int kids, int64_t maxc, int depth, int64_t M
       --sp;
     }
   }
+  // The descent table is filled by k_descent (one thread per entry).
   PaletteDev* h = out.hdr;
   h->count = cnt;
   h->assoc = assoc;
@@ -737,12 +743,25 @@ __global__ void k_reduce(TreeDev t, int kids, int64_t maxc, int depth, int64_t M
   h->rows = rows;
   h->bytes = L.bytes;
   h->off_ent = L.off_ent;
+  h->off_cent = L.off_cent;
   h->off_sub = L.off_sub;
+  h->off_desc = L.off_desc;
   h->off_child = L.off_child;
   h->off_cand = L.off_cand;
   h->pad_ = 0;
   h->image = out.image;
   (void)M;
+}
+
+// The three-level descent table of the search image (sa_palette.h).
+__global__ void k_descent(const PaletteDev* hdr, int32_t* info) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (info[6] != 0) return;
+  const PaletteDev h = *hdr;
+  if (e >= desc_entries(h.kids)) return;
+  unsigned char* img = (unsigned char*)h.image;
+  ((uint32_t*)(img + h.off_desc))[e] =
+      descent3((const int16_t*)(img + h.off_child), h.kids, (uint32_t)e);
 }
 
 // Exact root chain (only when the bound cannot separate it): the running
@@ -1102,6 +1121,7 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
     OT_TRY(hipMalloc((void**)&pb.entries, sizeof(double) * 4 * (size_t)maxc));
     out.entries = pb.entries;
     k_reduce<<<1, 1, 0, s>>>(t, kids, maxc, depth, M, assoc, root_qe, flat, tmp, info, pb);
+    k_descent<<<blocks(desc_entries(kids), 256), 256, 0, s>>>(pb.hdr, info);
     OT_TRY(hipGetLastError());
     OT_TRY(hipMemcpyAsync(hinfo, info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     OT_TRY(hipStreamSynchronize(s));

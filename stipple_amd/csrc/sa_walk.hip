@@ -6,33 +6,33 @@
 // (built with -ffp-contract=off).
 //
 // Mapping to CDNA4.  The walk is a serial chain, so one wavefront walks one
-// image and many images walk at once (a persistent grid pulls images from a
-// device queue, largest first).  Inside the wavefront the 64 lanes are four
-// DPP rows of 16: row c holds channel c (r, g, b, a) and lane j of a row holds
-// the running sum of one of the next 16 visits.  Visit T's diffusion is
+// image and many images walk at once: a persistent grid pulls images from a
+// device queue, largest first.  Inside the wavefront the 64 lanes are four DPP
+// rows of 16; row c holds channel c (r, g, b, a) and lane j of a row holds the
+// running sum of visit t+j.  Visit t's diffusion is
 //
-//     p_T = x_T + w0*e_{T-16} + w1*e_{T-15} + ... + w15*e_{T-1}
+//     p_t = x_t + w0*e_{t-16} + w1*e_{t-15} + ... + w15*e_{t-1}
 //
 // summed left to right.  Only the last term depends on the visit just
-// finished, so when e_{T-1} is known every in-flight visit T..T+15 adds its
-// own tap of it in ONE v_mul_f64 + ONE v_add_f64 for all four channels (lane j
-// holds the weight for its visit's tap).  The lane holding visit T is then
-// complete; after the visit it restarts with x_{T+16}.  Each visit therefore
-// issues the reference's 32 multiply-adds per channel spread over 16 lanes,
-// and the serial chain per visit is one mul, one add, the clamp, the memo
-// probe and the residual.
+// finished, so when e_{t-1} is known every in-flight visit adds its own tap of
+// it -- lane j adds w_{15-j} -- in ONE v_mul_f64 + ONE v_add_f64 for all
+// channels.  Lane 0 is then complete; after the visit every row rotates left by
+// one lane (DPP row_ror:15) and lane 15 restarts with x_{t+16}, taken from a
+// register chunk of the next inputs that rotates in step (prefetched two
+// chunks ahead with coalesced loads).  The serial chain per visit is one mul,
+// one add, the clamp, the memo probe and the residual.
 //
 // Memo (quantize.c's cache, src/rd_riemersma_cpu.cpp:75-94).  First-touch
-// semantics are observable, so it is reproduced exactly: an LDS direct-mapped
-// table of 2^S entries {valid, spilled, tag, index} in front of a per-image
-// global table that holds every entry ever evicted.  A key whose slot was never
-// evicted is known untouched without leaving LDS; a key whose slot has spilled
-// is looked up in the global table (whose group of keys is marked untouched on
-// the slot's first spill, so no per-image clear is needed).
+// semantics are observable, so it is reproduced exactly: a zeroed per-image
+// table in HBM (entry = index+1, 0 = untouched) is the authority, and an LDS
+// cache of its 64-entry lines (key >> 6, direct-mapped, dirty lines written
+// back on eviction) serves the walk.  A line whose slot has never held
+// another line is known to be all zero and is filled without touching HBM.
 //
-// Palette search: the search image (sa_palette.h) is copied into LDS once per
-// image; a memo miss descends the octree with uniform LDS reads and scores the
-// candidate range across lanes (last index at the minimum wins, which is the
+// Palette search on a miss: the search image (sa_palette.h) is staged in LDS
+// once per image; the first three descent levels are one table read, the
+// rest are uniform LDS reads, and the candidate range is scored across lanes
+// with a DPP min-reduction (the last index at the minimum wins, which is the
 // reference's `<=` rule).
 #include "sa_internal.h"
 #include "sa_math.h"
@@ -48,41 +48,59 @@ __device__ __forceinline__ double rl(double v, int lane) {
 }
 __device__ __forceinline__ int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-template <int L>
-__device__ __forceinline__ double row_bcast(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + L, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + L, 0xF, 0xF, false);
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v, double fill) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double wave_min(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
-  return v;
+constexpr int kRowBcast0 = 0x150;  // row_newbcast:0
+constexpr int kRowRor15 = 0x12F;   // lane j <- lane (j+1) mod 16
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+
+// Minimum over the wavefront of a per-lane double (lanes outside the
+// candidate range pass +inf).  Rows by DPP shift-scans, then the four row
+// minima.
+__device__ __forceinline__ double wave_min(double v, bool one_row) {
+  const double inf = __builtin_inf();
+  v = fmin(v, dpp<kRowShr1>(v, inf));
+  v = fmin(v, dpp<kRowShr2>(v, inf));
+  v = fmin(v, dpp<kRowShr4>(v, inf));
+  v = fmin(v, dpp<kRowShr8>(v, inf));
+  if (one_row) return rl(v, 15);
+  return fmin(fmin(rl(v, 15), rl(v, 31)), fmin(rl(v, 47), rl(v, 63)));
 }
 
 struct Search {
-  const double* col;     // 4 per entry (LDS)
-  const float4* ent;     // (LDS)
-  const int2* sub;       // (LDS)
-  const int16_t* child;  // (LDS)
-  const uint16_t* cand;  // (LDS)
+  const double* col;     // 4 per entry
+  const float4* cent;    // candidates in post-order
+  const int2* sub;
+  const uint32_t* desc;
+  const int16_t* child;
+  const uint16_t* cand;
   int kids;
 };
 
-// Descent + candidate scan for one clamped target; uniform result.
-// t0..t3: the target (double) already uniform; c0..c3 the scaled bytes.
+// Descent + ordered scan for one clamped target (src/rd_riemersma_cpu.cpp:82-92,
+// src/rd_octree.cpp:178-217); uniform result.
 template <bool ASSOC>
-__device__ __attribute__((noinline)) int lookup(const Search& s, unsigned c0, unsigned c1, unsigned c2, unsigned c3,
-                      double t0, double t1, double t2, double t3) {
+__device__ __forceinline__ int lookup(const Search& s, unsigned c0, unsigned c1, unsigned c2,
+                                      unsigned c3, double t0, double t1, double t2, double t3) {
   const int lane = threadIdx.x;
-  int row = 0, prev = 0;
-  for (int lev = kMaxDepth - 1; lev > 0; --lev) {
-    const int id = node_id(ASSOC, c0, c1, c2, c3, lev);
-    const int ch = rfl((int)s.child[row * s.kids + id]);
-    if (ch == kNoChild) break;
-    prev = row;
-    if (ch == kLeaf) break;
-    row = ch;
+  uint32_t e = (c0 >> 5) | ((c1 >> 5) << 3) | ((c2 >> 5) << 6);
+  if (ASSOC) e |= (c3 >> 5) << 9;
+  const uint32_t d3 = (uint32_t)rfl((int)s.desc[e]);
+  int prev = (int)((d3 >> 16) & 0x7FFFu);
+  if (!(d3 & kDescStop)) {
+    int row = (int)(d3 & 0x7FFFu);
+    for (int lev = 4; lev > 0; --lev) {
+      const int ch = rfl((int)s.child[row * s.kids + node_id(ASSOC, c0, c1, c2, c3, lev)]);
+      if (ch == kNoChild) break;
+      prev = row;
+      if (ch == kLeaf) break;
+      row = ch;
+    }
   }
   const int2 r = s.sub[prev];
   const int off = rfl(r.x), len = rfl(r.y);
@@ -91,17 +109,18 @@ __device__ __attribute__((noinline)) int lookup(const Search& s, unsigned c0, un
   int bidx = 0;
   for (int b = 0; b < len; b += kWave) {
     const int k = b + lane;
+    const bool in = k < len;
     double d = __builtin_inf();
     int ci = 0;
-    if (k < len) {
+    if (in) {
+      const float4 e4 = s.cent[off + k];
       ci = (int)s.cand[off + k];
-      const float4 e4 = s.ent[ci];
-      const float e[4] = {e4.x, e4.y, e4.z, e4.w};
-      d = candidate_distance(ASSOC, e, t);
+      const float ev[4] = {e4.x, e4.y, e4.z, e4.w};
+      d = candidate_distance(ASSOC, ev, t);
     }
-    const double m = wave_min(d);
+    const double m = wave_min(d, len - b <= 16);
     if (m <= best) {
-      const unsigned long long mask = __ballot(k < len && d == m);
+      const unsigned long long mask = __ballot(in && d == m);
       const int last = 63 - __builtin_clzll(mask);
       best = m;
       bidx = rl(ci, last);
@@ -110,143 +129,156 @@ __device__ __attribute__((noinline)) int lookup(const Search& s, unsigned c0, un
   return bidx;
 }
 
+// LDS cache of the HBM memo table's 64-entry lines.
 struct Memo {
-  uint32_t* tab;  // LDS, 1<<S entries
-  int32_t* spill; // global, 1<<K entries
-  int S;
-  uint32_t M;
-  int K;
+  uint32_t* tag;    // per slot: bit 31 empty, bit 30 dirty, bits 0..29 line id
+  uint16_t* line;   // 64 entries per slot: index+1, 0 = untouched
+  uint16_t* table;  // HBM, 1 << K entries, zeroed before the walk
+  int bits;         // log2(slots)
 };
 
-__device__ __forceinline__ int memo_slot_key(const Memo& m, uint32_t slot, uint32_t tag) {
-  return (int)((tag << m.S) | (slot ^ (tag & m.M)));
+constexpr uint32_t kEmptyTag = 0x80000000u;
+constexpr uint32_t kDirty = 0x40000000u;
+
+__device__ __forceinline__ uint32_t memo_slot(const Memo& m, uint32_t line) {
+  return (line * 0x9E3779B1u) >> (32 - m.bits);
 }
 
-// Memo miss for one visit (uniform): the key is not in its LDS slot.  `ent`
-// is the slot's current content.
-template <bool ASSOC>
-__device__ __attribute__((noinline)) int memo_miss(const Memo& m, const Search& s, uint32_t key,
-                                                   uint32_t ent, unsigned c0, unsigned c1,
-                                                   unsigned c2, unsigned c3, double t0, double t1,
-                                                   double t2, double t3) {
-  const uint32_t tag = key >> m.S;
-  const uint32_t slot = (key ^ tag) & m.M;
-  const bool valid = (ent >> 31) != 0;
-  const bool spilled = ((ent >> 30) & 1u) != 0;
+// Makes `line` resident in `slot` (uniform).  Returns 1 when it read HBM.
+__device__ __forceinline__ uint32_t memo_fill(const Memo& m, uint32_t slot, uint32_t line,
+                                              uint32_t old) {
   const int lane = threadIdx.x;
-  int found = -1;
-  if (spilled) found = rfl(m.spill[key]);
-  const int idx = found >= 0 ? found : lookup<ASSOC>(s, c0, c1, c2, c3, t0, t1, t2, t3);
-  bool now_spilled = spilled;
-  if (valid) {
-    // Evict the occupant to the global table.  On this slot's first spill,
-    // first mark every key that maps here as untouched.
-    if (!spilled) {
-      const int group = 1 << (m.K - m.S);
-      for (int t = lane; t < group; t += kWave) m.spill[memo_slot_key(m, slot, (uint32_t)t)] = -1;
-      __threadfence_block();
-    }
-    if (lane == 0) m.spill[memo_slot_key(m, slot, (ent >> 16) & 0x3FFFu)] = (int)(ent & 0xFFFFu);
-    __threadfence_block();
-    now_spilled = true;
+  uint16_t* dst = m.line + 64 * slot;
+  uint32_t loaded = 0;
+  if (old & kEmptyTag) {
+    // Never held a line: this line was never written, so it is all zero.
+    dst[lane] = 0;
+  } else {
+    if (old & kDirty) m.table[64 * (old & 0x3FFFFFFFu) + lane] = dst[lane];
+    dst[lane] = m.table[64 * line + lane];
+    loaded = 1;
   }
-  if (lane == 0)
-    m.tab[slot] = 0x80000000u | (now_spilled ? 0x40000000u : 0u) | (tag << 16) | (uint32_t)idx;
-  return idx;
+  if (lane == 0) m.tag[slot] = line;
+  __builtin_amdgcn_wave_barrier();
+  return loaded;
+}
+
+struct Counters {
+  uint32_t misses = 0, fills = 0, loads = 0;
+};
+
+// Global-address-space views: loads and stores through them are
+// global_load/global_store (vmcnt only), so they never hold up an LDS wait the
+// way a flat access would.
+typedef const double __attribute__((address_space(1))) gdouble;
+typedef uint16_t __attribute__((address_space(1))) gu16;
+
+// Inputs of visit v (clamped into the image, so the load never branches and
+// the wave does not wait for it until the value is used).
+__device__ __forceinline__ double load_x(const double* xs, int64_t v, int64_t n, int ch) {
+  return ((gdouble*)xs)[4 * (v < n ? v : n - 1) + ch];
 }
 
 struct WalkState {
-  double acc;   // running sum of this lane's in-flight visit
-  double e;     // residual of the visit this lane completed last
-  double xn1, xn2, xn3;  // restart values for the next three chunks
-  uint32_t obuf;         // palette indices of the current 64-visit window
+  double acc;     // running sum of this lane's in-flight visit
+  double e;       // residual of the visit completed last (lane 0 of each row)
+  uint32_t obuf;  // palette indices of the current 64-visit window
 };
 
-__device__ __forceinline__ double load_x(const double* xs, int64_t v, int64_t n, int ch) {
-  return v < n ? xs[4 * v + ch] : 0.0;
-}
-
-// One visit at phase I (visit T = base + I).  Returns false when T >= n.
-template <int I, bool ASSOC, bool CACHE>
-__device__ __forceinline__ void visit(WalkState& w, const double (&W)[16], int64_t base,
-                                      const Memo& m, const Search& s) {
+// One visit (visit t); on return lane j holds visit t+1+j.  `xq` is the
+// current chunk's input register, rotating in step with the accumulators.
+template <bool ASSOC, bool CACHE>
+__device__ __forceinline__ void visit(WalkState& w, double& xq, double W, int64_t t, int64_t n,
+                                      uint16_t* idx_out, const Memo& m, const Search& s,
+                                      Counters& cnt) {
   const int lane = threadIdx.x;
   const int j = lane & 15;
   const int ch = lane >> 4;
-  // e_{T-1} lives in lane (I-1)&15 of every row.
-  const double eb = row_bcast<(I + 15) & 15>(w.e);
-  const double prod = W[I] * eb;
-  w.acc = w.acc + prod;
+  // Every in-flight visit adds its tap of e_{t-1} (held by lane 0).
+  w.acc = w.acc + W * dpp<kRowBcast0>(w.e, 0.0);
   const float pc = clamp_pixel(w.acc);
   const double pd = (double)pc;
   const unsigned c8 = scale_char(pc);
-  const unsigned c0 = (unsigned)rl((int)c8, I);
-  const unsigned c1 = (unsigned)rl((int)c8, 16 + I);
-  const unsigned c2 = (unsigned)rl((int)c8, 32 + I);
-  const unsigned c3 = ASSOC ? (unsigned)rl((int)c8, 48 + I) : 0u;
+  const unsigned c0 = (unsigned)rl((int)c8, 0);
+  const unsigned c1 = (unsigned)rl((int)c8, 16);
+  const unsigned c2 = (unsigned)rl((int)c8, 32);
+  const unsigned c3 = ASSOC ? (unsigned)rl((int)c8, 48) : 0u;
   int idx;
   if (CACHE) {
     const uint32_t key = (uint32_t)memo_key(ASSOC, c0, c1, c2, c3);
-    const uint32_t tag = key >> m.S;
-    const uint32_t slot = (key ^ tag) & m.M;
-    const uint32_t ent = (uint32_t)rfl((int)m.tab[slot]);
-    if ((ent >> 31) != 0 && ((ent >> 16) & 0x3FFFu) == tag) {
-      idx = (int)(ent & 0xFFFFu);
+    const uint32_t line = key >> 6;
+    const uint32_t slot = memo_slot(m, line);
+    const uint32_t tg = (uint32_t)rfl((int)m.tag[slot]);
+    uint32_t ent = (uint32_t)rfl((int)m.line[64 * slot + (key & 63)]);
+    if ((tg & ~kDirty) != line) {
+      ++cnt.fills;
+      cnt.loads += memo_fill(m, slot, line, tg);
+      ent = (uint32_t)rfl((int)m.line[64 * slot + (key & 63)]);
+    }
+    if (ent != 0) {
+      idx = (int)ent - 1;
     } else {
-      idx = memo_miss<ASSOC>(m, s, key, ent, c0, c1, c2, c3, rl(pd, I), rl(pd, 16 + I),
-                             rl(pd, 32 + I), ASSOC ? rl(pd, 48 + I) : 65535.0);
+      ++cnt.misses;
+      idx = lookup<ASSOC>(s, c0, c1, c2, c3, rl(pd, 0), rl(pd, 16), rl(pd, 32),
+                          ASSOC ? rl(pd, 48) : 65535.0);
+      if (lane == 0) {
+        m.line[64 * slot + (key & 63)] = (uint16_t)(idx + 1);
+        m.tag[slot] = line | kDirty;
+      }
     }
   } else {
-    idx = lookup<ASSOC>(s, c0, c1, c2, c3, rl(pd, I), rl(pd, 16 + I), rl(pd, 32 + I),
-                        ASSOC ? rl(pd, 48 + I) : 65535.0);
+    ++cnt.misses;
+    idx = lookup<ASSOC>(s, c0, c1, c2, c3, rl(pd, 0), rl(pd, 16), rl(pd, 32),
+                        ASSOC ? rl(pd, 48) : 65535.0);
   }
   // Residual against the associated palette colour, per row's channel.
   w.e = pd - s.col[4 * idx + ch];
-  w.obuf = lane == (int)((base + I) & 63) ? (uint32_t)idx : w.obuf;
-  w.acc = (j == I) ? w.xn1 : w.acc;
+  w.obuf = lane == (int)(t & 63) ? (uint32_t)idx : w.obuf;
+  if ((t & 63) == 63 || t + 1 == n) {
+    const int64_t v = (t & ~(int64_t)63) + lane;
+    if (v <= t) ((gu16*)idx_out)[v] = (uint16_t)w.obuf;
+  }
+  // Rotate: lane j takes visit t+1+j; lane 15 restarts with x_{t+16}.  (+0.0
+  // is that visit's zero-error taps: it turns -0.0 into +0.0 as the
+  // reference's first adds do.)
+  w.acc = dpp<kRowRor15>(w.acc, 0.0);
+  xq = dpp<kRowRor15>(xq, 0.0);
+  w.acc = j == 15 ? xq + 0.0 : w.acc;
 }
 
-template <bool ASSOC, bool CACHE, int... Is>
-__device__ __forceinline__ bool chunk(WalkState& w, const double (&W)[16], int64_t base,
-                                      int64_t n, const Memo& m, const Search& s,
-                                      std::integer_sequence<int, Is...>) {
-  if (base + 16 <= n) {
-    (visit<Is, ASSOC, CACHE>(w, W, base, m, s), ...);
-    return true;
-  }
-  // Tail chunk: stop at n.
-  bool go = true;
-  ((go = go && (base + Is < n), go ? (visit<Is, ASSOC, CACHE>(w, W, base, m, s), 0) : 0), ...);
-  return false;
+// Chunk c (visits 16c .. 16c+15) restarts from its register `xq`, loaded two
+// chunks earlier; afterwards `xq` is refilled for chunk c+3.
+template <bool ASSOC, bool CACHE>
+__device__ __forceinline__ bool chunk(WalkState& w, double& xq, double W, int64_t base, int64_t n,
+                                      const SeqItem& it, const Memo& m, const Search& s,
+                                      Counters& cnt) {
+  const int lane = threadIdx.x;
+  const int64_t end = base + 16 < n ? base + 16 : n;
+  for (int64_t t = base; t < end; ++t) visit<ASSOC, CACHE>(w, xq, W, t, n, it.idx, m, s, cnt);
+  xq = load_x(it.xs, base + 64 + (lane & 15), n, lane >> 4);
+  return end < n;
 }
 
 template <bool ASSOC, bool CACHE>
-__device__ void walk_image(const SeqItem& it, const double (&W)[16], const Memo& m,
-                           const Search& s) {
+__device__ void walk_image(const SeqItem& it, double W, const Memo& m, const Search& s,
+                           Counters& cnt) {
   const int lane = threadIdx.x;
   const int j = lane & 15;
   const int ch = lane >> 4;
   const int64_t n = it.n;
-  const double* xs = it.xs;
   WalkState w;
   w.e = 0.0;
-  w.acc = load_x(xs, j, n, ch) + 0.0;
-  w.xn1 = load_x(xs, 16 + j, n, ch) + 0.0;
-  w.xn2 = load_x(xs, 32 + j, n, ch) + 0.0;
-  w.xn3 = load_x(xs, 48 + j, n, ch) + 0.0;
   w.obuf = 0;
-  for (int64_t base = 0; base < n; base += 16) {
-    const bool full = chunk<ASSOC, CACHE>(w, W, base, n, m, s,
-                                          std::make_integer_sequence<int, 16>{});
-    const int64_t next = base + 16;
-    if ((next & 63) == 0 || !full || next >= n) {
-      const int64_t wbase = (next - 1) & ~(int64_t)63;
-      const int64_t v = wbase + lane;
-      if (v < n && v < next) it.idx[v] = (uint16_t)w.obuf;
-    }
-    w.xn1 = w.xn2;
-    w.xn2 = w.xn3;
-    w.xn3 = load_x(xs, next + 48 + j, n, ch) + 0.0;
+  w.acc = load_x(it.xs, j, n, ch) + 0.0;
+  double x0 = load_x(it.xs, 16 + j, n, ch);
+  double x1 = load_x(it.xs, 32 + j, n, ch);
+  double x2 = load_x(it.xs, 48 + j, n, ch);
+  // Three chunk registers used round-robin: no register copy ever waits on
+  // an outstanding load.
+  for (int64_t base = 0;; base += 48) {
+    if (!chunk<ASSOC, CACHE>(w, x0, W, base, n, it, m, s, cnt)) break;
+    if (!chunk<ASSOC, CACHE>(w, x1, W, base + 16, n, it, m, s, cnt)) break;
+    if (!chunk<ASSOC, CACHE>(w, x2, W, base + 32, n, it, m, s, cnt)) break;
   }
 }
 
@@ -254,47 +286,60 @@ template <bool ASSOC, bool CACHE>
 __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restrict__ items,
                                                          int nitems, int32_t* queue,
                                                          const double* __restrict__ sw,
-                                                         int memo_bits) {
+                                                         int memo_bits, uint32_t* stats) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int lane = threadIdx.x;
-  const int j = lane & 15;
-  // Per-lane weight of each phase: lane j at phase I adds tap 15-((j-I)&15).
-  double W[16];
-#pragma unroll
-  for (int I = 0; I < 16; ++I) W[I] = sw[15 - ((j - I) & 15)];
+  // Lane j adds tap 15-j of the newest residual.
+  const double W = sw[15 - (lane & 15)];
   Memo m;
-  m.S = memo_bits;
-  m.M = (1u << memo_bits) - 1u;
-  m.K = ASSOC ? 24 : 18;
-  m.tab = (uint32_t*)lds;
-  unsigned char* img = lds + (CACHE ? (4u << memo_bits) : 0u);
+  m.bits = memo_bits;
+  const uint32_t slots = 1u << memo_bits;
+  m.tag = (uint32_t*)lds;
+  m.line = (uint16_t*)(lds + 4 * slots);
+  unsigned char* img = lds + (CACHE ? 132u * slots : 0u);
+  Counters cnt;
   for (;;) {
     int q = 0;
     if (lane == 0) q = atomicAdd(queue, 1);
-    q = __shfl(q, 0, kWave);
+    q = rfl(__shfl(q, 0, kWave));  // uniform: item fields load into SGPRs
     if (q >= nitems) break;
     const SeqItem it = items[q];
     const PaletteDev pd = *it.pal;
-    // Stage the search image and clear the memo.
     {
       const uint4* src = (const uint4*)pd.image;
       uint4* dst = (uint4*)img;
       for (int k = lane; k < pd.bytes / 16; k += kWave) dst[k] = src[k];
       if (CACHE)
-        for (uint32_t k = lane; k < (1u << memo_bits); k += kWave) m.tab[k] = 0u;
+        for (uint32_t k = lane; k < slots; k += kWave) m.tag[k] = kEmptyTag;
       __syncthreads();
     }
     Search s;
     s.col = (const double*)img;
-    s.ent = (const float4*)(img + pd.off_ent);
+    s.cent = (const float4*)(img + pd.off_cent);
     s.sub = (const int2*)(img + pd.off_sub);
+    s.desc = (const uint32_t*)(img + pd.off_desc);
     s.child = (const int16_t*)(img + pd.off_child);
     s.cand = (const uint16_t*)(img + pd.off_cand);
     s.kids = pd.kids;
-    m.spill = it.spill;
-    walk_image<ASSOC, CACHE>(it, W, m, s);
+    m.table = it.memo;
+    walk_image<ASSOC, CACHE>(it, W, m, s, cnt);
     __syncthreads();
   }
+  if (stats && lane == 0) {
+    atomicAdd(&stats[0], cnt.misses);
+    atomicAdd(&stats[1], cnt.fills);
+    atomicAdd(&stats[2], cnt.loads);
+  }
+}
+
+// Zeroes each image's HBM memo table (a fill kernel, so the sequence can be
+// captured in a graph).
+__global__ __launch_bounds__(256) void memo_clear_kernel(const SeqItem* __restrict__ items,
+                                                         int64_t words) {
+  const SeqItem it = items[blockIdx.y];
+  uint4* t = (uint4*)it.memo;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < words; k += (int64_t)gridDim.x * 256)
+    t[k] = make_uint4(0, 0, 0, 0);
 }
 
 // Gather: visit order, associated, +0.0 (the zero taps a visit near the start
@@ -387,15 +432,21 @@ hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipS
 
 hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, const double* sw,
                            int memo_bits, int use_cache, int lds_image_bytes, int waves,
-                           int assoc_any, hipStream_t s) {
+                           int assoc_any, uint32_t* stats, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  const size_t lds = (use_cache ? (4u << memo_bits) : 0u) + (size_t)lds_image_bytes;
+  if (use_cache) {
+    const int64_t words = ((int64_t)2 << (assoc_any ? 24 : 18)) / 16;
+    memo_clear_kernel<<<dim3(64, (unsigned)nitems), 256, 0, s>>>(items, words);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const size_t lds = (use_cache ? (132u << memo_bits) : 0u) + (size_t)lds_image_bytes;
   const dim3 grid((unsigned)waves), block(kWave);
   auto go = [&](auto kernel) -> hipError_t {
     hipError_t e = hipFuncSetAttribute((const void*)kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    kernel<<<grid, block, lds, s>>>(items, nitems, queue, sw, memo_bits);
+    kernel<<<grid, block, lds, s>>>(items, nitems, queue, sw, memo_bits, stats);
     return hipGetLastError();
   };
   if (assoc_any) {
