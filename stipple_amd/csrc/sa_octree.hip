@@ -2,9 +2,9 @@
 //
 // Restates src/rd_octree.cpp:219-409 (quantize.c's Classify / PruneToCubeDepth /
 // Reduce / DefineImageColormap).  The O(pixels) classification is data-parallel
-// integer work; the O(nodes) tree logic is a short sequential program on one
-// device thread.  Every double the reference accumulates is accumulated here
-// in the reference's order.
+// integer work; the O(nodes) tree walks run as level-synchronous sweeps of one
+// workgroup.  Every double the reference accumulates is accumulated here in the
+// reference's order.
 //
 //   1. Runs.  quantize.c collapses runs of identical adjacent pixels per row and
 //      weights them by their length (rd_octree.cpp:305-313).  A flag per pixel,
@@ -20,13 +20,13 @@
 //      node numbering.
 //   4. Ordered sums.  quantize_error, total_color and number_unique of a node
 //      are sums over its pairs in run order.  A stable radix sort of the pairs
-//      by node gives each node its pairs in that order; one thread per node then
-//      folds them sequentially, so every double matches the reference's.  The
-//      totals of loop 2 are folded after PruneToCubeDepth has merged loop 1's
-//      deep leaves, exactly when the reference adds them.
-//   5. Tree logic on one thread: the gated PruneToCubeDepth, the reduction
-//      threshold (a restatement of glibc 2.35 qsort's merge sort, because the
-//      reference's comparator is not a strict weak order), the Reduce passes,
+//      by node puts each node's pairs in that order; every pair's term is
+//      computed in parallel, then each node's chain is folded in order over
+//      contiguous memory.  The totals of loop 2 are folded after
+//      PruneToCubeDepth has merged loop 1's deep leaves, exactly when the
+//      reference adds them.
+//   5. Tree walks (section 5 explains the restatement): the gated
+//      PruneToCubeDepth, the reduction threshold, the Reduce passes,
 //      DefineImageColormap and the search image (sa_palette.h).
 //
 // The root's quantize_error.  The reference adds every node's running total
@@ -325,364 +325,736 @@ __global__ void k_links(const uint64_t* nkey, int64_t M, const uint64_t* hkeys,
 }
 
 // ---- 4. ordered sums --------------------------------------------------------------
+// Each node's quantize_error is a serial chain over its pairs in run order, and
+// the leaves' total_color four more chains.  The chains cannot be reassociated,
+// so the parallel work is everything around them: one thread per sorted pair
+// computes its term (distance x count) and the leaf values, written contiguously
+// in node order; then each chain is a plain in-order fold over contiguous
+// memory -- one thread per node for short segments, one wavefront per node for
+// long ones, staging chunks through LDS while the next chunks load.
 struct Runs {
   const int32_t* count;
   const double4* px;
 };
 
-__device__ __forceinline__ double pair_term(const Runs& R, int64_t r, const double mid[4],
-                                            bool assoc) {
+struct Terms {
+  double* term;   // count * |pixel - mid| per sorted pair
+  float4* lval;   // clamped leaf channels (leaf pairs only)
+  int32_t* lcnt;  // run length per sorted pair
+};
+
+__device__ __forceinline__ bool leaf_pair(uint32_t p, int64_t R1, int depth, int l) {
+  return p < 8ull * (uint64_t)R1 ? l == 8 : l == depth;
+}
+
+__global__ void k_pair_terms(const uint64_t* nkey, Runs R, const uint32_t* spair,
+                             const int32_t* snode, int64_t P, int64_t R1, int depth, int assoc,
+                             Terms T) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= P) return;
+  const uint32_t p = spair[i];
+  int64_t r;
+  int l;
+  pair_decode(p, R1, depth, &r, &l);
+  double mid[4];
+  node_mid(nkey[snode[i]], mid);
   const double4 v = R.px[r];
+  const int32_t cnt = R.count[r];
   const double e0 = kScale * (v.x - mid[0]);
   const double e1 = kScale * (v.y - mid[1]);
   const double e2 = kScale * (v.z - mid[2]);
   const double e3 = assoc ? kScale * (v.w - mid[3]) : 0.0;
   double d = e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
   if (d != d) d = 0.0;
-  return (double)R.count[r] * sqrt(d);
+  T.term[i] = (double)cnt * sqrt(d);
+  T.lcnt[i] = cnt;
+  if (leaf_pair(p, R1, depth, l))
+    T.lval[i] = make_float4(clamp_pixel(v.x), clamp_pixel(v.y), clamp_pixel(v.z),
+                            clamp_pixel(assoc ? v.w : kRange));
 }
 
-// Loop 1 and loop 2 quantize_error (one chain per node), the approximate sum of
-// its running values (the node's share of the root's chain), and the totals of
-// loop 1's leaves.  When `record` is set, each pair's running value is stored
-// for the exact root chain.
-__global__ void k_node_sums(TreeDev t, int64_t M, Runs R, const uint32_t* spair,
-                            const int32_t* seg_begin, const int32_t* seg_end, int64_t R1, int depth,
-                            int assoc, double* record) {
-  // With `record` set this only stores the running values (the node arrays
-  // already hold their final, post-prune contents).
+// First sorted position in [b, e) whose pair belongs to loop 2 (pairs of a
+// node are in ascending pair order, loop 1's first).
+__device__ __forceinline__ int32_t loop2_start(const uint32_t* spair, int32_t b, int32_t e,
+                                               uint32_t l1) {
+  int32_t lo = b, hi = e;
+  while (lo < hi) {
+    const int32_t m = lo + ((hi - lo) >> 1);
+    if (spair[m] < l1)
+      lo = m + 1;
+    else
+      hi = m;
+  }
+  return lo;
+}
+
+// One leaf term of total_color, as the reference forms it: count * scale * value.
+__device__ __forceinline__ double tot_term(int32_t cnt, float v) {
+  return (double)cnt * kScale * (double)v;
+}
+
+constexpr int kShortSeg = 256;
+
+// L2 = false: quantize_error (+ the node's share of the root chain) over all
+// pairs, and loop 1's leaf totals.  L2 = true: loop 2's leaf totals, folded onto
+// what PruneToCubeDepth left.  Segments longer than kShortSeg are queued for
+// k_fold_long.
+template <bool L2>
+__global__ void k_fold_short(TreeDev t, int64_t M, Terms T, const uint32_t* spair,
+                             const int32_t* seg_b, const int32_t* seg_e, int64_t R1, int depth,
+                             int32_t* long_list, int32_t* long_count) {
   const int64_t n = (int64_t)blockIdx.x * 64 + threadIdx.x + 1;
   if (n > M) return;
-  double mid[4];
-  node_mid(t.key[n], mid);
+  const int32_t b = seg_b[n], e = seg_e[n];
   const int level = t.level[n];
-  double qe = 0.0, share = 0.0;
-  double tot[4] = {0.0, 0.0, 0.0, 0.0};
-  uint64_t nu = 0;
-  for (int32_t i = seg_begin[n]; i < seg_end[n]; ++i) {
-    const uint32_t p = spair[i];
-    int64_t r;
-    int l;
-    pair_decode(p, R1, depth, &r, &l);
-    qe += pair_term(R, r, mid, assoc != 0);
-    share += qe;
-    if (record) record[p] = qe;
-    if (r < R1 && level == 8) {
-      const double4 v = R.px[r];
-      const double cnt = (double)R.count[r];
-      tot[0] += cnt * kScale * (double)clamp_pixel(v.x);
-      tot[1] += cnt * kScale * (double)clamp_pixel(v.y);
-      tot[2] += cnt * kScale * (double)clamp_pixel(v.z);
-      tot[3] += cnt * kScale * (double)clamp_pixel(assoc ? v.w : kRange);
-      nu += (uint64_t)R.count[r];
-    }
+  if (L2 && level != depth) return;
+  if (e - b > kShortSeg) {
+    long_list[atomicAdd(long_count, 1)] = (int32_t)n;
+    return;
   }
-  if (record) return;
-  t.qe[n] = qe;
-  t.share[n] = share;
-  for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
-  t.nu[n] = nu;
+  const uint32_t l1 = (uint32_t)(8 * R1);
+  const int32_t s = loop2_start(spair, b, e, l1);
+  if (!L2) {
+    double qe = 0.0, share = 0.0;
+    for (int32_t i = b; i < e; ++i) {
+      qe += T.term[i];
+      share += qe;
+    }
+    t.qe[n] = qe;
+    t.share[n] = share;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    uint64_t nu = 0;
+    if (level == 8) {
+      for (int32_t i = b; i < s; ++i) {
+        const float4 v = T.lval[i];
+        const int32_t c = T.lcnt[i];
+        tot[0] += tot_term(c, v.x);
+        tot[1] += tot_term(c, v.y);
+        tot[2] += tot_term(c, v.z);
+        tot[3] += tot_term(c, v.w);
+        nu += (uint64_t)c;
+      }
+    }
+    for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
+    t.nu[n] = nu;
+  } else {
+    if (s == e) return;
+    double tot[4];
+    for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * n + c];
+    uint64_t nu = t.nu[n];
+    for (int32_t i = s; i < e; ++i) {
+      const float4 v = T.lval[i];
+      const int32_t c = T.lcnt[i];
+      tot[0] += tot_term(c, v.x);
+      tot[1] += tot_term(c, v.y);
+      tot[2] += tot_term(c, v.z);
+      tot[3] += tot_term(c, v.w);
+      nu += (uint64_t)c;
+    }
+    for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
+    t.nu[n] = nu;
+  }
 }
 
-// Loop 2's leaf totals, folded onto whatever PruneToCubeDepth left.
-__global__ void k_node_sums_l2(TreeDev t, int64_t M, Runs R, const uint32_t* spair,
-                               const int32_t* seg_begin, const int32_t* seg_end, int64_t R1,
-                               int depth, int assoc) {
+constexpr int kChunk = 512;           // terms staged per step
+constexpr int kPerLane = kChunk / 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// quantize_error over [b, e): lane 0 folds each staged chunk in order while the
+// wave loads the chunk after next.  The share (sum of the running values) is
+// only needed approximately: sum_i term_i * (e - i), reduced in any order.
+__device__ void fold_qe_wave(const double* term, int32_t b, int32_t e, double* s_q, double* qe_out,
+                             double* share_out) {
+  const int lane = threadIdx.x;
+  const int32_t len = e - b;
+  const int32_t nch = (len + kChunk - 1) / kChunk;
+  double ra[kPerLane], rb[kPerLane];
+  auto load = [&](double* r, int32_t c) {
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const int32_t i = b + c * kChunk + k * 64 + lane;
+      r[k] = (c < nch && i < e) ? term[i] : 0.0;
+    }
+  };
+  load(ra, 0);
+  load(rb, 1);
+  double qe = 0.0, sh = 0.0;
+  auto step = [&](double* r, int32_t c) {
+    double* buf = s_q + (c & 1) * kChunk;
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const int32_t i = b + c * kChunk + k * 64 + lane;
+      buf[k * 64 + lane] = r[k];
+      sh += r[k] * (double)(e - i);
+    }
+    load(r, c + 2);
+    __syncthreads();
+    if (lane == 0) {
+      const int32_t cnt = min(kChunk, len - c * kChunk);
+      if (cnt == kChunk) {
+#pragma unroll 32
+        for (int j = 0; j < kChunk; ++j) qe += buf[j];
+      } else {
+        for (int j = 0; j < cnt; ++j) qe += buf[j];
+      }
+    }
+  };
+  for (int32_t c = 0; c < nch; c += 2) {
+    step(ra, c);
+    if (c + 1 < nch) step(rb, c + 1);
+  }
+  sh = wave_sum(sh);
+  qe = __shfl(qe, 0, 64);
+  *qe_out = qe;
+  *share_out = sh;
+}
+
+// total_color over [b, e) onto tot[] (lanes 0..3 fold one channel each);
+// number_unique is an integer sum, reduced across the wave.
+__device__ void fold_tot_wave(const Terms& T, int32_t b, int32_t e, double* s_p, double tot[4],
+                              uint64_t* nu) {
+  const int lane = threadIdx.x;
+  const int32_t len = e - b;
+  const int32_t nch = (len + kChunk - 1) / kChunk;
+  float4 va[kPerLane], vb[kPerLane];
+  int32_t ca[kPerLane], cb[kPerLane];
+  auto load = [&](float4* v, int32_t* cn, int32_t c) {
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const int32_t i = b + c * kChunk + k * 64 + lane;
+      const bool ok = c < nch && i < e;
+      v[k] = ok ? T.lval[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      cn[k] = ok ? T.lcnt[i] : 0;
+    }
+  };
+  load(va, ca, 0);
+  load(vb, cb, 1);
+  const int ch = lane & 3;
+  double acc = tot[ch];
+  uint64_t nsum = 0;
+  auto step = [&](float4* v, int32_t* cn, int32_t c) {
+    double* buf = s_p + (c & 1) * 4 * kChunk;
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const int j = k * 64 + lane;
+      buf[0 * kChunk + j] = tot_term(cn[k], v[k].x);
+      buf[1 * kChunk + j] = tot_term(cn[k], v[k].y);
+      buf[2 * kChunk + j] = tot_term(cn[k], v[k].z);
+      buf[3 * kChunk + j] = tot_term(cn[k], v[k].w);
+      nsum += (uint64_t)cn[k];
+    }
+    load(v, cn, c + 2);
+    __syncthreads();
+    if (lane < 4) {
+      const double* src = buf + ch * kChunk;
+      const int32_t cnt = min(kChunk, len - c * kChunk);
+      if (cnt == kChunk) {
+#pragma unroll 32
+        for (int j = 0; j < kChunk; ++j) acc += src[j];
+      } else {
+        for (int j = 0; j < cnt; ++j) acc += src[j];
+      }
+    }
+  };
+  for (int32_t c = 0; c < nch; c += 2) {
+    step(va, ca, c);
+    if (c + 1 < nch) step(vb, cb, c + 1);
+  }
+  nsum = wave_sum_u64(nsum);
+  for (int c = 0; c < 4; ++c) tot[c] = __shfl(acc, c, 64);
+  *nu += nsum;
+}
+
+template <bool L2>
+__global__ __launch_bounds__(64) void k_fold_long(TreeDev t, Terms T, const uint32_t* spair,
+                                                  const int32_t* seg_b, const int32_t* seg_e,
+                                                  int64_t R1, int depth, const int32_t* long_list,
+                                                  const int32_t* long_count) {
+  __shared__ double s_buf[2 * 4 * kChunk];
+  const int32_t items = *long_count;
+  const uint32_t l1 = (uint32_t)(8 * R1);
+  for (int32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int32_t n = long_list[it];
+    const int32_t b = seg_b[n], e = seg_e[n];
+    const int level = t.level[n];
+    const int32_t s = loop2_start(spair, b, e, l1);
+    if (!L2) {
+      double qe, share;
+      fold_qe_wave(T.term, b, e, s_buf, &qe, &share);
+      double tot[4] = {0.0, 0.0, 0.0, 0.0};
+      uint64_t nu = 0;
+      __syncthreads();
+      if (level == 8 && s > b) fold_tot_wave(T, b, s, s_buf, tot, &nu);
+      if (threadIdx.x == 0) {
+        t.qe[n] = qe;
+        t.share[n] = share;
+        for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
+        t.nu[n] = nu;
+      }
+    } else if (s < e) {
+      double tot[4];
+      for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * n + c];
+      uint64_t nu = t.nu[n];
+      fold_tot_wave(T, s, e, s_buf, tot, &nu);
+      if (threadIdx.x == 0) {
+        for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
+        t.nu[n] = nu;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// The running values of every pair, stored by pair index for the exact root
+// chain (only run when the bound cannot separate the root's error).
+__global__ void k_fold_record(int64_t M, const double* term, const uint32_t* spair,
+                              const int32_t* seg_b, const int32_t* seg_e, double* record) {
   const int64_t n = (int64_t)blockIdx.x * 64 + threadIdx.x + 1;
   if (n > M) return;
-  if (t.level[n] != depth) return;
-  double tot[4];
-  for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * n + c];
-  uint64_t nu = t.nu[n];
-  bool any = false;
-  for (int32_t i = seg_begin[n]; i < seg_end[n]; ++i) {
-    int64_t r;
-    int l;
-    pair_decode(spair[i], R1, depth, &r, &l);
-    if (r < R1) continue;
-    any = true;
-    const double4 v = R.px[r];
-    const double cnt = (double)R.count[r];
-    tot[0] += cnt * kScale * (double)clamp_pixel(v.x);
-    tot[1] += cnt * kScale * (double)clamp_pixel(v.y);
-    tot[2] += cnt * kScale * (double)clamp_pixel(v.z);
-    tot[3] += cnt * kScale * (double)clamp_pixel(assoc ? v.w : kRange);
-    nu += (uint64_t)R.count[r];
+  double qe = 0.0;
+  for (int32_t i = seg_b[n]; i < seg_e[n]; ++i) {
+    qe += term[i];
+    record[spair[i]] = qe;
   }
-  if (!any) return;
-  for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
-  t.nu[n] = nu;
 }
 
-// ---- 5. tree logic (one thread) -------------------------------------------------------
-struct Ctl {
-  TreeDev t;
-  int kids;
-  int64_t live;
-  int64_t maxc;
-  int depth;
-  int count;
-  double prune_thr;
-  double next_thr;
+// ---- 5. tree logic (one workgroup, level-synchronous) -------------------------------
+// PruneToCubeDepth, Reduce and DefineImageColormap are recursive walks whose
+// only order-dependent state is the live-node counter and the double sums a
+// merge adds into a parent.  Each walk is restated as a few level-by-level
+// sweeps over the nodes (the tree is at most 9 levels deep):
+//
+//   * A pass prunes with a predicate S (level > depth for PruneToCubeDepth,
+//     quantize_error <= threshold for Reduce).  PruneChild(a) is entered at the
+//     post-order visit of every S node a and attempts (rd_octree.cpp:69-93) each
+//     node still attached below a, children first.  So node x is first attempted
+//     in the "group" of a(x), its lowest S ancestor-or-self, and attempts are
+//     ordered by (post-order of a(x), post-order of x inside the group).  A
+//     merge happens while live > maxc, i.e. for the first live - maxc attempts;
+//     the rank of an attempt is a prefix sum of group sizes plus its position in
+//     its group.
+//   * Merged nodes form a downward-closed set, and a parent receives its merged
+//     children in attempt order: children that are S themselves (merged at their
+//     own visit) in index order, then the others (merged later, inside an
+//     ancestor's PruneChild) in index order.  One bottom-up sweep folds them in
+//     exactly that order, so every double matches the serial walk.
+//   * Reduce counts a node when it is not S and its number_unique, as it stands
+//     when the walk visits it (after its S children merged), is non-zero; the
+//     next threshold is the minimum quantize_error over those nodes.
+//   * QuantizeErrorFlatten + qsort (rd_octree.cpp:388-398) only use one order
+//     statistic.  glibc 2.35's qsort is a merge sort here whose merge takes the
+//     left element when cmp(left, right) <= 0; QuantizeErrorCompare returns 1
+//     exactly when left > right, so the merge is an ordinary stable ascending
+//     merge and the MagickEpsilon tie never changes the order.  The statistic is
+//     found by radix selection on the bit patterns (errors are >= 0).
+//   * DefineImageColormap numbers the colour-bearing nodes in post-order: a
+//     prefix sum over the post-order.  The candidate order of the search image is
+//     the same post-order, so candidate k is colour k.
+constexpr int kTreeThreads = 1024;
+constexpr int kTreeWaves = kTreeThreads / 64;
+
+// Per-node scratch of the tree sweeps (M + 1 entries each unless noted).
+struct TreeWork {
+  int32_t* lvl;      // node ids grouped by level
+  int32_t* lvl_off;  // 10 level offsets into lvl
+  int32_t* att;      // attached to the root at the start of the sweep
+  int32_t* anc;      // lowest S ancestor-or-self, -1 when none
+  int32_t* size;     // attached subtree size
+  int32_t* gsz;      // size of the node's part of its group
+  int32_t* start;    // post-order index of the first node of the subtree
+  int32_t* gstart;   // the same, counted inside the group
+  int32_t* pre;      // pre-order index
+  int32_t* merged;   // merged by this pass (later: internal-node flag)
+  int32_t* arr;      // scan buffers, indexed by post- or pre-order
+  int32_t* arr2;
 };
 
-// rd_octree.cpp:69-93 PruneChild: post-order over the current subtree, each
-// node merged into its parent while the live counter exceeds the palette size.
-__device__ void prune_child(Ctl& c, int32_t top, int32_t* stk_n, int8_t* stk_i) {
-  int sp = 0;
-  stk_n[0] = top;
-  stk_i[0] = 0;
-  while (sp >= 0) {
-    const int32_t n = stk_n[sp];
-    if (stk_i[sp] < c.kids) {
-      const int32_t ch = c.t.child[16 * n + stk_i[sp]];
-      ++stk_i[sp];
-      if (ch >= 0) {
-        ++sp;
-        stk_n[sp] = ch;
-        stk_i[sp] = 0;
-      }
-      continue;
-    }
-    if (c.live > c.maxc) {
-      const int32_t p = c.t.parent[n];
-      if (p >= 0) {
-        c.t.nu[p] += c.t.nu[n];
-        for (int k = 0; k < 4; ++k) c.t.total[4 * p + k] += c.t.total[4 * n + k];
-        c.t.child[16 * p + c.t.id[n]] = -1;
-      }
-      --c.live;
-    }
-    --sp;
-  }
-}
+struct TreeShared {
+  int32_t off[10];     // level l occupies lvl[off[l], off[l+1])
+  int32_t cur[10];
+  int64_t part_i[kTreeWaves];
+  double part_d[kTreeWaves];
+  int32_t part_s[kTreeWaves];
+  uint32_t hist[2048];
+  uint64_t sel[3];
+  int64_t scal[4];
+};
 
-// Post-order walk from the root calling `visit(node)` after the children; the
-// child links are re-read as the walk advances, as the recursion does.
+struct Tree {
+  TreeDev t;
+  TreeWork W;
+  int kids;
+  int64_t maxc;
+  int depth;
+  int64_t M;
+};
+
 template <typename Fn>
-__device__ void post_order(const Ctl& c, int32_t* stk_n, int8_t* stk_i, Fn&& visit) {
-  int sp = 0;
-  stk_n[0] = 0;
-  stk_i[0] = 0;
-  while (sp >= 0) {
-    const int32_t n = stk_n[sp];
-    if (stk_i[sp] < c.kids) {
-      const int32_t ch = c.t.child[16 * n + stk_i[sp]];
-      ++stk_i[sp];
-      if (ch >= 0) {
-        ++sp;
-        stk_n[sp] = ch;
-        stk_i[sp] = 0;
+__device__ __forceinline__ void for_level(const TreeShared& S, const TreeWork& W, int l, Fn&& fn) {
+  for (int32_t i = S.off[l] + (int32_t)threadIdx.x; i < S.off[l + 1]; i += kTreeThreads)
+    fn(W.lvl[i]);
+}
+
+__device__ int64_t block_sum(int64_t v, TreeShared& S) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) S.part_i[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int64_t s = 0;
+  for (int k = 0; k < kTreeWaves; ++k) s += S.part_i[k];
+  __syncthreads();
+  return s;
+}
+
+__device__ double block_min(double v, TreeShared& S) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) S.part_d[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = S.part_d[0];
+  for (int k = 1; k < kTreeWaves; ++k) m = fmin(m, S.part_d[k]);
+  __syncthreads();
+  return m;
+}
+
+// Exclusive prefix sum of a[0, n) in place; returns the total.
+__device__ int32_t block_scan(int32_t* a, int32_t n, TreeShared& S) {
+  const int32_t per = (n + kTreeThreads - 1) / kTreeThreads;
+  const int32_t lo = min(n, (int32_t)threadIdx.x * per), hi = min(n, lo + per);
+  int32_t local = 0;
+  for (int32_t i = lo; i < hi; ++i) local += a[i];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int32_t incl = local;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) S.part_s[wv] = incl;
+  __syncthreads();
+  int32_t before = 0, total = 0;
+  for (int k = 0; k < kTreeWaves; ++k) {
+    if (k < wv) before += S.part_s[k];
+    total += S.part_s[k];
+  }
+  int32_t run = before + incl - local;
+  for (int32_t i = lo; i < hi; ++i) {
+    const int32_t v = a[i];
+    a[i] = run;
+    run += v;
+  }
+  __syncthreads();
+  return total;
+}
+
+// Node ids grouped by level (the order inside a level is irrelevant).
+__device__ void level_lists(const Tree& T, TreeShared& S) {
+  if (threadIdx.x < 10) S.off[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t n = threadIdx.x; n <= T.M; n += kTreeThreads) atomicAdd(&S.off[T.t.level[n] + 1], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int l = 1; l < 10; ++l) S.off[l] += S.off[l - 1];
+    for (int l = 0; l < 10; ++l) S.cur[l] = S.off[l];
+  }
+  __syncthreads();
+  for (int64_t n = threadIdx.x; n <= T.M; n += kTreeThreads)
+    T.W.lvl[atomicAdd(&S.cur[T.t.level[n]], 1)] = (int32_t)n;
+  __syncthreads();
+}
+
+enum PassMode { kPruneCube = 0, kReduce = 1, kShape = 2 };
+
+// Attachment, the S ancestor, subtree and group sizes, post-order starts and
+// pre-order indices of the current tree.
+__device__ void tree_shape(const Tree& T, TreeShared& S, int mode, double thr) {
+  const TreeDev& t = T.t;
+  const TreeWork& W = T.W;
+  for (int l = 0; l <= 8; ++l) {
+    for_level(S, W, l, [&](int32_t y) {
+      int a = 1;
+      int32_t up = -1;
+      if (l > 0) {
+        const int32_t p = t.parent[y];
+        a = W.att[p] && t.child[16 * p + t.id[y]] == y;
+        up = W.anc[p];
       }
-      continue;
-    }
-    --sp;
-    visit(n);
+      bool s = false;
+      if (a && mode == kPruneCube) s = t.level[y] > T.depth;
+      if (a && mode == kReduce) s = t.qe[y] <= thr;
+      W.att[y] = a;
+      W.anc[y] = !a ? -1 : (s ? y : up);
+    });
+    __syncthreads();
+  }
+  for (int l = 8; l >= 0; --l) {
+    for_level(S, W, l, [&](int32_t y) {
+      if (!W.att[y]) return;
+      int32_t sz = 1, g = 1;
+      for (int k = 0; k < T.kids; ++k) {
+        const int32_t ch = t.child[16 * y + k];
+        if (ch < 0) continue;
+        sz += W.size[ch];
+        if (W.anc[ch] != ch) g += W.gsz[ch];
+      }
+      W.size[y] = sz;
+      W.gsz[y] = g;
+    });
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    W.start[0] = 0;
+    W.gstart[0] = 0;
+    W.pre[0] = 0;
+  }
+  __syncthreads();
+  for (int l = 0; l < 8; ++l) {
+    for_level(S, W, l, [&](int32_t y) {
+      if (!W.att[y]) return;
+      int32_t run = W.start[y], grun = W.gstart[y], prun = W.pre[y] + 1;
+      for (int k = 0; k < T.kids; ++k) {
+        const int32_t ch = t.child[16 * y + k];
+        if (ch < 0) continue;
+        W.start[ch] = run;
+        W.pre[ch] = prun;
+        run += W.size[ch];
+        prun += W.size[ch];
+        if (W.anc[ch] != ch) {
+          W.gstart[ch] = grun;
+          grun += W.gsz[ch];
+        } else {
+          W.gstart[ch] = 0;
+        }
+      }
+    });
+    __syncthreads();
   }
 }
 
-__global__ void k_prune_cube(TreeDev t, int kids, int64_t maxc, int depth, int64_t M,
-                             int32_t* info) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  Ctl c{t, kids, 1 + (int64_t)info[4], maxc, depth, 0, 0.0, 0.0};
+__device__ __forceinline__ int32_t post_of(const TreeWork& W, int32_t y) {
+  return W.start[y] + W.size[y] - 1;
+}
+
+// One PruneToCubeDepth or Reduce walk.  Updates *live; for Reduce returns the
+// colour count and the next threshold (seeded with root qe - 1).
+__device__ void tree_pass(const Tree& T, TreeShared& S, int mode, double thr, int64_t* live,
+                          int64_t* count, double* next) {
+  const TreeDev& t = T.t;
+  const TreeWork& W = T.W;
+  tree_shape(T, S, mode, thr);
+  int64_t mine = 0;
+  for (int64_t y = threadIdx.x; y <= T.M; y += kTreeThreads) mine += (W.att[y] && W.anc[y] >= 0);
+  const int64_t attempts = block_sum(mine, S);
+  const int64_t K = *live - T.maxc;
+  const bool cut = K > 0 && attempts > K;
+  if (cut) {
+    const int32_t A = W.size[0];
+    for (int32_t i = threadIdx.x; i < A; i += kTreeThreads) W.arr[i] = 0;
+    __syncthreads();
+    for (int64_t y = threadIdx.x; y <= T.M; y += kTreeThreads)
+      if (W.att[y] && W.anc[y] == (int32_t)y) W.arr[post_of(W, (int32_t)y)] = W.gsz[y];
+    __syncthreads();
+    block_scan(W.arr, A, S);
+  }
+  for (int64_t y = threadIdx.x; y <= T.M; y += kTreeThreads) {
+    int m = 0;
+    if (K > 0 && W.att[y] && W.anc[y] >= 0) {
+      m = 1;
+      if (cut) {
+        const int32_t a = W.anc[y];
+        const int64_t rank = (int64_t)W.arr[post_of(W, a)] + W.gstart[y] + W.gsz[y] - 1;
+        m = rank < K;
+      }
+    }
+    W.merged[y] = m;
+  }
+  __syncthreads();
+  const int64_t merged = K > 0 ? min(attempts, K) : 0;
+  int64_t cnt = 0;
+  double mn = INFINITY;
+  for (int l = 8; l >= 0; --l) {
+    for_level(S, W, l, [&](int32_t y) {
+      if (!W.att[y]) return;
+      int32_t* cl = t.child + 16 * y;
+      double tot[4];
+      for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * y + c];
+      uint64_t nu = t.nu[y];
+      bool any = false;
+      for (int k = 0; k < T.kids; ++k) {  // S children: merged at their own visit
+        const int32_t ch = cl[k];
+        if (ch < 0 || !W.merged[ch] || W.anc[ch] != ch) continue;
+        nu += t.nu[ch];
+        for (int c = 0; c < 4; ++c) tot[c] += t.total[4 * ch + c];
+        any = true;
+      }
+      const uint64_t nu_visit = nu;
+      for (int k = 0; k < T.kids; ++k) {  // the rest: merged inside an ancestor's PruneChild
+        const int32_t ch = cl[k];
+        if (ch < 0 || !W.merged[ch] || W.anc[ch] == ch) continue;
+        nu += t.nu[ch];
+        for (int c = 0; c < 4; ++c) tot[c] += t.total[4 * ch + c];
+        any = true;
+      }
+      if (y == 0 && W.merged[0]) {  // the root's parent is the root itself
+        nu += nu;
+        for (int c = 0; c < 4; ++c) tot[c] += tot[c];
+        any = true;
+      }
+      if (any) {
+        for (int c = 0; c < 4; ++c) t.total[4 * y + c] = tot[c];
+        t.nu[y] = nu;
+        for (int k = 0; k < T.kids; ++k)
+          if (cl[k] >= 0 && W.merged[cl[k]]) cl[k] = -1;
+        if (y == 0 && W.merged[0]) cl[t.id[0]] = -1;
+      }
+      if (mode == kReduce && W.anc[y] != y) {
+        if (nu_visit > 0) ++cnt;
+        mn = fmin(mn, t.qe[y]);
+      }
+    });
+    __syncthreads();
+  }
+  *live -= merged;
+  if (mode == kReduce) {
+    *count = block_sum(cnt, S);
+    *next = fmin(block_min(mn, S), t.qe[0] - 1.0);
+  }
+}
+
+// The k-th smallest (0-based) of {qe of attached nodes with pre-order < L}
+// plus `zeros` zeros: MSB-first radix selection on the bit patterns.
+__device__ double select_kth(const Tree& T, TreeShared& S, int64_t L, int64_t k, int64_t zeros) {
+  const TreeDev& t = T.t;
+  const TreeWork& W = T.W;
+  if (threadIdx.x == 0) {
+    S.sel[0] = 0;  // prefix
+    S.sel[1] = 0;  // mask
+    S.sel[2] = (uint64_t)k;
+  }
+  __syncthreads();
+  const int shifts[6] = {53, 42, 31, 20, 9, 0};
+  const int widths[6] = {11, 11, 11, 11, 11, 9};
+  for (int ps = 0; ps < 6; ++ps) {
+    const int sh = shifts[ps];
+    const uint64_t dm = (1ull << widths[ps]) - 1;
+    for (int i = threadIdx.x; i < 2048; i += kTreeThreads) S.hist[i] = 0;
+    __syncthreads();
+    const uint64_t prefix = S.sel[0], mask = S.sel[1];
+    for (int64_t y = threadIdx.x; y <= T.M; y += kTreeThreads) {
+      if (!W.att[y] || W.pre[y] >= L) continue;
+      const uint64_t b = (uint64_t)__double_as_longlong(t.qe[y]);
+      if ((b & mask) == prefix) atomicAdd(&S.hist[(b >> sh) & dm], 1u);
+    }
+    if (threadIdx.x == 0 && prefix == 0 && zeros > 0) atomicAdd(&S.hist[0], (uint32_t)zeros);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint64_t kk = S.sel[2], cum = 0;
+      uint64_t d = 0;
+      for (; d < dm; ++d) {
+        if (cum + S.hist[d] > kk) break;
+        cum += S.hist[d];
+      }
+      S.sel[2] = kk - cum;
+      S.sel[0] = prefix | (d << sh);
+      S.sel[1] = mask | (dm << sh);
+    }
+    __syncthreads();
+  }
+  return __longlong_as_double((long long)S.sel[0]);
+}
+
+__global__ __launch_bounds__(kTreeThreads) void k_prune_cube(Tree T, int32_t* info) {
+  __shared__ TreeShared S;
+  level_lists(T, S);
+  if (threadIdx.x < 10) T.W.lvl_off[threadIdx.x] = S.off[threadIdx.x];
+  int64_t live = 1 + (int64_t)info[4];
   // Loop 1's MaxQNodes test (rd_octree.cpp:366): the nodes loop 1 creates before
   // its last row are bounded by its node total.
-  if (1 + (int64_t)info[4] > kMaxQNodes) info[6] = 1;
-  int32_t stk_n[12], stk_x[12];
-  int8_t stk_i[12], stk_j[12];
-  if (info[0] < info[5]) {  // loop 1 broke: PruneToCubeDepth
-    post_order(c, stk_n, stk_i, [&](int32_t n) {
-      if (c.t.level[n] > c.depth) prune_child(c, n, stk_x, stk_j);
-    });
+  if (threadIdx.x == 0 && live > kMaxQNodes) info[6] = 1;
+  if (info[0] < info[5]) {  // loop 1 broke: PruneToCubeDepth (rd_octree.cpp:370-372)
+    int64_t cnt = 0;
+    double nx = 0.0;
+    tree_pass(T, S, kPruneCube, 0.0, &live, &cnt, &nx);
   }
   // Loop 2's nodes join the live count; its MaxQNodes test (rd_octree.cpp:380)
   // is bounded by the total.
-  const int64_t live = c.live + (M - (int64_t)info[4]);
-  info[7] = (int32_t)live;
-  if (live > kMaxQNodes) info[6] = 1;
-}
-
-// glibc 2.35 qsort for 8-byte elements: top-down merge sort, left half n/2,
-// merge takes the left element when cmp(left, right) <= 0.  Restated because
-// the comparator (rd_octree.cpp:23-29) is not a strict weak order, so only the
-// same algorithm gives the same permutation.
-__device__ int qe_cmp(double p, double q) {
-  if (p > q) return 1;
-  if (fabs(q - p) <= kEps) return 0;
-  return -1;
-}
-
-__device__ void msort(double* b, double* tmp, int64_t n) {
-  // Iterative form of the recursion: (lo, n, state) frames.
-  struct F {
-    int64_t lo, n;
-    int st;
-  };
-  F stk[64];
-  int sp = 0;
-  stk[0] = {0, n, 0};
-  while (sp >= 0) {
-    F& f = stk[sp];
-    if (f.n <= 1) {
-      --sp;
-      continue;
-    }
-    const int64_t n1 = f.n / 2;
-    if (f.st == 0) {
-      f.st = 1;
-      stk[sp + 1] = {f.lo, n1, 0};
-      ++sp;
-      continue;
-    }
-    if (f.st == 1) {
-      f.st = 2;
-      stk[sp + 1] = {f.lo + n1, f.n - n1, 0};
-      ++sp;
-      continue;
-    }
-    // merge [lo, lo+n1) and [lo+n1, lo+n)
-    int64_t i = f.lo, j = f.lo + n1, k = 0;
-    const int64_t ie = f.lo + n1, je = f.lo + f.n;
-    while (i < ie && j < je) {
-      if (qe_cmp(b[i], b[j]) <= 0)
-        tmp[k++] = b[i++];
-      else
-        tmp[k++] = b[j++];
-    }
-    while (i < ie) tmp[k++] = b[i++];
-    for (int64_t q = 0; q < k; ++q) b[f.lo + q] = tmp[q];
-    --sp;
+  live += T.M - (int64_t)info[4];
+  if (threadIdx.x == 0) {
+    info[7] = (int32_t)live;
+    if (live > kMaxQNodes) info[6] = 1;
   }
 }
 
-// Reduce, DefineImageColormap and the search image.  info layout in sa_octree.h.
-__global__ void k_reduce(TreeDev t, int kids, int64_t maxc, int depth, int64_t M, int assoc,
-                         double root_qe, double* flat, double* tmp, int32_t* info,
-                         PaletteBuild out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  Ctl c{t, kids, (int64_t)info[7], maxc, depth, info[2] + info[3], 0.0, 0.0};
-  t.qe[0] = root_qe;
-  int32_t stk_n[12], stk_x[12];
-  int8_t stk_i[12], stk_j[12];
-  // ReduceImageColors (rd_octree.cpp:388-404).
-  if (c.count > c.maxc) {
-    c.next_thr = 0.0;
-    if (c.live > 0) {
-      const int64_t L = c.live;
-      for (int64_t k = 0; k < L; ++k) flat[k] = 0.0;
-      // Pre-order flattening, cut off at the live counter (rd_octree.cpp:129-140).
-      int64_t off = 0;
-      int sp = 0;
-      stk_n[0] = 0;
-      stk_i[0] = -1;
-      while (sp >= 0 && off < L) {
-        const int32_t n = stk_n[sp];
-        if (stk_i[sp] < 0) {
-          flat[off++] = t.qe[n];
-          stk_i[sp] = 0;
-          continue;
-        }
-        if (stk_i[sp] < kids) {
-          const int32_t ch = t.child[16 * n + stk_i[sp]];
-          ++stk_i[sp];
-          if (ch >= 0) {
-            ++sp;
-            stk_n[sp] = ch;
-            stk_i[sp] = -1;
-          }
-          continue;
-        }
-        --sp;
-      }
-      msort(flat, tmp, L);
-      const int64_t aggressive = (110 * (maxc + 1)) / 100;
-      if (L > aggressive) c.next_thr = flat[L - aggressive];
+// ReduceImageColors (rd_octree.cpp:388-404), DefineImageColormap (:142-176) and
+// the search image.  info layout in sa_octree.h.
+__global__ __launch_bounds__(kTreeThreads) void k_reduce(Tree T, int assoc, double root_qe,
+                                                         int32_t* info, PaletteBuild out) {
+  __shared__ TreeShared S;
+  const TreeDev& t = T.t;
+  const TreeWork& W = T.W;
+  if (threadIdx.x < 10) S.off[threadIdx.x] = W.lvl_off[threadIdx.x];
+  if (threadIdx.x == 0) t.qe[0] = root_qe;
+  __syncthreads();
+  int64_t live = info[7];
+  int64_t count = (int64_t)info[2] + info[3];
+  double next = 0.0;
+  if (count > T.maxc) {
+    next = 0.0;
+    if (live > 0) {
+      tree_shape(T, S, kShape, 0.0);
+      const int64_t L = live;
+      int64_t mine = 0;
+      for (int64_t y = threadIdx.x; y <= T.M; y += kTreeThreads) mine += (W.att[y] && W.pre[y] < L);
+      const int64_t inc = block_sum(mine, S);
+      const int64_t aggressive = (110 * (T.maxc + 1)) / 100;
+      if (L > aggressive) next = select_kth(T, S, L, L - aggressive, L - inc);
     }
   }
   int passes = 0;
-  while (c.count > c.maxc) {
+  while (count > T.maxc) {
     // The reference loops forever if a pass prunes nothing; stop instead.
     if (++passes > 100000) {
-      info[6] = 3;
+      if (threadIdx.x == 0) info[6] = 3;
       return;
     }
-    c.prune_thr = c.next_thr;
-    c.next_thr = t.qe[0] - 1.0;
-    c.count = 0;
-    post_order(c, stk_n, stk_i, [&](int32_t n) {
-      if (c.t.qe[n] <= c.prune_thr) {
-        prune_child(c, n, stk_x, stk_j);
-      } else {
-        if (c.t.nu[n] > 0) ++c.count;
-        if (c.t.qe[n] < c.next_thr) c.next_thr = c.t.qe[n];
-      }
-    });
+    const double thr = next;
+    tree_pass(T, S, kReduce, thr, &live, &count, &next);
   }
-  // DefineImageColormap (rd_octree.cpp:142-176).
-  int cnt = 0;
-  post_order(c, stk_n, stk_i, [&](int32_t n) {
-    if (t.nu[n] == 0) return;
-    double* e = out.entries + 4 * cnt;
-    const double alpha = safe_recip((double)t.nu[n]);
-    const double* tot = t.total + 4 * n;
-    if (!assoc) {
-      e[0] = (double)(float)(alpha * kRange * tot[0]);
-      e[1] = (double)(float)(alpha * kRange * tot[1]);
-      e[2] = (double)(float)(alpha * kRange * tot[2]);
-      e[3] = kRange;
-    } else {
-      const double opacity = alpha * kRange * tot[3];
-      e[3] = (double)(float)opacity;
-      if (e[3] == kRange) {
-        e[0] = (double)(float)(alpha * kRange * tot[0]);
-        e[1] = (double)(float)(alpha * kRange * tot[1]);
-        e[2] = (double)(float)(alpha * kRange * tot[2]);
-      } else {
-        const double gamma = safe_recip(kScale * e[3]);
-        e[0] = (double)(float)(alpha * gamma * kRange * tot[0]);
-        e[1] = (double)(float)(alpha * gamma * kRange * tot[1]);
-        e[2] = (double)(float)(alpha * gamma * kRange * tot[2]);
-      }
-    }
-    t.color_number[n] = (uint32_t)cnt;
-    ++cnt;
-  });
-  info[8] = cnt;
-  info[9] = (int32_t)c.live;
-  // Search image: rows for internal reachable nodes (pre-order), candidate
-  // ranges from the post-order of colour-bearing nodes (see sa_palette.h).
-  int32_t rows = 0;
-  {
-    int sp = 0;
-    stk_n[0] = 0;
-    stk_i[0] = -1;
-    while (sp >= 0) {
-      const int32_t n = stk_n[sp];
-      if (stk_i[sp] < 0) {
-        bool internal = (n == 0);
-        for (int k = 0; k < kids; ++k) internal |= t.child[16 * n + k] >= 0;
-        t.row[n] = internal ? rows++ : -1;
-        stk_i[sp] = 0;
-        continue;
-      }
-      if (stk_i[sp] < kids) {
-        const int32_t ch = t.child[16 * n + stk_i[sp]];
-        ++stk_i[sp];
-        if (ch >= 0) {
-          ++sp;
-          stk_n[sp] = ch;
-          stk_i[sp] = -1;
-        }
-        continue;
-      }
-      --sp;
-    }
+  // DefineImageColormap: colour numbers are the post-order ranks of the
+  // colour-bearing nodes.  Search-image rows are the pre-order ranks of the
+  // internal nodes (the root always has a row).
+  tree_shape(T, S, kShape, 0.0);
+  const int32_t A = W.size[0];
+  for (int32_t i = threadIdx.x; i < A; i += kTreeThreads) {
+    W.arr[i] = 0;
+    W.arr2[i] = 0;
   }
-  const PaletteLayout L = palette_layout(cnt, rows, kids);
-  info[10] = rows;
-  info[11] = L.bytes;
-  if (L.bytes > out.image_cap || rows > 32767) {
-    info[6] = 2;
+  __syncthreads();
+  for (int64_t y = threadIdx.x; y <= T.M; y += kTreeThreads) {
+    if (!W.att[y]) continue;
+    bool internal = (y == 0);
+    for (int k = 0; k < T.kids; ++k) internal |= t.child[16 * y + k] >= 0;
+    W.merged[y] = internal;  // reused as the internal flag
+    if (t.nu[y] > 0) W.arr[post_of(W, (int32_t)y)] = 1;
+    if (internal) W.arr2[W.pre[y]] = 1;
+  }
+  __syncthreads();
+  const int32_t cnt = block_scan(W.arr, A, S);
+  const int32_t rows = block_scan(W.arr2, A, S);
+  const PaletteLayout L = palette_layout(cnt, rows, T.kids);
+  if (threadIdx.x == 0) {
+    info[8] = cnt;
+    info[9] = (int32_t)live;
+    info[10] = rows;
+    info[11] = L.bytes;
+  }
+  if (L.bytes > out.image_cap || rows > 32767 || cnt > T.maxc) {
+    if (threadIdx.x == 0) info[6] = 2;
     return;
   }
   unsigned char* img = out.image;
@@ -692,65 +1064,76 @@ __global__ void k_reduce(TreeDev t, int kids, int64_t maxc, int depth, int64_t M
   int32_t* sub = (int32_t*)(img + L.off_sub);
   int16_t* child = (int16_t*)(img + L.off_child);
   uint16_t* cand = (uint16_t*)(img + L.off_cand);
-  for (int i = 0; i < cnt; ++i) {
-    const double* e = out.entries + 4 * i;
-    for (int k = 0; k < 4; ++k) ent[4 * i + k] = (float)e[k];
-    assoc_entry(assoc != 0, e, col + 4 * i);
-  }
-  // Post-order with subtree starts.
-  int32_t ncand = 0;
-  {
-    int32_t first[12];
-    int sp = 0;
-    stk_n[0] = 0;
-    stk_i[0] = 0;
-    first[0] = 0;
-    while (sp >= 0) {
-      const int32_t n = stk_n[sp];
-      if (stk_i[sp] < kids) {
-        const int32_t ch = t.child[16 * n + stk_i[sp]];
-        ++stk_i[sp];
-        if (ch >= 0) {
-          ++sp;
-          stk_n[sp] = ch;
-          stk_i[sp] = 0;
-          first[sp] = ncand;
-        }
-        continue;
-      }
-      if (t.nu[n] != 0) {
-        const uint32_t cn = t.color_number[n];
-        for (int k = 0; k < 4; ++k) cent[4 * ncand + k] = ent[4 * cn + k];
-        cand[ncand++] = (uint16_t)cn;
-      }
-      const int32_t r = t.row[n];
-      if (r >= 0) {
-        sub[2 * r] = first[sp];
-        sub[2 * r + 1] = ncand - first[sp];
-        for (int k = 0; k < kids; ++k) {
-          const int32_t ch = t.child[16 * n + k];
-          child[r * kids + k] = ch < 0 ? kNoChild : (t.row[ch] >= 0 ? (int16_t)t.row[ch] : kLeaf);
+  for (int64_t y = threadIdx.x; y <= T.M; y += kTreeThreads) {
+    if (!W.att[y]) {
+      t.row[y] = -1;
+      continue;
+    }
+    const int32_t post = post_of(W, (int32_t)y);
+    if (t.nu[y] > 0) {
+      const int32_t cn = W.arr[post];
+      t.color_number[y] = (uint32_t)cn;
+      double* e = out.entries + 4 * cn;
+      const double alpha = safe_recip((double)t.nu[y]);
+      const double* tot = t.total + 4 * y;
+      if (!assoc) {
+        e[0] = (double)(float)(alpha * kRange * tot[0]);
+        e[1] = (double)(float)(alpha * kRange * tot[1]);
+        e[2] = (double)(float)(alpha * kRange * tot[2]);
+        e[3] = kRange;
+      } else {
+        const double opacity = alpha * kRange * tot[3];
+        e[3] = (double)(float)opacity;
+        if (e[3] == kRange) {
+          e[0] = (double)(float)(alpha * kRange * tot[0]);
+          e[1] = (double)(float)(alpha * kRange * tot[1]);
+          e[2] = (double)(float)(alpha * kRange * tot[2]);
+        } else {
+          const double gamma = safe_recip(kScale * e[3]);
+          e[0] = (double)(float)(alpha * gamma * kRange * tot[0]);
+          e[1] = (double)(float)(alpha * gamma * kRange * tot[1]);
+          e[2] = (double)(float)(alpha * gamma * kRange * tot[2]);
         }
       }
-      --sp;
+      for (int k = 0; k < 4; ++k) {
+        ent[4 * cn + k] = (float)e[k];
+        cent[4 * cn + k] = (float)e[k];
+      }
+      assoc_entry(assoc != 0, e, col + 4 * cn);
+      cand[cn] = (uint16_t)cn;
+    }
+    if (!W.merged[y]) {
+      t.row[y] = -1;
+      continue;
+    }
+    const int32_t r = W.arr2[W.pre[y]];
+    t.row[y] = r;
+    const int32_t first = W.arr[W.start[y]];
+    sub[2 * r] = first;
+    sub[2 * r + 1] = W.arr[post] + (t.nu[y] > 0 ? 1 : 0) - first;
+    for (int k = 0; k < T.kids; ++k) {
+      const int32_t ch = t.child[16 * y + k];
+      child[r * T.kids + k] =
+          ch < 0 ? kNoChild : (W.merged[ch] ? (int16_t)W.arr2[W.pre[ch]] : kLeaf);
     }
   }
   // The descent table is filled by k_descent (one thread per entry).
-  PaletteDev* h = out.hdr;
-  h->count = cnt;
-  h->assoc = assoc;
-  h->kids = kids;
-  h->rows = rows;
-  h->bytes = L.bytes;
-  h->off_ent = L.off_ent;
-  h->off_cent = L.off_cent;
-  h->off_sub = L.off_sub;
-  h->off_desc = L.off_desc;
-  h->off_child = L.off_child;
-  h->off_cand = L.off_cand;
-  h->pad_ = 0;
-  h->image = out.image;
-  (void)M;
+  if (threadIdx.x == 0) {
+    PaletteDev* h = out.hdr;
+    h->count = cnt;
+    h->assoc = assoc;
+    h->kids = T.kids;
+    h->rows = rows;
+    h->bytes = L.bytes;
+    h->off_ent = L.off_ent;
+    h->off_cent = L.off_cent;
+    h->off_sub = L.off_sub;
+    h->off_desc = L.off_desc;
+    h->off_child = L.off_child;
+    h->off_cand = L.off_cand;
+    h->pad_ = 0;
+    h->image = out.image;
+  }
 }
 
 // The three-level descent table of the search image (sa_palette.h).
@@ -897,7 +1280,11 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
   double* bound = nullptr;
   double hbound[2] = {0, 0};
   double root_qe = 0.0;
-  double *flat = nullptr, *tmp = nullptr, *record = nullptr;
+  double *flat = nullptr, *record = nullptr;
+  Terms T{};
+  int32_t *long_list = nullptr, *long_count = nullptr;
+  TreeWork W{};
+  int long_grid = 0;
   TreeDev t{};
   OctreeOut& out = *outp;
   if (N < 1) {
@@ -1060,16 +1447,38 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
   OT_TRY(hipGetLastError());
   {
     Runs RR{count, px};
-    k_node_sums<<<blocks(M, 64), 64, 0, s>>>(t, M, RR, spair, seg_b, seg_e, R1, depth, assoc,
-                                             nullptr);
+    OT_TRY(A.get(&T.term, P));
+    OT_TRY(A.get(&T.lval, P));
+    OT_TRY(A.get(&T.lcnt, P));
+    k_pair_terms<<<blocks(P, 256), 256, 0, s>>>(t.key, RR, spair, snode, P, R1, depth, assoc, T);
+    OT_TRY(hipGetLastError());
+    OT_TRY(A.get(&long_list, M + 1));
+    OT_TRY(A.get(&long_count, 2));
+    OT_TRY(hipMemsetAsync(long_count, 0, 8, s));
+    long_grid = (int)std::min<int64_t>(std::max<int64_t>(P / kShortSeg, 1), 2048);
+    k_fold_short<false><<<blocks(M, 64), 64, 0, s>>>(t, M, T, spair, seg_b, seg_e, R1, depth,
+                                                     long_list, long_count);
+    k_fold_long<false><<<long_grid, 64, 0, s>>>(t, T, spair, seg_b, seg_e, R1, depth, long_list,
+                                                long_count);
     OT_TRY(hipGetLastError());
     // 5a. PruneToCubeDepth, then loop 2's totals.
     {
       const int32_t hh = (int32_t)h;
       OT_TRY(hipMemcpyAsync(info + 5, &hh, 4, hipMemcpyHostToDevice, s));
     }
-    k_prune_cube<<<1, 1, 0, s>>>(t, kids, maxc, depth, M, info);
-    k_node_sums_l2<<<blocks(M, 64), 64, 0, s>>>(t, M, RR, spair, seg_b, seg_e, R1, depth, assoc);
+    for (int32_t** pp : {&W.lvl, &W.att, &W.anc, &W.size, &W.gsz, &W.start, &W.gstart, &W.pre,
+                         &W.merged, &W.arr, &W.arr2})
+      OT_TRY(A.get(pp, M + 2));
+    OT_TRY(A.get(&W.lvl_off, 10));
+    {
+      const Tree TT{t, W, kids, maxc, depth, M};
+      k_prune_cube<<<1, kTreeThreads, 0, s>>>(TT, info);
+    }
+    OT_TRY(hipMemsetAsync(long_count + 1, 0, 4, s));
+    k_fold_short<true><<<blocks(M, 64), 64, 0, s>>>(t, M, T, spair, seg_b, seg_e, R1, depth,
+                                                    long_list, long_count + 1);
+    k_fold_long<true><<<long_grid, 64, 0, s>>>(t, T, spair, seg_b, seg_e, R1, depth, long_list,
+                                               long_count + 1);
     OT_TRY(hipGetLastError());
     // The root's chain: bound, or exact when the bound is not enough.
     OT_TRY(A.get(&bound, 2));
@@ -1091,12 +1500,13 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
       goto done;
     }
     root_qe = hbound[0];
-    // |exact - approx| <= P * 2^-52 * sum; demand a clear separation of 1 + that.
+    // The approximation differs from the serial chain by at most ~3 P u sum
+    // (u = 2^-53): the reference's own rounding, the running values it adds,
+    // and the order of the parallel sum.  Demand a clear separation of 1 + 8 P u sum.
     const double err = (double)P * 2.3e-16 * std::fabs(root_qe) * 4.0;
     if (!(root_qe - 1.0 - err > hbound[1] + 1e-9 * hbound[1] + 2e-12) || M == 0) {
       OT_TRY(A.get(&record, P));
-      k_node_sums<<<blocks(M, 64), 64, 0, s>>>(t, M, RR, spair, seg_b, seg_e, R1, depth, assoc,
-                                               record);
+      k_fold_record<<<blocks(M, 64), 64, 0, s>>>(M, T.term, spair, seg_b, seg_e, record);
       OT_TRY(A.get(&flat, 1));
       k_root_exact<<<1, 1, 0, s>>>(record, P, flat);
       OT_TRY(hipGetLastError());
@@ -1107,9 +1517,6 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
   }
   // 5b. Reduce, colormap, search image.
   {
-    const int64_t live = hinfo[7];
-    OT_TRY(A.get(&flat, std::max<int64_t>(live, 1)));
-    OT_TRY(A.get(&tmp, std::max<int64_t>(live, 1)));
     // The palette outlives this call: plain allocations, owned by the caller.
     const PaletteLayout cap = palette_layout((int)maxc, (int)(M + 1), kids);
     PaletteBuild pb;
@@ -1120,7 +1527,10 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
     out.hdr = pb.hdr;
     OT_TRY(hipMalloc((void**)&pb.entries, sizeof(double) * 4 * (size_t)maxc));
     out.entries = pb.entries;
-    k_reduce<<<1, 1, 0, s>>>(t, kids, maxc, depth, M, assoc, root_qe, flat, tmp, info, pb);
+    {
+      const Tree TT{t, W, kids, maxc, depth, M};
+      k_reduce<<<1, kTreeThreads, 0, s>>>(TT, assoc, root_qe, info, pb);
+    }
     k_descent<<<blocks(desc_entries(kids), 256), 256, 0, s>>>(pb.hdr, info);
     OT_TRY(hipGetLastError());
     OT_TRY(hipMemcpyAsync(hinfo, info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
