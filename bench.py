@@ -27,8 +27,10 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import queue
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -78,6 +80,13 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-image-per-launch leg")
+    ap.add_argument("--build-threads", type=int, default=8,
+                    help="host threads (one stream each) classifying palettes concurrently")
+    ap.add_argument("--head", type=int, default=0,
+                    help="launch the walk of the N largest images before the rest are "
+                         "classified (0: one walk launch after all palettes)")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="first run a 256 MiB device copy (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the sequential walk shards by image with no "
@@ -92,6 +101,13 @@ def main():
     sa.init(0)
     dev = sa.device_name()
     log("device:", dev)
+    if args.copy_calibrate:
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     t0 = time.perf_counter()
     batch = make_batch(args.images, args.seed)
     log(f"generated {len(batch)} images in {time.perf_counter() - t0:.1f}s")
@@ -103,20 +119,47 @@ def main():
         dsts.append(torch.empty(img.shape[0] * img.shape[1] * 4, dtype=torch.uint8, device="cuda"))
     torch.cuda.synchronize()
 
-    def build_palettes(stream=None):
-        return [sa.Palette.build(src, img.shape[1], img.shape[0], sa.PIX_RGBA8, col, True,
-                                 synth.is_gray(img), stream=stream)
-                for src, (img, col) in zip(srcs, batch)]
+    grays = [synth.is_gray(img) for img, _ in batch]
+    order = sorted(range(len(batch)), key=lambda k: -batch[k][0].shape[0] * batch[k][0].shape[1])
+    build_streams = queue.Queue()
+    for _ in range(args.build_threads):
+        build_streams.put(torch.cuda.Stream())
+    pool = ThreadPoolExecutor(max_workers=args.build_threads)
+    walk_streams = [torch.cuda.Stream(), torch.cuda.Stream()]
 
-    def specs_for(pals):
-        return [sa.image_spec(src, dst, img.shape[1], img.shape[0], sa.PIX_RGBA8, sa.PIX_RGBA8, p)
-                for src, dst, (img, _), p in zip(srcs, dsts, batch, pals)]
+    def build_one(k):
+        img, col = batch[k]
+        st = build_streams.get()
+        try:
+            return sa.Palette.build(srcs[k], img.shape[1], img.shape[0], sa.PIX_RGBA8, col, True,
+                                    grays[k], stream=st)
+        finally:
+            build_streams.put(st)
+
+    def build_palettes(ks=None):
+        ks = list(range(len(batch))) if ks is None else ks
+        return dict(zip(ks, pool.map(build_one, ks)))
+
+    def specs_for(pals, ks=None):
+        ks = list(range(len(batch))) if ks is None else ks
+        return [sa.image_spec(srcs[k], dsts[k], batch[k][0].shape[1], batch[k][0].shape[0],
+                              sa.PIX_RGBA8, sa.PIX_RGBA8, pals[k]) for k in ks]
 
     def step():
-        pals = build_palettes()
-        sa.walk_seq_batch(specs_for(pals))
+        # Palettes are classified concurrently (one stream per host thread); the
+        # walk is one batched launch, largest images first.
+        if args.head > 0:
+            head, tail = order[:args.head], order[args.head:]
+            pals = build_palettes(head)
+            sa.walk_seq_batch(specs_for(pals, head), stream=walk_streams[0])
+            if tail:
+                pals.update(build_palettes(tail))
+                sa.walk_seq_batch(specs_for(pals, tail), stream=walk_streams[1])
+        else:
+            pals = build_palettes(order)
+            sa.walk_seq_batch(specs_for(pals, order))
         torch.cuda.synchronize()
-        return pals
+        return [pals[k] for k in range(len(batch))]
 
     for _ in range(args.warmup):
         step()
@@ -153,6 +196,12 @@ def main():
     build_palettes()
     torch.cuda.synchronize()
     t_pal = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in range(len(batch)):
+        img, col = batch[k]
+        sa.Palette.build(srcs[k], img.shape[1], img.shape[0], sa.PIX_RGBA8, col, True, grays[k])
+    torch.cuda.synchronize()
+    t_pal_serial = time.perf_counter() - t0
 
     plain = None
     if not args.no_plain:
@@ -190,6 +239,13 @@ def main():
 
     algo_bytes = 8.0 * pixels  # RGBA8 in + RGBA8 out per visit
     achieved = algo_bytes / (walk_ms * 1e-3) / 1e9
+    # HBM bytes per 64-image walk launch, measured with rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE on this same workload (scaled by a copy kernel).
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "seq_walk_c2_traffic.json")
+    if os.path.exists(tpath) and args.images == 64 and args.seed == 2:
+        with open(tpath) as fh:
+            traffic = round(json.load(fh)["traffic_bytes"])
     line = {
         "metric": METRIC,
         "value": round(pixels / t_batched / 1e6, 3),
@@ -212,11 +268,13 @@ def main():
                       "kernel_ms": round(walk_ms, 3), "gather_ms": round(gather_ms, 3),
                       "pack_ms": round(pack_ms, 3)},
         "palette_build_ms": round(t_pal * 1e3, 3),
+        "palette_build_serial_ms": round(t_pal_serial * 1e3, 3),
         "plain": plain,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "kernel": "seq_walk_kernel", "note": "latency-bound serial chain; bytes are "
-                     "algorithmic (8 B/visit), per launch over the whole batch"},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "kernel": "seq_walk_kernel", "note": "latency-bound serial chain; achieved is "
+                     "algorithmic (8 B/visit) over the whole 64-image launch; traffic is the "
+                     "launch's measured HBM bytes (profiles/seq_walk_c2_batch64.md)"},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -165,6 +165,12 @@ int sa_walk_blocks(const sa_image* frames, int n, int block, double diffusion, v
 int sa_walk_approx(const sa_image* frames, int n, int iterations, int taps, int fp64,
                    int use_tree, double diffusion, void* stream, char* err, size_t errlen);
 
+/* The inverse filter g[0..taps] of the Jacobi engine, (I - H)^-1 truncated
+ * to taps+1 terms, computed on the host with the reference's weight
+ * expression.  Replaces rd::ApproxInverseFilter (include/rd_riemersma.h:309;
+ * src/rd_approx_cuda.cu:73-88).  `g` holds taps+1 doubles; taps in 1..4096. */
+int sa_inverse_filter(int taps, double diffusion, double* g, char* err, size_t errlen);
+
 /* ---- kernel timing --------------------------------------------------------- */
 
 /* Brackets every engine kernel launch with HIP events on its own stream while

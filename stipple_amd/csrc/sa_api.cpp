@@ -140,6 +140,7 @@ struct CurveEntry {
   int64_t w, h;
   int32_t* order;  // w*h
   size_t bytes;
+  hipEvent_t ready;  // recorded after the build on the building stream
 };
 std::mutex g_curve_mu;
 std::list<CurveEntry> g_curves;  // most recently used first
@@ -174,6 +175,8 @@ int get_curve(int64_t w, int64_t h, hipStream_t s, const int32_t** out, char* er
   for (auto it = g_curves.begin(); it != g_curves.end(); ++it) {
     if (it->w == w && it->h == h) {
       g_curves.splice(g_curves.begin(), g_curves, it);
+      // Built on another stream, possibly still in flight: order after it.
+      HIP_TRY(hipStreamWaitEvent(s, g_curves.front().ready, 0));
       *out = g_curves.front().order;
       return SA_OK;
     }
@@ -188,6 +191,7 @@ int get_curve(int64_t w, int64_t h, hipStream_t s, const int32_t** out, char* er
     HIP_TRY(hipDeviceSynchronize());
     while (g_curve_bytes + e.bytes > kCurveBudget && !g_curves.empty()) {
       HIP_TRY(hipFree(g_curves.back().order));
+      HIP_TRY(hipEventDestroy(g_curves.back().ready));
       g_curve_bytes -= g_curves.back().bytes;
       g_curves.pop_back();
     }
@@ -195,9 +199,11 @@ int get_curve(int64_t w, int64_t h, hipStream_t s, const int32_t** out, char* er
   HIP_TRY(hipMalloc((void**)&e.order, e.bytes));
   const int rc = build_curve(w, h, e.order, s, err, errlen);
   if (rc != SA_OK) {
-    hipFree(e.order);
+    (void)hipFree(e.order);
     return rc;
   }
+  HIP_TRY(hipEventCreateWithFlags(&e.ready, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(e.ready, s));
   g_curves.push_front(e);
   g_curve_bytes += e.bytes;
   *out = e.order;
@@ -604,7 +610,7 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       const int32_t* curve = nullptr;
       rc = get_curve(im.width, im.height, s, &curve, err, errlen);
       if (rc) {
-        hipFreeAsync(scratch, s);
+        (void)hipFreeAsync(scratch, s);
         return rc;
       }
       it.src = im.src;
@@ -727,6 +733,16 @@ static void inverse_filter(int taps, double diffusion, std::vector<double>& g) {
     for (int k = 1; k <= kmax; ++k) acc += (diffusion / 16.0) * w[k - 1] * g[(size_t)(d - k)];
     g[(size_t)d] = acc;
   }
+}
+
+extern "C" int sa_inverse_filter(int taps, double diffusion, double* g, char* err,
+                                 size_t errlen) {
+  if (taps < 1 || taps > 4096) return fail(err, errlen, SA_EREFUSED, "--approx-taps out of range");
+  if (!g) return fail(err, errlen, SA_EINVAL, "null output");
+  std::vector<double> v;
+  inverse_filter(taps, diffusion, v);
+  std::copy(v.begin(), v.end(), g);
+  return SA_OK;
 }
 
 extern "C" int sa_walk_approx(const sa_image* frames, int n, int iterations, int taps, int fp64,

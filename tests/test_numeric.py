@@ -43,3 +43,36 @@ def test_scaled_weights_order(oracle):
     for diff in (1.0, 0.5, 0.75):
         sw = oracle.scaled_weights(diff)
         assert np.array_equal(sw, (np.float64(1 / 16) * np.float64(diff)) * w)
+
+
+def test_qsort_with_quantize_error_compare_sorts_ascending():
+    """The device build replaces QuantizeErrorFlatten + qsort (src/rd_octree.cpp:388-398)
+    by a radix selection of errors[n - aggressive].  That is only valid if this
+    libc's qsort, with QuantizeErrorCompare (src/rd_octree.cpp:23-29, which returns
+    0 for values within MagickEpsilon), still yields the plain ascending order.
+    Checked here on arrays full of near-ties, on both of glibc's buffer paths
+    (< 1024 bytes on the stack, larger ones malloc'd)."""
+    import ctypes as C
+    libc = C.CDLL(None)
+    cmp_t = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+    def qe_cmp(a, b):
+        p, q = a[0], b[0]
+        if p > q:
+            return 1
+        if abs(q - p) <= 1e-12:
+            return 0
+        return -1
+
+    cmp = cmp_t(qe_cmp)
+    libc.qsort.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, cmp_t]
+    rng = np.random.default_rng(7)
+    for n in (5, 64, 127, 129, 1000, 5000):
+        for trial in range(4):
+            base = rng.choice([0.0, 1e-6, 0.5, 3.0, 1234.5], n)
+            # Clusters of distinct values closer than MagickEpsilon.
+            v = base + rng.integers(-3, 4, n) * np.spacing(base + 1e-300) * rng.integers(1, 5, n)
+            v = np.abs(v)
+            a = np.ascontiguousarray(v.copy())
+            libc.qsort(a.ctypes.data, n, 8, cmp)
+            assert np.array_equal(a.view(np.uint64), np.sort(v).view(np.uint64)), (n, trial)
