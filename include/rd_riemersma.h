@@ -66,7 +66,8 @@ void HilbertPoint(std::int64_t n, int level, int* out_x, int* out_y);
 // :178-179.  The in-bounds pixel ids of the block engine's curve (generated
 // on the device): n in [0, 4^level - 1) plus a trailing origin, and the
 // last-writer owner of every pixel (-1 for the one unvisited pixel of a
-// power-of-two-wide image).  Returns the curve length.
+// power-of-two-wide image).  Returns the curve length; a 1x1 image (level 0)
+// has an empty block curve, as in the reference.
 std::size_t BuildCurveIndex(int level, std::size_t width, std::size_t height,
                             std::vector<int>* out, std::vector<int>* owner);
 

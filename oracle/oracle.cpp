@@ -6,7 +6,7 @@
 // float operation below rounds exactly once, in the order written, which is
 // the order the reference writes them in.
 //
-// How each part is pinned (tests/test_oracle.py, tests/test_golden.py):
+// How each part is pinned (tests/test_oracle.py, tests/test_numeric.py):
 //   * weights        -- exact IEEE hex of all 16 values (closed form 16^(-i/15))
 //                       and the reference's table; against oracle/_ref.
 //   * curve          -- closed form == explicit recursion for levels 0..9
@@ -24,12 +24,17 @@
 //                       block >= n equals the cache-free sequential walk on
 //                       the block curve.
 //   * inverse filter -- triangular Toeplitz solve of (I-H) g = delta (numpy).
-//   * Jacobi         -- PARITY UNPINNED as a whole: the reference has no CPU
-//                       Jacobi and its CUDA engine cannot be built here.  Its
-//                       parts are pinned: the inverse filter (above), the lookup
-//                       (shared with the octree), the fixed-point property
-//                       (a sweep applied to the cache-free walk's indices with a
-//                       long filter returns those indices, up to clamping).
+//   * Jacobi         -- parity PARTIAL: the reference has no CPU Jacobi and
+//                       its CUDA engine cannot be built here, so no golden
+//                       vector exists.  Pinned: the inverse filter (above); the
+//                       lookup (shared with the octree); zero diffusion equals
+//                       the block walk at diffusion 0 (InitIndex, lookup and
+//                       owner scatter); one sweep over a flat image from a
+//                       uniform index equals the closed form c + r * S_i with
+//                       S_i the partial sums of the filter (the tap sum, its
+//                       sign, indices and range).  The fp32/fp64 rounding of
+//                       the tap sum follows src/rd_approx_cuda.cu:90-228 and is
+//                       pinned only by that restatement.
 #include "oracle.h"
 
 #include <algorithm>

@@ -231,7 +231,12 @@ std::size_t BuildCurveIndex(int level, std::size_t width, std::size_t height,
   if (out) out->clear();
   if (owner) owner->clear();
   const size_t n = width * height;
-  if (n == 0 || level != ComputeCurveLevel(width, height) || !ensure_device(nullptr)) return 0;
+  if (n == 0 || level != ComputeCurveLevel(width, height)) return 0;
+  if (level == 0) {  // a 1x1 image: the block curve is empty (src/rd_riemersma_cpu.cpp:275)
+    if (owner) owner->assign(n, -1);
+    return 0;
+  }
+  if (!ensure_device(nullptr)) return 0;
   DevBuf d;
   if (!d.reserve(sizeof(int32_t) * n)) return 0;
   char err[kErr] = {0};
@@ -244,7 +249,7 @@ std::size_t BuildCurveIndex(int level, std::size_t width, std::size_t height,
   // The sequential order is HilbertPoint(k) for k in [0, 4^L) in bounds; its last
   // point (2^L - 1, 0) is in bounds exactly when width == 2^L.  The block curve
   // stops before it and ends with the origin (src/rd_riemersma_cpu.cpp:269-356).
-  const bool drop_last = level > 0 && width == ((size_t)1 << level);
+  const bool drop_last = width == ((size_t)1 << level);
   std::vector<int> order(seq.begin(), seq.end() - (drop_last ? 1 : 0));
   order.push_back(0);
   if (owner) {
@@ -259,14 +264,16 @@ std::size_t BuildCurveIndex(int level, std::size_t width, std::size_t height,
 namespace {
 struct CurvePair {
   size_t w = 0, h = 0;
+  bool valid = false;
   std::vector<int> order, owner;
 };
 CurvePair& curve_cache(size_t w, size_t h) {
   thread_local CurvePair c;
-  if (c.w != w || c.h != h || c.order.empty()) {
+  if (c.w != w || c.h != h || !c.valid) {
     c.w = w;
     c.h = h;
     BuildCurveIndex(ComputeCurveLevel(w, h), w, h, &c.order, &c.owner);
+    c.valid = !c.owner.empty();
   }
   return c;
 }
@@ -274,12 +281,12 @@ CurvePair& curve_cache(size_t w, size_t h) {
 
 const std::vector<int>* BlockCurveOrder(std::size_t width, std::size_t height) {
   CurvePair& c = curve_cache(width, height);
-  return c.order.empty() ? nullptr : &c.order;
+  return c.valid ? &c.order : nullptr;
 }
 
 const std::vector<int>* BlockCurveOwner(std::size_t width, std::size_t height) {
   CurvePair& c = curve_cache(width, height);
-  return c.order.empty() ? nullptr : &c.owner;
+  return c.valid ? &c.owner : nullptr;
 }
 
 // ---- runtime ---------------------------------------------------------------------

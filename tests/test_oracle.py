@@ -247,3 +247,62 @@ def test_reference_curve_self_test():
     ref = Ref()
     for level in range(1, 9):
         assert ref.curve_mismatches(level, 0) == 0
+
+
+# ---- Jacobi restatement: what the mathematics fixes --------------------------------
+# A sweep computes t[i] = c[i] + sum_{d=0..min(taps,i)} g[d] (x[i-d] - c[i-d]),
+# c = pal[q], g = (I - H)^-1 (pinned above by the Toeplitz solve).  Two cases
+# have closed forms that do not depend on how the sweep is coded:
+#   * zero diffusion: g = (1, 0, 0, ...), so every sweep maps q to Lookup(x) --
+#     the block walk at diffusion 0;
+#   * a flat image from a uniform q: every residual is the same r, so one sweep
+#     gives t[i] = c + r * S_i with S_i = sum_{d<=min(taps,i)} g[d], and q'[i]
+#     is the palette colour nearest to that.
+# A dropped tap, a wrong sign, a residual taken at the wrong index or an
+# off-by-one in the tap range each move the positions where q' flips.
+
+def _jac_image(seed, w, h, assoc):
+    from stipple_amd import synth
+    rng = np.random.default_rng(seed)
+    img = synth.sinusoid_image(w, h, rng, channels=4 if assoc else 3)
+    return synth.widen_q16(img)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_jacobi_zero_diffusion_is_pointwise_lookup(oracle, fp64):
+    f = _jac_image(5, 23, 17, True)
+    tree = oracle.tree_build(f, 12, True)
+    got, q = oracle.jacobi(tree, f, 3, taps=16, fp64=fp64, diffusion=0.0, want_q=True)
+    exp, qb = oracle.walk_blocks(tree, f, 10 ** 6, diffusion=0.0, want_index=True)
+    assert np.array_equal(q, qb[0])
+    assert np.array_equal(got, exp[0])
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("v,taps,diffusion", [(130, 96, 1.0), (135, 96, 1.0), (125, 7, 1.0),
+                                              (150, 40, 0.6), (110, 96, 1.0)])
+def test_jacobi_flat_image_one_sweep_closed_form(oracle, fp64, v, taps, diffusion):
+    w, h = 40, 30
+    two = np.zeros((h, w, 4), np.float32)
+    two[..., 3] = 65535.0
+    two[:, : w // 2, :3] = 100 * 257
+    two[:, w // 2:, :3] = 180 * 257
+    tree = oracle.tree_build(two, 2, False)
+    pal = tree.data().colormap[:, 0]                     # two grays, r == g == b
+    assert sorted(pal.tolist()) == [100 * 257, 180 * 257]
+    flat = np.zeros((h, w, 4), np.float32)
+    flat[..., 3] = 65535.0
+    flat[..., :3] = v * 257
+    n = w * h + 1
+    x = float(v * 257)
+    q0 = int(np.argmin(np.abs(pal - x)))
+    r = x - pal[q0]
+    g = oracle.inverse_filter(taps, diffusion)
+    S = np.cumsum(g)[np.minimum(np.arange(n), taps)]
+    t = np.clip(pal[q0] + r * S, 0.0, 65535.0)
+    mid = 0.5 * (pal[0] + pal[1])
+    assert np.min(np.abs(t - mid)) > 4.0                 # no near-ties to round either way
+    expect = np.where(np.abs(t - pal[0]) < np.abs(t - pal[1]), 0, 1)
+    _, q = oracle.jacobi(tree, flat, 1, taps=taps, fp64=fp64, use_tree=False,
+                         diffusion=diffusion, want_q=True)
+    assert np.array_equal(q, expect)
