@@ -141,6 +141,36 @@ typedef struct {
 int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_options* opt, void* stream,
                       char* err, size_t errlen);
 
+/* ---- resumable sequential walks (continuous batching) ------------------
+ * New, with no counterpart in the reference (its RiemersmaWalkCuda walks one
+ * batch per call).  A job holds images that join while others are on the GPU:
+ * each step advances every unfinished image by a slice of visits in one launch
+ * and packs the ones that finish.  A sliced walk saves its queue and memo
+ * cache between launches, so every image is bit-exact with one uninterrupted
+ * RiemersmaWalkCpu walk, whatever it shared the device with.  Progress is
+ * deterministic (min(n, t + slice) per step), so the host knows which images
+ * finish in a step without reading the device. */
+typedef struct sa_walk_job_s* sa_walk_job;
+
+int sa_walk_job_create(const sa_seq_options* opt, sa_walk_job* out, char* err, size_t errlen);
+
+/* Adds `n` images (palettes must outlive their walks).  Stream-ordered: the
+ * inputs are gathered and the memo tables cleared on `stream`; use the step
+ * stream, or order the two yourself.  ids[i] receives each image's id. */
+int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, int64_t* ids, void* stream,
+                    char* err, size_t errlen);
+
+/* One step on `stream`: every unfinished image advances by `visits` visits
+ * (rounded up to a multiple of 48; <= 0 runs them to the end), then the images
+ * that finished are packed into their dst.  done[0..min(cap, *n_done)) gets
+ * their ids; *active the images still unfinished.  Outputs of done images are
+ * ready when `stream` reaches this point. */
+int sa_walk_job_step(sa_walk_job job, int64_t visits, void* stream, int64_t* done, int cap,
+                     int* n_done, int* active, char* err, size_t errlen);
+
+/* Frees the job (waits for the device if images are unfinished). */
+int sa_walk_job_destroy(sa_walk_job job);
+
 /* Counters of the sequential walks since the last call, then reset:
  * out[0] memo misses (full octree searches), out[1] LDS memo line fills,
  * out[2] line fills that read HBM, out[3] reserved.  The analogue of the

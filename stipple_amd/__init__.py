@@ -24,11 +24,13 @@ from ._lib import (  # noqa: F401
     EngineError,
     Palette,
     SeqOptions,
+    WalkJob,
     available,
     curve_order,
     device_name,
     image_spec,
     init,
+    inverse_filter,
     lib,
     seq_stats,
     timing_collect,

@@ -77,6 +77,12 @@ def lib():
                 "sa_walk_blocks": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_walk_approx": (i32, [C.POINTER(_Image), i32, i32, i32, i32, i32, dbl, vp, cp, sz]),
                 "sa_seq_stats": (i32, [vp, cp, sz]),
+                "sa_inverse_filter": (i32, [i32, dbl, vp, cp, sz]),
+                "sa_walk_job_create": (i32, [C.POINTER(SeqOptions), C.POINTER(vp), cp, sz]),
+                "sa_walk_job_add": (i32, [vp, C.POINTER(_Image), i32, vp, vp, cp, sz]),
+                "sa_walk_job_step": (i32, [vp, i64, vp, vp, i32, C.POINTER(i32), C.POINTER(i32),
+                                           cp, sz]),
+                "sa_walk_job_destroy": (i32, [vp]),
                 "sa_timing_enable": (i32, [i32]),
                 "sa_timing_collect": (i32, [cp, C.POINTER(dbl), C.POINTER(i32), cp, sz]),
             }
@@ -233,6 +239,65 @@ def walk_seq_batch(images, options=None, stream=None):
     e = _err()
     rc = lib().sa_walk_seq_batch(arr, len(images), C.byref(opt), _stream_ptr(stream), e, len(e))
     _check(rc, e)
+
+
+class WalkJob:
+    """A resumable batch of sequential walks (sa_walk_job_*): images join with
+    add() while others are on the device; each step() advances every unfinished
+    image by a slice of visits and packs the ones that finish.  Every image is
+    bit-exact with one uninterrupted walk."""
+
+    def __init__(self, options=None):
+        opt = options or SeqOptions()
+        h = C.c_void_p()
+        e = _err()
+        _check(lib().sa_walk_job_create(C.byref(opt), C.byref(h), e, len(e)), e)
+        self.handle = h.value
+        self.active = 0
+
+    def add(self, images, stream=None):
+        """Adds image_spec()s; returns their ids."""
+        if not images:
+            return []
+        arr = (_Image * len(images))(*images)
+        ids = np.zeros(len(images), np.int64)
+        e = _err()
+        _check(lib().sa_walk_job_add(self.handle, arr, len(images), _np_ptr(ids),
+                                     _stream_ptr(stream), e, len(e)), e)
+        self.active += len(images)
+        return [int(i) for i in ids]
+
+    def step(self, visits, stream=None):
+        """One launch; returns the ids finished by it (outputs ready when the
+        stream reaches this point)."""
+        cap = max(self.active, 1)
+        done = np.zeros(cap, np.int64)
+        nd = C.c_int()
+        act = C.c_int()
+        e = _err()
+        _check(lib().sa_walk_job_step(self.handle, int(visits), _stream_ptr(stream), _np_ptr(done),
+                                      cap, C.byref(nd), C.byref(act), e, len(e)), e)
+        self.active = act.value
+        return [int(i) for i in done[:nd.value]]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sa_walk_job_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def inverse_filter(taps, diffusion=1.0):
+    """g[0..taps] of the Jacobi engine (sa_inverse_filter)."""
+    g = np.zeros(taps + 1, np.float64)
+    e = _err()
+    _check(lib().sa_inverse_filter(taps, diffusion, _np_ptr(g), e, len(e)), e)
+    return g
 
 
 def walk_blocks(frames, block=64, diffusion=1.0, stream=None):

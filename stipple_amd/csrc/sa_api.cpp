@@ -626,9 +626,13 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
         p += (size_t)2 << K;
       }
       it.pal = im.palette->dev_hdr;
+      it.state = nullptr;
+      it.snap = nullptr;
       it.n = px;
       it.src_fmt = im.src_fmt;
       it.dst_fmt = im.dst_fmt;
+      it.memo_bits = S;
+      it.pad_ = 0;
     }
     sa::SeqItem* d_items = (sa::SeqItem*)scratch;
     int32_t* d_queue = (int32_t*)(scratch + items_bytes);
@@ -636,14 +640,259 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     HIP_TRY(hipMemsetAsync(d_queue, 0, sizeof(int32_t), s));
     const int ni = (int)items.size();
     HIP_TRY(timed("seq_gather", s, [&] { return sa::launch_seq_gather(d_items, ni, max_n, s); }));
+    if (o.use_cache) HIP_TRY(sa::launch_memo_clear(d_items, ni, K, s));
     HIP_TRY(timed("seq_walk", s, [&] {
-      return sa::launch_seq_walk(d_items, ni, d_queue, sw, S, o.use_cache, max_img, waves, assoc,
+      return sa::launch_seq_walk(d_items, ni, d_queue, sw, o.use_cache, lds, waves, assoc, 0,
                                  g_stats, s);
     }));
     HIP_TRY(timed("seq_pack", s, [&] { return sa::launch_seq_pack(d_items, ni, max_n, s); }));
     HIP_TRY(hipFreeAsync(scratch, s));
     (void)total;
   }
+  return SA_OK;
+}
+
+// ---- resumable walks: continuous batching -----------------------------------
+namespace {
+
+int memo_bits_for(const sa_seq_options& o, int img_bytes) {
+  if (o.memo_bits > 0) return o.memo_bits;
+  int S = 10;
+  while (S > 4 && (132u << S) + (size_t)img_bytes > 158 * 1024) --S;
+  return S;
+}
+
+struct JobItem {
+  int64_t id = 0;
+  sa::SeqItem it{};
+  int64_t t = 0;               // progress (every step advances it deterministically)
+  unsigned char* mem = nullptr;  // xs, idx, memo, state, snapshot
+  int assoc = 0;
+  int img_bytes = 0;
+};
+
+// Device copy of `items` in stream-ordered scratch (freed by the caller).
+int upload_items(const std::vector<sa::SeqItem>& items, hipStream_t s, unsigned char** out,
+                 char* err, size_t errlen) {
+  const size_t bytes = (sizeof(sa::SeqItem) * items.size() + 255) & ~(size_t)255;
+  HIP_TRY(hipMallocAsync((void**)out, bytes + 256, s));
+  HIP_TRY(staged_copy(*out, items.data(), sizeof(sa::SeqItem) * items.size(), s));
+  return SA_OK;
+}
+
+}  // namespace
+
+struct sa_walk_job_s {
+  sa_seq_options opt{1.0, 1, 0, 0, 0};
+  const double* sw = nullptr;
+  std::vector<JobItem> items;  // unfinished
+  int64_t next_id = 0;
+  std::mutex mu;
+};
+
+extern "C" int sa_walk_job_create(const sa_seq_options* opt, sa_walk_job* out, char* err,
+                                  size_t errlen) {
+  if (!out) return fail(err, errlen, SA_EINVAL, "null job");
+  *out = nullptr;
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  sa_walk_job j = new sa_walk_job_s;
+  if (opt) j->opt = *opt;
+  if (j->opt.memo_bits != 0 && (j->opt.memo_bits < 4 || j->opt.memo_bits > 10)) {
+    delete j;
+    return fail(err, errlen, SA_EINVAL, "memo_bits %d outside 4..10", opt->memo_bits);
+  }
+  rc = get_weights(j->opt.diffusion, nullptr, &j->sw, err, errlen);
+  if (rc) {
+    delete j;
+    return rc;
+  }
+  *out = j;
+  return SA_OK;
+}
+
+extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, int64_t* ids,
+                               void* stream, char* err, size_t errlen) {
+  if (!job) return fail(err, errlen, SA_EINVAL, "null job");
+  if (n < 0 || (n > 0 && !images)) return fail(err, errlen, SA_EINVAL, "bad image list");
+  if (n == 0) return SA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(job->mu);
+  const sa_seq_options& o = job->opt;
+  for (int i = 0; i < n; ++i) {
+    const sa_image& im = images[i];
+    if (!im.src || !im.dst || !im.palette || im.width < 1 || im.height < 1)
+      return fail(err, errlen, SA_EINVAL, "image %d: null pointer or empty geometry", i);
+    if (!valid_in_fmt(im.src_fmt) || !valid_out_fmt(im.dst_fmt))
+      return fail(err, errlen, SA_EINVAL, "image %d: bad pixel format", i);
+    if (im.dst == im.src && (im.src_fmt != im.dst_fmt))
+      return fail(err, errlen, SA_EINVAL, "image %d: in-place needs the same format", i);
+    if (im.palette->hdr.bytes > 96 * 1024)
+      return fail(err, errlen, SA_EUNSUPPORTED, "image %d: palette search image above 96 KiB", i);
+    if (im.palette->count > 65535)
+      return fail(err, errlen, SA_EUNSUPPORTED, "image %d: the walk's memo holds indices below 65535", i);
+    const int S = memo_bits_for(o, im.palette->hdr.bytes);
+    const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)im.palette->hdr.bytes;
+    if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "image %d: LDS %zu too large", i, lds);
+  }
+  std::vector<JobItem> fresh;
+  std::vector<sa::SeqItem> dev[2];
+  int64_t max_n = 0;
+  for (int i = 0; i < n; ++i) {
+    const sa_image& im = images[i];
+    JobItem J;
+    J.id = job->next_id + i;
+    J.assoc = im.palette->assoc;
+    J.img_bytes = im.palette->hdr.bytes;
+    const int K = J.assoc ? 24 : 18;
+    const int S = memo_bits_for(o, J.img_bytes);
+    const int64_t px = im.width * im.height;
+    const size_t xs_b = 32 * (size_t)px, idx_b = (2 * (size_t)px + 255) & ~(size_t)255;
+    const size_t memo_b = o.use_cache ? ((size_t)2 << K) : 0;
+    const size_t st_b = (sizeof(sa::SeqState) + 255) & ~(size_t)255;
+    const size_t snap_b = o.use_cache ? ((size_t)132 << S) : 0;
+    int rc = SA_OK;
+    const int32_t* curve = nullptr;
+    rc = get_curve(im.width, im.height, s, &curve, err, errlen);
+    if (rc == SA_OK) {
+      hipError_t e = hipMallocAsync((void**)&J.mem, xs_b + idx_b + memo_b + st_b + snap_b, s);
+      if (e != hipSuccess) rc = fail(err, errlen, SA_ENOMEM, "image %d: %s", i, hipGetErrorString(e));
+    }
+    if (rc != SA_OK) {
+      for (JobItem& f : fresh) (void)hipFreeAsync(f.mem, s);
+      return rc;
+    }
+    unsigned char* p = J.mem;
+    sa::SeqItem& it = J.it;
+    it.src = im.src;
+    it.dst = im.dst;
+    it.curve = curve;
+    it.xs = (double*)p;
+    p += xs_b;
+    it.idx = (uint16_t*)p;
+    p += idx_b;
+    it.memo = o.use_cache ? (uint16_t*)p : nullptr;
+    p += memo_b;
+    it.state = (sa::SeqState*)p;
+    p += st_b;
+    it.snap = o.use_cache ? p : nullptr;
+    it.pal = im.palette->dev_hdr;
+    it.n = px;
+    it.src_fmt = im.src_fmt;
+    it.dst_fmt = im.dst_fmt;
+    it.memo_bits = S;
+    it.pad_ = 0;
+    max_n = std::max(max_n, px);
+    dev[J.assoc].push_back(it);
+    fresh.push_back(J);
+  }
+  // Inputs in visit order, zeroed memo tables, progress 0 -- on `stream`.
+  for (int assoc = 0; assoc < 2; ++assoc) {
+    if (dev[assoc].empty()) continue;
+    unsigned char* d = nullptr;
+    int rc = upload_items(dev[assoc], s, &d, err, errlen);
+    if (rc) return rc;
+    const int ni = (int)dev[assoc].size();
+    const sa::SeqItem* di = (const sa::SeqItem*)d;
+    HIP_TRY(timed("seq_gather", s, [&] { return sa::launch_seq_gather(di, ni, max_n, s); }));
+    if (o.use_cache) HIP_TRY(sa::launch_memo_clear(di, ni, assoc ? 24 : 18, s));
+    HIP_TRY(sa::launch_seq_state_init(di, ni, s));
+    HIP_TRY(hipFreeAsync(d, s));
+  }
+  for (int i = 0; i < n; ++i) {
+    if (ids) ids[i] = fresh[i].id;
+    job->items.push_back(fresh[i]);
+  }
+  job->next_id += n;
+  return SA_OK;
+}
+
+extern "C" int sa_walk_job_step(sa_walk_job job, int64_t visits, void* stream, int64_t* done,
+                                int cap, int* n_done, int* active, char* err, size_t errlen) {
+  if (!job) return fail(err, errlen, SA_EINVAL, "null job");
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(job->mu);
+  const sa_seq_options& o = job->opt;
+  const int64_t slice = visits > 0 ? (visits + 47) / 48 * 48 : 0;
+  int nd = 0;
+  for (int assoc = 0; assoc < 2; ++assoc) {
+    std::vector<size_t> sel;
+    for (size_t k = 0; k < job->items.size(); ++k)
+      if (job->items[k].assoc == assoc) sel.push_back(k);
+    if (sel.empty()) continue;
+    // Longest remaining walk first: it sets the step's length.
+    std::stable_sort(sel.begin(), sel.end(), [&](size_t a, size_t b) {
+      return job->items[a].it.n - job->items[a].t > job->items[b].it.n - job->items[b].t;
+    });
+    std::vector<sa::SeqItem> its;
+    size_t lds = 0;
+    for (size_t k : sel) {
+      const JobItem& J = job->items[k];
+      its.push_back(J.it);
+      lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes);
+    }
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
+    int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
+    waves = std::min<int>(waves, (int)its.size());
+    unsigned char* d = nullptr;
+    int rc = upload_items(its, s, &d, err, errlen);
+    if (rc) return rc;
+    const size_t items_bytes = (sizeof(sa::SeqItem) * its.size() + 255) & ~(size_t)255;
+    int32_t* queue = (int32_t*)(d + items_bytes);
+    HIP_TRY(hipMemsetAsync(queue, 0, sizeof(int32_t), s));
+    const int ni = (int)its.size();
+    HIP_TRY(timed("seq_walk", s, [&] {
+      return sa::launch_seq_walk((const sa::SeqItem*)d, ni, queue, job->sw, o.use_cache, lds,
+                                 waves, assoc, slice, g_stats, s);
+    }));
+    HIP_TRY(hipFreeAsync(d, s));
+    // Progress is known without reading the device back.
+    std::vector<sa::SeqItem> fin;
+    int64_t max_n = 0;
+    for (size_t k : sel) {
+      JobItem& J = job->items[k];
+      J.t = slice > 0 ? std::min(J.it.n, J.t + slice) : J.it.n;
+      if (J.t == J.it.n) {
+        fin.push_back(J.it);
+        max_n = std::max(max_n, J.it.n);
+      }
+    }
+    if (!fin.empty()) {
+      unsigned char* f = nullptr;
+      rc = upload_items(fin, s, &f, err, errlen);
+      if (rc) return rc;
+      const int nf = (int)fin.size();
+      HIP_TRY(timed("seq_pack", s, [&] {
+        return sa::launch_seq_pack((const sa::SeqItem*)f, nf, max_n, s);
+      }));
+      HIP_TRY(hipFreeAsync(f, s));
+    }
+  }
+  // Retire the finished items (their scratch is freed after this step's work).
+  std::vector<JobItem> keep;
+  for (JobItem& J : job->items) {
+    if (J.t == J.it.n) {
+      if (done && nd < cap) done[nd] = J.id;
+      ++nd;
+      HIP_TRY(hipFreeAsync(J.mem, s));
+    } else {
+      keep.push_back(J);
+    }
+  }
+  job->items.swap(keep);
+  if (n_done) *n_done = nd;
+  if (active) *active = (int)job->items.size();
+  return SA_OK;
+}
+
+extern "C" int sa_walk_job_destroy(sa_walk_job job) {
+  if (!job) return SA_OK;
+  {
+    std::lock_guard<std::mutex> g(job->mu);
+    if (!job->items.empty()) (void)hipDeviceSynchronize();
+    for (JobItem& J : job->items) (void)hipFree(J.mem);
+  }
+  delete job;
   return SA_OK;
 }
 

@@ -66,6 +66,15 @@ __device__ __forceinline__ float4 load_pixel(int fmt, const void* base, int64_t 
   return o;
 }
 
+// Where a resumable walk stopped: the next visit, and the 64 lanes' in-flight
+// running sums and residuals (the rotating queue of sa_walk.hip).
+struct SeqState {
+  int64_t t;
+  int64_t pad_;
+  double acc[64];
+  double e[64];
+};
+
 // One image of the sequential engine (device array of these).
 struct SeqItem {
   const void* src;        // source pixels (raster)
@@ -75,9 +84,13 @@ struct SeqItem {
   uint16_t* idx;          // scratch: palette index per visit
   uint16_t* memo;         // scratch: 1<<K memo entries (K = 18, or 24 with alpha)
   const PaletteDev* pal;  // device palette header
+  SeqState* state;        // null: walk to the end in one launch
+  unsigned char* snap;    // with state: the LDS memo cache between launches (132 << memo_bits B)
   int64_t n;              // visits == w*h
   int32_t src_fmt;
   int32_t dst_fmt;
+  int32_t memo_bits;      // log2 of the LDS memo line slots
+  int32_t pad_;
 };
 
 // One frame of the block engine.
@@ -103,9 +116,16 @@ hipError_t launch_curve_emit(int level, int64_t w, int64_t h, const int32_t* chu
 int64_t curve_chunks(int level);
 
 hipError_t launch_seq_gather(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
+// Sets every item's state->t to 0.
+hipError_t launch_seq_state_init(const SeqItem* items, int nitems, hipStream_t s);
+// Zeroes the items' HBM memo tables (2 << key_bits bytes each).
+hipError_t launch_memo_clear(const SeqItem* items, int nitems, int key_bits, hipStream_t s);
+// `lds_bytes`: the largest (132 << memo_bits) + search image of the items.
+// `slice` > 0 advances every item by at most `slice` visits (a multiple of 48,
+// items with a state only); otherwise to the end.
 hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, const double* sw,
-                           int memo_bits, int use_cache, int lds_image_bytes, int waves,
-                           int assoc_any, uint32_t* stats, hipStream_t s);
+                           int use_cache, size_t lds_bytes, int waves, int assoc_any,
+                           int64_t slice, uint32_t* stats, hipStream_t s);
 hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
 
 hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,

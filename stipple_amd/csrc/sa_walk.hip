@@ -247,38 +247,60 @@ __device__ __forceinline__ void visit(WalkState& w, double& xq, double W, int64_
 }
 
 // Chunk c (visits 16c .. 16c+15) restarts from its register `xq`, loaded two
-// chunks earlier; afterwards `xq` is refilled for chunk c+3.
+// chunks earlier; afterwards `xq` is refilled for chunk c+3.  Returns false
+// when `stop` (the image's end, or where this launch's slice ends) is reached.
 template <bool ASSOC, bool CACHE>
 __device__ __forceinline__ bool chunk(WalkState& w, double& xq, double W, int64_t base, int64_t n,
-                                      const SeqItem& it, const Memo& m, const Search& s,
-                                      Counters& cnt) {
+                                      int64_t stop, const SeqItem& it, const Memo& m,
+                                      const Search& s, Counters& cnt) {
   const int lane = threadIdx.x;
-  const int64_t end = base + 16 < n ? base + 16 : n;
+  const int64_t end = base + 16 < stop ? base + 16 : stop;
   for (int64_t t = base; t < end; ++t) visit<ASSOC, CACHE>(w, xq, W, t, n, it.idx, m, s, cnt);
   xq = load_x(it.xs, base + 64 + (lane & 15), n, lane >> 4);
-  return end < n;
+  return end < stop;
 }
 
+// Visits [t0, stop).  t0 and stop - t0 are multiples of 48 unless stop is the
+// end, so the three chunk registers restart in their usual roles.  A walk that
+// stops early leaves its queue (the in-flight sums and residuals) in it.state
+// and flushes the indices of its last, partial 64-visit window; resuming loads
+// them back, so a sliced walk computes exactly what an uninterrupted one does.
 template <bool ASSOC, bool CACHE>
 __device__ void walk_image(const SeqItem& it, double W, const Memo& m, const Search& s,
-                           Counters& cnt) {
+                           Counters& cnt, int64_t t0, int64_t stop) {
   const int lane = threadIdx.x;
   const int j = lane & 15;
   const int ch = lane >> 4;
   const int64_t n = it.n;
   WalkState w;
-  w.e = 0.0;
-  w.obuf = 0;
-  w.acc = load_x(it.xs, j, n, ch) + 0.0;
-  double x0 = load_x(it.xs, 16 + j, n, ch);
-  double x1 = load_x(it.xs, 32 + j, n, ch);
-  double x2 = load_x(it.xs, 48 + j, n, ch);
+  if (t0 == 0) {
+    w.e = 0.0;
+    w.obuf = 0;
+    w.acc = load_x(it.xs, j, n, ch) + 0.0;
+  } else {
+    w.acc = it.state->acc[lane];
+    w.e = it.state->e[lane];
+    const int64_t v = (t0 & ~(int64_t)63) + lane;
+    w.obuf = v < t0 ? (uint32_t)it.idx[v] : 0u;
+  }
+  double x0 = load_x(it.xs, t0 + 16 + j, n, ch);
+  double x1 = load_x(it.xs, t0 + 32 + j, n, ch);
+  double x2 = load_x(it.xs, t0 + 48 + j, n, ch);
   // Three chunk registers used round-robin: no register copy ever waits on
   // an outstanding load.
-  for (int64_t base = 0;; base += 48) {
-    if (!chunk<ASSOC, CACHE>(w, x0, W, base, n, it, m, s, cnt)) break;
-    if (!chunk<ASSOC, CACHE>(w, x1, W, base + 16, n, it, m, s, cnt)) break;
-    if (!chunk<ASSOC, CACHE>(w, x2, W, base + 32, n, it, m, s, cnt)) break;
+  for (int64_t base = t0;; base += 48) {
+    if (!chunk<ASSOC, CACHE>(w, x0, W, base, n, stop, it, m, s, cnt)) break;
+    if (!chunk<ASSOC, CACHE>(w, x1, W, base + 16, n, stop, it, m, s, cnt)) break;
+    if (!chunk<ASSOC, CACHE>(w, x2, W, base + 32, n, stop, it, m, s, cnt)) break;
+  }
+  if (stop < n) {
+    const int64_t last = stop - 1;
+    if ((last & 63) != 63) {
+      const int64_t v = (last & ~(int64_t)63) + lane;
+      if (v <= last) ((gu16*)it.idx)[v] = (uint16_t)w.obuf;
+    }
+    it.state->acc[lane] = w.acc;
+    it.state->e[lane] = w.e;
   }
 }
 
@@ -286,17 +308,11 @@ template <bool ASSOC, bool CACHE>
 __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restrict__ items,
                                                          int nitems, int32_t* queue,
                                                          const double* __restrict__ sw,
-                                                         int memo_bits, uint32_t* stats) {
+                                                         int64_t slice, uint32_t* stats) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int lane = threadIdx.x;
   // Lane j adds tap 15-j of the newest residual.
   const double W = sw[15 - (lane & 15)];
-  Memo m;
-  m.bits = memo_bits;
-  const uint32_t slots = 1u << memo_bits;
-  m.tag = (uint32_t*)lds;
-  m.line = (uint16_t*)(lds + 4 * slots);
-  unsigned char* img = lds + (CACHE ? 132u * slots : 0u);
   Counters cnt;
   for (;;) {
     int q = 0;
@@ -304,13 +320,29 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     q = rfl(__shfl(q, 0, kWave));  // uniform: item fields load into SGPRs
     if (q >= nitems) break;
     const SeqItem it = items[q];
+    const int64_t t0 = it.state ? it.state->t : 0;
+    if (t0 >= it.n) continue;
+    const int64_t stop = (slice > 0 && it.state && t0 + slice < it.n) ? t0 + slice : it.n;
+    Memo m;
+    m.bits = it.memo_bits;
+    const uint32_t slots = 1u << m.bits;
+    const uint32_t memo_bytes = CACHE ? 132u * slots : 0u;
+    m.tag = (uint32_t*)lds;
+    m.line = (uint16_t*)(lds + 4 * slots);
+    unsigned char* img = lds + memo_bytes;
     const PaletteDev pd = *it.pal;
     {
       const uint4* src = (const uint4*)pd.image;
       uint4* dst = (uint4*)img;
       for (int k = lane; k < pd.bytes / 16; k += kWave) dst[k] = src[k];
-      if (CACHE)
-        for (uint32_t k = lane; k < slots; k += kWave) m.tag[k] = kEmptyTag;
+      if (CACHE) {
+        if (t0 == 0) {
+          for (uint32_t k = lane; k < slots; k += kWave) m.tag[k] = kEmptyTag;
+        } else {  // resume: the memo cache exactly as the last launch left it
+          const uint4* sn = (const uint4*)it.snap;
+          for (uint32_t k = lane; k < memo_bytes / 16; k += kWave) ((uint4*)lds)[k] = sn[k];
+        }
+      }
       __syncthreads();
     }
     Search s;
@@ -322,7 +354,15 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     s.cand = (const uint16_t*)(img + pd.off_cand);
     s.kids = pd.kids;
     m.table = it.memo;
-    walk_image<ASSOC, CACHE>(it, W, m, s, cnt);
+    walk_image<ASSOC, CACHE>(it, W, m, s, cnt, t0, stop);
+    __syncthreads();
+    if (it.state) {
+      if (CACHE && stop < it.n) {
+        uint4* sn = (uint4*)it.snap;
+        for (uint32_t k = lane; k < memo_bytes / 16; k += kWave) sn[k] = ((const uint4*)lds)[k];
+      }
+      if (lane == 0) it.state->t = stop;
+    }
     __syncthreads();
   }
   if (stats && lane == 0) {
@@ -330,6 +370,12 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     atomicAdd(&stats[1], cnt.fills);
     atomicAdd(&stats[2], cnt.loads);
   }
+}
+
+// Resumable walks start at visit 0.
+__global__ void seq_state_init_kernel(const SeqItem* __restrict__ items, int nitems) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i < nitems && items[i].state) items[i].state->t = 0;
 }
 
 // Zeroes each image's HBM memo table (a fill kernel, so the sequence can be
@@ -430,23 +476,30 @@ hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipS
   return hipGetLastError();
 }
 
-hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, const double* sw,
-                           int memo_bits, int use_cache, int lds_image_bytes, int waves,
-                           int assoc_any, uint32_t* stats, hipStream_t s) {
+hipError_t launch_seq_state_init(const SeqItem* items, int nitems, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  if (use_cache) {
-    const int64_t words = ((int64_t)2 << (assoc_any ? 24 : 18)) / 16;
-    memo_clear_kernel<<<dim3(64, (unsigned)nitems), 256, 0, s>>>(items, words);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  const size_t lds = (use_cache ? (132u << memo_bits) : 0u) + (size_t)lds_image_bytes;
+  seq_state_init_kernel<<<(unsigned)((nitems + 63) / 64), 64, 0, s>>>(items, nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_memo_clear(const SeqItem* items, int nitems, int key_bits, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  const int64_t words = ((int64_t)2 << key_bits) / 16;
+  memo_clear_kernel<<<dim3(64, (unsigned)nitems), 256, 0, s>>>(items, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, const double* sw,
+                           int use_cache, size_t lds_bytes, int waves, int assoc_any,
+                           int64_t slice, uint32_t* stats, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  const size_t lds = lds_bytes;
   const dim3 grid((unsigned)waves), block(kWave);
   auto go = [&](auto kernel) -> hipError_t {
     hipError_t e = hipFuncSetAttribute((const void*)kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    kernel<<<grid, block, lds, s>>>(items, nitems, queue, sw, memo_bits, stats);
+    kernel<<<grid, block, lds, s>>>(items, nitems, queue, sw, slice, stats);
     return hipGetLastError();
   };
   if (assoc_any) {
