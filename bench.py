@@ -7,8 +7,12 @@ from random opaque/transparent rectangles, RGB from seeded sinusoids plus
 noise; palette size log-uniform on {2..256}; associated alpha.
 
 One step = the whole hot path over the batch, inputs already in HBM:
-octree classify -> reduce -> colormap on the device for every image, then the
-batched exact walk and the pack into RGBA8 outputs.
+octree classify -> reduce -> colormap on the device for every image (8 host
+threads, one stream each, largest images first), the exact walk, and the pack
+into RGBA8 outputs.  With the default pipeline chain each image joins one
+resumable walk job as soon as its palette exists and the job advances every
+admitted image by a few milliseconds of walking per launch (continuous
+batching); the step ends when the last image is packed.
 
 Reported beside it, in the same run:
   * plain: the same work one image per launch, synchronised per image (the
@@ -82,9 +86,15 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the one-image-per-launch leg")
     ap.add_argument("--build-threads", type=int, default=8,
                     help="host threads (one stream each) classifying palettes concurrently")
+    ap.add_argument("--chain", choices=["pipeline", "batch"], default="pipeline",
+                    help="pipeline: images join a resumable walk as their palettes are "
+                         "classified (continuous batching); batch: classify all, then one "
+                         "walk launch")
+    ap.add_argument("--budget-us", type=float, default=16000.0,
+                    help="pipeline: microseconds each image walks per step")
     ap.add_argument("--head", type=int, default=0,
-                    help="launch the walk of the N largest images before the rest are "
-                         "classified (0: one walk launch after all palettes)")
+                    help="batch chain: launch the walk of the N largest images before the "
+                         "rest are classified (0: one walk launch after all palettes)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run a 256 MiB device copy (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -145,7 +155,35 @@ def main():
         return [sa.image_spec(srcs[k], dsts[k], batch[k][0].shape[1], batch[k][0].shape[0],
                               sa.PIX_RGBA8, sa.PIX_RGBA8, pals[k]) for k in ks]
 
+    from stipple_amd.serve import SeqPipeline, Still
+    pipe = SeqPipeline(build_threads=args.build_threads, budget_us=args.budget_us) \
+        if args.chain == "pipeline" else None
+    last_lat = []
+
+    def step_pipeline():
+        # Largest images are classified first; each joins the walk as soon as
+        # its palette exists, and every launch lets each admitted image walk
+        # for a time budget (continuous batching on one resumable walk job).
+        stills = [Still(srcs[k], dsts[k], batch[k][0].shape[1], batch[k][0].shape[0],
+                        sa.PIX_RGBA8, sa.PIX_RGBA8, batch[k][1], True, grays[k]) for k in order]
+        t0 = time.perf_counter()
+        for st in stills:
+            st.t_submit = t0
+            pipe.submit(st)
+        pipe.run()
+        torch.cuda.synchronize()
+        last_lat[:] = [st.t_done - st.t_submit for st in stills]
+        errs = [st.error for st in stills if st.error]
+        if errs:
+            raise RuntimeError(errs[0])
+        pals = [None] * len(batch)
+        for k, st in zip(order, stills):
+            pals[k] = st.palette
+        return pals
+
     def step():
+        if pipe is not None:
+            return step_pipeline()
         # Palettes are classified concurrently (one stream per host thread); the
         # walk is one batched launch, largest images first.
         if args.head > 0:
@@ -170,6 +208,12 @@ def main():
     torch.cuda.synchronize()
     t_batched = (time.perf_counter() - t0) / args.steps
     log(f"batched chain: {t_batched * 1e3:.1f} ms/step  {pixels / t_batched / 1e6:.1f} Mpx/s")
+    chain_lat = None
+    if last_lat:
+        chain_lat = {"p50": round(float(np.percentile(last_lat, 50)) * 1e3, 3),
+                     "p99": round(float(np.percentile(last_lat, 99)) * 1e3, 3)}
+        log(f"per-image latency in the batch: p50 {chain_lat['p50']} ms, p99 {chain_lat['p99']} ms, "
+            f"{pipe.steps} walk steps so far")
 
     # Engine alone, palettes resident, with the walk kernel timed by events.
     specs = specs_for(pals)
@@ -267,6 +311,8 @@ def main():
         "walk_only": {"value": round(pixels / t_walk / 1e6, 3), "ms": round(t_walk * 1e3, 3),
                       "kernel_ms": round(walk_ms, 3), "gather_ms": round(gather_ms, 3),
                       "pack_ms": round(pack_ms, 3)},
+        "chain": args.chain,
+        "latency_ms": chain_lat,
         "palette_build_ms": round(t_pal * 1e3, 3),
         "palette_build_serial_ms": round(t_pal_serial * 1e3, 3),
         "plain": plain,

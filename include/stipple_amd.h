@@ -168,6 +168,22 @@ int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, int64_t* ids
 int sa_walk_job_step(sa_walk_job job, int64_t visits, void* stream, int64_t* done, int cap,
                      int* n_done, int* active, char* err, size_t errlen);
 
+/* The same step with a time budget instead of a visit count: every image
+ * walks until `budget_us` microseconds after its wavefront picked it up
+ * (checked every 48 visits), so images progress at their own pace and none
+ * waits for the slowest.  Progress is read back: this synchronises `stream`. */
+int sa_walk_job_step_timed(sa_walk_job job, double budget_us, void* stream, int64_t* done,
+                           int cap, int* n_done, int* active, char* err, size_t errlen);
+
+/* The timed step with the host out of the device's way: launches this step,
+ * then waits for the previous step's progress (copied back asynchronously) and
+ * packs the images that finished in it, ordered after this step.  done[] and
+ * *active describe what is known after the previous step, so an image is
+ * reported one call after it finishes; call until *active is 0.  Outputs of
+ * done images are ready when `stream` reaches this point. */
+int sa_walk_job_step_overlapped(sa_walk_job job, double budget_us, void* stream, int64_t* done,
+                                int cap, int* n_done, int* active, char* err, size_t errlen);
+
 /* Frees the job (waits for the device if images are unfinished). */
 int sa_walk_job_destroy(sa_walk_job job);
 

@@ -82,6 +82,10 @@ def lib():
                 "sa_walk_job_add": (i32, [vp, C.POINTER(_Image), i32, vp, vp, cp, sz]),
                 "sa_walk_job_step": (i32, [vp, i64, vp, vp, i32, C.POINTER(i32), C.POINTER(i32),
                                            cp, sz]),
+                "sa_walk_job_step_timed": (i32, [vp, dbl, vp, vp, i32, C.POINTER(i32),
+                                                 C.POINTER(i32), cp, sz]),
+                "sa_walk_job_step_overlapped": (i32, [vp, dbl, vp, vp, i32, C.POINTER(i32),
+                                                      C.POINTER(i32), cp, sz]),
                 "sa_walk_job_destroy": (i32, [vp]),
                 "sa_timing_enable": (i32, [i32]),
                 "sa_timing_collect": (i32, [cp, C.POINTER(dbl), C.POINTER(i32), cp, sz]),
@@ -277,6 +281,35 @@ class WalkJob:
         e = _err()
         _check(lib().sa_walk_job_step(self.handle, int(visits), _stream_ptr(stream), _np_ptr(done),
                                       cap, C.byref(nd), C.byref(act), e, len(e)), e)
+        self.active = act.value
+        return [int(i) for i in done[:nd.value]]
+
+    def step_timed(self, budget_us, stream=None):
+        """One launch in which every image walks for about `budget_us`
+        microseconds; synchronises the stream and returns the finished ids."""
+        cap = max(self.active, 1)
+        done = np.zeros(cap, np.int64)
+        nd = C.c_int()
+        act = C.c_int()
+        e = _err()
+        _check(lib().sa_walk_job_step_timed(self.handle, float(budget_us), _stream_ptr(stream),
+                                            _np_ptr(done), cap, C.byref(nd), C.byref(act), e,
+                                            len(e)), e)
+        self.active = act.value
+        return [int(i) for i in done[:nd.value]]
+
+    def step_overlapped(self, budget_us, stream=None):
+        """Launches a timed step, then returns the ids that finished in the
+        previous one (their outputs are ready when the stream reaches this
+        point); the host waits only for the previous step."""
+        cap = max(self.active, 1)
+        done = np.zeros(cap, np.int64)
+        nd = C.c_int()
+        act = C.c_int()
+        e = _err()
+        _check(lib().sa_walk_job_step_overlapped(self.handle, float(budget_us),
+                                                 _stream_ptr(stream), _np_ptr(done), cap,
+                                                 C.byref(nd), C.byref(act), e, len(e)), e)
         self.active = act.value
         return [int(i) for i in done[:nd.value]]
 

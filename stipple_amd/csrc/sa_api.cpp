@@ -642,7 +642,7 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     HIP_TRY(timed("seq_gather", s, [&] { return sa::launch_seq_gather(d_items, ni, max_n, s); }));
     if (o.use_cache) HIP_TRY(sa::launch_memo_clear(d_items, ni, K, s));
     HIP_TRY(timed("seq_walk", s, [&] {
-      return sa::launch_seq_walk(d_items, ni, d_queue, sw, o.use_cache, lds, waves, assoc, 0,
+      return sa::launch_seq_walk(d_items, ni, d_queue, sw, o.use_cache, lds, waves, assoc, 0, 0,
                                  g_stats, s);
     }));
     HIP_TRY(timed("seq_pack", s, [&] { return sa::launch_seq_pack(d_items, ni, max_n, s); }));
@@ -688,6 +688,16 @@ struct sa_walk_job_s {
   std::vector<JobItem> items;  // unfinished
   int64_t next_id = 0;
   std::mutex mu;
+  // Overlapped steps: the progress of the step in flight, copied to pinned
+  // memory, and the ids it covered (two slots, alternating).
+  struct Flight {
+    std::vector<int64_t> ids;
+    int64_t* host = nullptr;  // pinned, `cap` entries
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool live = false;
+  } flight[2];
+  int cur = 0;
 };
 
 extern "C" int sa_walk_job_create(const sa_seq_options* opt, sa_walk_job* out, char* err,
@@ -807,14 +817,24 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
   return SA_OK;
 }
 
-extern "C" int sa_walk_job_step(sa_walk_job job, int64_t visits, void* stream, int64_t* done,
-                                int cap, int* n_done, int* active, char* err, size_t errlen) {
-  if (!job) return fail(err, errlen, SA_EINVAL, "null job");
-  hipStream_t s = (hipStream_t)stream;
-  std::lock_guard<std::mutex> g(job->mu);
+namespace {
+
+// One step of a job: a visit slice (slice > 0, progress known on the host),
+// a time budget (budget > 0 ticks, progress read back) or to the end.
+int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int64_t* done,
+             int cap, int* n_done, int* active, char* err, size_t errlen) {
   const sa_seq_options& o = job->opt;
-  const int64_t slice = visits > 0 ? (visits + 47) / 48 * 48 : 0;
   int nd = 0;
+  // Progress of overlapped steps still in flight comes first.
+  for (int k = 0; k < 2; ++k) {
+    auto& F = job->flight[(job->cur + k) & 1];
+    if (!F.live) continue;
+    HIP_TRY(hipEventSynchronize(F.ev));
+    for (size_t q = 0; q < F.ids.size(); ++q)
+      for (JobItem& J : job->items)
+        if (J.id == F.ids[q]) J.t = std::max(J.t, F.host[q]);
+    F.live = false;
+  }
   for (int assoc = 0; assoc < 2; ++assoc) {
     std::vector<size_t> sel;
     for (size_t k = 0; k < job->items.size(); ++k)
@@ -834,24 +854,34 @@ extern "C" int sa_walk_job_step(sa_walk_job job, int64_t visits, void* stream, i
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
     int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
     waves = std::min<int>(waves, (int)its.size());
-    unsigned char* d = nullptr;
-    int rc = upload_items(its, s, &d, err, errlen);
-    if (rc) return rc;
     const size_t items_bytes = (sizeof(sa::SeqItem) * its.size() + 255) & ~(size_t)255;
+    unsigned char* d = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d, items_bytes + 256 + 8 * its.size(), s));
+    HIP_TRY(staged_copy(d, its.data(), sizeof(sa::SeqItem) * its.size(), s));
     int32_t* queue = (int32_t*)(d + items_bytes);
+    int64_t* prog = (int64_t*)(d + items_bytes + 256);
     HIP_TRY(hipMemsetAsync(queue, 0, sizeof(int32_t), s));
     const int ni = (int)its.size();
+    const sa::SeqItem* di = (const sa::SeqItem*)d;
     HIP_TRY(timed("seq_walk", s, [&] {
-      return sa::launch_seq_walk((const sa::SeqItem*)d, ni, queue, job->sw, o.use_cache, lds,
-                                 waves, assoc, slice, g_stats, s);
+      return sa::launch_seq_walk(di, ni, queue, job->sw, o.use_cache, lds, waves, assoc, slice,
+                                 budget, g_stats, s);
     }));
+    std::vector<int64_t> t(its.size());
+    if (budget > 0) {
+      HIP_TRY(sa::launch_seq_progress(di, ni, prog, s));
+      HIP_TRY(hipMemcpyAsync(t.data(), prog, 8 * its.size(), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+    }
     HIP_TRY(hipFreeAsync(d, s));
-    // Progress is known without reading the device back.
     std::vector<sa::SeqItem> fin;
     int64_t max_n = 0;
-    for (size_t k : sel) {
-      JobItem& J = job->items[k];
-      J.t = slice > 0 ? std::min(J.it.n, J.t + slice) : J.it.n;
+    for (size_t q = 0; q < sel.size(); ++q) {
+      JobItem& J = job->items[sel[q]];
+      if (budget > 0)
+        J.t = t[q];
+      else
+        J.t = slice > 0 ? std::min(J.it.n, J.t + slice) : J.it.n;
       if (J.t == J.it.n) {
         fin.push_back(J.it);
         max_n = std::max(max_n, J.it.n);
@@ -859,7 +889,7 @@ extern "C" int sa_walk_job_step(sa_walk_job job, int64_t visits, void* stream, i
     }
     if (!fin.empty()) {
       unsigned char* f = nullptr;
-      rc = upload_items(fin, s, &f, err, errlen);
+      int rc = upload_items(fin, s, &f, err, errlen);
       if (rc) return rc;
       const int nf = (int)fin.size();
       HIP_TRY(timed("seq_pack", s, [&] {
@@ -885,12 +915,156 @@ extern "C" int sa_walk_job_step(sa_walk_job job, int64_t visits, void* stream, i
   return SA_OK;
 }
 
+}  // namespace
+
+extern "C" int sa_walk_job_step(sa_walk_job job, int64_t visits, void* stream, int64_t* done,
+                                int cap, int* n_done, int* active, char* err, size_t errlen) {
+  if (!job) return fail(err, errlen, SA_EINVAL, "null job");
+  std::lock_guard<std::mutex> g(job->mu);
+  const int64_t slice = visits > 0 ? (visits + 47) / 48 * 48 : 0;
+  return job_step(job, slice, 0, (hipStream_t)stream, done, cap, n_done, active, err, errlen);
+}
+
+extern "C" int sa_walk_job_step_timed(sa_walk_job job, double budget_us, void* stream,
+                                      int64_t* done, int cap, int* n_done, int* active, char* err,
+                                      size_t errlen) {
+  if (!job) return fail(err, errlen, SA_EINVAL, "null job");
+  if (!(budget_us > 0.0)) return fail(err, errlen, SA_EINVAL, "budget must be positive");
+  std::lock_guard<std::mutex> g(job->mu);
+  // s_memrealtime counts at 100 MHz.
+  const uint64_t ticks = (uint64_t)std::max(1.0, budget_us * 100.0);
+  return job_step(job, 0, ticks, (hipStream_t)stream, done, cap, n_done, active, err, errlen);
+}
+
+namespace {
+
+// Packs and retires the items whose progress reached their end; `s` orders it.
+int retire_finished(sa_walk_job job, hipStream_t s, int64_t* done, int cap, int* n_done,
+                    char* err, size_t errlen) {
+  int nd = 0;
+  for (int assoc = 0; assoc < 2; ++assoc) {
+    std::vector<sa::SeqItem> fin;
+    int64_t max_n = 0;
+    for (const JobItem& J : job->items)
+      if (J.assoc == assoc && J.t == J.it.n) {
+        fin.push_back(J.it);
+        max_n = std::max(max_n, J.it.n);
+      }
+    if (fin.empty()) continue;
+    unsigned char* f = nullptr;
+    int rc = upload_items(fin, s, &f, err, errlen);
+    if (rc) return rc;
+    const int nf = (int)fin.size();
+    HIP_TRY(timed("seq_pack", s, [&] {
+      return sa::launch_seq_pack((const sa::SeqItem*)f, nf, max_n, s);
+    }));
+    HIP_TRY(hipFreeAsync(f, s));
+  }
+  std::vector<JobItem> keep;
+  for (JobItem& J : job->items) {
+    if (J.t == J.it.n) {
+      if (done && nd < cap) done[nd] = J.id;
+      ++nd;
+      HIP_TRY(hipFreeAsync(J.mem, s));
+    } else {
+      keep.push_back(J);
+    }
+  }
+  job->items.swap(keep);
+  if (n_done) *n_done = nd;
+  return SA_OK;
+}
+
+}  // namespace
+
+extern "C" int sa_walk_job_step_overlapped(sa_walk_job job, double budget_us, void* stream,
+                                           int64_t* done, int cap, int* n_done, int* active,
+                                           char* err, size_t errlen) {
+  if (!job) return fail(err, errlen, SA_EINVAL, "null job");
+  if (!(budget_us > 0.0)) return fail(err, errlen, SA_EINVAL, "budget must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(job->mu);
+  const sa_seq_options& o = job->opt;
+  const uint64_t ticks = (uint64_t)std::max(1.0, budget_us * 100.0);
+  // 1. Launch this step over every image not known to be finished (images
+  //    that finished in the step still in flight are skipped by the kernel).
+  auto& F = job->flight[job->cur];
+  F.ids.clear();
+  if (!job->items.empty()) {
+    const size_t total = job->items.size();
+    if (F.cap < total) {
+      if (F.host) (void)hipHostFree(F.host);
+      F.host = nullptr;
+      F.cap = 0;
+      HIP_TRY(hipHostMalloc((void**)&F.host, 8 * std::max<size_t>(total, 64), hipHostMallocDefault));
+      F.cap = std::max<size_t>(total, 64);
+    }
+    if (!F.ev) HIP_TRY(hipEventCreateWithFlags(&F.ev, hipEventDisableTiming));
+    size_t off = 0;
+    for (int assoc = 0; assoc < 2; ++assoc) {
+      std::vector<sa::SeqItem> its;
+      size_t lds = 0;
+      for (const JobItem& J : job->items) {
+        if (J.assoc != assoc) continue;
+        its.push_back(J.it);
+        F.ids.push_back(J.id);
+        lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes);
+      }
+      if (its.empty()) continue;
+      const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
+      int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
+      waves = std::min<int>(waves, (int)its.size());
+      const size_t items_bytes = (sizeof(sa::SeqItem) * its.size() + 255) & ~(size_t)255;
+      unsigned char* d = nullptr;
+      HIP_TRY(hipMallocAsync((void**)&d, items_bytes + 256 + 8 * its.size(), s));
+      HIP_TRY(staged_copy(d, its.data(), sizeof(sa::SeqItem) * its.size(), s));
+      int32_t* queue = (int32_t*)(d + items_bytes);
+      int64_t* prog = (int64_t*)(d + items_bytes + 256);
+      HIP_TRY(hipMemsetAsync(queue, 0, sizeof(int32_t), s));
+      const int ni = (int)its.size();
+      const sa::SeqItem* di = (const sa::SeqItem*)d;
+      HIP_TRY(timed("seq_walk", s, [&] {
+        return sa::launch_seq_walk(di, ni, queue, job->sw, o.use_cache, lds, waves, assoc, 0,
+                                   ticks, g_stats, s);
+      }));
+      HIP_TRY(sa::launch_seq_progress(di, ni, prog, s));
+      HIP_TRY(hipMemcpyAsync(F.host + off, prog, 8 * its.size(), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipFreeAsync(d, s));
+      off += its.size();
+    }
+    HIP_TRY(hipEventRecord(F.ev, s));
+    F.live = true;
+  }
+  // 2. Collect the previous step's progress while this one runs.
+  auto& P = job->flight[job->cur ^ 1];
+  if (P.live) {
+    HIP_TRY(hipEventSynchronize(P.ev));
+    std::map<int64_t, int64_t> prog;
+    for (size_t q = 0; q < P.ids.size(); ++q) prog[P.ids[q]] = P.host[q];
+    for (JobItem& J : job->items) {
+      auto it = prog.find(J.id);
+      if (it != prog.end()) J.t = std::max(J.t, it->second);
+    }
+    P.live = false;
+  }
+  job->cur ^= 1;
+  // 3. Pack and retire what has finished (ordered after this step).
+  int rc = retire_finished(job, s, done, cap, n_done, err, errlen);
+  if (rc) return rc;
+  if (active) *active = (int)job->items.size();
+  return SA_OK;
+}
+
 extern "C" int sa_walk_job_destroy(sa_walk_job job) {
   if (!job) return SA_OK;
   {
     std::lock_guard<std::mutex> g(job->mu);
-    if (!job->items.empty()) (void)hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     for (JobItem& J : job->items) (void)hipFree(J.mem);
+    for (auto& F : job->flight) {
+      if (F.host) (void)hipHostFree(F.host);
+      if (F.ev) (void)hipEventDestroy(F.ev);
+    }
   }
   delete job;
   return SA_OK;

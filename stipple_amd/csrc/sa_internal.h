@@ -120,12 +120,16 @@ hipError_t launch_seq_gather(const SeqItem* items, int nitems, int64_t max_n, hi
 hipError_t launch_seq_state_init(const SeqItem* items, int nitems, hipStream_t s);
 // Zeroes the items' HBM memo tables (2 << key_bits bytes each).
 hipError_t launch_memo_clear(const SeqItem* items, int nitems, int key_bits, hipStream_t s);
+// state->t of every item (n for items without a state).
+hipError_t launch_seq_progress(const SeqItem* items, int nitems, int64_t* out, hipStream_t s);
 // `lds_bytes`: the largest (132 << memo_bits) + search image of the items.
-// `slice` > 0 advances every item by at most `slice` visits (a multiple of 48,
-// items with a state only); otherwise to the end.
+// Items with a state advance by at most `slice` visits when slice > 0 (a
+// multiple of 48), and stop at the first 48-visit boundary `budget`
+// s_memrealtime ticks (100 MHz) after they start when budget > 0; otherwise,
+// and items without a state, walk to the end.
 hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, const double* sw,
                            int use_cache, size_t lds_bytes, int waves, int assoc_any,
-                           int64_t slice, uint32_t* stats, hipStream_t s);
+                           int64_t slice, uint64_t budget, uint32_t* stats, hipStream_t s);
 hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
 
 hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,

@@ -226,6 +226,14 @@ __device__ __forceinline__ void pair_decode(uint32_t p, int64_t R1, int depth, i
   }
 }
 
+// Pair (r-1, l) always exists and precedes (r, l) in pair order (both loops
+// classify every run down to level depth <= 8).  When the two runs share the
+// level-l node, (r, l) cannot be its node's first pair, so it skips the insert
+// (kNoSlot) and k_node_lookup finds its slot later without atomics.  This
+// keeps the smooth regions of an image, where thousands of consecutive runs
+// share their low-level nodes, from serialising on a few hash slots.
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
 __global__ void k_node_insert(const uint32_t* pack, int64_t R1, int depth, int64_t P, int assoc,
                               uint64_t* hkeys, uint32_t* hfirst, uint32_t cap, uint32_t* pair_slot,
                               int32_t* info) {
@@ -235,6 +243,10 @@ __global__ void k_node_insert(const uint32_t* pack, int64_t R1, int depth, int64
   int l;
   pair_decode((uint32_t)p, R1, depth, &r, &l);
   const uint64_t key = node_key(pack[r], l, assoc != 0);
+  if (r > 0 && node_key(pack[r - 1], l, assoc != 0) == key) {
+    pair_slot[p] = kNoSlot;
+    return;
+  }
   uint32_t slot = hash32(key) & (cap - 1);
   for (uint32_t probe = 0;; ++probe) {
     if (probe >= cap) {  // table full: report, never spin
@@ -249,6 +261,30 @@ __global__ void k_node_insert(const uint32_t* pack, int64_t R1, int depth, int64
     slot = (slot + 1) & (cap - 1);
   }
   atomicMin(&hfirst[slot], (uint32_t)p);
+  pair_slot[p] = slot;
+}
+
+// Slots of the pairs k_node_insert skipped (their key is in the table).
+__global__ void k_node_lookup(const uint32_t* pack, int64_t R1, int depth, int64_t P, int assoc,
+                              const uint64_t* hkeys, uint32_t cap, uint32_t* pair_slot,
+                              int32_t* info) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= P || pair_slot[p] != kNoSlot) return;
+  int64_t r;
+  int l;
+  pair_decode((uint32_t)p, R1, depth, &r, &l);
+  const uint64_t key = node_key(pack[r], l, assoc != 0);
+  uint32_t slot = hash32(key) & (cap - 1);
+  uint32_t probe = 0;
+  while (hkeys[slot] != key && probe < cap) {
+    slot = (slot + 1) & (cap - 1);
+    ++probe;
+  }
+  if (probe >= cap) {  // never inserted: report
+    info[6] = 4;
+    pair_slot[p] = 0;
+    return;
+  }
   pair_slot[p] = slot;
 }
 
@@ -1364,6 +1400,8 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
   k_fill_u32<<<blocks(ncap, 256), 256, 0, s>>>(nfirst_h, ncap, kEmpty32);
   k_node_insert<<<blocks(P, 256), 256, 0, s>>>(pack, R1, depth, P, assoc, nkeys_h, nfirst_h, ncap,
                                                pair_slot, info);
+  k_node_lookup<<<blocks(P, 256), 256, 0, s>>>(pack, R1, depth, P, assoc, nkeys_h, ncap, pair_slot,
+                                               info);
   OT_TRY(hipGetLastError());
   OT_TRY(A.get(&ncount, 1));
   OT_TRY(hipMemsetAsync(ncount, 0, 4, s));

@@ -265,9 +265,11 @@ __device__ __forceinline__ bool chunk(WalkState& w, double& xq, double W, int64_
 // stops early leaves its queue (the in-flight sums and residuals) in it.state
 // and flushes the indices of its last, partial 64-visit window; resuming loads
 // them back, so a sliced walk computes exactly what an uninterrupted one does.
+// `deadline` (a s_memrealtime tick, 0: none) ends the walk at the first 48-visit
+// boundary past it.  Returns where the walk stopped.
 template <bool ASSOC, bool CACHE>
-__device__ void walk_image(const SeqItem& it, double W, const Memo& m, const Search& s,
-                           Counters& cnt, int64_t t0, int64_t stop) {
+__device__ int64_t walk_image(const SeqItem& it, double W, const Memo& m, const Search& s,
+                              Counters& cnt, int64_t t0, int64_t stop, uint64_t deadline) {
   const int lane = threadIdx.x;
   const int j = lane & 15;
   const int ch = lane >> 4;
@@ -289,6 +291,10 @@ __device__ void walk_image(const SeqItem& it, double W, const Memo& m, const Sea
   // Three chunk registers used round-robin: no register copy ever waits on
   // an outstanding load.
   for (int64_t base = t0;; base += 48) {
+    if (deadline && base > t0 && __builtin_amdgcn_s_memrealtime() >= deadline) {
+      stop = base;
+      break;
+    }
     if (!chunk<ASSOC, CACHE>(w, x0, W, base, n, stop, it, m, s, cnt)) break;
     if (!chunk<ASSOC, CACHE>(w, x1, W, base + 16, n, stop, it, m, s, cnt)) break;
     if (!chunk<ASSOC, CACHE>(w, x2, W, base + 32, n, stop, it, m, s, cnt)) break;
@@ -302,13 +308,34 @@ __device__ void walk_image(const SeqItem& it, double W, const Memo& m, const Sea
     it.state->acc[lane] = w.acc;
     it.state->e[lane] = w.e;
   }
+  return stop;
+}
+
+// Wavefront copy of `words` 16-byte words with eight loads in flight per
+// lane (the palette and memo snapshot moves at the start and end of a walk).
+__device__ __forceinline__ void copy16(uint4* dst, const uint4* src, uint32_t words) {
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t k = lane; k < words; k += 8 * kWave) {
+    uint4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t q = k + u * kWave;
+      v[u] = q < words ? src[q] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t q = k + u * kWave;
+      if (q < words) dst[q] = v[u];
+    }
+  }
 }
 
 template <bool ASSOC, bool CACHE>
 __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restrict__ items,
                                                          int nitems, int32_t* queue,
                                                          const double* __restrict__ sw,
-                                                         int64_t slice, uint32_t* stats) {
+                                                         int64_t slice, uint64_t budget,
+                                                         uint32_t* stats) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int lane = threadIdx.x;
   // Lane j adds tap 15-j of the newest residual.
@@ -322,7 +349,7 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     const SeqItem it = items[q];
     const int64_t t0 = it.state ? it.state->t : 0;
     if (t0 >= it.n) continue;
-    const int64_t stop = (slice > 0 && it.state && t0 + slice < it.n) ? t0 + slice : it.n;
+    int64_t stop = (slice > 0 && it.state && t0 + slice < it.n) ? t0 + slice : it.n;
     Memo m;
     m.bits = it.memo_bits;
     const uint32_t slots = 1u << m.bits;
@@ -332,15 +359,12 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     unsigned char* img = lds + memo_bytes;
     const PaletteDev pd = *it.pal;
     {
-      const uint4* src = (const uint4*)pd.image;
-      uint4* dst = (uint4*)img;
-      for (int k = lane; k < pd.bytes / 16; k += kWave) dst[k] = src[k];
+      copy16((uint4*)img, (const uint4*)pd.image, (uint32_t)pd.bytes / 16);
       if (CACHE) {
         if (t0 == 0) {
           for (uint32_t k = lane; k < slots; k += kWave) m.tag[k] = kEmptyTag;
         } else {  // resume: the memo cache exactly as the last launch left it
-          const uint4* sn = (const uint4*)it.snap;
-          for (uint32_t k = lane; k < memo_bytes / 16; k += kWave) ((uint4*)lds)[k] = sn[k];
+          copy16((uint4*)lds, (const uint4*)it.snap, memo_bytes / 16);
         }
       }
       __syncthreads();
@@ -354,13 +378,12 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     s.cand = (const uint16_t*)(img + pd.off_cand);
     s.kids = pd.kids;
     m.table = it.memo;
-    walk_image<ASSOC, CACHE>(it, W, m, s, cnt, t0, stop);
+    const uint64_t deadline =
+        (budget && it.state) ? __builtin_amdgcn_s_memrealtime() + budget : 0;
+    stop = walk_image<ASSOC, CACHE>(it, W, m, s, cnt, t0, stop, deadline);
     __syncthreads();
     if (it.state) {
-      if (CACHE && stop < it.n) {
-        uint4* sn = (uint4*)it.snap;
-        for (uint32_t k = lane; k < memo_bytes / 16; k += kWave) sn[k] = ((const uint4*)lds)[k];
-      }
+      if (CACHE && stop < it.n) copy16((uint4*)it.snap, (const uint4*)lds, memo_bytes / 16);
       if (lane == 0) it.state->t = stop;
     }
     __syncthreads();
@@ -370,6 +393,12 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     atomicAdd(&stats[1], cnt.fills);
     atomicAdd(&stats[2], cnt.loads);
   }
+}
+
+// Progress of resumable walks, gathered for one copy to the host.
+__global__ void seq_progress_kernel(const SeqItem* __restrict__ items, int nitems, int64_t* out) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i < nitems) out[i] = items[i].state ? items[i].state->t : items[i].n;
 }
 
 // Resumable walks start at visit 0.
@@ -476,6 +505,12 @@ hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipS
   return hipGetLastError();
 }
 
+hipError_t launch_seq_progress(const SeqItem* items, int nitems, int64_t* out, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  seq_progress_kernel<<<(unsigned)((nitems + 63) / 64), 64, 0, s>>>(items, nitems, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_seq_state_init(const SeqItem* items, int nitems, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   seq_state_init_kernel<<<(unsigned)((nitems + 63) / 64), 64, 0, s>>>(items, nitems);
@@ -491,7 +526,7 @@ hipError_t launch_memo_clear(const SeqItem* items, int nitems, int key_bits, hip
 
 hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, const double* sw,
                            int use_cache, size_t lds_bytes, int waves, int assoc_any,
-                           int64_t slice, uint32_t* stats, hipStream_t s) {
+                           int64_t slice, uint64_t budget, uint32_t* stats, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   const size_t lds = lds_bytes;
   const dim3 grid((unsigned)waves), block(kWave);
@@ -499,7 +534,7 @@ hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, con
     hipError_t e = hipFuncSetAttribute((const void*)kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    kernel<<<grid, block, lds, s>>>(items, nitems, queue, sw, slice, stats);
+    kernel<<<grid, block, lds, s>>>(items, nitems, queue, sw, slice, budget, stats);
     return hipGetLastError();
   };
   if (assoc_any) {

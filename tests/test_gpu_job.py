@@ -150,3 +150,55 @@ def test_job_output_formats(gpu, oracle):
     got = H.from_device(c["dst"], (70, 150, 4), np.uint8)
     assert np.array_equal(got, H.scale_char_np(exp))
     job.close()
+
+
+@pytest.mark.parametrize("budget_us", [20.0, 500.0])
+def test_job_timed_steps(gpu, oracle, budget_us):
+    """Time-budgeted steps: images stop wherever the clock says, at their own
+    pace; the result is still one uninterrupted walk, and so are the counters."""
+    import torch
+    say("job timed", budget_us)
+    imgs = _images(7)
+    a = [_case(gpu, oracle, img, col) for img, col in imgs]
+    b = [_case(gpu, oracle, img, col) for img, col in imgs]
+    opts = gpu.SeqOptions(memo_bits=6)
+    gpu.seq_stats()
+    gpu.walk_seq_batch([c["spec"] for c in a], opts)
+    torch.cuda.synchronize()
+    one = gpu.seq_stats()
+    job = gpu.WalkJob(opts)
+    ids = job.add([c["spec"] for c in b])
+    done, steps = [], 0
+    while job.active:
+        done += job.step_timed(budget_us)
+        steps += 1
+        assert steps < 100000
+    torch.cuda.synchronize()
+    assert sorted(done) == ids
+    assert gpu.seq_stats() == one
+    for c in b:
+        _check(oracle, c)
+    job.close()
+
+
+def test_job_overlapped_steps(gpu, oracle):
+    """Overlapped steps: launch, then collect the previous step.  Images join
+    mid-flight; each result is one uninterrupted walk."""
+    import torch
+    say("job overlapped")
+    cases = [_case(gpu, oracle, img, col) for img, col in _images(9)]
+    job = gpu.WalkJob(gpu.SeqOptions(memo_bits=5))
+    job.add([c["spec"] for c in cases[:3]])
+    pending = list(range(3, len(cases)))
+    finished, steps = set(), 0
+    while job.active or pending:
+        if pending and steps % 2 == 0:
+            job.add([cases[pending.pop(0)]["spec"]])
+        finished.update(job.step_overlapped(50.0))
+        steps += 1
+        assert steps < 100000
+    torch.cuda.synchronize()
+    assert finished == set(range(len(cases)))
+    for c in cases:
+        _check(oracle, c)
+    job.close()
