@@ -103,6 +103,19 @@ SA_HD int memo_key(bool assoc, unsigned c0, unsigned c1, unsigned c2, unsigned c
   return k;
 }
 
+// Where the walk stores memo entry `memo_key` (a bijection of it, so the
+// memo's first-touch semantics are untouched): entries are grouped in lines of
+// 64 that cover a 4x4x4 cube of the 6-bit r,g,b codes at one alpha code --
+// offset = the low two bits of r, g, b; line = the high four bits of r, g, b
+// and the alpha code.  Targets along the walk stay in a few neighbouring
+// cubes, where a line of one channel's 64 codes would be touched once or twice.
+SA_HD uint32_t memo_store_key(bool assoc, unsigned c0, unsigned c1, unsigned c2, unsigned c3) {
+  const uint32_t r = c0 >> 2, g = c1 >> 2, b = c2 >> 2;
+  uint32_t line = (r >> 2) | ((g >> 2) << 4) | ((b >> 2) << 8);
+  if (assoc) line |= (uint32_t)(c3 >> 2) << 12;
+  return (line << 6) | (r & 3u) | ((g & 3u) << 2) | ((b & 3u) << 4);
+}
+
 // src/rd_octree.cpp:59-67 ColorToQNodeId on packed channel bytes.
 SA_HD int node_id(bool assoc, unsigned c0, unsigned c1, unsigned c2, unsigned c3, int index) {
   int id = (int)((c0 >> index) & 1u);

@@ -25,9 +25,11 @@
 // Memo (quantize.c's cache, src/rd_riemersma_cpu.cpp:75-94).  First-touch
 // semantics are observable, so it is reproduced exactly: a zeroed per-image
 // table in HBM (entry = index+1, 0 = untouched) is the authority, and an LDS
-// cache of its 64-entry lines (key >> 6, direct-mapped, dirty lines written
-// back on eviction) serves the walk.  A line whose slot has never held
-// another line is known to be all zero and is filled without touching HBM.
+// cache of its 64-entry lines (direct-mapped, dirty lines written back on
+// eviction) serves the walk.  Entries are stored under memo_store_key, so a
+// line is a 4x4x4 cube of colour codes and the targets of neighbouring visits
+// share lines.  A line whose slot has never held another line is known to be
+// all zero and is filled without touching HBM.
 //
 // Palette search on a miss: the search image (sa_palette.h) is staged in LDS
 // once per image; the first three descent levels are one table read, the
@@ -129,6 +131,12 @@ __device__ __forceinline__ int lookup(const Search& s, unsigned c0, unsigned c1,
   return bidx;
 }
 
+// Global-address-space views: loads and stores through them are
+// global_load/global_store (vmcnt only), so they never hold up an LDS wait the
+// way a flat access would.
+typedef const double __attribute__((address_space(1))) gdouble;
+typedef uint16_t __attribute__((address_space(1))) gu16;
+
 // LDS cache of the HBM memo table's 64-entry lines.
 struct Memo {
   uint32_t* tag;    // per slot: bit 31 empty, bit 30 dirty, bits 0..29 line id
@@ -154,8 +162,8 @@ __device__ __forceinline__ uint32_t memo_fill(const Memo& m, uint32_t slot, uint
     // Never held a line: this line was never written, so it is all zero.
     dst[lane] = 0;
   } else {
-    if (old & kDirty) m.table[64 * (old & 0x3FFFFFFFu) + lane] = dst[lane];
-    dst[lane] = m.table[64 * line + lane];
+    if (old & kDirty) ((gu16*)m.table)[64 * (old & 0x3FFFFFFFu) + lane] = dst[lane];
+    dst[lane] = ((const gu16*)m.table)[64 * line + lane];
     loaded = 1;
   }
   if (lane == 0) m.tag[slot] = line;
@@ -167,11 +175,6 @@ struct Counters {
   uint32_t misses = 0, fills = 0, loads = 0;
 };
 
-// Global-address-space views: loads and stores through them are
-// global_load/global_store (vmcnt only), so they never hold up an LDS wait the
-// way a flat access would.
-typedef const double __attribute__((address_space(1))) gdouble;
-typedef uint16_t __attribute__((address_space(1))) gu16;
 
 // Inputs of visit v (clamped into the image, so the load never branches and
 // the wave does not wait for it until the value is used).
@@ -205,7 +208,7 @@ __device__ __forceinline__ void visit(WalkState& w, double& xq, double W, int64_
   const unsigned c3 = ASSOC ? (unsigned)rl((int)c8, 48) : 0u;
   int idx;
   if (CACHE) {
-    const uint32_t key = (uint32_t)memo_key(ASSOC, c0, c1, c2, c3);
+    const uint32_t key = memo_store_key(ASSOC, c0, c1, c2, c3);
     const uint32_t line = key >> 6;
     const uint32_t slot = memo_slot(m, line);
     const uint32_t tg = (uint32_t)rfl((int)m.tag[slot]);
@@ -280,10 +283,14 @@ __device__ int64_t walk_image(const SeqItem& it, double W, const Memo& m, const 
     w.obuf = 0;
     w.acc = load_x(it.xs, j, n, ch) + 0.0;
   } else {
-    w.acc = it.state->acc[lane];
-    w.e = it.state->e[lane];
+    w.acc = ((gdouble*)it.state->acc)[lane];
+    w.e = ((gdouble*)it.state->e)[lane];
     const int64_t v = (t0 & ~(int64_t)63) + lane;
-    w.obuf = v < t0 ? (uint32_t)it.idx[v] : 0u;
+    w.obuf = v < t0 ? (uint32_t)((const gu16*)it.idx)[v] : 0u;
+    // Settle these loads here, before the prefetches below: left pending, the
+    // compiler waits for them at the top of every visit of the first chunk,
+    // which then also drains the prefetches once per 48 visits.
+    __builtin_amdgcn_s_waitcnt(0);
   }
   double x0 = load_x(it.xs, t0 + 16 + j, n, ch);
   double x1 = load_x(it.xs, t0 + 32 + j, n, ch);
