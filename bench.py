@@ -75,6 +75,246 @@ def cpu_baseline(batch, threads_list):
     return ("reference" if use_ref else "port"), pixels, out
 
 
+def _traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc on the same workload
+    (profiles/<name>.json), or None."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return round(json.load(fh)["traffic_bytes"])
+
+
+def _ev_ms(fn, stream=None):
+    import torch
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    fn()
+    b.record(stream)
+    b.synchronize()
+    return a.elapsed_time(b)
+
+
+def _line(args, metric, value, unit, ms, dtype, config, roofline, cpu, extra=None):
+    line = {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    if extra:
+        line.update(extra)
+    if cpu:
+        line["gpu_vs_cpu_1core"] = round(value / cpu["value"], 2)
+    return line
+
+
+def run_ramp(args, sa, torch, synth, dev):
+    """640x480 RGB8 ramp + sine + noise, 16 colours: palette and exact walk of
+    one image (the reference's CPU-runnable case)."""
+    img = synth.ramp_image()
+    h, w = img.shape[:2]
+    src = torch.from_numpy(img.reshape(-1).copy()).cuda()
+    dst = torch.empty(w * h * 3, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        pal = sa.Palette.build(src, w, h, sa.PIX_RGB8, 16, False, synth.is_gray(img))
+        sa.walk_seq_batch([sa.image_spec(src, dst, w, h, sa.PIX_RGB8, sa.PIX_RGB8, pal)])
+        torch.cuda.synchronize()
+        return pal
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pal = step()
+    t = (time.perf_counter() - t0) / args.steps
+    spec = sa.image_spec(src, dst, w, h, sa.PIX_RGB8, sa.PIX_RGB8, pal)
+    sa.timing_enable(True)
+    sa.timing_collect()
+    sa.walk_seq_batch([spec])
+    torch.cuda.synchronize()
+    walk_ms, _ = sa.timing_collect("seq_walk")
+    sa.timing_enable(False)
+    px = w * h
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import Ref, Oracle
+        lib = Ref() if Ref.available() else Oracle()
+        f = synth.widen_q16(img)
+        reps = 5
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            tr = lib.tree_build(f, 16, False, False)
+            lib.walk_seq(tr, f)
+        tc = (time.perf_counter() - t1) / reps
+        cpu = {"value": round(px / tc / 1e6, 3), "unit": "Mpixel/s", "cores": 1,
+               "kind": "reference" if Ref.available() else "port",
+               "sample": f"the same image, {reps} runs: ColorTree::Build + RiemersmaWalkCpu"}
+    achieved = 6.0 * px / (walk_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 8), "traffic": None, "kernel": "seq_walk_kernel",
+            "note": "one wavefront walks one image: the serial chain's latency bounds it, not HBM"}
+    return _line(args, "Mpixel/s, one 640x480 image (palette + exact walk)", px / t / 1e6,
+                 "Mpixel/s", t * 1e3, "f64",
+                 {"workload": "640x480 RGB8 ramp/sine + N(0,8), 16-colour octree palette on the "
+                              "device, exact sequential walk, RGB8 out", "device": dev},
+                 roof, cpu, {"walk_ms": round(walk_ms, 3)})
+
+
+def run_video(args, sa, torch, synth, dev):
+    """1920x1080 RGB8 plasma frames, a 16-colour palette per frame, block walk B=64."""
+    nf = args.frames
+    params = synth.plasma_params(3)
+    t0 = time.perf_counter()
+    frames = [synth.plasma_frame(t, params=params) for t in range(nf)]
+    log(f"generated {nf} frames in {time.perf_counter() - t0:.1f}s")
+    h, w = frames[0].shape[:2]
+    srcs = [torch.from_numpy(f.reshape(-1).copy()).cuda() for f in frames]
+    dsts = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in frames]
+    torch.cuda.synchronize()
+    streams = queue.Queue()
+    for _ in range(args.build_threads):
+        streams.put(torch.cuda.Stream())
+    pool = ThreadPoolExecutor(max_workers=args.build_threads)
+
+    def build(k):
+        st = streams.get()
+        try:
+            return sa.Palette.build(srcs[k], w, h, sa.PIX_RGB8, 16, False, False, stream=st)
+        finally:
+            streams.put(st)
+
+    def step():
+        pals = list(pool.map(build, range(nf)))
+        sa.walk_blocks([sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, pals[k])
+                        for k in range(nf)], block=64)
+        torch.cuda.synchronize()
+        return pals
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pals = step()
+    t = (time.perf_counter() - t0) / args.steps
+    specs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, pals[k])
+             for k in range(nf)]
+    sa.timing_enable(True)
+    sa.timing_collect()
+    for _ in range(3):
+        sa.walk_blocks(specs, block=64)
+    torch.cuda.synchronize()
+    walk_ms, launches = sa.timing_collect("blocks_walk")
+    sa.timing_enable(False)
+    walk_ms /= max(launches, 1)
+    t0 = time.perf_counter()
+    list(pool.map(build, range(nf)))
+    torch.cuda.synchronize()
+    pal_ms = (time.perf_counter() - t0) * 1e3
+    # plain: one frame per launch, palette + walk, synchronised per frame
+    lat = []
+    for k in range(nf):
+        a = time.perf_counter()
+        p = sa.Palette.build(srcs[k], w, h, sa.PIX_RGB8, 16, False, False)
+        sa.walk_blocks([sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, p)], block=64)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - a)
+    px = w * h
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import Ref, Oracle
+        lib = Ref() if Ref.available() else Oracle()
+        f = synth.widen_q16(frames[0])
+        t1 = time.perf_counter()
+        tr = lib.tree_build(f, 16, False, False)
+        lib.walk_blocks(tr, f, 64)
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(px / tc / 1e6, 3), "unit": "Mpixel/s", "cores": 1,
+               "kind": "reference" if Ref.available() else "port",
+               "sample": "one 1920x1080 frame: ColorTree::Build + RiemersmaBlocksCpu B=64"}
+    achieved = 6.0 * px * nf / (walk_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": _traffic("blocks_c3_traffic"),
+            "kernel": "blocks_kernel",
+            "note": f"algorithmic 6 B/px (RGB8 in + out) over one {nf}-frame launch; the kernel "
+                    "is FP64-VALU bound (about 99 + 15 x candidates non-FMA FP64 ops per px)"}
+    return _line(args, "Mpixel/s and frames/s, 1920x1080 video, block engine B=64",
+                 px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f64",
+                 {"workload": f"{nf} 1920x1080 RGB8 plasma frames, a 16-colour octree palette per "
+                              "frame built on the device, block walk B=64, RGB8 out",
+                  "frames": nf, "device": dev}, roof, cpu,
+                 {"fps": round(nf / t, 2), "walk_ms": round(walk_ms, 3),
+                  "walk_fps": round(nf / (walk_ms * 1e-3), 1), "palette_build_ms": round(pal_ms, 3),
+                  "plain": {"fps": round(nf / sum(lat), 2),
+                            "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                           "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)}}})
+
+
+def run_approx4k(args, sa, torch, synth, dev):
+    """3840x2160 RGB16 radial gradients + salt and pepper, 256 colours from
+    frame 0 held on the device, Jacobi 8 sweeps, taps 96, fp32."""
+    nf = args.frames
+    t0 = time.perf_counter()
+    frames = [synth.radial_frame16(t) for t in range(nf)]
+    log(f"generated {nf} frames in {time.perf_counter() - t0:.1f}s")
+    h, w = frames[0].shape[:2]
+    srcs = [torch.from_numpy(f.reshape(-1).view(np.uint8).copy()).cuda() for f in frames]
+    dsts = [torch.empty(w * h * 6, dtype=torch.uint8, device="cuda") for _ in frames]
+    torch.cuda.synchronize()
+    pal = sa.Palette.build(srcs[0], w, h, sa.PIX_RGB16, 256, False, False)
+    specs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB16, sa.PIX_RGB16, pal)
+             for k in range(nf)]
+
+    def step():
+        sa.walk_approx(specs, iterations=8, taps=96)
+        torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    t = (time.perf_counter() - t0) / args.steps
+    sa.timing_enable(True)
+    sa.timing_collect()
+    step()
+    ap_ms, _ = sa.timing_collect("approx")
+    sa.timing_enable(False)
+    px = w * h
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import Oracle
+        o = Oracle()
+        f = synth.widen_q16(frames[0])
+        tree = o.tree_build(f, 256, False, False)
+        crop = np.ascontiguousarray(f[:540, :960])
+        t1 = time.perf_counter()
+        o.jacobi(tree, crop, 8, 96)
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(crop.shape[0] * crop.shape[1] / tc / 1e6, 3), "unit": "Mpixel/s",
+               "cores": 1, "kind": "port",
+               "sample": "a 960x540 crop of frame 0 through the CPU restatement of the Jacobi "
+                         "engine (the reference has no CPU Jacobi), 8 sweeps, taps 96"}
+    # per frame: RGB16 in (6 B) and out (6 B) + per sweep the curve-order
+    # input read (16 B) and q in/out (2 B)
+    algo = px * nf * (12 + 8 * 18)
+    achieved = algo / (ap_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": _traffic("approx_c4_traffic"),
+            "kernel": "approx (gather, 8 sweeps, scatter)",
+            "note": "algorithmic 12 B/px per frame + 18 B/px per sweep; the sweep is FP32-VALU "
+                    "bound at taps=96 (about 873 FP32 ops per px per sweep)"}
+    return _line(args, "Mpixel/s and frames/s, 3840x2160 RGB16, Jacobi 8 sweeps",
+                 px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f32",
+                 {"workload": f"{nf} 3840x2160 RGB16 radial-gradient frames with 5% salt and "
+                              "pepper, one 256-colour palette from frame 0 resident on the device, "
+                              "Jacobi 8 sweeps, taps 96, fp32, octree lookup",
+                  "frames": nf, "device": dev}, roof, cpu,
+                 {"fps": round(nf / t, 2), "approx_ms": round(ap_ms, 3),
+                  "tolerance": "0 differing indices against the CPU restatement (bit-exact)"})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +335,12 @@ def main():
     ap.add_argument("--head", type=int, default=0,
                     help="batch chain: launch the walk of the N largest images before the "
                          "rest are classified (0: one walk launch after all palettes)")
+    ap.add_argument("--workload", choices=["stills", "ramp", "video", "approx4k"],
+                    default="stills",
+                    help="stills: 64 uneven RGBA8 stills, sequential engine (the headline); "
+                         "ramp: one 640x480 image; video: 1080p frames, block engine; "
+                         "approx4k: 4K RGB16 frames, Jacobi engine")
+    ap.add_argument("--frames", type=int, default=16, help="video / approx4k: frames per step")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run a 256 MiB device copy (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -118,6 +364,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    other = {"ramp": run_ramp, "video": run_video, "approx4k": run_approx4k}.get(args.workload)
+    if other:
+        print(json.dumps(other(args, sa, torch, synth, dev)), flush=True)
+        return
     t0 = time.perf_counter()
     batch = make_batch(args.images, args.seed)
     log(f"generated {len(batch)} images in {time.perf_counter() - t0:.1f}s")

@@ -98,19 +98,25 @@ def plasma_params(seed=3):
 
 
 def plasma_frame(t, w=1920, h=1080, params=None, sigma=4.0, seed=3) -> np.ndarray:
+    """Per channel 127.5 + 127.5 * mean of 4 sinusoids sin(2 pi (fx x + fy y) + phase
+    + drift t), plus N(0, sigma); sin(a + b) is evaluated separably as
+    sin(a) cos(b) + cos(a) sin(b)."""
     p = params or plasma_params(seed)
     rng = np.random.default_rng(seed * 100003 + t)
-    x = np.arange(w, dtype=np.float64)[None, :]
-    y = np.arange(h, dtype=np.float64)[:, None]
-    planes = []
+    x = np.arange(w, dtype=np.float64)
+    y = np.arange(h, dtype=np.float64)
+    planes = np.empty((h, w, 3), np.float32)
     for c in range(3):
-        acc = np.zeros((h, w))
+        acc = np.zeros((h, w), np.float32)
         for k in range(4):
             fx, fy = p["freq"][c, k]
-            acc += np.sin(2 * math.pi * (fx * x + fy * y) + p["phase"][c, k] + p["drift"] * t)
-        planes.append(127.5 + 127.5 * acc / 4.0)
-    rgb = np.stack(planes, -1) + rng.normal(0.0, sigma, (h, w, 3))
-    return np.clip(np.round(rgb), 0, 255).astype(np.uint8)
+            a = 2 * math.pi * fx * x + p["phase"][c, k] + p["drift"] * t
+            b = 2 * math.pi * fy * y
+            acc += np.outer(np.cos(b), np.sin(a)).astype(np.float32)
+            acc += np.outer(np.sin(b), np.cos(a)).astype(np.float32)
+        planes[..., c] = 127.5 + 127.5 * acc / 4.0
+    planes += rng.standard_normal((h, w, 3), dtype=np.float32) * np.float32(sigma)
+    return np.clip(np.rint(planes), 0, 255).astype(np.uint8)
 
 
 # -- C4: RGB16 radial gradients with salt and pepper -------------------------
