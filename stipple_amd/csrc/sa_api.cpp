@@ -83,6 +83,15 @@ int ensure_init(char* err, size_t errlen) {
     HIP_TRY(hipMemset(g_stats, 0, 16));
   }
   snprintf(g_name, sizeof(g_name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName, g_cus);
+  // Every build and walk allocates its scratch stream-ordered; keep the freed
+  // memory in the pool instead of returning it to the driver at each
+  // synchronisation (the default threshold, 0, makes every palette build map
+  // its ~100 MB of pair arrays afresh).
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
   g_device = dev;
   return SA_OK;
 }
