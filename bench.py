@@ -173,20 +173,13 @@ def run_video(args, sa, torch, synth, dev):
     srcs = [torch.from_numpy(f.reshape(-1).copy()).cuda() for f in frames]
     dsts = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in frames]
     torch.cuda.synchronize()
-    streams = queue.Queue()
-    for _ in range(args.build_threads):
-        streams.put(torch.cuda.Stream())
-    pool = ThreadPoolExecutor(max_workers=args.build_threads)
 
-    def build(k):
-        st = streams.get()
-        try:
-            return sa.Palette.build(srcs[k], w, h, sa.PIX_RGB8, 16, False, False, stream=st)
-        finally:
-            streams.put(st)
+    def build():
+        # every frame's palette in one batched device build
+        return sa.Palette.build_batch([(srcs[k], w, h, sa.PIX_RGB8, 16, False) for k in range(nf)])
 
     def step():
-        pals = list(pool.map(build, range(nf)))
+        pals = build()
         sa.walk_blocks([sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, pals[k])
                         for k in range(nf)], block=64)
         torch.cuda.synchronize()
@@ -209,7 +202,7 @@ def run_video(args, sa, torch, synth, dev):
     sa.timing_enable(False)
     walk_ms /= max(launches, 1)
     t0 = time.perf_counter()
-    list(pool.map(build, range(nf)))
+    build()
     torch.cuda.synchronize()
     pal_ms = (time.perf_counter() - t0) * 1e3
     # plain: one frame per launch, palette + walk, synchronised per frame

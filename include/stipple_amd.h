@@ -87,6 +87,25 @@ int sa_palette_build(const void* src, int src_fmt, int64_t width, int64_t height
                      int assoc, int gray, int keep_tree, void* stream, sa_palette* out,
                      char* err, size_t errlen);
 
+/* One image of a batched build: the arguments of sa_palette_build. */
+typedef struct {
+  const void* src;
+  int src_fmt;
+  int64_t width, height;
+  int colors;
+  int assoc, gray, keep_tree;
+} sa_build;
+
+/* ColorTree::Build for n images at once (rd::ColorTree::Build,
+ * include/rd_octree.h:62-63, once per image): every kernel is launched once
+ * for the whole batch and `stream` is synchronised four times per batch, not
+ * per image.  out[i] is exactly the palette sa_palette_build would return
+ * for items[i], or NULL with status[i] (status may be NULL) set to its
+ * error.  Returns SA_OK when every item built, else the first failing
+ * item's code, its message in err. */
+int sa_palette_build_batch(const sa_build* items, int n, void* stream, sa_palette* out,
+                           int* status, char* err, size_t errlen);
+
 /* Palette facts: colours, association, tree depth, node count (ColorTree::
  * color_count / depth / node_count, include/rd_octree.h:65-99) and whether the
  * root's quantize_error was summed serially (1) or bounded (0). */

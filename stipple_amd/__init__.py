@@ -21,6 +21,12 @@ from ._lib import (  # noqa: F401
     PIX_RGBA16,
     PIX_RGBA8,
     PIX_RGBAF32,
+    SA_EDEVICE,
+    SA_EINVAL,
+    SA_ENOMEM,
+    SA_EREFUSED,
+    SA_EUNSUPPORTED,
+    SA_OK,
     EngineError,
     Palette,
     SeqOptions,

@@ -19,6 +19,10 @@ _PIX_CHANNELS = {PIX_RGBAF32: 4, PIX_RGB8: 3, PIX_RGBA8: 4, PIX_RGB16: 3, PIX_RG
                  PIX_INDEX8: 1, PIX_INDEX16: 1}
 
 
+# Return codes of the C-ABI (SA_* in include/stipple_amd.h).
+SA_OK, SA_EINVAL, SA_EREFUSED, SA_EDEVICE, SA_ENOMEM, SA_EUNSUPPORTED = range(6)
+
+
 class EngineError(RuntimeError):
     """An engine refused its input or failed on the device (code, message)."""
 
@@ -32,6 +36,13 @@ class _Image(C.Structure):
     _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("width", C.c_int64),
                 ("height", C.c_int64), ("src_fmt", C.c_int32), ("dst_fmt", C.c_int32),
                 ("palette", C.c_void_p)]
+
+
+class SaBuild(C.Structure):
+    """sa_build: one image of a batched palette build."""
+    _fields_ = [("src", C.c_void_p), ("src_fmt", C.c_int), ("width", C.c_int64),
+                ("height", C.c_int64), ("colors", C.c_int), ("assoc", C.c_int), ("gray", C.c_int),
+                ("keep_tree", C.c_int)]
 
 
 class SeqOptions(C.Structure):
@@ -69,6 +80,8 @@ def lib():
                 "sa_curve_order": (i32, [i64, i64, vp, vp, cp, sz]),
                 "sa_palette_build": (i32, [vp, i32, i64, i64, i32, i32, i32, i32, vp,
                                            C.POINTER(vp), cp, sz]),
+                "sa_palette_build_batch": (i32, [C.POINTER(SaBuild), i32, vp, C.POINTER(vp),
+                                                 C.POINTER(i32), cp, sz]),
                 "sa_palette_info": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                           C.POINTER(i64), C.POINTER(i32)]),
                 "sa_palette_entries": (i32, [vp, vp, cp, sz]),
@@ -147,7 +160,7 @@ class Palette:
         self.handle = h.value
         self.count = entries.shape[0]
         self.assoc = bool(assoc)
-        self.entries = entries
+        self._entries = entries
 
     @classmethod
     def build(cls, src, width, height, src_fmt, colors, assoc, gray=False, keep_tree=False,
@@ -159,13 +172,54 @@ class Palette:
                                     int(bool(gray)), int(bool(keep_tree)), _stream_ptr(stream),
                                     C.byref(h), e, len(e))
         _check(rc, e)
+        return cls._built(h.value, assoc)
+
+    @classmethod
+    def _built(cls, handle, assoc):
         self = cls.__new__(cls)
-        self.handle = h.value
+        self.handle = handle
         self.assoc = bool(assoc)
-        info = self.info()
-        self.count = info["count"]
-        self.entries = self.colormap()
+        self.count = self.info()["count"]
+        self._entries = None
         return self
+
+    @classmethod
+    def build_batch(cls, items, stream=None, strict=True):
+        """ColorTree::Build for many device images in one batched build
+        (sa_palette_build_batch).  `items` are (src, width, height, src_fmt,
+        colors, assoc[, gray[, keep_tree]]) tuples.  Returns one Palette per
+        item; with strict, raises EngineError naming the first item that
+        failed, otherwise returns (palettes with None where an item failed,
+        the SA_* status of every item)."""
+        n = len(items)
+        arr = (SaBuild * max(n, 1))()
+        keep = []
+        for k, it in enumerate(items):
+            src, w, h, fmt, colors, assoc = it[:6]
+            gray = it[6] if len(it) > 6 else False
+            kt = it[7] if len(it) > 7 else False
+            keep.append(src)
+            arr[k] = SaBuild(_dev_ptr(src), fmt, w, h, colors, int(bool(assoc)), int(bool(gray)),
+                             int(bool(kt)))
+        out = (C.c_void_p * max(n, 1))()
+        status = (C.c_int * max(n, 1))()
+        e = _err()
+        rc = lib().sa_palette_build_batch(arr, n, _stream_ptr(stream), out, status, e, len(e))
+        pals = [cls._built(out[k], items[k][5]) if out[k] else None for k in range(n)]
+        if not strict:
+            return pals, [int(status[k]) for k in range(n)]
+        if rc != 0:
+            for p in pals:
+                if p is not None:
+                    p.close()
+            _check(rc, e)
+        return pals
+
+    @property
+    def entries(self):
+        if self._entries is None:
+            self._entries = self.colormap()
+        return self._entries
 
     def info(self):
         c, a, d, ex = C.c_int(), C.c_int(), C.c_int(), C.c_int()

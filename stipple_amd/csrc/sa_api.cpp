@@ -37,6 +37,8 @@ struct sa_palette_s {
   int exact_root = 0;
   bool has_tree = false;
   sa::TreeDev tree{};
+  void* dev_block = nullptr;   // built palettes: one allocation for image, hdr, entries
+  void* tree_block = nullptr;  // built with keep_tree: the node arrays
 };
 
 namespace {
@@ -430,19 +432,55 @@ extern "C" int sa_palette_create(const double* entries, int count, int assoc,
 extern "C" int sa_palette_destroy(sa_palette p) {
   if (!p) return SA_OK;
   (void)hipDeviceSynchronize();
-  if (p->dev_image) (void)hipFree(p->dev_image);
-  if (p->dev_hdr) (void)hipFree(p->dev_hdr);
-  if (p->dev_entries) (void)hipFree(p->dev_entries);
-  if (p->has_tree) {
-    sa::TreeDev& t = p->tree;
-    for (void* q : {(void*)t.key, (void*)t.child, (void*)t.parent, (void*)t.id, (void*)t.level,
-                    (void*)t.qe, (void*)t.share, (void*)t.total, (void*)t.nu,
-                    (void*)t.color_number, (void*)t.row})
-      if (q) (void)hipFreeAsync(q, nullptr);
+  if (p->dev_block) {
+    (void)hipFree(p->dev_block);
+  } else {
+    if (p->dev_image) (void)hipFree(p->dev_image);
+    if (p->dev_hdr) (void)hipFree(p->dev_hdr);
+    if (p->dev_entries) (void)hipFree(p->dev_entries);
   }
+  if (p->tree_block) (void)hipFree(p->tree_block);
   delete p;
   return SA_OK;
 }
+
+namespace {
+// A palette around a device build's outputs.  The header is the one k_reduce
+// wrote (the same layout function gives the same offsets), so it needs no
+// read back.
+sa_palette wrap_built(const sa::OctreeOut& o, const sa::OctreeResult& r, int assoc) {
+  sa_palette p = new sa_palette_s();
+  p->dev_block = o.block;
+  p->dev_image = o.image;
+  p->dev_hdr = o.hdr;
+  p->dev_entries = o.entries;
+  p->count = r.count;
+  p->assoc = assoc ? 1 : 0;
+  p->depth = r.depth;
+  p->nodes = r.nodes;
+  p->exact_root = r.exact_root;
+  p->has_tree = o.keep_tree != 0;
+  p->tree = o.tree;
+  p->tree_block = o.tree_block;
+  const int kids = assoc ? 16 : 8;
+  const sa::PaletteLayout L = sa::palette_layout(r.count, r.rows, kids);
+  PaletteDev& h = p->hdr;
+  h.count = r.count;
+  h.assoc = p->assoc;
+  h.kids = kids;
+  h.rows = r.rows;
+  h.bytes = L.bytes;
+  h.off_ent = L.off_ent;
+  h.off_cent = L.off_cent;
+  h.off_sub = L.off_sub;
+  h.off_desc = L.off_desc;
+  h.off_child = L.off_child;
+  h.off_cand = L.off_cand;
+  h.pad_ = 0;
+  h.image = o.image;
+  return p;
+}
+}  // namespace
 
 extern "C" int sa_palette_build(const void* src, int src_fmt, int64_t w, int64_t h, int colors,
                                 int assoc, int gray, int keep_tree, void* stream, sa_palette* out,
@@ -457,30 +495,43 @@ extern "C" int sa_palette_build(const void* src, int src_fmt, int64_t w, int64_t
   sa::OctreeOut o;
   o.keep_tree = keep_tree ? 1 : 0;
   const sa::OctreeResult r = sa::build_octree(src, src_fmt, w, h, colors, assoc, gray, s, &o);
-  if (r.status != 0) {
-    if (o.image) (void)hipFree(o.image);
-    if (o.hdr) (void)hipFree(o.hdr);
-    if (o.entries) (void)hipFree(o.entries);
-    return fail(err, errlen, r.status, "%s", r.msg);
+  if (r.status != 0) return fail(err, errlen, r.status, "%s", r.msg);
+  *out = wrap_built(o, r, assoc);
+  return SA_OK;
+}
+
+extern "C" int sa_palette_build_batch(const sa_build* items, int n, void* stream, sa_palette* out,
+                                      int* status, char* err, size_t errlen) {
+  if (n < 0 || (n > 0 && (!items || !out))) return fail(err, errlen, SA_EINVAL, "bad build list");
+  for (int i = 0; i < n; ++i) out[i] = nullptr;
+  if (n == 0) return SA_OK;
+  for (int i = 0; i < n; ++i) {
+    const sa_build& b = items[i];
+    if (!b.src || b.width < 1 || b.height < 1)
+      return fail(err, errlen, SA_EINVAL, "item %d: bad image", i);
+    if (!valid_in_fmt(b.src_fmt)) return fail(err, errlen, SA_EINVAL, "item %d: bad pixel format", i);
   }
-  sa_palette p = new sa_palette_s();
-  p->dev_image = o.image;
-  p->dev_hdr = o.hdr;
-  p->dev_entries = o.entries;
-  p->count = r.count;
-  p->assoc = assoc ? 1 : 0;
-  p->depth = r.depth;
-  p->nodes = r.nodes;
-  p->exact_root = r.exact_root;
-  p->has_tree = o.keep_tree != 0;
-  p->tree = o.tree;
-  hipError_t e = hipMemcpyAsync(&p->hdr, o.hdr, sizeof(sa::PaletteDev), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
-    sa_palette_destroy(p);
-    return fail(err, errlen, SA_EDEVICE, "palette header: %s", hipGetErrorString(e));
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  std::vector<sa::OctreeReq> req(n);
+  std::vector<sa::OctreeOut> o(n);
+  std::vector<sa::OctreeResult> r(n);
+  for (int i = 0; i < n; ++i) {
+    const sa_build& b = items[i];
+    req[i] = sa::OctreeReq{b.src, b.src_fmt, b.width, b.height, b.colors, b.assoc, b.gray};
+    o[i].keep_tree = b.keep_tree ? 1 : 0;
   }
-  *out = p;
+  sa::build_octree_batch(req.data(), n, (hipStream_t)stream, o.data(), r.data());
+  int first = -1;
+  for (int i = 0; i < n; ++i) {
+    if (status) status[i] = r[i].status;
+    if (r[i].status == 0) {
+      out[i] = wrap_built(o[i], r[i], items[i].assoc);
+    } else if (first < 0) {
+      first = i;
+    }
+  }
+  if (first >= 0) return fail(err, errlen, r[first].status, "item %d: %s", first, r[first].msg);
   return SA_OK;
 }
 

@@ -32,11 +32,23 @@ struct PaletteBuild {
 
 struct OctreeOut {
   int keep_tree = 0;      // keep the node arrays (for the ColorTree drop-in)
-  TreeDev tree{};         // valid when keep_tree
+  TreeDev tree{};         // valid when keep_tree; points into tree_block
   int64_t tree_nodes = 0;
+  void* tree_block = nullptr;      // one allocation holding the node arrays
+  void* block = nullptr;           // one allocation holding image, hdr and entries;
   unsigned char* image = nullptr;  // owned by the caller after a successful build
   PaletteDev* hdr = nullptr;
   double* entries = nullptr;
+};
+
+// One image of a build request.
+struct OctreeReq {
+  const void* src;        // device pixels in `fmt`
+  int fmt;
+  int64_t w, h;
+  int maxc;               // palette size (<= 0: 65536)
+  int assoc;
+  int gray;
 };
 
 struct OctreeResult {
@@ -55,5 +67,12 @@ struct OctreeResult {
 int octree_depth(int maxc, bool assoc, bool gray);
 OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int maxc, int assoc,
                           int gray, hipStream_t s, OctreeOut* out);
+
+// Builds n palettes together, one launch per kernel for all of them (images
+// are grouped into sub-batches of at most 2^25 pixels).  Each image gets its
+// own result and outputs, exactly as build_octree would give it; the stream is
+// synchronised four times per sub-batch.
+void build_octree_batch(const OctreeReq* req, int n, hipStream_t s, OctreeOut* out,
+                        OctreeResult* res);
 
 }  // namespace sa

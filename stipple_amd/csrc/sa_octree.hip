@@ -42,6 +42,13 @@
 // reference starts pruning levels mid-image (rd_octree.cpp:366-369, 380-383).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
 #include "sa_internal.h"
 #include "sa_math.h"
 #include "sa_octree.h"
@@ -103,46 +110,189 @@ __device__ __forceinline__ void node_mid(uint64_t k, double mid[4]) {
   }
 }
 
+struct Runs {
+  const int32_t* count;
+  const double4* px;
+};
+
+struct Terms {
+  double* term;   // count * |pixel - mid| per sorted pair
+  float4* lval;   // clamped leaf channels (leaf pairs only)
+  int32_t* lcnt;  // run length per sorted pair
+};
+
+// Per-node scratch of the tree sweeps (M + 1 entries each unless noted).
+struct TreeWork {
+  int32_t* lvl;      // node ids grouped by level
+  int32_t* lvl_off;  // 10 level offsets into lvl
+  int32_t* att;      // attached to the root at the start of the sweep
+  int32_t* anc;      // lowest S ancestor-or-self, -1 when none
+  int32_t* size;     // attached subtree size
+  int32_t* gsz;      // size of the node's part of its group
+  int32_t* start;    // post-order index of the first node of the subtree
+  int32_t* gstart;   // the same, counted inside the group
+  int32_t* pre;      // pre-order index
+  int32_t* merged;   // merged by this pass (later: internal-node flag)
+  int32_t* arr;      // scan buffers, indexed by post- or pre-order
+  int32_t* arr2;
+};
+
+// ---- batches ----------------------------------------------------------------------
+// A batch of images is built phase by phase, each phase one launch per kernel
+// for every image: element-parallel kernels flatten (image, element) onto one
+// grid through a table of each image's first block, the tree-logic kernels run
+// one workgroup per image, and the batch's pair and node arrays are the
+// images' arrays laid end to end (an image's part starts at its pair_base /
+// node_base).  The host reads back between phases only the few scalars that
+// size the next phase's arrays.
+struct Frame {
+  const void* src;
+  int32_t fmt, assoc, depth, kids;
+  int32_t maxc;
+  int32_t ok;              // 0 once the image has failed: later phases skip it
+  int64_t w, h, N;
+  int32_t* info;           // 16 ints, layout in sa_octree.h
+  double* dinfo;           // [0] parallel root chain, [1] max non-root qe, [2] root qe used
+  // 1. runs
+  int32_t* flags;
+  int32_t* rid;            // exclusive scan of flags over the batch
+  int64_t R;
+  int32_t* start;
+  int32_t* count;
+  int32_t* row;
+  double4* px;
+  uint32_t* pack;
+  // 2. loop-1 extent
+  uint64_t* bkeys;
+  uint32_t* bfirst;
+  uint32_t bcap;
+  // 3. nodes
+  uint32_t ncap;
+  int64_t R1, P;
+  int64_t pair_base;       // offset of this image's pairs in the batch's pair arrays
+  uint64_t* nkeys;
+  uint32_t* nfirst;
+  uint32_t* pair_slot;
+  // 4. ordered sums and tree logic
+  int64_t M;
+  int64_t node_base;       // offset of this image's nodes in the batch's node arrays
+  int64_t entry_base;      // offset of this image's nodes in the sorted node entries
+  int32_t* slot_node;
+  TreeDev t;
+  TreeWork W;
+  int32_t* seg_b;
+  int32_t* seg_e;
+  const int32_t* snode;    // batch node ids (node_base + node) in sorted pair order
+  const uint32_t* spair;   // pair indices of this image in sorted pair order
+  Terms T;
+  double* record;
+  PaletteBuild pb;
+};
+
+// Batch-wide arrays and counters.
+struct Batch {
+  int32_t* ctl;            // [0] collected node entries, [1] [2] long segments (loops 1, 2)
+  uint32_t* ekey;          // node entries: pair_base + first pair
+  uint32_t* eslot;         // node entries: hash slot
+  const uint32_t* skey;    // the same, sorted
+  const uint32_t* sslot;
+  int32_t* pnode;          // batch node id per pair
+  uint32_t* pidx;          // pair index per pair
+  uint64_t* long_list;     // (image << 32) | node
+};
+
+struct Map {
+  const int64_t* first;    // first block of image f; first[F] = grid size
+  int32_t F;
+};
+
+struct At {
+  int32_t f;
+  int64_t i;
+};
+
+// The image of this block (the last one whose first block is <= blockIdx.x,
+// which skips images with no blocks) and the thread's element in it.
+__device__ __forceinline__ At map_at(const Map& m, int per) {
+  int32_t lo = 0, hi = m.F - 1;
+  const int64_t b = blockIdx.x;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (m.first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return {lo, (b - m.first[lo]) * per + (int64_t)threadIdx.x};
+}
+
 // ---- 1. runs ------------------------------------------------------------------
-__global__ void k_flags(const void* src, int fmt, int64_t w, int64_t n, int32_t* flags) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= n) return;
+// info zeroed, info[5] = h, the double results zeroed (one workgroup per image).
+__global__ void k_frame_init(const Frame* __restrict__ fr) {
+  const Frame& d = fr[blockIdx.x];
+  if (threadIdx.x < 16) d.info[threadIdx.x] = threadIdx.x == 5 ? (int32_t)d.h : 0;
+  if (threadIdx.x < 4) d.dinfo[threadIdx.x] = 0.0;
+}
+
+__global__ void k_flags(const Frame* __restrict__ fr, Map m) {
+  const At a = map_at(m, 256);
+  const Frame& d = fr[a.f];
+  const int64_t p = a.i;
+  if (p >= d.N) return;
   int f = 1;
-  if (p % w != 0) f = same_px(load_pixel(fmt, src, p), load_pixel(fmt, src, p - 1)) ? 0 : 1;
-  flags[p] = f;
+  if (p % d.w != 0)
+    f = same_px(load_pixel(d.fmt, d.src, p), load_pixel(d.fmt, d.src, p - 1)) ? 0 : 1;
+  d.flags[p] = f;
 }
 
-__global__ void k_run_starts(const int32_t* flags, const int32_t* rid, int64_t n, int32_t* start) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= n) return;
-  if (flags[p]) start[rid[p]] = (int32_t)p;
+// info[12] <- R.  An image's first pixel always starts a run, so its run ids
+// are the batch scan minus the scan at that pixel.
+__global__ void k_run_count(const Frame* __restrict__ fr) {
+  const Frame& d = fr[blockIdx.x];
+  if (threadIdx.x == 0) d.info[12] = d.rid[d.N - 1] + d.flags[d.N - 1] - d.rid[0];
 }
 
-__global__ void k_run_data(const void* src, int fmt, int64_t w, int64_t n, int assoc,
-                           const int32_t* start, int64_t R, int32_t* count, double4* px,
-                           uint32_t* pack, int32_t* row) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__global__ void k_run_starts(const Frame* __restrict__ fr, Map m) {
+  const At a = map_at(m, 256);
+  const Frame& d = fr[a.f];
+  const int64_t p = a.i;
+  if (p >= d.N) return;
+  if (d.flags[p]) d.start[d.rid[p] - d.rid[0]] = (int32_t)p;
+}
+
+__global__ void k_run_data(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t r = at.i;
+  const int64_t R = d.R;
   if (r >= R) return;
-  const int64_t p = start[r];
-  const int64_t e = (r + 1 < R) ? start[r + 1] : n;
-  count[r] = (int32_t)(e - p);
-  const float4 f = load_pixel(fmt, src, p);
+  const int64_t p = d.start[r];
+  const int64_t e = (r + 1 < R) ? d.start[r + 1] : d.N;
+  d.count[r] = (int32_t)(e - p);
+  const float4 f = load_pixel(d.fmt, d.src, p);
   double a[4];
-  assoc_pixel(assoc != 0, f.x, f.y, f.z, f.w, a);
-  px[r] = make_double4(a[0], a[1], a[2], a[3]);
+  assoc_pixel(d.assoc != 0, f.x, f.y, f.z, f.w, a);
+  d.px[r] = make_double4(a[0], a[1], a[2], a[3]);
   const uint32_t c0 = scale_char(clamp_pixel(a[0])), c1 = scale_char(clamp_pixel(a[1]));
   const uint32_t c2 = scale_char(clamp_pixel(a[2])), c3 = scale_char(clamp_pixel(a[3]));
-  pack[r] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
-  row[r] = (int32_t)(p / w);
+  d.pack[r] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+  d.row[r] = (int32_t)(p / d.w);
 }
 
 // ---- 2. loop-1 extent -----------------------------------------------------------
 // info[0] <- y1 (the row after which the distinct leaf count exceeds maxc, or h)
 // info[1] <- R1 (first run of loop 2, R when loop 1 covers the image)
-__global__ __launch_bounds__(1024) void k_break(const uint32_t* pack, const int32_t* row,
-                                                int64_t R, int64_t h, int assoc, int maxc,
-                                                uint64_t* hkeys, uint32_t* hfirst, uint32_t cap,
-                                                int32_t* info) {
+__global__ __launch_bounds__(1024) void k_break(const Frame* __restrict__ fr) {
+  const Frame& d = fr[blockIdx.x];
+  if (!d.ok) return;
+  const uint32_t* pack = d.pack;
+  const int32_t* row = d.row;
+  const int64_t R = d.R, h = d.h;
+  const int assoc = d.assoc, maxc = d.maxc;
+  uint64_t* hkeys = d.bkeys;
+  uint32_t* hfirst = d.bfirst;
+  const uint32_t cap = d.bcap;
+  int32_t* info = d.info;
   __shared__ int32_t s_total;
   __shared__ int32_t s_wave[16];
   __shared__ int64_t s_hit;
@@ -234,11 +384,19 @@ __device__ __forceinline__ void pair_decode(uint32_t p, int64_t R1, int depth, i
 // share their low-level nodes, from serialising on a few hash slots.
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
-__global__ void k_node_insert(const uint32_t* pack, int64_t R1, int depth, int64_t P, int assoc,
-                              uint64_t* hkeys, uint32_t* hfirst, uint32_t cap, uint32_t* pair_slot,
-                              int32_t* info) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= P) return;
+__global__ void k_node_insert(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t p = at.i;
+  if (p >= d.P) return;
+  const uint32_t* pack = d.pack;
+  const int64_t R1 = d.R1;
+  const int depth = d.depth, assoc = d.assoc;
+  uint64_t* hkeys = d.nkeys;
+  uint32_t* hfirst = d.nfirst;
+  const uint32_t cap = d.ncap;
+  uint32_t* pair_slot = d.pair_slot;
+  int32_t* info = d.info;
   int64_t r;
   int l;
   pair_decode((uint32_t)p, R1, depth, &r, &l);
@@ -265,11 +423,18 @@ __global__ void k_node_insert(const uint32_t* pack, int64_t R1, int depth, int64
 }
 
 // Slots of the pairs k_node_insert skipped (their key is in the table).
-__global__ void k_node_lookup(const uint32_t* pack, int64_t R1, int depth, int64_t P, int assoc,
-                              const uint64_t* hkeys, uint32_t cap, uint32_t* pair_slot,
-                              int32_t* info) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= P || pair_slot[p] != kNoSlot) return;
+__global__ void k_node_lookup(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t p = at.i;
+  uint32_t* pair_slot = d.pair_slot;
+  if (p >= d.P || pair_slot[p] != kNoSlot) return;
+  const uint32_t* pack = d.pack;
+  const int64_t R1 = d.R1;
+  const int depth = d.depth, assoc = d.assoc;
+  const uint64_t* hkeys = d.nkeys;
+  const uint32_t cap = d.ncap;
+  int32_t* info = d.info;
   int64_t r;
   int l;
   pair_decode((uint32_t)p, R1, depth, &r, &l);
@@ -288,55 +453,89 @@ __global__ void k_node_lookup(const uint32_t* pack, int64_t R1, int depth, int64
   pair_slot[p] = slot;
 }
 
-__global__ void k_node_collect(const uint64_t* hkeys, const uint32_t* hfirst, uint32_t cap,
-                               int32_t* ncount, uint32_t* lfirst, uint32_t* lslot) {
-  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (s >= cap) return;
-  if (hkeys[s] == kEmpty64) return;
-  const int k = atomicAdd(ncount, 1);
-  lfirst[k] = hfirst[s];
-  lslot[k] = (uint32_t)s;
+// Node entries (first pair, slot) of every image into one list; sorting it by
+// pair_base + first pair orders the batch by image, then by creation.
+__global__ void k_node_collect(const Frame* __restrict__ fr, Map m, Batch B) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t s = at.i;
+  if (s >= d.ncap) return;
+  if (d.nkeys[s] == kEmpty64) return;
+  const int k = atomicAdd(&B.ctl[0], 1);
+  atomicAdd(&d.info[13], 1);
+  B.ekey[k] = (uint32_t)d.pair_base + d.nfirst[s];
+  B.eslot[k] = (uint32_t)s;
 }
 
 // Node ids in creation order (root is 0); creation counters for the colour count.
-__global__ void k_node_assign(const uint32_t* sfirst, const uint32_t* sslot, int64_t M,
-                              const uint64_t* hkeys, int64_t R1, int depth, int32_t* slot_node,
-                              uint64_t* nkey, int32_t* info) {
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= M) return;
+__global__ void k_node_assign(const Frame* __restrict__ fr, Map m, Batch B) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t k = at.i;
+  if (k >= d.M || !d.ok) return;
+  int32_t* info = d.info;
   const int32_t node = (int32_t)(k + 1);
-  const uint32_t slot = sslot[k];
-  slot_node[slot] = node;
-  const uint64_t key = hkeys[slot];
-  nkey[node] = key;
+  const uint32_t slot = B.sslot[d.entry_base + k];
+  const uint32_t first = B.skey[d.entry_base + k] - (uint32_t)d.pair_base;
+  d.slot_node[slot] = node;
+  const uint64_t key = d.nkeys[slot];
+  d.t.key[node] = key;
   const int level = (int)(key >> 40);
-  const bool loop1 = sfirst[k] < 8ull * (uint64_t)R1;
-  if (loop1 && level == 8) atomicAdd(&info[2], 1);        // leaves counted in loop 1
-  if (!loop1 && level == depth) atomicAdd(&info[3], 1);   // level-depth nodes new in loop 2
-  if (loop1) atomicAdd(&info[4], 1);                      // nodes created by loop 1
+  const bool loop1 = first < 8ull * (uint64_t)d.R1;
+  if (loop1 && level == 8) atomicAdd(&info[2], 1);          // leaves counted in loop 1
+  if (!loop1 && level == d.depth) atomicAdd(&info[3], 1);   // level-depth nodes new in loop 2
+  if (loop1) atomicAdd(&info[4], 1);                        // nodes created by loop 1
 }
 
-__global__ void k_pair_nodes(const uint32_t* pair_slot, const int32_t* slot_node, int64_t P,
-                             int32_t* pnode, uint32_t* pidx) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= P) return;
-  pnode[p] = slot_node[pair_slot[p]];
-  pidx[p] = (uint32_t)p;
+// Sort keys: the batch node id of every pair.  The pairs of an image that
+// failed sort onto its own single node id, so they stay inside its range.
+__global__ void k_pair_nodes(const Frame* __restrict__ fr, Map m, Batch B) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t p = at.i;
+  if (p >= d.P) return;
+  const int64_t g = d.pair_base + p;
+  B.pnode[g] = (int32_t)d.node_base + (d.ok ? d.slot_node[d.pair_slot[p]] : 0);
+  B.pidx[g] = (uint32_t)p;
 }
 
-__global__ void k_segments(const int32_t* snode, int64_t P, int32_t* seg_begin, int32_t* seg_end) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= P) return;
+__global__ void k_segments(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t i = at.i, P = d.P;
+  if (i >= P || !d.ok) return;
+  const int32_t* snode = d.snode;
   const int32_t n = snode[i];
-  if (i == 0 || snode[i - 1] != n) seg_begin[n] = (int32_t)i;
-  if (i == P - 1 || snode[i + 1] != n) seg_end[n] = (int32_t)(i + 1);
+  if (i == 0 || snode[i - 1] != n) d.seg_b[n - d.node_base] = (int32_t)i;
+  if (i == P - 1 || snode[i + 1] != n) d.seg_e[n - d.node_base] = (int32_t)(i + 1);
 }
 
-__global__ void k_links(const uint64_t* nkey, int64_t M, const uint64_t* hkeys,
-                        const int32_t* slot_node, uint32_t cap, TreeDev t, int32_t* info) {
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x + 1;
-  if (k > M) return;
-  const uint64_t key = nkey[k];
+// The root of every image: key 0, its own parent, level 0, zero sums.
+__global__ void k_root_init(const Frame* __restrict__ fr) {
+  const Frame& d = fr[blockIdx.x];
+  if (!d.ok || threadIdx.x != 0) return;
+  const TreeDev& t = d.t;
+  t.key[0] = 0;
+  t.parent[0] = 0;
+  t.id[0] = 0;
+  t.level[0] = 0;
+  t.qe[0] = 0.0;
+  t.share[0] = 0.0;
+  for (int c = 0; c < 4; ++c) t.total[c] = 0.0;
+  t.nu[0] = 0;
+}
+
+__global__ void k_links(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t k = at.i + 1;
+  if (k > d.M || !d.ok) return;
+  const uint64_t* hkeys = d.nkeys;
+  const int32_t* slot_node = d.slot_node;
+  const uint32_t cap = d.ncap;
+  const TreeDev& t = d.t;
+  int32_t* info = d.info;
+  const uint64_t key = t.key[k];
   const int level = (int)(key >> 40);
   int32_t par = 0;
   if (level > 1) {
@@ -368,32 +567,25 @@ __global__ void k_links(const uint64_t* nkey, int64_t M, const uint64_t* hkeys,
 // in node order; then each chain is a plain in-order fold over contiguous
 // memory -- one thread per node for short segments, one wavefront per node for
 // long ones, staging chunks through LDS while the next chunks load.
-struct Runs {
-  const int32_t* count;
-  const double4* px;
-};
-
-struct Terms {
-  double* term;   // count * |pixel - mid| per sorted pair
-  float4* lval;   // clamped leaf channels (leaf pairs only)
-  int32_t* lcnt;  // run length per sorted pair
-};
-
 __device__ __forceinline__ bool leaf_pair(uint32_t p, int64_t R1, int depth, int l) {
   return p < 8ull * (uint64_t)R1 ? l == 8 : l == depth;
 }
 
-__global__ void k_pair_terms(const uint64_t* nkey, Runs R, const uint32_t* spair,
-                             const int32_t* snode, int64_t P, int64_t R1, int depth, int assoc,
-                             Terms T) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= P) return;
-  const uint32_t p = spair[i];
+__global__ void k_pair_terms(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& fd = fr[at.f];
+  const int64_t i = at.i;
+  if (i >= fd.P || !fd.ok) return;
+  const Runs R{fd.count, fd.px};
+  const Terms& T = fd.T;
+  const int64_t R1 = fd.R1;
+  const int depth = fd.depth, assoc = fd.assoc;
+  const uint32_t p = fd.spair[i];
   int64_t r;
   int l;
   pair_decode(p, R1, depth, &r, &l);
   double mid[4];
-  node_mid(nkey[snode[i]], mid);
+  node_mid(fd.t.key[fd.snode[i] - fd.node_base], mid);
   const double4 v = R.px[r];
   const int32_t cnt = R.count[r];
   const double e0 = kScale * (v.x - mid[0]);
@@ -436,16 +628,21 @@ constexpr int kShortSeg = 256;
 // what PruneToCubeDepth left.  Segments longer than kShortSeg are queued for
 // k_fold_long.
 template <bool L2>
-__global__ void k_fold_short(TreeDev t, int64_t M, Terms T, const uint32_t* spair,
-                             const int32_t* seg_b, const int32_t* seg_e, int64_t R1, int depth,
-                             int32_t* long_list, int32_t* long_count) {
-  const int64_t n = (int64_t)blockIdx.x * 64 + threadIdx.x + 1;
-  if (n > M) return;
-  const int32_t b = seg_b[n], e = seg_e[n];
+__global__ void k_fold_short(const Frame* __restrict__ fr, Map m, Batch B) {
+  const At at = map_at(m, 64);
+  const Frame& d = fr[at.f];
+  const int64_t n = at.i + 1;
+  if (n > d.M || !d.ok) return;
+  const TreeDev& t = d.t;
+  const Terms& T = d.T;
+  const uint32_t* spair = d.spair;
+  const int64_t R1 = d.R1;
+  const int depth = d.depth;
+  const int32_t b = d.seg_b[n], e = d.seg_e[n];
   const int level = t.level[n];
   if (L2 && level != depth) return;
   if (e - b > kShortSeg) {
-    long_list[atomicAdd(long_count, 1)] = (int32_t)n;
+    B.long_list[atomicAdd(&B.ctl[L2 ? 2 : 1], 1)] = ((uint64_t)at.f << 32) | (uint64_t)n;
     return;
   }
   const uint32_t l1 = (uint32_t)(8 * R1);
@@ -610,16 +807,18 @@ __device__ void fold_tot_wave(const Terms& T, int32_t b, int32_t e, double* s_p,
 }
 
 template <bool L2>
-__global__ __launch_bounds__(64) void k_fold_long(TreeDev t, Terms T, const uint32_t* spair,
-                                                  const int32_t* seg_b, const int32_t* seg_e,
-                                                  int64_t R1, int depth, const int32_t* long_list,
-                                                  const int32_t* long_count) {
+__global__ __launch_bounds__(64) void k_fold_long(const Frame* __restrict__ fr, Batch B) {
   __shared__ double s_buf[2 * 4 * kChunk];
-  const int32_t items = *long_count;
-  const uint32_t l1 = (uint32_t)(8 * R1);
+  const int32_t items = B.ctl[L2 ? 2 : 1];
   for (int32_t it = blockIdx.x; it < items; it += gridDim.x) {
-    const int32_t n = long_list[it];
-    const int32_t b = seg_b[n], e = seg_e[n];
+    const uint64_t ent = B.long_list[it];
+    const Frame& d = fr[(int32_t)(ent >> 32)];
+    const int32_t n = (int32_t)(ent & 0xFFFFFFFFu);
+    const TreeDev& t = d.t;
+    const Terms& T = d.T;
+    const uint32_t* spair = d.spair;
+    const uint32_t l1 = (uint32_t)(8 * d.R1);
+    const int32_t b = d.seg_b[n], e = d.seg_e[n];
     const int level = t.level[n];
     const int32_t s = loop2_start(spair, b, e, l1);
     if (!L2) {
@@ -651,14 +850,15 @@ __global__ __launch_bounds__(64) void k_fold_long(TreeDev t, Terms T, const uint
 
 // The running values of every pair, stored by pair index for the exact root
 // chain (only run when the bound cannot separate the root's error).
-__global__ void k_fold_record(int64_t M, const double* term, const uint32_t* spair,
-                              const int32_t* seg_b, const int32_t* seg_e, double* record) {
-  const int64_t n = (int64_t)blockIdx.x * 64 + threadIdx.x + 1;
-  if (n > M) return;
+__global__ void k_fold_record(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 64);
+  const Frame& d = fr[at.f];
+  const int64_t n = at.i + 1;
+  if (n > d.M || !d.ok || d.info[6] != 0 || !d.info[14]) return;
   double qe = 0.0;
-  for (int32_t i = seg_b[n]; i < seg_e[n]; ++i) {
-    qe += term[i];
-    record[spair[i]] = qe;
+  for (int32_t i = d.seg_b[n]; i < d.seg_e[n]; ++i) {
+    qe += d.T.term[i];
+    d.record[d.spair[i]] = qe;
   }
 }
 
@@ -696,22 +896,6 @@ __global__ void k_fold_record(int64_t M, const double* term, const uint32_t* spa
 //     the same post-order, so candidate k is colour k.
 constexpr int kTreeThreads = 1024;
 constexpr int kTreeWaves = kTreeThreads / 64;
-
-// Per-node scratch of the tree sweeps (M + 1 entries each unless noted).
-struct TreeWork {
-  int32_t* lvl;      // node ids grouped by level
-  int32_t* lvl_off;  // 10 level offsets into lvl
-  int32_t* att;      // attached to the root at the start of the sweep
-  int32_t* anc;      // lowest S ancestor-or-self, -1 when none
-  int32_t* size;     // attached subtree size
-  int32_t* gsz;      // size of the node's part of its group
-  int32_t* start;    // post-order index of the first node of the subtree
-  int32_t* gstart;   // the same, counted inside the group
-  int32_t* pre;      // pre-order index
-  int32_t* merged;   // merged by this pass (later: internal-node flag)
-  int32_t* arr;      // scan buffers, indexed by post- or pre-order
-  int32_t* arr2;
-};
 
 struct TreeShared {
   int32_t off[10];     // level l occupies lvl[off[l], off[l+1])
@@ -1004,7 +1188,11 @@ __device__ double select_kth(const Tree& T, TreeShared& S, int64_t L, int64_t k,
   return __longlong_as_double((long long)S.sel[0]);
 }
 
-__global__ __launch_bounds__(kTreeThreads) void k_prune_cube(Tree T, int32_t* info) {
+__global__ __launch_bounds__(kTreeThreads) void k_prune_cube(const Frame* __restrict__ fr) {
+  const Frame& d = fr[blockIdx.x];
+  if (!d.ok || d.info[6] != 0) return;
+  int32_t* info = d.info;
+  const Tree T{d.t, d.W, d.kids, d.maxc, d.depth, d.M};
   __shared__ TreeShared S;
   level_lists(T, S);
   if (threadIdx.x < 10) T.W.lvl_off[threadIdx.x] = S.off[threadIdx.x];
@@ -1028,8 +1216,14 @@ __global__ __launch_bounds__(kTreeThreads) void k_prune_cube(Tree T, int32_t* in
 
 // ReduceImageColors (rd_octree.cpp:388-404), DefineImageColormap (:142-176) and
 // the search image.  info layout in sa_octree.h.
-__global__ __launch_bounds__(kTreeThreads) void k_reduce(Tree T, int assoc, double root_qe,
-                                                         int32_t* info, PaletteBuild out) {
+__global__ __launch_bounds__(kTreeThreads) void k_reduce(const Frame* __restrict__ fr) {
+  const Frame& d = fr[blockIdx.x];
+  if (!d.ok || d.info[6] != 0) return;
+  const Tree T{d.t, d.W, d.kids, d.maxc, d.depth, d.M};
+  const int assoc = d.assoc;
+  const double root_qe = d.dinfo[2];
+  int32_t* info = d.info;
+  const PaletteBuild out = d.pb;
   __shared__ TreeShared S;
   const TreeDev& t = T.t;
   const TreeWork& W = T.W;
@@ -1173,10 +1367,12 @@ __global__ __launch_bounds__(kTreeThreads) void k_reduce(Tree T, int assoc, doub
 }
 
 // The three-level descent table of the search image (sa_palette.h).
-__global__ void k_descent(const PaletteDev* hdr, int32_t* info) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (info[6] != 0) return;
-  const PaletteDev h = *hdr;
+__global__ void k_descent(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t e = at.i;
+  if (!d.ok || d.info[6] != 0) return;
+  const PaletteDev h = *d.pb.hdr;
   if (e >= desc_entries(h.kids)) return;
   unsigned char* img = (unsigned char*)h.image;
   ((uint32_t*)(img + h.off_desc))[e] =
@@ -1185,15 +1381,26 @@ __global__ void k_descent(const PaletteDev* hdr, int32_t* info) {
 
 // Exact root chain (only when the bound cannot separate it): the running
 // values of every pair, in pair order.
-__global__ void k_root_exact(const double* record, int64_t P, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_root_exact(const Frame* __restrict__ fr) {
+  const Frame& d = fr[blockIdx.x];
+  if (threadIdx.x != 0 || !d.ok || d.info[6] != 0 || !d.info[14]) return;
   double s = 0.0;
-  for (int64_t p = 0; p < P; ++p) s += record[p];
-  *out = s;
+  for (int64_t p = 0; p < d.P; ++p) s += d.record[p];
+  d.dinfo[2] = s;
 }
 
-__global__ void k_root_bound(TreeDev t, int64_t M, double* out) {
-  // out[0] = sum of shares, out[1] = max non-root qe (one workgroup).
+// dinfo[0] = the parallel root chain (sum of shares), dinfo[1] = max non-root
+// qe, dinfo[2] = the root qe Reduce uses; info[14] = 1 when the serial chain
+// must decide it.  The approximation differs from the serial chain by at most
+// ~3 P u sum (u = 2^-53): the reference's own rounding, the running values it
+// adds, and the order of the parallel sum.  A clear separation of 1 + 8 P u sum
+// from every other node's error is demanded.
+__global__ void k_root_bound(const Frame* __restrict__ fr) {
+#pragma clang fp contract(off)
+  const Frame& d = fr[blockIdx.x];
+  if (!d.ok || d.info[6] != 0) return;
+  const TreeDev& t = d.t;
+  const int64_t M = d.M;
   __shared__ double ssum[256], smax[256];
   double s = 0.0, m = 0.0;
   for (int64_t n = 1 + threadIdx.x; n <= M; n += 256) {
@@ -1211,8 +1418,13 @@ __global__ void k_root_bound(TreeDev t, int64_t M, double* out) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = ssum[0];
-    out[1] = smax[0];
+    const double root_qe = ssum[0], mx = smax[0];
+    const double err = (double)d.P * 2.3e-16 * fabs(root_qe) * 4.0;
+    const bool exact = !(root_qe - 1.0 - err > mx + 1e-9 * mx + 2e-12) || M == 0;
+    d.dinfo[0] = root_qe;
+    d.dinfo[1] = mx;
+    d.dinfo[2] = root_qe;
+    d.info[14] = exact ? 1 : 0;
   }
 }
 
@@ -1238,8 +1450,11 @@ inline unsigned blocks(int64_t n, int per) { return (unsigned)((n + per - 1) / p
 }  // namespace
 
 // ---------------------------------------------------------------------------------
-// Host orchestration.  Allocations are stream-ordered; the four scalar reads
-// (runs, loop-1 extent, node count, root bound) synchronise the stream.
+// Host orchestration.  A batch runs in four phases, each closed by one
+// synchronisation that reads back the scalars sizing the next phase (runs per
+// image; the loop-1 extent; nodes per image; the results).  Each phase's
+// arrays are one stream-ordered allocation; descriptors go up and results come
+// back through a pinned buffer kept per host thread.
 // ---------------------------------------------------------------------------------
 int octree_depth(int maxc, bool assoc, bool gray) {  // rd_octree.cpp:255-275
   int depth = 1;
@@ -1252,360 +1467,638 @@ int octree_depth(int maxc, bool assoc, bool gray) {  // rd_octree.cpp:255-275
   return depth;
 }
 
-#define OT_TRY(expr)                                          \
-  do {                                                        \
-    hipError_t e_ = (expr);                                   \
-    if (e_ != hipSuccess) {                                   \
-      res.status = 3;                                         \
-      snprintf(res.msg, sizeof(res.msg), "%s: %s", #expr,     \
-               hipGetErrorString(e_));                        \
-      goto done;                                              \
-    }                                                         \
-  } while (0)
+namespace {
 
-struct Arena {
-  std::vector<void*> ptrs;
-  hipStream_t s;
-  template <typename T>
-  hipError_t get(T** p, int64_t n) {
-    void* q = nullptr;
-    hipError_t e = hipMallocAsync(&q, (size_t)std::max<int64_t>(n, 1) * sizeof(T), s);
-    if (e == hipSuccess) ptrs.push_back(q);
-    *p = (T*)q;
-    return e;
-  }
-  void release() {
-    for (void* q : ptrs) (void)hipFreeAsync(q, s);
-    ptrs.clear();
-  }
-};
+// Images are grouped into sub-batches of at most this many pixels (a larger
+// image is a sub-batch of its own), which bounds the scratch of one phase and
+// keeps every batch-wide index within 32 bits.
+constexpr int64_t kBatchPixels = (int64_t)1 << 25;
 
-static uint32_t pow2_at_least(int64_t v) {
+enum MapKind { kMapN, kMapR, kMapP, kMapCap, kMapM256, kMapM64, kMapDesc, kMaps };
+
+uint32_t pow2_at_least(int64_t v) {
   uint32_t c = 1;
   while ((int64_t)c < v) c <<= 1;
   return c;
 }
 
-OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int maxc_in, int assoc_in,
-                          int gray, hipStream_t s, OctreeOut* outp) {
-  OctreeResult res;
-  res.status = 0;
-  res.msg[0] = 0;
-  const bool assoc = assoc_in != 0;
-  const int64_t maxc = maxc_in <= 0 ? 65536 : std::min(maxc_in, 65536);
-  const int depth = octree_depth((int)maxc, assoc, gray != 0);
-  const int kids = assoc ? 16 : 8;
-  const int64_t N = w * h;
-  Arena A{{}, s};
-  int32_t *flags = nullptr, *rid = nullptr, *start = nullptr, *count = nullptr, *row = nullptr;
-  double4* px = nullptr;
-  uint32_t* pack = nullptr;
-  int32_t* info = nullptr;
-  int32_t hinfo[16] = {0};
-  int64_t R = 0, R1 = 0, P = 0, M = 0;
-  uint32_t bcap = 0, ncap = 0;
-  uint64_t* bkeys = nullptr;
-  uint32_t* bfirst = nullptr;
-  uint64_t* nkeys_h = nullptr;
-  uint32_t *nfirst_h = nullptr, *pair_slot = nullptr, *lfirst = nullptr, *lslot = nullptr;
-  uint32_t *sfirst = nullptr, *sslot = nullptr, *pidx = nullptr, *spair = nullptr;
-  int32_t *slot_node = nullptr, *pnode = nullptr, *snode = nullptr, *seg_b = nullptr, *seg_e = nullptr;
-  int32_t* ncount = nullptr;
-  void* cub_tmp = nullptr;
-  size_t cub_bytes = 0;
-  double* bound = nullptr;
-  double hbound[2] = {0, 0};
-  double root_qe = 0.0;
-  double *flat = nullptr, *record = nullptr;
-  Terms T{};
-  int32_t *long_list = nullptr, *long_count = nullptr;
-  TreeWork W{};
-  int long_grid = 0;
-  TreeDev t{};
-  OctreeOut& out = *outp;
-  if (N < 1) {
-    res.status = 1;
-    snprintf(res.msg, sizeof(res.msg), "empty image");
-    return res;
+int bits_for(int64_t n) {  // radix bits that hold every key in [0, n)
+  int b = 1;
+  while (b < 32 && ((int64_t)1 << b) < n) ++b;
+  return b;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// One stream-ordered allocation carved into 256-byte aligned arrays.
+struct Carve {
+  struct Req {
+    void** p;
+    size_t bytes;
+  };
+  std::vector<Req> reqs;
+  template <typename T>
+  void want(T** p, int64_t n) {
+    reqs.push_back({(void**)p, align256((size_t)std::max<int64_t>(n, 1) * sizeof(T))});
   }
-  if (N > ((int64_t)1 << 28)) {
-    res.status = 5;
-    snprintf(res.msg, sizeof(res.msg), "image above 2^28 pixels");
-    return res;
-  }
-  // 1. runs
-  OT_TRY(A.get(&flags, N));
-  OT_TRY(A.get(&rid, N));
-  OT_TRY(A.get(&info, 16));
-  OT_TRY(hipMemsetAsync(info, 0, 16 * sizeof(int32_t), s));
-  k_flags<<<blocks(N, 256), 256, 0, s>>>(src, fmt, w, N, flags);
-  OT_TRY(hipGetLastError());
-  OT_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, cub_bytes, flags, rid, (int)N, s));
-  OT_TRY(hipMallocAsync(&cub_tmp, cub_bytes, s));
-  OT_TRY(hipcub::DeviceScan::ExclusiveSum(cub_tmp, cub_bytes, flags, rid, (int)N, s));
-  OT_TRY(hipFreeAsync(cub_tmp, s));
-  cub_tmp = nullptr;
-  {
-    int32_t last[2];
-    OT_TRY(hipMemcpyAsync(&last[0], rid + N - 1, 4, hipMemcpyDeviceToHost, s));
-    OT_TRY(hipMemcpyAsync(&last[1], flags + N - 1, 4, hipMemcpyDeviceToHost, s));
-    OT_TRY(hipStreamSynchronize(s));
-    R = (int64_t)last[0] + last[1];
-  }
-  OT_TRY(A.get(&start, R));
-  OT_TRY(A.get(&count, R));
-  OT_TRY(A.get(&row, R));
-  OT_TRY(A.get(&px, R));
-  OT_TRY(A.get(&pack, R));
-  k_run_starts<<<blocks(N, 256), 256, 0, s>>>(flags, rid, N, start);
-  k_run_data<<<blocks(R, 256), 256, 0, s>>>(src, fmt, w, N, assoc, start, R, count, px, pack, row);
-  OT_TRY(hipGetLastError());
-  // 2. loop-1 extent
-  bcap = pow2_at_least(2 * (maxc + 1 + 1024));
-  OT_TRY(A.get(&bkeys, bcap));
-  OT_TRY(A.get(&bfirst, bcap));
-  k_fill_u64<<<blocks(bcap, 256), 256, 0, s>>>(bkeys, bcap, kEmpty64);
-  k_fill_u32<<<blocks(bcap, 256), 256, 0, s>>>(bfirst, bcap, kEmpty32);
-  k_break<<<1, 1024, 0, s>>>(pack, row, R, h, assoc, (int)maxc, bkeys, bfirst, bcap, info);
-  OT_TRY(hipGetLastError());
-  OT_TRY(hipMemcpyAsync(hinfo, info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  OT_TRY(hipStreamSynchronize(s));
-  if (hinfo[6] == 4) {
-    res.status = 3;
-    snprintf(res.msg, sizeof(res.msg), "internal: leaf hash table overflow");
-    goto done;
-  }
-  R1 = hinfo[1];
-  P = 8 * R1 + (int64_t)depth * (R - R1);
-  if (P >= ((int64_t)1 << 32) - 1) {
-    res.status = 5;
-    snprintf(res.msg, sizeof(res.msg), "too many runs");
-    goto done;
-  }
-  // 3. nodes
-  {
-    int64_t bound_nodes = 8 * (maxc + 1 + w) + 1;
-    int64_t pw = 1;
-    for (int l = 1; l <= depth; ++l) {
-      pw = std::min<int64_t>(pw * kids, (int64_t)1 << 40);
-      bound_nodes += std::min<int64_t>(R, pw);
+  hipError_t take(std::vector<void*>& owned, hipStream_t s) {
+    size_t total = 0;
+    for (const Req& r : reqs) total += r.bytes;
+    void* base = nullptr;
+    const hipError_t e = hipMallocAsync(&base, total, s);
+    if (e != hipSuccess) return e;
+    owned.push_back(base);
+    size_t off = 0;
+    for (const Req& r : reqs) {
+      *r.p = (char*)base + off;
+      off += r.bytes;
     }
-    bound_nodes = std::min<int64_t>(bound_nodes, P + 1);
-    ncap = pow2_at_least(2 * bound_nodes + 64);
+    reqs.clear();
+    return hipSuccess;
   }
-  OT_TRY(A.get(&nkeys_h, ncap));
-  OT_TRY(A.get(&nfirst_h, ncap));
-  OT_TRY(A.get(&pair_slot, P));
-  k_fill_u64<<<blocks(ncap, 256), 256, 0, s>>>(nkeys_h, ncap, kEmpty64);
-  k_fill_u32<<<blocks(ncap, 256), 256, 0, s>>>(nfirst_h, ncap, kEmpty32);
-  k_node_insert<<<blocks(P, 256), 256, 0, s>>>(pack, R1, depth, P, assoc, nkeys_h, nfirst_h, ncap,
-                                               pair_slot, info);
-  k_node_lookup<<<blocks(P, 256), 256, 0, s>>>(pack, R1, depth, P, assoc, nkeys_h, ncap, pair_slot,
-                                               info);
-  OT_TRY(hipGetLastError());
-  OT_TRY(A.get(&ncount, 1));
-  OT_TRY(hipMemsetAsync(ncount, 0, 4, s));
-  OT_TRY(A.get(&lfirst, ncap));
-  OT_TRY(A.get(&lslot, ncap));
-  k_node_collect<<<blocks(ncap, 256), 256, 0, s>>>(nkeys_h, nfirst_h, ncap, ncount, lfirst, lslot);
-  OT_TRY(hipGetLastError());
-  {
-    int32_t m = 0;
-    OT_TRY(hipMemcpyAsync(&m, ncount, 4, hipMemcpyDeviceToHost, s));
-    OT_TRY(hipStreamSynchronize(s));
-    M = m;
+};
+
+// Pinned staging per host thread (grown on demand, kept for the process).
+struct Staging {
+  void* p = nullptr;
+  size_t n = 0;
+};
+thread_local Staging tl_staging;
+
+hipError_t staging(size_t bytes, char** out) {
+  Staging& st = tl_staging;
+  if (st.n < bytes) {
+    if (st.p) (void)hipHostFree(st.p);
+    st.p = nullptr;
+    st.n = 0;
+    const size_t want = std::max<size_t>(bytes, 64 << 10);
+    const hipError_t e = hipHostMalloc(&st.p, want, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    st.n = want;
   }
-  OT_TRY(A.get(&sfirst, M));
-  OT_TRY(A.get(&sslot, M));
-  cub_bytes = 0;
-  OT_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, cub_bytes, lfirst, sfirst, lslot, sslot, (int)M,
-                                            0, 32, s));
-  OT_TRY(hipMallocAsync(&cub_tmp, cub_bytes, s));
-  OT_TRY(hipcub::DeviceRadixSort::SortPairs(cub_tmp, cub_bytes, lfirst, sfirst, lslot, sslot, (int)M,
-                                            0, 32, s));
-  OT_TRY(hipFreeAsync(cub_tmp, s));
-  cub_tmp = nullptr;
-  // Node arrays (index 0 is the root).
-  OT_TRY(A.get(&t.key, M + 1));
-  OT_TRY(A.get(&t.child, 16 * (M + 1)));
-  OT_TRY(A.get(&t.parent, M + 1));
-  OT_TRY(A.get(&t.id, M + 1));
-  OT_TRY(A.get(&t.level, M + 1));
-  OT_TRY(A.get(&t.qe, M + 1));
-  OT_TRY(A.get(&t.share, M + 1));
-  OT_TRY(A.get(&t.total, 4 * (M + 1)));
-  OT_TRY(A.get(&t.nu, M + 1));
-  OT_TRY(A.get(&t.color_number, M + 1));
-  OT_TRY(A.get(&t.row, M + 1));
-  OT_TRY(A.get(&slot_node, ncap));
-  k_fill_i32<<<blocks(16 * (M + 1), 256), 256, 0, s>>>(t.child, 16 * (M + 1), -1);
-  k_fill_u32<<<blocks(M + 1, 256), 256, 0, s>>>(t.color_number, M + 1, 0u);
-  k_node_assign<<<blocks(M, 256), 256, 0, s>>>(sfirst, sslot, M, nkeys_h, R1, depth, slot_node,
-                                               t.key, info);
-  OT_TRY(hipGetLastError());
-  {
-    // Root: key 0, its own parent, level 0.
-    const uint64_t zero = 0;
-    const int32_t z = 0;
-    const double dz = 0.0;
-    OT_TRY(hipMemcpyAsync(t.key, &zero, 8, hipMemcpyHostToDevice, s));
-    OT_TRY(hipMemcpyAsync(t.parent, &z, 4, hipMemcpyHostToDevice, s));
-    OT_TRY(hipMemcpyAsync(t.id, &z, 4, hipMemcpyHostToDevice, s));
-    OT_TRY(hipMemcpyAsync(t.level, &z, 4, hipMemcpyHostToDevice, s));
-    OT_TRY(hipMemcpyAsync(t.qe, &dz, 8, hipMemcpyHostToDevice, s));
-    OT_TRY(hipMemcpyAsync(t.share, &dz, 8, hipMemcpyHostToDevice, s));
-    OT_TRY(hipMemsetAsync(t.total, 0, 32, s));
-    OT_TRY(hipMemsetAsync(t.nu, 0, 8, s));
+  *out = (char*)st.p;
+  return hipSuccess;
+}
+
+class SubBuild {
+ public:
+  SubBuild(const OctreeReq* rq, int F, hipStream_t s, OctreeOut* outs, OctreeResult* res)
+      : rq_(rq), F_(F), s_(s), out_(outs), res_(res), H_(F) {}
+
+  void run() {
+    hipError_t e = hipSuccess;
+    const char* where = "";
+    if ((e = phase_runs()) != hipSuccess) where = "runs";
+    if (e == hipSuccess && (e = phase_extent()) != hipSuccess) where = "loop-1 extent";
+    if (e == hipSuccess && (e = phase_nodes()) != hipSuccess) where = "nodes";
+    if (e == hipSuccess && (e = phase_tree()) != hipSuccess) where = "tree";
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(s_);
+      for (int f = 0; f < F_; ++f)
+        if (res_[f].status == 0) fail(f, 3, "octree build (%s): %s", where, hipGetErrorString(e));
+    }
+    for (int f = 0; f < F_; ++f)
+      if (res_[f].status != 0) drop_outputs(f);
+    for (void* q : owned_) (void)hipFreeAsync(q, s_);
+    owned_.clear();
   }
-  // 4. ordered sums
-  OT_TRY(A.get(&pnode, P));
-  OT_TRY(A.get(&pidx, P));
-  OT_TRY(A.get(&snode, P));
-  OT_TRY(A.get(&spair, P));
-  k_pair_nodes<<<blocks(P, 256), 256, 0, s>>>(pair_slot, slot_node, P, pnode, pidx);
-  OT_TRY(hipGetLastError());
-  {
-    int end_bit = 1;
-    while (end_bit < 32 && ((int64_t)1 << end_bit) <= M) ++end_bit;
-    cub_bytes = 0;
-    OT_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, cub_bytes, pnode, snode, pidx, spair, (int)P,
-                                              0, end_bit, s));
-    OT_TRY(hipMallocAsync(&cub_tmp, cub_bytes, s));
-    OT_TRY(hipcub::DeviceRadixSort::SortPairs(cub_tmp, cub_bytes, pnode, snode, pidx, spair, (int)P,
-                                              0, end_bit, s));
-    OT_TRY(hipFreeAsync(cub_tmp, s));
-    cub_tmp = nullptr;
+
+ private:
+  const OctreeReq* rq_;
+  const int F_;
+  hipStream_t s_;
+  OctreeOut* out_;
+  OctreeResult* res_;
+  std::vector<Frame> H_;
+  std::vector<void*> owned_;
+  char* pin_ = nullptr;
+  Frame* d_fr_ = nullptr;
+  int64_t* d_tab_ = nullptr;
+  char* d_rb_ = nullptr;
+  int64_t grid_[kMaps] = {};
+  Batch B_{};
+  // host copies of what the device reports
+  const int32_t* hinfo(int f) const { return (const int32_t*)(pin_ + up_bytes()) + 16 * f; }
+  const double* hdinfo(int f) const {
+    return (const double*)(pin_ + up_bytes() + 64 * (size_t)F_) + 4 * f;
   }
-  OT_TRY(A.get(&seg_b, M + 1));
-  OT_TRY(A.get(&seg_e, M + 1));
-  OT_TRY(hipMemsetAsync(seg_b, 0, 4 * (M + 1), s));
-  OT_TRY(hipMemsetAsync(seg_e, 0, 4 * (M + 1), s));
-  k_segments<<<blocks(P, 256), 256, 0, s>>>(snode, P, seg_b, seg_e);
-  k_links<<<blocks(M, 256), 256, 0, s>>>(t.key, M, nkeys_h, slot_node, ncap, t, info);
-  OT_TRY(hipGetLastError());
-  {
-    Runs RR{count, px};
-    OT_TRY(A.get(&T.term, P));
-    OT_TRY(A.get(&T.lval, P));
-    OT_TRY(A.get(&T.lcnt, P));
-    k_pair_terms<<<blocks(P, 256), 256, 0, s>>>(t.key, RR, spair, snode, P, R1, depth, assoc, T);
-    OT_TRY(hipGetLastError());
-    OT_TRY(A.get(&long_list, M + 1));
-    OT_TRY(A.get(&long_count, 2));
-    OT_TRY(hipMemsetAsync(long_count, 0, 8, s));
-    long_grid = (int)std::min<int64_t>(std::max<int64_t>(P / kShortSeg, 1), 2048);
-    k_fold_short<false><<<blocks(M, 64), 64, 0, s>>>(t, M, T, spair, seg_b, seg_e, R1, depth,
-                                                     long_list, long_count);
-    k_fold_long<false><<<long_grid, 64, 0, s>>>(t, T, spair, seg_b, seg_e, R1, depth, long_list,
-                                                long_count);
-    OT_TRY(hipGetLastError());
-    // 5a. PruneToCubeDepth, then loop 2's totals.
-    {
-      const int32_t hh = (int32_t)h;
-      OT_TRY(hipMemcpyAsync(info + 5, &hh, 4, hipMemcpyHostToDevice, s));
-    }
-    for (int32_t** pp : {&W.lvl, &W.att, &W.anc, &W.size, &W.gsz, &W.start, &W.gstart, &W.pre,
-                         &W.merged, &W.arr, &W.arr2})
-      OT_TRY(A.get(pp, M + 2));
-    OT_TRY(A.get(&W.lvl_off, 10));
-    {
-      const Tree TT{t, W, kids, maxc, depth, M};
-      k_prune_cube<<<1, kTreeThreads, 0, s>>>(TT, info);
-    }
-    OT_TRY(hipMemsetAsync(long_count + 1, 0, 4, s));
-    k_fold_short<true><<<blocks(M, 64), 64, 0, s>>>(t, M, T, spair, seg_b, seg_e, R1, depth,
-                                                    long_list, long_count + 1);
-    k_fold_long<true><<<long_grid, 64, 0, s>>>(t, T, spair, seg_b, seg_e, R1, depth, long_list,
-                                               long_count + 1);
-    OT_TRY(hipGetLastError());
-    // The root's chain: bound, or exact when the bound is not enough.
-    OT_TRY(A.get(&bound, 2));
-    k_root_bound<<<1, 256, 0, s>>>(t, M, bound);
-    OT_TRY(hipGetLastError());
-    OT_TRY(hipMemcpyAsync(hbound, bound, 16, hipMemcpyDeviceToHost, s));
-    OT_TRY(hipMemcpyAsync(hinfo, info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    OT_TRY(hipStreamSynchronize(s));
-    if (hinfo[6] == 4) {
-      res.status = 3;
-      snprintf(res.msg, sizeof(res.msg), "internal: node hash table overflow");
-      goto done;
-    }
-    if (hinfo[6] != 0) {
-      res.status = 5;
-      snprintf(res.msg, sizeof(res.msg),
-               "the octree would exceed MaxQNodes (266817) during classification; the level "
-               "pruning quantize.c performs then is not implemented on the device");
-      goto done;
-    }
-    root_qe = hbound[0];
-    // The approximation differs from the serial chain by at most ~3 P u sum
-    // (u = 2^-53): the reference's own rounding, the running values it adds,
-    // and the order of the parallel sum.  Demand a clear separation of 1 + 8 P u sum.
-    const double err = (double)P * 2.3e-16 * std::fabs(root_qe) * 4.0;
-    if (!(root_qe - 1.0 - err > hbound[1] + 1e-9 * hbound[1] + 2e-12) || M == 0) {
-      OT_TRY(A.get(&record, P));
-      k_fold_record<<<blocks(M, 64), 64, 0, s>>>(M, T.term, spair, seg_b, seg_e, record);
-      OT_TRY(A.get(&flat, 1));
-      k_root_exact<<<1, 1, 0, s>>>(record, P, flat);
-      OT_TRY(hipGetLastError());
-      OT_TRY(hipMemcpyAsync(&root_qe, flat, 8, hipMemcpyDeviceToHost, s));
-      OT_TRY(hipStreamSynchronize(s));
-      res.exact_root = 1;
+  const int32_t* hctl() const { return (const int32_t*)(pin_ + up_bytes() + 96 * (size_t)F_); }
+  size_t up_bytes() const { return sizeof(Frame) * F_ + sizeof(int64_t) * kMaps * (F_ + 1); }
+  size_t rb_bytes() const { return 96 * (size_t)F_ + 16; }
+  Map map(int kind) const { return Map{d_tab_ + (size_t)kind * (F_ + 1), F_}; }
+
+  void fail(int f, int status, const char* fmt, ...) __attribute__((format(printf, 4, 5))) {
+    res_[f].status = status;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(res_[f].msg, sizeof(res_[f].msg), fmt, ap);
+    va_end(ap);
+    H_[f].ok = 0;
+  }
+
+  void drop_outputs(int f) {
+    OctreeOut& o = out_[f];
+    if (o.block) (void)hipFree(o.block);
+    if (o.tree_block) (void)hipFree(o.tree_block);
+    o.block = o.tree_block = nullptr;
+    o.image = nullptr;
+    o.hdr = nullptr;
+    o.entries = nullptr;
+    o.tree = TreeDev{};
+  }
+
+  // Element counts per image for every launch shape, from what is known so far.
+  void fill_maps(int64_t* tab) {
+    for (int k = 0; k < kMaps; ++k) {
+      int64_t* first = tab + (size_t)k * (F_ + 1);
+      const int per = k == kMapM64 ? 64 : 256;
+      int64_t run = 0;
+      for (int f = 0; f < F_; ++f) {
+        const Frame& d = H_[f];
+        int64_t n = 0;
+        switch (k) {
+          case kMapN: n = d.ok ? d.N : 0; break;
+          case kMapR: n = d.ok ? d.R : 0; break;
+          case kMapP: n = d.P; break;  // failed images keep their pairs in the sort
+          case kMapCap: n = d.ok ? d.ncap : 0; break;
+          case kMapM256:
+          case kMapM64: n = d.ok ? d.M : 0; break;
+          case kMapDesc: n = d.ok && d.pb.hdr ? desc_entries(d.kids) : 0; break;
+        }
+        first[f] = run;
+        run += blocks(n, per);
+      }
+      first[F_] = run;
+      grid_[k] = run;
     }
   }
-  // 5b. Reduce, colormap, search image.
-  {
-    // The palette outlives this call: plain allocations, owned by the caller.
-    const PaletteLayout cap = palette_layout((int)maxc, (int)(M + 1), kids);
-    PaletteBuild pb;
-    pb.image_cap = cap.bytes;
-    OT_TRY(hipMalloc((void**)&pb.image, (size_t)cap.bytes));
-    out.image = pb.image;
-    OT_TRY(hipMalloc((void**)&pb.hdr, sizeof(PaletteDev)));
-    out.hdr = pb.hdr;
-    OT_TRY(hipMalloc((void**)&pb.entries, sizeof(double) * 4 * (size_t)maxc));
-    out.entries = pb.entries;
-    {
-      const Tree TT{t, W, kids, maxc, depth, M};
-      k_reduce<<<1, kTreeThreads, 0, s>>>(TT, assoc, root_qe, info, pb);
+
+  hipError_t upload() {
+    memcpy(pin_, H_.data(), sizeof(Frame) * F_);
+    fill_maps((int64_t*)(pin_ + sizeof(Frame) * F_));
+    return hipMemcpyAsync(d_fr_, pin_, up_bytes(), hipMemcpyHostToDevice, s_);
+  }
+
+  hipError_t readback() {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(pin_ + up_bytes(), d_rb_, rb_bytes(), hipMemcpyDeviceToHost, s_);
+    if (e == hipSuccess) e = hipStreamSynchronize(s_);
+    return e;
+  }
+
+  template <typename T>
+  void fill(T* p, int64_t n, T v);
+
+  // 1. runs: flags, one scan over the batch, runs per image.
+  hipError_t phase_runs() {
+    hipError_t e = staging(up_bytes() + rb_bytes(), &pin_);
+    if (e != hipSuccess) return e;
+    int64_t sumN = 0;
+    for (int f = 0; f < F_; ++f) sumN += rq_[f].w * rq_[f].h;
+    int32_t *flags = nullptr, *rid = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, flags, rid, (int)sumN, s_);
+    if (e != hipSuccess) return e;
+    char* up = nullptr;
+    Carve c;
+    c.want(&up, (int64_t)up_bytes());  // descriptors, then the block tables
+    c.want(&d_rb_, (int64_t)rb_bytes());
+    c.want(&flags, sumN);
+    c.want(&rid, sumN);
+    c.want((char**)&tmp, (int64_t)tmp_bytes);
+    if ((e = c.take(owned_, s_)) != hipSuccess) return e;
+    d_fr_ = (Frame*)up;
+    d_tab_ = (int64_t*)(up + sizeof(Frame) * F_);
+    int32_t* info = (int32_t*)d_rb_;
+    double* dinfo = (double*)(d_rb_ + 64 * (size_t)F_);
+    B_.ctl = (int32_t*)(d_rb_ + 96 * (size_t)F_);
+    int64_t off = 0;
+    for (int f = 0; f < F_; ++f) {
+      const OctreeReq& q = rq_[f];
+      Frame& d = H_[f];
+      memset(&d, 0, sizeof(Frame));
+      d.src = q.src;
+      d.fmt = q.fmt;
+      d.assoc = q.assoc ? 1 : 0;
+      d.maxc = q.maxc <= 0 ? 65536 : std::min(q.maxc, 65536);
+      d.depth = octree_depth(d.maxc, d.assoc != 0, q.gray != 0);
+      d.kids = d.assoc ? 16 : 8;
+      d.ok = 1;
+      d.w = q.w;
+      d.h = q.h;
+      d.N = q.w * q.h;
+      d.info = info + 16 * f;
+      d.dinfo = dinfo + 4 * f;
+      d.flags = flags + off;
+      d.rid = rid + off;
+      off += d.N;
     }
-    k_descent<<<blocks(desc_entries(kids), 256), 256, 0, s>>>(pb.hdr, info);
-    OT_TRY(hipGetLastError());
-    OT_TRY(hipMemcpyAsync(hinfo, info, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    OT_TRY(hipStreamSynchronize(s));
-    if (hinfo[6] == 2) {
-      res.status = 5;
-      snprintf(res.msg, sizeof(res.msg), "search image above its capacity (%d rows, %d bytes)",
-               hinfo[10], hinfo[11]);
-      goto done;
+    if ((e = upload()) != hipSuccess) return e;
+    k_frame_init<<<F_, 64, 0, s_>>>(d_fr_);
+    k_flags<<<grid_[kMapN], 256, 0, s_>>>(d_fr_, map(kMapN));
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, flags, rid, (int)sumN, s_);
+    if (e != hipSuccess) return e;
+    k_run_count<<<F_, 64, 0, s_>>>(d_fr_);
+    return readback();
+  }
+
+  // 2. run records and loop 1's extent (one workgroup per image).
+  hipError_t phase_extent() {
+    int64_t sumR = 0, sumB = 0;
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      d.R = hinfo(f)[12];
+      d.bcap = pow2_at_least(2 * ((int64_t)d.maxc + 1 + 1024));
+      sumR += d.R;
+      sumB += d.bcap;
     }
-    if (hinfo[6] == 3) {
-      res.status = 2;
-      snprintf(res.msg, sizeof(res.msg), "colour reduction does not converge");
-      goto done;
+    int32_t *start = nullptr, *count = nullptr, *row = nullptr;
+    double4* px = nullptr;
+    uint32_t *pack = nullptr, *bfirst = nullptr;
+    uint64_t* bkeys = nullptr;
+    Carve c;
+    c.want(&start, sumR);
+    c.want(&count, sumR);
+    c.want(&row, sumR);
+    c.want(&px, sumR);
+    c.want(&pack, sumR);
+    c.want(&bkeys, sumB);
+    c.want(&bfirst, sumB);
+    hipError_t e = c.take(owned_, s_);
+    if (e != hipSuccess) return e;
+    int64_t ro = 0, bo = 0;
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      d.start = start + ro;
+      d.count = count + ro;
+      d.row = row + ro;
+      d.px = px + ro;
+      d.pack = pack + ro;
+      d.bkeys = bkeys + bo;
+      d.bfirst = bfirst + bo;
+      ro += d.R;
+      bo += d.bcap;
     }
-    res.count = hinfo[8];
-    res.rows = hinfo[10];
-    res.image_bytes = hinfo[11];
-    res.nodes = M + 1;
-    res.depth = depth;
-    res.live = hinfo[9];
-    res.root_qe = root_qe;
-    if (out.keep_tree) {
-      // Hand the node arrays to the caller (they are freed with the palette).
-      out.tree = t;
-      out.tree_nodes = M + 1;
-      for (void** pp : {(void**)&t.key, (void**)&t.child, (void**)&t.parent, (void**)&t.id,
-                        (void**)&t.level, (void**)&t.qe, (void**)&t.share, (void**)&t.total,
-                        (void**)&t.nu, (void**)&t.color_number, (void**)&t.row}) {
-        auto it = std::find(A.ptrs.begin(), A.ptrs.end(), *pp);
-        if (it != A.ptrs.end()) A.ptrs.erase(it);
+    if ((e = upload()) != hipSuccess) return e;
+    k_run_starts<<<grid_[kMapN], 256, 0, s_>>>(d_fr_, map(kMapN));  // every image has pixels
+    k_run_data<<<grid_[kMapR], 256, 0, s_>>>(d_fr_, map(kMapR));    // and runs
+    fill(bkeys, sumB, kEmpty64);
+    fill(bfirst, sumB, kEmpty32);
+    k_break<<<F_, 1024, 0, s_>>>(d_fr_);
+    return readback();
+  }
+
+  // 3. the node hash tables and the list of node entries.
+  hipError_t phase_nodes() {
+    int64_t sumP = 0, sumC = 0;
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (!d.ok) continue;
+      if (hinfo(f)[6] == 4) {
+        fail(f, 3, "internal: leaf hash table overflow");
+        continue;
+      }
+      d.R1 = hinfo(f)[1];
+      const int64_t P = 8 * d.R1 + (int64_t)d.depth * (d.R - d.R1);
+      if (P >= ((int64_t)1 << 31)) {
+        fail(f, 5, "too many runs");
+        continue;
+      }
+      int64_t bound_nodes = 8 * ((int64_t)d.maxc + 1 + d.w) + 1;
+      int64_t pw = 1;
+      for (int l = 1; l <= d.depth; ++l) {
+        pw = std::min<int64_t>(pw * d.kids, (int64_t)1 << 40);
+        bound_nodes += std::min<int64_t>(d.R, pw);
+      }
+      bound_nodes = std::min<int64_t>(bound_nodes, P + 1);
+      d.P = P;
+      d.ncap = pow2_at_least(2 * bound_nodes + 64);
+      d.pair_base = sumP;
+      sumP += P;
+      sumC += d.ncap;
+    }
+    if (sumP >= ((int64_t)1 << 32) - 1) {  // one image per sub-batch is bounded above
+      for (int f = 0; f < F_; ++f)
+        if (H_[f].ok) fail(f, 5, "too many runs");
+      return hipSuccess;
+    }
+    sumP_ = sumP;
+    uint64_t* nkeys = nullptr;
+    uint32_t *nfirst = nullptr, *pair_slot = nullptr;
+    Carve c;
+    c.want(&nkeys, sumC);
+    c.want(&nfirst, sumC);
+    c.want(&pair_slot, sumP);
+    c.want(&B_.ekey, sumC);
+    c.want(&B_.eslot, sumC);
+    c.want(&slot_node_, sumC);
+    hipError_t e = c.take(owned_, s_);
+    if (e != hipSuccess) return e;
+    int64_t co = 0;
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (!d.ok) continue;
+      d.nkeys = nkeys + co;
+      d.nfirst = nfirst + co;
+      d.slot_node = slot_node_ + co;
+      d.pair_slot = pair_slot + d.pair_base;
+      co += d.ncap;
+    }
+    if ((e = upload()) != hipSuccess) return e;
+    fill(B_.ctl, 4, 0);
+    fill(nkeys, sumC, kEmpty64);
+    fill(nfirst, sumC, kEmpty32);
+    if (grid_[kMapP]) {
+      k_node_insert<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
+      k_node_lookup<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
+      k_node_collect<<<grid_[kMapCap], 256, 0, s_>>>(d_fr_, map(kMapCap), B_);
+    }
+    return readback();
+  }
+
+  // 4. node order, ordered sums, PruneToCubeDepth, the root, Reduce, colormap.
+  hipError_t phase_tree() {
+    const int64_t E = hctl()[0];
+    const int64_t sumP = sumP_;
+    int64_t eb = 0, sumNodes = 0, sumW = 0;
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (d.P == 0) continue;  // took no part in phase 3
+      d.entry_base = eb;
+      eb += hinfo(f)[13];
+      if (d.ok && hinfo(f)[6] == 4) fail(f, 3, "internal: node hash table overflow");
+      d.M = d.ok ? hinfo(f)[13] : 0;
+      d.node_base = sumNodes;
+      sumNodes += d.ok ? d.M + 1 : 1;
+    }
+    if (sumP == 0) return readback();
+    // Outputs first: they outlive the batch (plain allocations owned by the caller).
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (!d.ok) continue;
+      const PaletteLayout cap =
+          palette_layout(d.maxc, (int)std::min<int64_t>(d.M + 1, 32767), d.kids);
+      const size_t img = align256((size_t)cap.bytes), hdr = align256(sizeof(PaletteDev));
+      void* blk = nullptr;
+      const hipError_t e = hipMalloc(&blk, img + hdr + sizeof(double) * 4 * (size_t)d.maxc);
+      if (e != hipSuccess) return e;
+      OctreeOut& o = out_[f];
+      o.block = blk;
+      o.image = (unsigned char*)blk;
+      o.hdr = (PaletteDev*)((char*)blk + img);
+      o.entries = (double*)((char*)blk + img + hdr);
+      d.pb.image = o.image;
+      d.pb.image_cap = cap.bytes;
+      d.pb.hdr = o.hdr;
+      d.pb.entries = o.entries;
+    }
+    for (int f = 0; f < F_; ++f)
+      if (H_[f].ok) sumW += H_[f].M + 2;
+    const int bitsA = bits_for(sumP), bitsB = bits_for(sumNodes);
+    size_t tmpA = 0, tmpB = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpA, B_.ekey, (uint32_t*)nullptr,
+                                                      B_.eslot, (uint32_t*)nullptr, (int)E, 0,
+                                                      bitsA, s_);
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpB, (int32_t*)nullptr, (int32_t*)nullptr,
+                                           (uint32_t*)nullptr, (uint32_t*)nullptr, (int)sumP, 0,
+                                           bitsB, s_);
+    if (e != hipSuccess) return e;
+    TreeDev t{};
+    uint32_t *skey = nullptr, *sslot = nullptr, *spair = nullptr;
+    int32_t *pp = nullptr, *snode = nullptr, *seg_b = nullptr, *seg_e = nullptr, *lcnt = nullptr;
+    double* term = nullptr;
+    float4* lval = nullptr;
+    int32_t* w[11] = {};
+    int32_t* lvl_off = nullptr;
+    char* tmp = nullptr;
+    Carve c;
+    c.want(&t.key, sumNodes);
+    c.want(&t.child, 16 * sumNodes);
+    c.want(&t.parent, sumNodes);
+    c.want(&t.id, sumNodes);
+    c.want(&t.level, sumNodes);
+    c.want(&t.qe, sumNodes);
+    c.want(&t.share, sumNodes);
+    c.want(&t.total, 4 * sumNodes);
+    c.want(&t.nu, sumNodes);
+    c.want(&t.color_number, sumNodes);
+    c.want(&t.row, sumNodes);
+    c.want(&skey, E);
+    c.want(&sslot, E);
+    c.want(&pp, 2 * sumP);  // pnode | pidx; later the running values of the exact root
+    c.want(&snode, sumP);
+    c.want(&spair, sumP);
+    c.want(&seg_b, sumNodes);
+    c.want(&seg_e, sumNodes);
+    c.want(&term, sumP);
+    c.want(&lval, sumP);
+    c.want(&lcnt, sumP);
+    c.want(&B_.long_list, sumNodes);
+    for (int k = 0; k < 11; ++k) c.want(&w[k], sumW);
+    c.want(&lvl_off, 10 * (int64_t)F_);
+    c.want(&tmp, (int64_t)std::max(tmpA, tmpB));
+    if ((e = c.take(owned_, s_)) != hipSuccess) return e;
+    B_.skey = skey;
+    B_.sslot = sslot;
+    B_.pnode = pp;
+    B_.pidx = (uint32_t*)(pp + sumP);
+    int64_t wo = 0;
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (d.P == 0) continue;
+      const int64_t nb = d.node_base, pb = d.pair_base;
+      TreeDev& td = d.t;
+      td.key = t.key + nb;
+      td.child = t.child + 16 * nb;
+      td.parent = t.parent + nb;
+      td.id = t.id + nb;
+      td.level = t.level + nb;
+      td.qe = t.qe + nb;
+      td.share = t.share + nb;
+      td.total = t.total + 4 * nb;
+      td.nu = t.nu + nb;
+      td.color_number = t.color_number + nb;
+      td.row = t.row + nb;
+      d.seg_b = seg_b + nb;
+      d.seg_e = seg_e + nb;
+      d.snode = snode + pb;
+      d.spair = spair + pb;
+      d.T.term = term + pb;
+      d.T.lval = lval + pb;
+      d.T.lcnt = lcnt + pb;
+      d.record = (double*)pp + pb;
+      if (d.ok) {
+        int32_t** wp[11] = {&d.W.lvl, &d.W.att, &d.W.anc, &d.W.size, &d.W.gsz, &d.W.start,
+                            &d.W.gstart, &d.W.pre, &d.W.merged, &d.W.arr, &d.W.arr2};
+        for (int k = 0; k < 11; ++k) *wp[k] = w[k] + wo;
+        d.W.lvl_off = lvl_off + 10 * f;
+        wo += d.M + 2;
       }
     }
+    if ((e = upload()) != hipSuccess) return e;
+    fill(B_.ctl, 4, 0);
+    if (E > 0) {
+      e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpA, B_.ekey, skey, B_.eslot, sslot, (int)E, 0,
+                                             bitsA, s_);
+      if (e != hipSuccess) return e;
+    }
+    fill(t.child, 16 * sumNodes, -1);
+    fill(t.color_number, sumNodes, 0u);
+    fill(seg_b, sumNodes, 0);
+    fill(seg_e, sumNodes, 0);
+    k_root_init<<<F_, 64, 0, s_>>>(d_fr_);
+    if (grid_[kMapM256]) k_node_assign<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256), B_);
+    k_pair_nodes<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP), B_);
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpB, B_.pnode, snode, B_.pidx, spair, (int)sumP,
+                                           0, bitsB, s_);
+    if (e != hipSuccess) return e;
+    k_segments<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
+    if (grid_[kMapM256]) k_links<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256));
+    k_pair_terms<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
+    const int long_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 2048);
+    if (grid_[kMapM64]) k_fold_short<false><<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64), B_);
+    k_fold_long<false><<<long_grid, 64, 0, s_>>>(d_fr_, B_);
+    k_prune_cube<<<F_, kTreeThreads, 0, s_>>>(d_fr_);
+    if (grid_[kMapM64]) k_fold_short<true><<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64), B_);
+    k_fold_long<true><<<long_grid, 64, 0, s_>>>(d_fr_, B_);
+    k_root_bound<<<F_, 256, 0, s_>>>(d_fr_);
+    if (grid_[kMapM64]) k_fold_record<<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64));
+    k_root_exact<<<F_, 64, 0, s_>>>(d_fr_);
+    k_reduce<<<F_, kTreeThreads, 0, s_>>>(d_fr_);
+    if (grid_[kMapDesc]) k_descent<<<grid_[kMapDesc], 256, 0, s_>>>(d_fr_, map(kMapDesc));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = keep_trees()) != hipSuccess) return e;
+    if ((e = readback()) != hipSuccess) return e;
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (!d.ok) continue;
+      const int32_t* in = hinfo(f);
+      OctreeResult& r = res_[f];
+      if (in[6] == 4) {
+        fail(f, 3, "internal: node hash table overflow");
+      } else if (in[6] == 1) {
+        fail(f, 5,
+             "the octree would exceed MaxQNodes (266817) during classification; the level "
+             "pruning quantize.c performs then is not implemented on the device");
+      } else if (in[6] == 2) {
+        fail(f, 5, "search image above its capacity (%d rows, %d bytes)", in[10], in[11]);
+      } else if (in[6] == 3) {
+        fail(f, 2, "colour reduction does not converge");
+      } else {
+        r.count = in[8];
+        r.rows = in[10];
+        r.image_bytes = in[11];
+        r.nodes = d.M + 1;
+        r.depth = d.depth;
+        r.live = in[9];
+        r.root_qe = hdinfo(f)[2];
+        r.exact_root = in[14];
+        out_[f].tree_nodes = out_[f].keep_tree ? d.M + 1 : 0;
+      }
+    }
+    return hipSuccess;
   }
-done:
-  A.release();
-  return res;
+
+  // Copies of the node arrays for the images that keep their tree.
+  hipError_t keep_trees() {
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      OctreeOut& o = out_[f];
+      if (!d.ok || !o.keep_tree) continue;
+      const size_t n = (size_t)d.M + 1;
+      const size_t sz[11] = {8 * n, 64 * n, 4 * n, 4 * n, 4 * n, 8 * n,
+                             8 * n, 32 * n, 8 * n, 4 * n, 4 * n};
+      void* src[11] = {d.t.key, d.t.child, d.t.parent, d.t.id, d.t.level, d.t.qe,
+                       d.t.share, d.t.total, d.t.nu, d.t.color_number, d.t.row};
+      size_t total = 0;
+      for (size_t b : sz) total += align256(b);
+      char* blk = nullptr;
+      hipError_t e = hipMalloc((void**)&blk, total);
+      if (e != hipSuccess) return e;
+      o.tree_block = blk;
+      void** dst[11] = {(void**)&o.tree.key, (void**)&o.tree.child, (void**)&o.tree.parent,
+                        (void**)&o.tree.id, (void**)&o.tree.level, (void**)&o.tree.qe,
+                        (void**)&o.tree.share, (void**)&o.tree.total, (void**)&o.tree.nu,
+                        (void**)&o.tree.color_number, (void**)&o.tree.row};
+      size_t off = 0;
+      for (int k = 0; k < 11; ++k) {
+        *dst[k] = blk + off;
+        e = hipMemcpyAsync(blk + off, src[k], sz[k], hipMemcpyDeviceToDevice, s_);
+        if (e != hipSuccess) return e;
+        off += align256(sz[k]);
+      }
+    }
+    return hipSuccess;
+  }
+
+  int64_t sumP_ = 0;
+  int32_t* slot_node_ = nullptr;
+};
+
+template <typename T>
+void SubBuild::fill(T* p, int64_t n, T v) {
+  if (n <= 0) return;
+  if constexpr (sizeof(T) == 8)
+    k_fill_u64<<<blocks(n, 256), 256, 0, s_>>>((uint64_t*)p, n, (uint64_t)v);
+  else if constexpr (std::is_same<T, int32_t>::value)
+    k_fill_i32<<<blocks(n, 256), 256, 0, s_>>>(p, n, v);
+  else
+    k_fill_u32<<<blocks(n, 256), 256, 0, s_>>>((uint32_t*)p, n, (uint32_t)v);
+}
+
+}  // namespace
+
+void build_octree_batch(const OctreeReq* rq, int n, hipStream_t s, OctreeOut* outs,
+                        OctreeResult* res) {
+  std::vector<int> group;
+  int64_t px = 0;
+  auto flush = [&]() {
+    if (group.empty()) return;
+    std::vector<OctreeReq> q;
+    std::vector<OctreeOut> o;
+    std::vector<OctreeResult> r(group.size());
+    for (int i : group) {
+      q.push_back(rq[i]);
+      o.push_back(outs[i]);
+    }
+    SubBuild(q.data(), (int)group.size(), s, o.data(), r.data()).run();
+    for (size_t k = 0; k < group.size(); ++k) {
+      outs[group[k]] = o[k];
+      res[group[k]] = r[k];
+    }
+    group.clear();
+    px = 0;
+  };
+  for (int i = 0; i < n; ++i) {
+    res[i] = OctreeResult{};
+    res[i].msg[0] = 0;
+    const int64_t N = rq[i].w * rq[i].h;
+    if (rq[i].w < 1 || rq[i].h < 1) {
+      res[i].status = 1;
+      snprintf(res[i].msg, sizeof(res[i].msg), "empty image");
+      continue;
+    }
+    if (N > ((int64_t)1 << 28)) {
+      res[i].status = 5;
+      snprintf(res[i].msg, sizeof(res[i].msg), "image above 2^28 pixels");
+      continue;
+    }
+    if (!group.empty() && px + N > kBatchPixels) flush();
+    group.push_back(i);
+    px += N;
+  }
+  flush();
+}
+
+OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int maxc, int assoc,
+                          int gray, hipStream_t s, OctreeOut* out) {
+  const OctreeReq q{src, fmt, w, h, maxc, assoc, gray};
+  OctreeResult r;
+  build_octree_batch(&q, 1, s, out, &r);
+  return r;
 }
 
 }  // namespace sa

@@ -24,6 +24,10 @@ def _compare(gpu, oracle, img_dev, fmt, w, h, f, colours, assoc, gray):
     say("octree", w, h, "colours", colours, "assoc", assoc, "gray", gray)
     pal = gpu.Palette.build(img_dev, w, h, fmt, colours, assoc, gray, keep_tree=True)
     _KEEP.append(pal)
+    return _check_built(oracle, pal, f, colours, assoc, gray)
+
+
+def _check_built(oracle, pal, f, colours, assoc, gray):
     t = oracle.tree_build(f, colours, assoc, gray)
     d = t.data()
     info = pal.info()
@@ -111,3 +115,86 @@ def test_device_palette_drives_the_walk(gpu, oracle):
     t = oracle.tree_build(f, 32, True)
     got = H.from_device(dst, (110, 150, 4), np.float32)
     assert np.array_equal(got, oracle.walk_seq(t, f))
+
+
+def _batch_cases(seed):
+    rng = np.random.default_rng(seed)
+    shapes = [(1, 1, 16, 3), (3, 5, 4, 3), (17, 13, 16, 4), (200, 150, 16, 3), (96, 64, 256, 4),
+              (640, 480, 16, 3), (257, 3, 8, 3), (37, 91, 32, 4), (1000, 20, 7, 3), (64, 64, 2, 3)]
+    out = []
+    for w, h, colours, ch in shapes:
+        img = synth.sinusoid_image(w, h, rng, channels=ch)
+        out.append((img, colours))
+    g = np.clip(np.round(np.linspace(0, 255, 120)[None, :] + rng.normal(0, 9, (80, 120))), 0, 255)
+    out.append((np.repeat(g.astype(np.uint8)[..., None], 3, axis=-1), 16))
+    return out
+
+
+def test_octree_batch_matches_oracle(gpu, oracle):
+    """One batched build of uneven images (sizes, palette sizes, alpha, grey):
+    every image gets exactly the palette and tree ColorTree::Build gives it."""
+    cases = _batch_cases(11)
+    items, fs = [], []
+    for img, colours in cases:
+        h, w = img.shape[:2]
+        src = H.to_device(img)
+        _KEEP.append(src)
+        gray = synth.is_gray(img)
+        items.append((src, w, h, H.fmt_of(img), colours, img.shape[2] == 4, gray, True))
+        fs.append(synth.widen_q16(img))
+    say("octree batch", len(items))
+    pals = gpu.Palette.build_batch(items)
+    _KEEP.extend(pals)
+    for (src, w, h, fmt, colours, assoc, gray, _), f, pal in zip(items, fs, pals):
+        say("  item", w, h, colours, assoc, gray)
+        _check_built(oracle, pal, f, colours, assoc, gray)
+
+
+def _same_palette(a, b):
+    assert a.info() == b.info()
+    assert np.array_equal(a.colormap(), b.colormap())
+    ta, tb = a.tree(), b.tree()
+    for k in ta:
+        assert np.array_equal(ta[k].view(np.uint8), tb[k].view(np.uint8)), k
+
+
+def test_octree_batch_splits_into_sub_batches(gpu):
+    """17 frames of 1080p exceed one sub-batch (2^25 pixels): the build splits
+    and each frame still gets its single build's palette, bit for bit."""
+    import torch
+    params = synth.plasma_params(5)
+    srcs = [H.to_device(synth.plasma_frame(t, params=params)) for t in range(17)]
+    say("octree batch 17 x 1080p")
+    pals = gpu.Palette.build_batch([(s, 1920, 1080, gpu.PIX_RGB8, 16, False, False, True)
+                                    for s in srcs])
+    torch.cuda.synchronize()
+    for k in (0, 7, 15, 16):
+        say("  frame", k)
+        one = gpu.Palette.build(srcs[k], 1920, 1080, gpu.PIX_RGB8, 16, False, False, keep_tree=True)
+        _same_palette(pals[k], one)
+        one.close()
+    for p in pals:
+        p.close()
+
+
+def test_octree_batch_item_failure_is_its_own(gpu, oracle):
+    """An image the device does not build (its tree would exceed MaxQNodes)
+    fails alone with SA_EUNSUPPORTED; the images beside it build exactly."""
+    rng = np.random.default_rng(12)
+    noise = rng.integers(0, 256, (700, 700, 3), dtype=np.uint8)
+    cases = _batch_cases(13)[:4]
+    srcs = [H.to_device(img) for img, _ in cases]
+    nsrc = H.to_device(noise)
+    _KEEP.extend(srcs + [nsrc])
+    items = [(s, img.shape[1], img.shape[0], H.fmt_of(img), c, img.shape[2] == 4, False, True)
+             for s, (img, c) in zip(srcs, cases)]
+    items.insert(2, (nsrc, 700, 700, gpu.PIX_RGB8, 65536, False, False, True))
+    say("octree batch with a failing item")
+    pals, status = gpu.Palette.build_batch(items, strict=False)
+    assert status[2] == gpu.SA_EUNSUPPORTED and pals[2] is None
+    assert [s for k, s in enumerate(status) if k != 2] == [0] * 4
+    with pytest.raises(gpu.EngineError, match="item 2"):
+        gpu.Palette.build_batch(items)
+    for k, (img, c) in enumerate(cases):
+        pal = pals[k if k < 2 else k + 1]
+        _check_built(oracle, pal, synth.widen_q16(img), c, img.shape[2] == 4, False)
