@@ -191,14 +191,17 @@ struct Frame {
 
 // Batch-wide arrays and counters.
 struct Batch {
-  int32_t* ctl;            // [0] collected node entries, [1] [2] long segments (loops 1, 2)
+  int32_t* ctl;            // [0] node entries; medium [1] [2], long [3] [4], huge [5] [6]
+                           // segments of the loop-1 / loop-2 folds; [8 + FoldMode] queue heads
   uint32_t* ekey;          // node entries: pair_base + first pair
   uint32_t* eslot;         // node entries: hash slot
   const uint32_t* skey;    // the same, sorted
   const uint32_t* sslot;
   int32_t* pnode;          // batch node id per pair
   uint32_t* pidx;          // pair index per pair
-  uint64_t* long_list;     // (image << 32) | node
+  uint64_t* mid_list;      // (image << 32) | node, segments folded by one wavefront
+  uint64_t* long_list;     // the same, segments folded by one workgroup
+  uint64_t* huge_list;     // the same, the longest: taken first
 };
 
 struct Map {
@@ -405,6 +408,9 @@ __global__ void k_node_insert(const Frame* __restrict__ fr, Map m) {
     pair_slot[p] = kNoSlot;
     return;
   }
+  // A plain read first: a key never changes once set and `first` only
+  // decreases, so a stale read costs at most an atomic that finds the same.
+  // Most pairs find their node already inserted, by an earlier pair.
   uint32_t slot = hash32(key) & (cap - 1);
   for (uint32_t probe = 0;; ++probe) {
     if (probe >= cap) {  // table full: report, never spin
@@ -412,13 +418,17 @@ __global__ void k_node_insert(const Frame* __restrict__ fr, Map m) {
       pair_slot[p] = 0;
       return;
     }
-    const unsigned long long prev =
-        atomicCAS((unsigned long long*)&hkeys[slot], (unsigned long long)kEmpty64,
-                  (unsigned long long)key);
-    if (prev == kEmpty64 || prev == key) break;
+    const uint64_t cur = hkeys[slot];
+    if (cur == key) break;
+    if (cur == kEmpty64) {
+      const unsigned long long prev =
+          atomicCAS((unsigned long long*)&hkeys[slot], (unsigned long long)kEmpty64,
+                    (unsigned long long)key);
+      if (prev == kEmpty64 || prev == key) break;
+    }
     slot = (slot + 1) & (cap - 1);
   }
-  atomicMin(&hfirst[slot], (uint32_t)p);
+  if (hfirst[slot] > (uint32_t)p) atomicMin(&hfirst[slot], (uint32_t)p);
   pair_slot[p] = slot;
 }
 
@@ -621,7 +631,12 @@ __device__ __forceinline__ double tot_term(int32_t cnt, float v) {
   return (double)cnt * kScale * (double)v;
 }
 
-constexpr int kShortSeg = 256;
+constexpr int kShortSeg = 256;   // longer: one wavefront (k_fold_mid)
+constexpr int kMidSeg = 8192;    // longer: one workgroup, in parallel (k_fold_long)
+// Loop 2's four total_color chains: one wavefront folds them together, which
+// keeps up with a workgroup's parallel fold far longer.
+constexpr int kMidSegTot = 65536;
+constexpr int kHugeSeg = 65536;  // longer: queued ahead of the other long segments
 
 // L2 = false: quantize_error (+ the node's share of the root chain) over all
 // pairs, and loop 1's leaf totals.  L2 = true: loop 2's leaf totals, folded onto
@@ -642,7 +657,13 @@ __global__ void k_fold_short(const Frame* __restrict__ fr, Map m, Batch B) {
   const int level = t.level[n];
   if (L2 && level != depth) return;
   if (e - b > kShortSeg) {
-    B.long_list[atomicAdd(&B.ctl[L2 ? 2 : 1], 1)] = ((uint64_t)at.f << 32) | (uint64_t)n;
+    const uint64_t ent = ((uint64_t)at.f << 32) | (uint64_t)n;
+    if (e - b > kHugeSeg)
+      B.huge_list[atomicAdd(&B.ctl[L2 ? 6 : 5], 1)] = ent;
+    else if (e - b > (L2 ? kMidSegTot : kMidSeg))
+      B.long_list[atomicAdd(&B.ctl[L2 ? 4 : 3], 1)] = ent;
+    else
+      B.mid_list[atomicAdd(&B.ctl[L2 ? 2 : 1], 1)] = ent;
     return;
   }
   const uint32_t l1 = (uint32_t)(8 * R1);
@@ -806,12 +827,13 @@ __device__ void fold_tot_wave(const Terms& T, int32_t b, int32_t e, double* s_p,
   *nu += nsum;
 }
 
+// Medium segments: one wavefront each, the chain folded serially.
 template <bool L2>
-__global__ __launch_bounds__(64) void k_fold_long(const Frame* __restrict__ fr, Batch B) {
+__global__ __launch_bounds__(64) void k_fold_mid(const Frame* __restrict__ fr, Batch B) {
   __shared__ double s_buf[2 * 4 * kChunk];
   const int32_t items = B.ctl[L2 ? 2 : 1];
   for (int32_t it = blockIdx.x; it < items; it += gridDim.x) {
-    const uint64_t ent = B.long_list[it];
+    const uint64_t ent = B.mid_list[it];
     const Frame& d = fr[(int32_t)(ent >> 32)];
     const int32_t n = (int32_t)(ent & 0xFFFFFFFFu);
     const TreeDev& t = d.t;
@@ -839,6 +861,539 @@ __global__ __launch_bounds__(64) void k_fold_long(const Frame* __restrict__ fr, 
       for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * n + c];
       uint64_t nu = t.nu[n];
       fold_tot_wave(T, s, e, s_buf, tot, &nu);
+      if (threadIdx.x == 0) {
+        for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
+        t.nu[n] = nu;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- exact serial sums, in parallel --------------------------------------------------
+// Every chain folded here adds finite non-negative terms, so its running value
+// only grows.  While it stays in one binade [2^x, 2^(x+1)) every double there is
+// a multiple of u = 2^(x-52), so s + t rounds to s + rn(t / u) u: an integer
+// addition of the term rounded on its own.  Two kinds of step are not of that
+// form: a tie (t / u exactly halfway, where round-half-even looks at s) and the
+// step whose exact sum leaves the binade (rounded on the next binade's grid).
+// A workgroup takes the chain a window at a time: each thread rounds its terms
+// on the current grid, a block scan gives every step's running value, the
+// first special step is found, everything before it is committed exactly, and
+// that step is done as the plain IEEE addition it is.  The result equals the
+// serial loop bit for bit; the serial work left is one addition per binade
+// crossing or tie, and a short serial prologue where crossings are dense.
+constexpr int kFoldThreads = 512;
+constexpr int kFoldWaves = kFoldThreads / 64;
+constexpr int kFoldPrologue = 512;                 // terms added serially first
+constexpr int kFoldRounds = 8;                     // special steps resolved per window in
+                                                   // parallel; the rest of it serially
+constexpr double kFoldSat = 0x1p54;                // saturation of the integer scan
+
+struct FoldShared {
+  double part[2][kFoldWaves][4];                   // scan partials (alternating passes)
+  int32_t first[2][kFoldWaves][4];                 // min partials
+  double dpart[kFoldWaves];
+  uint64_t upart[kFoldWaves];
+  double Sk[4];
+  double tk[4];
+  double ser[4];
+};
+
+// Wavefront sums through DPP: quad permutes and row rotates give every lane
+// its 16-lane row's sum in four steps; two shuffles join the rows.
+template <int Ctrl>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, Ctrl, 0xF, 0xF, false);
+}
+template <int Ctrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i32<Ctrl>((int)(b & 0xFFFFFFFF));
+  const int hi = dpp_i32<Ctrl>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int kDppXor1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppRor4 = 0x124;  // row_ror:4
+constexpr int kDppRor8 = 0x128;  // row_ror:8
+
+__device__ __forceinline__ double wave_total(double v) {  // every lane: the wavefront's sum
+  v += dpp_f64<kDppXor1>(v);
+  v += dpp_f64<kDppXor2>(v);
+  v += dpp_f64<kDppRor4>(v);
+  v += dpp_f64<kDppRor8>(v);
+  v += __shfl_xor(v, 16, 64);
+  return v + __shfl_xor(v, 32, 64);
+}
+
+__device__ __forceinline__ int wave_or(int v) {
+  v |= dpp_i32<kDppXor1>(v);
+  v |= dpp_i32<kDppXor2>(v);
+  v |= dpp_i32<kDppRor4>(v);
+  v |= dpp_i32<kDppRor8>(v);
+  v |= __shfl_xor(v, 16, 64);
+  return v | __shfl_xor(v, 32, 64);
+}
+
+__device__ double fold_sum_d(double v, FoldShared& sh) {  // any order (approximate sums)
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh.dpart[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = 0.0;
+  for (int k = 0; k < kFoldWaves; ++k) m += sh.dpart[k];
+  __syncthreads();
+  return m;
+}
+
+__device__ uint64_t fold_sum_u64(uint64_t v, FoldShared& sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh.upart[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint64_t m = 0;
+  for (int k = 0; k < kFoldWaves; ++k) m += sh.upart[k];
+  __syncthreads();
+  return m;
+}
+
+// Terms of the chains, read through LDS: `load` fetches the raw record of an
+// index from global memory (coalesced, by the whole workgroup), `term` forms
+// chain c's term from it.
+struct QeSrc {  // side sum: the share of the root's chain, sum_i term_i * (e - i)
+  using Raw = double;
+  const double* term_;
+  __device__ Raw load(int32_t i) const { return term_[i]; }
+  __device__ double term(const Raw& r, int) const { return r; }
+  __device__ double side(const Raw& r, double e_minus_i) const { return r * e_minus_i; }
+};
+
+struct TotRaw {
+  float4 v;
+  int32_t n;
+};
+
+struct TotSrc {  // side sum: number_unique (an integer, exact in a double here)
+  using Raw = TotRaw;
+  const Terms* T;
+  __device__ Raw load(int32_t i) const { return TotRaw{T->lval[i], T->lcnt[i]}; }
+  __device__ double term(const Raw& r, int c) const {
+    const float f = c == 0 ? r.v.x : c == 1 ? r.v.y : c == 2 ? r.v.z : r.v.w;
+    return tot_term(r.n, f);
+  }
+  __device__ double side(const Raw& r, double) const { return (double)r.n; }
+};
+
+// Records [from, to) of one window into LDS (to - from <= the stage size),
+// thread t's K consecutive records contiguous, one record of padding after
+// every K so that threads reading their own run do not collide on banks.
+template <int K, class Src>
+__device__ __forceinline__ int32_t stage_slot(int32_t p) {
+  return p + p / K;
+}
+
+template <int K, class Src>
+__device__ void stage_in(const Src& src, int32_t from, int32_t to, typename Src::Raw* stage) {
+  typename Src::Raw r[K];
+  const int32_t n = to - from;  // <= K * kFoldThreads
+#pragma unroll
+  for (int j = 0; j < K; ++j) {  // every load in flight before the first store
+    const int32_t p = (int32_t)threadIdx.x + j * kFoldThreads;
+    if (p < n) r[j] = src.load(from + p);
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int32_t p = (int32_t)threadIdx.x + j * kFoldThreads;
+    if (p < n) stage[stage_slot<K, Src>(p)] = r[j];
+  }
+}
+
+// v + term(p0, c) + ... + term(p1 - 1, c) from the stage, left to right; the
+// records are read a batch ahead of the dependent additions.
+template <int K, class Src>
+__device__ double serial_lds(double v, const Src& src, const typename Src::Raw* stage,
+                             int32_t p0, int32_t p1, int c) {
+  constexpr int kB = sizeof(typename Src::Raw) > 8 ? 8 : 16;
+  int32_t p = p0;
+  for (; p + kB <= p1; p += kB) {
+    typename Src::Raw r[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k) r[k] = stage[stage_slot<K, Src>(p + k)];
+#pragma unroll
+    for (int k = 0; k < kB; ++k) v += src.term(r[k], c);
+  }
+  for (; p < p1; ++p) v += src.term(stage[stage_slot<K, Src>(p)], c);
+  return v;
+}
+
+// NC chains over the same index range [b, e), each summed left to right in
+// double from s[c]: s[c] + term(b, c) + ... + term(e - 1, c).  Every thread of
+// the workgroup calls it and gets the results.  The chains share windows
+// (each skips indices it has already passed), loads and reductions.  Integers
+// on a binade's grid are held in doubles: exact below 2^53, and a sum that
+// reaches 2^53 always lies past a special step.  A window without a special
+// step (the common case: its total stays inside the binade and no term ties)
+// needs one block reduction; only a window holding one is scanned to find it.
+// The next window's records are loaded while the current one is summed.
+//
+// It also returns, summed in any order, the side sum src.side over [b, e)
+// (each index counted once: `seen` marks how far it has been taken).
+template <int NC, int K, class Src>
+__device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, FoldShared& sh,
+                             typename Src::Raw* stage) {
+  using Raw = typename Src::Raw;
+  constexpr int32_t kWin = kFoldThreads * K;
+  static_assert(kFoldPrologue <= kWin, "the prologue fits the stage");
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int32_t me = (int32_t)threadIdx.x;
+  double side = 0.0;
+  const int32_t pro = min(e, b + kFoldPrologue);
+  int32_t seen = pro;
+  if (pro > b) {  // the first terms serially: binade crossings come densely there
+    __syncthreads();
+    stage_in<K>(src, b, pro, stage);
+    __syncthreads();
+    if (me < NC) {
+      double v = 0.0;
+#pragma unroll
+      for (int k = 0; k < NC; ++k)
+        if (k == me) v = s[k];
+      sh.ser[me] = serial_lds<K>(v, src, stage, 0, pro - b, me);
+    }
+    for (int32_t p = me; p < pro - b; p += kFoldThreads)
+      side += src.side(stage[stage_slot<K, Src>(p)], (double)(e - (b + p)));
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NC; ++k) s[k] = sh.ser[k];
+  }
+  int32_t pos[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) pos[c] = pro;
+  int par = 0;
+  // Window records, two buffers in turn: one summed while the next loads.
+  // Thread t holds records W + t + j * kFoldThreads (coalesced loads; the
+  // index is clamped so that every load is in bounds and none waits on a branch).
+  Raw bufA[K], bufB[K];
+  int32_t wA = -1, wB = -1;
+  auto fetch = [&](Raw (&buf)[K], int32_t& wb, int32_t w) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) buf[j] = src.load(min(w + me + j * kFoldThreads, e - 1));
+    wb = w;
+  };
+  auto pass = [&](Raw (&cur)[K], int32_t& wc, Raw (&nxt)[K], int32_t& wn) -> bool {
+    int32_t W = e;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (!(s[c] <= 0x1.fffffffffffffp1023)) pos[c] = e;  // inf (or NaN) absorbs the rest
+      W = min(W, pos[c]);
+    }
+    if (W >= e) return false;
+    const int32_t wend = min(e, W + kWin);
+    if (W > seen) {  // serial stretches passed [seen, W): their side sums, in parallel
+      for (int32_t i = seen + me; i < W; i += kFoldThreads)
+        side += src.side(src.load(i), (double)(e - i));
+      seen = W;
+    }
+    if (wc != W) fetch(cur, wc, W);
+    if (wend < e && wn != wend) fetch(nxt, wn, wend);
+    // Each chain's grid: the binade of s (below 2^-1022, the subnormal grid).
+    int x[NC];
+    double S[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      x[c] = s[c] < 0x1p-1022 ? -1022 : ilogb(s[c]);
+      S[c] = ldexp(s[c], 52 - x[c]);
+    }
+    // The window's total on each grid, and whether any term ties (any order:
+    // sums of these integers are exact below 2^53 and stay >= 2^53 above it).
+    double total[NC];
+    int tie = 0;
+    {
+      // t / u as two multiplications by powers of two (each representable,
+      // the product exact wherever it matters: an intermediate underflow
+      // only meets terms that round to 0).
+      double sc1[NC], sc2[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int k = 52 - x[c], k1 = k / 2;
+        sc1[c] = ldexp(1.0, k1);
+        sc2[c] = ldexp(1.0, k - k1);
+      }
+      double acc[NC][2];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c][0] = acc[c][1] = 0.0;
+      const double ei0 = (double)(e - (W + me));  // e - i, exact integers in a double
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int32_t i = W + me + j * kFoldThreads;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const bool act = i < e && i >= pos[c];
+          const double q = act ? (src.term(cur[j], c) * sc1[c]) * sc2[c] : 0.0;
+          const double r = rint(q);
+          tie |= fabs(q - r) == 0.5 ? 1 << c : 0;  // inf or NaN: the total fails below
+          acc[c][j & 1] += r;
+        }
+        if (i < wend && i >= seen) side += src.side(cur[j], ei0 - (double)(j * kFoldThreads));
+      }
+      seen = max(seen, wend);
+      // Integers of at most 2^54 each (or inf / NaN, which fail the test
+      // below): their sums are exact below 2^53 and stay >= 2^53 above it.
+      double red[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) red[c] = wave_total(acc[c][0] + acc[c][1]);
+      const int t = wave_or(tie);
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sh.part[par][wv][c] = red[c];
+        sh.first[par][wv][0] = t;
+      }
+    }
+    __syncthreads();
+    bool clean = true;
+    int ties = 0;
+    for (int k = 0; k < kFoldWaves; ++k) ties |= sh.first[par][k][0];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      double tot = 0.0;
+      for (int k = 0; k < kFoldWaves; ++k) tot += sh.part[par][k][c];
+      total[c] = fmin(tot, kFoldSat);
+      if ((ties >> c) & 1 || !(S[c] + tot < 0x1p53 - 1.0)) clean = false;
+    }
+    par ^= 1;
+    if (clean) {  // every step of the window stays on its binade's grid
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (pos[c] < wend) {
+          s[c] = ldexp(S[c] + total[c], x[c] - 52);
+          pos[c] = wend;
+        }
+      return true;
+    }
+    // A special step in this window: resolve the window from LDS, one
+    // special step per round (scan for every step's running value, find each
+    // chain's first special step, commit everything before it, take the step
+    // as the IEEE addition it is, go on from there on the new grid).  The next
+    // window is then the one already loading.
+#pragma unroll
+    for (int j = 0; j < K; ++j) stage[stage_slot<K, Src>(me + j * kFoldThreads)] = cur[j];
+    __syncthreads();
+    const int32_t base = W + me * K;
+    for (int round = 0;; ++round) {
+      bool live = false;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) live |= pos[c] < wend;
+      if (!live) break;
+      if (round == kFoldRounds) {  // specials come densely (ties): the rest serially
+        if (me < NC) {
+          double v = 0.0;
+          int32_t from = wend;
+#pragma unroll
+          for (int k = 0; k < NC; ++k)
+            if (k == me) {
+              v = s[k];
+              from = pos[k];
+            }
+          sh.ser[me] = serial_lds<K>(v, src, stage, from - W, wend - W, me);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (pos[c] < wend) {
+            s[c] = sh.ser[c];
+            pos[c] = wend;
+          }
+        __syncthreads();
+        break;
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        x[c] = s[c] < 0x1p-1022 ? -1022 : ilogb(s[c]);
+        S[c] = ldexp(s[c], 52 - x[c]);
+      }
+      double q[NC][K];
+      double L[NC];
+      int mytie = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int32_t i = base + j;
+        const Raw r = stage[stage_slot<K, Src>(me * K + j)];  // past wend: masked
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const bool act = i < wend && i >= pos[c];
+          q[c][j] = act ? ldexp(src.term(r, c), 52 - x[c]) : 0.0;
+          mytie |= (q[c][j] - floor(q[c][j]) == 0.5) ? 1 << c : 0;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; j += 2) {
+          a0 += rint(fmin(q[c][j], kFoldSat));
+          if (j + 1 < K) a1 += rint(fmin(q[c][j + 1], kFoldSat));
+        }
+        L[c] = fmin(a0 + a1, kFoldSat);
+      }
+      double excl[NC];
+      {
+        double incl[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) incl[c] = L[c];
+        for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const double t = __shfl_up(incl[c], off, 64);
+            if (lane >= off) incl[c] = fmin(incl[c] + t, kFoldSat);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          excl[c] = __shfl_up(incl[c], 1, 64);
+          if (lane == 0) excl[c] = 0.0;
+          if (lane == 63) sh.part[par][wv][c] = incl[c];
+        }
+      }
+      __syncthreads();
+      int32_t mine[NC];
+      double Rk[NC], tk[NC], tot[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double before = 0.0, all = 0.0;
+        for (int k = 0; k < kFoldWaves; ++k) {
+          const double v = sh.part[par][k][c];
+          if (k < wv) before = fmin(before + v, kFoldSat);
+          all = fmin(all + v, kFoldSat);
+        }
+        tot[c] = all;
+        double R = fmin(S[c] + fmin(before + excl[c], kFoldSat), kFoldSat);
+        mine[c] = INT32_MAX;
+        Rk[c] = 0.0;
+        tk[c] = 0.0;
+        if (R + L[c] < 0x1p53 - 1.0 && !((mytie >> c) & 1)) continue;  // nothing special here
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          const int32_t i = base + j;
+          if (mine[c] == INT32_MAX && i < wend && i >= pos[c]) {
+            const double qq = q[c][j];
+            if (qq - floor(qq) == 0.5 || !(qq < 0x1p53 - R)) {
+              mine[c] = i;
+              Rk[c] = R;
+              tk[c] = src.term(stage[stage_slot<K, Src>(me * K + j)], c);
+            } else {
+              R += rint(fmin(qq, kFoldSat));
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        int32_t v = mine[c];
+        for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+        if (lane == 0) sh.first[par][wv][c] = v;
+      }
+      __syncthreads();
+      int32_t first[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        int32_t m = sh.first[par][0][c];
+        for (int k = 1; k < kFoldWaves; ++k) m = min(m, sh.first[par][k][c]);
+        first[c] = m;
+        if (m == INT32_MAX) {
+          if (pos[c] < wend) {  // the rest of the window, none special
+            s[c] = ldexp(fmin(S[c] + tot[c], kFoldSat), x[c] - 52);
+            pos[c] = wend;
+          }
+        } else if (mine[c] == m) {
+          sh.Sk[c] = Rk[c];
+          sh.tk[c] = tk[c];
+        }
+      }
+      par ^= 1;
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        if (first[c] == INT32_MAX) continue;
+        s[c] = ldexp(sh.Sk[c], x[c] - 52) + sh.tk[c];  // the serial step itself
+        pos[c] = first[c] + 1;
+        if (!(s[c] <= 0x1.fffffffffffffp1023)) pos[c] = max(pos[c], wend);  // inf absorbs
+      }
+      __syncthreads();  // Sk / tk / part are rewritten by the next round
+    }
+    return true;
+  };
+  for (;;) {
+    if (!pass(bufA, wA, bufB, wB)) break;
+    if (!pass(bufB, wB, bufA, wA)) break;
+  }
+  if (seen < e)  // the last serial stretch reached e
+    for (int32_t i = seen + me; i < e; i += kFoldThreads)
+      side += src.side(src.load(i), (double)(e - i));
+  return fold_sum_d(side, sh);
+}
+
+// LDS stage of one window (plus one record of padding per K): kQeK records
+// per thread of quantize_error terms, kTotK per thread of leaf records.
+constexpr int kQeK = 14;
+constexpr int kTotK = 2;
+constexpr int kQeStage = kFoldThreads * (kQeK + 1) * (int)sizeof(double);
+constexpr int kTotStage = kFoldThreads * (kTotK + 1) * (int)sizeof(TotRaw);
+constexpr int kStageBytes = kQeStage > kTotStage ? kQeStage : kTotStage;
+
+// total_color over [b, e) onto tot[] (one chain per channel) and number_unique.
+__device__ void fold_tot(const Terms& T, int32_t b, int32_t e, double tot[4], uint64_t* nu,
+                         FoldShared& sh, TotRaw* stage) {
+  *nu += (uint64_t)exact_sums<4, kTotK>(tot, b, e, TotSrc{&T}, sh, stage);
+}
+
+// One workgroup per long segment.  quantize_error is a serial chain; its share
+// of the root's chain is only needed approximately: sum_i term_i * (e - i).
+// The quantize_error chain and the four total_color chains are separate
+// launches (kFoldQe, kFoldTot1 over loop 1's pairs; kFoldTot2 over loop 2's).
+enum FoldMode { kFoldQe = 0, kFoldTot2 = 1, kFoldTot1 = 2 };
+
+template <int Mode>
+__global__ __launch_bounds__(kFoldThreads) void k_fold_long(const Frame* __restrict__ fr, Batch B) {
+  constexpr bool L2 = Mode == kFoldTot2;
+  __shared__ FoldShared sh;
+  __shared__ __align__(16) unsigned char stage[kStageBytes];
+  __shared__ int32_t s_item;
+  const int32_t nh = B.ctl[L2 ? 6 : 5], nl = B.ctl[L2 ? 4 : 3];
+  for (;;) {  // a queue: the huge segments first, then the others
+    if (threadIdx.x == 0) s_item = atomicAdd(&B.ctl[8 + Mode], 1);
+    __syncthreads();
+    const int32_t it = s_item;
+    __syncthreads();
+    if (it >= nh + nl) break;
+    const uint64_t ent = it < nh ? B.huge_list[it] : B.long_list[it - nh];
+    const Frame& d = fr[(int32_t)(ent >> 32)];
+    const int32_t n = (int32_t)(ent & 0xFFFFFFFFu);
+    const TreeDev& t = d.t;
+    const Terms& T = d.T;
+    const uint32_t l1 = (uint32_t)(8 * d.R1);
+    const int32_t b = d.seg_b[n], e = d.seg_e[n];
+    const int level = t.level[n];
+    const int32_t s = loop2_start(d.spair, b, e, l1);
+    if (Mode == kFoldQe) {
+      const double* term = T.term;
+      double qe = 0.0;
+      const double share = exact_sums<1, kQeK>(&qe, b, e, QeSrc{term}, sh, (double*)stage);
+      if (threadIdx.x == 0) {
+        t.qe[n] = qe;
+        t.share[n] = share;
+      }
+    } else if (Mode == kFoldTot1) {  // loop 1's totals: leaves only
+      double tot[4] = {0.0, 0.0, 0.0, 0.0};
+      uint64_t nu = 0;
+      if (level == 8 && s > b) fold_tot(T, b, s, tot, &nu, sh, (TotRaw*)stage);
+      if (threadIdx.x == 0) {
+        for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
+        t.nu[n] = nu;
+      }
+    } else if (s < e) {
+      double tot[4];
+      for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * n + c];
+      uint64_t nu = t.nu[n];
+      fold_tot(T, s, e, tot, &nu, sh, (TotRaw*)stage);
       if (threadIdx.x == 0) {
         for (int c = 0; c < 4; ++c) t.total[4 * n + c] = tot[c];
         t.nu[n] = nu;
@@ -1583,8 +2138,9 @@ class SubBuild {
     return (const double*)(pin_ + up_bytes() + 64 * (size_t)F_) + 4 * f;
   }
   const int32_t* hctl() const { return (const int32_t*)(pin_ + up_bytes() + 96 * (size_t)F_); }
+  static constexpr int kCtl = 16;
   size_t up_bytes() const { return sizeof(Frame) * F_ + sizeof(int64_t) * kMaps * (F_ + 1); }
-  size_t rb_bytes() const { return 96 * (size_t)F_ + 16; }
+  size_t rb_bytes() const { return 96 * (size_t)F_ + 4 * kCtl; }
   Map map(int kind) const { return Map{d_tab_ + (size_t)kind * (F_ + 1), F_}; }
 
   void fail(int f, int status, const char* fmt, ...) __attribute__((format(printf, 4, 5))) {
@@ -1807,7 +2363,7 @@ class SubBuild {
       co += d.ncap;
     }
     if ((e = upload()) != hipSuccess) return e;
-    fill(B_.ctl, 4, 0);
+    fill(B_.ctl, kCtl, 0);
     fill(nkeys, sumC, kEmpty64);
     fill(nfirst, sumC, kEmpty32);
     if (grid_[kMapP]) {
@@ -1896,7 +2452,9 @@ class SubBuild {
     c.want(&term, sumP);
     c.want(&lval, sumP);
     c.want(&lcnt, sumP);
+    c.want(&B_.mid_list, sumNodes);
     c.want(&B_.long_list, sumNodes);
+    c.want(&B_.huge_list, sumNodes);
     for (int k = 0; k < 11; ++k) c.want(&w[k], sumW);
     c.want(&lvl_off, 10 * (int64_t)F_);
     c.want(&tmp, (int64_t)std::max(tmpA, tmpB));
@@ -1939,7 +2497,7 @@ class SubBuild {
       }
     }
     if ((e = upload()) != hipSuccess) return e;
-    fill(B_.ctl, 4, 0);
+    fill(B_.ctl, kCtl, 0);
     if (E > 0) {
       e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpA, B_.ekey, skey, B_.eslot, sslot, (int)E, 0,
                                              bitsA, s_);
@@ -1958,12 +2516,17 @@ class SubBuild {
     k_segments<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
     if (grid_[kMapM256]) k_links<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256));
     k_pair_terms<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
-    const int long_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 2048);
+    // At most sumP / kShortSeg segments are medium, sumP / kMidSeg long.
+    const int mid_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 8192);
+    const int long_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kMidSeg, 1), 512);
     if (grid_[kMapM64]) k_fold_short<false><<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64), B_);
-    k_fold_long<false><<<long_grid, 64, 0, s_>>>(d_fr_, B_);
+    k_fold_long<kFoldQe><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
+    k_fold_long<kFoldTot1><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
+    k_fold_mid<false><<<mid_grid, 64, 0, s_>>>(d_fr_, B_);
     k_prune_cube<<<F_, kTreeThreads, 0, s_>>>(d_fr_);
     if (grid_[kMapM64]) k_fold_short<true><<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64), B_);
-    k_fold_long<true><<<long_grid, 64, 0, s_>>>(d_fr_, B_);
+    k_fold_long<kFoldTot2><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
+    k_fold_mid<true><<<mid_grid, 64, 0, s_>>>(d_fr_, B_);
     k_root_bound<<<F_, 256, 0, s_>>>(d_fr_);
     if (grid_[kMapM64]) k_fold_record<<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64));
     k_root_exact<<<F_, 64, 0, s_>>>(d_fr_);
