@@ -236,6 +236,35 @@ int sa_walk_approx(const sa_image* frames, int n, int iterations, int taps, int 
  * src/rd_approx_cuda.cu:73-88).  `g` holds taps+1 doubles; taps in 1..4096. */
 int sa_inverse_filter(int taps, double diffusion, double* g, char* err, size_t errlen);
 
+/* ---- frame chains (hipGraph) --------------------------------------------
+ * sa_walk_blocks or sa_walk_approx for a fixed batch of frame buffers,
+ * captured once into a hipGraph that owns its scratch, its copy of the curve
+ * and its weights.  A video loop that refills the same src buffers (and keeps
+ * the palettes) replays the whole batch with one sa_chain_run: the engine's
+ * launches without their per-call validation, uploads and allocations.  Each
+ * replay computes exactly what the eager call computes on the buffers'
+ * current contents.  New, with no counterpart in the reference (whose
+ * RiemersmaBlocksCuda / RiemersmaApproxCuda, include/rd_riemersma.h:450-464,
+ * :296-307, are one eager call per batch). */
+typedef struct sa_chain_s* sa_chain;
+
+/* Same arguments, validation and refusals as sa_walk_blocks.  `stream` orders
+ * the set-up copies; the graph itself is captured on a private stream. */
+int sa_chain_create_blocks(const sa_image* frames, int n, int block, double diffusion,
+                           void* stream, sa_chain* out, char* err, size_t errlen);
+
+/* Same arguments, validation and refusals as sa_walk_approx. */
+int sa_chain_create_approx(const sa_image* frames, int n, int iterations, int taps, int fp64,
+                           int use_tree, double diffusion, void* stream, sa_chain* out,
+                           char* err, size_t errlen);
+
+/* Replays the chain on `stream` (stream-ordered; the frames' buffers and
+ * palettes must be alive and not written by other work meanwhile). */
+int sa_chain_run(sa_chain chain, void* stream, char* err, size_t errlen);
+
+/* Frees the chain (waits for the device). */
+int sa_chain_destroy(sa_chain chain);
+
 /* ---- kernel timing --------------------------------------------------------- */
 
 /* Brackets every engine kernel launch with HIP events on its own stream while

@@ -28,6 +28,7 @@ from ._lib import (  # noqa: F401
     SA_EUNSUPPORTED,
     SA_OK,
     EngineError,
+    FrameChain,
     Palette,
     SeqOptions,
     WalkJob,
@@ -49,6 +50,7 @@ from ._lib import (  # noqa: F401
 
 __all__ = [
     "EngineError",
+    "FrameChain",
     "Palette",
     "SeqOptions",
     "available",

@@ -100,6 +100,12 @@ def lib():
                 "sa_walk_job_step_overlapped": (i32, [vp, dbl, vp, vp, i32, C.POINTER(i32),
                                                       C.POINTER(i32), cp, sz]),
                 "sa_walk_job_destroy": (i32, [vp]),
+                "sa_chain_create_blocks": (i32, [C.POINTER(_Image), i32, i32, dbl, vp,
+                                                 C.POINTER(vp), cp, sz]),
+                "sa_chain_create_approx": (i32, [C.POINTER(_Image), i32, i32, i32, i32, i32, dbl,
+                                                 vp, C.POINTER(vp), cp, sz]),
+                "sa_chain_run": (i32, [vp, vp, cp, sz]),
+                "sa_chain_destroy": (i32, [vp]),
                 "sa_timing_enable": (i32, [i32]),
                 "sa_timing_collect": (i32, [cp, C.POINTER(dbl), C.POINTER(i32), cp, sz]),
             }
@@ -403,6 +409,52 @@ def walk_approx(frames, iterations=20, taps=96, fp64=False, use_tree=True, diffu
     rc = lib().sa_walk_approx(arr, len(frames), iterations, taps, int(bool(fp64)),
                               int(bool(use_tree)), diffusion, _stream_ptr(stream), e, len(e))
     _check(rc, e)
+
+
+class FrameChain:
+    """An engine's work over a fixed batch of frame buffers, captured once
+    into a hipGraph (sa_chain_*): refill the src buffers, then run() replays
+    the batch with one launch."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def blocks(cls, frames, block=64, diffusion=1.0, stream=None):
+        arr = (_Image * len(frames))(*frames)
+        h = C.c_void_p()
+        e = _err()
+        rc = lib().sa_chain_create_blocks(arr, len(frames), block, diffusion, _stream_ptr(stream),
+                                          C.byref(h), e, len(e))
+        _check(rc, e)
+        return cls(h.value)
+
+    @classmethod
+    def approx(cls, frames, iterations=20, taps=96, fp64=False, use_tree=True, diffusion=1.0,
+               stream=None):
+        arr = (_Image * len(frames))(*frames)
+        h = C.c_void_p()
+        e = _err()
+        rc = lib().sa_chain_create_approx(arr, len(frames), iterations, taps, int(bool(fp64)),
+                                          int(bool(use_tree)), diffusion, _stream_ptr(stream),
+                                          C.byref(h), e, len(e))
+        _check(rc, e)
+        return cls(h.value)
+
+    def run(self, stream=None):
+        e = _err()
+        _check(lib().sa_chain_run(self.handle, _stream_ptr(stream), e, len(e)), e)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sa_chain_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def curve_order(width, height, stream=None):

@@ -1131,8 +1131,18 @@ extern "C" int sa_walk_job_destroy(sa_walk_job job) {
 }
 
 // ---- block engine ----------------------------------------------------------
-extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double diffusion,
-                              void* stream, char* err, size_t errlen) {
+namespace {
+// A validated block-engine call: the reference's refusals, the weights, the
+// block curve's length and gap cell, and the frame descriptors.
+struct BlocksPlan {
+  int64_t w = 0, h = 0, nb = 0, gap = -1;
+  int n = 0, block = 0, assoc = -1, max_img = 0;
+  const double* sw = nullptr;
+  std::vector<sa::BlockFrame> fr;
+};
+
+int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipStream_t s,
+                BlocksPlan& p, char* err, size_t errlen) {
   if (n < 1 || !frames)
     return fail(err, errlen, SA_EREFUSED,
                 "invalid dither batch: frames must be >= 1 and block must be >= 16 (got frames=%d, block=%d)",
@@ -1143,13 +1153,15 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
                 n, block);
   int rc = ensure_init(err, errlen);
   if (rc) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  const int64_t w = frames[0].width, h = frames[0].height;
-  int assoc = -1, max_img = 0;
+  p.n = n;
+  p.block = block;
+  p.w = frames[0].width;
+  p.h = frames[0].height;
   for (int i = 0; i < n; ++i) {
     const sa_image& f = frames[i];
     if (!f.src || !f.dst || !f.palette) return fail(err, errlen, SA_EINVAL, "frame %d: null pointer", i);
-    if (f.width != w || f.height != h) return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
+    if (f.width != p.w || f.height != p.h)
+      return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
     if (f.dst == f.src) return fail(err, errlen, SA_EINVAL, "frame %d: dst aliases src", i);
     if (!valid_in_fmt(f.src_fmt) || !valid_out_fmt(f.dst_fmt))
       return fail(err, errlen, SA_EINVAL, "frame %d: bad pixel format", i);
@@ -1157,44 +1169,63 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
       return fail(err, errlen, SA_EREFUSED,
                   "the blocks engine carries one palette index per byte, so the palette would alias "
                   "onto the first 256 (%d colours requested)", f.palette->count);
-    if (assoc < 0) assoc = f.palette->assoc;
-    if (assoc != f.palette->assoc)
+    if (p.assoc < 0) p.assoc = f.palette->assoc;
+    if (p.assoc != f.palette->assoc)
       return fail(err, errlen, SA_EUNSUPPORTED, "frames mix associated and plain alpha");
-    max_img = std::max(max_img, f.palette->hdr.bytes);
+    p.max_img = std::max(p.max_img, f.palette->hdr.bytes);
   }
-  if (w < 1 || h < 1) return fail(err, errlen, SA_EINVAL, "bad geometry");
-  const double* sw = nullptr;
-  rc = get_weights(diffusion, s, &sw, err, errlen);
+  if (p.w < 1 || p.h < 1) return fail(err, errlen, SA_EINVAL, "bad geometry");
+  rc = get_weights(diffusion, s, &p.sw, err, errlen);
   if (rc) return rc;
-  const int level = curve_level(w, h);
-  const int32_t* curve = nullptr;
-  rc = get_curve(w, h, s, &curve, err, errlen);
-  if (rc) return rc;
+  const int level = curve_level(p.w, p.h);
   // Block curve length: in-bounds of [0, 4^L - 1) plus the origin; the gap
   // cell (2^L-1, 0) is in bounds exactly when w == 2^L.
-  int64_t nb = 0, gap = -1;
   if (level > 0) {
-    const bool gap_in = w == ((int64_t)1 << level);
-    nb = w * h - (gap_in ? 1 : 0) + 1;
-    if (gap_in) gap = w - 1;
+    const bool gap_in = p.w == ((int64_t)1 << level);
+    p.nb = p.w * p.h - (gap_in ? 1 : 0) + 1;
+    if (gap_in) p.gap = p.w - 1;
   } else {
-    gap = 0;  // 1x1: the block curve is empty (src/rd_riemersma_cpu.cpp:275)
+    p.gap = 0;  // 1x1: the block curve is empty (src/rd_riemersma_cpu.cpp:275)
   }
-  std::vector<sa::BlockFrame> fr((size_t)n);
+  p.fr.resize((size_t)n);
   for (int i = 0; i < n; ++i) {
-    fr[(size_t)i].src = frames[i].src;
-    fr[(size_t)i].dst = frames[i].dst;
-    fr[(size_t)i].pal = frames[i].palette->dev_hdr;
-    fr[(size_t)i].src_fmt = frames[i].src_fmt;
-    fr[(size_t)i].dst_fmt = frames[i].dst_fmt;
+    p.fr[(size_t)i].src = frames[i].src;
+    p.fr[(size_t)i].dst = frames[i].dst;
+    p.fr[(size_t)i].pal = frames[i].palette->dev_hdr;
+    p.fr[(size_t)i].src_fmt = frames[i].src_fmt;
+    p.fr[(size_t)i].dst_fmt = frames[i].dst_fmt;
   }
+  return SA_OK;
+}
+
+// The block walk and the gap fill, launched on s (kernel launches only, so a
+// stream capture records exactly these).
+hipError_t launch_blocks_plan(const BlocksPlan& p, const sa::BlockFrame* d_fr,
+                              const int32_t* curve, const double* sw, hipStream_t s) {
+  hipError_t e = sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, sw, p.assoc,
+                                   p.max_img, s);
+  if (e == hipSuccess && p.gap >= 0) e = sa::launch_blocks_fill(d_fr, p.n, p.gap, s);
+  return e;
+}
+}  // namespace
+
+extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double diffusion,
+                              void* stream, char* err, size_t errlen) {
+  hipStream_t s = (hipStream_t)stream;
+  BlocksPlan p;
+  int rc = plan_blocks(frames, n, block, diffusion, s, p, err, errlen);
+  if (rc) return rc;
+  const int32_t* curve = nullptr;
+  rc = get_curve(p.w, p.h, s, &curve, err, errlen);
+  if (rc) return rc;
   sa::BlockFrame* d_fr = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::BlockFrame) * fr.size(), s));
-  HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::BlockFrame) * fr.size(), s));
+  HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::BlockFrame) * p.fr.size(), s));
+  HIP_TRY(staged_copy(d_fr, p.fr.data(), sizeof(sa::BlockFrame) * p.fr.size(), s));
   HIP_TRY(timed("blocks_walk", s, [&] {
-    return sa::launch_blocks(d_fr, n, curve, w, h, nb, block, sw, assoc, max_img, s);
+    return sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, p.sw, p.assoc, p.max_img,
+                             s);
   }));
-  if (gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, n, gap, s));
+  if (p.gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, p.n, p.gap, s));
   HIP_TRY(hipFreeAsync(d_fr, s));
   return SA_OK;
 }
@@ -1228,19 +1259,39 @@ extern "C" int sa_inverse_filter(int taps, double diffusion, double* g, char* er
   return SA_OK;
 }
 
-extern "C" int sa_walk_approx(const sa_image* frames, int n, int iterations, int taps, int fp64,
-                              int use_tree, double diffusion, void* stream, char* err,
-                              size_t errlen) {
+namespace {
+// A validated Jacobi call: refusals, the inverse filter, the curve length and
+// the scratch layout (descriptors, filter, iterate, two index buffers).
+struct ApproxPlan {
+  int64_t w = 0, h = 0, nb = 0;
+  bool gap_in = false;
+  int n = 0, iterations = 0, taps = 0, fp64 = 0, use_tree = 0, assoc = -1, max_img = 0;
+  std::vector<sa::BlockFrame> fr;
+  std::vector<double> gd;
+  std::vector<float> gf;
+  size_t b_fr = 0, b_g = 0, b_cx = 0, b_q = 0;
+  size_t scratch() const { return b_fr + b_g + b_cx + 2 * b_q; }
+  const void* g_host() const { return fp64 ? (const void*)gd.data() : (const void*)gf.data(); }
+  size_t g_bytes() const { return fp64 ? 8 * gd.size() : 4 * gf.size(); }
+};
+
+int plan_approx(const sa_image* frames, int n, int iterations, int taps, int fp64, int use_tree,
+                double diffusion, ApproxPlan& p, char* err, size_t errlen) {
   if (n < 1 || !frames) return fail(err, errlen, SA_EINVAL, "no frames");
   int rc = ensure_init(err, errlen);
   if (rc) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  const int64_t w = frames[0].width, h = frames[0].height;
-  int assoc = -1, max_img = 0;
+  p.n = n;
+  p.iterations = iterations;
+  p.taps = taps;
+  p.fp64 = fp64;
+  p.use_tree = use_tree;
+  p.w = frames[0].width;
+  p.h = frames[0].height;
   for (int i = 0; i < n; ++i) {
     const sa_image& f = frames[i];
     if (!f.src || !f.dst || !f.palette) return fail(err, errlen, SA_EINVAL, "frame %d: null pointer", i);
-    if (f.width != w || f.height != h) return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
+    if (f.width != p.w || f.height != p.h)
+      return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
     if (f.dst == f.src) return fail(err, errlen, SA_EINVAL, "frame %d: dst aliases src", i);
     if (!valid_in_fmt(f.src_fmt) || !valid_out_fmt(f.dst_fmt))
       return fail(err, errlen, SA_EINVAL, "frame %d: bad pixel format", i);
@@ -1249,58 +1300,197 @@ extern "C" int sa_walk_approx(const sa_image* frames, int n, int iterations, int
       return fail(err, errlen, SA_EREFUSED,
                   "the approx engine carries one palette index per byte, so %d colours would alias "
                   "onto the first 256", f.palette->count);
-    if (assoc < 0) assoc = f.palette->assoc;
-    if (assoc != f.palette->assoc)
+    if (p.assoc < 0) p.assoc = f.palette->assoc;
+    if (p.assoc != f.palette->assoc)
       return fail(err, errlen, SA_EUNSUPPORTED, "frames mix associated and plain alpha");
-    max_img = std::max(max_img, f.palette->hdr.bytes);
+    p.max_img = std::max(p.max_img, f.palette->hdr.bytes);
   }
   if (taps < 1 || taps > 4096) return fail(err, errlen, SA_EREFUSED, "--approx-taps out of range");
   if (iterations < 0) return fail(err, errlen, SA_EINVAL, "negative iterations");
-  const int level = curve_level(w, h);
-  const bool gap_in = level > 0 && w == ((int64_t)1 << level);
-  const int64_t nb = level > 0 ? w * h - (gap_in ? 1 : 0) + 1 : 0;
-  if (nb < w * h)  // src/rd_approx_cuda.cu:271-274 (a 1x1 image has an empty curve)
+  const int level = curve_level(p.w, p.h);
+  p.gap_in = level > 0 && p.w == ((int64_t)1 << level);
+  p.nb = level > 0 ? p.w * p.h - (p.gap_in ? 1 : 0) + 1 : 0;
+  if (p.nb < p.w * p.h)  // src/rd_approx_cuda.cu:271-274 (a 1x1 image has an empty curve)
     return fail(err, errlen, SA_EREFUSED,
                 "the compacted curve is shorter than the image; the Hilbert transcription is wrong");
-  const size_t lds = sa::approx_lds_bytes(max_img, taps, fp64, assoc);
+  const size_t lds = sa::approx_lds_bytes(p.max_img, taps, fp64, p.assoc);
   if (lds > g_lds_per_cu)
     return fail(err, errlen, SA_EUNSUPPORTED, "taps %d with this palette need %zu B of LDS", taps, lds);
-  const int32_t* curve = nullptr;
-  rc = get_curve(w, h, s, &curve, err, errlen);
-  if (rc) return rc;
-  std::vector<double> gd;
-  inverse_filter(taps, diffusion, gd);
-  std::vector<float> gf(gd.size());
-  for (size_t i = 0; i < gd.size(); ++i) gf[i] = (float)gd[i];
-  const size_t gbytes = fp64 ? 8 * gd.size() : 4 * gf.size();
+  inverse_filter(taps, diffusion, p.gd);
+  p.gf.resize(p.gd.size());
+  for (size_t i = 0; i < p.gd.size(); ++i) p.gf[i] = (float)p.gd[i];
   const size_t elem = fp64 ? 8 : 4;
-  std::vector<sa::BlockFrame> fr((size_t)n);
+  p.fr.resize((size_t)n);
   for (int i = 0; i < n; ++i) {
-    fr[(size_t)i].src = frames[i].src;
-    fr[(size_t)i].dst = frames[i].dst;
-    fr[(size_t)i].pal = frames[i].palette->dev_hdr;
-    fr[(size_t)i].src_fmt = frames[i].src_fmt;
-    fr[(size_t)i].dst_fmt = frames[i].dst_fmt;
+    p.fr[(size_t)i].src = frames[i].src;
+    p.fr[(size_t)i].dst = frames[i].dst;
+    p.fr[(size_t)i].pal = frames[i].palette->dev_hdr;
+    p.fr[(size_t)i].src_fmt = frames[i].src_fmt;
+    p.fr[(size_t)i].dst_fmt = frames[i].dst_fmt;
   }
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-  const size_t b_fr = up(sizeof(sa::BlockFrame) * fr.size()), b_g = up(gbytes);
-  const size_t b_cx = up(elem * 4 * (size_t)nb * (size_t)n), b_q = up((size_t)nb * (size_t)n);
-  unsigned char* scratch = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&scratch, b_fr + b_g + b_cx + 2 * b_q, s));
+  p.b_fr = up(sizeof(sa::BlockFrame) * p.fr.size());
+  p.b_g = up(p.g_bytes());
+  p.b_cx = up(elem * 4 * (size_t)p.nb * (size_t)n);
+  p.b_q = up((size_t)p.nb * (size_t)n);
+  return SA_OK;
+}
+
+// The sweeps and the gap fill on s, over a scratch laid out as the plan says
+// (kernel launches only).
+hipError_t launch_approx_plan(const ApproxPlan& p, unsigned char* scratch, const int32_t* curve,
+                              hipStream_t s) {
   sa::BlockFrame* d_fr = (sa::BlockFrame*)scratch;
-  void* d_g = scratch + b_fr;
-  void* d_cx = scratch + b_fr + b_g;
-  uint8_t* q0 = scratch + b_fr + b_g + b_cx;
-  uint8_t* q1 = q0 + b_q;
-  HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::BlockFrame) * fr.size(), s));
-  HIP_TRY(staged_copy(d_g, fp64 ? (const void*)gd.data() : (const void*)gf.data(), gbytes, s));
+  void* d_g = scratch + p.b_fr;
+  void* d_cx = scratch + p.b_fr + p.b_g;
+  uint8_t* q0 = scratch + p.b_fr + p.b_g + p.b_cx;
+  uint8_t* q1 = q0 + p.b_q;
   uint8_t* result = nullptr;
-  HIP_TRY(timed("approx", s, [&] {
-    return sa::launch_approx(d_fr, n, curve, nb, iterations, taps, d_g, fp64, use_tree, assoc,
-                             max_img, d_cx, q0, q1, &result, s);
-  }));
-  if (gap_in) HIP_TRY(sa::launch_blocks_fill(d_fr, n, w - 1, s));
+  hipError_t e = sa::launch_approx(d_fr, p.n, curve, p.nb, p.iterations, p.taps, d_g, p.fp64,
+                                   p.use_tree, p.assoc, p.max_img, d_cx, q0, q1, &result, s);
+  if (e == hipSuccess && p.gap_in) e = sa::launch_blocks_fill(d_fr, p.n, p.w - 1, s);
+  return e;
+}
+}  // namespace
+
+extern "C" int sa_walk_approx(const sa_image* frames, int n, int iterations, int taps, int fp64,
+                              int use_tree, double diffusion, void* stream, char* err,
+                              size_t errlen) {
+  hipStream_t s = (hipStream_t)stream;
+  ApproxPlan p;
+  int rc = plan_approx(frames, n, iterations, taps, fp64, use_tree, diffusion, p, err, errlen);
+  if (rc) return rc;
+  const int32_t* curve = nullptr;
+  rc = get_curve(p.w, p.h, s, &curve, err, errlen);
+  if (rc) return rc;
+  unsigned char* scratch = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&scratch, p.scratch(), s));
+  HIP_TRY(staged_copy(scratch, p.fr.data(), sizeof(sa::BlockFrame) * p.fr.size(), s));
+  HIP_TRY(staged_copy(scratch + p.b_fr, p.g_host(), p.g_bytes(), s));
+  HIP_TRY(timed("approx", s, [&] { return launch_approx_plan(p, scratch, curve, s); }));
   HIP_TRY(hipFreeAsync(scratch, s));
+  return SA_OK;
+}
+
+// ---- frame chains (hipGraph) -----------------------------------------------------------
+// One engine's work for a fixed set of frame buffers, captured once into a
+// graph that owns its scratch, curve and weights; a video loop that refills
+// the same buffers replays it per batch of frames (one launch on the host).
+struct sa_chain_s {
+  void* mem = nullptr;  // descriptors, curve copy, weights, engine scratch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+template <typename Launch>
+int capture_chain(sa_chain c, Launch&& launch, char* err, size_t errlen) {
+  hipStream_t cs = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    const hipError_t le = launch(cs);
+    e = hipStreamEndCapture(cs, &c->graph);
+    if (le != hipSuccess) e = le;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess)
+    return fail(err, errlen, SA_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+  return SA_OK;
+}
+}  // namespace
+
+extern "C" int sa_chain_destroy(sa_chain c) {
+  if (!c) return SA_OK;
+  (void)hipDeviceSynchronize();
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->mem) (void)hipFree(c->mem);
+  delete c;
+  return SA_OK;
+}
+
+extern "C" int sa_chain_create_blocks(const sa_image* frames, int n, int block, double diffusion,
+                                      void* stream, sa_chain* out, char* err, size_t errlen) {
+  if (!out) return fail(err, errlen, SA_EINVAL, "null output handle");
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  BlocksPlan p;
+  int rc = plan_blocks(frames, n, block, diffusion, s, p, err, errlen);
+  if (rc) return rc;
+  const int32_t* curve = nullptr;
+  rc = get_curve(p.w, p.h, s, &curve, err, errlen);
+  if (rc) return rc;
+  const size_t b_fr = (sizeof(sa::BlockFrame) * p.fr.size() + 255) & ~(size_t)255;
+  const size_t b_sw = 256, b_curve = sizeof(int32_t) * (size_t)std::max<int64_t>(p.w * p.h, 1);
+  sa_chain c = new sa_chain_s();
+  auto bail = [&](int code) {
+    sa_chain_destroy(c);
+    return code;
+  };
+  if (hipMalloc(&c->mem, b_fr + b_sw + b_curve) != hipSuccess)
+    return bail(fail(err, errlen, SA_ENOMEM, "chain: device allocation failed"));
+  unsigned char* m = (unsigned char*)c->mem;
+  sa::BlockFrame* d_fr = (sa::BlockFrame*)m;
+  double* d_sw = (double*)(m + b_fr);
+  int32_t* d_curve = (int32_t*)(m + b_fr + b_sw);
+  hipError_t e = hipMemcpyAsync(d_fr, p.fr.data(), sizeof(sa::BlockFrame) * p.fr.size(),
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_sw, p.sw, 16 * sizeof(double), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess && p.w * p.h > 1)
+    e = hipMemcpyAsync(d_curve, curve, sizeof(int32_t) * (size_t)(p.w * p.h), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess)
+    return bail(fail(err, errlen, SA_EDEVICE, "chain setup: %s", hipGetErrorString(e)));
+  rc = capture_chain(c, [&](hipStream_t cs) { return launch_blocks_plan(p, d_fr, d_curve, d_sw, cs); },
+                     err, errlen);
+  if (rc) return bail(rc);
+  *out = c;
+  return SA_OK;
+}
+
+extern "C" int sa_chain_create_approx(const sa_image* frames, int n, int iterations, int taps,
+                                      int fp64, int use_tree, double diffusion, void* stream,
+                                      sa_chain* out, char* err, size_t errlen) {
+  if (!out) return fail(err, errlen, SA_EINVAL, "null output handle");
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  ApproxPlan p;
+  int rc = plan_approx(frames, n, iterations, taps, fp64, use_tree, diffusion, p, err, errlen);
+  if (rc) return rc;
+  const int32_t* curve = nullptr;
+  rc = get_curve(p.w, p.h, s, &curve, err, errlen);
+  if (rc) return rc;
+  const size_t b_curve = ((sizeof(int32_t) * (size_t)(p.w * p.h)) + 255) & ~(size_t)255;
+  sa_chain c = new sa_chain_s();
+  auto bail = [&](int code) {
+    sa_chain_destroy(c);
+    return code;
+  };
+  if (hipMalloc(&c->mem, p.scratch() + b_curve) != hipSuccess)
+    return bail(fail(err, errlen, SA_ENOMEM, "chain: device allocation failed"));
+  unsigned char* scratch = (unsigned char*)c->mem;
+  int32_t* d_curve = (int32_t*)(scratch + p.scratch());
+  hipError_t e = hipMemcpyAsync(scratch, p.fr.data(), sizeof(sa::BlockFrame) * p.fr.size(),
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(scratch + p.b_fr, p.g_host(), p.g_bytes(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_curve, curve, sizeof(int32_t) * (size_t)(p.w * p.h), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess)
+    return bail(fail(err, errlen, SA_EDEVICE, "chain setup: %s", hipGetErrorString(e)));
+  rc = capture_chain(c, [&](hipStream_t cs) { return launch_approx_plan(p, scratch, d_curve, cs); },
+                     err, errlen);
+  if (rc) return bail(rc);
+  *out = c;
+  return SA_OK;
+}
+
+extern "C" int sa_chain_run(sa_chain c, void* stream, char* err, size_t errlen) {
+  if (!c || !c->exec) return fail(err, errlen, SA_EINVAL, "null chain");
+  HIP_TRY(hipGraphLaunch(c->exec, (hipStream_t)stream));
   return SA_OK;
 }
 

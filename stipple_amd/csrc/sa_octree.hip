@@ -2329,6 +2329,7 @@ class SubBuild {
         bound_nodes += std::min<int64_t>(d.R, pw);
       }
       bound_nodes = std::min<int64_t>(bound_nodes, P + 1);
+      bound_[f] = bound_nodes;
       d.P = P;
       d.ncap = pow2_at_least(2 * bound_nodes + 64);
       d.pair_base = sumP;
@@ -2371,6 +2372,27 @@ class SubBuild {
       k_node_lookup<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
       k_node_collect<<<grid_[kMapCap], 256, 0, s_>>>(d_fr_, map(kMapCap), B_);
     }
+    // The outputs outlive the batch (plain allocations owned by the caller):
+    // allocated while the node kernels run, sized by the node bound.
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (!d.ok) continue;
+      const PaletteLayout cap =
+          palette_layout(d.maxc, (int)std::min<int64_t>(bound_[f] + 1, 32767), d.kids);
+      const size_t img = align256((size_t)cap.bytes), hdr = align256(sizeof(PaletteDev));
+      void* blk = nullptr;
+      if ((e = hipMalloc(&blk, img + hdr + sizeof(double) * 4 * (size_t)d.maxc)) != hipSuccess)
+        return e;
+      OctreeOut& o = out_[f];
+      o.block = blk;
+      o.image = (unsigned char*)blk;
+      o.hdr = (PaletteDev*)((char*)blk + img);
+      o.entries = (double*)((char*)blk + img + hdr);
+      d.pb.image = o.image;
+      d.pb.image_cap = cap.bytes;
+      d.pb.hdr = o.hdr;
+      d.pb.entries = o.entries;
+    }
     return readback();
   }
 
@@ -2390,26 +2412,6 @@ class SubBuild {
       sumNodes += d.ok ? d.M + 1 : 1;
     }
     if (sumP == 0) return readback();
-    // Outputs first: they outlive the batch (plain allocations owned by the caller).
-    for (int f = 0; f < F_; ++f) {
-      Frame& d = H_[f];
-      if (!d.ok) continue;
-      const PaletteLayout cap =
-          palette_layout(d.maxc, (int)std::min<int64_t>(d.M + 1, 32767), d.kids);
-      const size_t img = align256((size_t)cap.bytes), hdr = align256(sizeof(PaletteDev));
-      void* blk = nullptr;
-      const hipError_t e = hipMalloc(&blk, img + hdr + sizeof(double) * 4 * (size_t)d.maxc);
-      if (e != hipSuccess) return e;
-      OctreeOut& o = out_[f];
-      o.block = blk;
-      o.image = (unsigned char*)blk;
-      o.hdr = (PaletteDev*)((char*)blk + img);
-      o.entries = (double*)((char*)blk + img + hdr);
-      d.pb.image = o.image;
-      d.pb.image_cap = cap.bytes;
-      d.pb.hdr = o.hdr;
-      d.pb.entries = o.entries;
-    }
     for (int f = 0; f < F_; ++f)
       if (H_[f].ok) sumW += H_[f].M + 2;
     const int bitsA = bits_for(sumP), bitsB = bits_for(sumNodes);
@@ -2599,6 +2601,7 @@ class SubBuild {
 
   int64_t sumP_ = 0;
   int32_t* slot_node_ = nullptr;
+  std::vector<int64_t> bound_ = std::vector<int64_t>(F_, 0);  // node bound per image
 };
 
 template <typename T>
