@@ -185,6 +185,16 @@ def run_video(args, sa, torch, synth, dev):
         torch.cuda.synchronize()
         return pals
 
+    tuned = None
+    if not args.no_tune:
+        from stipple_amd import tune
+        tp = build()
+        tspecs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, tp[k])
+                  for k in range(nf)]
+        tuned = tune.tune("blocks.threads", lambda: sa.walk_blocks(tspecs, block=64),
+                          outputs=lambda: dsts)
+        log(f"tuned {tuned}")
+
     for _ in range(args.warmup):
         step()
     t0 = time.perf_counter()
@@ -237,7 +247,7 @@ def run_video(args, sa, torch, synth, dev):
                  {"workload": f"{nf} 1920x1080 RGB8 plasma frames, a 16-colour octree palette per "
                               "frame built on the device, block walk B=64, RGB8 out",
                   "frames": nf, "device": dev}, roof, cpu,
-                 {"fps": round(nf / t, 2), "walk_ms": round(walk_ms, 3),
+                 {"fps": round(nf / t, 2), "walk_ms": round(walk_ms, 3), "tuned": tuned,
                   "walk_fps": round(nf / (walk_ms * 1e-3), 1), "palette_build_ms": round(pal_ms, 3),
                   "plain": {"fps": round(nf / sum(lat), 2),
                             "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
@@ -259,7 +269,21 @@ def run_approx4k(args, sa, torch, synth, dev):
     specs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB16, sa.PIX_RGB16, pal)
              for k in range(nf)]
 
+    tuned = None
+    if not args.no_tune:
+        from stipple_amd import tune
+        tuned = tune.tune("approx.per", lambda: sa.walk_approx(specs, iterations=8, taps=96),
+                          outputs=lambda: dsts, reps=2)
+        log(f"tuned {tuned}")
+    # The per-frame chain with the palette resident: captured once as a graph
+    # over the frame buffers and replayed each step.
+    chain = sa.FrameChain.approx(specs, iterations=8, taps=96)
+
     def step():
+        chain.run()
+        torch.cuda.synchronize()
+
+    def eager():
         sa.walk_approx(specs, iterations=8, taps=96)
         torch.cuda.synchronize()
 
@@ -269,11 +293,15 @@ def run_approx4k(args, sa, torch, synth, dev):
     for _ in range(args.steps):
         step()
     t = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    eager()
+    t_eager = time.perf_counter() - t0
     sa.timing_enable(True)
     sa.timing_collect()
-    step()
+    eager()
     ap_ms, _ = sa.timing_collect("approx")
     sa.timing_enable(False)
+    chain.close()
     px = w * h
     cpu = None
     if not args.no_cpu:
@@ -305,6 +333,8 @@ def run_approx4k(args, sa, torch, synth, dev):
                               "Jacobi 8 sweeps, taps 96, fp32, octree lookup",
                   "frames": nf, "device": dev}, roof, cpu,
                  {"fps": round(nf / t, 2), "approx_ms": round(ap_ms, 3),
+                  "chain": "hipGraph replay of the 16-frame batch (sa_chain_run)", "tuned": tuned,
+                  "eager_ms": round(t_eager * 1e3, 3),
                   "tolerance": "0 differing indices against the CPU restatement (bit-exact)"})
 
 
@@ -317,6 +347,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-image-per-launch leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--build-threads", type=int, default=8,
                     help="host threads (one stream each) classifying palettes concurrently")
     ap.add_argument("--chain", choices=["pipeline", "batch"], default="pipeline",

@@ -265,6 +265,18 @@ int sa_chain_run(sa_chain chain, void* stream, char* err, size_t errlen);
 /* Frees the chain (waits for the device). */
 int sa_chain_destroy(sa_chain chain);
 
+/* ---- tuning knobs -------------------------------------------------------
+ * Launch shapes an autotuner may set (stipple_amd/tune.py measures them);
+ * no knob changes a result.  New, with no counterpart in the reference.
+ *   "blocks.threads"  block walk: Hilbert blocks per workgroup (64..512; 256)
+ *   "approx.per"      Jacobi sweep: outputs per thread (2, 4, 8; 4)
+ *   "seq.memo_bits"   sequential walk: LDS memo slots, log2 (4..10; 0 = the
+ *                     largest that fits, which leaves one image per CU)
+ * Calls made after a set use the new value; chains keep the value they were
+ * created with. */
+int sa_tune_set(const char* knob, int value, char* err, size_t errlen);
+int sa_tune_get(const char* knob, int* value);
+
 /* ---- kernel timing --------------------------------------------------------- */
 
 /* Brackets every engine kernel launch with HIP events on its own stream while

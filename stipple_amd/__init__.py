@@ -42,6 +42,8 @@ from ._lib import (  # noqa: F401
     seq_stats,
     timing_collect,
     timing_enable,
+    tune_get,
+    tune_set,
     library_path,
     walk_approx,
     walk_blocks,

@@ -106,6 +106,8 @@ def lib():
                                                  vp, C.POINTER(vp), cp, sz]),
                 "sa_chain_run": (i32, [vp, vp, cp, sz]),
                 "sa_chain_destroy": (i32, [vp]),
+                "sa_tune_set": (i32, [cp, i32, cp, sz]),
+                "sa_tune_get": (i32, [cp, C.POINTER(i32)]),
                 "sa_timing_enable": (i32, [i32]),
                 "sa_timing_collect": (i32, [cp, C.POINTER(dbl), C.POINTER(i32), cp, sz]),
             }
@@ -465,6 +467,20 @@ def curve_order(width, height, stream=None):
     rc = lib().sa_curve_order(width, height, out.data_ptr(), _stream_ptr(stream), e, len(e))
     _check(rc, e)
     return out
+
+
+def tune_set(knob, value):
+    """Sets a launch-shape knob (sa_tune_set); no knob changes a result."""
+    e = _err()
+    _check(lib().sa_tune_set(knob.encode(), int(value), e, len(e)), e)
+
+
+def tune_get(knob):
+    v = C.c_int()
+    rc = lib().sa_tune_get(knob.encode(), C.byref(v))
+    if rc != 0:
+        raise EngineError(rc, f"unknown knob {knob}")
+    return v.value
 
 
 def timing_enable(on=True):

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -42,6 +43,15 @@ struct sa_palette_s {
 };
 
 namespace {
+
+// ---- tuning knobs (set by an autotuner; none of them changes a result) ------
+struct Knobs {
+  std::atomic<int> blocks_threads{256};  // block walk: blocks per workgroup
+  std::atomic<int> approx_per{4};        // Jacobi sweep: outputs per thread
+  std::atomic<int> seq_memo_bits{0};     // sequential walk: memo slots (log2); 0 = largest fit
+};
+Knobs g_knobs;
+
 
 using sa::PaletteDev;
 
@@ -642,6 +652,8 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     if (S <= 0) {
       S = 10;
       while (S > 4 && (132u << S) + (size_t)max_img > 158 * 1024) --S;
+      const int tuned = g_knobs.seq_memo_bits.load();
+      if (tuned > 0 && tuned < S) S = tuned;
     }
     if (S < 4 || S > 10) return fail(err, errlen, SA_EINVAL, "memo_bits %d outside 4..10", S);
     const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)max_img;
@@ -719,6 +731,8 @@ int memo_bits_for(const sa_seq_options& o, int img_bytes) {
   if (o.memo_bits > 0) return o.memo_bits;
   int S = 10;
   while (S > 4 && (132u << S) + (size_t)img_bytes > 158 * 1024) --S;
+  const int tuned = g_knobs.seq_memo_bits.load();
+  if (tuned > 0 && tuned < S) S = tuned;
   return S;
 }
 
@@ -1137,6 +1151,7 @@ namespace {
 struct BlocksPlan {
   int64_t w = 0, h = 0, nb = 0, gap = -1;
   int n = 0, block = 0, assoc = -1, max_img = 0;
+  int threads = 256;  // tuned launch shape
   const double* sw = nullptr;
   std::vector<sa::BlockFrame> fr;
 };
@@ -1155,6 +1170,7 @@ int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipS
   if (rc) return rc;
   p.n = n;
   p.block = block;
+  p.threads = g_knobs.blocks_threads.load();
   p.w = frames[0].width;
   p.h = frames[0].height;
   for (int i = 0; i < n; ++i) {
@@ -1203,7 +1219,7 @@ int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipS
 hipError_t launch_blocks_plan(const BlocksPlan& p, const sa::BlockFrame* d_fr,
                               const int32_t* curve, const double* sw, hipStream_t s) {
   hipError_t e = sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, sw, p.assoc,
-                                   p.max_img, s);
+                                   p.max_img, p.threads, s);
   if (e == hipSuccess && p.gap >= 0) e = sa::launch_blocks_fill(d_fr, p.n, p.gap, s);
   return e;
 }
@@ -1223,7 +1239,7 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
   HIP_TRY(staged_copy(d_fr, p.fr.data(), sizeof(sa::BlockFrame) * p.fr.size(), s));
   HIP_TRY(timed("blocks_walk", s, [&] {
     return sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, p.sw, p.assoc, p.max_img,
-                             s);
+                             p.threads, s);
   }));
   if (p.gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, p.n, p.gap, s));
   HIP_TRY(hipFreeAsync(d_fr, s));
@@ -1266,6 +1282,7 @@ struct ApproxPlan {
   int64_t w = 0, h = 0, nb = 0;
   bool gap_in = false;
   int n = 0, iterations = 0, taps = 0, fp64 = 0, use_tree = 0, assoc = -1, max_img = 0;
+  int per = 4;  // tuned launch shape
   std::vector<sa::BlockFrame> fr;
   std::vector<double> gd;
   std::vector<float> gf;
@@ -1285,6 +1302,7 @@ int plan_approx(const sa_image* frames, int n, int iterations, int taps, int fp6
   p.taps = taps;
   p.fp64 = fp64;
   p.use_tree = use_tree;
+  p.per = g_knobs.approx_per.load();
   p.w = frames[0].width;
   p.h = frames[0].height;
   for (int i = 0; i < n; ++i) {
@@ -1313,7 +1331,7 @@ int plan_approx(const sa_image* frames, int n, int iterations, int taps, int fp6
   if (p.nb < p.w * p.h)  // src/rd_approx_cuda.cu:271-274 (a 1x1 image has an empty curve)
     return fail(err, errlen, SA_EREFUSED,
                 "the compacted curve is shorter than the image; the Hilbert transcription is wrong");
-  const size_t lds = sa::approx_lds_bytes(p.max_img, taps, fp64, p.assoc);
+  const size_t lds = sa::approx_lds_bytes(p.max_img, taps, fp64, p.assoc, p.per);
   if (lds > g_lds_per_cu)
     return fail(err, errlen, SA_EUNSUPPORTED, "taps %d with this palette need %zu B of LDS", taps, lds);
   inverse_filter(taps, diffusion, p.gd);
@@ -1347,7 +1365,8 @@ hipError_t launch_approx_plan(const ApproxPlan& p, unsigned char* scratch, const
   uint8_t* q1 = q0 + p.b_q;
   uint8_t* result = nullptr;
   hipError_t e = sa::launch_approx(d_fr, p.n, curve, p.nb, p.iterations, p.taps, d_g, p.fp64,
-                                   p.use_tree, p.assoc, p.max_img, d_cx, q0, q1, &result, s);
+                                   p.use_tree, p.assoc, p.max_img, d_cx, q0, q1, &result, p.per,
+                                   s);
   if (e == hipSuccess && p.gap_in) e = sa::launch_blocks_fill(d_fr, p.n, p.w - 1, s);
   return e;
 }
@@ -1491,6 +1510,36 @@ extern "C" int sa_chain_create_approx(const sa_image* frames, int n, int iterati
 extern "C" int sa_chain_run(sa_chain c, void* stream, char* err, size_t errlen) {
   if (!c || !c->exec) return fail(err, errlen, SA_EINVAL, "null chain");
   HIP_TRY(hipGraphLaunch(c->exec, (hipStream_t)stream));
+  return SA_OK;
+}
+
+// ---- tuning -----------------------------------------------------------------------
+extern "C" int sa_tune_set(const char* knob, int value, char* err, size_t errlen) {
+  if (!knob) return fail(err, errlen, SA_EINVAL, "null knob");
+  if (!std::strcmp(knob, "blocks.threads")) {
+    if (value != 64 && value != 128 && value != 256 && value != 512)
+      return fail(err, errlen, SA_EINVAL, "blocks.threads must be 64, 128, 256 or 512");
+    g_knobs.blocks_threads = value;
+  } else if (!std::strcmp(knob, "approx.per")) {
+    if (value != 2 && value != 4 && value != 8)
+      return fail(err, errlen, SA_EINVAL, "approx.per must be 2, 4 or 8");
+    g_knobs.approx_per = value;
+  } else if (!std::strcmp(knob, "seq.memo_bits")) {
+    if (value != 0 && (value < 4 || value > 10))
+      return fail(err, errlen, SA_EINVAL, "seq.memo_bits must be 0 (largest fit) or 4..10");
+    g_knobs.seq_memo_bits = value;
+  } else {
+    return fail(err, errlen, SA_EINVAL, "unknown knob '%s'", knob);
+  }
+  return SA_OK;
+}
+
+extern "C" int sa_tune_get(const char* knob, int* value) {
+  if (!knob || !value) return SA_EINVAL;
+  if (!std::strcmp(knob, "blocks.threads")) *value = g_knobs.blocks_threads;
+  else if (!std::strcmp(knob, "approx.per")) *value = g_knobs.approx_per;
+  else if (!std::strcmp(knob, "seq.memo_bits")) *value = g_knobs.seq_memo_bits;
+  else return SA_EINVAL;
   return SA_OK;
 }
 

@@ -26,7 +26,6 @@
 namespace sa {
 namespace {
 
-constexpr int kThreads = 256;
 
 __device__ __forceinline__ void store_out(const BlockFrame& f, int64_t pix, int idx,
                                           const float4& e, bool assoc) {
@@ -162,8 +161,8 @@ __device__ __forceinline__ void block_group(Queue<ASSOC>& q, const double* __res
    ...);
 }
 
-template <bool ASSOC>
-__global__ __launch_bounds__(kThreads) void blocks_kernel(const BlockFrame* __restrict__ frames,
+template <bool ASSOC, int NT>
+__global__ __launch_bounds__(NT) void blocks_kernel(const BlockFrame* __restrict__ frames,
                                                           const int32_t* __restrict__ curve,
                                                           int64_t nb, int block, int64_t nblocks,
                                                           const double* __restrict__ sw) {
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void blocks_kernel(const BlockFrame* __re
   {
     const uint4* src = (const uint4*)pd.image;
     uint4* dst = (uint4*)lds;
-    for (int k = threadIdx.x; k < pd.bytes / 16; k += kThreads) dst[k] = src[k];
+    for (int k = threadIdx.x; k < pd.bytes / 16; k += NT) dst[k] = src[k];
     __syncthreads();
   }
   LSearch s;
@@ -183,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void blocks_kernel(const BlockFrame* __re
   s.child = (const int16_t*)(lds + pd.off_child);
   s.cand = (const uint16_t*)(lds + pd.off_cand);
   s.kids = pd.kids;
-  const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t b = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (b >= nblocks) return;
   const int64_t begin = b * block;
   const int64_t end = begin + block < nb ? begin + block : nb;
@@ -243,20 +242,34 @@ __global__ void blocks_fill_kernel(const BlockFrame* __restrict__ frames, int nf
 
 }  // namespace
 
+template <int NT>
+void launch_blocks_nt(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
+                      int block, int64_t nblocks, const double* sw, int assoc,
+                      int lds_image_bytes, hipStream_t s) {
+  const dim3 grid((unsigned)((nblocks + NT - 1) / NT), (unsigned)nframes);
+  if (assoc)
+    blocks_kernel<true, NT><<<grid, dim3(NT), (size_t)lds_image_bytes, s>>>(frames, curve, nb,
+                                                                         block, nblocks, sw);
+  else
+    blocks_kernel<false, NT><<<grid, dim3(NT), (size_t)lds_image_bytes, s>>>(frames, curve, nb,
+                                                                          block, nblocks, sw);
+}
+
+// `threads` (64, 128, 256 or 512) blocks per workgroup: every workgroup stages
+// the palette once, so larger ones amortise it and smaller ones fill more CUs.
 hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,
                          int64_t h, int64_t nb, int block, const double* sw, int assoc,
-                         int lds_image_bytes, hipStream_t s) {
+                         int lds_image_bytes, int threads, hipStream_t s) {
   (void)w;
   (void)h;
   if (nframes <= 0 || nb <= 0) return hipSuccess;
   const int64_t nblocks = (nb + block - 1) / block;
-  const dim3 grid((unsigned)((nblocks + kThreads - 1) / kThreads), (unsigned)nframes);
-  if (assoc)
-    blocks_kernel<true><<<grid, dim3(kThreads), (size_t)lds_image_bytes, s>>>(frames, curve, nb,
-                                                                           block, nblocks, sw);
-  else
-    blocks_kernel<false><<<grid, dim3(kThreads), (size_t)lds_image_bytes, s>>>(frames, curve, nb,
-                                                                            block, nblocks, sw);
+  switch (threads) {
+    case 64: launch_blocks_nt<64>(frames, nframes, curve, nb, block, nblocks, sw, assoc, lds_image_bytes, s); break;
+    case 128: launch_blocks_nt<128>(frames, nframes, curve, nb, block, nblocks, sw, assoc, lds_image_bytes, s); break;
+    case 512: launch_blocks_nt<512>(frames, nframes, curve, nb, block, nblocks, sw, assoc, lds_image_bytes, s); break;
+    default: launch_blocks_nt<256>(frames, nframes, curve, nb, block, nblocks, sw, assoc, lds_image_bytes, s); break;
+  }
   return hipGetLastError();
 }
 

@@ -134,14 +134,14 @@ hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipS
 
 hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,
                          int64_t h, int64_t nb, int block, const double* sw, int assoc,
-                         int lds_image_bytes, hipStream_t s);
+                         int lds_image_bytes, int threads, hipStream_t s);
 hipError_t launch_blocks_fill(const BlockFrame* frames, int nframes, int64_t gap_pixel,
                               hipStream_t s);
 
-size_t approx_lds_bytes(int img_bytes, int taps, int fp64, int assoc);
+size_t approx_lds_bytes(int img_bytes, int taps, int fp64, int assoc, int per);
 hipError_t launch_approx(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
                          int iterations, int taps, const void* g, int fp64, int use_tree,
                          int assoc, int img_bytes, void* cx, uint8_t* q0, uint8_t* q1,
-                         uint8_t** result, hipStream_t s);
+                         uint8_t** result, int per, hipStream_t s);
 
 }  // namespace sa

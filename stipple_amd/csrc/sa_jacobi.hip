@@ -27,8 +27,9 @@ namespace sa {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPer = 4;
-constexpr int kTile = kThreads * kPer;
+// Outputs per thread of a sweep (its tile is kThreads * per outputs): 2, 4 or
+// 8; 4 unless tuned.
+constexpr int kPerDefault = 4;
 
 struct JSearch {
   const double* col;
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void k_init(const BlockFrame* __restrict_
 }
 
 // One sweep (JacobiStepKernel, src/rd_approx_cuda.cu:158-228).
-template <typename T, bool ASSOC>
+template <typename T, bool ASSOC, int kPer>
 __global__ __launch_bounds__(kThreads) void k_sweep(const BlockFrame* __restrict__ frames,
                                                     const T* __restrict__ cx,
                                                     const uint8_t* __restrict__ q_in,
@@ -166,6 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(const BlockFrame* __restrict
   const PaletteDev pd = *f.pal;
   const JSearch s = stage(pd, lds);
   __syncthreads();  // the tile below reads the staged entries
+  constexpr int kTile = kThreads * kPer;
   const int window = kTile + taps;
   T* r = (T*)(lds + img_bytes);  // C planes of `window` values
   const int64_t fbase = (int64_t)blockIdx.y * nb;
@@ -324,10 +326,11 @@ hipError_t allow_lds(K kernel, size_t bytes) {
                              (int)bytes);
 }
 
-template <typename T, bool ASSOC>
+template <typename T, bool ASSOC, int kPer>
 hipError_t run(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
                int iterations, int taps, const T* g, int use_tree, int img_bytes, T* cx,
                uint8_t* q0, uint8_t* q1, uint8_t** result, hipStream_t s) {
+  constexpr int kTile = kThreads * kPer;
   const dim3 grid_pos((unsigned)((nb + kThreads - 1) / kThreads), (unsigned)nframes);
   k_gather<T><<<grid_pos, kThreads, 0, s>>>(frames, curve, nb, cx);
   hipError_t e = hipGetLastError();
@@ -338,12 +341,12 @@ hipError_t run(const BlockFrame* frames, int nframes, const int32_t* curve, int6
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t lds = (size_t)img_bytes + sizeof(T) * (ASSOC ? 4 : 3) * (size_t)(kTile + taps);
-  e = allow_lds(k_sweep<T, ASSOC>, lds);
+  e = allow_lds(k_sweep<T, ASSOC, kPer>, lds);
   if (e != hipSuccess) return e;
   const dim3 grid_tile((unsigned)((nb + kTile - 1) / kTile), (unsigned)nframes);
   for (int it = 0; it < iterations; ++it) {
-    k_sweep<T, ASSOC><<<grid_tile, kThreads, lds, s>>>(frames, cx, q0, q1, g, taps, nb, use_tree,
-                                                       img_bytes);
+    k_sweep<T, ASSOC, kPer><<<grid_tile, kThreads, lds, s>>>(frames, cx, q0, q1, g, taps, nb,
+                                                             use_tree, img_bytes);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint8_t* t = q0;
@@ -357,27 +360,48 @@ hipError_t run(const BlockFrame* frames, int nframes, const int32_t* curve, int6
 
 }  // namespace
 
-size_t approx_lds_bytes(int img_bytes, int taps, int fp64, int assoc) {
-  return (size_t)img_bytes + (fp64 ? 8 : 4) * (size_t)(assoc ? 4 : 3) * (size_t)(kTile + taps);
+size_t approx_lds_bytes(int img_bytes, int taps, int fp64, int assoc, int per) {
+  const int tile = kThreads * (per == 2 || per == 8 ? per : kPerDefault);
+  return (size_t)img_bytes + (fp64 ? 8 : 4) * (size_t)(assoc ? 4 : 3) * (size_t)(tile + taps);
+}
+
+template <int kPer>
+hipError_t run_per(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
+                   int iterations, int taps, const void* g, int fp64, int use_tree, int assoc,
+                   int img_bytes, void* cx, uint8_t* q0, uint8_t* q1, uint8_t** result,
+                   hipStream_t s) {
+  if (fp64) {
+    return assoc ? run<double, true, kPer>(frames, nframes, curve, nb, iterations, taps,
+                                           (const double*)g, use_tree, img_bytes, (double*)cx,
+                                           q0, q1, result, s)
+                 : run<double, false, kPer>(frames, nframes, curve, nb, iterations, taps,
+                                            (const double*)g, use_tree, img_bytes, (double*)cx,
+                                            q0, q1, result, s);
+  }
+  return assoc ? run<float, true, kPer>(frames, nframes, curve, nb, iterations, taps,
+                                        (const float*)g, use_tree, img_bytes, (float*)cx, q0, q1,
+                                        result, s)
+               : run<float, false, kPer>(frames, nframes, curve, nb, iterations, taps,
+                                         (const float*)g, use_tree, img_bytes, (float*)cx, q0, q1,
+                                         result, s);
 }
 
 hipError_t launch_approx(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
                          int iterations, int taps, const void* g, int fp64, int use_tree,
                          int assoc, int img_bytes, void* cx, uint8_t* q0, uint8_t* q1,
-                         uint8_t** result, hipStream_t s) {
+                         uint8_t** result, int per, hipStream_t s) {
   if (nframes <= 0 || nb <= 0) return hipSuccess;
-  if (fp64) {
-    return assoc ? run<double, true>(frames, nframes, curve, nb, iterations, taps,
-                                     (const double*)g, use_tree, img_bytes, (double*)cx, q0, q1,
-                                     result, s)
-                 : run<double, false>(frames, nframes, curve, nb, iterations, taps,
-                                      (const double*)g, use_tree, img_bytes, (double*)cx, q0, q1,
-                                      result, s);
+  switch (per) {
+    case 2:
+      return run_per<2>(frames, nframes, curve, nb, iterations, taps, g, fp64, use_tree, assoc,
+                        img_bytes, cx, q0, q1, result, s);
+    case 8:
+      return run_per<8>(frames, nframes, curve, nb, iterations, taps, g, fp64, use_tree, assoc,
+                        img_bytes, cx, q0, q1, result, s);
+    default:
+      return run_per<kPerDefault>(frames, nframes, curve, nb, iterations, taps, g, fp64, use_tree,
+                                  assoc, img_bytes, cx, q0, q1, result, s);
   }
-  return assoc ? run<float, true>(frames, nframes, curve, nb, iterations, taps, (const float*)g,
-                                  use_tree, img_bytes, (float*)cx, q0, q1, result, s)
-               : run<float, false>(frames, nframes, curve, nb, iterations, taps, (const float*)g,
-                                   use_tree, img_bytes, (float*)cx, q0, q1, result, s);
 }
 
 }  // namespace sa
