@@ -5,11 +5,13 @@ when many images walk at once.  `SeqPipeline` keeps one resumable walk job
 (sa_walk_job_*) stepping on its own stream: in every step each admitted image
 walks for a time budget (a few milliseconds, at its own pace), images whose
 palette has just been classified join at the next step, and images that
-finish leave with their output packed.  Palettes are classified concurrently on a pool of host
-threads, one stream each (the device octree build synchronises its stream a
-few times, so it needs its own thread to overlap).  Short steps also bound how
-long any other stream's work can wait behind the walk on a shared hardware
-queue.
+finish leave with their output packed.  Short steps also bound how long any
+other stream's work can wait behind the walk on a shared hardware queue.
+
+Palettes are classified by a `PaletteBuilder`: whatever has arrived is built
+as one batch (sa_palette_build_batch: one launch per kernel for all images),
+and requests that arrive meanwhile join the next batch.  A few builder
+threads, one stream each, keep batches in flight while another reads back.
 
 `Server` admits mixed requests -- stills and video segments, for the
 sequential, block and Jacobi engines -- against an HBM budget, runs the block
@@ -30,6 +32,70 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib as L
+
+
+class PaletteBuilder:
+    """Continuous batching of the octree build.  submit() queues one image;
+    its callback(palette, error) runs on a builder thread once the batch that
+    took it has built."""
+
+    def __init__(self, threads=2, max_batch_px=1 << 25, max_items=128):
+        import torch
+        self.max_px = int(max_batch_px)
+        self.max_items = int(max_items)
+        self._q = queue.Queue()
+        self._stop = False
+        self.batches = 0
+        self._threads = []
+        for _ in range(max(1, threads)):
+            th = threading.Thread(target=self._loop, args=(torch.cuda.Stream(),), daemon=True)
+            th.start()
+            self._threads.append(th)
+
+    def submit(self, src, width, height, src_fmt, colors, assoc, gray, callback):
+        self._q.put((src, width, height, src_fmt, colors, assoc, gray, callback))
+
+    def _take(self):
+        first = self._q.get()
+        if first is None:
+            return None
+        got, px = [first], first[1] * first[2]
+        while len(got) < self.max_items:
+            try:
+                nxt = self._q.get_nowait()
+            except queue.Empty:
+                break
+            if nxt is None:
+                self._q.put(None)  # let the other threads see the stop too
+                break
+            if px + nxt[1] * nxt[2] > self.max_px:
+                self._q.put(nxt)
+                break
+            got.append(nxt)
+            px += nxt[1] * nxt[2]
+        return got
+
+    def _loop(self, stream):
+        while True:
+            got = self._take()
+            if got is None:
+                self._q.put(None)
+                return
+            items = [g[:7] for g in got]
+            try:
+                pals, status = L.Palette.build_batch(items, stream=stream, strict=False)
+                self.batches += 1
+                errs = [None if p is not None else f"palette build failed (SA code {c})"
+                        for p, c in zip(pals, status)]
+            except L.EngineError as e:
+                pals, errs = [None] * len(got), [str(e)] * len(got)
+            for g, pal, err in zip(got, pals, errs):
+                g[7](pal, err)
+
+    def close(self):
+        self._q.put(None)
+        for th in self._threads:
+            th.join()
 
 
 @dataclass
@@ -60,17 +126,15 @@ class Still:
 class SeqPipeline:
     """Sequential engine with continuous batching (see the module docstring)."""
 
-    def __init__(self, build_threads=8, budget_us=8000.0, options=None, on_done=None,
-                 slice_visits=0):
+    def __init__(self, build_threads=2, budget_us=8000.0, options=None, on_done=None,
+                 slice_visits=0, builder=None):
         import torch
         self.budget_us = float(budget_us)
         self.slice = int(slice_visits)      # > 0: fixed visit slices instead of a time budget
         self.options = options or L.SeqOptions()
         self.on_done = on_done
-        self._streams = queue.Queue()
-        for _ in range(build_threads):
-            self._streams.put(torch.cuda.Stream())
-        self._pool = ThreadPoolExecutor(max_workers=build_threads)
+        self._own_builder = builder is None
+        self.builder = builder or PaletteBuilder(threads=build_threads)
         self._walk_stream = torch.cuda.Stream()
         self._ready = queue.Queue()
         self._live = {}                  # job id -> Still
@@ -89,19 +153,13 @@ class SeqPipeline:
             st.t_ready = st.t_submit
             self._ready.put(st)
         else:
-            self._pool.submit(self._build, st)
-
-    def _build(self, st: Still):
-        s = self._streams.get()
-        try:
-            st.palette = L.Palette.build(st.src, st.width, st.height, st.src_fmt, st.colors,
-                                         st.assoc, st.gray, stream=s)
-        except L.EngineError as e:
-            st.error = str(e)
-        finally:
-            self._streams.put(s)
-        st.t_ready = time.perf_counter()
-        self._ready.put(st)
+            def built(pal, err, st=st):
+                st.palette = pal
+                st.error = err or ""
+                st.t_ready = time.perf_counter()
+                self._ready.put(st)
+            self.builder.submit(st.src, st.width, st.height, st.src_fmt, st.colors, st.assoc,
+                                st.gray, built)
 
     # -- the driver (one thread) -------------------------------------------------
     def _admit(self, block):
@@ -179,7 +237,8 @@ class SeqPipeline:
         self._finish(block=True)
 
     def close(self):
-        self._pool.shutdown(wait=True)
+        if self._own_builder:
+            self.builder.close()
         self._job.close()
 
 
@@ -228,20 +287,27 @@ def footprint(req: Request) -> int:
 
 class Server:
     """Admits requests by HBM footprint, overlaps them on the device and
-    records latency.  `run(arrivals)` takes (t_offset, Request) pairs."""
+    records latency.  `run(arrivals)` takes (t_offset, Request) pairs.
 
-    def __init__(self, hbm_budget=64 << 30, build_threads=8, budget_us=8000.0, slice_visits=0):
+    Palettes of every engine come from one PaletteBuilder.  Block and Jacobi
+    requests are launched in arrival-of-palette order on a stream of their
+    own without waiting for each other; a finisher thread stamps each one done
+    when its event completes."""
+
+    def __init__(self, hbm_budget=64 << 30, build_threads=2, budget_us=8000.0, slice_visits=0):
         import torch
         self.budget = hbm_budget
         self.used = 0
         self.done = []
         self._cv = threading.Condition()
-        self._seq = SeqPipeline(build_threads, budget_us, on_done=self._seq_done,
-                                slice_visits=slice_visits)
+        self.builder = PaletteBuilder(threads=build_threads)
+        self._seq = SeqPipeline(budget_us=budget_us, on_done=self._seq_done,
+                                slice_visits=slice_visits, builder=self.builder)
         self._fast_stream = torch.cuda.Stream()
         self._fast = ThreadPoolExecutor(max_workers=1)
-        self._builders = self._seq._pool
-        self._build_streams = self._seq._streams
+        self._events = queue.Queue()
+        self._finisher = threading.Thread(target=self._finish_fast, daemon=True)
+        self._finisher.start()
 
     def _release(self, req):
         with self._cv:
@@ -260,14 +326,37 @@ class Server:
             req.t_done = st.t_done
             self._release(req)
 
-    def _palette(self, req):
-        s = self._build_streams.get()
+    def _fail(self, req, err):
+        req.error = err
+        req.t_done = time.perf_counter()
+        self._release(req)
+
+    def _launch_fast(self, req):
+        """Block / Jacobi work on the fast stream; completion is stamped by the finisher."""
+        import torch
         try:
-            src0 = req.frames[0][0]
-            return L.Palette.build(src0, req.width, req.height, req.src_fmt, req.colors,
-                                   req.assoc, False, stream=s)
-        finally:
-            self._build_streams.put(s)
+            specs = [L.image_spec(src, dst, req.width, req.height, req.src_fmt, req.dst_fmt,
+                                  req.palette) for src, dst in req.frames]
+            if req.engine == "blocks":
+                L.walk_blocks(specs, block=req.block, stream=self._fast_stream)
+            else:
+                L.walk_approx(specs, iterations=req.iterations, taps=req.taps,
+                              stream=self._fast_stream)
+            ev = torch.cuda.Event()
+            ev.record(self._fast_stream)
+            self._events.put((ev, req))
+        except L.EngineError as e:
+            self._fail(req, str(e))
+
+    def _finish_fast(self):
+        while True:
+            got = self._events.get()
+            if got is None:
+                return
+            ev, req = got
+            ev.synchronize()
+            req.t_done = time.perf_counter()
+            self._release(req)
 
     def _start(self, req):
         req.t_start = time.perf_counter()
@@ -278,39 +367,28 @@ class Server:
                            req.src_fmt, req.dst_fmt, req.colors, req.assoc)
                 st.request = req
                 self._seq.submit(st)
-            else:
-                def seg():
-                    try:
-                        pal = self._palette(req)
-                    except L.EngineError as e:
-                        req.error = str(e)
-                        req.t_done = time.perf_counter()
-                        self._release(req)
-                        return
-                    req.palette = pal
-                    for src, dst in req.frames:
-                        st = Still(src, dst, req.width, req.height, req.src_fmt, req.dst_fmt,
-                                   req.colors, req.assoc, palette=pal)
-                        st.request = req
-                        self._seq.submit(st)
-                self._builders.submit(seg)
+                return
+
+            def seg(pal, err):
+                if err:
+                    self._fail(req, err)
+                    return
+                req.palette = pal
+                for src, dst in req.frames:
+                    st = Still(src, dst, req.width, req.height, req.src_fmt, req.dst_fmt,
+                               req.colors, req.assoc, palette=pal)
+                    st.request = req
+                    self._seq.submit(st)
         else:
-            def fast():
-                try:
-                    req.palette = self._palette(req)
-                    specs = [L.image_spec(src, dst, req.width, req.height, req.src_fmt,
-                                          req.dst_fmt, req.palette) for src, dst in req.frames]
-                    if req.engine == "blocks":
-                        L.walk_blocks(specs, block=req.block, stream=self._fast_stream)
-                    else:
-                        L.walk_approx(specs, iterations=req.iterations, taps=req.taps,
-                                      stream=self._fast_stream)
-                    self._fast_stream.synchronize()
-                except L.EngineError as e:
-                    req.error = str(e)
-                req.t_done = time.perf_counter()
-                self._release(req)
-            self._fast.submit(fast)
+            def seg(pal, err):
+                if err:
+                    self._fail(req, err)
+                    return
+                req.palette = pal
+                self._fast.submit(self._launch_fast, req)
+        src0 = req.frames[0][0]
+        self.builder.submit(src0, req.width, req.height, req.src_fmt, req.colors, req.assoc,
+                            False, seg)
 
     def run(self, arrivals, stop_after=None):
         """Feeds (t_offset, Request) pairs at their times, admitting each when
@@ -345,7 +423,10 @@ class Server:
 
     def close(self):
         self._fast.shutdown(wait=True)
+        self._events.put(None)
+        self._finisher.join()
         self._seq.close()
+        self.builder.close()
 
 
 def latency_summary(reqs):
