@@ -116,6 +116,22 @@ SA_HD uint32_t memo_store_key(bool assoc, unsigned c0, unsigned c1, unsigned c2,
   return (line << 6) | (r & 3u) | ((g & 3u) << 2) | ((b & 3u) << 4);
 }
 
+// scale_char for q = clamp_pixel(v), where q is NaN or in [0, 65535]: the
+// guard becomes a max (s + 0.5 >= 0.5 for every q >= 0, and a NaN sum gives
+// 0, as the guard does), so the result is the same with no compare and select.
+SA_HD unsigned scale_char_clamped(float q) {
+  const float s = (float)((double)q * kInv257);
+  return (unsigned)fmaxf(s + 0.5f, 0.0f);
+}
+
+// memo_store_key split by channel: memo_store_key(assoc, c0, c1, c2, c3) ==
+// memo_key_part(0, c0) | memo_key_part(1, c1) | memo_key_part(2, c2) |
+// (assoc ? memo_key_part(3, c3) : 0).
+SA_HD uint32_t memo_key_part(int ch, unsigned c8) {
+  if (ch == 3) return (uint32_t)(c8 >> 2) << 18;
+  return ((uint32_t)(c8 >> 4) << (6 + 4 * ch)) | (((c8 >> 2) & 3u) << (2 * ch));
+}
+
 // src/rd_octree.cpp:59-67 ColorToQNodeId on packed channel bytes.
 SA_HD int node_id(bool assoc, unsigned c0, unsigned c1, unsigned c2, unsigned c3, int index) {
   int id = (int)((c0 >> index) & 1u);

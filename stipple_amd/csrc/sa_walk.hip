@@ -143,7 +143,21 @@ struct Memo {
   uint16_t* line;   // 64 entries per slot: index+1, 0 = untouched
   uint16_t* table;  // HBM, 1 << K entries, zeroed before the walk
   int bits;         // log2(slots)
+  // This lane's channel part of the key (memo_key_part for its row):
+  // ((c8 >> k_hi_shr) & k_hi_mask) << k_hi_shl | ((c8 >> 2) & k_lo_mask) << k_lo_shl.
+  uint32_t k_hi_shr, k_hi_mask, k_hi_shl, k_lo_mask, k_lo_shl;
 };
+
+// Rows 0..2 (r, g, b) give their top 4 bits to the line and bits 2..3 to the
+// offset; row 3 (alpha) gives its top 6 bits to the line when associated.
+template <bool ASSOC>
+__device__ __forceinline__ void memo_key_consts(Memo& m, int ch) {
+  m.k_hi_shr = ch == 3 ? 2u : 4u;
+  m.k_hi_mask = ch == 3 ? (ASSOC ? 63u : 0u) : 15u;
+  m.k_hi_shl = ch == 3 ? 18u : (uint32_t)(6 + 4 * ch);
+  m.k_lo_mask = ch == 3 ? 0u : 3u;
+  m.k_lo_shl = ch == 3 ? 0u : (uint32_t)(2 * ch);
+}
 
 constexpr uint32_t kEmptyTag = 0x80000000u;
 constexpr uint32_t kDirty = 0x40000000u;
@@ -201,28 +215,39 @@ __device__ __forceinline__ void visit(WalkState& w, double& xq, double W, int64_
   w.acc = w.acc + W * dpp<kRowBcast0>(w.e, 0.0);
   const float pc = clamp_pixel(w.acc);
   const double pd = (double)pc;
-  const unsigned c8 = scale_char(pc);
-  const unsigned c0 = (unsigned)rl((int)c8, 0);
-  const unsigned c1 = (unsigned)rl((int)c8, 16);
-  const unsigned c2 = (unsigned)rl((int)c8, 32);
-  const unsigned c3 = ASSOC ? (unsigned)rl((int)c8, 48) : 0u;
+  const unsigned c8 = scale_char_clamped(pc);
+  // The channel bytes of visit t sit in lanes 0, 16, 32, 48.
+  auto chan = [&]() {
+    struct C4 {
+      unsigned c0, c1, c2, c3;
+    };
+    return C4{(unsigned)rl((int)c8, 0), (unsigned)rl((int)c8, 16), (unsigned)rl((int)c8, 32),
+              ASSOC ? (unsigned)rl((int)c8, 48) : 0u};
+  };
   int idx;
   if (CACHE) {
-    const uint32_t key = memo_store_key(ASSOC, c0, c1, c2, c3);
+    // Each row forms its channel's part of the memo key (memo_key_part with
+    // this lane's constants: no branch); four lane reads and three ORs join
+    // them (the channel bytes themselves are read only on a miss).
+    const uint32_t kc = (((c8 >> m.k_hi_shr) & m.k_hi_mask) << m.k_hi_shl) |
+                        (((c8 >> 2) & m.k_lo_mask) << m.k_lo_shl);
+    uint32_t key = (uint32_t)rl((int)kc, 0) | (uint32_t)rl((int)kc, 16) | (uint32_t)rl((int)kc, 32);
+    if (ASSOC) key |= (uint32_t)rl((int)kc, 48);
     const uint32_t line = key >> 6;
     const uint32_t slot = memo_slot(m, line);
     const uint32_t tg = (uint32_t)rfl((int)m.tag[slot]);
     uint32_t ent = (uint32_t)rfl((int)m.line[64 * slot + (key & 63)]);
-    if ((tg & ~kDirty) != line) {
+    if (__builtin_expect((tg & ~kDirty) != line, 0)) {
       ++cnt.fills;
       cnt.loads += memo_fill(m, slot, line, tg);
       ent = (uint32_t)rfl((int)m.line[64 * slot + (key & 63)]);
     }
-    if (ent != 0) {
+    if (__builtin_expect(ent != 0, 1)) {
       idx = (int)ent - 1;
     } else {
       ++cnt.misses;
-      idx = lookup<ASSOC>(s, c0, c1, c2, c3, rl(pd, 0), rl(pd, 16), rl(pd, 32),
+      const auto c = chan();
+      idx = lookup<ASSOC>(s, c.c0, c.c1, c.c2, c.c3, rl(pd, 0), rl(pd, 16), rl(pd, 32),
                           ASSOC ? rl(pd, 48) : 65535.0);
       if (lane == 0) {
         m.line[64 * slot + (key & 63)] = (uint16_t)(idx + 1);
@@ -231,7 +256,8 @@ __device__ __forceinline__ void visit(WalkState& w, double& xq, double W, int64_
     }
   } else {
     ++cnt.misses;
-    idx = lookup<ASSOC>(s, c0, c1, c2, c3, rl(pd, 0), rl(pd, 16), rl(pd, 32),
+    const auto c = chan();
+    idx = lookup<ASSOC>(s, c.c0, c.c1, c.c2, c.c3, rl(pd, 0), rl(pd, 16), rl(pd, 32),
                         ASSOC ? rl(pd, 48) : 65535.0);
   }
   // Residual against the associated palette colour, per row's channel.
@@ -359,6 +385,7 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     int64_t stop = (slice > 0 && it.state && t0 + slice < it.n) ? t0 + slice : it.n;
     Memo m;
     m.bits = it.memo_bits;
+    memo_key_consts<ASSOC>(m, lane >> 4);
     const uint32_t slots = 1u << m.bits;
     const uint32_t memo_bytes = CACHE ? 132u * slots : 0u;
     m.tag = (uint32_t*)lds;
