@@ -164,7 +164,7 @@ def run_ramp(args, sa, torch, synth, dev):
 
 def run_video(args, sa, torch, synth, dev):
     """1920x1080 RGB8 plasma frames, a 16-colour palette per frame, block walk B=64."""
-    nf = args.frames
+    nf = args.frames or 32
     params = synth.plasma_params(3)
     t0 = time.perf_counter()
     frames = [synth.plasma_frame(t, params=params) for t in range(nf)]
@@ -257,7 +257,7 @@ def run_video(args, sa, torch, synth, dev):
 def run_approx4k(args, sa, torch, synth, dev):
     """3840x2160 RGB16 radial gradients + salt and pepper, 256 colours from
     frame 0 held on the device, Jacobi 8 sweeps, taps 96, fp32."""
-    nf = args.frames
+    nf = args.frames or 16
     t0 = time.perf_counter()
     frames = [synth.radial_frame16(t) for t in range(nf)]
     log(f"generated {nf} frames in {time.perf_counter() - t0:.1f}s")
@@ -364,7 +364,8 @@ def main():
                     help="stills: 64 uneven RGBA8 stills, sequential engine (the headline); "
                          "ramp: one 640x480 image; video: 1080p frames, block engine; "
                          "approx4k: 4K RGB16 frames, Jacobi engine")
-    ap.add_argument("--frames", type=int, default=16, help="video / approx4k: frames per step")
+    ap.add_argument("--frames", type=int, default=0,
+                    help="frames per step (0: 32 for video, one batched build; 16 for approx4k)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run a 256 MiB device copy (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()

@@ -69,7 +69,7 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
                           int gray, hipStream_t s, OctreeOut* out);
 
 // Builds n palettes together, one launch per kernel for all of them (images
-// are grouped into sub-batches of at most 2^25 pixels).  Each image gets its
+// are grouped into sub-batches of at most 2^26 pixels).  Each image gets its
 // own result and outputs, exactly as build_octree would give it; the stream is
 // synchronised four times per sub-batch.
 void build_octree_batch(const OctreeReq* req, int n, hipStream_t s, OctreeOut* out,

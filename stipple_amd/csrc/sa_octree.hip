@@ -2027,7 +2027,7 @@ namespace {
 // Images are grouped into sub-batches of at most this many pixels (a larger
 // image is a sub-batch of its own), which bounds the scratch of one phase and
 // keeps every batch-wide index within 32 bits.
-constexpr int64_t kBatchPixels = (int64_t)1 << 25;
+constexpr int64_t kBatchPixels = (int64_t)1 << 26;
 
 enum MapKind { kMapN, kMapR, kMapP, kMapCap, kMapM256, kMapM64, kMapDesc, kMaps };
 
