@@ -1150,7 +1150,7 @@ namespace {
 // block curve's length and gap cell, and the frame descriptors.
 struct BlocksPlan {
   int64_t w = 0, h = 0, nb = 0, gap = -1;
-  int n = 0, block = 0, assoc = -1, max_img = 0;
+  int n = 0, block = 0, assoc = -1, max_img = 0, max_count = 0, src_class = 0;
   int threads = 256;  // tuned launch shape
   const double* sw = nullptr;
   std::vector<sa::BlockFrame> fr;
@@ -1189,8 +1189,14 @@ int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipS
     if (p.assoc != f.palette->assoc)
       return fail(err, errlen, SA_EUNSUPPORTED, "frames mix associated and plain alpha");
     p.max_img = std::max(p.max_img, f.palette->hdr.bytes);
+    p.max_count = std::max(p.max_count, f.palette->count);
+    p.src_class = std::max(p.src_class, sa::blocks_src_class(f.src_fmt));
   }
   if (p.w < 1 || p.h < 1) return fail(err, errlen, SA_EINVAL, "bad geometry");
+  if (sa::blocks_lds_bytes(p.max_img, p.max_count, p.src_class, 64) > sa::kBlocksMaxLds)
+    return fail(err, errlen, SA_EUNSUPPORTED,
+                "the blocks engine stages the palette's search image in LDS (%d bytes): too large",
+                p.max_img);
   rc = get_weights(diffusion, s, &p.sw, err, errlen);
   if (rc) return rc;
   const int level = curve_level(p.w, p.h);
@@ -1219,7 +1225,7 @@ int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipS
 hipError_t launch_blocks_plan(const BlocksPlan& p, const sa::BlockFrame* d_fr,
                               const int32_t* curve, const double* sw, hipStream_t s) {
   hipError_t e = sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, sw, p.assoc,
-                                   p.max_img, p.threads, s);
+                                   p.max_img, p.max_count, p.src_class, p.threads, s);
   if (e == hipSuccess && p.gap >= 0) e = sa::launch_blocks_fill(d_fr, p.n, p.gap, s);
   return e;
 }
@@ -1239,7 +1245,7 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
   HIP_TRY(staged_copy(d_fr, p.fr.data(), sizeof(sa::BlockFrame) * p.fr.size(), s));
   HIP_TRY(timed("blocks_walk", s, [&] {
     return sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, p.sw, p.assoc, p.max_img,
-                             p.threads, s);
+                             p.max_count, p.src_class, p.threads, s);
   }));
   if (p.gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, p.n, p.gap, s));
   HIP_TRY(hipFreeAsync(d_fr, s));

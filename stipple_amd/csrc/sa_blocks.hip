@@ -10,25 +10,110 @@
 //
 // Mapping to CDNA4: one lane per block, so a wavefront advances 64 blocks in
 // lock step and the serial chain of each block is hidden by the other lanes
-// and the other wavefronts.  The 16-entry queue is a register ring: the
-// position loop is unrolled 16 times, so the oldest entry's slot is a
-// compile-time constant and shifting the queue costs nothing.  The palette's
-// search image (sa_palette.h) is staged in LDS once per workgroup; every
-// workgroup serves blocks of a single frame.  The weights are wave-uniform
-// and live in SGPRs.
+// and the other wavefronts.  A workgroup owns NT consecutive blocks and walks
+// them 16 positions at a time:
+//   1. stage -- the workgroup reads the 16 curve entries of each of its blocks
+//      (16 consecutive ints per block, so a wavefront reads four 64-byte runs)
+//      and gathers those pixels into LDS, position-major, as the batch's
+//      widest source class (8-bit words, 16-bit pairs or float4);
+//   2. walk -- each lane runs its block's 16 positions from LDS: no global
+//      load on the serial chain;
+//   3. write -- the workgroup writes the 16 x NT results back, each thread the
+//      entries it staged (their pixel ids stay in its registers).
+// The 16-entry queue is a register ring: the 16 positions of a chunk are
+// unrolled, so the oldest entry's slot is a compile-time constant and
+// shifting the queue costs nothing.  The palette's search image (sa_palette.h)
+// is staged in LDS once per workgroup, with each candidate's distance
+// constants (its colour, premultiplied when associated) as doubles beside it;
+// every workgroup serves blocks of a single frame.  The weights are
+// wave-uniform and live in SGPRs.
 //
-// Output is fused (no index buffer, no separate scatter): the only pixel the
-// block curve visits twice is the origin (position 0 and the trailing
-// position), so every position except 0 owns its pixel.
+// Only the origin is visited twice (position 0 and the trailing position), so
+// every position except 0 owns its pixel and the write needs no index buffer.
 #include "sa_internal.h"
 #include "sa_math.h"
 
 namespace sa {
 namespace {
 
+constexpr int kChunk = 16;  // positions per stage/walk/write round (the queue length)
 
+// Source classes: every format of a batch is staged as the widest one.
+enum SrcClass : int { kSrc8 = 0, kSrc16 = 1, kSrcF = 2 };
+
+template <int SRC>
+struct Raw;
+template <>
+struct Raw<kSrc8> {
+  typedef uint32_t T;  // r | g << 8 | b << 16 | a << 24
+};
+template <>
+struct Raw<kSrc16> {
+  typedef uint2 T;  // r | g << 16, b | a << 16
+};
+template <>
+struct Raw<kSrcF> {
+  typedef float4 T;
+};
+
+// One source pixel in class SRC.  Widening is exact: an 8-bit sample is
+// staged as itself (8-bit class) or as sample * 257 (16-bit class), which is
+// the value load_pixel gives it; formats without alpha stage it opaque.
+template <int SRC>
+__device__ __forceinline__ typename Raw<SRC>::T stage_pixel(int fmt, const void* base, int64_t p) {
+  typedef typename Raw<SRC>::T T;
+  if constexpr (SRC == kSrc8) {
+    if (fmt == kRgba8) return ((const uint32_t*)base)[p];
+    const uint8_t* s = (const uint8_t*)base + 3 * p;
+    return (T)s[0] | ((T)s[1] << 8) | ((T)s[2] << 16) | 0xFF000000u;
+  } else if constexpr (SRC == kSrc16) {
+    switch (fmt) {
+      case kRgba16: return ((const uint2*)base)[p];
+      case kRgb16: {
+        const uint16_t* s = (const uint16_t*)base + 3 * p;
+        return make_uint2((uint32_t)s[0] | ((uint32_t)s[1] << 16), (uint32_t)s[2] | 0xFFFF0000u);
+      }
+      case kRgba8: {
+        const uint32_t v = ((const uint32_t*)base)[p];
+        return make_uint2((v & 255u) * 257u | (((v >> 8) & 255u) * 257u) << 16,
+                          ((v >> 16) & 255u) * 257u | ((v >> 24) * 257u) << 16);
+      }
+      default: {
+        const uint8_t* s = (const uint8_t*)base + 3 * p;
+        return make_uint2((uint32_t)s[0] * 257u | ((uint32_t)s[1] * 257u) << 16,
+                          (uint32_t)s[2] * 257u | 0xFFFF0000u);
+      }
+    }
+  } else {
+    return load_pixel(fmt, base, p);
+  }
+}
+
+// The staged pixel as the Q16 floats load_pixel gives.
+template <int SRC>
+__device__ __forceinline__ float4 unstage(const typename Raw<SRC>::T& v) {
+  if constexpr (SRC == kSrc8) {
+    return make_float4((float)(v & 255u) * 257.0f, (float)((v >> 8) & 255u) * 257.0f,
+                       (float)((v >> 16) & 255u) * 257.0f, (float)(v >> 24) * 257.0f);
+  } else if constexpr (SRC == kSrc16) {
+    return make_float4((float)(v.x & 0xFFFFu), (float)(v.x >> 16), (float)(v.y & 0xFFFFu),
+                       (float)(v.y >> 16));
+  } else {
+    return v;
+  }
+}
+
+template <int SRC>
+__device__ __forceinline__ float unstage_alpha(const typename Raw<SRC>::T& v) {
+  if constexpr (SRC == kSrc8) return (float)(v >> 24) * 257.0f;
+  else if constexpr (SRC == kSrc16) return (float)(v.y >> 16);
+  else return v.w;
+}
+
+// The output pixel for palette entry `e` (src/rd_riemersma_cpu.cpp:700-760):
+// alpha is the entry's when associated, else the source pixel's (`a`).
 __device__ __forceinline__ void store_out(const BlockFrame& f, int64_t pix, int idx,
-                                          const float4& e, bool assoc) {
+                                          const float4& e, float a) {
   switch (f.dst_fmt) {
     case kIndex8: ((uint8_t*)f.dst)[pix] = (uint8_t)idx; break;
     case kIndex16: ((uint16_t*)f.dst)[pix] = (uint16_t)idx; break;
@@ -39,12 +124,10 @@ __device__ __forceinline__ void store_out(const BlockFrame& f, int64_t pix, int 
       d[2] = (uint8_t)scale_char(e.z);
       break;
     }
-    case kRgba8: {
-      const float a = assoc ? e.w : load_pixel(f.src_fmt, f.src, pix).w;
+    case kRgba8:
       ((uint32_t*)f.dst)[pix] = scale_char(e.x) | (scale_char(e.y) << 8) |
                                 (scale_char(e.z) << 16) | (scale_char(a) << 24);
       break;
-    }
     case kRgb16: {
       uint16_t* d = (uint16_t*)f.dst + 3 * pix;
       d[0] = (uint16_t)e.x;
@@ -53,8 +136,6 @@ __device__ __forceinline__ void store_out(const BlockFrame& f, int64_t pix, int 
       break;
     }
     case kRgba16: {
-      // rd_riemersma_cpu.cpp:747-752: alpha is the SOURCE alpha unless associated.
-      const float a = assoc ? e.w : load_pixel(f.src_fmt, f.src, pix).w;
       ushort4 o;
       o.x = (uint16_t)e.x;
       o.y = (uint16_t)e.y;
@@ -63,137 +144,259 @@ __device__ __forceinline__ void store_out(const BlockFrame& f, int64_t pix, int 
       ((ushort4*)f.dst)[pix] = o;
       break;
     }
-    default: {
+    default:
       // rd_riemersma_cpu.cpp:704-708: r,g,b written, alpha only when associated.
-      const float a = assoc ? e.w : load_pixel(f.src_fmt, f.src, pix).w;
       ((float4*)f.dst)[pix] = make_float4(e.x, e.y, e.z, a);
       break;
-    }
   }
 }
 
+// The search image in LDS plus each candidate's distance constants:
+// dcent[k] = (alpha e0, alpha e1, alpha e2, e3) of the k-th candidate in
+// post-order, alpha = QuantumScale e3 when associated and 1 otherwise -- the
+// products candidate_distance forms from the entry, formed the same way.
 struct LSearch {
   const double* col;
   const float4* ent;
   const int2* sub;
+  const uint32_t* desc;
   const int16_t* child;
   const uint16_t* cand;
+  const double4* dcent;
   int kids;
 };
 
-// Per-lane descent and ordered scan (src/rd_riemersma_cpu.cpp:671-682 with
-// src/rd_octree.cpp:178-217).
+// Descent (src/rd_riemersma_cpu.cpp:82-89; the first three levels are one
+// table read) and the ordered scan of src/rd_octree.cpp:178-217 with the
+// distance of candidate_distance: the last candidate at the running minimum.
 template <bool ASSOC>
-__device__ __forceinline__ int lane_lookup(const LSearch& s, const double t[4]) {
-  const unsigned c0 = scale_char(clamp_pixel(t[0]));
-  const unsigned c1 = scale_char(clamp_pixel(t[1]));
-  const unsigned c2 = scale_char(clamp_pixel(t[2]));
-  const unsigned c3 = ASSOC ? scale_char(clamp_pixel(t[3])) : 0u;
-  int row = 0, prev = 0;
-  for (int lev = kMaxDepth - 1; lev > 0; --lev) {
-    const int ch = s.child[row * s.kids + node_id(ASSOC, c0, c1, c2, c3, lev)];
-    if (ch == kNoChild) break;
-    prev = row;
-    if (ch == kLeaf) break;
-    row = ch;
-  }
-  const int2 r = s.sub[prev];
-  double best = kFarDistance;
-  int bidx = 0;
-  for (int k = 0; k < r.y; ++k) {
-    const int ci = s.cand[r.x + k];
-    const float4 e4 = s.ent[ci];
-    const float e[4] = {e4.x, e4.y, e4.z, e4.w};
-    const double d = candidate_distance(ASSOC, e, t);
-    if (d <= best) {
-      best = d;
-      bidx = ci;
+__device__ __forceinline__ int lane_lookup(const LSearch& s, const float pc[4], const double t[4]) {
+  const unsigned c0 = scale_char(pc[0]);
+  const unsigned c1 = scale_char(pc[1]);
+  const unsigned c2 = scale_char(pc[2]);
+  const unsigned c3 = ASSOC ? scale_char(pc[3]) : 0u;
+  uint32_t de = (c0 >> 5) | ((c1 >> 5) << 3) | ((c2 >> 5) << 6);
+  if (ASSOC) de |= (c3 >> 5) << 9;
+  const uint32_t d3 = s.desc[de];
+  int prev = (int)((d3 >> 16) & 0x7FFFu);
+  if (!(d3 & kDescStop)) {
+    int row = (int)(d3 & 0x7FFFu);
+    for (int lev = 4; lev > 0; --lev) {
+      const int ch = s.child[row * s.kids + node_id(ASSOC, c0, c1, c2, c3, lev)];
+      if (ch == kNoChild) break;
+      prev = row;
+      if (ch == kLeaf) break;
+      row = ch;
     }
   }
-  return bidx;
+  const int2 r = s.sub[prev];
+  // candidate_distance's per-pixel factors: beta = QuantumScale t3 when
+  // associated (beta t_c formed once), 1 otherwise (1 * t_c == t_c).
+  double bt0 = t[0], bt1 = t[1], bt2 = t[2];
+  if (ASSOC) {
+    const double beta = kScale * t[3];
+    bt0 = beta * t[0];
+    bt1 = beta * t[1];
+    bt2 = beta * t[2];
+  }
+  double best = kFarDistance;
+  int bk = 0;
+  const double4* dc = s.dcent + r.x;
+  for (int k = 0; k < r.y; ++k) {
+    const double4 e = dc[k];
+    double v = e.x - bt0;
+    double d = v * v;
+    v = e.y - bt1;
+    d += v * v;
+    v = e.z - bt2;
+    d += v * v;
+    if (ASSOC) {
+      v = e.w - t[3];
+      d += v * v;
+    }
+    if (d <= best) {
+      best = d;
+      bk = k;
+    }
+  }
+  return (int)s.cand[r.x + bk];
 }
 
 template <bool ASSOC>
 struct Queue {
-  double e[ASSOC ? 4 : 3][16];
+  double e[ASSOC ? 4 : 3][kChunk];
 };
 
-// One position; R is the ring slot of the oldest queue entry.
-template <int R, bool ASSOC>
+// Position S of a chunk (ring slot S holds the oldest queue entry).  `cur` is
+// its staged pixel, read one position ahead; the scheduling barrier keeps the
+// compiler from hoisting all sixteen reads (and their conversions) into
+// registers at once.
+template <int S, bool ASSOC, int SRC>
 __device__ __forceinline__ void block_step(Queue<ASSOC>& q, const double* __restrict__ sw,
-                                           const BlockFrame& f, const LSearch& s,
-                                           int64_t pix, bool write) {
-  const float4 p = load_pixel(f.src_fmt, f.src, pix);
+                                           const LSearch& s, const typename Raw<SRC>::T* raw,
+                                           uint16_t* oidx, int stride, int lane, int live,
+                                           typename Raw<SRC>::T& cur) {
+  if (S >= live) return;
+  typename Raw<SRC>::T nxt = cur;
+  if (S + 1 < kChunk && S + 1 < live) nxt = raw[(S + 1) * stride + lane];
+  const float4 p = unstage<SRC>(cur);
   double x[4];
   assoc_pixel(ASSOC, p.x, p.y, p.z, p.w, x);
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
+  for (int k = 0; k < kQueue; ++k) {
     const double wk = sw[k];
-    x[0] += wk * q.e[0][(R + k) & 15];
-    x[1] += wk * q.e[1][(R + k) & 15];
-    x[2] += wk * q.e[2][(R + k) & 15];
-    if (ASSOC) x[3] += wk * q.e[ASSOC ? 3 : 0][(R + k) & 15];
+    x[0] += wk * q.e[0][(S + k) & 15];
+    x[1] += wk * q.e[1][(S + k) & 15];
+    x[2] += wk * q.e[2][(S + k) & 15];
+    if (ASSOC) x[3] += wk * q.e[ASSOC ? 3 : 0][(S + k) & 15];
   }
-  x[0] = (double)clamp_pixel(x[0]);
-  x[1] = (double)clamp_pixel(x[1]);
-  x[2] = (double)clamp_pixel(x[2]);
-  if (ASSOC) x[3] = (double)clamp_pixel(x[3]);
-  const int idx = lane_lookup<ASSOC>(s, x);
+  float pc[4];
+  pc[0] = clamp_pixel(x[0]);
+  pc[1] = clamp_pixel(x[1]);
+  pc[2] = clamp_pixel(x[2]);
+  pc[3] = ASSOC ? clamp_pixel(x[3]) : 0.0f;
+  x[0] = (double)pc[0];
+  x[1] = (double)pc[1];
+  x[2] = (double)pc[2];
+  if (ASSOC) x[3] = (double)pc[3];
+  const int idx = lane_lookup<ASSOC>(s, pc, x);
   const double* c = s.col + 4 * idx;
-  q.e[0][R] = x[0] - c[0];
-  q.e[1][R] = x[1] - c[1];
-  q.e[2][R] = x[2] - c[2];
-  if (ASSOC) q.e[ASSOC ? 3 : 0][R] = x[3] - c[3];
-  if (write) store_out(f, pix, idx, s.ent[idx], ASSOC);
+  q.e[0][S] = x[0] - c[0];
+  q.e[1][S] = x[1] - c[1];
+  q.e[2][S] = x[2] - c[2];
+  if (ASSOC) q.e[ASSOC ? 3 : 0][S] = x[3] - c[3];
+  oidx[S * stride + lane] = (uint16_t)idx;
+  cur = nxt;
+  __builtin_amdgcn_sched_barrier(0);
 }
 
-template <bool ASSOC, int... Rs>
-__device__ __forceinline__ void block_group(Queue<ASSOC>& q, const double* __restrict__ sw,
-                                            const BlockFrame& f, const LSearch& s,
-                                            const int32_t* __restrict__ curve, int64_t m,
-                                            int64_t i0, int64_t end,
-                                            std::integer_sequence<int, Rs...>) {
-  ((i0 + Rs < end
-        ? (block_step<Rs, ASSOC>(q, sw, f, s, (i0 + Rs < m) ? (int64_t)curve[i0 + Rs] : 0,
-                                 i0 + Rs != 0),
-           0)
-        : 0),
-   ...);
+template <bool ASSOC, int SRC, int... Ss>
+__device__ __forceinline__ void walk_chunk(Queue<ASSOC>& q, const double* __restrict__ sw,
+                                           const LSearch& s, const typename Raw<SRC>::T* raw,
+                                           uint16_t* oidx, int stride, int lane, int live,
+                                           std::integer_sequence<int, Ss...>) {
+  typename Raw<SRC>::T cur = raw[lane];
+  (block_step<Ss, ASSOC, SRC>(q, sw, s, raw, oidx, stride, lane, live, cur), ...);
 }
 
-template <bool ASSOC, int NT>
-__global__ __launch_bounds__(NT) void blocks_kernel(const BlockFrame* __restrict__ frames,
-                                                          const int32_t* __restrict__ curve,
-                                                          int64_t nb, int block, int64_t nblocks,
-                                                          const double* __restrict__ sw) {
+}  // namespace
+
+// LDS bytes of a block-walk workgroup: the search image (img bytes, 16-byte
+// multiple), the candidates' distance constants, and one chunk's staged
+// pixels and result indices (rows of NT + 1 to spread the banks).
+size_t blocks_lds_bytes(int img_bytes, int count, int src_class, int threads) {
+  const size_t raw = src_class == kSrcF ? 16 : src_class == kSrc16 ? 8 : 4;
+  const size_t row = (size_t)threads + 1;
+  const size_t oidx = (2 * kChunk * row + 15) & ~(size_t)15;
+  return (size_t)img_bytes + 32 * (size_t)count + raw * kChunk * row + oidx;
+}
+
+namespace {
+
+// Workgroup w walks blocks [x * NT, x * NT + NT) of frame w % nframes, x =
+// w / nframes: the frames of one block range are dispatched together, so the
+// XCDs they land on share that range of the curve in their L2.
+// Occupancy: 3 wavefronts per SIMD (2 with alpha), the queue's registers
+// allowing; the 16 staged reads are otherwise hoisted and kept live.
+template <bool ASSOC, int SRC, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ASSOC ? 2 : 3)))
+void blocks_kernel(const BlockFrame* __restrict__ frames,
+                                                    int nframes,
+                                                    const int32_t* __restrict__ curve,
+                                                    int64_t nb, int block, int64_t nblocks,
+                                                    const double* __restrict__ sw, int img_bytes,
+                                                    int count) {
+  typedef typename Raw<SRC>::T T;
   extern __shared__ __align__(16) unsigned char lds[];
-  const BlockFrame f = frames[blockIdx.y];
+  const int fi = (int)(blockIdx.x % (unsigned)nframes);
+  const int64_t x = blockIdx.x / (unsigned)nframes;
+  const BlockFrame f = frames[fi];
   const PaletteDev pd = *f.pal;
+  const int tid = threadIdx.x;
+  double4* dcent = (double4*)(lds + img_bytes);
+  T* raw = (T*)(lds + img_bytes + 32 * count);
+  uint16_t* oidx = (uint16_t*)(raw + kChunk * (NT + 1));
   {
     const uint4* src = (const uint4*)pd.image;
     uint4* dst = (uint4*)lds;
-    for (int k = threadIdx.x; k < pd.bytes / 16; k += NT) dst[k] = src[k];
-    __syncthreads();
+    for (int k = tid; k < pd.bytes / 16; k += NT) dst[k] = src[k];
+    const float4* cent = (const float4*)((const unsigned char*)pd.image + pd.off_cent);
+    for (int k = tid; k < pd.count; k += NT) {
+      const float4 e = cent[k];
+      double a = 1.0;
+      if (ASSOC) a = kScale * (double)e.w;
+      dcent[k] = make_double4(a * (double)e.x, a * (double)e.y, a * (double)e.z, (double)e.w);
+    }
   }
   LSearch s;
   s.col = (const double*)lds;
   s.ent = (const float4*)(lds + pd.off_ent);
   s.sub = (const int2*)(lds + pd.off_sub);
+  s.desc = (const uint32_t*)(lds + pd.off_desc);
   s.child = (const int16_t*)(lds + pd.off_child);
   s.cand = (const uint16_t*)(lds + pd.off_cand);
+  s.dcent = dcent;
   s.kids = pd.kids;
-  const int64_t b = (int64_t)blockIdx.x * NT + threadIdx.x;
-  if (b >= nblocks) return;
-  const int64_t begin = b * block;
-  const int64_t end = begin + block < nb ? begin + block : nb;
+  const int64_t b0 = x * NT;
   const int64_t m = nb - 1;  // positions [0, m) follow the curve; m is the origin
+  // This lane's block and its length (0 past the last block).
+  const int64_t mb = b0 + tid;
+  const int64_t mbegin = mb * block;
+  const int mlen = mb < nblocks ? (int)(nb - mbegin < block ? nb - mbegin : block) : 0;
   Queue<ASSOC> q;
 #pragma unroll
   for (int c = 0; c < (ASSOC ? 4 : 3); ++c)
 #pragma unroll
-    for (int k = 0; k < 16; ++k) q.e[c][k] = 0.0;
-  for (int64_t i0 = begin; i0 < end; i0 += 16)
-    block_group<ASSOC>(q, sw, f, s, curve, m, i0, end, std::make_integer_sequence<int, 16>{});
+    for (int k = 0; k < kQueue; ++k) q.e[c][k] = 0.0;
+  // Staged entry k of this thread: position c0 + sp of block b0 + jt + k NT/16
+  // (sp = tid & 15, jt = tid >> 4), i.e. curve position ibase + c0 + k istep.
+  // Past the curve's end (i >= nb) an entry is empty; so is every entry of a
+  // block past the last one, whose positions all lie past the end.
+  const int sp = tid & 15, jt = tid >> 4;
+  const int64_t ibase = (b0 + jt) * block + sp;
+  const int64_t istep = (int64_t)(NT / 16) * block;
+  T* const rawk = raw + sp * (NT + 1) + jt;  // entry k at rawk[k NT/16]
+  for (int c0 = 0; c0 < block; c0 += kChunk) {
+    // The chunk's base is opaque to the optimiser, so the sixteen entries'
+    // positions are formed here each chunk instead of being hoisted out of the
+    // loop (and held in registers beside the queue).
+    int64_t i0 = ibase + c0;
+    asm volatile("" : "+v"(i0));
+    const bool in_block = c0 + sp < block;
+    // Four entries at a time: the queue is live throughout, so the loads of
+    // all sixteen (and their addresses) must not be in flight at once.
+    int32_t pix[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const int64_t i = i0 + k * istep;
+      pix[k] = -1;
+      if (in_block && i < nb) {
+        const int64_t p = i < m ? (int64_t)curve[i] : 0;
+        rawk[k * (NT / 16)] = stage_pixel<SRC>(f.src_fmt, f.src, p);
+        pix[k] = i != 0 ? (int32_t)p : -1;
+      }
+      if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+    // Walk: this lane's block, positions c0 .. c0 + 15.
+    const int live = mlen - c0;
+    if (live > 0)
+      walk_chunk<ASSOC, SRC>(q, sw, s, raw, oidx, NT + 1, tid, live,
+                             std::make_integer_sequence<int, kChunk>{});
+    __syncthreads();
+    // Write: the entries this thread staged.
+    uint16_t* const oidxk = oidx + sp * (NT + 1) + jt;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      if (pix[k] >= 0) {
+        const int idx = oidxk[k * (NT / 16)];
+        const float4 en = s.ent[idx];
+        const float a = ASSOC ? en.w : unstage_alpha<SRC>(rawk[k * (NT / 16)]);
+        store_out(f, pix[k], idx, en, a);
+      }
+      if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
 }
 
 // The pixel the block curve never reaches keeps its source value
@@ -242,33 +445,55 @@ __global__ void blocks_fill_kernel(const BlockFrame* __restrict__ frames, int nf
 
 }  // namespace
 
-template <int NT>
+template <int SRC, int NT>
 void launch_blocks_nt(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
-                      int block, int64_t nblocks, const double* sw, int assoc,
-                      int lds_image_bytes, hipStream_t s) {
-  const dim3 grid((unsigned)((nblocks + NT - 1) / NT), (unsigned)nframes);
+                      int block, int64_t nblocks, const double* sw, int assoc, int img_bytes,
+                      int count, size_t lds, hipStream_t s) {
+  const dim3 grid((unsigned)(((nblocks + NT - 1) / NT) * nframes));
   if (assoc)
-    blocks_kernel<true, NT><<<grid, dim3(NT), (size_t)lds_image_bytes, s>>>(frames, curve, nb,
-                                                                         block, nblocks, sw);
+    blocks_kernel<true, SRC, NT><<<grid, dim3(NT), lds, s>>>(frames, nframes, curve, nb, block,
+                                                             nblocks, sw, img_bytes, count);
   else
-    blocks_kernel<false, NT><<<grid, dim3(NT), (size_t)lds_image_bytes, s>>>(frames, curve, nb,
-                                                                          block, nblocks, sw);
+    blocks_kernel<false, SRC, NT><<<grid, dim3(NT), lds, s>>>(frames, nframes, curve, nb, block,
+                                                              nblocks, sw, img_bytes, count);
+}
+
+template <int SRC>
+void launch_blocks_src(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
+                       int block, int64_t nblocks, const double* sw, int assoc, int img_bytes,
+                       int count, int threads, size_t lds, hipStream_t s) {
+  switch (threads) {
+    case 64: launch_blocks_nt<SRC, 64>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, lds, s); break;
+    case 128: launch_blocks_nt<SRC, 128>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, lds, s); break;
+    case 512: launch_blocks_nt<SRC, 512>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, lds, s); break;
+    default: launch_blocks_nt<SRC, 256>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, lds, s); break;
+  }
+}
+
+int blocks_src_class(int fmt) {
+  return fmt == kRgbaF32 ? kSrcF : (fmt == kRgb16 || fmt == kRgba16) ? kSrc16 : kSrc8;
 }
 
 // `threads` (64, 128, 256 or 512) blocks per workgroup: every workgroup stages
 // the palette once, so larger ones amortise it and smaller ones fill more CUs.
+// A shape whose LDS would not fit is halved until it does (the result does not
+// depend on the shape).
 hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,
                          int64_t h, int64_t nb, int block, const double* sw, int assoc,
-                         int lds_image_bytes, int threads, hipStream_t s) {
+                         int img_bytes, int count, int src_class, int threads, hipStream_t s) {
   (void)w;
   (void)h;
   if (nframes <= 0 || nb <= 0) return hipSuccess;
+  if (threads != 64 && threads != 128 && threads != 512) threads = 256;
+  while (threads > 64 && blocks_lds_bytes(img_bytes, count, src_class, threads) > kBlocksMaxLds)
+    threads /= 2;
+  const size_t lds = blocks_lds_bytes(img_bytes, count, src_class, threads);
+  if (lds > kBlocksMaxLds) return hipErrorInvalidValue;
   const int64_t nblocks = (nb + block - 1) / block;
-  switch (threads) {
-    case 64: launch_blocks_nt<64>(frames, nframes, curve, nb, block, nblocks, sw, assoc, lds_image_bytes, s); break;
-    case 128: launch_blocks_nt<128>(frames, nframes, curve, nb, block, nblocks, sw, assoc, lds_image_bytes, s); break;
-    case 512: launch_blocks_nt<512>(frames, nframes, curve, nb, block, nblocks, sw, assoc, lds_image_bytes, s); break;
-    default: launch_blocks_nt<256>(frames, nframes, curve, nb, block, nblocks, sw, assoc, lds_image_bytes, s); break;
+  switch (src_class) {
+    case kSrcF: launch_blocks_src<kSrcF>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, lds, s); break;
+    case kSrc16: launch_blocks_src<kSrc16>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, lds, s); break;
+    default: launch_blocks_src<kSrc8>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, lds, s); break;
   }
   return hipGetLastError();
 }

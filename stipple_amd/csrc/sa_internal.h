@@ -132,9 +132,16 @@ hipError_t launch_seq_walk(const SeqItem* items, int nitems, int32_t* queue, con
                            int64_t slice, uint64_t budget, uint32_t* stats, hipStream_t s);
 hipError_t launch_seq_pack(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
 
+// LDS of the block walk's workgroups at most (gfx950: 160 KiB per CU).
+constexpr size_t kBlocksMaxLds = 160 * 1024;
+// Source class of a pixel format for the block walk (the batch stages every
+// frame as the widest class among them: 0 8-bit, 1 16-bit, 2 float).
+int blocks_src_class(int fmt);
+size_t blocks_lds_bytes(int img_bytes, int count, int src_class, int threads);
+// img_bytes / count: the largest search image and palette of the frames.
 hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,
                          int64_t h, int64_t nb, int block, const double* sw, int assoc,
-                         int lds_image_bytes, int threads, hipStream_t s);
+                         int img_bytes, int count, int src_class, int threads, hipStream_t s);
 hipError_t launch_blocks_fill(const BlockFrame* frames, int nframes, int64_t gap_pixel,
                               hipStream_t s);
 

@@ -171,6 +171,73 @@ def test_blocks_u16(gpu, oracle, assoc):
         assert np.array_equal(got, exp)
 
 
+def _mixed_frames(rng, w, h, kinds):
+    out = []
+    for kind in kinds:
+        base = synth.sinusoid_image(w, h, rng, channels=3 if kind in ("rgb8", "rgb16", "f32") else 4)
+        if kind in ("rgb8", "rgba8"):
+            out.append(base)
+        elif kind in ("rgb16", "rgba16"):
+            noise = rng.integers(0, 257, base.shape)
+            out.append(np.clip(base.astype(np.int64) * 257 + noise - 128, 0, 65535).astype(np.uint16))
+        else:
+            f = synth.widen_q16(base) + rng.uniform(-300, 300, (h, w, 4)).astype(np.float32)
+            f[..., 3] = 65535.0
+            out.append(np.clip(f, 0, 65535).astype(np.float32))
+    return out
+
+
+def _expect_blocks_out(oracle, tree, f, block, dst_name, w, h):
+    exp = oracle.walk_blocks(tree, f, block)[0]
+    if dst_name == "RGBAF32":
+        return exp
+    if dst_name == "RGBA16":
+        return exp.astype(np.uint16)
+    if dst_name in ("RGB8", "RGBA8"):
+        return H.scale_char_np(exp[..., :3 if dst_name == "RGB8" else 4])
+    # INDEX8: the last visit of each pixel (position 0, the origin, is visited
+    # again at the end); the pixel the curve never reaches stays 0.
+    _, idx = oracle.walk_blocks(tree, f, block, want_index=True)
+    order = oracle.block_order(w, h)[0]
+    want = np.zeros(h * w, np.uint8)
+    want[order[1:]] = idx[0][1:]
+    return want.reshape(h, w)
+
+
+@pytest.mark.parametrize("kinds,block", [
+    (("rgb8", "rgba8", "rgb16", "rgba16", "f32"), 64),
+    (("rgb8", "rgb16", "rgba8"), 40),
+    (("rgba8", "rgb8"), 100),
+])
+def test_blocks_mixed_sources_and_outputs(gpu, oracle, kinds, block):
+    """Frames of every source class in one batch (staged as the widest class),
+    each written in another output format, against the oracle; block lengths
+    that are not a multiple of the 16-position staging chunk."""
+    import torch
+    w, h = 150, 110
+    rng = np.random.default_rng(len(kinds) * 100 + block)
+    imgs = _mixed_frames(rng, w, h, kinds)
+    outs = ["RGBA8", "RGB8", "INDEX8", "RGBA16", "RGBAF32"]
+    specs, cases = [], []
+    for k, img in enumerate(imgs):
+        tree, f = H.oracle_tree(oracle, img, 8 + 24 * k, assoc=False)
+        pal = gpu.Palette.from_tree(tree.data())
+        dst_name = outs[k % len(outs)]
+        dst_fmt = getattr(gpu, "PIX_" + dst_name)
+        src = H.to_device(img)
+        dst = H.out_alloc((h, w), dst_fmt)
+        specs.append(gpu.image_spec(src, dst, w, h, H.fmt_of(img), dst_fmt, pal))
+        cases.append((tree, f, dst_name, dst, pal, src))
+    say("blocks mixed", kinds, "B", block)
+    gpu.walk_blocks(specs, block=block)
+    torch.cuda.synchronize()
+    for tree, f, dst_name, dst, _, _ in cases:
+        say("  frame out", dst_name)
+        exp = _expect_blocks_out(oracle, tree, f, block, dst_name, w, h)
+        got = H.from_device(dst, exp.shape, exp.dtype)
+        assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), dst_name
+
+
 def test_blocks_refusals(gpu, oracle):
     rng = np.random.default_rng(1)
     img = synth.sinusoid_image(32, 32, rng, channels=3)
