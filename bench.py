@@ -356,6 +356,9 @@ def main():
                          "walk launch")
     ap.add_argument("--budget-us", type=float, default=16000.0,
                     help="pipeline: microseconds each image walks per step")
+    ap.add_argument("--served-batches", type=int, default=8,
+                    help="stills: batches submitted at once to the serving pipeline for the "
+                         "served-throughput leg (0: skip)")
     ap.add_argument("--head", type=int, default=0,
                     help="batch chain: launch the walk of the N largest images before the "
                          "rest are classified (0: one walk launch after all palettes)")
@@ -490,6 +493,44 @@ def main():
         log(f"per-image latency in the batch: p50 {chain_lat['p50']} ms, p99 {chain_lat['p99']} ms, "
             f"{pipe.steps} walk steps so far")
 
+    # Served: several copies of the batch submitted at once to the pipeline;
+    # their images walk side by side (continuous batching), so the largest
+    # image's serial walk no longer bounds the throughput.  Every output must
+    # equal the same image's output walked in its own batch.
+    served = None
+    if pipe is not None and args.served_batches > 0:
+        nb_s = args.served_batches
+        sdsts = [[torch.empty_like(d) for d in dsts] for _ in range(nb_s)]
+        torch.cuda.synchronize()
+        stills = []
+        for b in range(nb_s):
+            for k in range(len(batch)):
+                stills.append((k, Still(srcs[k], sdsts[b][k], batch[k][0].shape[1],
+                                        batch[k][0].shape[0], sa.PIX_RGBA8, sa.PIX_RGBA8,
+                                        batch[k][1], True, grays[k])))
+        stills.sort(key=lambda e: -batch[e[0]][0].shape[0] * batch[e[0]][0].shape[1])
+        t0 = time.perf_counter()
+        for _, st in stills:
+            st.t_submit = t0
+            pipe.submit(st)
+        pipe.run()
+        torch.cuda.synchronize()
+        t_served = time.perf_counter() - t0
+        errs = [st.error for _, st in stills if st.error]
+        if errs:
+            raise RuntimeError(errs[0])
+        same = all(torch.equal(sdsts[b][k], dsts[k]) for b in range(nb_s) for k in range(len(batch)))
+        lat = [st.t_done - st.t_submit for _, st in stills]
+        served = {"batches": nb_s, "images": nb_s * len(batch),
+                  "value": round(nb_s * pixels / t_served / 1e6, 3), "ms": round(t_served * 1e3, 2),
+                  "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                 "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+                  "same_as_alone": same}
+        log(f"served: {nb_s} batches at once, {served['value']} Mpx/s, "
+            f"{t_served * 1e3:.0f} ms, outputs equal to the batch's own: {same}")
+        del sdsts, stills
+        torch.cuda.synchronize()
+
     # Engine alone, palettes resident, with the walk kernel timed by events.
     specs = specs_for(pals)
     sa.timing_enable(True)
@@ -588,6 +629,7 @@ def main():
                       "pack_ms": round(pack_ms, 3)},
         "chain": args.chain,
         "latency_ms": chain_lat,
+        "served": served,
         "palette_build_ms": round(t_pal * 1e3, 3),
         "palette_build_serial_ms": round(t_pal_serial * 1e3, 3),
         "plain": plain,
@@ -602,6 +644,12 @@ def main():
         line["gpu_vs_cpu_1core"] = round(line["value"] / cpu["value"], 2)
     if plain:
         line["batched_vs_plain"] = round(line["value"] / plain["value"], 2)
+    if served:
+        if cpu:
+            served["vs_cpu_1core"] = round(served["value"] / cpu["value"], 2)
+            served["vs_cpu_all_cores"] = round(served["value"] / cpu["all_cores"]["value"], 2)
+        if plain:
+            served["vs_plain"] = round(served["value"] / plain["value"], 2)
     print(json.dumps(line), flush=True)
 
 

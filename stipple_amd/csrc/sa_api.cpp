@@ -918,15 +918,19 @@ int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int
     std::stable_sort(sel.begin(), sel.end(), [&](size_t a, size_t b) {
       return job->items[a].it.n - job->items[a].t > job->items[b].it.n - job->items[b].t;
     });
-    std::vector<sa::SeqItem> its;
     size_t lds = 0;
     for (size_t k : sel) {
       const JobItem& J = job->items[k];
-      its.push_back(J.it);
       lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes);
     }
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
     int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
+    // A bounded step (slice or budget) over more images than wavefronts walks
+    // the longest ones only; a step to the end walks every image (the
+    // persistent grid pulls the rest as walks finish).
+    if ((slice > 0 || budget > 0) && (int)sel.size() > waves) sel.resize((size_t)waves);
+    std::vector<sa::SeqItem> its;
+    for (size_t k : sel) its.push_back(job->items[k].it);
     waves = std::min<int>(waves, (int)its.size());
     const size_t items_bytes = (sizeof(sa::SeqItem) * its.size() + 255) & ~(size_t)255;
     unsigned char* d = nullptr;
@@ -1076,17 +1080,30 @@ extern "C" int sa_walk_job_step_overlapped(sa_walk_job job, double budget_us, vo
     if (!F.ev) HIP_TRY(hipEventCreateWithFlags(&F.ev, hipEventDisableTiming));
     size_t off = 0;
     for (int assoc = 0; assoc < 2; ++assoc) {
-      std::vector<sa::SeqItem> its;
+      std::vector<const JobItem*> cand;
       size_t lds = 0;
       for (const JobItem& J : job->items) {
         if (J.assoc != assoc) continue;
-        its.push_back(J.it);
-        F.ids.push_back(J.id);
+        cand.push_back(&J);
         lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes);
       }
-      if (its.empty()) continue;
+      if (cand.empty()) continue;
       const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
       int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
+      // Most visits left first: the grid's first wavefronts (one per CU) take
+      // the longest serial walks.  With more images than wavefronts the step
+      // walks only the first `waves` (the rest wait), so the longest walks
+      // advance a full budget every step instead of sharing rounds of the
+      // launch with short ones.
+      std::stable_sort(cand.begin(), cand.end(), [](const JobItem* a, const JobItem* b) {
+        return a->it.n - a->t > b->it.n - b->t;
+      });
+      if ((int)cand.size() > waves) cand.resize((size_t)waves);
+      std::vector<sa::SeqItem> its;
+      for (const JobItem* J : cand) {
+        its.push_back(J->it);
+        F.ids.push_back(J->id);
+      }
       waves = std::min<int>(waves, (int)its.size());
       const size_t items_bytes = (sizeof(sa::SeqItem) * its.size() + 255) & ~(size_t)255;
       unsigned char* d = nullptr;

@@ -202,3 +202,30 @@ def test_job_overlapped_steps(gpu, oracle):
     for c in cases:
         _check(oracle, c)
     job.close()
+
+
+@pytest.mark.parametrize("mode", ["overlapped", "timed", "slice"])
+def test_job_more_images_than_wavefronts(gpu, oracle, mode):
+    """Seven images on two wavefronts: a bounded step walks only the images
+    with the most visits left and the others wait their turn; every image
+    still finishes as one uninterrupted walk."""
+    import torch
+    say("job waves=2", mode)
+    cases = [_case(gpu, oracle, img, col) for img, col in _images(15)]
+    job = gpu.WalkJob(gpu.SeqOptions(memo_bits=5, waves=2))
+    ids = job.add([c["spec"] for c in cases])
+    finished, steps = [], 0
+    while job.active:
+        if mode == "overlapped":
+            finished += job.step_overlapped(40.0)
+        elif mode == "timed":
+            finished += job.step_timed(40.0)
+        else:
+            finished += job.step(480)
+        steps += 1
+        assert steps < 200000
+    torch.cuda.synchronize()
+    assert sorted(finished) == ids
+    for c in cases:
+        _check(oracle, c)
+    job.close()
