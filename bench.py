@@ -361,6 +361,9 @@ def main():
     ap.add_argument("--served-batches", type=int, default=8,
                     help="stills: batches submitted at once to the serving pipeline for the "
                          "served-throughput leg (0: skip)")
+    ap.add_argument("--served-memo-bits", type=int, default=0,
+                    help="stills: LDS memo size (seq.memo_bits, 4..10) for the served leg; "
+                         "smaller memos let more images share a CU (0: library default)")
     ap.add_argument("--head", type=int, default=0,
                     help="batch chain: launch the walk of the N largest images before the "
                          "rest are classified (0: one walk launch after all palettes)")
@@ -511,6 +514,9 @@ def main():
                                         batch[k][0].shape[0], sa.PIX_RGBA8, sa.PIX_RGBA8,
                                         batch[k][1], True, grays[k])))
         stills.sort(key=lambda e: -batch[e[0]][0].shape[0] * batch[e[0]][0].shape[1])
+        memo_before = sa.tune_get("seq.memo_bits")
+        if args.served_memo_bits:
+            sa.tune_set("seq.memo_bits", args.served_memo_bits)
         t0 = time.perf_counter()
         for _, st in stills:
             st.t_submit = t0
@@ -518,12 +524,14 @@ def main():
         pipe.run()
         torch.cuda.synchronize()
         t_served = time.perf_counter() - t0
+        sa.tune_set("seq.memo_bits", memo_before)
         errs = [st.error for _, st in stills if st.error]
         if errs:
             raise RuntimeError(errs[0])
         same = all(torch.equal(sdsts[b][k], dsts[k]) for b in range(nb_s) for k in range(len(batch)))
         lat = [st.t_done - st.t_submit for _, st in stills]
         served = {"batches": nb_s, "images": nb_s * len(batch),
+                  "memo_bits": args.served_memo_bits or None,
                   "value": round(nb_s * pixels / t_served / 1e6, 3), "ms": round(t_served * 1e3, 2),
                   "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                                  "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},

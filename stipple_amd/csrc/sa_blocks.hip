@@ -32,6 +32,7 @@
 // every position except 0 owns its pixel and the write needs no index buffer.
 #include "sa_internal.h"
 #include "sa_math.h"
+#include "sa_search.h"
 
 namespace sa {
 namespace {
@@ -151,77 +152,6 @@ __device__ __forceinline__ void store_out(const BlockFrame& f, int64_t pix, int 
   }
 }
 
-// The search image in LDS plus each candidate's distance constants:
-// dcent[k] = (alpha e0, alpha e1, alpha e2, e3) of the k-th candidate in
-// post-order, alpha = QuantumScale e3 when associated and 1 otherwise -- the
-// products candidate_distance forms from the entry, formed the same way.
-struct LSearch {
-  const double* col;
-  const float4* ent;
-  const int2* sub;
-  const uint32_t* desc;
-  const int16_t* child;
-  const uint16_t* cand;
-  const double4* dcent;
-  int kids;
-};
-
-// Descent (src/rd_riemersma_cpu.cpp:82-89; the first three levels are one
-// table read) and the ordered scan of src/rd_octree.cpp:178-217 with the
-// distance of candidate_distance: the last candidate at the running minimum.
-template <bool ASSOC>
-__device__ __forceinline__ int lane_lookup(const LSearch& s, const float pc[4], const double t[4]) {
-  const unsigned c0 = scale_char(pc[0]);
-  const unsigned c1 = scale_char(pc[1]);
-  const unsigned c2 = scale_char(pc[2]);
-  const unsigned c3 = ASSOC ? scale_char(pc[3]) : 0u;
-  uint32_t de = (c0 >> 5) | ((c1 >> 5) << 3) | ((c2 >> 5) << 6);
-  if (ASSOC) de |= (c3 >> 5) << 9;
-  const uint32_t d3 = s.desc[de];
-  int prev = (int)((d3 >> 16) & 0x7FFFu);
-  if (!(d3 & kDescStop)) {
-    int row = (int)(d3 & 0x7FFFu);
-    for (int lev = 4; lev > 0; --lev) {
-      const int ch = s.child[row * s.kids + node_id(ASSOC, c0, c1, c2, c3, lev)];
-      if (ch == kNoChild) break;
-      prev = row;
-      if (ch == kLeaf) break;
-      row = ch;
-    }
-  }
-  const int2 r = s.sub[prev];
-  // candidate_distance's per-pixel factors: beta = QuantumScale t3 when
-  // associated (beta t_c formed once), 1 otherwise (1 * t_c == t_c).
-  double bt0 = t[0], bt1 = t[1], bt2 = t[2];
-  if (ASSOC) {
-    const double beta = kScale * t[3];
-    bt0 = beta * t[0];
-    bt1 = beta * t[1];
-    bt2 = beta * t[2];
-  }
-  double best = kFarDistance;
-  int bk = 0;
-  const double4* dc = s.dcent + r.x;
-  for (int k = 0; k < r.y; ++k) {
-    const double4 e = dc[k];
-    double v = e.x - bt0;
-    double d = v * v;
-    v = e.y - bt1;
-    d += v * v;
-    v = e.z - bt2;
-    d += v * v;
-    if (ASSOC) {
-      v = e.w - t[3];
-      d += v * v;
-    }
-    if (d <= best) {
-      best = d;
-      bk = k;
-    }
-  }
-  return (int)s.cand[r.x + bk];
-}
-
 template <bool ASSOC>
 struct Queue {
   double e[ASSOC ? 4 : 3][kChunk];
@@ -316,27 +246,7 @@ void blocks_kernel(const BlockFrame* __restrict__ frames,
   double4* dcent = (double4*)(lds + img_bytes);
   T* raw = (T*)(lds + img_bytes + 32 * count);
   uint16_t* oidx = (uint16_t*)(raw + kChunk * (NT + 1));
-  {
-    const uint4* src = (const uint4*)pd.image;
-    uint4* dst = (uint4*)lds;
-    for (int k = tid; k < pd.bytes / 16; k += NT) dst[k] = src[k];
-    const float4* cent = (const float4*)((const unsigned char*)pd.image + pd.off_cent);
-    for (int k = tid; k < pd.count; k += NT) {
-      const float4 e = cent[k];
-      double a = 1.0;
-      if (ASSOC) a = kScale * (double)e.w;
-      dcent[k] = make_double4(a * (double)e.x, a * (double)e.y, a * (double)e.z, (double)e.w);
-    }
-  }
-  LSearch s;
-  s.col = (const double*)lds;
-  s.ent = (const float4*)(lds + pd.off_ent);
-  s.sub = (const int2*)(lds + pd.off_sub);
-  s.desc = (const uint32_t*)(lds + pd.off_desc);
-  s.child = (const int16_t*)(lds + pd.off_child);
-  s.cand = (const uint16_t*)(lds + pd.off_cand);
-  s.dcent = dcent;
-  s.kids = pd.kids;
+  const LSearch s = stage_search<ASSOC>(pd, lds, dcent, tid, NT);
   const int64_t b0 = x * NT;
   const int64_t m = nb - 1;  // positions [0, m) follow the curve; m is the origin
   // This lane's block and its length (0 past the last block).
