@@ -19,10 +19,14 @@
 //      nodes by first pair reproduces the reference's creation order, hence its
 //      node numbering.
 //   4. Ordered sums.  quantize_error, total_color and number_unique of a node
-//      are sums over its pairs in run order.  A stable radix sort of the pairs
-//      by node puts each node's pairs in that order; every pair's term is
-//      computed in parallel, then each node's chain is folded in order over
-//      contiguous memory.  The totals of loop 2 are folded after
+//      are sums over its pairs in run order.  Pairs are stored level-major
+//      (each level's pairs are one run sequence), so the consecutive runs that
+//      share a node at a level form an interval of consecutive pairs; a stable
+//      radix sort of the intervals by node, expanded, puts each node's pairs in
+//      run order (at low levels an interval spans many runs, so there are far
+//      fewer intervals than pairs); every pair's term is computed in parallel,
+//      then each node's chain is folded in order over contiguous memory.  The
+//      totals of loop 2 are folded after
 //      PruneToCubeDepth has merged loop 1's deep leaves, exactly when the
 //      reference adds them.
 //   5. Tree walks (section 5 explains the restatement): the gated
@@ -110,11 +114,6 @@ __device__ __forceinline__ void node_mid(uint64_t k, double mid[4]) {
   }
 }
 
-struct Runs {
-  const int32_t* count;
-  const double4* px;
-};
-
 struct Terms {
   double* term;   // count * |pixel - mid| per sorted pair
   float4* lval;   // clamped leaf channels (leaf pairs only)
@@ -182,7 +181,6 @@ struct Frame {
   TreeWork W;
   int32_t* seg_b;
   int32_t* seg_e;
-  const int32_t* snode;    // batch node ids (node_base + node) in sorted pair order
   const uint32_t* spair;   // pair indices of this image in sorted pair order
   Terms T;
   double* record;
@@ -197,8 +195,6 @@ struct Batch {
   uint32_t* eslot;         // node entries: hash slot
   const uint32_t* skey;    // the same, sorted
   const uint32_t* sslot;
-  int32_t* pnode;          // batch node id per pair
-  uint32_t* pidx;          // pair index per pair
   uint64_t* mid_list;      // (image << 32) | node, segments folded by one wavefront
   uint64_t* long_list;     // the same, segments folded by one workgroup
   uint64_t* huge_list;     // the same, the longest: taken first
@@ -366,51 +362,102 @@ __global__ __launch_bounds__(1024) void k_break(const Frame* __restrict__ fr) {
 }
 
 // ---- 3. nodes ---------------------------------------------------------------------
-__device__ __forceinline__ void pair_decode(uint32_t p, int64_t R1, int depth, int64_t* r,
-                                            int* l) {
-  const uint64_t l1 = 8ull * (uint64_t)R1;
-  if (p < l1) {
-    *r = p >> 3;
-    *l = (int)(p & 7) + 1;
+// Pair numbering.  Loop 1 classifies runs [0, R1) at levels 1..8, loop 2 runs
+// [R1, R) at levels 1..depth.  Pairs are stored level-major within each loop:
+//   loop 1: p = (l - 1) R1 + r,                     p in [0, 8 R1)
+//   loop 2: p = 8 R1 + (l - 1) (R - R1) + (r - R1), p in [8 R1, P)
+// so a level's pairs are consecutive runs.  The reference's order of the pairs
+// (run-major: its creation order, and the order of the root chain) is
+// pair_rmajor; both numberings put loop 1 first and order one node's pairs
+// (all at one level) by run.
+__device__ __forceinline__ void pair_decode(uint32_t p, int64_t R1, int64_t R, int depth,
+                                            int64_t* r, int* l) {
+  const int64_t l1 = 8 * R1;
+  int64_t q, n;
+  int64_t base;
+  int levels;
+  if ((int64_t)p < l1) {
+    q = p;
+    n = R1;
+    base = 0;
+    levels = 8;
   } else {
-    const uint64_t q = p - l1;
-    *r = R1 + (int64_t)(q / depth);
-    *l = (int)(q % depth) + 1;
+    q = (int64_t)p - l1;
+    n = R - R1;
+    base = R1;
+    levels = depth;
   }
+  int k = 0;  // level - 1: the number of level blocks wholly before q
+  int64_t edge = n;
+  for (int j = 1; j < 8; ++j) {
+    if (j < levels && q >= edge) {
+      ++k;
+      edge += n;
+    }
+  }
+  *l = k + 1;
+  *r = base + q - (int64_t)k * n;
 }
 
-// Pair (r-1, l) always exists and precedes (r, l) in pair order (both loops
-// classify every run down to level depth <= 8).  When the two runs share the
-// level-l node, (r, l) cannot be its node's first pair, so it skips the insert
-// (kNoSlot) and k_node_lookup finds its slot later without atomics.  This
-// keeps the smooth regions of an image, where thousands of consecutive runs
-// share their low-level nodes, from serialising on a few hash slots.
+__device__ __forceinline__ uint32_t pair_rmajor(int64_t r, int l, int64_t R1, int depth) {
+  return (uint32_t)(r < R1 ? 8 * r + (l - 1) : 8 * R1 + (r - R1) * depth + (l - 1));
+}
+
+// A pair that reaches a leaf: level 8 in loop 1, level depth in loop 2.
+__device__ __forceinline__ bool leaf_pair(uint32_t p, int64_t R1, int depth, int l) {
+  return p < 8ull * (uint64_t)R1 ? l == 8 : l == depth;
+}
+
+// The first pair of an interval (see the top of the file) inserts its node;
+// the other pairs of an interval share the node of the pair before them, so
+// they insert nothing and never need their slot: k_compact reads the node of
+// each interval's first pair.  This also keeps the smooth regions of an image,
+// where thousands of consecutive runs share their low-level nodes, from
+// serialising on a few hash slots.  A node's first pair (its creation order)
+// is the least run-major index of its pairs, which is always an interval's
+// first.  wgc[workgroup] counts the intervals that begin in the workgroup's
+// 256 pairs (their sum, the batch's intervals, is reduced into ctl[12]).
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
-__global__ void k_node_insert(const Frame* __restrict__ fr, Map m) {
+__global__ __launch_bounds__(256) void k_node_insert(const Frame* __restrict__ fr, Map m,
+                                                     int32_t* __restrict__ wgc) {
+  __shared__ int s_starts;
+  if (threadIdx.x == 0) s_starts = 0;
+  __syncthreads();
   const At at = map_at(m, 256);
   const Frame& d = fr[at.f];
   const int64_t p = at.i;
-  if (p >= d.P) return;
-  const uint32_t* pack = d.pack;
+  const bool valid = p < d.P;
   const int64_t R1 = d.R1;
   const int depth = d.depth, assoc = d.assoc;
-  uint64_t* hkeys = d.nkeys;
-  uint32_t* hfirst = d.nfirst;
-  const uint32_t cap = d.ncap;
+  int64_t r = 0;
+  int l = 1;
+  uint64_t key = 0;
+  bool start = false;
+  if (valid) {
+    pair_decode((uint32_t)p, R1, d.R, depth, &r, &l);
+    key = node_key(d.pack[r], l, assoc != 0);
+    const bool first_run = r == 0 || r == R1;  // the first run of its loop
+    start = first_run || node_key(d.pack[r - 1], l, assoc != 0) != key;
+  }
+  const unsigned long long starts = __ballot(start);
+  if ((threadIdx.x & 63) == 0 && starts) atomicAdd(&s_starts, __popcll(starts));
+  __syncthreads();
+  if (threadIdx.x == 0) wgc[blockIdx.x] = s_starts;
+  if (!valid) return;
   uint32_t* pair_slot = d.pair_slot;
-  int32_t* info = d.info;
-  int64_t r;
-  int l;
-  pair_decode((uint32_t)p, R1, depth, &r, &l);
-  const uint64_t key = node_key(pack[r], l, assoc != 0);
-  if (r > 0 && node_key(pack[r - 1], l, assoc != 0) == key) {
+  if (!start) {
     pair_slot[p] = kNoSlot;
     return;
   }
+  uint64_t* hkeys = d.nkeys;
+  uint32_t* hfirst = d.nfirst;
+  const uint32_t cap = d.ncap;
+  int32_t* info = d.info;
+  const uint32_t c = pair_rmajor(r, l, R1, depth);
   // A plain read first: a key never changes once set and `first` only
   // decreases, so a stale read costs at most an atomic that finds the same.
-  // Most pairs find their node already inserted, by an earlier pair.
+  // Most intervals find their node already inserted, by an earlier one.
   uint32_t slot = hash32(key) & (cap - 1);
   for (uint32_t probe = 0;; ++probe) {
     if (probe >= cap) {  // table full: report, never spin
@@ -428,38 +475,7 @@ __global__ void k_node_insert(const Frame* __restrict__ fr, Map m) {
     }
     slot = (slot + 1) & (cap - 1);
   }
-  if (hfirst[slot] > (uint32_t)p) atomicMin(&hfirst[slot], (uint32_t)p);
-  pair_slot[p] = slot;
-}
-
-// Slots of the pairs k_node_insert skipped (their key is in the table).
-__global__ void k_node_lookup(const Frame* __restrict__ fr, Map m) {
-  const At at = map_at(m, 256);
-  const Frame& d = fr[at.f];
-  const int64_t p = at.i;
-  uint32_t* pair_slot = d.pair_slot;
-  if (p >= d.P || pair_slot[p] != kNoSlot) return;
-  const uint32_t* pack = d.pack;
-  const int64_t R1 = d.R1;
-  const int depth = d.depth, assoc = d.assoc;
-  const uint64_t* hkeys = d.nkeys;
-  const uint32_t cap = d.ncap;
-  int32_t* info = d.info;
-  int64_t r;
-  int l;
-  pair_decode((uint32_t)p, R1, depth, &r, &l);
-  const uint64_t key = node_key(pack[r], l, assoc != 0);
-  uint32_t slot = hash32(key) & (cap - 1);
-  uint32_t probe = 0;
-  while (hkeys[slot] != key && probe < cap) {
-    slot = (slot + 1) & (cap - 1);
-    ++probe;
-  }
-  if (probe >= cap) {  // never inserted: report
-    info[6] = 4;
-    pair_slot[p] = 0;
-    return;
-  }
+  if (hfirst[slot] > c) atomicMin(&hfirst[slot], c);
   pair_slot[p] = slot;
 }
 
@@ -497,27 +513,176 @@ __global__ void k_node_assign(const Frame* __restrict__ fr, Map m, Batch B) {
   if (loop1) atomicAdd(&info[4], 1);                        // nodes created by loop 1
 }
 
-// Sort keys: the batch node id of every pair.  The pairs of an image that
-// failed sort onto its own single node id, so they stay inside its range.
-__global__ void k_pair_nodes(const Frame* __restrict__ fr, Map m, Batch B) {
+// The intervals in pair order: the workgroups of k_node_insert's grid again,
+// each writing its intervals from its offset wgo (the exclusive scan of wgc)
+// on.  Interval k: its first pair istart[k] (batch-wide, ascending) and its
+// sort key, the batch node id of that pair (an image that failed sorts onto its
+// own single node id, so its pairs stay inside its range).  An interval runs
+// up to the next one's first pair (every level block, and every image, begins
+// one), the end of the batch's pairs after the last.
+__global__ __launch_bounds__(256) void k_compact(const Frame* __restrict__ fr, Map m,
+                                                 const int32_t* __restrict__ wgo,
+                                                 const uint32_t* __restrict__ pair_slot,
+                                                 uint32_t* __restrict__ istart,
+                                                 int32_t* __restrict__ ikey,
+                                                 uint32_t* __restrict__ ival) {
+  __shared__ int s_wave[4];
   const At at = map_at(m, 256);
   const Frame& d = fr[at.f];
   const int64_t p = at.i;
-  if (p >= d.P) return;
   const int64_t g = d.pair_base + p;
-  B.pnode[g] = (int32_t)d.node_base + (d.ok ? d.slot_node[d.pair_slot[p]] : 0);
-  B.pidx[g] = (uint32_t)p;
+  const bool start = p < d.P && pair_slot[g] != kNoSlot;
+  const unsigned long long bits = __ballot(start);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_wave[wave] = __popcll(bits);
+  __syncthreads();
+  if (!start) return;
+  int k = wgo[blockIdx.x] + __popcll(bits & ((1ull << lane) - 1));
+  for (int w = 0; w < wave; ++w) k += s_wave[w];
+  istart[k] = (uint32_t)g;
+  ikey[k] = (int32_t)d.node_base + (d.ok ? d.slot_node[pair_slot[g]] : 0);
+  ival[k] = (uint32_t)k;
 }
 
-__global__ void k_segments(const Frame* __restrict__ fr, Map m) {
-  const At at = map_at(m, 256);
-  const Frame& d = fr[at.f];
-  const int64_t i = at.i, P = d.P;
-  if (i >= P || !d.ok) return;
-  const int32_t* snode = d.snode;
-  const int32_t n = snode[i];
-  if (i == 0 || snode[i - 1] != n) d.seg_b[n - d.node_base] = (int32_t)i;
-  if (i == P - 1 || snode[i + 1] != n) d.seg_e[n - d.node_base] = (int32_t)(i + 1);
+// Lengths in sorted order (then an exclusive scan gives each sorted
+// interval's first position).
+__global__ void k_interval_len(const uint32_t* __restrict__ sival,
+                               const uint32_t* __restrict__ istart, int64_t ni, int64_t sum_p,
+                               int32_t* __restrict__ slen) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ni) return;
+  const int64_t k = sival[j];
+  slen[j] = (int32_t)((k + 1 < ni ? (int64_t)istart[k + 1] : sum_p) - (int64_t)istart[k]);
+}
+
+// The image whose pair range holds batch position i: the last with
+// pair_base <= i (images without pairs have empty ranges at the running
+// offset; an image's sorted positions are its pairs' range).
+__device__ __forceinline__ int image_of_pair(const Frame* __restrict__ fr, int F, int64_t i) {
+  int lo = 0, hi = F - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fr[mid].pair_base <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Sorted positions are expanded a tile of kExpandTile at a time.  Sorted
+// interval j covers positions [spos[j], spos[j] + slen[j]); it is the first
+// interval of every tile whose first position it covers.  A node's segment
+// (its pairs' sorted positions, as positions of its image) begins with its
+// first interval and ends with its last.
+constexpr int kExpandTile = 1024;
+
+__global__ void k_interval_ends(const Frame* __restrict__ fr, int F,
+                                const int32_t* __restrict__ sikey,
+                                const int32_t* __restrict__ spos, const int32_t* __restrict__ slen,
+                                int64_t ni, int32_t* __restrict__ tile_first) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ni) return;
+  const int64_t b = spos[j], e = b + slen[j];
+  for (int64_t t = (b + kExpandTile - 1) / kExpandTile; t * kExpandTile < e; ++t)
+    tile_first[t] = (int32_t)j;
+  const int32_t n = sikey[j];
+  const bool first = j == 0 || sikey[j - 1] != n, last = j + 1 == ni || sikey[j + 1] != n;
+  if (!first && !last) return;
+  const Frame& d = fr[image_of_pair(fr, F, b)];
+  if (!d.ok) return;
+  if (first) d.seg_b[n - d.node_base] = (int32_t)(b - d.pair_base);
+  if (last) d.seg_e[n - d.node_base] = (int32_t)(e - d.pair_base);
+}
+
+// Sorted position i of image d (li = i - pair_base): pair p of interval
+// `node` (a batch node id); its term (distance x count), run length and leaf
+// value, formed as quantize.c's classification accumulates them.
+__device__ __forceinline__ void expand_one(const Frame& d, int64_t li, uint32_t p, int32_t node,
+                                           uint32_t* __restrict__ spair) {
+  spair[d.pair_base + li] = p;
+  if (!d.ok) return;
+  const int64_t R1 = d.R1;
+  const int depth = d.depth, assoc = d.assoc;
+  int64_t r;
+  int l;
+  pair_decode(p, R1, d.R, depth, &r, &l);
+  double mid[4];
+  node_mid(d.t.key[node - d.node_base], mid);
+  const double4 v = d.px[r];
+  const int32_t cnt = d.count[r];
+  const double e0 = kScale * (v.x - mid[0]);
+  const double e1 = kScale * (v.y - mid[1]);
+  const double e2 = kScale * (v.z - mid[2]);
+  const double e3 = assoc ? kScale * (v.w - mid[3]) : 0.0;
+  double dd = e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
+  if (dd != dd) dd = 0.0;
+  const Terms& T = d.T;
+  T.term[li] = (double)cnt * sqrt(dd);
+  T.lcnt[li] = cnt;
+  if (leaf_pair(p, R1, depth, l))
+    T.lval[li] = make_float4(clamp_pixel(v.x), clamp_pixel(v.y), clamp_pixel(v.z),
+                             clamp_pixel(assoc ? v.w : kRange));
+}
+
+// Tile t: its intervals (at most kExpandTile + 1 touch it) are staged in LDS;
+// sorted position i takes the node of the last interval starting at or before
+// it, and the pair that interval's first pair plus the offset into it.
+__global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, int F, int64_t sum_p,
+                                                int64_t ni, const int32_t* __restrict__ tile_first,
+                                                const int32_t* __restrict__ sikey,
+                                                const uint32_t* __restrict__ sival,
+                                                const uint32_t* __restrict__ istart,
+                                                const int32_t* __restrict__ spos,
+                                                uint32_t* __restrict__ spair) {
+  __shared__ int32_t l_pos[kExpandTile + 1];
+  __shared__ int32_t l_node[kExpandTile + 1];
+  __shared__ uint32_t l_first[kExpandTile + 1];
+  __shared__ int s_f;
+  __shared__ int64_t s_next;
+  const int64_t i0 = (int64_t)blockIdx.x * kExpandTile;
+  const int64_t i1 = i0 + kExpandTile < sum_p ? i0 + kExpandTile : sum_p;
+  const int64_t j0 = tile_first[blockIdx.x];
+  for (int k = threadIdx.x; k <= kExpandTile; k += blockDim.x) {
+    const int64_t j = j0 + k;
+    int32_t pos = INT32_MAX;
+    if (j < ni) {
+      pos = spos[j];
+      if (pos < i1) {
+        l_node[k] = sikey[j];
+        l_first[k] = istart[sival[j]];
+      } else {
+        pos = INT32_MAX;
+      }
+    }
+    l_pos[k] = pos;
+  }
+  if (threadIdx.x == 0) {
+    const int f = image_of_pair(fr, F, i0);
+    s_f = f;
+    s_next = f + 1 < F ? fr[f + 1].pair_base : INT64_MAX;
+  }
+  __syncthreads();
+  const int f0 = __builtin_amdgcn_readfirstlane(s_f);
+  const bool one_image = i1 <= s_next;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    int lo = 0, hi = kExpandTile;  // l_pos[0] <= i0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (l_pos[mid] <= i)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int64_t g = (int64_t)l_first[lo] + (i - l_pos[lo]);
+    if (one_image) {
+      const Frame& d = fr[f0];
+      expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[lo], spair);
+    } else {
+      const Frame& d = fr[image_of_pair(fr, F, i)];
+      expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[lo], spair);
+    }
+  }
 }
 
 // The root of every image: key 0, its own parent, level 0, zero sums.
@@ -577,39 +742,6 @@ __global__ void k_links(const Frame* __restrict__ fr, Map m) {
 // in node order; then each chain is a plain in-order fold over contiguous
 // memory -- one thread per node for short segments, one wavefront per node for
 // long ones, staging chunks through LDS while the next chunks load.
-__device__ __forceinline__ bool leaf_pair(uint32_t p, int64_t R1, int depth, int l) {
-  return p < 8ull * (uint64_t)R1 ? l == 8 : l == depth;
-}
-
-__global__ void k_pair_terms(const Frame* __restrict__ fr, Map m) {
-  const At at = map_at(m, 256);
-  const Frame& fd = fr[at.f];
-  const int64_t i = at.i;
-  if (i >= fd.P || !fd.ok) return;
-  const Runs R{fd.count, fd.px};
-  const Terms& T = fd.T;
-  const int64_t R1 = fd.R1;
-  const int depth = fd.depth, assoc = fd.assoc;
-  const uint32_t p = fd.spair[i];
-  int64_t r;
-  int l;
-  pair_decode(p, R1, depth, &r, &l);
-  double mid[4];
-  node_mid(fd.t.key[fd.snode[i] - fd.node_base], mid);
-  const double4 v = R.px[r];
-  const int32_t cnt = R.count[r];
-  const double e0 = kScale * (v.x - mid[0]);
-  const double e1 = kScale * (v.y - mid[1]);
-  const double e2 = kScale * (v.z - mid[2]);
-  const double e3 = assoc ? kScale * (v.w - mid[3]) : 0.0;
-  double d = e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
-  if (d != d) d = 0.0;
-  T.term[i] = (double)cnt * sqrt(d);
-  T.lcnt[i] = cnt;
-  if (leaf_pair(p, R1, depth, l))
-    T.lval[i] = make_float4(clamp_pixel(v.x), clamp_pixel(v.y), clamp_pixel(v.z),
-                            clamp_pixel(assoc ? v.w : kRange));
-}
 
 // First sorted position in [b, e) whose pair belongs to loop 2 (pairs of a
 // node are in ascending pair order, loop 1's first).
@@ -1403,8 +1535,8 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_long(const Frame* __restr
   }
 }
 
-// The running values of every pair, stored by pair index for the exact root
-// chain (only run when the bound cannot separate the root's error).
+// The running values of every pair, stored by run-major pair index for the
+// exact root chain (only run when the bound cannot separate the root's error).
 __global__ void k_fold_record(const Frame* __restrict__ fr, Map m) {
   const At at = map_at(m, 64);
   const Frame& d = fr[at.f];
@@ -1413,7 +1545,10 @@ __global__ void k_fold_record(const Frame* __restrict__ fr, Map m) {
   double qe = 0.0;
   for (int32_t i = d.seg_b[n]; i < d.seg_e[n]; ++i) {
     qe += d.T.term[i];
-    d.record[d.spair[i]] = qe;
+    int64_t r;
+    int l;
+    pair_decode(d.spair[i], d.R1, d.R, d.depth, &r, &l);
+    d.record[pair_rmajor(r, l, d.R1, d.depth)] = qe;  // the root chain's order
   }
 }
 
@@ -2342,17 +2477,32 @@ class SubBuild {
       return hipSuccess;
     }
     sumP_ = sumP;
+    // Every image's pair range at its running offset (empty for the images
+    // that failed), so the ranges are ascending and tile [0, sumP).
+    nwg_ = 0;
+    for (int64_t f = 0, run = 0; f < F_; ++f) {
+      H_[f].pair_base = run;
+      run += H_[f].P;
+      nwg_ += blocks(H_[f].P, 256);  // k_node_insert's workgroups (the kMapP grid)
+    }
     uint64_t* nkeys = nullptr;
     uint32_t *nfirst = nullptr, *pair_slot = nullptr;
     Carve c;
     c.want(&nkeys, sumC);
     c.want(&nfirst, sumC);
+    size_t tmp_red = 0;
+    hipError_t e = hipcub::DeviceReduce::Sum(nullptr, tmp_red, (int32_t*)nullptr,
+                                             (int32_t*)nullptr, (int)nwg_, s_);
+    if (e != hipSuccess) return e;
+    char* red_tmp = nullptr;
     c.want(&pair_slot, sumP);
+    c.want(&wgc_, nwg_);
+    c.want(&red_tmp, (int64_t)tmp_red);
     c.want(&B_.ekey, sumC);
     c.want(&B_.eslot, sumC);
     c.want(&slot_node_, sumC);
-    hipError_t e = c.take(owned_, s_);
-    if (e != hipSuccess) return e;
+    if ((e = c.take(owned_, s_)) != hipSuccess) return e;
+    pair_slot_ = pair_slot;
     int64_t co = 0;
     for (int f = 0; f < F_; ++f) {
       Frame& d = H_[f];
@@ -2368,8 +2518,10 @@ class SubBuild {
     fill(nkeys, sumC, kEmpty64);
     fill(nfirst, sumC, kEmpty32);
     if (grid_[kMapP]) {
-      k_node_insert<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
-      k_node_lookup<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
+      k_node_insert<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP), wgc_);
+      if ((e = hipcub::DeviceReduce::Sum(red_tmp, tmp_red, wgc_, B_.ctl + 12, (int)nwg_, s_)) !=
+          hipSuccess)
+        return e;
       k_node_collect<<<grid_[kMapCap], 256, 0, s_>>>(d_fr_, map(kMapCap), B_);
     }
     // The outputs outlive the batch (plain allocations owned by the caller):
@@ -2414,19 +2566,27 @@ class SubBuild {
     if (sumP == 0) return readback();
     for (int f = 0; f < F_; ++f)
       if (H_[f].ok) sumW += H_[f].M + 2;
+    const int64_t NI = hctl()[12];  // intervals (k_node_insert)
     const int bitsA = bits_for(sumP), bitsB = bits_for(sumNodes);
-    size_t tmpA = 0, tmpB = 0;
+    size_t tmpA = 0, tmpB = 0, tmpSel = 0, tmpSum = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpA, B_.ekey, (uint32_t*)nullptr,
                                                       B_.eslot, (uint32_t*)nullptr, (int)E, 0,
                                                       bitsA, s_);
     if (e != hipSuccess) return e;
     e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpB, (int32_t*)nullptr, (int32_t*)nullptr,
-                                           (uint32_t*)nullptr, (uint32_t*)nullptr, (int)sumP, 0,
+                                           (uint32_t*)nullptr, (uint32_t*)nullptr, (int)NI, 0,
                                            bitsB, s_);
     if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmpSel, (int32_t*)nullptr, (int32_t*)nullptr,
+                                         (int)nwg_, s_);
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmpSum, (int32_t*)nullptr, (int32_t*)nullptr,
+                                         (int)NI, s_);
+    if (e != hipSuccess) return e;
+    const size_t tmpBytes = std::max({tmpA, tmpB, tmpSel, tmpSum});
     TreeDev t{};
     uint32_t *skey = nullptr, *sslot = nullptr, *spair = nullptr;
-    int32_t *pp = nullptr, *snode = nullptr, *seg_b = nullptr, *seg_e = nullptr, *lcnt = nullptr;
+    int32_t *pp = nullptr, *seg_b = nullptr, *seg_e = nullptr, *lcnt = nullptr;
     double* term = nullptr;
     float4* lval = nullptr;
     int32_t* w[11] = {};
@@ -2446,9 +2606,21 @@ class SubBuild {
     c.want(&t.row, sumNodes);
     c.want(&skey, E);
     c.want(&sslot, E);
-    c.want(&pp, 2 * sumP);  // pnode | pidx; later the running values of the exact root
-    c.want(&snode, sumP);
+    c.want(&pp, 2 * sumP);  // the running values of the exact root chain
     c.want(&spair, sumP);
+    uint32_t *istart = nullptr, *ival = nullptr, *sival = nullptr;
+    int32_t *ikey = nullptr, *sikey = nullptr, *slen = nullptr, *spos = nullptr, *wgo = nullptr;
+    c.want(&istart, NI);
+    c.want(&ival, NI);
+    c.want(&sival, NI);
+    c.want(&ikey, NI);
+    c.want(&sikey, NI);
+    c.want(&slen, NI);
+    c.want(&spos, NI);
+    c.want(&wgo, nwg_);
+    const int64_t ntiles = (sumP + kExpandTile - 1) / kExpandTile;
+    int32_t* tile_first = nullptr;
+    c.want(&tile_first, ntiles);
     c.want(&seg_b, sumNodes);
     c.want(&seg_e, sumNodes);
     c.want(&term, sumP);
@@ -2459,12 +2631,10 @@ class SubBuild {
     c.want(&B_.huge_list, sumNodes);
     for (int k = 0; k < 11; ++k) c.want(&w[k], sumW);
     c.want(&lvl_off, 10 * (int64_t)F_);
-    c.want(&tmp, (int64_t)std::max(tmpA, tmpB));
+    c.want(&tmp, (int64_t)tmpBytes);
     if ((e = c.take(owned_, s_)) != hipSuccess) return e;
     B_.skey = skey;
     B_.sslot = sslot;
-    B_.pnode = pp;
-    B_.pidx = (uint32_t*)(pp + sumP);
     int64_t wo = 0;
     for (int f = 0; f < F_; ++f) {
       Frame& d = H_[f];
@@ -2484,7 +2654,6 @@ class SubBuild {
       td.row = t.row + nb;
       d.seg_b = seg_b + nb;
       d.seg_e = seg_e + nb;
-      d.snode = snode + pb;
       d.spair = spair + pb;
       d.T.term = term + pb;
       d.T.lval = lval + pb;
@@ -2511,13 +2680,24 @@ class SubBuild {
     fill(seg_e, sumNodes, 0);
     k_root_init<<<F_, 64, 0, s_>>>(d_fr_);
     if (grid_[kMapM256]) k_node_assign<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256), B_);
-    k_pair_nodes<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP), B_);
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpB, B_.pnode, snode, B_.pidx, spair, (int)sumP,
-                                           0, bitsB, s_);
+    // Pairs into node order through their intervals: the intervals' first pairs
+    // in pair order, a stable sort of the intervals by node, each sorted
+    // interval's first position, then every position from its tile's intervals.
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmpSel, wgc_, wgo, (int)nwg_, s_);
     if (e != hipSuccess) return e;
-    k_segments<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
+    k_compact<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP), wgo, pair_slot_, istart, ikey,
+                                            ival);
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpB, ikey, sikey, ival, sival, (int)NI, 0, bitsB,
+                                           s_);
+    if (e != hipSuccess) return e;
+    const unsigned gi = (unsigned)blocks(NI, 256);
+    k_interval_len<<<gi, 256, 0, s_>>>(sival, istart, NI, sumP, slen);
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmpSum, slen, spos, (int)NI, s_);
+    if (e != hipSuccess) return e;
+    k_interval_ends<<<gi, 256, 0, s_>>>(d_fr_, F_, sikey, spos, slen, NI, tile_first);
+    k_expand<<<(unsigned)ntiles, 256, 0, s_>>>(d_fr_, F_, sumP, NI, tile_first, sikey, sival,
+                                               istart, spos, spair);
     if (grid_[kMapM256]) k_links<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256));
-    k_pair_terms<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP));
     // At most sumP / kShortSeg segments are medium, sumP / kMidSeg long.
     const int mid_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 8192);
     const int long_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kMidSeg, 1), 512);
@@ -2601,6 +2781,9 @@ class SubBuild {
 
   int64_t sumP_ = 0;
   int32_t* slot_node_ = nullptr;
+  uint32_t* pair_slot_ = nullptr;  // batch pair array: slot of an interval's first pair
+  int32_t* wgc_ = nullptr;         // intervals per workgroup of k_node_insert
+  int64_t nwg_ = 0;
   std::vector<int64_t> bound_ = std::vector<int64_t>(F_, 0);  // node bound per image
 };
 
