@@ -159,7 +159,7 @@ struct Frame {
   int32_t* start;
   int32_t* count;
   int32_t* row;
-  double4* px;
+  float4* px;              // the run's pixel as loaded (associated where it is read)
   uint32_t* pack;
   // 2. loop-1 extent
   uint64_t* bkeys;
@@ -271,7 +271,7 @@ __global__ void k_run_data(const Frame* __restrict__ fr, Map m) {
   const float4 f = load_pixel(d.fmt, d.src, p);
   double a[4];
   assoc_pixel(d.assoc != 0, f.x, f.y, f.z, f.w, a);
-  d.px[r] = make_double4(a[0], a[1], a[2], a[3]);
+  d.px[r] = f;
   const uint32_t c0 = scale_char(clamp_pixel(a[0])), c1 = scale_char(clamp_pixel(a[1]));
   const uint32_t c2 = scale_char(clamp_pixel(a[2])), c3 = scale_char(clamp_pixel(a[3]));
   d.pack[r] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
@@ -609,7 +609,10 @@ __device__ __forceinline__ void expand_one(const Frame& d, int64_t li, uint32_t 
   pair_decode(p, R1, d.R, depth, &r, &l);
   double mid[4];
   node_mid(d.t.key[node - d.node_base], mid);
-  const double4 v = d.px[r];
+  const float4 f = d.px[r];
+  double a[4];
+  assoc_pixel(assoc != 0, f.x, f.y, f.z, f.w, a);
+  const double4 v = make_double4(a[0], a[1], a[2], a[3]);
   const int32_t cnt = d.count[r];
   const double e0 = kScale * (v.x - mid[0]);
   const double e1 = kScale * (v.y - mid[1]);
@@ -2406,7 +2409,7 @@ class SubBuild {
       sumB += d.bcap;
     }
     int32_t *start = nullptr, *count = nullptr, *row = nullptr;
-    double4* px = nullptr;
+    float4* px = nullptr;
     uint32_t *pack = nullptr, *bfirst = nullptr;
     uint64_t* bkeys = nullptr;
     Carve c;
