@@ -628,9 +628,12 @@ __device__ __forceinline__ void expand_one(const Frame& d, int64_t li, uint32_t 
                              clamp_pixel(assoc ? v.w : kRange));
 }
 
-// Tile t: its intervals (at most kExpandTile + 1 touch it) are staged in LDS;
-// sorted position i takes the node of the last interval starting at or before
-// it, and the pair that interval's first pair plus the offset into it.
+// Tile t: its intervals (at most kExpandTile + 1 touch it) are staged in LDS,
+// and a bitmap marks the tile offsets where one of them begins (after the
+// first, which covers the tile's first position).  Sorted position i takes the
+// interval whose start is the last at or before it -- the count of marks up
+// to its offset -- that interval's node, and the pair its first pair plus the
+// offset into it.
 __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, int F, int64_t sum_p,
                                                 int64_t ni, const int32_t* __restrict__ tile_first,
                                                 const int32_t* __restrict__ sikey,
@@ -638,27 +641,33 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
                                                 const uint32_t* __restrict__ istart,
                                                 const int32_t* __restrict__ spos,
                                                 uint32_t* __restrict__ spair) {
+  constexpr int kWords = kExpandTile / 32;
   __shared__ int32_t l_pos[kExpandTile + 1];
   __shared__ int32_t l_node[kExpandTile + 1];
   __shared__ uint32_t l_first[kExpandTile + 1];
+  __shared__ uint32_t s_mark[kWords];
+  __shared__ int32_t s_before[kWords];  // marks in the words before
   __shared__ int s_f;
   __shared__ int64_t s_next;
   const int64_t i0 = (int64_t)blockIdx.x * kExpandTile;
   const int64_t i1 = i0 + kExpandTile < sum_p ? i0 + kExpandTile : sum_p;
   const int64_t j0 = tile_first[blockIdx.x];
+  if (threadIdx.x < kWords) s_mark[threadIdx.x] = 0;
+  __syncthreads();
   for (int k = threadIdx.x; k <= kExpandTile; k += blockDim.x) {
     const int64_t j = j0 + k;
-    int32_t pos = INT32_MAX;
     if (j < ni) {
-      pos = spos[j];
+      const int32_t pos = spos[j];
       if (pos < i1) {
+        l_pos[k] = pos;
         l_node[k] = sikey[j];
         l_first[k] = istart[sival[j]];
-      } else {
-        pos = INT32_MAX;
+        if (k > 0) {
+          const int o = (int)(pos - i0);  // in (0, kExpandTile)
+          atomicOr(&s_mark[o >> 5], 1u << (o & 31));
+        }
       }
     }
-    l_pos[k] = pos;
   }
   if (threadIdx.x == 0) {
     const int f = image_of_pair(fr, F, i0);
@@ -666,24 +675,30 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
     s_next = f + 1 < F ? fr[f + 1].pair_base : INT64_MAX;
   }
   __syncthreads();
+  if (threadIdx.x < 64) {  // one wavefront: exclusive prefix of the word counts
+    const int w = threadIdx.x;
+    int c = w < kWords ? __popc(s_mark[w]) : 0;
+    int incl = c;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(incl, d, 64);
+      if (w >= d) incl += v;
+    }
+    if (w < kWords) s_before[w] = incl - c;
+  }
+  __syncthreads();
   const int f0 = __builtin_amdgcn_readfirstlane(s_f);
   const bool one_image = i1 <= s_next;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    int lo = 0, hi = kExpandTile;  // l_pos[0] <= i0
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (l_pos[mid] <= i)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const int64_t g = (int64_t)l_first[lo] + (i - l_pos[lo]);
+    const int o = (int)(i - i0);
+    const uint32_t below = (o & 31) == 31 ? 0xFFFFFFFFu : ((2u << (o & 31)) - 1u);
+    const int k = s_before[o >> 5] + __popc(s_mark[o >> 5] & below);
+    const int64_t g = (int64_t)l_first[k] + (i - l_pos[k]);
     if (one_image) {
       const Frame& d = fr[f0];
-      expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[lo], spair);
+      expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[k], spair);
     } else {
       const Frame& d = fr[image_of_pair(fr, F, i)];
-      expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[lo], spair);
+      expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[k], spair);
     }
   }
 }
