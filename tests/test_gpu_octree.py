@@ -198,3 +198,57 @@ def test_octree_batch_item_failure_is_its_own(gpu, oracle):
     for k, (img, c) in enumerate(cases):
         pal = pals[k if k < 2 else k + 1]
         _check_built(oracle, pal, synth.widen_q16(img), c, img.shape[2] == 4, False)
+
+
+def _random_mix(seed, n):
+    """Uneven images of every source class: tiny ones (many share one tile of
+    the interval expansion), flat and grey ones (loop 1 covers the image or
+    little of it), alpha, 16-bit and float sources, palettes 2..256."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        kind = k % 6
+        if kind == 0:
+            w, h = int(rng.integers(1, 6)), int(rng.integers(1, 6))
+        else:
+            w, h = int(rng.integers(1, 180)), int(rng.integers(1, 140))
+        ch = 4 if rng.random() < 0.4 else 3
+        img = synth.sinusoid_image(w, h, rng, channels=ch)
+        if kind == 1:   # few distinct colours: loop 1 covers the image
+            lut = rng.integers(0, 256, (3, ch)).astype(np.uint8)
+            img = lut[rng.integers(0, 3, (h, w))]
+        elif kind == 2:  # grey
+            g = img[..., :1]
+            img = np.concatenate([g, g, g] + ([img[..., 3:]] if ch == 4 else []), axis=-1)
+        elif kind == 3:  # 16-bit
+            img = (img.astype(np.uint16) * 257 + rng.integers(0, 257, img.shape).astype(np.uint16))
+        elif kind == 4 and ch == 3:  # float RGBA, opaque
+            f = synth.widen_q16(img) + rng.uniform(-200, 200, (h, w, 4)).astype(np.float32)
+            f[..., 3] = 65535.0
+            img = np.clip(f, 0, 65535).astype(np.float32)
+        colours = int(np.exp(rng.uniform(np.log(2), np.log(256))))
+        out.append((img, colours))
+    return out
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_octree_batch_random_mix(gpu, oracle, seed):
+    """One batched build of 40 random images against the oracle, image by
+    image: every palette and tree as ColorTree::Build gives it alone."""
+    cases = _random_mix(seed, 40)
+    items, meta = [], []
+    for img, colours in cases:
+        h, w = img.shape[:2]
+        src = H.to_device(img)
+        _KEEP.append(src)
+        assoc = img.shape[2] == 4 and img.dtype != np.float32
+        gray = synth.is_gray(img)
+        f = img if img.dtype == np.float32 else synth.widen_q16(img)
+        items.append((src, w, h, H.fmt_of(img), colours, assoc, gray, True))
+        meta.append((f, colours, assoc, gray, w, h))
+    say("octree random mix", seed, len(items))
+    pals = gpu.Palette.build_batch(items)
+    _KEEP.extend(pals)
+    for pal, (f, colours, assoc, gray, w, h) in zip(pals, meta):
+        say("  item", w, h, colours, assoc, gray)
+        _check_built(oracle, pal, f, colours, assoc, gray)
