@@ -153,6 +153,14 @@ __global__ __launch_bounds__(kThreads) void k_init(const BlockFrame* __restrict_
   q[(int64_t)blockIdx.y * nb + i] = (uint8_t)lookup<ASSOC>(s, t, use_tree);
 }
 
+// The filter's slots in LDS (taps + 1 rounded up to 8, zero padded) and its
+// offset after n values of the residual planes (16-byte aligned).
+__host__ __device__ inline int g_slots(int taps) { return (taps + 1 + 7) & ~7; }
+template <typename T>
+__host__ __device__ inline int gl_offset(int n) {
+  return (int)(((size_t)n * sizeof(T) + 15) / 16 * 16 / sizeof(T));
+}
+
 // One sweep (JacobiStepKernel, src/rd_approx_cuda.cu:158-228).
 template <typename T, bool ASSOC, int kPer>
 __global__ __launch_bounds__(kThreads) void k_sweep(const BlockFrame* __restrict__ frames,
@@ -170,6 +178,11 @@ __global__ __launch_bounds__(kThreads) void k_sweep(const BlockFrame* __restrict
   constexpr int kTile = kThreads * kPer;
   const int window = kTile + taps;
   T* r = (T*)(lds + img_bytes);  // C planes of `window` values
+  // The filter beside the planes, padded with zeros to a whole unrolled step:
+  // read from LDS, the taps keep the loop's waits in order (a scalar load per
+  // tap shares their counter and would drain every LDS read in flight).
+  T* gl = r + gl_offset<T>(C * window);
+  for (int k = threadIdx.x; k < g_slots(taps); k += kThreads) gl[k] = k <= taps ? g[k] : T(0);
   const int64_t fbase = (int64_t)blockIdx.y * nb;
   const int64_t tile = (int64_t)blockIdx.x * kTile;
   const int64_t base = tile - taps;
@@ -207,23 +220,23 @@ __global__ __launch_bounds__(kThreads) void k_sweep(const BlockFrame* __restrict
     for (; d + kPer <= taps + 1; d += kPer) {
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
-        const T gk = g[d + u];
+        const T gk = gl[d + u];
 #pragma unroll
         for (int j = 0; j < kPer; ++j)
 #pragma unroll
           for (int c = 0; c < C; ++c) acc[j][c] += gk * wv[c][(j - u + kPer) % kPer];
         // Slot (kPer-1-u) mod kPer now holds the value no output needs again;
         // refill it with the next step's new entry, window slot
-        // o0 + taps - (d+u) - 1 (>= o0 because another step follows).
-        if (d + u < taps) {
+        // o0 + taps - (d+u) - 1.  After the last step (d+u == taps) the slot is
+        // never read; its load (slot o0 - 1, at worst the word before the
+        // planes, inside the staged search image) is left unconditional.
 #pragma unroll
-          for (int c = 0; c < C; ++c)
-            wv[c][(kPer - 1 - u) % kPer] = r[c * window + o0 + taps - (d + u) - 1];
-        }
+        for (int c = 0; c < C; ++c)
+          wv[c][(kPer - 1 - u) % kPer] = r[c * window + o0 + taps - (d + u) - 1];
       }
     }
     for (; d <= taps; ++d) {
-      const T gk = g[d];
+      const T gk = gl[d];
 #pragma unroll
       for (int j = 0; j < kPer; ++j)
 #pragma unroll
@@ -236,7 +249,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(const BlockFrame* __restrict
       const int64_t i = i0 + j;
       const int dmax = (int)(i < taps ? i : taps);
       for (int d = 0; d <= dmax; ++d) {
-        const T gk = g[d];
+        const T gk = gl[d];
 #pragma unroll
         for (int c = 0; c < C; ++c) acc[j][c] += gk * r[c * window + o0 + taps + j - d];
       }
@@ -340,7 +353,9 @@ hipError_t run(const BlockFrame* frames, int nframes, const int32_t* curve, int6
   k_init<T, ASSOC><<<grid_pos, kThreads, (size_t)img_bytes, s>>>(frames, cx, nb, use_tree, q0);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const size_t lds = (size_t)img_bytes + sizeof(T) * (ASSOC ? 4 : 3) * (size_t)(kTile + taps);
+  const size_t lds = (size_t)img_bytes +
+                    sizeof(T) * ((size_t)gl_offset<T>((ASSOC ? 4 : 3) * (kTile + taps)) +
+                                 (size_t)g_slots(taps));
   e = allow_lds(k_sweep<T, ASSOC, kPer>, lds);
   if (e != hipSuccess) return e;
   const dim3 grid_tile((unsigned)((nb + kTile - 1) / kTile), (unsigned)nframes);
@@ -362,7 +377,9 @@ hipError_t run(const BlockFrame* frames, int nframes, const int32_t* curve, int6
 
 size_t approx_lds_bytes(int img_bytes, int taps, int fp64, int assoc, int per) {
   const int tile = kThreads * (per == 2 || per == 8 ? per : kPerDefault);
-  return (size_t)img_bytes + (fp64 ? 8 : 4) * (size_t)(assoc ? 4 : 3) * (size_t)(tile + taps);
+  const int n = (assoc ? 4 : 3) * (tile + taps);
+  const size_t planes = fp64 ? (size_t)gl_offset<double>(n) * 8 : (size_t)gl_offset<float>(n) * 4;
+  return (size_t)img_bytes + planes + (fp64 ? 8 : 4) * (size_t)g_slots(taps);
 }
 
 template <int kPer>
