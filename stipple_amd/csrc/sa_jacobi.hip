@@ -34,42 +34,51 @@ constexpr int kPerDefault = 4;
 struct JSearch {
   const double* col;
   const float4* ent;
+  const float4* cent;  // candidates' entries in post-order (cent[k] = ent[cand[k]])
   const int2* sub;
+  const uint32_t* desc;
   const int16_t* child;
   const uint16_t* cand;
   int kids;
   int count;
 };
 
-// d_select_index without a memo (include/rd_cuda_common.cuh:408-446).
+// d_select_index without a memo (include/rd_cuda_common.cuh:408-446): the
+// descent (the first three levels one table read), then the ordered scan of
+// the candidates in post-order.
 template <bool ASSOC>
 __device__ __forceinline__ int tree_lookup(const JSearch& s, const double t[4]) {
   const unsigned c0 = scale_char(clamp_pixel(t[0]));
   const unsigned c1 = scale_char(clamp_pixel(t[1]));
   const unsigned c2 = scale_char(clamp_pixel(t[2]));
   const unsigned c3 = ASSOC ? scale_char(clamp_pixel(t[3])) : 0u;
-  int row = 0, prev = 0;
-  for (int lev = kMaxDepth - 1; lev > 0; --lev) {
-    const int ch = s.child[row * s.kids + node_id(ASSOC, c0, c1, c2, c3, lev)];
-    if (ch == kNoChild) break;
-    prev = row;
-    if (ch == kLeaf) break;
-    row = ch;
+  uint32_t de = (c0 >> 5) | ((c1 >> 5) << 3) | ((c2 >> 5) << 6);
+  if (ASSOC) de |= (c3 >> 5) << 9;
+  const uint32_t d3 = s.desc[de];
+  int prev = (int)((d3 >> 16) & 0x7FFFu);
+  if (!(d3 & kDescStop)) {
+    int row = (int)(d3 & 0x7FFFu);
+    for (int lev = 4; lev > 0; --lev) {
+      const int ch = s.child[row * s.kids + node_id(ASSOC, c0, c1, c2, c3, lev)];
+      if (ch == kNoChild) break;
+      prev = row;
+      if (ch == kLeaf) break;
+      row = ch;
+    }
   }
   const int2 r = s.sub[prev];
   double best = kFarDistance;
-  int bidx = 0;
+  int bk = 0;
   for (int k = 0; k < r.y; ++k) {
-    const int ci = s.cand[r.x + k];
-    const float4 e4 = s.ent[ci];
+    const float4 e4 = s.cent[r.x + k];
     const float e[4] = {e4.x, e4.y, e4.z, e4.w};
     const double d = candidate_distance(ASSOC, e, t);
     if (d <= best) {
       best = d;
-      bidx = ci;
+      bk = k;
     }
   }
-  return bidx;
+  return (int)s.cand[r.x + bk];
 }
 
 // d_nearest_linear (include/rd_cuda_common.cuh:451-476): ascending scan over
@@ -110,7 +119,9 @@ __device__ __forceinline__ JSearch stage(const PaletteDev& pd, unsigned char* ld
   JSearch s;
   s.col = (const double*)lds;
   s.ent = (const float4*)(lds + pd.off_ent);
+  s.cent = (const float4*)(lds + pd.off_cent);
   s.sub = (const int2*)(lds + pd.off_sub);
+  s.desc = (const uint32_t*)(lds + pd.off_desc);
   s.child = (const int16_t*)(lds + pd.off_child);
   s.cand = (const uint16_t*)(lds + pd.off_cand);
   s.kids = pd.kids;
