@@ -326,7 +326,8 @@ def run_approx4k(args, sa, torch, synth, dev):
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": _traffic("approx_c4_traffic"),
             "kernel": "approx (gather, 8 sweeps, scatter)",
             "note": "algorithmic 12 B/px per frame + 18 B/px per sweep; the sweep is FP32-VALU "
-                    "bound at taps=96 (about 873 FP32 ops per px per sweep)"}
+                    "bound at taps=96 (97 taps x 3 channels x multiply and add = 582 FP32 ops "
+                    "per px per sweep)"}
     return _line(args, "Mpixel/s and frames/s, 3840x2160 RGB16, Jacobi 8 sweeps",
                  px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f32",
                  {"workload": f"{nf} 3840x2160 RGB16 radial-gradient frames with 5% salt and "
