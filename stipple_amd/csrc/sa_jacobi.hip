@@ -22,6 +22,7 @@
 // beside the tile for the per-output octree lookup.
 #include "sa_internal.h"
 #include "sa_math.h"
+#include "sa_search.h"
 
 namespace sa {
 namespace {
@@ -31,60 +32,10 @@ constexpr int kThreads = 256;
 // 8; 4 unless tuned.
 constexpr int kPerDefault = 4;
 
-struct JSearch {
-  const double* col;
-  const float4* ent;
-  const float4* cent;  // candidates' entries in post-order (cent[k] = ent[cand[k]])
-  const int2* sub;
-  const uint32_t* desc;
-  const int16_t* child;
-  const uint16_t* cand;
-  int kids;
-  int count;
-};
-
-// d_select_index without a memo (include/rd_cuda_common.cuh:408-446): the
-// descent (the first three levels one table read), then the ordered scan of
-// the candidates in post-order.
-template <bool ASSOC>
-__device__ __forceinline__ int tree_lookup(const JSearch& s, const double t[4]) {
-  const unsigned c0 = scale_char(clamp_pixel(t[0]));
-  const unsigned c1 = scale_char(clamp_pixel(t[1]));
-  const unsigned c2 = scale_char(clamp_pixel(t[2]));
-  const unsigned c3 = ASSOC ? scale_char(clamp_pixel(t[3])) : 0u;
-  uint32_t de = (c0 >> 5) | ((c1 >> 5) << 3) | ((c2 >> 5) << 6);
-  if (ASSOC) de |= (c3 >> 5) << 9;
-  const uint32_t d3 = s.desc[de];
-  int prev = (int)((d3 >> 16) & 0x7FFFu);
-  if (!(d3 & kDescStop)) {
-    int row = (int)(d3 & 0x7FFFu);
-    for (int lev = 4; lev > 0; --lev) {
-      const int ch = s.child[row * s.kids + node_id(ASSOC, c0, c1, c2, c3, lev)];
-      if (ch == kNoChild) break;
-      prev = row;
-      if (ch == kLeaf) break;
-      row = ch;
-    }
-  }
-  const int2 r = s.sub[prev];
-  double best = kFarDistance;
-  int bk = 0;
-  for (int k = 0; k < r.y; ++k) {
-    const float4 e4 = s.cent[r.x + k];
-    const float e[4] = {e4.x, e4.y, e4.z, e4.w};
-    const double d = candidate_distance(ASSOC, e, t);
-    if (d <= best) {
-      best = d;
-      bk = k;
-    }
-  }
-  return (int)s.cand[r.x + bk];
-}
-
 // d_nearest_linear (include/rd_cuda_common.cuh:451-476): ascending scan over
 // the whole palette, d = dr^2 + (dg^2 + db^2), beta from the target's alpha.
 template <bool ASSOC>
-__device__ __forceinline__ int linear_lookup(const JSearch& s, const double t[4]) {
+__device__ __forceinline__ int linear_lookup(const LSearch& s, const double t[4]) {
   double dist = kFarDistance;
   int best = 0;
   const double beta = kScale * t[3];
@@ -107,23 +58,42 @@ __device__ __forceinline__ int linear_lookup(const JSearch& s, const double t[4]
   return best;
 }
 
+// The tree search (sa_search.h) on the clamped target, or the linear scan.
 template <bool ASSOC>
-__device__ __forceinline__ int lookup(const JSearch& s, const double t[4], int use_tree) {
-  return use_tree ? tree_lookup<ASSOC>(s, t) : linear_lookup<ASSOC>(s, t);
+__device__ __forceinline__ int lookup(const LSearch& s, const double t[4], int use_tree) {
+  if (!use_tree) return linear_lookup<ASSOC>(s, t);
+  const float pc[4] = {clamp_pixel(t[0]), clamp_pixel(t[1]), clamp_pixel(t[2]),
+                       ASSOC ? clamp_pixel(t[3]) : 0.0f};
+  return lane_lookup<ASSOC>(s, pc, t);
 }
 
-__device__ __forceinline__ JSearch stage(const PaletteDev& pd, unsigned char* lds) {
-  const uint4* src = (const uint4*)pd.image;
+// The search image without its first array (the associated colours, which
+// the engine never reads) at lds[0, bytes - off_ent), and the candidates'
+// distance constants in the room that array leaves: 32 bytes per colour,
+// like it, so the workgroup's LDS is the image's size.
+template <bool ASSOC>
+__device__ __forceinline__ LSearch stage(const PaletteDev& pd, unsigned char* lds) {
+  const int skip = pd.off_ent;
+  const uint4* src = (const uint4*)((const unsigned char*)pd.image + skip);
   uint4* dst = (uint4*)lds;
-  for (int k = threadIdx.x; k < pd.bytes / 16; k += blockDim.x) dst[k] = src[k];
-  JSearch s;
-  s.col = (const double*)lds;
-  s.ent = (const float4*)(lds + pd.off_ent);
-  s.cent = (const float4*)(lds + pd.off_cent);
-  s.sub = (const int2*)(lds + pd.off_sub);
-  s.desc = (const uint32_t*)(lds + pd.off_desc);
-  s.child = (const int16_t*)(lds + pd.off_child);
-  s.cand = (const uint16_t*)(lds + pd.off_cand);
+  const int words = (pd.bytes - skip) / 16;
+  for (int k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
+  double4* dcent = (double4*)(lds + (pd.bytes - skip));
+  const float4* cent = (const float4*)((const unsigned char*)pd.image + pd.off_cent);
+  for (int k = threadIdx.x; k < pd.count; k += blockDim.x) {
+    const float4 e = cent[k];
+    double a = 1.0;
+    if (ASSOC) a = kScale * (double)e.w;
+    dcent[k] = make_double4(a * (double)e.x, a * (double)e.y, a * (double)e.z, (double)e.w);
+  }
+  LSearch s;
+  s.col = nullptr;
+  s.ent = (const float4*)(lds + (pd.off_ent - skip));
+  s.sub = (const int2*)(lds + (pd.off_sub - skip));
+  s.desc = (const uint32_t*)(lds + (pd.off_desc - skip));
+  s.child = (const int16_t*)(lds + (pd.off_child - skip));
+  s.cand = (const uint16_t*)(lds + (pd.off_cand - skip));
+  s.dcent = dcent;
   s.kids = pd.kids;
   s.count = pd.count;
   return s;
@@ -154,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_init(const BlockFrame* __restrict_
   extern __shared__ __align__(16) unsigned char lds[];
   const BlockFrame f = frames[blockIdx.y];
   const PaletteDev pd = *f.pal;
-  const JSearch s = stage(pd, lds);
+  const LSearch s = stage<ASSOC>(pd, lds);
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= nb) return;
@@ -184,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(const BlockFrame* __restrict
   constexpr int C = ASSOC ? 4 : 3;
   const BlockFrame f = frames[blockIdx.y];
   const PaletteDev pd = *f.pal;
-  const JSearch s = stage(pd, lds);
+  const LSearch s = stage<ASSOC>(pd, lds);
   __syncthreads();  // the tile below reads the staged entries
   constexpr int kTile = kThreads * kPer;
   const int window = kTile + taps;

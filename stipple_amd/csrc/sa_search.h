@@ -1,4 +1,4 @@
-// sa_search.h -- the per-lane palette search of the block engine.
+// sa_search.h -- the per-lane palette search of the block and Jacobi engines.
 //
 // d_select_index without a memo (include/rd_cuda_common.cuh:408-446): the
 // descent of src/rd_riemersma_cpu.cpp:82-89 and the ordered post-order scan of
@@ -11,9 +11,9 @@
 //   dcent[k] = (alpha e0, alpha e1, alpha e2, e3) of the k-th candidate in
 //   post-order, alpha = QuantumScale e3 when associated, 1 otherwise --
 // the products candidate_distance forms from the entry, formed the same way,
-// so the scan reads 32 bytes per candidate and converts nothing.  (The Jacobi
-// sweep keeps its own search: the 8 KB of constants for 256 colours cost it a
-// workgroup per CU, and its time is the filter's.)
+// so the scan reads 32 bytes per candidate and converts nothing.  The block
+// walk stages the whole image and the constants after it; the Jacobi sweep,
+// which never reads the associated colours, puts the constants in their room.
 #pragma once
 
 #include "sa_math.h"
