@@ -351,7 +351,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the one-image (one-frame) per launch leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
-    ap.add_argument("--build-threads", type=int, default=8,
+    ap.add_argument("--build-threads", type=int, default=2,
                     help="host threads (one stream each) classifying palettes concurrently")
     ap.add_argument("--chain", choices=["pipeline", "batch"], default="pipeline",
                     help="pipeline: images join a resumable walk as their palettes are "
@@ -561,12 +561,16 @@ def main():
     pack_ms /= max(walk_launches, 1)
     log(f"walk only: {t_walk * 1e3:.1f} ms  kernel {walk_ms:.1f} ms  gather {gather_ms:.2f} pack {pack_ms:.2f}")
 
-    # Palette build alone.
+    # Palette build alone: the 64 palettes as one batched device build, and
+    # one image at a time.
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    build_palettes()
+    built = sa.Palette.build_batch([(srcs[k], batch[k][0].shape[1], batch[k][0].shape[0],
+                                     sa.PIX_RGBA8, batch[k][1], True, grays[k]) for k in order])
     torch.cuda.synchronize()
     t_pal = time.perf_counter() - t0
+    for p in built:
+        p.close()
     t0 = time.perf_counter()
     for k in range(len(batch)):
         img, col = batch[k]
