@@ -96,7 +96,7 @@ int ensure_init(char* err, size_t errlen) {
   }
   snprintf(g_name, sizeof(g_name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName, g_cus);
   // Every build and walk allocates its scratch stream-ordered; keep the freed
-  // memory in the pool instead of returning it to the driver at each
+  // memory in the pool instead of returning it to the HIP runtime at each
   // synchronisation (the default threshold, 0, makes every palette build map
   // its ~100 MB of pair arrays afresh).
   hipMemPool_t pool = nullptr;

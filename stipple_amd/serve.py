@@ -161,7 +161,7 @@ class SeqPipeline:
             self.builder.submit(st.src, st.width, st.height, st.src_fmt, st.colors, st.assoc,
                                 st.gray, built)
 
-    # -- the driver (one thread) -------------------------------------------------
+    # -- the stepping loop (one thread) ------------------------------------------
     def _admit(self, block):
         got = []
         try:
