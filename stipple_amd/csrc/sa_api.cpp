@@ -741,6 +741,8 @@ struct JobItem {
   sa::SeqItem it{};
   int64_t t = 0;               // progress (every step advances it deterministically)
   unsigned char* mem = nullptr;  // xs, idx, memo, state, snapshot
+  size_t mem_bytes = 0;
+  hipEvent_t mem_ev = nullptr;   // recorded after the block's last use when it is parked
   int assoc = 0;
   int img_bytes = 0;
 };
@@ -762,6 +764,18 @@ struct sa_walk_job_s {
   std::vector<JobItem> items;  // unfinished
   int64_t next_id = 0;
   std::mutex mu;
+  // Scratch blocks of retired images, kept for images added later: a
+  // stream-ordered free costs milliseconds per large block on this runtime,
+  // so a serving loop that retires hundreds of images would stall on them.
+  // Each block carries an event recorded after its last use; a reuse waits on
+  // it.  Oldest first; above kSpareBytes the oldest go back to the runtime.
+  struct Spare {
+    unsigned char* mem;
+    size_t bytes;
+    hipEvent_t ev;
+  };
+  std::vector<Spare> spare;
+  size_t spare_bytes = 0;
   // Overlapped steps: the progress of the step in flight, copied to pinned
   // memory, and the ids it covered (two slots, alternating).
   struct Flight {
@@ -773,6 +787,53 @@ struct sa_walk_job_s {
   } flight[2];
   int cur = 0;
 };
+
+namespace {
+
+constexpr size_t kSpareBytes = (size_t)16 << 30;
+
+// Parks an image's scratch block once the work queued on `s` so far is done
+// with it; the oldest blocks above kSpareBytes are freed.
+int park_block(sa_walk_job job, JobItem& J, hipStream_t s, char* err, size_t errlen) {
+  if (!J.mem) return SA_OK;
+  if (!J.mem_ev) HIP_TRY(hipEventCreateWithFlags(&J.mem_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(J.mem_ev, s));
+  job->spare.push_back({J.mem, J.mem_bytes, J.mem_ev});
+  job->spare_bytes += J.mem_bytes;
+  J.mem = nullptr;
+  J.mem_ev = nullptr;
+  while (job->spare_bytes > kSpareBytes) {
+    const sa_walk_job_s::Spare o = job->spare.front();
+    job->spare.erase(job->spare.begin());
+    job->spare_bytes -= o.bytes;
+    HIP_TRY(hipStreamWaitEvent(s, o.ev, 0));
+    HIP_TRY(hipFreeAsync(o.mem, s));
+    HIP_TRY(hipEventDestroy(o.ev));
+  }
+  return SA_OK;
+}
+
+// Gives J the smallest parked block of `bytes` to twice that, with `s`
+// ordered after its last use; J.mem stays null when none fits.
+int reuse_block(sa_walk_job job, JobItem& J, size_t bytes, hipStream_t s, char* err,
+                size_t errlen) {
+  int best = -1;
+  for (int k = 0; k < (int)job->spare.size(); ++k) {
+    const size_t b = job->spare[k].bytes;
+    if (b >= bytes && b / 2 <= bytes && (best < 0 || b < job->spare[best].bytes)) best = k;
+  }
+  if (best < 0) return SA_OK;
+  const sa_walk_job_s::Spare o = job->spare[best];
+  job->spare.erase(job->spare.begin() + best);
+  job->spare_bytes -= o.bytes;
+  J.mem = o.mem;
+  J.mem_bytes = o.bytes;
+  J.mem_ev = o.ev;
+  HIP_TRY(hipStreamWaitEvent(s, o.ev, 0));
+  return SA_OK;
+}
+
+}  // namespace
 
 extern "C" int sa_walk_job_create(const sa_seq_options* opt, sa_walk_job* out, char* err,
                                   size_t errlen) {
@@ -838,12 +899,16 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     int rc = SA_OK;
     const int32_t* curve = nullptr;
     rc = get_curve(im.width, im.height, s, &curve, err, errlen);
-    if (rc == SA_OK) {
-      hipError_t e = hipMallocAsync((void**)&J.mem, xs_b + idx_b + memo_b + st_b + snap_b, s);
+    const size_t need = xs_b + idx_b + memo_b + st_b + snap_b;
+    if (rc == SA_OK) rc = reuse_block(job, J, need, s, err, errlen);
+    if (rc == SA_OK && !J.mem) {
+      hipError_t e = hipMallocAsync((void**)&J.mem, need, s);
       if (e != hipSuccess) rc = fail(err, errlen, SA_ENOMEM, "image %d: %s", i, hipGetErrorString(e));
+      J.mem_bytes = need;
     }
     if (rc != SA_OK) {
-      for (JobItem& f : fresh) (void)hipFreeAsync(f.mem, s);
+      for (JobItem& f : fresh) (void)park_block(job, f, s, nullptr, 0);
+      (void)park_block(job, J, s, nullptr, 0);
       return rc;
     }
     unsigned char* p = J.mem;
@@ -875,7 +940,10 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     if (dev[assoc].empty()) continue;
     unsigned char* d = nullptr;
     int rc = upload_items(dev[assoc], s, &d, err, errlen);
-    if (rc) return rc;
+    if (rc) {
+      for (JobItem& f : fresh) (void)park_block(job, f, s, nullptr, 0);
+      return rc;
+    }
     const int ni = (int)dev[assoc].size();
     const sa::SeqItem* di = (const sa::SeqItem*)d;
     HIP_TRY(timed("seq_gather", s, [&] { return sa::launch_seq_gather(di, ni, max_n, s); }));
@@ -982,7 +1050,8 @@ int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int
     if (J.t == J.it.n) {
       if (done && nd < cap) done[nd] = J.id;
       ++nd;
-      HIP_TRY(hipFreeAsync(J.mem, s));
+      int rc = park_block(job, J, s, err, errlen);
+      if (rc) return rc;
     } else {
       keep.push_back(J);
     }
@@ -1043,7 +1112,8 @@ int retire_finished(sa_walk_job job, hipStream_t s, int64_t* done, int cap, int*
     if (J.t == J.it.n) {
       if (done && nd < cap) done[nd] = J.id;
       ++nd;
-      HIP_TRY(hipFreeAsync(J.mem, s));
+      int rc = park_block(job, J, s, err, errlen);
+      if (rc) return rc;
     } else {
       keep.push_back(J);
     }
@@ -1151,7 +1221,14 @@ extern "C" int sa_walk_job_destroy(sa_walk_job job) {
   {
     std::lock_guard<std::mutex> g(job->mu);
     (void)hipDeviceSynchronize();
-    for (JobItem& J : job->items) (void)hipFree(J.mem);
+    for (JobItem& J : job->items) {
+      (void)hipFree(J.mem);
+      if (J.mem_ev) (void)hipEventDestroy(J.mem_ev);
+    }
+    for (auto& o : job->spare) {
+      (void)hipFree(o.mem);
+      (void)hipEventDestroy(o.ev);
+    }
     for (auto& F : job->flight) {
       if (F.host) (void)hipHostFree(F.host);
       if (F.ev) (void)hipEventDestroy(F.ev);
