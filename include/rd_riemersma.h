@@ -125,7 +125,7 @@ struct BlockOptions {
   int block = 512;
   int frames = 1;
   bool emit_u16 = false;     // write packed rgba64le to out_u16 instead of batch
-  bool emit_yuv444 = false;  // not built on this engine (refused)
+  bool emit_yuv444 = false;  // with emit_u16: planar 8-bit 4:4:4 to out_u16 (3 bytes/px)
   bool upload_u16 = false;   // read interleaved uint16 from in_u16 instead of batch
   int in_channels = 4;       // 3 (rgb48le) or 4 (rgba64le) when upload_u16
   enum class InMode {
@@ -134,7 +134,9 @@ struct BlockOptions {
     PlanarYuv444Prepass = 2,
     PlanarYuv420 = 3,
   };
-  InMode in_mode = InMode::Interleaved16;  // only Interleaved16 is built
+  // Planar modes read 8-bit planes through in_u16: 4:4:4 (both 4:4:4 modes
+  // give the same values) or 4:2:0 with even width and height.
+  InMode in_mode = InMode::Interleaved16;
   bool in_u16_pinned = false;
   bool out_u16_pinned = false;
   bool batch_pinned = false;

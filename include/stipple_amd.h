@@ -34,7 +34,19 @@ enum {
   SA_EUNSUPPORTED = 5  /* valid input this build does not handle */
 };
 
-/* Pixel formats: interleaved, row-major, dense (width*height pixels). */
+/* Pixel formats: interleaved, row-major, dense (width*height pixels), except
+ * the planar YUV wire formats of the video path, which only the block engine
+ * (sa_walk_blocks, sa_chain_create_blocks) and the palette build
+ * (sa_palette_build, _batch) read, and only the block engine writes:
+ *   SA_PIX_YUV444P8: Y, Cb, Cr planes of width*height bytes each.  Input
+ *     widens through swscale's integer matrix (SwsYuvToRgb16,
+ *     include/rd_types.h:75-97; BlockOptions::InMode::PlanarYuv444 and
+ *     PlanarYuv444Prepass, include/rd_riemersma.h:325-443) with alpha 65535;
+ *     output is the limited-range BT.601 matrix of the chosen entry
+ *     (BlockOptions::emit_yuv444; src/rd_riemersma_cpu.cpp:718-730).
+ *   SA_PIX_YUV420P8: Y (width*height), then Cb and Cr at (width/2)*(height/2);
+ *     input only, even width and height; chroma is upsampled bilinearly
+ *     (InMode::PlanarYuv420, src/rd_blocks_cuda.cu:651-740). */
 enum {
   SA_PIX_RGBAF32 = 0, /* RgbaF: 4 x float Q16, the reference's in-memory layout */
   SA_PIX_RGB8 = 1,    /* 8-bit samples widen x257 on input */
@@ -42,7 +54,9 @@ enum {
   SA_PIX_RGB16 = 3,   /* 16-bit samples are Q16 as-is */
   SA_PIX_RGBA16 = 4,  /* rgba64le */
   SA_PIX_INDEX8 = 5,  /* output only: palette index per pixel */
-  SA_PIX_INDEX16 = 6
+  SA_PIX_INDEX16 = 6,
+  SA_PIX_YUV444P8 = 7,
+  SA_PIX_YUV420P8 = 8
 };
 
 typedef struct sa_palette_s* sa_palette;
@@ -78,7 +92,8 @@ int sa_palette_count(sa_palette p);
 /* ColorTree::Build on the device (classify, reduce, colormap).  Replaces
  * rd::ColorTree::Build (include/rd_octree.h:62-63, src/rd_octree.cpp:219-409);
  * bit-exact with it: same colormap, same node numbering, the same doubles in
- * every node.  `src` is a device image in src_fmt.  Synchronises `stream`
+ * every node.  `src` is a device image in src_fmt (planar YUV included: the
+ * tree of the frame as the reader widens it).  Synchronises `stream`
  * (the build reads back a few scalars).  With keep_tree the node arrays stay
  * on the device for sa_palette_tree.  Trees that would exceed MaxQNodes
  * during classification (quantize.c then prunes whole levels mid-image)

@@ -23,6 +23,14 @@
 //   * blocks         -- golden fixtures from oracle/_ref RiemersmaBlocksCpu;
 //                       block >= n equals the cache-free sequential walk on
 //                       the block curve.
+//   * planar YUV     -- 4:4:4 in (swscale's matrix) and out (BT.601), and
+//                       rgba64 out of YUV in: against oracle/_ref's fused
+//                       gather/scatter of RiemersmaBlocksCpu.  4:2:0 in (a
+//                       CUDA-only reader in the reference): even coordinates
+//                       copy the chroma sample, constant chroma stays
+//                       constant, and odd ones are the rounded bilinear mean,
+//                       checked in numpy; the result then feeds the pinned
+//                       4:4:4 path.
 //   * inverse filter -- triangular Toeplitz solve of (I-H) g = delta (numpy).
 //   * Jacobi         -- parity PARTIAL: the reference has no CPU Jacobi and
 //                       its CUDA engine cannot be built here, so no golden
@@ -855,6 +863,119 @@ extern "C" int orc_walk_blocks_u16(const OrcTree* t, const float* batch, int64_t
     for (int64_t pix = 0; pix < pixels; ++pix) {  // unvisited: source value
       if (owner[(size_t)pix] >= 0) continue;
       for (int c = 0; c < 4; ++c) u[4 * pix + c] = (uint16_t)src[4 * pix + c];
+    }
+  }
+  return 0;
+}
+
+// ===========================================================================
+// Video wire formats of the block engine: planar 8-bit YUV in and out.
+// ===========================================================================
+namespace {
+
+// SwsYuvToRgb16, include/rd_types.h:75-97, in the reference's statement order.
+uint16_t sws_clip16(int x) {
+  const int v = x + (1 << 15);
+  return (uint16_t)(v < 0 ? 0 : (v > 65535 ? 65535 : v));
+}
+void sws_yuv_to_rgb16(unsigned y, unsigned u, unsigned v, uint16_t* out) {
+  int yy = (int)((y - 128u) * 512u);
+  const int uu = (int)((u - 128u) * 512u);
+  const int vv = (int)((v - 128u) * 512u);
+  yy += 0x10000;
+  yy -= 8192;
+  yy *= 9539;
+  yy += (1 << 13) - (1 << 29);
+  out[0] = sws_clip16((vv * 13075 + yy) >> 14);
+  out[1] = sws_clip16((vv * -6660 + uu * -3209 + yy) >> 14);
+  out[2] = sws_clip16((uu * 16525 + yy) >> 14);
+}
+
+// d_chroma_bilinear, src/rd_blocks_cuda.cu:651-675.
+unsigned chroma_bilinear(const uint8_t* plane, int64_t stride, int64_t cx, int64_t cy, int ox,
+                         int oy, int64_t w2, int64_t h2) {
+  const int64_t x1 = cx + 1 < w2 ? cx + 1 : cx;
+  const int64_t y1 = cy + 1 < h2 ? cy + 1 : cy;
+  const int a = plane[cy * stride + cx], b = plane[cy * stride + x1];
+  const int c = plane[y1 * stride + cx], d = plane[y1 * stride + x1];
+  const int top = a * (2 - ox) + b * ox;
+  const int bot = c * (2 - ox) + d * ox;
+  return (unsigned)((top * (2 - oy) + bot * oy + 2) >> 2);
+}
+
+// The fused scatter's RGB -> YUV 4:4:4 (src/rd_riemersma_cpu.cpp:718-730).
+void rgb_to_yuv444(float fr, float fg, float fb, uint8_t* y, uint8_t* u, uint8_t* v) {
+  const int r8 = (int)fr >> 8, g8 = (int)fg >> 8, b8 = (int)fb >> 8;
+  const int yv = ((66 * r8 + 129 * g8 + 25 * b8 + 128) >> 8) + 16;
+  const int uv = ((-38 * r8 - 74 * g8 + 112 * b8 + 128) >> 8) + 128;
+  const int vv = ((112 * r8 - 94 * g8 - 18 * b8 + 128) >> 8) + 128;
+  *y = (uint8_t)(yv < 0 ? 0 : (yv > 255 ? 255 : yv));
+  *u = (uint8_t)(uv < 0 ? 0 : (uv > 255 ? 255 : uv));
+  *v = (uint8_t)(vv < 0 ? 0 : (vv > 255 ? 255 : vv));
+}
+
+}  // namespace
+
+// Planar frames (fmt 7: 4:4:4, 8: 4:2:0 with even w, h) widened to RgbaF, as
+// the reader and the gathers do (RawYuv444ToFloats / gather_pixel,
+// src/rd_riemersma_cpu.cpp:496-507; BlkGatherYuv420Kernel,
+// src/rd_blocks_cuda.cu:699-740): alpha 65535.
+extern "C" int orc_yuv_to_rgbaf(const uint8_t* planes, int64_t w, int64_t h, int frames, int fmt,
+                                float* out) {
+  if (fmt != 7 && fmt != 8) return -1;
+  if (fmt == 8 && ((w & 1) || (h & 1))) return -1;
+  const int64_t pp = w * h, w2 = w / 2, h2 = h / 2;
+  const int64_t fbytes = fmt == 7 ? 3 * pp : pp + 2 * w2 * h2;
+  for (int f = 0; f < frames; ++f) {
+    const uint8_t* s = planes + (size_t)f * (size_t)fbytes;
+    float* o = out + 4 * (size_t)f * (size_t)pp;
+    for (int64_t p = 0; p < pp; ++p) {
+      unsigned u, v;
+      if (fmt == 7) {
+        u = s[pp + p];
+        v = s[2 * pp + p];
+      } else {
+        const int64_t y = p / w, x = p % w;
+        u = chroma_bilinear(s + pp, w2, x >> 1, y >> 1, (int)(x & 1), (int)(y & 1), w2, h2);
+        v = chroma_bilinear(s + pp + w2 * h2, w2, x >> 1, y >> 1, (int)(x & 1), (int)(y & 1), w2,
+                            h2);
+      }
+      uint16_t rgb[3];
+      sws_yuv_to_rgb16(s[p], u, v, rgb);
+      o[4 * p + 0] = (float)rgb[0];
+      o[4 * p + 1] = (float)rgb[1];
+      o[4 * p + 2] = (float)rgb[2];
+      o[4 * p + 3] = (float)65535.0;
+    }
+  }
+  return 0;
+}
+
+// The block walk with planar 4:4:4 output (the CPU engine's fused scatter,
+// src/rd_riemersma_cpu.cpp:711-730, and its unvisited pixels, :776-792):
+// Y, Cb, Cr planes of w*h bytes per frame.
+extern "C" int orc_walk_blocks_yuv444(const OrcTree* t, const float* batch, int64_t w, int64_t h,
+                                      int frames, int block, double diffusion, uint8_t* out) {
+  std::vector<int32_t> curve, owner;
+  std::vector<uint8_t> chosen;
+  if (blocks_indices(t, batch, w, h, frames, block, diffusion, curve, owner, chosen) != 0)
+    return -1;
+  const int64_t n = (int64_t)curve.size();
+  const int64_t pp = w * h;
+  for (int f = 0; f < frames; ++f) {
+    const float* src = batch + 4 * (size_t)f * (size_t)pp;
+    uint8_t* o = out + 3 * (size_t)f * (size_t)pp;
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t pix = curve[(size_t)i];
+      if (owner[(size_t)pix] != i) continue;
+      const double* e = &t->cmap[4 * (size_t)chosen[(size_t)f * (size_t)n + (size_t)i]];
+      rgb_to_yuv444((float)e[0], (float)e[1], (float)e[2], o + pix, o + pp + pix,
+                    o + 2 * pp + pix);
+    }
+    for (int64_t pix = 0; pix < pp; ++pix) {  // unvisited: the source, converted
+      if (owner[(size_t)pix] >= 0) continue;
+      rgb_to_yuv444(src[4 * pix + 0], src[4 * pix + 1], src[4 * pix + 2], o + pix, o + pp + pix,
+                    o + 2 * pp + pix);
     }
   }
   return 0;

@@ -88,6 +88,17 @@ int orc_walk_blocks_u16(const OrcTree* t, const float* batch, int64_t w,
                         int64_t h, int frames, int block, double diffusion,
                         uint16_t* out_u16);
 
+/* Planar 8-bit YUV frames (fmt 7: 4:4:4, Y/Cb/Cr planes of w*h; fmt 8: 4:2:0,
+ * Y then Cb, Cr at (w/2)*(h/2), even w and h) widened to RgbaF through
+ * swscale's matrix (include/rd_types.h:75-97), alpha 65535; 4:2:0 chroma
+ * upsampled as src/rd_blocks_cuda.cu:651-740.  Returns 0 or -1. */
+int orc_yuv_to_rgbaf(const uint8_t* planes, int64_t w, int64_t h, int frames, int fmt,
+                     float* out);
+/* Planar 4:4:4 output of the fused scatter (rd_riemersma_cpu.cpp:711-730,
+ * 776-792): frames*3*w*h bytes from the same walk as orc_walk_blocks. */
+int orc_walk_blocks_yuv444(const OrcTree* t, const float* batch, int64_t w, int64_t h,
+                           int frames, int block, double diffusion, uint8_t* out);
+
 /* Inverse filter g[0..taps] (src/rd_approx_cuda.cu:73-88). */
 void orc_inverse_filter(int taps, double diffusion, double* g);
 /* The Jacobi engine restated on the CPU (src/rd_approx_cuda.cu:90-384):

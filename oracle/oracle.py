@@ -100,6 +100,8 @@ class Oracle(_Lib):
         self._walk_idx = f("walk_seq_index", None, _p, _p, _i64, _i64, _dbl, _i32, _p)
         self._blocks = f("walk_blocks", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p)
         self._blocks16 = f("walk_blocks_u16", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p)
+        self._yuv2f = f("yuv_to_rgbaf", _i32, _p, _i64, _i64, _i32, _i32, _p)
+        self._blocksyuv = f("walk_blocks_yuv444", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p)
         self._inv = f("inverse_filter", None, _i32, _dbl, _p)
         self._jac = f("jacobi", _i32, _p, _p, _i64, _i64, _i32, _i32, _i32, _i32, _dbl, _p)
         self._many = f("walk_seq_many", _i32, _p, _p, _p, _p, _i32, _dbl, _i32)
@@ -191,6 +193,28 @@ class Oracle(_Lib):
         out = np.zeros((frames, h, w, 4), np.uint16)
         rc = self._blocks16(tree.h, _ptr(a), w, h, frames, block, diffusion, _ptr(out))
         if rc != 0:
+            raise ValueError("refused")
+        return out
+
+    def yuv_to_rgbaf(self, planes, w, h, fmt):
+        """Planar 8-bit frames (fmt 7 = 4:4:4, 8 = 4:2:0), as a flat uint8
+        array of whole frames, widened to (frames, h, w, 4) float32."""
+        a = np.ascontiguousarray(planes, dtype=np.uint8).reshape(-1)
+        fb = 3 * w * h if fmt == 7 else w * h + 2 * (w // 2) * (h // 2)
+        frames = a.size // fb
+        out = np.zeros((frames, h, w, 4), np.float32)
+        if self._yuv2f(_ptr(a), w, h, frames, fmt, _ptr(out)) != 0:
+            raise ValueError("refused")
+        return out
+
+    def walk_blocks_yuv444(self, tree, batch, block, diffusion=1.0):
+        """The block walk with planar 4:4:4 output: (frames, 3, h, w) uint8."""
+        a = np.ascontiguousarray(batch, dtype=np.float32)
+        if a.ndim == 3:
+            a = a[None]
+        frames, h, w = a.shape[:3]
+        out = np.zeros((frames, 3, h, w), np.uint8)
+        if self._blocksyuv(tree.h, _ptr(a), w, h, frames, block, diffusion, _ptr(out)) != 0:
             raise ValueError("refused")
         return out
 
@@ -297,6 +321,8 @@ class Ref(_Lib):
         self._walk = f("walk_cpu", None, _p, _p, _i64, _i64, _dbl, _i32)
         self._blocks = f("blocks_cpu", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl)
         self._blocks16 = f("blocks_cpu_u16", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p)
+        self._fused = f("blocks_cpu_fused", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p, _i32,
+                        _p, _i64, _i32, _i32)
         self._bci = f("build_curve_index", _i64, _i32, _i64, _i64, _p, _p)
         self._hp = f("hilbert_point", None, _i64, _i32, _p, _p)
         self._mism = f("curve_mismatches", _i64, _i32, _i32)
@@ -368,6 +394,29 @@ class Ref(_Lib):
         ws = np.array([im.shape[1] for im in images], np.int64)
         hs = np.array([im.shape[0] for im in images], np.int64)
         self._many(tp, ip, _ptr(ws), _ptr(hs), n, diffusion, threads)
+
+
+    def walk_blocks_fused(self, tree, block, w, h, frames, batch=None, yuv_in=None,
+                          yuv_out=True, diffusion=1.0):
+        """RiemersmaBlocksCpu's fused path: reads `batch` (frames, h, w, 4)
+        float32 or planar 4:4:4 `yuv_in` (frames * 3 * h * w bytes); returns
+        planar 4:4:4 (frames, 3, h, w) uint8 or rgba64le (frames, h, w, 4)."""
+        a = None
+        if batch is not None:
+            a = np.ascontiguousarray(batch, dtype=np.float32).copy()
+        g = None
+        if yuv_in is not None:
+            g = np.ascontiguousarray(yuv_in, dtype=np.uint8).reshape(-1)
+        if yuv_out:
+            out = np.zeros((frames, 3, h, w), np.uint8)
+        else:
+            out = np.zeros((frames, h, w, 4), np.uint16)
+        rc = self._fused(tree.h, _ptr(a) if a is not None else None, w, h, frames, block,
+                         diffusion, _ptr(out), int(yuv_out), _ptr(g) if g is not None else None,
+                         3 * w * h, 3 if g is not None else 0, 0)
+        if rc != 0:
+            raise ValueError("refused")
+        return out
 
 
 class RefTree(OracleTree):

@@ -117,6 +117,24 @@ int ref_blocks_cpu_u16(void* p, float* batch, int64_t w, int64_t h, int frames, 
   return err.empty() ? 0 : -1;
 }
 
+// The fused gather and scatter of RiemersmaBlocksCpu: gather_layout 3 reads
+// planar 4:4:4 from gather_raw (4 = interleaved uint16 of gather_channels),
+// 0 reads `batch`; raw_out receives planar 4:4:4 (out_yuv444) or rgba64le.
+int ref_blocks_cpu_fused(void* p, float* batch, int64_t w, int64_t h, int frames, int block,
+                         double diffusion, uint8_t* raw_out, int out_yuv444,
+                         const uint8_t* gather_raw, int64_t gather_frame_bytes,
+                         int gather_layout, int gather_channels) {
+  RefTree* t = static_cast<RefTree*>(p);
+  rd::DitherParams params;
+  params.diffusion = diffusion;
+  std::string err;
+  rd::RiemersmaBlocksCpu(*t->palette, params, t->tree, (std::size_t)w, (std::size_t)h,
+                         reinterpret_cast<rd::RgbaF*>(batch), frames, block, raw_out,
+                         (std::size_t)(w * h), out_yuv444 != 0, &err, gather_raw,
+                         (std::size_t)gather_frame_bytes, gather_layout, gather_channels);
+  return err.empty() ? 0 : -1;
+}
+
 int64_t ref_build_curve_index(int level, int64_t w, int64_t h, int32_t* out,
                               int32_t* owner) {
   std::vector<int> o, ow;

@@ -21,6 +21,8 @@ from ._lib import (  # noqa: F401
     PIX_RGBA16,
     PIX_RGBA8,
     PIX_RGBAF32,
+    PIX_YUV420,
+    PIX_YUV444,
     SA_EDEVICE,
     SA_EINVAL,
     SA_ENOMEM,

@@ -15,8 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "libstipple_amd.so")
 
 PIX_RGBAF32, PIX_RGB8, PIX_RGBA8, PIX_RGB16, PIX_RGBA16, PIX_INDEX8, PIX_INDEX16 = range(7)
-_PIX_CHANNELS = {PIX_RGBAF32: 4, PIX_RGB8: 3, PIX_RGBA8: 4, PIX_RGB16: 3, PIX_RGBA16: 4,
-                 PIX_INDEX8: 1, PIX_INDEX16: 1}
+# Planar 8-bit YUV (block engine only): 4:4:4 in and out, 4:2:0 in.
+PIX_YUV444, PIX_YUV420 = 7, 8
 
 
 # Return codes of the C-ABI (SA_* in include/stipple_amd.h).

@@ -281,6 +281,10 @@ hipError_t timed(const char* name, hipStream_t s, Fn&& launch) {
 
 bool valid_in_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kRgba16; }
 bool valid_out_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kIndex16; }
+// The block engine and the palette build also read planar YUV 4:4:4 / 4:2:0;
+// the block engine writes 4:4:4.
+bool valid_blocks_in_fmt(int f) { return valid_in_fmt(f) || f == sa::kYuv444 || f == sa::kYuv420; }
+bool valid_blocks_out_fmt(int f) { return valid_out_fmt(f) || f == sa::kYuv444; }
 
 }  // namespace
 
@@ -498,7 +502,9 @@ extern "C" int sa_palette_build(const void* src, int src_fmt, int64_t w, int64_t
   if (!out) return fail(err, errlen, SA_EINVAL, "null output handle");
   *out = nullptr;
   if (!src || w < 1 || h < 1) return fail(err, errlen, SA_EINVAL, "bad image");
-  if (!valid_in_fmt(src_fmt)) return fail(err, errlen, SA_EINVAL, "bad pixel format");
+  if (!valid_blocks_in_fmt(src_fmt)) return fail(err, errlen, SA_EINVAL, "bad pixel format");
+  if (src_fmt == sa::kYuv420 && ((w & 1) || (h & 1)))
+    return fail(err, errlen, SA_EUNSUPPORTED, "4:2:0 input needs even width and height");
   int rc = ensure_init(err, errlen);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -519,7 +525,11 @@ extern "C" int sa_palette_build_batch(const sa_build* items, int n, void* stream
     const sa_build& b = items[i];
     if (!b.src || b.width < 1 || b.height < 1)
       return fail(err, errlen, SA_EINVAL, "item %d: bad image", i);
-    if (!valid_in_fmt(b.src_fmt)) return fail(err, errlen, SA_EINVAL, "item %d: bad pixel format", i);
+    if (!valid_blocks_in_fmt(b.src_fmt))
+      return fail(err, errlen, SA_EINVAL, "item %d: bad pixel format", i);
+    if (b.src_fmt == sa::kYuv420 && ((b.width & 1) || (b.height & 1)))
+      return fail(err, errlen, SA_EUNSUPPORTED, "item %d: 4:2:0 input needs even width and height",
+                  i);
   }
   int rc = ensure_init(err, errlen);
   if (rc) return rc;
@@ -1273,8 +1283,11 @@ int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipS
     if (f.width != p.w || f.height != p.h)
       return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
     if (f.dst == f.src) return fail(err, errlen, SA_EINVAL, "frame %d: dst aliases src", i);
-    if (!valid_in_fmt(f.src_fmt) || !valid_out_fmt(f.dst_fmt))
+    if (!valid_blocks_in_fmt(f.src_fmt) || !valid_blocks_out_fmt(f.dst_fmt))
       return fail(err, errlen, SA_EINVAL, "frame %d: bad pixel format", i);
+    if (f.src_fmt == sa::kYuv420 && ((f.width & 1) || (f.height & 1)))
+      return fail(err, errlen, SA_EUNSUPPORTED, "frame %d: 4:2:0 input needs even width and height",
+                  i);
     if (f.palette->count > 256)
       return fail(err, errlen, SA_EREFUSED,
                   "the blocks engine carries one palette index per byte, so the palette would alias "
@@ -1320,7 +1333,7 @@ hipError_t launch_blocks_plan(const BlocksPlan& p, const sa::BlockFrame* d_fr,
                               const int32_t* curve, const double* sw, hipStream_t s) {
   hipError_t e = sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, sw, p.assoc,
                                    p.max_img, p.max_count, p.src_class, p.threads, s);
-  if (e == hipSuccess && p.gap >= 0) e = sa::launch_blocks_fill(d_fr, p.n, p.gap, s);
+  if (e == hipSuccess && p.gap >= 0) e = sa::launch_blocks_fill(d_fr, p.n, p.gap, p.w, p.h, s);
   return e;
 }
 }  // namespace
@@ -1341,7 +1354,7 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
     return sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, p.sw, p.assoc, p.max_img,
                              p.max_count, p.src_class, p.threads, s);
   }));
-  if (p.gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, p.n, p.gap, s));
+  if (p.gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, p.n, p.gap, p.w, p.h, s));
   HIP_TRY(hipFreeAsync(d_fr, s));
   return SA_OK;
 }
@@ -1467,7 +1480,7 @@ hipError_t launch_approx_plan(const ApproxPlan& p, unsigned char* scratch, const
   hipError_t e = sa::launch_approx(d_fr, p.n, curve, p.nb, p.iterations, p.taps, d_g, p.fp64,
                                    p.use_tree, p.assoc, p.max_img, d_cx, q0, q1, &result, p.per,
                                    s);
-  if (e == hipSuccess && p.gap_in) e = sa::launch_blocks_fill(d_fr, p.n, p.w - 1, s);
+  if (e == hipSuccess && p.gap_in) e = sa::launch_blocks_fill(d_fr, p.n, p.w - 1, p.w, p.h, s);
   return e;
 }
 }  // namespace

@@ -60,8 +60,11 @@ struct Raw<kSrcF> {
 // One source pixel in class SRC.  Widening is exact: an 8-bit sample is
 // staged as itself (8-bit class) or as sample * 257 (16-bit class), which is
 // the value load_pixel gives it; formats without alpha stage it opaque.
+// Planar YUV (w x h planes) widens through swscale's matrix to 16 bits and is
+// never in the 8-bit class.
 template <int SRC>
-__device__ __forceinline__ typename Raw<SRC>::T stage_pixel(int fmt, const void* base, int64_t p) {
+__device__ __forceinline__ typename Raw<SRC>::T stage_pixel(int fmt, const void* base, int64_t p,
+                                                            int64_t w, int64_t h) {
   typedef typename Raw<SRC>::T T;
   if constexpr (SRC == kSrc8) {
     if (fmt == kRgba8) return ((const uint32_t*)base)[p];
@@ -70,6 +73,8 @@ __device__ __forceinline__ typename Raw<SRC>::T stage_pixel(int fmt, const void*
   } else if constexpr (SRC == kSrc16) {
     switch (fmt) {
       case kRgba16: return ((const uint2*)base)[p];
+      case kYuv444:
+      case kYuv420: return load_yuv(fmt, base, p, w, h);
       case kRgb16: {
         const uint16_t* s = (const uint16_t*)base + 3 * p;
         return make_uint2((uint32_t)s[0] | ((uint32_t)s[1] << 16), (uint32_t)s[2] | 0xFFFF0000u);
@@ -86,7 +91,7 @@ __device__ __forceinline__ typename Raw<SRC>::T stage_pixel(int fmt, const void*
       }
     }
   } else {
-    return load_pixel(fmt, base, p);
+    return load_any(fmt, base, p, w, h);
   }
 }
 
@@ -113,9 +118,11 @@ __device__ __forceinline__ float unstage_alpha(const typename Raw<SRC>::T& v) {
 
 // The output pixel for palette entry `e` (src/rd_riemersma_cpu.cpp:700-760):
 // alpha is the entry's when associated, else the source pixel's (`a`).
+// Planar output has planes of pp bytes.
 __device__ __forceinline__ void store_out(const BlockFrame& f, int64_t pix, int idx,
-                                          const float4& e, float a) {
+                                          const float4& e, float a, int64_t pp) {
   switch (f.dst_fmt) {
+    case kYuv444: store_yuv444(f.dst, pix, pp, e.x, e.y, e.z); break;
     case kIndex8: ((uint8_t*)f.dst)[pix] = (uint8_t)idx; break;
     case kIndex16: ((uint16_t*)f.dst)[pix] = (uint16_t)idx; break;
     case kRgb8: {
@@ -235,7 +242,7 @@ void blocks_kernel(const BlockFrame* __restrict__ frames,
                                                     const int32_t* __restrict__ curve,
                                                     int64_t nb, int block, int64_t nblocks,
                                                     const double* __restrict__ sw, int img_bytes,
-                                                    int count) {
+                                                    int count, int64_t w, int64_t h) {
   typedef typename Raw<SRC>::T T;
   extern __shared__ __align__(16) unsigned char lds[];
   const int fi = (int)(blockIdx.x % (unsigned)nframes);
@@ -282,7 +289,7 @@ void blocks_kernel(const BlockFrame* __restrict__ frames,
       pix[k] = -1;
       if (in_block && i < nb) {
         const int64_t p = i < m ? (int64_t)curve[i] : 0;
-        rawk[k * (NT / 16)] = stage_pixel<SRC>(f.src_fmt, f.src, p);
+        rawk[k * (NT / 16)] = stage_pixel<SRC>(f.src_fmt, f.src, p, w, h);
         pix[k] = i != 0 ? (int32_t)p : -1;
       }
       if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
@@ -302,7 +309,7 @@ void blocks_kernel(const BlockFrame* __restrict__ frames,
         const int idx = oidxk[k * (NT / 16)];
         const float4 en = s.ent[idx];
         const float a = ASSOC ? en.w : unstage_alpha<SRC>(rawk[k * (NT / 16)]);
-        store_out(f, pix[k], idx, en, a);
+        store_out(f, pix[k], idx, en, a, w * h);
       }
       if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
@@ -310,14 +317,16 @@ void blocks_kernel(const BlockFrame* __restrict__ frames,
 }
 
 // The pixel the block curve never reaches keeps its source value
-// (src/rd_riemersma_cpu.cpp:776-803, src/rd_blocks_cuda.cu:389-443).
+// (src/rd_riemersma_cpu.cpp:776-803, src/rd_blocks_cuda.cu:389-443); planar
+// output gets the source converted, as the CPU engine's fused scatter does.
 __global__ void blocks_fill_kernel(const BlockFrame* __restrict__ frames, int nframes,
-                                   int64_t pix) {
+                                   int64_t pix, int64_t w, int64_t h) {
   const int fi = blockIdx.x * blockDim.x + threadIdx.x;
   if (fi >= nframes) return;
   const BlockFrame f = frames[fi];
-  const float4 p = load_pixel(f.src_fmt, f.src, pix);
+  const float4 p = load_any(f.src_fmt, f.src, pix, w, h);
   switch (f.dst_fmt) {
+    case kYuv444: store_yuv444(f.dst, pix, w * h, p.x, p.y, p.z); break;
     case kIndex8: ((uint8_t*)f.dst)[pix] = 0; break;
     case kIndex16: ((uint16_t*)f.dst)[pix] = 0; break;
     case kRgb8: {
@@ -358,30 +367,31 @@ __global__ void blocks_fill_kernel(const BlockFrame* __restrict__ frames, int nf
 template <int SRC, int NT>
 void launch_blocks_nt(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
                       int block, int64_t nblocks, const double* sw, int assoc, int img_bytes,
-                      int count, size_t lds, hipStream_t s) {
+                      int count, int64_t w, int64_t h, size_t lds, hipStream_t s) {
   const dim3 grid((unsigned)(((nblocks + NT - 1) / NT) * nframes));
   if (assoc)
     blocks_kernel<true, SRC, NT><<<grid, dim3(NT), lds, s>>>(frames, nframes, curve, nb, block,
-                                                             nblocks, sw, img_bytes, count);
+                                                             nblocks, sw, img_bytes, count, w, h);
   else
     blocks_kernel<false, SRC, NT><<<grid, dim3(NT), lds, s>>>(frames, nframes, curve, nb, block,
-                                                              nblocks, sw, img_bytes, count);
+                                                              nblocks, sw, img_bytes, count, w, h);
 }
 
 template <int SRC>
 void launch_blocks_src(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,
                        int block, int64_t nblocks, const double* sw, int assoc, int img_bytes,
-                       int count, int threads, size_t lds, hipStream_t s) {
+                       int count, int threads, int64_t w, int64_t h, size_t lds, hipStream_t s) {
   switch (threads) {
-    case 64: launch_blocks_nt<SRC, 64>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, lds, s); break;
-    case 128: launch_blocks_nt<SRC, 128>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, lds, s); break;
-    case 512: launch_blocks_nt<SRC, 512>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, lds, s); break;
-    default: launch_blocks_nt<SRC, 256>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, lds, s); break;
+    case 64: launch_blocks_nt<SRC, 64>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, w, h, lds, s); break;
+    case 128: launch_blocks_nt<SRC, 128>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, w, h, lds, s); break;
+    case 512: launch_blocks_nt<SRC, 512>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, w, h, lds, s); break;
+    default: launch_blocks_nt<SRC, 256>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, w, h, lds, s); break;
   }
 }
 
 int blocks_src_class(int fmt) {
-  return fmt == kRgbaF32 ? kSrcF : (fmt == kRgb16 || fmt == kRgba16) ? kSrc16 : kSrc8;
+  if (fmt == kRgbaF32) return kSrcF;
+  return (fmt == kRgb16 || fmt == kRgba16 || fmt == kYuv444 || fmt == kYuv420) ? kSrc16 : kSrc8;
 }
 
 // `threads` (64, 128, 256 or 512) blocks per workgroup: every workgroup stages
@@ -391,8 +401,6 @@ int blocks_src_class(int fmt) {
 hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t w,
                          int64_t h, int64_t nb, int block, const double* sw, int assoc,
                          int img_bytes, int count, int src_class, int threads, hipStream_t s) {
-  (void)w;
-  (void)h;
   if (nframes <= 0 || nb <= 0) return hipSuccess;
   if (threads != 64 && threads != 128 && threads != 512) threads = 256;
   while (threads > 64 && blocks_lds_bytes(img_bytes, count, src_class, threads) > kBlocksMaxLds)
@@ -401,18 +409,18 @@ hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* c
   if (lds > kBlocksMaxLds) return hipErrorInvalidValue;
   const int64_t nblocks = (nb + block - 1) / block;
   switch (src_class) {
-    case kSrcF: launch_blocks_src<kSrcF>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, lds, s); break;
-    case kSrc16: launch_blocks_src<kSrc16>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, lds, s); break;
-    default: launch_blocks_src<kSrc8>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, lds, s); break;
+    case kSrcF: launch_blocks_src<kSrcF>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, w, h, lds, s); break;
+    case kSrc16: launch_blocks_src<kSrc16>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, w, h, lds, s); break;
+    default: launch_blocks_src<kSrc8>(frames, nframes, curve, nb, block, nblocks, sw, assoc, img_bytes, count, threads, w, h, lds, s); break;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_blocks_fill(const BlockFrame* frames, int nframes, int64_t gap_pixel,
-                              hipStream_t s) {
+                              int64_t w, int64_t h, hipStream_t s) {
   if (nframes <= 0) return hipSuccess;
-  blocks_fill_kernel<<<dim3((unsigned)((nframes + 63) / 64)), dim3(64), 0, s>>>(frames, nframes,
-                                                                              gap_pixel);
+  blocks_fill_kernel<<<dim3((unsigned)((nframes + 63) / 64)), dim3(64), 0, s>>>(
+      frames, nframes, gap_pixel, w, h);
   return hipGetLastError();
 }
 

@@ -17,6 +17,8 @@ enum PixFmt : int32_t {
   kRgba16 = 4,
   kIndex8 = 5,
   kIndex16 = 6,
+  kYuv444 = 7,  // planar 8-bit Y, Cb, Cr planes of w*h bytes (block engine only)
+  kYuv420 = 8,  // planar 8-bit Y (w*h), Cb and Cr ((w/2)*(h/2)); block engine input only
 };
 
 inline int pix_bytes(int fmt) {
@@ -30,6 +32,74 @@ inline int pix_bytes(int fmt) {
     case kIndex16: return 2;
   }
   return 0;
+}
+
+// Bytes of one w x h image in a planar format.
+inline int64_t planar_bytes(int fmt, int64_t w, int64_t h) {
+  return fmt == kYuv444 ? 3 * w * h : fmt == kYuv420 ? w * h + 2 * (w / 2) * (h / 2) : 0;
+}
+
+// swscale's 8-bit YUV to 16-bit RGB in integers (SwsYuvToRgb16,
+// include/rd_types.h:75-97), returned as r | g << 16 and b | 0xFFFF << 16:
+// the 16-bit staging word with the opaque alpha the video reader synthesises.
+__device__ __forceinline__ int sws_clip16(int x) {
+  const int v = x + (1 << 15);
+  return v < 0 ? 0 : (v > 65535 ? 65535 : v);
+}
+__device__ __forceinline__ uint2 sws_yuv_rgb16(unsigned y, unsigned u, unsigned v) {
+  // Luma: (y - 128) * 512 + 2^16, less the 8192 black level, times the
+  // 9539/2^13 range gain, plus the rounding term 2^13 and the -2^29 that
+  // re-centres the later +2^15.
+  int yy = (int)((y - 128u) * 512u);
+  const int uu = (int)((u - 128u) * 512u);
+  const int vv = (int)((v - 128u) * 512u);
+  yy = (yy + 0x10000 - 8192) * 9539 + (1 << 13) - (1 << 29);
+  const int r = sws_clip16((vv * 13075 + yy) >> 14);
+  const int g = sws_clip16((vv * -6660 + uu * -3209 + yy) >> 14);
+  const int b = sws_clip16((uu * 16525 + yy) >> 14);
+  return make_uint2((uint32_t)r | ((uint32_t)g << 16), (uint32_t)b | 0xFFFF0000u);
+}
+
+// 4:2:0 chroma at luma (x, y): the 2x2 neighbourhood of (x/2, y/2), edges
+// repeated, weighted (2 - ox, ox) x (2 - oy, oy) with ox = x & 1, oy = y & 1,
+// rounded by +2 >> 2 (src/rd_blocks_cuda.cu:651-675).  Even coordinates copy.
+__device__ __forceinline__ unsigned chroma420(const uint8_t* plane, int64_t w2, int64_t h2,
+                                              int64_t x, int64_t y) {
+  const int64_t cx = x >> 1, cy = y >> 1;
+  const int ox = (int)(x & 1), oy = (int)(y & 1);
+  const int64_t cx1 = cx + 1 < w2 ? cx + 1 : cx;
+  const int64_t cy1 = cy + 1 < h2 ? cy + 1 : cy;
+  const int a = plane[cy * w2 + cx], b = plane[cy * w2 + cx1];
+  const int c = plane[cy1 * w2 + cx], d = plane[cy1 * w2 + cx1];
+  const int top = a * (2 - ox) + b * ox;
+  const int bot = c * (2 - ox) + d * ox;
+  return (unsigned)((top * (2 - oy) + bot * oy + 2) >> 2);
+}
+
+// Pixel p of a planar YUV image as the 16-bit staging word (sws_yuv_rgb16).
+__device__ __forceinline__ uint2 load_yuv(int fmt, const void* base, int64_t p, int64_t w,
+                                          int64_t h) {
+  const uint8_t* s = (const uint8_t*)base;
+  const int64_t pp = w * h;
+  if (fmt == kYuv444) return sws_yuv_rgb16(s[p], s[pp + p], s[2 * pp + p]);
+  const int64_t w2 = w >> 1, h2 = h >> 1;
+  const int64_t y = p / w, x = p - y * w;
+  return sws_yuv_rgb16(s[p], chroma420(s + pp, w2, h2, x, y),
+                       chroma420(s + pp + w2 * h2, w2, h2, x, y));
+}
+
+// Limited-range BT.601 8-bit Y, Cb, Cr of Q16 floats r, g, b (their integer
+// parts' high bytes), clamped to 0..255 (src/rd_riemersma_cpu.cpp:718-730).
+__device__ __forceinline__ void store_yuv444(void* dst, int64_t pix, int64_t pp, float r, float g,
+                                             float b) {
+  const int r8 = (int)r >> 8, g8 = (int)g >> 8, b8 = (int)b >> 8;
+  const int yv = ((66 * r8 + 129 * g8 + 25 * b8 + 128) >> 8) + 16;
+  const int uv = ((-38 * r8 - 74 * g8 + 112 * b8 + 128) >> 8) + 128;
+  const int vv = ((112 * r8 - 94 * g8 - 18 * b8 + 128) >> 8) + 128;
+  uint8_t* d = (uint8_t*)dst;
+  d[pix] = (uint8_t)(yv < 0 ? 0 : (yv > 255 ? 255 : yv));
+  d[pp + pix] = (uint8_t)(uv < 0 ? 0 : (uv > 255 ? 255 : uv));
+  d[2 * pp + pix] = (uint8_t)(vv < 0 ? 0 : (vv > 255 ? 255 : vv));
 }
 
 // Loads one source pixel as Q16 floats: 8-bit samples widen x257 (exact),
@@ -64,6 +134,18 @@ __device__ __forceinline__ float4 load_pixel(int fmt, const void* base, int64_t 
       break;
   }
   return o;
+}
+
+// Any source pixel as Q16 floats, planar YUV (w x h) included; the block
+// engine and the palette build read every format through this.
+__device__ __forceinline__ float4 load_any(int fmt, const void* base, int64_t p, int64_t w,
+                                           int64_t h) {
+  if (fmt == kYuv444 || fmt == kYuv420) {
+    const uint2 v = load_yuv(fmt, base, p, w, h);
+    return make_float4((float)(v.x & 0xFFFFu), (float)(v.x >> 16), (float)(v.y & 0xFFFFu),
+                       65535.0f);
+  }
+  return load_pixel(fmt, base, p);
 }
 
 // Where a resumable walk stopped: the next visit, and the 64 lanes' in-flight
@@ -143,7 +225,7 @@ hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* c
                          int64_t h, int64_t nb, int block, const double* sw, int assoc,
                          int img_bytes, int count, int src_class, int threads, hipStream_t s);
 hipError_t launch_blocks_fill(const BlockFrame* frames, int nframes, int64_t gap_pixel,
-                              hipStream_t s);
+                              int64_t w, int64_t h, hipStream_t s);
 
 size_t approx_lds_bytes(int img_bytes, int taps, int fp64, int assoc, int per);
 hipError_t launch_approx(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,

@@ -240,7 +240,8 @@ __global__ void k_flags(const Frame* __restrict__ fr, Map m) {
   if (p >= d.N) return;
   int f = 1;
   if (p % d.w != 0)
-    f = same_px(load_pixel(d.fmt, d.src, p), load_pixel(d.fmt, d.src, p - 1)) ? 0 : 1;
+    f = same_px(load_any(d.fmt, d.src, p, d.w, d.h), load_any(d.fmt, d.src, p - 1, d.w, d.h)) ? 0
+                                                                                             : 1;
   d.flags[p] = f;
 }
 
@@ -268,7 +269,7 @@ __global__ void k_run_data(const Frame* __restrict__ fr, Map m) {
   const int64_t p = d.start[r];
   const int64_t e = (r + 1 < R) ? d.start[r + 1] : d.N;
   d.count[r] = (int32_t)(e - p);
-  const float4 f = load_pixel(d.fmt, d.src, p);
+  const float4 f = load_any(d.fmt, d.src, p, d.w, d.h);
   double a[4];
   assoc_pixel(d.assoc != 0, f.x, f.y, f.z, f.w, a);
   d.px[r] = f;

@@ -452,9 +452,11 @@ std::string RiemersmaBlocksCuda(const Palette& palette, const DitherParams& para
   if (options.frames < 1) return "RiemersmaBlocksCuda: frames must be >= 1";
   if (state_slot < 0 || state_slot >= kGpuStateSlots)
     return "RiemersmaBlocksCuda: state_slot out of range";
-  if (options.emit_yuv444 || options.in_mode != BlockOptions::InMode::Interleaved16)
-    return "RiemersmaBlocksCuda: planar YUV input/output is not built on this engine";
-  if (options.upload_u16 && (!in_u16 || (options.in_channels != 3 && options.in_channels != 4)))
+  // The planar modes pass 8-bit planes through the uint16 pointers, as the
+  // reference does (src/rd_blocks_cuda.cu:1066-1081, :1532-1545).
+  const bool in_planar = options.upload_u16 && options.in_mode != BlockOptions::InMode::Interleaved16;
+  if (options.upload_u16 && !in_u16) return "RiemersmaBlocksCuda: upload_u16 needs in_u16";
+  if (options.upload_u16 && !in_planar && options.in_channels != 3 && options.in_channels != 4)
     return "RiemersmaBlocksCuda: upload_u16 needs in_u16 with 3 or 4 channels";
   if (options.emit_u16 && !out_u16) return "RiemersmaBlocksCuda: emit_u16 needs out_u16";
   if (!options.upload_u16 && !batch) return "RiemersmaBlocksCuda: null batch";
@@ -466,20 +468,41 @@ std::string RiemersmaBlocksCuda(const Palette& palette, const DitherParams& para
   BlockSlot& slot = g_slots[state_slot];
   std::lock_guard<std::mutex> lock(slot.mu);
   const int frames = options.frames;
-  const int in_fmt = options.upload_u16 ? (options.in_channels == 3 ? SA_PIX_RGB16 : SA_PIX_RGBA16)
-                                        : SA_PIX_RGBAF32;
-  const size_t in_px = options.upload_u16 ? 2 * (size_t)options.in_channels : sizeof(RgbaF);
-  const int out_fmt = options.emit_u16 ? SA_PIX_RGBA16 : SA_PIX_RGBAF32;
-  const size_t out_px = options.emit_u16 ? 8 : sizeof(RgbaF);
-  if (!slot.src.reserve(in_px * px * frames) || !slot.dst.reserve(out_px * px * frames))
+  // Bytes per frame of the source and the result.
+  int in_fmt = SA_PIX_RGBAF32;
+  size_t in_frame = sizeof(RgbaF) * px;
+  if (options.upload_u16) {
+    switch (options.in_mode) {
+      case BlockOptions::InMode::PlanarYuv444:
+      case BlockOptions::InMode::PlanarYuv444Prepass:  // the same values, widened in one pass
+        in_fmt = SA_PIX_YUV444P8;
+        in_frame = 3 * px;
+        break;
+      case BlockOptions::InMode::PlanarYuv420:
+        in_fmt = SA_PIX_YUV420P8;
+        in_frame = px + 2 * (width / 2) * (height / 2);
+        break;
+      default:
+        in_fmt = options.in_channels == 3 ? SA_PIX_RGB16 : SA_PIX_RGBA16;
+        in_frame = 2 * (size_t)options.in_channels * px;
+        break;
+    }
+  }
+  int out_fmt = SA_PIX_RGBAF32;
+  size_t out_frame = sizeof(RgbaF) * px;
+  if (options.emit_u16) {
+    out_fmt = options.emit_yuv444 ? SA_PIX_YUV444P8 : SA_PIX_RGBA16;
+    out_frame = options.emit_yuv444 ? 3 * px : 8 * px;
+  }
+  if (!slot.src.reserve(in_frame * frames) || !slot.dst.reserve(out_frame * frames))
     return "device allocation failed";
   const void* host_in = options.upload_u16 ? (const void*)in_u16 : (const void*)batch;
-  hipError_t he = hipMemcpy(slot.src.p, host_in, in_px * px * frames, hipMemcpyHostToDevice);
+  hipError_t he = hipMemcpy(slot.src.p, host_in, in_frame * frames, hipMemcpyHostToDevice);
   if (he != hipSuccess) return hip_fail("upload", he);
   std::vector<sa_image> fr(frames);
   for (int f = 0; f < frames; ++f)
-    fr[f] = sa_image{(unsigned char*)slot.src.p + in_px * px * f,
-                     (unsigned char*)slot.dst.p + out_px * px * f, (int64_t)width,
+    fr[f] = sa_image{(unsigned char*)slot.src.p + in_frame * f,
+                     (unsigned char*)slot.dst.p + out_frame * f, (int64_t)width,
                      (int64_t)height, in_fmt, out_fmt, pr.p};
   char err[kErr] = {0};
   if (sa_walk_blocks(fr.data(), frames, options.block, params.diffusion, nullptr, err,
@@ -487,7 +510,7 @@ std::string RiemersmaBlocksCuda(const Palette& palette, const DitherParams& para
     return prefixed("RiemersmaBlocksCuda", err);
   void* host_out = options.emit_u16 ? (void*)out_u16 : (void*)batch;
   if (!host_out) return "RiemersmaBlocksCuda: nowhere to write the result";
-  he = hipMemcpy(host_out, slot.dst.p, out_px * px * frames, hipMemcpyDeviceToHost);
+  he = hipMemcpy(host_out, slot.dst.p, out_frame * frames, hipMemcpyDeviceToHost);
   if (he != hipSuccess) return hip_fail("download", he);
   return "";
 }

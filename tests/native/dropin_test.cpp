@@ -5,6 +5,7 @@
 // ColorTree, hand Palette + tree + host RgbaF pixels to the engines, compare
 // with the CPU restatement (oracle/oracle.h) on the same inputs.  Prints one
 // line per check and exits nonzero on any failure.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -212,6 +213,52 @@ int main() {
     check(!rd::RiemersmaBlocksCuda(*pal, params, tree, W, H, src.data(), bad, nullptr, out.data())
                .empty(),
           "RiemersmaBlocksCuda refuses frames < 1");
+  }
+
+  // ---- RiemersmaBlocksCuda: planar YUV through the uint16 pointers -------------------------
+  {
+    const int w = 174, h = 120, F = 2;
+    const size_t pp = (size_t)w * h;
+    unsigned s = 77u;
+    auto rnd = [&]() {
+      s = s * 1664525u + 1013904223u;
+      return (uint8_t)(s >> 24);
+    };
+    std::vector<uint8_t> y444(3 * pp * F), y420(pp * 3 / 2 * F);
+    for (uint8_t& b : y444) b = rnd();
+    for (uint8_t& b : y420) b = rnd();
+    std::vector<float> wide(4 * pp * F);
+    rd::BlockOptions bo;
+    bo.block = 64;
+    bo.frames = F;
+    bo.upload_u16 = true;
+    bo.emit_u16 = true;
+    bo.emit_yuv444 = true;
+    std::vector<uint8_t> out(3 * pp * F), expect(3 * pp * F);
+    orc_yuv_to_rgbaf(y444.data(), w, h, F, 7, wide.data());
+    orc_walk_blocks_yuv444(ot, wide.data(), w, h, F, 64, 1.0, expect.data());
+    for (rd::BlockOptions::InMode m : {rd::BlockOptions::InMode::PlanarYuv444,
+                                       rd::BlockOptions::InMode::PlanarYuv444Prepass}) {
+      bo.in_mode = m;
+      std::fill(out.begin(), out.end(), 0);
+      const std::string e = rd::RiemersmaBlocksCuda(
+          *pal, params, tree, w, h, nullptr, bo, nullptr, reinterpret_cast<uint16_t*>(out.data()),
+          0, reinterpret_cast<const uint16_t*>(y444.data()));
+      check(e.empty() && out == expect, "RiemersmaBlocksCuda planar 4:4:4 in, emit_yuv444 out", e);
+    }
+    bo.in_mode = rd::BlockOptions::InMode::PlanarYuv420;
+    bo.emit_yuv444 = false;
+    std::vector<uint16_t> o16(4 * pp * F), e16(4 * pp * F);
+    orc_yuv_to_rgbaf(y420.data(), w, h, F, 8, wide.data());
+    orc_walk_blocks_u16(ot, wide.data(), w, h, F, 64, 1.0, e16.data());
+    const std::string e = rd::RiemersmaBlocksCuda(*pal, params, tree, w, h, nullptr, bo, nullptr,
+                                                  o16.data(), 1,
+                                                  reinterpret_cast<const uint16_t*>(y420.data()));
+    check(e.empty() && o16 == e16, "RiemersmaBlocksCuda planar 4:2:0 in, rgba64le out", e);
+    check(!rd::RiemersmaBlocksCuda(*pal, params, tree, w - 1, h, nullptr, bo, nullptr,
+                                   o16.data(), 1, reinterpret_cast<const uint16_t*>(y420.data()))
+               .empty(),
+          "RiemersmaBlocksCuda refuses 4:2:0 with an odd side");
   }
 
   // ---- RiemersmaApproxCuda --------------------------------------------------------------
