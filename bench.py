@@ -163,24 +163,31 @@ def run_ramp(args, sa, torch, synth, dev):
 
 
 def run_video(args, sa, torch, synth, dev):
-    """1920x1080 RGB8 plasma frames, a 16-colour palette per frame, block walk B=64."""
+    """1920x1080 plasma frames, a 16-colour palette per frame, block walk B=64:
+    RGB8 in and out, or (--video-format yuv420) the video path's wire formats,
+    planar 4:2:0 in and 4:4:4 out."""
     nf = args.frames or 32
+    yuv = args.video_format == "yuv420"
     params = synth.plasma_params(3)
     t0 = time.perf_counter()
     frames = [synth.plasma_frame(t, params=params) for t in range(nf)]
+    if yuv:
+        frames = [synth.rgb8_to_yuv420(f) for f in frames]
     log(f"generated {nf} frames in {time.perf_counter() - t0:.1f}s")
-    h, w = frames[0].shape[:2]
+    h, w = 1080, 1920
+    fin = sa.PIX_YUV420 if yuv else sa.PIX_RGB8
+    fout = sa.PIX_YUV444 if yuv else sa.PIX_RGB8
     srcs = [torch.from_numpy(f.reshape(-1).copy()).cuda() for f in frames]
     dsts = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in frames]
     torch.cuda.synchronize()
 
     def build():
         # every frame's palette in one batched device build
-        return sa.Palette.build_batch([(srcs[k], w, h, sa.PIX_RGB8, 16, False) for k in range(nf)])
+        return sa.Palette.build_batch([(srcs[k], w, h, fin, 16, False) for k in range(nf)])
 
     def step():
         pals = build()
-        sa.walk_blocks([sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, pals[k])
+        sa.walk_blocks([sa.image_spec(srcs[k], dsts[k], w, h, fin, fout, pals[k])
                         for k in range(nf)], block=64)
         torch.cuda.synchronize()
         return pals
@@ -189,8 +196,7 @@ def run_video(args, sa, torch, synth, dev):
     if not args.no_tune:
         from stipple_amd import tune
         tp = build()
-        tspecs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, tp[k])
-                  for k in range(nf)]
+        tspecs = [sa.image_spec(srcs[k], dsts[k], w, h, fin, fout, tp[k]) for k in range(nf)]
         tuned = tune.tune("blocks.threads", lambda: sa.walk_blocks(tspecs, block=64),
                           outputs=lambda: dsts)
         log(f"tuned {tuned}")
@@ -201,8 +207,7 @@ def run_video(args, sa, torch, synth, dev):
     for _ in range(args.steps):
         pals = step()
     t = (time.perf_counter() - t0) / args.steps
-    specs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, pals[k])
-             for k in range(nf)]
+    specs = [sa.image_spec(srcs[k], dsts[k], w, h, fin, fout, pals[k]) for k in range(nf)]
     sa.timing_enable(True)
     sa.timing_collect()
     for _ in range(3):
@@ -219,8 +224,8 @@ def run_video(args, sa, torch, synth, dev):
     lat = []
     for k in range(0 if args.no_plain else nf):
         a = time.perf_counter()
-        p = sa.Palette.build(srcs[k], w, h, sa.PIX_RGB8, 16, False, False)
-        sa.walk_blocks([sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, p)], block=64)
+        p = sa.Palette.build(srcs[k], w, h, fin, 16, False, False)
+        sa.walk_blocks([sa.image_spec(srcs[k], dsts[k], w, h, fin, fout, p)], block=64)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - a)
     px = w * h
@@ -228,25 +233,43 @@ def run_video(args, sa, torch, synth, dev):
     if not args.no_cpu:
         from oracle.oracle import Ref, Oracle
         lib = Ref() if Ref.available() else Oracle()
-        f = synth.widen_q16(frames[0])
+        if yuv:
+            # the CPU engine reads 4:4:4 only: upsampled beforehand, untimed
+            f = Oracle().yuv_to_rgbaf(frames[0], w, h, sa.PIX_YUV420)[0]
+        else:
+            f = synth.widen_q16(frames[0])
         t1 = time.perf_counter()
         tr = lib.tree_build(f, 16, False, False)
-        lib.walk_blocks(tr, f, 64)
+        if yuv and isinstance(lib, Ref):
+            lib.walk_blocks_fused(tr, 64, w, h, 1, batch=f, yuv_out=True)
+        elif yuv:
+            lib.walk_blocks_yuv444(tr, f, 64)
+        else:
+            lib.walk_blocks(tr, f, 64)
         tc = time.perf_counter() - t1
         cpu = {"value": round(px / tc / 1e6, 3), "unit": "Mpixel/s", "cores": 1,
                "kind": "reference" if Ref.available() else "port",
-               "sample": "one 1920x1080 frame: ColorTree::Build + RiemersmaBlocksCpu B=64"}
-    achieved = 6.0 * px * nf / (walk_ms * 1e-3) / 1e9
+               "sample": "one 1920x1080 frame: ColorTree::Build + RiemersmaBlocksCpu B=64" +
+                         (" with the fused 4:4:4 scatter (the frame widened from 4:2:0 "
+                          "beforehand, untimed)" if yuv else "")}
+    bpp = 4.5 if yuv else 6.0
+    achieved = bpp * px * nf / (walk_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": _traffic("blocks_c3_traffic"),
+            "frac": round(achieved / HBM_PEAK_GBS, 6),
+            "traffic": _traffic("blocks_yuv420_traffic" if yuv else "blocks_c3_traffic"),
             "kernel": "blocks_kernel",
-            "note": f"algorithmic 6 B/px (RGB8 in + out) over one {nf}-frame launch; the kernel "
-                    "is FP64-VALU bound (about 99 + 15 x candidates non-FMA FP64 ops per px)"}
+            "note": (f"algorithmic 4.5 B/px (4:2:0 in + 4:4:4 out)" if yuv else
+                     f"algorithmic 6 B/px (RGB8 in + out)") +
+                    f" over one {nf}-frame launch; the kernel is FP64-VALU bound (about 99 + "
+                    "15 x candidates non-FMA FP64 ops per px)"}
+    workload = (f"{nf} 1920x1080 plasma frames as planar 4:2:0, a 16-colour octree palette per "
+                "frame built on the device from the planes, block walk B=64, planar 4:4:4 out"
+                if yuv else
+                f"{nf} 1920x1080 RGB8 plasma frames, a 16-colour octree palette per "
+                "frame built on the device, block walk B=64, RGB8 out")
     return _line(args, "Mpixel/s and frames/s, 1920x1080 video, block engine B=64",
                  px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f64",
-                 {"workload": f"{nf} 1920x1080 RGB8 plasma frames, a 16-colour octree palette per "
-                              "frame built on the device, block walk B=64, RGB8 out",
-                  "frames": nf, "device": dev}, roof, cpu,
+                 {"workload": workload, "frames": nf, "device": dev}, roof, cpu,
                  {"fps": round(nf / t, 2), "walk_ms": round(walk_ms, 3), "tuned": tuned,
                   "walk_fps": round(nf / (walk_ms * 1e-3), 1), "palette_build_ms": round(pal_ms, 3),
                   "plain": None if not lat else {
@@ -368,6 +391,8 @@ def main():
     ap.add_argument("--head", type=int, default=0,
                     help="batch chain: launch the walk of the N largest images before the "
                          "rest are classified (0: one walk launch after all palettes)")
+    ap.add_argument("--video-format", choices=["rgb8", "yuv420"], default="rgb8",
+                    help="video workload: RGB8 in and out, or planar 4:2:0 in and 4:4:4 out")
     ap.add_argument("--workload", choices=["stills", "ramp", "video", "approx4k"],
                     default="stills",
                     help="stills: 64 uneven RGBA8 stills, sequential engine (the headline); "

@@ -137,3 +137,20 @@ def radial_frame16(t, w=3840, h=2160, seed=4, salt=0.05) -> np.ndarray:
     vals = np.where(rng.random((h, w)) < 0.5, 0.0, 65535.0)
     img[mask] = vals[mask][:, None]
     return np.clip(np.round(img), 0, 65535).astype(np.uint16)
+
+
+def rgb8_to_yuv420(img: np.ndarray) -> np.ndarray:
+    """An RGB8 frame (even sides) as planar 8-bit 4:2:0 bytes, the way an
+    encoder would hand it over: limited-range BT.601 luma per pixel, chroma of
+    each 2x2 block's mean colour.  Synthetic input only (the engines read the
+    result; nothing here is on their path)."""
+    h, w = img.shape[:2]
+    rgb = img[..., :3].astype(np.int64)
+    r, g, b = rgb[..., 0], rgb[..., 1], rgb[..., 2]
+    y = ((66 * r + 129 * g + 25 * b + 128) >> 8) + 16
+    m = rgb.reshape(h // 2, 2, w // 2, 2, 3).sum(axis=(1, 3)) // 4
+    r2, g2, b2 = m[..., 0], m[..., 1], m[..., 2]
+    u = ((-38 * r2 - 74 * g2 + 112 * b2 + 128) >> 8) + 128
+    v = ((112 * r2 - 94 * g2 - 18 * b2 + 128) >> 8) + 128
+    planes = [np.clip(p, 0, 255).astype(np.uint8).reshape(-1) for p in (y, u, v)]
+    return np.concatenate(planes)
