@@ -229,6 +229,10 @@ def run_video(args, sa, torch, synth, dev):
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - a)
     px = w * h
+    piped = None
+    if yuv:
+        piped = _video_pipeline(sa, np, frames, w, h)
+        log(f"pipeline {piped}")
     cpu = None
     if not args.no_cpu:
         from oracle.oracle import Ref, Oracle
@@ -271,11 +275,51 @@ def run_video(args, sa, torch, synth, dev):
                  px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f64",
                  {"workload": workload, "frames": nf, "device": dev}, roof, cpu,
                  {"fps": round(nf / t, 2), "walk_ms": round(walk_ms, 3), "tuned": tuned,
+                  "pipeline": piped,
                   "walk_fps": round(nf / (walk_ms * 1e-3), 1), "palette_build_ms": round(pal_ms, 3),
                   "plain": None if not lat else {
                       "fps": round(nf / sum(lat), 2),
                       "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                                      "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)}}})
+
+
+def _video_pipeline(sa, np, frames, w, h, n=128, batch=8, slots=3):
+    """Host to host through the frame pipeline: 4:2:0 frames copied by the
+    reader into pinned staging, uploaded, walked (each slot's captured chain,
+    palette of frame 0 resident) and downloaded as 4:4:4, in order.  The rate
+    includes both PCIe directions, so it is reported beside the line, never
+    as its value."""
+    import torch
+    from stipple_amd.pipeline import FramePipeline
+    src0 = torch.from_numpy(frames[0].copy()).cuda()
+    pal = sa.Palette.build(src0, w, h, sa.PIX_YUV420, 16, False)
+    pipe = FramePipeline(w, h, sa.PIX_YUV420, sa.PIX_YUV444, palette=pal, batch=batch,
+                         slots=slots)
+
+    def go(count):
+        it = iter(range(count))
+
+        def read(buf):
+            k = next(it, None)
+            if k is None:
+                return False
+            np.copyto(buf, frames[k % len(frames)])
+            return True
+
+        t0 = time.perf_counter()
+        got = pipe.run(read, lambda i, buf: None)
+        return got, time.perf_counter() - t0
+
+    go(2 * batch * slots)
+    pipe.latency_s.clear()
+    got, t = go(n)
+    lat = np.array(pipe.latency_s) * 1e3
+    pipe.close()
+    return {"frames": got, "batch": batch, "slots": slots, "fps": round(got / t, 1),
+            "value": round(got * w * h / t / 1e6, 1), "unit": "Mpixel/s",
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3)},
+            "note": "host-resident 4:2:0 in, host 4:4:4 out, PCIe both ways included"}
 
 
 def run_approx4k(args, sa, torch, synth, dev):

@@ -1,0 +1,117 @@
+"""The frame pipeline (reader -> device slots -> ordered writer) on the GPU.
+
+Every frame comes back in order and equal, byte for byte, to the engine's
+result for that frame alone; reader and writer stalls shuffle the timing.
+Each case prints its name first.
+"""
+import random
+import time
+
+import numpy as np
+import pytest
+
+from stipple_amd import synth
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _frames_420(n, w, h, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        rgb = synth.sinusoid_image(w, h, rng, channels=3)
+        out.append(synth.rgb8_to_yuv420(rgb))
+    return out
+
+
+def _run(pipe, frames, stall):
+    rnd = random.Random(len(frames))
+    it = iter(frames)
+    got = []
+
+    def read(buf):
+        f = next(it, None)
+        if f is None:
+            return False
+        if stall and rnd.random() < 0.3:
+            time.sleep(0.002)
+        buf[:] = f
+        return True
+
+    def write(index, buf):
+        if stall and rnd.random() < 0.3:
+            time.sleep(0.003)
+        got.append((index, buf.copy()))
+
+    n = pipe.run(read, write)
+    return n, got
+
+
+@pytest.mark.parametrize("batch,slots,stall", [(4, 3, True), (3, 1, False), (5, 2, True)])
+def test_pipeline_resident_palette_yuv(gpu, oracle, batch, slots, stall):
+    """4:2:0 in, 4:4:4 out, one resident palette: full batches replay each
+    slot's captured chain, the short last batch walks eagerly."""
+    from stipple_amd.pipeline import FramePipeline
+    w, h, n = 96, 64, 11
+    frames = _frames_420(n, w, h, 7)
+    f0 = oracle.yuv_to_rgbaf(frames[0], w, h, gpu.PIX_YUV420)[0]
+    tree = oracle.tree_build(f0, 12, False)
+    pal = gpu.Palette.from_tree(tree.data())
+    say("pipeline resident", batch, slots, stall)
+    pipe = FramePipeline(w, h, gpu.PIX_YUV420, gpu.PIX_YUV444, palette=pal, batch=batch,
+                         slots=slots, block=64)
+    count, got = _run(pipe, frames, stall)
+    pipe.close()
+    assert count == n and [i for i, _ in got] == list(range(n))
+    for i, out in got:
+        f = oracle.yuv_to_rgbaf(frames[i], w, h, gpu.PIX_YUV420)[0]
+        exp = oracle.walk_blocks_yuv444(tree, f, 64)[0]
+        assert np.array_equal(out, exp.reshape(-1)), i
+
+
+def test_pipeline_own_palettes_rgb8(gpu):
+    """RGB8 in and out, each frame's own palette built on the device: equal to
+    the frame dithered alone (its own build, its own block walk)."""
+    import torch
+    from stipple_amd.pipeline import FramePipeline
+    w, h, n = 80, 48, 9
+    rng = np.random.default_rng(3)
+    frames = [synth.sinusoid_image(w, h, rng, channels=3) for _ in range(n)]
+    say("pipeline own palettes")
+    pipe = FramePipeline(w, h, gpu.PIX_RGB8, gpu.PIX_RGB8, palette=None, colors=10, batch=4,
+                         slots=2, block=32)
+    count, got = _run(pipe, [f.reshape(-1) for f in frames], True)
+    pipe.close()
+    assert count == n and [i for i, _ in got] == list(range(n))
+    for i, out in got:
+        src = H.to_device(frames[i])
+        dst = torch.zeros(w * h * 3, dtype=torch.uint8, device="cuda")
+        p = gpu.Palette.build(src, w, h, gpu.PIX_RGB8, 10, False)
+        gpu.walk_blocks([gpu.image_spec(src, dst, w, h, gpu.PIX_RGB8, gpu.PIX_RGB8, p)], block=32)
+        torch.cuda.synchronize()
+        assert np.array_equal(out, dst.cpu().numpy()), i
+
+
+def test_pipeline_empty_and_exact_batches(gpu, oracle):
+    """No frames at all, and a count that fills the last batch exactly."""
+    from stipple_amd.pipeline import FramePipeline
+    w, h = 64, 32
+    frames = _frames_420(6, w, h, 11)
+    f0 = oracle.yuv_to_rgbaf(frames[0], w, h, gpu.PIX_YUV420)[0]
+    tree = oracle.tree_build(f0, 8, False)
+    pal = gpu.Palette.from_tree(tree.data())
+    say("pipeline empty / exact")
+    pipe = FramePipeline(w, h, gpu.PIX_YUV420, gpu.PIX_RGBA16, palette=pal, batch=3, slots=2)
+    assert _run(pipe, [], False) == (0, [])
+    count, got = _run(pipe, frames, False)
+    pipe.close()
+    assert count == 6 and [i for i, _ in got] == list(range(6))
+    for i, out in got:
+        f = oracle.yuv_to_rgbaf(frames[i], w, h, gpu.PIX_YUV420)[0]
+        exp = oracle.walk_blocks_u16(tree, f, 64)[0]
+        assert np.array_equal(out, exp.reshape(-1).view(np.uint8)), i
