@@ -322,6 +322,59 @@ def _video_pipeline(sa, np, frames, w, h, n=128, batch=8, slots=3):
             "note": "host-resident 4:2:0 in, host 4:4:4 out, PCIe both ways included"}
 
 
+def run_screens(args, sa, torch, synth, dev):
+    """The threshold-screen plugin dithers (bayer-ordered, void-and-cluster,
+    void-and-cluster-fast, clustered-dot) over 1920x1080 RGB8 plasma frames
+    with a resident 16-colour palette built on the device from frame 0."""
+    nf = args.frames or 32
+    params = synth.plasma_params(3)
+    frames = [synth.plasma_frame(t, params=params) for t in range(nf)]
+    h, w = frames[0].shape[:2]
+    srcs = [torch.from_numpy(f.reshape(-1).copy()).cuda() for f in frames]
+    dsts = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in frames]
+    pal = sa.Palette.build(srcs[0], w, h, sa.PIX_RGB8, 16, False)
+    specs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, pal)
+             for k in range(nf)]
+    stream = torch.cuda.current_stream()
+    names = ["bayer-ordered", "void-and-cluster", "void-and-cluster-fast", "clustered-dot"]
+    per = {}
+    for screen, name in enumerate(names):
+        for _ in range(args.warmup):
+            sa.dither_screen(specs, screen)
+        torch.cuda.synchronize()
+        ms = _ev_ms(lambda: [sa.dither_screen(specs, screen) for _ in range(args.steps)],
+                    stream) / args.steps
+        per[name] = {"ms": round(ms, 4), "Mpixel_s": round(nf * w * h / ms / 1e3, 1),
+                     "GB_s": round(6.0 * nf * w * h / ms / 1e6, 1)}
+    t = per["bayer-ordered"]["ms"] * 1e-3
+    px = w * h
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import Ref, Oracle
+        ent = pal.entries
+        f = synth.widen_q16(frames[0])
+        t1 = time.perf_counter()
+        if Ref.available():
+            Ref().dither("bayer-ordered", ent, f)
+        else:
+            Oracle().dither_screen(ent, f, 0)
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(px / tc / 1e6, 3), "unit": "Mpixel/s", "cores": 1,
+               "kind": "reference" if Ref.available() else "port",
+               "sample": "one 1920x1080 frame through the reference's bayer-ordered plugin"}
+    cl = per["clustered-dot"]
+    roof = {"bound": "hbm", "achieved": cl["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(cl["GB_s"] / HBM_PEAK_GBS, 6), "traffic": _traffic("screens_traffic"),
+            "kernel": "k_screen<true> (clustered-dot)",
+            "note": "algorithmic 6 B/px (RGB8 in + out); the OrderedPick screens are FP32-VALU "
+                    "bound (16 candidates x ~14 ops per pixel), see per_screen"}
+    return _line(args, "Mpixel/s per screen dither, 1920x1080 RGB8 frames",
+                 px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f32",
+                 {"workload": f"{nf} 1920x1080 RGB8 plasma frames, bayer-ordered with a resident "
+                              "16-colour device palette (value); every screen in per_screen",
+                  "frames": nf, "device": dev}, roof, cpu, {"per_screen": per})
+
+
 def run_approx4k(args, sa, torch, synth, dev):
     """3840x2160 RGB16 radial gradients + salt and pepper, 256 colours from
     frame 0 held on the device, Jacobi 8 sweeps, taps 96, fp32."""
@@ -437,7 +490,7 @@ def main():
                          "rest are classified (0: one walk launch after all palettes)")
     ap.add_argument("--video-format", choices=["rgb8", "yuv420"], default="rgb8",
                     help="video workload: RGB8 in and out, or planar 4:2:0 in and 4:4:4 out")
-    ap.add_argument("--workload", choices=["stills", "ramp", "video", "approx4k"],
+    ap.add_argument("--workload", choices=["stills", "ramp", "video", "approx4k", "screens"],
                     default="stills",
                     help="stills: 64 uneven RGBA8 stills, sequential engine (the headline); "
                          "ramp: one 640x480 image; video: 1080p frames, block engine; "
@@ -467,7 +520,8 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    other = {"ramp": run_ramp, "video": run_video, "approx4k": run_approx4k}.get(args.workload)
+    other = {"ramp": run_ramp, "video": run_video, "approx4k": run_approx4k,
+             "screens": run_screens}.get(args.workload)
     if other:
         print(json.dumps(other(args, sa, torch, synth, dev)), flush=True)
         return

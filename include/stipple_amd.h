@@ -235,6 +235,22 @@ int sa_seq_stats(uint64_t* out, char* err, size_t errlen);
 int sa_walk_blocks(const sa_image* frames, int n, int block, double diffusion, void* stream,
                    char* err, size_t errlen);
 
+/* The threshold-screen dithers of the plugin seam, one thread per pixel, over
+ * frames of one geometry (each with its own palette, any entry count):
+ *   SA_SCREEN_BAYER8     "bayer-ordered" (src/rd_dithers_ordered.cc:147-168)
+ *   SA_SCREEN_VOID64     "void-and-cluster" (:220-330, :447-482)
+ *   SA_SCREEN_VOID32     "void-and-cluster-fast" (:484-486)
+ *   SA_SCREEN_CLUSTERED8 "clustered-dot" (src/rd_dithers.cc:294-358)
+ * Each replaces that DitherAlgorithm's run (include/rd_plugin.h:132-136) on
+ * device frames: the chosen entry's r, g, b with the source's alpha
+ * (WriteChosen), bit-exact with the plugin.  `diffusion` is DitherJob's.
+ * dst may equal src when the formats match.  Refusals: a palette without
+ * entries ("<name>: no palette") and an empty frame ("<name>: empty frame"),
+ * as Prepare (src/rd_dither_common.h:108-120). */
+enum { SA_SCREEN_BAYER8 = 0, SA_SCREEN_VOID64 = 1, SA_SCREEN_VOID32 = 2, SA_SCREEN_CLUSTERED8 = 3 };
+int sa_dither_screen(const sa_image* frames, int n, int screen, double diffusion, void* stream,
+                     char* err, size_t errlen);
+
 /* The Jacobi approximation: `iterations` fixed-point sweeps of
  * t = pal[q] + F (x - pal[q]) along the block curve, F = (I-H)^-1 truncated
  * to `taps`+1 terms, then an owner-filtered scatter.  Replaces

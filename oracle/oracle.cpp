@@ -31,6 +31,9 @@
 //                       constant, and odd ones are the rounded bilinear mean,
 //                       checked in numpy; the result then feeds the pinned
 //                       4:4:4 path.
+//   * screen dithers -- bayer-ordered, void-and-cluster (64, 32), clustered-
+//                       dot: against oracle/_ref's own plugins run through
+//                       FindDither (tests/test_screens.py).
 //   * inverse filter -- triangular Toeplitz solve of (I-H) g = delta (numpy).
 //   * Jacobi         -- parity PARTIAL: the reference has no CPU Jacobi and
 //                       its CUDA engine cannot be built here, so no golden
@@ -978,6 +981,170 @@ extern "C" int orc_walk_blocks_yuv444(const OrcTree* t, const float* batch, int6
                     o + 2 * pp + pix);
     }
   }
+  return 0;
+}
+
+// ===========================================================================
+// Threshold-screen plugin dithers (src/rd_dithers_ordered.cc, src/rd_dithers.cc).
+// ===========================================================================
+namespace {
+
+// BuildVoidCluster, src/rd_dithers_ordered.cc:220-330.
+std::vector<uint16_t> vc_ranks(int n) {
+  const size_t np = (size_t)n * n;
+  const double k5[5] = {0.12012, 0.23387, 0.29202, 0.23387, 0.12012};
+  std::vector<double> on(np), tmp(np), filt(np);
+  uint64_t s = 0x9E3779B97F4A7C15ull;
+  for (size_t i = 0; i < np; ++i) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    on[i] = ((s >> 33) % 10 == 0) ? 1.0 : 0.0;
+  }
+  auto wrap = [n](int a) { return (a % n + n) % n; };
+  auto blur = [&]() {
+    for (int y = 0; y < n; ++y)
+      for (int x = 0; x < n; ++x) {
+        double acc = 0.0;
+        for (int k = 0; k < 5; ++k) acc += k5[k] * on[(size_t)y * n + wrap(x + k - 2)];
+        tmp[(size_t)y * n + x] = acc;
+      }
+    for (int y = 0; y < n; ++y)
+      for (int x = 0; x < n; ++x) {
+        double acc = 0.0;
+        for (int k = 0; k < 5; ++k) acc += k5[k] * tmp[(size_t)wrap(y + k - 2) * n + x];
+        filt[(size_t)y * n + x] = acc;
+      }
+  };
+  auto amax_on = [&]() {
+    size_t b = np;
+    double bv = 0.0;
+    for (size_t i = 0; i < np; ++i)
+      if (on[i] > 0.5 && (b == np || filt[i] > bv)) { bv = filt[i]; b = i; }
+    return b;
+  };
+  auto amin_off = [&]() {
+    size_t b = np;
+    double bv = 0.0;
+    for (size_t i = 0; i < np; ++i)
+      if (on[i] <= 0.5 && (b == np || filt[i] < bv)) { bv = filt[i]; b = i; }
+    return b;
+  };
+  for (size_t it = 0; it < 4 * np; ++it) {
+    blur();
+    const size_t t = amax_on();
+    if (t == np) break;
+    on[t] = 0.0;
+    blur();
+    const size_t v = amin_off();
+    if (v == np) break;
+    on[v] = 1.0;
+    if (t == v) break;
+  }
+  const std::vector<double> eq = on;
+  size_t dots = 0;
+  for (size_t i = 0; i < np; ++i) if (on[i] > 0.5) ++dots;
+  std::vector<uint16_t> rank(np, 0);
+  size_t r = dots;
+  while (r > 0) {
+    --r;
+    blur();
+    const size_t t = amax_on();
+    if (t == np) break;
+    on[t] = 0.0;
+    rank[t] = (uint16_t)(r + 1);
+  }
+  on = eq;
+  for (size_t q = dots; q < np; ++q) {
+    blur();
+    const size_t v = amin_off();
+    if (v == np) break;
+    on[v] = 1.0;
+    rank[v] = (uint16_t)(q + 1);
+  }
+  return rank;
+}
+
+// OrderedPick, src/rd_dithers_ordered.cc:79-145.
+int ordered_pick(const double* ent, int count, float r, float g, float b, float t) {
+  int first = 0, second = 0;
+  float d1 = -1.0f, d2 = -1.0f;
+  for (int i = 0; i < count; ++i) {
+    const float dr = r - (float)ent[4 * i], dg = g - (float)ent[4 * i + 1];
+    const float db = b - (float)ent[4 * i + 2];
+    const float d = dr * dr + dg * dg + db * db;
+    if (d1 < 0.0f || d < d1) {
+      d2 = d1; second = first; d1 = d; first = i;
+    } else if (d2 < 0.0f || d < d2) {
+      d2 = d; second = i;
+    }
+  }
+  if (d2 < 0.0f) return first;
+  if (d2 <= 0.0f) return second;
+  const float s1 = std::sqrt(d1), s2 = std::sqrt(d2);
+  return t < s1 / (s1 + s2) ? second : first;
+}
+
+const unsigned char kBayer[64] = {0,  32, 8,  40, 2,  34, 10, 42, 48, 16, 56, 24, 50, 18, 58, 26,
+                                  12, 44, 4,  36, 14, 46, 6,  38, 60, 28, 52, 20, 62, 30, 54, 22,
+                                  3,  35, 11, 43, 1,  33, 9,  41, 51, 19, 59, 27, 49, 17, 57, 25,
+                                  15, 47, 7,  39, 13, 45, 5,  37, 63, 31, 55, 23, 61, 29, 53, 21};
+const unsigned char kCell[64] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 0,
+                                 0, 0, 1, 1, 1, 1, 0, 0, 0, 1, 1, 1, 1, 1, 1, 0,
+                                 0, 1, 1, 1, 1, 1, 1, 0, 0, 0, 1, 1, 1, 1, 0, 0,
+                                 0, 0, 0, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+}  // namespace
+
+// The plugin's run over `frames` RgbaF frames in place: screen 0 bayer-ordered,
+// 1 void-and-cluster (64), 2 void-and-cluster-fast (32), 3 clustered-dot.
+// Returns 0, or -1 for no palette / an empty frame (Prepare).
+extern "C" int orc_dither_screen(const double* ent, int count, float* rgba, int64_t w, int64_t h,
+                                 int frames, int screen, double diffusion) {
+  if (count <= 0 || w <= 0 || h <= 0 || screen < 0 || screen > 3) return -1;
+  static std::mutex mu;
+  static std::vector<uint16_t> v64, v32;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    if (screen == 1 && v64.empty()) v64 = vc_ranks(64);
+    if (screen == 2 && v32.empty()) v32 = vc_ranks(32);
+  }
+  int lo_i = 0, hi_i = 0;
+  double lo = 0.0, hi = 0.0;
+  for (int i = 0; i < count; ++i) {  // rd_dithers.cc:317-328
+    const double l = 0.299 * ent[4 * i] + 0.587 * ent[4 * i + 1] + 0.114 * ent[4 * i + 2];
+    if (i == 0 || l < lo) { lo = l; lo_i = i; }
+    if (i == 0 || l > hi) { hi = l; hi_i = i; }
+  }
+  const double span = hi > lo ? hi - lo : 1.0;
+  const int64_t pp = w * h;
+  for (int f = 0; f < frames; ++f)
+    for (int64_t y = 0; y < h; ++y)
+      for (int64_t x = 0; x < w; ++x) {
+        float* px = rgba + 4 * ((size_t)f * pp + (size_t)(y * w + x));
+        int best;
+        if (screen == 3) {
+          const double lum = 0.299 * px[0] + 0.587 * px[1] + 0.114 * px[2];
+          const double t = (lum - lo) / span;
+          const double cell = (double)kCell[(y & 7) * 8 + (x & 7)];
+          double dark = cell / 32.0;
+          if (cell == 0) dark = 0.0;
+          dark = 0.5 + (dark - 0.5) * diffusion;
+          best = (1.0 - t) >= dark ? lo_i : hi_i;
+        } else {
+          float t;
+          if (screen == 0) {
+            t = ((float)kBayer[(y & 7) * 8 + (x & 7)] + 0.5f) / 64.0f;
+          } else {
+            const int n = screen == 1 ? 64 : 32;
+            const std::vector<uint16_t>& tab = screen == 1 ? v64 : v32;
+            t = (float)tab[(size_t)(y % n) * n + (size_t)(x % n)] / (float)((double)n * n);
+          }
+          const float sc = 0.5f + (t - 0.5f) * (float)diffusion;
+          best = ordered_pick(ent, count, px[0], px[1], px[2], sc);
+        }
+        px[0] = (float)ent[4 * best];
+        px[1] = (float)ent[4 * best + 1];
+        px[2] = (float)ent[4 * best + 2];
+      }
   return 0;
 }
 

@@ -99,6 +99,13 @@ int orc_yuv_to_rgbaf(const uint8_t* planes, int64_t w, int64_t h, int frames, in
 int orc_walk_blocks_yuv444(const OrcTree* t, const float* batch, int64_t w, int64_t h,
                            int frames, int block, double diffusion, uint8_t* out);
 
+/* The threshold-screen plugin dithers in place on RgbaF frames (screen 0
+ * bayer-ordered, 1 void-and-cluster, 2 void-and-cluster-fast, 3 clustered-dot;
+ * src/rd_dithers_ordered.cc, src/rd_dithers.cc:294-358); ent is count x 4
+ * doubles.  Returns 0 or -1. */
+int orc_dither_screen(const double* ent, int count, float* rgba, int64_t w, int64_t h,
+                      int frames, int screen, double diffusion);
+
 /* Inverse filter g[0..taps] (src/rd_approx_cuda.cu:73-88). */
 void orc_inverse_filter(int taps, double diffusion, double* g);
 /* The Jacobi engine restated on the CPU (src/rd_approx_cuda.cu:90-384):

@@ -102,6 +102,7 @@ class Oracle(_Lib):
         self._blocks16 = f("walk_blocks_u16", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p)
         self._yuv2f = f("yuv_to_rgbaf", _i32, _p, _i64, _i64, _i32, _i32, _p)
         self._blocksyuv = f("walk_blocks_yuv444", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p)
+        self._screen = f("dither_screen", _i32, _p, _i32, _p, _i64, _i64, _i32, _i32, _dbl)
         self._inv = f("inverse_filter", None, _i32, _dbl, _p)
         self._jac = f("jacobi", _i32, _p, _p, _i64, _i64, _i32, _i32, _i32, _i32, _dbl, _p)
         self._many = f("walk_seq_many", _i32, _p, _p, _p, _p, _i32, _dbl, _i32)
@@ -206,6 +207,18 @@ class Oracle(_Lib):
         if self._yuv2f(_ptr(a), w, h, frames, fmt, _ptr(out)) != 0:
             raise ValueError("refused")
         return out
+
+    def dither_screen(self, entries, frames, screen, diffusion=1.0):
+        """A threshold-screen plugin dither (screen 0..3) of (frames, h, w, 4)
+        float32 RgbaF with a (count, 4) float64 palette; returns the result."""
+        e = np.ascontiguousarray(entries, dtype=np.float64)
+        a = np.ascontiguousarray(frames, dtype=np.float32).copy()
+        if a.ndim == 3:
+            a = a[None]
+        n, h, w = a.shape[:3]
+        if self._screen(_ptr(e), len(e), _ptr(a), w, h, n, screen, diffusion) != 0:
+            raise ValueError("refused")
+        return a
 
     def walk_blocks_yuv444(self, tree, batch, block, diffusion=1.0):
         """The block walk with planar 4:4:4 output: (frames, 3, h, w) uint8."""
@@ -321,6 +334,7 @@ class Ref(_Lib):
         self._walk = f("walk_cpu", None, _p, _p, _i64, _i64, _dbl, _i32)
         self._blocks = f("blocks_cpu", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl)
         self._blocks16 = f("blocks_cpu_u16", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p)
+        self._dither = f("dither", _i32, C.c_char_p, _p, _i32, _p, _i64, _i64, _i32, _dbl)
         self._fused = f("blocks_cpu_fused", _i32, _p, _p, _i64, _i64, _i32, _i32, _dbl, _p, _i32,
                         _p, _i64, _i32, _i32)
         self._bci = f("build_curve_index", _i64, _i32, _i64, _i64, _p, _p)
@@ -395,6 +409,19 @@ class Ref(_Lib):
         hs = np.array([im.shape[0] for im in images], np.int64)
         self._many(tp, ip, _ptr(ws), _ptr(hs), n, diffusion, threads)
 
+
+    def dither(self, name, entries, frames, diffusion=1.0):
+        """The reference's registered plugin `name` run on (frames, h, w, 4)
+        float32 RgbaF with a (count, 4) float64 palette; returns the result."""
+        e = np.ascontiguousarray(entries, dtype=np.float64)
+        a = np.ascontiguousarray(frames, dtype=np.float32).copy()
+        if a.ndim == 3:
+            a = a[None]
+        n, h, w = a.shape[:3]
+        rc = self._dither(name.encode(), _ptr(e), len(e), _ptr(a), w, h, n, diffusion)
+        if rc != 0:
+            raise ValueError("refused" if rc == -1 else "no such plugin")
+        return a
 
     def walk_blocks_fused(self, tree, block, w, h, frames, batch=None, yuv_in=None,
                           yuv_out=True, diffusion=1.0):

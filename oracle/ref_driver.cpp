@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "rd_octree.h"
+#include "rd_plugin.h"
 #include "rd_riemersma.h"
 #include "rd_types.h"
 
@@ -133,6 +134,31 @@ int ref_blocks_cpu_fused(void* p, float* batch, int64_t w, int64_t h, int frames
                          (std::size_t)(w * h), out_yuv444 != 0, &err, gather_raw,
                          (std::size_t)gather_frame_bytes, gather_layout, gather_channels);
   return err.empty() ? 0 : -1;
+}
+
+// A registered plugin dither of the reference (FindDither(name)->run) in
+// place on RgbaF frames; 0, or -1 with the plugin's error.
+int ref_dither(const char* name, const double* ent, int count, float* rgba, int64_t w,
+               int64_t h, int frames, double diffusion) {
+  const rd::DitherAlgorithm* a = rd::FindDither(name);
+  if (!a) return -2;
+  static rd::Palette pal;  // 2 MiB: not on the stack
+  pal.count = count;
+  pal.associate_alpha = false;
+  for (int i = 0; i < count; ++i) {
+    pal.entries[i].r = ent[4 * i];
+    pal.entries[i].g = ent[4 * i + 1];
+    pal.entries[i].b = ent[4 * i + 2];
+    pal.entries[i].a = ent[4 * i + 3];
+  }
+  rd::DitherJob job;
+  job.palette = count > 0 ? &pal : nullptr;
+  job.width = (std::size_t)w;
+  job.height = (std::size_t)h;
+  job.frames = frames;
+  job.diffusion = diffusion;
+  const rd::DitherResult r = a->run(job, reinterpret_cast<rd::RgbaF*>(rgba));
+  return r.error.empty() ? 0 : -1;
 }
 
 int64_t ref_build_curve_index(int level, int64_t w, int64_t h, int32_t* out,

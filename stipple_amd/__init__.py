@@ -36,6 +36,7 @@ from ._lib import (  # noqa: F401
     WalkJob,
     available,
     curve_order,
+    dither_screen,
     device_name,
     image_spec,
     init,

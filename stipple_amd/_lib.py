@@ -89,6 +89,7 @@ def lib():
                 "sa_walk_seq_batch": (i32, [C.POINTER(_Image), i32, C.POINTER(SeqOptions), vp, cp, sz]),
                 "sa_walk_blocks": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_walk_approx": (i32, [C.POINTER(_Image), i32, i32, i32, i32, i32, dbl, vp, cp, sz]),
+                "sa_dither_screen": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_seq_stats": (i32, [vp, cp, sz]),
                 "sa_inverse_filter": (i32, [i32, dbl, vp, cp, sz]),
                 "sa_walk_job_create": (i32, [C.POINTER(SeqOptions), C.POINTER(vp), cp, sz]),
@@ -401,6 +402,22 @@ def walk_blocks(frames, block=64, diffusion=1.0, stream=None):
     e = _err()
     rc = lib().sa_walk_blocks(arr, len(frames), block, diffusion, _stream_ptr(stream), e, len(e))
     _check(rc, e)
+
+
+SCREEN_BAYER8, SCREEN_VOID64, SCREEN_VOID32, SCREEN_CLUSTERED8 = range(4)
+SCREENS = {"bayer-ordered": SCREEN_BAYER8, "void-and-cluster": SCREEN_VOID64,
+           "void-and-cluster-fast": SCREEN_VOID32, "clustered-dot": SCREEN_CLUSTERED8}
+
+
+def dither_screen(frames, screen, diffusion=1.0, stream=None):
+    """A threshold-screen dither of the plugin seam over device frames
+    (sa_dither_screen); `screen` is a SCREEN_* value or a plugin name."""
+    if isinstance(screen, str):
+        screen = SCREENS[screen]
+    arr = (_Image * max(len(frames), 1))(*frames)
+    e = _err()
+    _check(lib().sa_dither_screen(arr, len(frames), screen, diffusion, _stream_ptr(stream), e,
+                                  len(e)), e)
 
 
 def walk_approx(frames, iterations=20, taps=96, fp64=False, use_tree=True, diffusion=1.0,

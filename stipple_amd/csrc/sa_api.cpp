@@ -435,6 +435,11 @@ extern "C" int sa_palette_create(const double* entries, int count, int assoc,
   h.image = p->dev_image;
   if (e == hipSuccess) e = hipMalloc((void**)&p->dev_hdr, sizeof(PaletteDev));
   if (e == hipSuccess) e = hipMemcpy(p->dev_hdr, &h, sizeof(PaletteDev), hipMemcpyHostToDevice);
+  // The entries as given (doubles), for the screen dithers and sa_palette_entries.
+  if (e == hipSuccess) e = hipMalloc((void**)&p->dev_entries, sizeof(double) * 4 * (size_t)count);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->dev_entries, entries, sizeof(double) * 4 * (size_t)count,
+                  hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     sa_palette_destroy(p);
     return fail(err, errlen, SA_EDEVICE, "palette upload: %s", hipGetErrorString(e));
@@ -1355,6 +1360,82 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
                              p.max_count, p.src_class, p.threads, s);
   }));
   if (p.gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, p.n, p.gap, p.w, p.h, s));
+  HIP_TRY(hipFreeAsync(d_fr, s));
+  return SA_OK;
+}
+
+// ---- threshold-screen dithers ------------------------------------------------------
+namespace {
+// The four screens in one device buffer, built once: Bayer 8x8, void-and-
+// cluster 64x64 and 32x32 ranks, the clustered-dot cell.
+struct Screens {
+  std::once_flag once;
+  uint16_t* dev = nullptr;
+  hipError_t status = hipSuccess;
+};
+Screens g_screens;
+const int kScreenSide[4] = {8, 64, 32, 8};
+const size_t kScreenOff[4] = {0, 64, 64 + 4096, 64 + 4096 + 1024};
+
+hipError_t screens_ready() {
+  std::call_once(g_screens.once, [] {
+    std::vector<uint16_t> all(kScreenOff[3] + 64);
+    std::copy(sa::kBayer8, sa::kBayer8 + 64, all.begin());
+    const std::vector<uint16_t> v64 = sa::void_cluster_ranks(64), v32 = sa::void_cluster_ranks(32);
+    std::copy(v64.begin(), v64.end(), all.begin() + kScreenOff[1]);
+    std::copy(v32.begin(), v32.end(), all.begin() + kScreenOff[2]);
+    std::copy(sa::kDot8, sa::kDot8 + 64, all.begin() + kScreenOff[3]);
+    hipError_t e = hipMalloc((void**)&g_screens.dev, sizeof(uint16_t) * all.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(g_screens.dev, all.data(), sizeof(uint16_t) * all.size(), hipMemcpyHostToDevice);
+    g_screens.status = e;
+  });
+  return g_screens.status;
+}
+}  // namespace
+
+extern "C" int sa_dither_screen(const sa_image* frames, int n, int screen, double diffusion,
+                                void* stream, char* err, size_t errlen) {
+  static const char* kWho[4] = {"bayer-ordered", "void-and-cluster", "void-and-cluster-fast",
+                                "clustered-dot"};
+  if (screen < 0 || screen > 3) return fail(err, errlen, SA_EINVAL, "unknown screen %d", screen);
+  if (n < 1 || !frames) return fail(err, errlen, SA_EINVAL, "%s: no frames", kWho[screen]);
+  const int64_t w = frames[0].width, h = frames[0].height;
+  for (int i = 0; i < n; ++i) {
+    const sa_image& f = frames[i];
+    if (!f.palette || f.palette->count <= 0)
+      return fail(err, errlen, SA_EREFUSED, "%s: no palette", kWho[screen]);
+    if (f.width <= 0 || f.height <= 0)
+      return fail(err, errlen, SA_EREFUSED, "%s: empty frame", kWho[screen]);
+    if (!f.src || !f.dst) return fail(err, errlen, SA_EINVAL, "frame %d: null pointer", i);
+    if (f.width != w || f.height != h)
+      return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
+    if (!valid_blocks_in_fmt(f.src_fmt) || !valid_blocks_out_fmt(f.dst_fmt))
+      return fail(err, errlen, SA_EINVAL, "frame %d: bad pixel format", i);
+    if (f.dst == f.src && f.src_fmt != f.dst_fmt)
+      return fail(err, errlen, SA_EINVAL, "frame %d: in-place needs the same format", i);
+    if (f.src_fmt == sa::kYuv420 && ((w & 1) || (h & 1)))
+      return fail(err, errlen, SA_EUNSUPPORTED, "frame %d: 4:2:0 input needs even width and height",
+                  i);
+    if (!f.palette->dev_entries)
+      return fail(err, errlen, SA_EINVAL, "frame %d: palette without entries", i);
+  }
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  HIP_TRY(screens_ready());
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<sa::ScreenFrame> fr((size_t)n);
+  for (int i = 0; i < n; ++i)
+    fr[(size_t)i] = sa::ScreenFrame{frames[i].src, frames[i].dst, frames[i].palette->dev_hdr,
+                                    frames[i].palette->dev_entries, frames[i].src_fmt,
+                                    frames[i].dst_fmt};
+  sa::ScreenFrame* d_fr = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::ScreenFrame) * fr.size(), s));
+  HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::ScreenFrame) * fr.size(), s));
+  HIP_TRY(timed("screen_dither", s, [&] {
+    return sa::launch_screen(d_fr, n, w, h, g_screens.dev + kScreenOff[screen], kScreenSide[screen],
+                             screen, diffusion, s);
+  }));
   HIP_TRY(hipFreeAsync(d_fr, s));
   return SA_OK;
 }

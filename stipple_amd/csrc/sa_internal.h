@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "sa_palette.h"
 
 namespace sa {
@@ -184,6 +186,17 @@ struct BlockFrame {
   int32_t dst_fmt;
 };
 
+// One frame of the screen dithers: the palette's header and its entries as
+// doubles (clustered-dot's luma is taken from those).
+struct ScreenFrame {
+  const void* src;
+  void* dst;
+  const PaletteDev* pal;
+  const double* entries;
+  int32_t src_fmt;
+  int32_t dst_fmt;
+};
+
 struct Launch {
   hipStream_t stream;
   int cus;  // compute units on the device
@@ -226,6 +239,15 @@ hipError_t launch_blocks(const BlockFrame* frames, int nframes, const int32_t* c
                          int img_bytes, int count, int src_class, int threads, hipStream_t s);
 hipError_t launch_blocks_fill(const BlockFrame* frames, int nframes, int64_t gap_pixel,
                               int64_t w, int64_t h, hipStream_t s);
+
+// The screen dithers (screen 0 Bayer, 1 void-and-cluster 64, 2 void-and-cluster
+// 32, 3 clustered-dot); `table` is the screen's side x side thresholds.
+hipError_t launch_screen(const ScreenFrame* frames, int nframes, int64_t w, int64_t h,
+                         const uint16_t* table, int side, int screen, double diffusion,
+                         hipStream_t s);
+std::vector<uint16_t> void_cluster_ranks(int n);
+extern const uint16_t kBayer8[64];
+extern const uint16_t kDot8[64];
 
 size_t approx_lds_bytes(int img_bytes, int taps, int fp64, int assoc, int per);
 hipError_t launch_approx(const BlockFrame* frames, int nframes, const int32_t* curve, int64_t nb,

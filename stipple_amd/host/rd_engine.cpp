@@ -515,6 +515,75 @@ std::string RiemersmaBlocksCuda(const Palette& palette, const DitherParams& para
   return "";
 }
 
+// ---- threshold-screen plugins on the device ----------------------------------------------
+// The reference's four threshold-screen dithers (src/rd_dithers_ordered.cc:488-502,
+// src/rd_dithers.cc:370-371) registered under their own names: linking this
+// library gives a caller the same --dither names, run by sa_dither_screen.
+namespace {
+
+DitherResult screen_run(DitherJob& job, RgbaF* px, int screen, const char* who) {
+  DitherResult res;
+  if (!job.palette || job.palette->count <= 0) {  // Prepare, src/rd_dither_common.h:108-120
+    res.error = std::string(who) + ": no palette";
+    return res;
+  }
+  if (job.width == 0 || job.height == 0) {
+    res.error = std::string(who) + ": empty frame";
+    return res;
+  }
+  if (!ensure_device(&res.error)) return res;
+  // A screen reads only the entries: a one-leaf tree carries them.
+  int32_t child[32];
+  for (int k = 0; k < 32; ++k) child[k] = -1;
+  child[0] = 1;
+  const int32_t parent[2] = {-1, 0};
+  const uint32_t cn[2] = {0, 0};
+  const uint64_t nu[2] = {1, 1};
+  char err[kErr] = {0};
+  sa_palette pal = nullptr;
+  if (sa_palette_create(&job.palette->entries[0].r, job.palette->count, 0, child, parent, cn, nu, 2,
+                        &pal, err, sizeof(err)) != SA_OK) {
+    res.error = prefixed(who, err);
+    return res;
+  }
+  const int frames = std::max(job.frames, 1);
+  const size_t px_n = job.width * job.height, bytes = sizeof(RgbaF) * px_n * (size_t)frames;
+  DevBuf buf;
+  std::vector<sa_image> fr((size_t)frames);
+  hipError_t he = buf.reserve(bytes) ? hipMemcpy(buf.p, px, bytes, hipMemcpyHostToDevice)
+                                     : hipErrorOutOfMemory;
+  if (he == hipSuccess) {
+    for (int f = 0; f < frames; ++f) {
+      unsigned char* d = (unsigned char*)buf.p + sizeof(RgbaF) * px_n * (size_t)f;
+      fr[(size_t)f] = sa_image{d, d, (int64_t)job.width, (int64_t)job.height, SA_PIX_RGBAF32,
+                               SA_PIX_RGBAF32, pal};
+    }
+    if (sa_dither_screen(fr.data(), frames, screen, job.diffusion, nullptr, err, sizeof(err)) !=
+        SA_OK)
+      res.error = prefixed(who, err);
+    else
+      he = hipMemcpy(px, buf.p, bytes, hipMemcpyDeviceToHost);
+  }
+  if (he != hipSuccess && res.error.empty()) res.error = hip_fail(who, he);
+  sa_palette_destroy(pal);
+  return res;
+}
+
+DitherResult run_bayer(DitherJob& j, RgbaF* p) {
+  return screen_run(j, p, SA_SCREEN_BAYER8, "bayer-ordered");
+}
+DitherResult run_void64(DitherJob& j, RgbaF* p) {
+  return screen_run(j, p, SA_SCREEN_VOID64, "void-and-cluster");
+}
+DitherResult run_void32(DitherJob& j, RgbaF* p) {
+  return screen_run(j, p, SA_SCREEN_VOID32, "void-and-cluster-fast");
+}
+DitherResult run_cluster(DitherJob& j, RgbaF* p) {
+  return screen_run(j, p, SA_SCREEN_CLUSTERED8, "clustered-dot");
+}
+
+}  // namespace
+
 // ---- plugin registry ---------------------------------------------------------------------
 
 namespace {
@@ -548,6 +617,18 @@ const DitherAlgorithm* FindDither(const char* name) {
 }
 
 const std::vector<DitherAlgorithm>& RegisteredDithers() { return reg().algos; }
+
+namespace {
+const bool kScreensRegistered = [] {
+  RegisterDither({"bayer-ordered", "ordered 8x8 Bayer, no error diffusion", run_bayer});
+  RegisterDither({"void-and-cluster", "blue-noise thresholds, 64x64 matrix", run_void64});
+  RegisterDither({"void-and-cluster-fast", "blue-noise thresholds, 32x32 matrix (quicker)",
+                  run_void32});
+  RegisterDither({"clustered-dot", "8x8 halftone screen, no diffusion; shows the 8x8 period",
+                  run_cluster});
+  return true;
+}();
+}  // namespace
 
 std::string DitherListText() {
   Registry& r = reg();

@@ -300,10 +300,44 @@ int main() {
     const bool r1 = rd::RegisterDither(a);
     const bool r2 = rd::RegisterDither(a);
     const bool r3 = rd::RegisterDither(nul);
+    const std::string list = rd::DitherListText();
+    const std::string tail = "\n  probe           a test dither";
     check(r1 && !r2 && !r3 && rd::FindDither("probe") && !rd::FindDither("nope") &&
-              rd::RegisteredDithers().size() == 1 &&
-              rd::DitherListText() == "  probe           a test dither",
-          "RegisterDither / FindDither / DitherListText");
+              rd::RegisteredDithers().size() == 5 && list.size() > tail.size() &&
+              list.compare(list.size() - tail.size(), tail.size(), tail) == 0 &&
+              list.rfind("  bayer-ordered   ordered 8x8 Bayer", 0) == 0,
+          "RegisterDither / FindDither / DitherListText (the four screen dithers registered)");
+  }
+
+  // ---- the screen dithers behind DitherAlgorithm, on the device ---------------------------
+  {
+    const char* names[4] = {"bayer-ordered", "void-and-cluster", "void-and-cluster-fast",
+                            "clustered-dot"};
+    std::vector<double> ent(4 * (size_t)pal->count);
+    for (int i = 0; i < pal->count; ++i) {
+      ent[4 * i] = pal->entries[i].r;
+      ent[4 * i + 1] = pal->entries[i].g;
+      ent[4 * i + 2] = pal->entries[i].b;
+      ent[4 * i + 3] = pal->entries[i].a;
+    }
+    for (int k = 0; k < 4; ++k) {
+      const rd::DitherAlgorithm* a = rd::FindDither(names[k]);
+      std::vector<rd::RgbaF> got = img, want = img;
+      rd::DitherJob job;
+      job.palette = pal;
+      job.width = W;
+      job.height = H;
+      job.diffusion = 0.8;
+      const rd::DitherResult r = a ? a->run(job, got.data()) : rd::DitherResult{"missing"};
+      orc_dither_screen(ent.data(), pal->count, &want[0].r, W, H, 1, k, 0.8);
+      check(r.error.empty() && same_bytes(got.data(), want.data(), 16 * got.size()), names[k],
+            r.error);
+    }
+    rd::DitherJob empty;
+    empty.palette = pal;
+    check(rd::FindDither("clustered-dot")->run(empty, nullptr).error ==
+              "clustered-dot: empty frame",
+          "screen dither refuses an empty frame");
   }
 
   orc_tree_free(ot);

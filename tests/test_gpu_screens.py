@@ -1,0 +1,107 @@
+"""The threshold-screen dithers on the GPU (sa_dither_screen) against the
+oracle, which is pinned to the reference's own plugins: bayer-ordered,
+void-and-cluster, void-and-cluster-fast and clustered-dot, from float and
+8-bit sources into float (in place and not), 8-bit and index outputs.  Each
+case prints its name first."""
+import numpy as np
+import pytest
+
+from stipple_amd import synth
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _palette(gpu, oracle, rng, colors):
+    img = synth.sinusoid_image(64, 48, rng, channels=3)
+    tree, _ = H.oracle_tree(oracle, img, colors, assoc=False)
+    return tree.data().colormap.copy(), gpu.Palette.from_tree(tree.data())
+
+
+@pytest.mark.parametrize("screen", range(4))
+@pytest.mark.parametrize("w,h,colors,diffusion", [(1, 1, 4, 1.0), (67, 45, 16, 1.0),
+                                                   (256, 130, 200, 0.6), (300, 7, 2, 0.0)])
+def test_screen_float_frames(gpu, oracle, screen, w, h, colors, diffusion):
+    import torch
+    rng = np.random.default_rng(screen * 1000 + w + colors)
+    ent, pal = _palette(gpu, oracle, rng, colors)
+    px = rng.uniform(-300, 65900, (2, h, w, 4)).astype(np.float32)
+    want = oracle.dither_screen(ent, px, screen, diffusion)
+    say("screen", screen, w, h, colors, diffusion)
+    inplace = [H.to_device(px[k]) for k in range(2)]
+    outs = [torch.zeros(h * w * 16, dtype=torch.uint8, device="cuda") for _ in range(2)]
+    srcs = [H.to_device(px[k]) for k in range(2)]
+    specs = [gpu.image_spec(inplace[0], inplace[0], w, h, gpu.PIX_RGBAF32, gpu.PIX_RGBAF32, pal),
+             gpu.image_spec(srcs[1], outs[1], w, h, gpu.PIX_RGBAF32, gpu.PIX_RGBAF32, pal)]
+    gpu.dither_screen(specs, screen, diffusion)
+    torch.cuda.synchronize()
+    got0 = H.from_device(inplace[0], (h, w, 4), np.float32)
+    got1 = H.from_device(outs[1], (h, w, 4), np.float32)
+    assert np.array_equal(got0.view(np.uint32), want[0].view(np.uint32))
+    assert np.array_equal(got1.view(np.uint32), want[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("screen", range(4))
+def test_screen_8bit_frames(gpu, oracle, screen):
+    """RGB8 in, RGB8 / RGBA8 / INDEX16 / planar 4:4:4 out, one call."""
+    import torch
+    w, h = 150, 96
+    rng = np.random.default_rng(40 + screen)
+    ent, pal = _palette(gpu, oracle, rng, 24)
+    img = synth.sinusoid_image(w, h, rng, channels=3)
+    f = synth.widen_q16(img)
+    want = oracle.dither_screen(ent, f, screen, 1.0)[0]
+    say("screen 8-bit", screen)
+    src = H.to_device(img)
+    fmts = [("RGB8", 3), ("RGBA8", 4), ("INDEX16", 2), ("YUV444", 3)]
+    outs = {n: torch.zeros(h * w * b, dtype=torch.uint8, device="cuda") for n, b in fmts}
+    gpu.dither_screen([gpu.image_spec(src, outs[n], w, h, gpu.PIX_RGB8, getattr(gpu, "PIX_" + n), pal)
+                       for n, _ in fmts], screen)
+    torch.cuda.synchronize()
+    assert np.array_equal(outs["RGB8"].cpu().numpy().reshape(h, w, 3), H.scale_char_np(want[..., :3]))
+    assert np.array_equal(outs["RGBA8"].cpu().numpy().reshape(h, w, 4), H.scale_char_np(want))
+    idx = outs["INDEX16"].cpu().numpy().view(np.uint16).reshape(h, w)
+    assert np.array_equal(ent[idx][..., :3].astype(np.float32), want[..., :3])
+    r8 = want[..., :3].astype(np.int64) >> 8
+    yv = ((66 * r8[..., 0] + 129 * r8[..., 1] + 25 * r8[..., 2] + 128) >> 8) + 16
+    assert np.array_equal(outs["YUV444"].cpu().numpy()[:w * h].reshape(h, w), np.clip(yv, 0, 255))
+
+
+def test_screen_refusals(gpu, oracle):
+    import torch
+    rng = np.random.default_rng(2)
+    _, pal = _palette(gpu, oracle, rng, 8)
+    src = torch.zeros(16, dtype=torch.uint8, device="cuda")
+    say("screen refusals")
+    with pytest.raises(gpu.EngineError) as ei:
+        gpu.dither_screen([gpu.image_spec(src, src, 0, 1, gpu.PIX_RGBAF32, gpu.PIX_RGBAF32, pal)],
+                          "bayer-ordered")
+    assert "bayer-ordered: empty frame" in ei.value.message
+    with pytest.raises(gpu.EngineError):
+        gpu.dither_screen([gpu.image_spec(src, src, 1, 1, gpu.PIX_RGBAF32, gpu.PIX_RGB8, pal)], 0)
+
+
+def test_screen_rgb8_unaligned_frames(gpu, oracle):
+    """RGB8 frames packed back to back at odd byte offsets (17 x 13 x 3 = 663
+    bytes each), and a width that leaves a partial group of four pixels."""
+    import torch
+    w, h, n = 17, 13, 3
+    rng = np.random.default_rng(77)
+    ent, pal = _palette(gpu, oracle, rng, 12)
+    imgs = [synth.sinusoid_image(w, h, rng, channels=3) for _ in range(n)]
+    src = torch.from_numpy(np.concatenate([i.reshape(-1) for i in imgs])).cuda()
+    dst = torch.zeros(n * w * h * 3, dtype=torch.uint8, device="cuda")
+    fb = w * h * 3
+    say("screen rgb8 packed frames")
+    gpu.dither_screen([gpu.image_spec(src[k * fb:(k + 1) * fb], dst[k * fb:(k + 1) * fb], w, h,
+                                      gpu.PIX_RGB8, gpu.PIX_RGB8, pal) for k in range(n)], 1)
+    torch.cuda.synchronize()
+    out = dst.cpu().numpy().reshape(n, h, w, 3)
+    for k in range(n):
+        want = oracle.dither_screen(ent, synth.widen_q16(imgs[k]), 1, 1.0)[0]
+        assert np.array_equal(out[k], H.scale_char_np(want[..., :3])), k
+
