@@ -30,6 +30,7 @@ def test_screen_float_frames(gpu, oracle, screen, w, h, colors, diffusion):
     rng = np.random.default_rng(screen * 1000 + w + colors)
     ent, pal = _palette(gpu, oracle, rng, colors)
     px = rng.uniform(-300, 65900, (2, h, w, 4)).astype(np.float32)
+    px[:, 0, 0, 1] = np.nan  # a NaN channel takes the in-order scan
     want = oracle.dither_screen(ent, px, screen, diffusion)
     say("screen", screen, w, h, colors, diffusion)
     inplace = [H.to_device(px[k]) for k in range(2)]
