@@ -1430,11 +1430,13 @@ extern "C" int sa_dither_screen(const sa_image* frames, int n, int screen, doubl
                                     frames[i].palette->dev_entries, frames[i].src_fmt,
                                     frames[i].dst_fmt};
   sa::ScreenFrame* d_fr = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::ScreenFrame) * fr.size(), s));
+  const size_t fr_bytes = (sizeof(sa::ScreenFrame) * fr.size() + 255) & ~(size_t)255;
+  HIP_TRY(hipMallocAsync((void**)&d_fr, fr_bytes + sizeof(sa::ScreenRange) * fr.size(), s));
+  sa::ScreenRange* d_rng = (sa::ScreenRange*)((unsigned char*)d_fr + fr_bytes);
   HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::ScreenFrame) * fr.size(), s));
   HIP_TRY(timed("screen_dither", s, [&] {
     return sa::launch_screen(d_fr, n, w, h, g_screens.dev + kScreenOff[screen], kScreenSide[screen],
-                             screen, diffusion, s);
+                             screen, diffusion, d_rng, s);
   }));
   HIP_TRY(hipFreeAsync(d_fr, s));
   return SA_OK;

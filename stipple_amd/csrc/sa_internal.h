@@ -242,9 +242,15 @@ hipError_t launch_blocks_fill(const BlockFrame* frames, int nframes, int64_t gap
 
 // The screen dithers (screen 0 Bayer, 1 void-and-cluster 64, 2 void-and-cluster
 // 32, 3 clustered-dot); `table` is the screen's side x side thresholds.
+// clustered-dot's per-frame candidates: darkest luma, its span, the entries.
+struct ScreenRange {
+  double lo, span;
+  int32_t lo_i, hi_i;
+};
+// `ranges`: nframes entries of scratch, written and read by clustered-dot.
 hipError_t launch_screen(const ScreenFrame* frames, int nframes, int64_t w, int64_t h,
                          const uint16_t* table, int side, int screen, double diffusion,
-                         hipStream_t s);
+                         ScreenRange* ranges, hipStream_t s);
 std::vector<uint16_t> void_cluster_ranks(int n);
 extern const uint16_t kBayer8[64];
 extern const uint16_t kDot8[64];

@@ -8,7 +8,8 @@
 // pick, write the entry.  Each thread takes 4 pixels, so each palette entry
 // (the search image's `ent`, the reference's static_cast<float>(entry); a
 // wave-uniform load) serves four distance updates; clustered-dot's two
-// candidates come from one wavefront's pass over the entries' doubles.  Arithmetic is the reference's, one
+// candidates come from one wavefront per frame over the entries' doubles,
+// before the stream.  Arithmetic is the reference's, one
 // rounding per operation in its order (no contraction, correctly rounded
 // sqrt and division), so results are bit-exact.
 #include "sa_internal.h"
@@ -131,6 +132,30 @@ __device__ __forceinline__ void store_pick(const ScreenFrame& f, int64_t pix, in
   }
 }
 
+// clustered-dot's two candidates per frame, the palette's darkest and
+// lightest entries by luma, first wins (src/rd_dithers.cc:317-328): one
+// wavefront per frame, lexicographic on (luma, index) through the shuffles.
+__global__ __launch_bounds__(64) void k_luma_range(const ScreenFrame* __restrict__ frames,
+                                                   ScreenRange* __restrict__ out) {
+  const ScreenFrame f = frames[blockIdx.x];
+  const int count = f.pal->count;
+  double lv = 0.0, hv = 0.0;
+  int li = -1, hi = -1;
+  for (int i = threadIdx.x; i < count; i += 64) {
+    const double* c = f.entries + 4 * (int64_t)i;
+    const double l = luma(c[0], c[1], c[2]);
+    if (li < 0 || l < lv) { lv = l; li = i; }
+    if (hi < 0 || l > hv) { hv = l; hi = i; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double olv = __shfl_xor(lv, o), ohv = __shfl_xor(hv, o);
+    const int oli = __shfl_xor(li, o), ohi = __shfl_xor(hi, o);
+    if (oli >= 0 && (li < 0 || olv < lv || (olv == lv && oli < li))) { lv = olv; li = oli; }
+    if (ohi >= 0 && (hi < 0 || ohv > hv || (ohv == hv && ohi < hi))) { hv = ohv; hi = ohi; }
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = ScreenRange{lv, hv > lv ? hv - lv : 1.0, li, hi};
+}
+
 // Workgroup w serves kThreads * kPer pixels of frame w % nframes (chunk
 // w / nframes).  screen: 0 Bayer 8x8, 1/2 void-and-cluster 64/32 (ranks),
 // 3 clustered-dot cell (CLUSTER).
@@ -138,9 +163,8 @@ template <bool CLUSTER>
 __global__ __launch_bounds__(kThreads) void k_screen(const ScreenFrame* __restrict__ frames,
                                                      int nframes, int64_t w, int64_t h,
                                                      const uint16_t* __restrict__ table,
-                                                     int side, int screen, double diffusion) {
-  __shared__ double s_val[2];
-  __shared__ int s_at[2];
+                                                     int side, int screen, double diffusion,
+                                                     const ScreenRange* __restrict__ ranges) {
   const int fi = (int)(blockIdx.x % (unsigned)nframes);
   const int64_t chunk = blockIdx.x / (unsigned)nframes;
   const ScreenFrame f = frames[fi];
@@ -171,40 +195,14 @@ __global__ __launch_bounds__(kThreads) void k_screen(const ScreenFrame* __restri
       p[k] = base + k < pp ? load_any(f.src_fmt, f.src, base + k, w, h)
                            : make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  // The pixels' loads are in flight while clustered-dot ranks its palette.
   int lo_i = 0, hi_i = 0;
   double lo = 0.0, span = 1.0;
   if (CLUSTER) {
-    // The darkest and lightest entries by luma, first wins
-    // (src/rd_dithers.cc:317-328): one wavefront, lexicographic on
-    // (luma, index) through the shuffles.
-    if (threadIdx.x < 64) {
-      double lv = 0.0, hv = 0.0;
-      int li = -1, hi = -1;
-      for (int i = threadIdx.x; i < count; i += 64) {
-        const double* c = f.entries + 4 * (int64_t)i;
-        const double l = luma(c[0], c[1], c[2]);
-        if (li < 0 || l < lv) { lv = l; li = i; }
-        if (hi < 0 || l > hv) { hv = l; hi = i; }
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        const double olv = __shfl_xor(lv, o), ohv = __shfl_xor(hv, o);
-        const int oli = __shfl_xor(li, o), ohi = __shfl_xor(hi, o);
-        if (oli >= 0 && (li < 0 || olv < lv || (olv == lv && oli < li))) { lv = olv; li = oli; }
-        if (ohi >= 0 && (hi < 0 || ohv > hv || (ohv == hv && ohi < hi))) { hv = ohv; hi = ohi; }
-      }
-      if (threadIdx.x == 0) {
-        s_val[0] = lv;
-        s_val[1] = hv;
-        s_at[0] = li;
-        s_at[1] = hi;
-      }
-    }
-    __syncthreads();
-    lo_i = s_at[0];
-    hi_i = s_at[1];
-    lo = s_val[0];
-    span = s_val[1] > lo ? s_val[1] - lo : 1.0;
+    const ScreenRange r = ranges[fi];
+    lo = r.lo;
+    span = r.span;
+    lo_i = r.lo_i;
+    hi_i = r.hi_i;
   }
   // Coordinates in 32 bits when the frame allows (screen sides are powers of two).
   const bool narrow = pp < ((int64_t)1 << 31);
@@ -264,16 +262,18 @@ __global__ __launch_bounds__(kThreads) void k_screen(const ScreenFrame* __restri
 
 hipError_t launch_screen(const ScreenFrame* frames, int nframes, int64_t w, int64_t h,
                          const uint16_t* table, int side, int screen, double diffusion,
-                         hipStream_t s) {
+                         ScreenRange* ranges, hipStream_t s) {
   if (nframes <= 0 || w * h <= 0) return hipSuccess;
   const int64_t chunks = (w * h + kThreads * kPer - 1) / (kThreads * kPer);
   const dim3 grid((unsigned)(chunks * nframes));
-  if (screen == 3)
+  if (screen == 3) {
+    k_luma_range<<<dim3((unsigned)nframes), dim3(64), 0, s>>>(frames, ranges);
     k_screen<true><<<grid, dim3(kThreads), 0, s>>>(frames, nframes, w, h, table, side, screen,
-                                                   diffusion);
-  else
+                                                   diffusion, ranges);
+  } else {
     k_screen<false><<<grid, dim3(kThreads), 0, s>>>(frames, nframes, w, h, table, side, screen,
-                                                    diffusion);
+                                                    diffusion, ranges);
+  }
   return hipGetLastError();
 }
 
