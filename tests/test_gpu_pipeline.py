@@ -115,3 +115,29 @@ def test_pipeline_empty_and_exact_batches(gpu, oracle):
         f = oracle.yuv_to_rgbaf(frames[i], w, h, gpu.PIX_YUV420)[0]
         exp = oracle.walk_blocks_u16(tree, f, 64)[0]
         assert np.array_equal(out, exp.reshape(-1).view(np.uint8)), i
+
+
+def test_pipeline_jacobi_engine(gpu, oracle):
+    """The Jacobi engine through the pipeline (each slot's sweeps captured as a
+    chain): every frame equal to the engine on that frame alone."""
+    import torch
+    from stipple_amd.pipeline import FramePipeline
+    w, h, n = 72, 40, 7
+    rng = np.random.default_rng(5)
+    frames = [synth.sinusoid_image(w, h, rng, channels=3) for _ in range(n)]
+    tree, _ = H.oracle_tree(oracle, frames[0], 16, assoc=False)
+    pal = gpu.Palette.from_tree(tree.data())
+    say("pipeline jacobi")
+    pipe = FramePipeline(w, h, gpu.PIX_RGB8, gpu.PIX_RGB8, palette=pal, batch=3, slots=2,
+                         engine="approx", iterations=4, taps=24)
+    count, got = _run(pipe, [f.reshape(-1) for f in frames], True)
+    pipe.close()
+    assert count == n and [i for i, _ in got] == list(range(n))
+    for i, out in got:
+        src = H.to_device(frames[i])
+        dst = torch.zeros(w * h * 3, dtype=torch.uint8, device="cuda")
+        gpu.walk_approx([gpu.image_spec(src, dst, w, h, gpu.PIX_RGB8, gpu.PIX_RGB8, pal)],
+                        iterations=4, taps=24)
+        torch.cuda.synchronize()
+        assert np.array_equal(out, dst.cpu().numpy()), i
+
