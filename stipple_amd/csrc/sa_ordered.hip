@@ -156,10 +156,10 @@ __global__ __launch_bounds__(64) void k_luma_range(const ScreenFrame* __restrict
   if (threadIdx.x == 0) out[blockIdx.x] = ScreenRange{lv, hv > lv ? hv - lv : 1.0, li, hi};
 }
 
-// Workgroup w serves kThreads * kPer pixels of frame w % nframes (chunk
+// Workgroup w serves kThreads * P pixels of frame w % nframes (chunk
 // w / nframes).  screen: 0 Bayer 8x8, 1/2 void-and-cluster 64/32 (ranks),
 // 3 clustered-dot cell (CLUSTER).
-template <bool CLUSTER>
+template <bool CLUSTER, int P = CLUSTER ? 16 : kPer>
 __global__ __launch_bounds__(kThreads) void k_screen(const ScreenFrame* __restrict__ frames,
                                                      int nframes, int64_t w, int64_t h,
                                                      const uint16_t* __restrict__ table,
@@ -172,26 +172,29 @@ __global__ __launch_bounds__(kThreads) void k_screen(const ScreenFrame* __restri
   const int count = pd.count;
   const float4* ent = (const float4*)((const unsigned char*)pd.image + pd.off_ent);
   const int64_t pp = w * h;
-  const int64_t base = (chunk * kThreads + threadIdx.x) * kPer;
-  const bool whole = base + kPer <= pp;
-  int64_t pix[kPer];
-  float4 p[kPer];
-  float t[kPer];
-  int idx[kPer];
+  const int64_t base = (chunk * kThreads + threadIdx.x) * P;
+  const bool whole = base + P <= pp;
+  int64_t pix[P];
+  float4 p[P];
+  float t[P];
+  int idx[P];
   if (whole && f.src_fmt == kRgb8 && ((uintptr_t)f.src & 3) == 0) {
-    // 4 RGB8 pixels from 12 aligned bytes (base is a multiple of 4).
+    // RGB8 pixels four at a time from 12 aligned bytes (base is a multiple of 4).
     const uint32_t* q = (const uint32_t*)((const uint8_t*)f.src + 3 * base);
-    const uint32_t a = q[0], b = q[1], c = q[2];
-    const uint32_t by[12] = {a & 255u, (a >> 8) & 255u, (a >> 16) & 255u, a >> 24,
-                             b & 255u, (b >> 8) & 255u, (b >> 16) & 255u, b >> 24,
-                             c & 255u, (c >> 8) & 255u, (c >> 16) & 255u, c >> 24};
 #pragma unroll
-    for (int k = 0; k < kPer; ++k)
-      p[k] = make_float4((float)by[3 * k] * 257.0f, (float)by[3 * k + 1] * 257.0f,
-                         (float)by[3 * k + 2] * 257.0f, 65535.0f);
+    for (int g = 0; g < P / 4; ++g) {
+      const uint32_t a = q[3 * g], b = q[3 * g + 1], c = q[3 * g + 2];
+      const uint32_t by[12] = {a & 255u, (a >> 8) & 255u, (a >> 16) & 255u, a >> 24,
+                               b & 255u, (b >> 8) & 255u, (b >> 16) & 255u, b >> 24,
+                               c & 255u, (c >> 8) & 255u, (c >> 16) & 255u, c >> 24};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        p[4 * g + k] = make_float4((float)by[3 * k] * 257.0f, (float)by[3 * k + 1] * 257.0f,
+                                   (float)by[3 * k + 2] * 257.0f, 65535.0f);
+    }
   } else {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k)
+    for (int k = 0; k < P; ++k)
       p[k] = base + k < pp ? load_any(f.src_fmt, f.src, base + k, w, h)
                            : make_float4(0.f, 0.f, 0.f, 0.f);
   }
@@ -207,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_screen(const ScreenFrame* __restri
   // Coordinates in 32 bits when the frame allows (screen sides are powers of two).
   const bool narrow = pp < ((int64_t)1 << 31);
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
+  for (int k = 0; k < P; ++k) {
     pix[k] = base + k;
     int64_t y, x;
     if (narrow) {
@@ -237,23 +240,26 @@ __global__ __launch_bounds__(kThreads) void k_screen(const ScreenFrame* __restri
       t[k] = 0.5f + (tk - 0.5f) * (float)diffusion;
     }
   }
-  if (!CLUSTER) ordered_pick4(ent, count, p, t, idx);
+  if constexpr (!CLUSTER) ordered_pick4(ent, count, p, t, idx);
   if (whole && f.dst_fmt == kRgb8 && ((uintptr_t)f.dst & 3) == 0) {
-    uint32_t by[12];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const float4 e = ent[idx[k]];
-      by[3 * k] = scale_char(e.x);
-      by[3 * k + 1] = scale_char(e.y);
-      by[3 * k + 2] = scale_char(e.z);
-    }
     uint32_t* d = (uint32_t*)((uint8_t*)f.dst + 3 * base);
-    d[0] = by[0] | (by[1] << 8) | (by[2] << 16) | (by[3] << 24);
-    d[1] = by[4] | (by[5] << 8) | (by[6] << 16) | (by[7] << 24);
-    d[2] = by[8] | (by[9] << 8) | (by[10] << 16) | (by[11] << 24);
+#pragma unroll
+    for (int g = 0; g < P / 4; ++g) {
+      uint32_t by[12];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float4 e = ent[idx[4 * g + k]];
+        by[3 * k] = scale_char(e.x);
+        by[3 * k + 1] = scale_char(e.y);
+        by[3 * k + 2] = scale_char(e.z);
+      }
+      d[3 * g] = by[0] | (by[1] << 8) | (by[2] << 16) | (by[3] << 24);
+      d[3 * g + 1] = by[4] | (by[5] << 8) | (by[6] << 16) | (by[7] << 24);
+      d[3 * g + 2] = by[8] | (by[9] << 8) | (by[10] << 16) | (by[11] << 24);
+    }
   } else {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k)
+    for (int k = 0; k < P; ++k)
       if (pix[k] < pp) store_pick(f, pix[k], pp, idx[k], ent[idx[k]], p[k].w);
   }
 }
@@ -264,7 +270,8 @@ hipError_t launch_screen(const ScreenFrame* frames, int nframes, int64_t w, int6
                          const uint16_t* table, int side, int screen, double diffusion,
                          ScreenRange* ranges, hipStream_t s) {
   if (nframes <= 0 || w * h <= 0) return hipSuccess;
-  const int64_t chunks = (w * h + kThreads * kPer - 1) / (kThreads * kPer);
+  const int64_t per = screen == 3 ? 16 : kPer;  // pixels per thread of the kernel launched
+  const int64_t chunks = (w * h + kThreads * per - 1) / (kThreads * per);
   const dim3 grid((unsigned)(chunks * nframes));
   if (screen == 3) {
     k_luma_range<<<dim3((unsigned)nframes), dim3(64), 0, s>>>(frames, ranges);
