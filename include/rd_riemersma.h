@@ -4,8 +4,9 @@
 // include/rd_riemersma.h (line numbers below are that file's):
 //   * engines return std::string, empty on success and never empty on
 //     failure (:214-217; src/rd_cuda_stub.cpp:60-74);
-//   * pixels are caller-owned host RgbaF Q16, `frames` consecutive images of
-//     one geometry, dithered in place (:204-217, :224-268);
+//   * pixels are caller-owned host RgbaF Q16, dithered in place: one frame
+//     for RiemersmaWalkCuda and RiemersmaApproxCuda, `frames` consecutive
+//     frames of one geometry for RiemersmaBlocksCuda (:204-217, :224-268);
 //   * calls are synchronous on return (src/rd_riemersma_cuda.cu:742-746);
 //   * refusals: block < 16, frames < 1, palette > 256 for blocks and approx,
 //     taps outside 1..4096 (:296-307, :450-464).
@@ -77,9 +78,11 @@ const std::vector<int>* BlockCurveOwner(std::size_t width, std::size_t height);
 
 enum class Engine { kCpu, kCuda, kApprox, kBlocks, kOpenCL };
 
-// :214-217.  The exact sequential walk of `frames` images of one geometry,
-// each with its own error queue and memo; bit-exact per frame with
-// RiemersmaWalkCpu.  Frames walk concurrently, one wavefront each.
+// :214-217.  The exact sequential walk, bit-exact with RiemersmaWalkCpu.
+// `pixels` holds one width*height frame, dithered in place.  As in the
+// reference (src/rd_riemersma_cuda.cu:688, :747), `frames` copies of that one
+// frame would be walked and one frame is returned; the copies are identical,
+// so one walk is done.  Distinct images go through RiemersmaWalkBatch.
 std::string RiemersmaWalkCuda(const Palette& palette, const DitherParams& params,
                               const ColorTree& tree, std::size_t width, std::size_t height,
                               RgbaF* pixels, int frames, std::string* device_name);

@@ -35,17 +35,25 @@
 //                       dot: against oracle/_ref's own plugins run through
 //                       FindDither (tests/test_screens.py).
 //   * inverse filter -- triangular Toeplitz solve of (I-H) g = delta (numpy).
-//   * Jacobi         -- parity PARTIAL: the reference has no CPU Jacobi and
-//                       its CUDA engine cannot be built here, so no golden
-//                       vector exists.  Pinned: the inverse filter (above); the
-//                       lookup (shared with the octree); zero diffusion equals
-//                       the block walk at diffusion 0 (InitIndex, lookup and
-//                       owner scatter); one sweep over a flat image from a
-//                       uniform index equals the closed form c + r * S_i with
-//                       S_i the partial sums of the filter (the tap sum, its
-//                       sign, indices and range).  The fp32/fp64 rounding of
-//                       the tap sum follows src/rd_approx_cuda.cu:90-228 and is
-//                       pinned only by that restatement.
+//   * Jacobi         -- the reference has no CPU Jacobi and its CUDA engine
+//                       cannot be built here, so no golden vector exists; it
+//                       is pinned by what the mathematics fixes.  One sweep
+//                       from a given q (orc_jacobi_sweep) equals the exact
+//                       closed form over piecewise-constant residuals --
+//                       prefix sums of g over each segment's overlap with the
+//                       tap range, in rational arithmetic -- within the
+//                       standard rounding bound of the sweep's float type,
+//                       and picks the nearest entry of the exact target
+//                       wherever the margin clears that bound; an in-place
+//                       update, a dropped tap, a flipped sign and a late
+//                       residual planted here each fail it.  iterations = k
+//                       equals k composed sweeps from InitIndex (the
+//                       ping-pong), and the scatter writes pal[q] at each
+//                       pixel's owning position.  Zero diffusion equals the
+//                       block walk at diffusion 0.  What stays fixed only by
+//                       this restatement is the rounding order inside the
+//                       bound (d ascending, one rounding per multiply and per
+//                       add, src/rd_approx_cuda.cu:209-216).
 #include "oracle.h"
 
 #include <algorithm>
@@ -1207,6 +1215,100 @@ int nearest_linear(const OrcTree* t, const D4& tg) {
   return best;
 }
 
+// One sweep of JacobiStepKernel (src/rd_approx_cuda.cu:156-228) over every
+// position: t = pal[q_in[i]] + sum_{d=0..min(taps,i)} g[d] (x[i-d] - pal[q_in[i-d]]),
+// accumulated in T per channel with d ascending, widened to double, clamped
+// and associated (rd_cuda_common.cuh:57-72), then looked up.  Positions are
+// independent given q_in, so they are split over threads; the result does
+// not depend on the split.  t_out (n x 4 doubles, or null) receives the
+// unclamped target before association.
+//
+// `mutant` plants a known mistake for the tests that show the closed forms
+// catch it (0 = the restatement): 1 updates q in place (Gauss-Seidel),
+// 2 drops the last tap, 3 flips the residual's sign, 4 takes the residual
+// one position late.
+template <typename T>
+void sweep_range(const OrcTree* t, const T* cx, const T* g, int taps, int use_tree,
+                 const uint8_t* q_in, uint8_t* q_out, double* t_out, int64_t lo, int64_t hi,
+                 int mutant) {
+  const bool assoc = t->assoc;
+  const uint8_t* qr = mutant == 1 ? q_out : q_in;
+  for (int64_t i = lo; i < hi; ++i) {
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    int64_t dmax = taps < i ? taps : i;
+    if (mutant == 2 && dmax == taps) dmax = taps - 1;
+    for (int64_t d = 0; d <= dmax; ++d) {
+      int64_t j = i - d;
+      if (mutant == 4 && j > 0) j -= 1;
+      const T* xs = &cx[4 * (size_t)j];
+      const uint8_t qj = d == 0 ? q_in[(size_t)i] : qr[(size_t)j];
+      const double* p = &t->cmap[4 * (size_t)qj];
+      const T gk = g[(size_t)d];
+      for (int c = 0; c < 4; ++c) {
+        const T r = mutant == 3 ? ((T)p[c] - xs[c]) : (xs[c] - (T)p[c]);
+        acc[c] += gk * r;
+      }
+    }
+    const double* p0 = &t->cmap[4 * (size_t)q_in[(size_t)i]];
+    const double vr = p0[0] + (double)acc[0];
+    const double vg = p0[1] + (double)acc[1];
+    const double vb = p0[2] + (double)acc[2];
+    const double va = assoc ? p0[3] + (double)acc[3] : kRange;
+    if (t_out) {
+      t_out[4 * (size_t)i + 0] = vr;
+      t_out[4 * (size_t)i + 1] = vg;
+      t_out[4 * (size_t)i + 2] = vb;
+      t_out[4 * (size_t)i + 3] = va;
+    }
+    const D4 tg = assoc_from_double(assoc, vr, vg, vb, va);
+    q_out[(size_t)i] = (uint8_t)(use_tree ? t->lookup(tg) : nearest_linear(t, tg));
+  }
+}
+
+template <typename T>
+void sweep(const OrcTree* t, const T* cx, const T* g, int taps, int use_tree, const uint8_t* q_in,
+           uint8_t* q_out, double* t_out, int64_t n, int threads, int mutant) {
+  if (mutant == 1 || threads <= 1 || n < 65536) {
+    sweep_range<T>(t, cx, g, taps, use_tree, q_in, q_out, t_out, 0, n, mutant);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const int64_t per = (n + threads - 1) / threads;
+  for (int k = 0; k < threads; ++k) {
+    const int64_t lo = k * per, hi = std::min(n, lo + per);
+    if (lo >= hi) break;
+    pool.emplace_back(sweep_range<T>, t, cx, g, taps, use_tree, q_in, q_out, t_out, lo, hi,
+                      mutant);
+  }
+  for (auto& th : pool) th.join();
+}
+
+// InitIndexKernel (src/rd_approx_cuda.cu:127-148): the nearest entry of the
+// raw pixel, narrowed to float and associated as the pixel path does.
+template <typename T>
+void init_index(const OrcTree* t, const T* cx, int use_tree, uint8_t* q, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    float v[4];
+    for (int c = 0; c < 4; ++c) v[c] = (float)cx[4 * (size_t)i + c];
+    const D4 tg = assoc_px(t->assoc, v);
+    q[(size_t)i] = (uint8_t)(use_tree ? t->lookup(tg) : nearest_linear(t, tg));
+  }
+}
+
+template <typename T>
+std::vector<T> filter_as(int taps, double diffusion) {
+  std::vector<double> gd((size_t)taps + 1);
+  orc_inverse_filter(taps, diffusion, gd.data());
+  std::vector<T> g((size_t)taps + 1);
+  for (size_t i = 0; i < g.size(); ++i) g[i] = (T)gd[i];
+  return g;
+}
+
+int jacobi_threads() {
+  const unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hc, 16u));
+}
+
 template <typename T>
 int jacobi_impl(const OrcTree* t, float* rgba, int64_t w, int64_t h, int iterations,
                 int taps, int use_tree, double diffusion, uint8_t* q_out) {
@@ -1215,38 +1317,15 @@ int jacobi_impl(const OrcTree* t, float* rgba, int64_t w, int64_t h, int iterati
   const bool assoc = t->assoc;
   std::vector<int32_t> curve((size_t)(w * h + 1)), owner((size_t)(w * h));
   const int64_t n = orc_block_order(w, h, curve.data(), owner.data());
-  std::vector<double> gd((size_t)taps + 1);
-  orc_inverse_filter(taps, diffusion, gd.data());
-  std::vector<T> g((size_t)taps + 1);
-  for (size_t i = 0; i < g.size(); ++i) g[i] = (T)gd[i];
+  const std::vector<T> g = filter_as<T>(taps, diffusion);
   std::vector<T> cx((size_t)n * 4);
   for (int64_t i = 0; i < n; ++i)  // GatherKernelT
     for (int c = 0; c < 4; ++c) cx[4 * (size_t)i + c] = (T)rgba[4 * (size_t)curve[(size_t)i] + c];
   std::vector<uint8_t> q0((size_t)n), q1((size_t)n);
-  for (int64_t i = 0; i < n; ++i) {  // InitIndexKernel
-    float v[4];
-    for (int c = 0; c < 4; ++c) v[c] = (float)cx[4 * (size_t)i + c];
-    const D4 tg = assoc_px(assoc, v);
-    q0[(size_t)i] = (uint8_t)(use_tree ? t->lookup(tg) : nearest_linear(t, tg));
-  }
-  for (int it = 0; it < iterations; ++it) {  // JacobiStepKernel
-    for (int64_t i = 0; i < n; ++i) {
-      T acc[4] = {T(0), T(0), T(0), T(0)};
-      const int64_t dmax = taps < i ? taps : i;
-      for (int64_t d = 0; d <= dmax; ++d) {
-        const T* xs = &cx[4 * (size_t)(i - d)];
-        const double* p = &t->cmap[4 * (size_t)q0[(size_t)(i - d)]];
-        const T gk = g[(size_t)d];
-        for (int c = 0; c < 4; ++c) acc[c] += gk * (xs[c] - (T)p[c]);
-      }
-      const double* p0 = &t->cmap[4 * (size_t)q0[(size_t)i]];
-      const double vr = p0[0] + (double)acc[0];
-      const double vg = p0[1] + (double)acc[1];
-      const double vb = p0[2] + (double)acc[2];
-      const double va = assoc ? p0[3] + (double)acc[3] : kRange;
-      const D4 tg = assoc_from_double(assoc, vr, vg, vb, va);
-      q1[(size_t)i] = (uint8_t)(use_tree ? t->lookup(tg) : nearest_linear(t, tg));
-    }
+  init_index<T>(t, cx.data(), use_tree, q0.data(), n);
+  const int threads = jacobi_threads();
+  for (int it = 0; it < iterations; ++it) {  // the ping-pong of src/rd_approx_cuda.cu:356-368
+    sweep<T>(t, cx.data(), g.data(), taps, use_tree, q0.data(), q1.data(), nullptr, n, threads, 0);
     std::swap(q0, q1);
   }
   for (int64_t i = 0; i < n; ++i) {  // ScatterKernel (owner-filtered)
@@ -1259,6 +1338,7 @@ int jacobi_impl(const OrcTree* t, float* rgba, int64_t w, int64_t h, int iterati
     rgba[4 * pix + 3] = assoc ? (float)p[3] : (float)65535.0;
   }
   if (q_out) std::memcpy(q_out, q0.data(), (size_t)n);
+  (void)assoc;
   return 0;
 }
 
@@ -1269,4 +1349,33 @@ extern "C" int orc_jacobi(const OrcTree* t, float* rgba, int64_t w, int64_t h,
                           double diffusion, uint8_t* q_out) {
   if (fp64) return jacobi_impl<double>(t, rgba, w, h, iterations, taps, use_tree, diffusion, q_out);
   return jacobi_impl<float>(t, rgba, w, h, iterations, taps, use_tree, diffusion, q_out);
+}
+
+extern "C" int orc_jacobi_init(const OrcTree* t, const void* cx, int64_t n, int fp64,
+                               int use_tree, uint8_t* q) {
+  if (t->count < 1 || t->count > 256 || n < 0) return -1;
+  if (fp64)
+    init_index<double>(t, (const double*)cx, use_tree, q, n);
+  else
+    init_index<float>(t, (const float*)cx, use_tree, q, n);
+  return 0;
+}
+
+extern "C" int orc_jacobi_sweep(const OrcTree* t, const void* cx, int64_t n, const uint8_t* q_in,
+                                uint8_t* q_out, double* t_out, int taps, int fp64, int use_tree,
+                                double diffusion, int mutant) {
+  if (t->count < 1 || t->count > 256 || n < 0) return -1;
+  if (taps < 1 || taps > 4096) return -1;
+  if (mutant == 1) std::memcpy(q_out, q_in, (size_t)n);
+  const int threads = jacobi_threads();
+  if (fp64) {
+    const std::vector<double> g = filter_as<double>(taps, diffusion);
+    sweep<double>(t, (const double*)cx, g.data(), taps, use_tree, q_in, q_out, t_out, n, threads,
+                  mutant);
+  } else {
+    const std::vector<float> g = filter_as<float>(taps, diffusion);
+    sweep<float>(t, (const float*)cx, g.data(), taps, use_tree, q_in, q_out, t_out, n, threads,
+                 mutant);
+  }
+  return 0;
 }

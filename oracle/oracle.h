@@ -116,6 +116,18 @@ void orc_inverse_filter(int taps, double diffusion, double* g);
 int orc_jacobi(const OrcTree* t, float* rgba, int64_t w, int64_t h,
                int iterations, int taps, int fp64, int use_tree,
                double diffusion, uint8_t* q_out);
+/* InitIndexKernel alone (src/rd_approx_cuda.cu:127-148): q[i] = the entry
+ * nearest the curve-order input cx[i] (n x 4 floats, or doubles with fp64). */
+int orc_jacobi_init(const OrcTree* t, const void* cx, int64_t n, int fp64, int use_tree,
+                    uint8_t* q);
+/* One JacobiStepKernel sweep (src/rd_approx_cuda.cu:156-228) from a given
+ * q_in into q_out (distinct arrays: the reference ping-pongs, :356-368).
+ * t_out (n x 4 doubles, may be null) receives each target before clamping.
+ * mutant != 0 plants a known mistake for the tests (1 in-place update, 2 last
+ * tap dropped, 3 residual sign flipped, 4 residual one position late). */
+int orc_jacobi_sweep(const OrcTree* t, const void* cx, int64_t n, const uint8_t* q_in,
+                     uint8_t* q_out, double* t_out, int taps, int fp64, int use_tree,
+                     double diffusion, int mutant);
 
 /* Thread-pool CPU baseline: walks `n` independent images (each with its own
  * tree) with `threads` std::threads, largest first.  Returns 0. */

@@ -105,6 +105,9 @@ class Oracle(_Lib):
         self._screen = f("dither_screen", _i32, _p, _i32, _p, _i64, _i64, _i32, _i32, _dbl)
         self._inv = f("inverse_filter", None, _i32, _dbl, _p)
         self._jac = f("jacobi", _i32, _p, _p, _i64, _i64, _i32, _i32, _i32, _i32, _dbl, _p)
+        self._jinit = f("jacobi_init", _i32, _p, _p, _i64, _i32, _i32, _p)
+        self._jsweep = f("jacobi_sweep", _i32, _p, _p, _i64, _p, _p, _p, _i32, _i32, _i32, _dbl,
+                         _i32)
         self._many = f("walk_seq_many", _i32, _p, _p, _p, _p, _i32, _dbl, _i32)
         self._misses = f("walk_seq_misses", _i64, _p, _p, _i64, _i64, _dbl)
 
@@ -247,6 +250,39 @@ class Oracle(_Lib):
         if rc != 0:
             raise ValueError("refused")
         return (a, q[:n]) if want_q else a
+
+    def jacobi_gather(self, img, fp64=False):
+        """GatherKernelT (src/rd_approx_cuda.cu:110-121): the pixels in block
+        curve order, n x 4, as float or double."""
+        a = _rgba_f32(img)
+        h, w = a.shape[:2]
+        order = self.block_order(w, h)[0]
+        return np.ascontiguousarray(a.reshape(-1, 4)[order].astype(np.float64 if fp64 else np.float32))
+
+    def jacobi_init(self, tree, cx, use_tree=True):
+        cx = np.ascontiguousarray(cx)
+        q = np.zeros(max(len(cx), 1), np.uint8)
+        if self._jinit(tree.h, _ptr(cx), len(cx), int(cx.dtype == np.float64), int(use_tree),
+                       _ptr(q)) != 0:
+            raise ValueError("refused")
+        return q[:len(cx)]
+
+    def jacobi_sweep(self, tree, cx, q_in, taps=96, use_tree=True, diffusion=1.0, want_t=False,
+                     mutant=0):
+        """One sweep from q_in; returns q_out (and the unclamped targets, n x 4
+        doubles, with want_t)."""
+        cx = np.ascontiguousarray(cx)
+        n = len(cx)
+        q_in = np.ascontiguousarray(q_in, np.uint8)
+        assert len(q_in) == n
+        q = np.zeros(max(n, 1), np.uint8)
+        t = np.zeros((max(n, 1), 4), np.float64) if want_t else None
+        rc = self._jsweep(tree.h, _ptr(cx), n, _ptr(q_in), _ptr(q),
+                          _ptr(t) if want_t else None, taps, int(cx.dtype == np.float64),
+                          int(use_tree), diffusion, mutant)
+        if rc != 0:
+            raise ValueError("refused")
+        return (q[:n], t[:n]) if want_t else q[:n]
 
     def walk_seq_many(self, trees, images, threads=1, diffusion=1.0):
         """Dithers every image in place (CPU baseline)."""
@@ -400,6 +436,39 @@ class Ref(_Lib):
         if self._blocks16(tree.h, _ptr(a), w, h, frames, block, diffusion, _ptr(out)) != 0:
             raise ValueError("refused")
         return out
+
+    def jacobi_gather(self, img, fp64=False):
+        """GatherKernelT (src/rd_approx_cuda.cu:110-121): the pixels in block
+        curve order, n x 4, as float or double."""
+        a = _rgba_f32(img)
+        h, w = a.shape[:2]
+        order = self.block_order(w, h)[0]
+        return np.ascontiguousarray(a.reshape(-1, 4)[order].astype(np.float64 if fp64 else np.float32))
+
+    def jacobi_init(self, tree, cx, use_tree=True):
+        cx = np.ascontiguousarray(cx)
+        q = np.zeros(max(len(cx), 1), np.uint8)
+        if self._jinit(tree.h, _ptr(cx), len(cx), int(cx.dtype == np.float64), int(use_tree),
+                       _ptr(q)) != 0:
+            raise ValueError("refused")
+        return q[:len(cx)]
+
+    def jacobi_sweep(self, tree, cx, q_in, taps=96, use_tree=True, diffusion=1.0, want_t=False,
+                     mutant=0):
+        """One sweep from q_in; returns q_out (and the unclamped targets, n x 4
+        doubles, with want_t)."""
+        cx = np.ascontiguousarray(cx)
+        n = len(cx)
+        q_in = np.ascontiguousarray(q_in, np.uint8)
+        assert len(q_in) == n
+        q = np.zeros(max(n, 1), np.uint8)
+        t = np.zeros((max(n, 1), 4), np.float64) if want_t else None
+        rc = self._jsweep(tree.h, _ptr(cx), n, _ptr(q_in), _ptr(q),
+                          _ptr(t) if want_t else None, taps, int(cx.dtype == np.float64),
+                          int(use_tree), diffusion, mutant)
+        if rc != 0:
+            raise ValueError("refused")
+        return (q[:n], t[:n]) if want_t else q[:n]
 
     def walk_seq_many(self, trees, images, threads=1, diffusion=1.0):
         n = len(images)

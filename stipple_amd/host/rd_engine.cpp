@@ -333,20 +333,22 @@ std::string RiemersmaWalkCuda(const Palette& palette, const DitherParams& params
   PaletteRef pr;
   std::string e = palette_for(palette, tree, &pr);
   if (!e.empty()) return e;
-  const size_t bytes = sizeof(RgbaF) * px * frames;
+  // `pixels` holds ONE frame: the reference uploads frame 0 as the shared
+  // source of its `frames` walks and downloads one frame
+  // (src/rd_riemersma_cuda.cu:688, :747).  Those walks read the same source
+  // with the same palette and state, so they are identical, and one walk
+  // yields the frame it returns.  Distinct images walk concurrently through
+  // RiemersmaWalkBatch.
+  const size_t bytes = sizeof(RgbaF) * px;
   DevBuf d;
   if (!d.reserve(bytes)) return "device allocation failed";
   hipError_t he = hipMemcpy(d.p, pixels, bytes, hipMemcpyHostToDevice);
   if (he != hipSuccess) return hip_fail("upload", he);
-  std::vector<sa_image> items(frames);
-  for (int f = 0; f < frames; ++f) {
-    unsigned char* base = (unsigned char*)d.p + sizeof(RgbaF) * px * f;
-    items[f] = sa_image{base, base, (int64_t)width, (int64_t)height, SA_PIX_RGBAF32,
-                        SA_PIX_RGBAF32, pr.p};
-  }
+  const sa_image item{d.p, d.p, (int64_t)width, (int64_t)height, SA_PIX_RGBAF32,
+                      SA_PIX_RGBAF32, pr.p};
   sa_seq_options o{params.diffusion, params.use_cache ? 1 : 0, 0, 0, 0};
   char err[kErr] = {0};
-  if (sa_walk_seq_batch(items.data(), frames, &o, nullptr, err, sizeof(err)) != SA_OK)
+  if (sa_walk_seq_batch(&item, 1, &o, nullptr, err, sizeof(err)) != SA_OK)
     return prefixed("RiemersmaWalkCuda", err);
   he = hipMemcpy(pixels, d.p, bytes, hipMemcpyDeviceToHost);
   if (he != hipSuccess) return hip_fail("download", he);

@@ -306,3 +306,189 @@ def test_jacobi_flat_image_one_sweep_closed_form(oracle, fp64, v, taps, diffusio
     _, q = oracle.jacobi(tree, flat, 1, taps=taps, fp64=fp64, use_tree=False,
                          diffusion=diffusion, want_q=True)
     assert np.array_equal(q, expect)
+
+
+# ---- Jacobi: one sweep from a given q, by closed form ------------------------------
+# With x and q piecewise constant along the curve, the residual r[j] = x[j] - pal[q[j]]
+# is piecewise constant, and a sweep's target is
+#     t[i] = pal[q[i]] + sum over segments s of r_s * (S(hi_s(i)) - S(lo_s(i) - 1)),
+# S(k) = g[0] + ... + g[k], where [lo_s(i), hi_s(i)] is the part of the tap range
+# 0..min(taps, i) that reaches back into segment s.  That is computed here from
+# prefix sums of g in exact rational arithmetic, independently of the per-tap loop;
+# the restatement's rounded sum must sit within the standard bound of it
+# (gamma_(k+1) * sum |g r| for k+1 terms in the sweep's float type), and every
+# position whose decision margin exceeds that bound must get the nearest entry of
+# the exact target.  Residuals read from q_in (not from the sweep's own output),
+# the tap range, the sign and the indices each fail this for some segment layout,
+# which the planted mutants below confirm.
+from fractions import Fraction as _Fr
+
+
+def _gray_like_palette(oracle, colours):
+    """An OrcTree whose colormap is exactly `colours` (flat regions, one per colour)."""
+    k = len(colours)
+    img = np.zeros((4, 8 * k, 4), np.float32)
+    img[..., 3] = 65535.0
+    for i, c in enumerate(colours):
+        img[:, 8 * i: 8 * i + 8, :3] = np.asarray(c, np.float32)
+    tree = oracle.tree_build(img, k, False)
+    cm = tree.data().colormap
+    assert sorted(map(tuple, cm[:, :3].tolist())) == sorted(map(tuple, np.asarray(colours, float).tolist()))
+    return tree, cm
+
+
+def _closed_form_targets(cx, q, cm, g, taps):
+    """Exact targets t[i] (n x 4 Fractions as floats and the bound terms)."""
+    n = len(cx)
+    gf = [_Fr(float(v)) for v in g]
+    S = [_Fr(0)]
+    for v in gf:
+        S.append(S[-1] + v)                      # S[k+1] = g[0] + ... + g[k]
+    T = cx.dtype.type
+    res = np.array([[float(T(cx[j, c]) - T(cm[q[j], c])) for c in range(4)] for j in range(n)])
+    # segments of constant residual
+    cuts = [0] + [j for j in range(1, n) if not np.array_equal(res[j], res[j - 1])] + [n]
+    segs = [(cuts[s], cuts[s + 1] - 1, res[cuts[s]]) for s in range(len(cuts) - 1)]
+    t = np.zeros((n, 4))
+    mag = np.zeros((n, 4))
+    for i in range(n):
+        dmax = min(taps, i)
+        acc = [_Fr(0)] * 4
+        macc = [_Fr(0)] * 4
+        for (a, b, r) in segs:
+            lo, hi = max(0, i - b), min(dmax, i - a)   # taps d with i-d in [a, b]
+            if lo > hi:
+                continue
+            part = S[hi + 1] - S[lo]
+            apart = sum((abs(gf[d]) for d in range(lo, hi + 1)), _Fr(0)) if lo <= hi else _Fr(0)
+            for c in range(4):
+                acc[c] += _Fr(float(r[c])) * part
+                macc[c] += abs(_Fr(float(r[c]))) * apart
+        for c in range(4):
+            t[i, c] = float(_Fr(float(cm[q[i], c])) + acc[c])
+            mag[i, c] = float(macc[c])
+    return t, mag, np.array([min(taps, i) + 1 for i in range(n)])
+
+
+def _check_sweep(oracle, tree, cm, cx, q_in, taps, diffusion, mutant=0):
+    """Returns the list of failures of the closed-form checks (empty when pinned)."""
+    T = cx.dtype
+    u = 2.0 ** -24 if T == np.float32 else 2.0 ** -53
+    g = oracle.inverse_filter(taps, diffusion).astype(T)
+    q_out, t_got = oracle.jacobi_sweep(tree, cx, q_in, taps=taps, use_tree=False,
+                                       diffusion=diffusion, want_t=True, mutant=mutant)
+    t_exp, mag, terms = _closed_form_targets(cx, q_in, cm, g, taps)
+    gam = terms * u / (1 - terms * u)
+    bound = (gam[:, None] + u) * mag + 2.0 ** -52 * np.abs(t_exp) + 1e-9
+    bad = []
+    err = np.abs(t_got[:, :3] - t_exp[:, :3])
+    if np.any(err > bound[:, :3]):
+        bad.append(("target", int(np.argmax(np.any(err > bound[:, :3], axis=1)))))
+    # nearest entry of the exact clamped target, where the margin clears the bound
+    tc = np.clip(t_exp[:, :3], 0.0, 65535.0)
+    d2 = ((cm[None, :, :3] - tc[:, None, :]) ** 2).sum(-1)
+    order = np.argsort(d2, axis=1, kind="stable")
+    best, second = order[:, 0], order[:, 1]
+    dist = np.sqrt(d2)
+    margin = dist[np.arange(len(tc)), second] - dist[np.arange(len(tc)), best]
+    sure = margin > 4.0 * (bound[:, :3].max(axis=1) + 0.01)
+    if np.any(q_out[sure] != best[sure]):
+        bad.append(("index", int(np.argmax((q_out != best) & sure))))
+    return bad, int(sure.sum())
+
+
+_JAC_GREYS = [(100 * 257,) * 3, (180 * 257,) * 3]
+_JAC_COLOURS = [(9000, 20000, 41000), (30000, 30000, 8000), (52000, 12000, 30000),
+                (61000, 60000, 58000)]
+
+
+def _piecewise_cases(n, k):
+    rng = np.random.default_rng(1234)
+    out = []
+    for m in (0, 1, 57, n // 2, n - 1):                      # two segments of q, flat x
+        out.append(("flat-x", m))
+    for s in range(4):                                       # several segments of x and q
+        out.append(("random", s))
+    cases = []
+    for kind, m in out:
+        if kind == "flat-x":
+            xv = np.full(4, 130 * 257.0)
+            qi = np.array([0] * m + [1 % k] * (n - m), np.uint8)
+            xs = np.tile(xv, (n, 1))
+        else:
+            cuts = np.sort(rng.choice(np.arange(1, n), size=5, replace=False))
+            xs = np.zeros((n, 4))
+            qi = np.zeros(n, np.uint8)
+            edges = [0, *cuts.tolist(), n]
+            for a, b in zip(edges[:-1], edges[1:]):
+                xs[a:b, :3] = rng.integers(0, 256, 3) * 257.0
+                qi[a:b] = rng.integers(0, k)
+            xs[:, 3] = 65535.0
+        xs[:, 3] = 65535.0
+        cases.append((kind, m, xs, qi))
+    return cases
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("taps,diffusion", [(96, 1.0), (7, 1.0), (40, 0.6)])
+@pytest.mark.parametrize("palette", ["greys", "colours"])
+def test_jacobi_sweep_piecewise_closed_form(oracle, fp64, taps, diffusion, palette):
+    tree, cm = _gray_like_palette(oracle, _JAC_GREYS if palette == "greys" else _JAC_COLOURS)
+    n = 260
+    decided = 0
+    for kind, m, xs, qi in _piecewise_cases(n, len(cm)):
+        cx = xs.astype(np.float64 if fp64 else np.float32)
+        bad, sure = _check_sweep(oracle, tree, cm, cx, qi, taps, diffusion)
+        assert not bad, (kind, m, bad)
+        decided += sure
+    assert decided > 0.9 * n * 9                             # the index check bites almost everywhere
+
+
+@pytest.mark.parametrize("mutant", [1, 2, 3, 4])
+@pytest.mark.parametrize("fp64", [False, True])
+def test_jacobi_sweep_closed_form_catches_planted_mistakes(oracle, mutant, fp64):
+    """In-place (Gauss-Seidel) update, a dropped last tap, a flipped residual sign
+    and a residual one position late, each planted in the oracle, fail the
+    closed-form checks above."""
+    caught = {}
+    for taps in (96, 7):        # at taps 96 the last tap (g[96] ~ 1e-5) hides under fp32's bound
+        for palette in (_JAC_GREYS, _JAC_COLOURS):
+            tree, cm = _gray_like_palette(oracle, palette)
+            for kind, m, xs, qi in _piecewise_cases(260, len(cm)):
+                cx = xs.astype(np.float64 if fp64 else np.float32)
+                bad, _ = _check_sweep(oracle, tree, cm, cx, qi, taps, 1.0, mutant=mutant)
+                caught[taps] = caught.get(taps, 0) + bool(bad)
+    assert caught[7] > 0
+    if fp64 or mutant != 2:
+        assert caught[96] > 0
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("use_tree,assoc", [(True, False), (False, False), (True, True)])
+def test_jacobi_iterations_are_composed_sweeps(oracle, fp64, use_tree, assoc):
+    """iterations=k is k sweeps, each reading only the previous sweep's q
+    (src/rd_approx_cuda.cu:356-368), started from InitIndex; the final scatter
+    writes pal[q] at each pixel's owning position."""
+    f = _jac_image(11, 53, 37, assoc)
+    tree = oracle.tree_build(f, 24, assoc)
+    cx = oracle.jacobi_gather(f, fp64)
+    q = oracle.jacobi_init(tree, cx, use_tree)
+    for k in range(0, 5):
+        out, qk = oracle.jacobi(tree, f, k, taps=33, fp64=fp64, use_tree=use_tree, want_q=True)
+        assert np.array_equal(qk, q), k
+        order, owner = oracle.block_order(53, 37)
+        cm = tree.data().colormap
+        own = owner >= 0
+        exp = np.array(f, np.float32).reshape(-1, 4).copy()
+        sel = cm[q[owner[own]]]
+        exp[own, :3] = sel[:, :3].astype(np.float32)
+        exp[own, 3] = sel[:, 3].astype(np.float32) if assoc else np.float32(65535.0)
+        assert np.array_equal(out.reshape(-1, 4), exp), k
+        q = oracle.jacobi_sweep(tree, cx, q, taps=33, use_tree=use_tree)
+    # the in-place mutant's composition differs from the ping-pong
+    q = oracle.jacobi_init(tree, cx, use_tree)
+    qm = q
+    for _ in range(3):
+        qm = oracle.jacobi_sweep(tree, cx, qm, taps=33, use_tree=use_tree, mutant=1)
+    _, q3 = oracle.jacobi(tree, f, 3, taps=33, fp64=fp64, use_tree=use_tree, want_q=True)
+    assert not np.array_equal(qm, q3)
