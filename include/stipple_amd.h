@@ -235,6 +235,19 @@ int sa_seq_stats(uint64_t* out, char* err, size_t errlen);
 int sa_walk_blocks(const sa_image* frames, int n, int block, double diffusion, void* stream,
                    char* err, size_t errlen);
 
+/* sa_walk_blocks without allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * results and refusals as sa_walk_blocks; the walk's scratch (frame table,
+ * weights, curve) lives in one fixed 40 MiB device array per device, bound by
+ * sa_init, so the call makes no device or pinned-host allocation.  Limits:
+ * sides up to 4096, width*height up to 10469376 (a 3840x2160 frame fits), at
+ * most 1024 frames; beyond them it returns SA_EUNSUPPORTED.  Requires sa_init
+ * on the current device (else SA_EINVAL).  Every call on a device shares that
+ * array: calls that overlap on one device race. */
+int sa_walk_blocks_noalloc(const sa_image* frames, int n, int block, double diffusion,
+                           void* stream, char* err, size_t errlen);
+
 /* The threshold-screen dithers of the plugin seam, one thread per pixel, over
  * frames of one geometry (each with its own palette, any entry count):
  *   SA_SCREEN_BAYER8     "bayer-ordered" (src/rd_dithers_ordered.cc:147-168)

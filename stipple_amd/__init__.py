@@ -50,6 +50,7 @@ from ._lib import (  # noqa: F401
     library_path,
     walk_approx,
     walk_blocks,
+    walk_blocks_noalloc,
     walk_seq_batch,
 )
 
@@ -70,5 +71,6 @@ __all__ = [
     "library_path",
     "walk_approx",
     "walk_blocks",
+    "walk_blocks_noalloc",
     "walk_seq_batch",
 ]

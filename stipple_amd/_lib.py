@@ -88,6 +88,7 @@ def lib():
                 "sa_palette_tree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, cp, sz]),
                 "sa_walk_seq_batch": (i32, [C.POINTER(_Image), i32, C.POINTER(SeqOptions), vp, cp, sz]),
                 "sa_walk_blocks": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
+                "sa_walk_blocks_noalloc": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_walk_approx": (i32, [C.POINTER(_Image), i32, i32, i32, i32, i32, dbl, vp, cp, sz]),
                 "sa_dither_screen": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_seq_stats": (i32, [vp, cp, sz]),
@@ -401,6 +402,16 @@ def walk_blocks(frames, block=64, diffusion=1.0, stream=None):
     arr = (_Image * len(frames))(*frames)
     e = _err()
     rc = lib().sa_walk_blocks(arr, len(frames), block, diffusion, _stream_ptr(stream), e, len(e))
+    _check(rc, e)
+
+
+def walk_blocks_noalloc(frames, block=64, diffusion=1.0, stream=None):
+    """walk_blocks with the library's fixed device scratch (sa_walk_blocks_noalloc):
+    no device allocation; calls that overlap on one device race."""
+    arr = (_Image * len(frames))(*frames)
+    e = _err()
+    rc = lib().sa_walk_blocks_noalloc(arr, len(frames), block, diffusion, _stream_ptr(stream), e,
+                                      len(e))
     _check(rc, e)
 
 

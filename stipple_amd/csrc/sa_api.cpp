@@ -44,6 +44,8 @@ struct sa_palette_s {
 
 namespace {
 
+int bind_noalloc_scratch(int device, char* err, size_t errlen);  // sa_walk_blocks_noalloc
+
 // ---- tuning knobs (set by an autotuner; none of them changes a result) ------
 struct Knobs {
   std::atomic<int> blocks_threads{256};  // block walk: blocks per workgroup
@@ -305,7 +307,9 @@ extern "C" int sa_init(int device, char* err, size_t errlen) {
     std::lock_guard<std::mutex> g(g_mu);
     g_device = -1;
   }
-  return ensure_init(err, errlen);
+  const int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  return bind_noalloc_scratch(device, err, errlen);
 }
 
 extern "C" int sa_device_name(char* out, size_t len) {
@@ -1266,7 +1270,7 @@ struct BlocksPlan {
 };
 
 int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipStream_t s,
-                BlocksPlan& p, char* err, size_t errlen) {
+                BlocksPlan& p, char* err, size_t errlen, bool no_alloc = false) {
   if (n < 1 || !frames)
     return fail(err, errlen, SA_EREFUSED,
                 "invalid dither batch: frames must be >= 1 and block must be >= 16 (got frames=%d, block=%d)",
@@ -1309,8 +1313,10 @@ int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipS
     return fail(err, errlen, SA_EUNSUPPORTED,
                 "the blocks engine stages the palette's search image in LDS (%d bytes): too large",
                 p.max_img);
-  rc = get_weights(diffusion, s, &p.sw, err, errlen);
-  if (rc) return rc;
+  if (!no_alloc) {
+    rc = get_weights(diffusion, s, &p.sw, err, errlen);
+    if (rc) return rc;
+  }
   const int level = curve_level(p.w, p.h);
   // Block curve length: in-bounds of [0, 4^L - 1) plus the origin; the gap
   // cell (2^L-1, 0) is in bounds exactly when w == 2^L.
@@ -1321,6 +1327,7 @@ int plan_blocks(const sa_image* frames, int n, int block, double diffusion, hipS
   } else {
     p.gap = 0;  // 1x1: the block curve is empty (src/rd_riemersma_cpu.cpp:275)
   }
+  if (no_alloc) return SA_OK;
   p.fr.resize((size_t)n);
   for (int i = 0; i < n; ++i) {
     p.fr[(size_t)i].src = frames[i].src;
@@ -1361,6 +1368,114 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
   }));
   if (p.gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, p.n, p.gap, p.w, p.h, s));
   HIP_TRY(hipFreeAsync(d_fr, s));
+  return SA_OK;
+}
+
+// ---- block walk without allocation ---------------------------------------------------
+// sa_walk_blocks for callers that cannot allocate: the walk's scratch (frame
+// table, weights, curve chunk counts, curve) lives in one fixed device array,
+// so the call makes no hipMalloc/hipFree and no pinned-host allocation.  Its
+// device address is taken once per device by sa_init (hipGetSymbolAddress);
+// host code uses only that address, never the array's name.  Every call on a
+// device shares its array: calls that overlap on one device race.
+namespace {
+constexpr size_t kNoAllocBytes = (size_t)40 << 20;        // the array: 40 MiB
+constexpr size_t kNoAllocHead = (size_t)64 << 10;         // frames, weights, counts
+constexpr int kNoAllocMaxFrames = 1024;
+constexpr int kNoAllocMaxLevel = 12;                       // sides up to 4096
+constexpr size_t kNoAllocWeightsOff = sizeof(sa::BlockFrame) * kNoAllocMaxFrames;
+constexpr size_t kNoAllocCountsOff = kNoAllocWeightsOff + 256;
+static_assert(kNoAllocCountsOff + sizeof(int32_t) * (((size_t)1 << (2 * kNoAllocMaxLevel)) / 4096) <=
+                  kNoAllocHead,
+              "the head holds the frame table, the weights and the chunk counts");
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[kMaxDevices] = {};  // scratch's address on each device
+
+// Called by sa_init with `device` current.
+int bind_noalloc_scratch(int device, char* err, size_t errlen) {
+  if (device < 0 || device >= kMaxDevices) return SA_OK;  // the entry point then refuses
+  std::lock_guard<std::mutex> g(g_na_mu);
+  if (g_na_ptr[device]) return SA_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(err, errlen, SA_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  g_na_ptr[device] = (unsigned char*)ptr;
+  return SA_OK;
+}
+
+// Up to 32 frame descriptors and the weights, carried by value in the kernel
+// arguments (the runtime copies them at launch): no host staging is needed.
+constexpr int kSetupFrames = 32;
+struct NoAllocSetup {
+  sa::BlockFrame fr[kSetupFrames];
+  double sw[16];
+  int first, count, with_weights;
+};
+
+__global__ void noalloc_setup_kernel(NoAllocSetup a, sa::BlockFrame* frames, double* sw) {
+  const int t = threadIdx.x;
+  if (t < a.count) frames[a.first + t] = a.fr[t];
+  if (a.with_weights && t < 16) sw[t] = a.sw[t];
+}
+}  // namespace
+
+extern "C" int sa_walk_blocks_noalloc(const sa_image* frames, int n, int block, double diffusion,
+                                      void* stream, char* err, size_t errlen) {
+  hipStream_t s = (hipStream_t)stream;
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (dev >= 0 && dev < kMaxDevices) base = g_na_ptr[dev];
+  }
+  if (!base)
+    return fail(err, errlen, SA_EINVAL, "sa_walk_blocks_noalloc: call sa_init on device %d first",
+                dev);
+  if (n > kNoAllocMaxFrames)
+    return fail(err, errlen, SA_EUNSUPPORTED,
+                "sa_walk_blocks_noalloc: %d frames, at most %d per call", n, kNoAllocMaxFrames);
+  BlocksPlan p;
+  int rc = plan_blocks(frames, n, block, diffusion, s, p, err, errlen, true);
+  if (rc) return rc;
+  const int level = curve_level(p.w, p.h);
+  const size_t curve_bytes = sizeof(int32_t) * (size_t)(p.w * p.h);
+  if (level > kNoAllocMaxLevel || kNoAllocHead + curve_bytes > kNoAllocBytes)
+    return fail(err, errlen, SA_EUNSUPPORTED,
+                "sa_walk_blocks_noalloc: a %lldx%lld frame needs %zu bytes of scratch; the fixed "
+                "maximum is %zu (sides up to 4096, width*height up to %zu)",
+                (long long)p.w, (long long)p.h, kNoAllocHead + curve_bytes, kNoAllocBytes,
+                (kNoAllocBytes - kNoAllocHead) / sizeof(int32_t));
+  sa::BlockFrame* d_fr = (sa::BlockFrame*)base;
+  double* d_sw = (double*)(base + kNoAllocWeightsOff);
+  int32_t* d_counts = (int32_t*)(base + kNoAllocCountsOff);
+  int32_t* d_curve = (int32_t*)(base + kNoAllocHead);
+  // Frame table and weights, 32 frames per launch.
+  for (int first = 0; first < n; first += kSetupFrames) {
+    NoAllocSetup a{};
+    a.first = first;
+    a.count = std::min(kSetupFrames, n - first);
+    a.with_weights = first == 0;
+    if (first == 0) scaled_weights(diffusion, a.sw);
+    for (int k = 0; k < a.count; ++k) {
+      const sa_image& f = frames[first + k];
+      a.fr[k] = sa::BlockFrame{f.src, f.dst, f.palette->dev_hdr, f.src_fmt, f.dst_fmt};
+    }
+    noalloc_setup_kernel<<<dim3(1), dim3(64), 0, s>>>(a, d_fr, d_sw);
+    HIP_TRY(hipGetLastError());
+  }
+  // The curve, rebuilt on every call (the array keeps only the last call's).
+  const int64_t chunks = sa::curve_chunks(level);
+  HIP_TRY(sa::launch_curve_count(level, p.w, p.h, d_counts, chunks, s));
+  HIP_TRY(sa::launch_curve_scan(d_counts, chunks, s));
+  HIP_TRY(sa::launch_curve_emit(level, p.w, p.h, d_counts, chunks, d_curve, s));
+  p.sw = d_sw;
+  HIP_TRY(launch_blocks_plan(p, d_fr, d_curve, d_sw, s));
   return SA_OK;
 }
 
