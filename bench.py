@@ -104,7 +104,8 @@ def _line(args, metric, value, unit, ms, dtype, config, roofline, cpu, extra=Non
     if extra:
         line.update(extra)
     if cpu:
-        line["gpu_vs_cpu_1core"] = round(value / cpu["value"], 2)
+        key = "gpu_vs_cpu_1core" if cpu.get("cores", 1) == 1 else f"gpu_vs_cpu_{cpu['cores']}_cores"
+        line[key] = round(value / cpu["value"], 2)
     return line
 
 
@@ -425,29 +426,51 @@ def run_approx4k(args, sa, torch, synth, dev):
     chain.close()
     px = w * h
     cpu = None
+    tolerance = None
     if not args.no_cpu:
+        # The CPU sample: a 960x540 crop of frame 0 through the restatement, with
+        # frame 0's palette.  The same crop through the GPU engine gives the
+        # stated tolerance (differing indices, mean per-channel Q16 error).
         from oracle.oracle import Oracle
         o = Oracle()
         f = synth.widen_q16(frames[0])
         tree = o.tree_build(f, 256, False, False)
-        crop = np.ascontiguousarray(f[:540, :960])
+        crop16 = np.ascontiguousarray(frames[0][:540, :960])
+        crop = synth.widen_q16(crop16)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         t1 = time.perf_counter()
-        o.jacobi(tree, crop, 8, 96)
+        exp = o.jacobi(tree, crop, 8, 96)
         tc = time.perf_counter() - t1
         cpu = {"value": round(crop.shape[0] * crop.shape[1] / tc / 1e6, 3), "unit": "Mpixel/s",
-               "cores": 1, "kind": "port",
+               "cores": min(threads, 16), "kind": "port",
                "sample": "a 960x540 crop of frame 0 through the CPU restatement of the Jacobi "
-                         "engine (the reference has no CPU Jacobi), 8 sweeps, taps 96"}
-    # per frame: RGB16 in (6 B) and out (6 B) + per sweep the curve-order
-    # input read (16 B) and q in/out (2 B)
-    algo = px * nf * (12 + 8 * 18)
+                         "engine (the reference has no CPU Jacobi), 8 sweeps, taps 96, the "
+                         "sweep's positions split over host threads"}
+        cpal = sa.Palette.from_tree(tree.data())
+        csrc = torch.from_numpy(crop16.reshape(-1).view(np.uint8).copy()).cuda()
+        cdst = torch.empty(540 * 960 * 16, dtype=torch.uint8, device="cuda")
+        sa.walk_approx([sa.image_spec(csrc, cdst, 960, 540, sa.PIX_RGB16, sa.PIX_RGBAF32, cpal)],
+                       iterations=8, taps=96)
+        torch.cuda.synchronize()
+        got = cdst.cpu().numpy().view(np.float32).reshape(540, 960, 4)
+        differ = float(np.any(got[..., :3] != exp[..., :3], axis=-1).mean())
+        q16 = float(np.abs(got[..., :3].astype(np.float64) - exp[..., :3]).mean())
+        tolerance = {"differing_index_fraction": differ, "mean_q16_error": round(q16, 6),
+                     "limit": 0.005, "within": differ <= 0.005,
+                     "sample": "960x540 crop of frame 0, frame 0's 256-colour palette, "
+                               "against the CPU restatement"}
+        cpal.close()
+    # Algorithmic bytes: per frame RGB16 in (6 B) and out (6 B); per sweep
+    # the curve-order input read (6 B of RGB16) and the index in and out (2 B).
+    algo = px * nf * (12 + 8 * 8)
     achieved = algo / (ap_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": _traffic("approx_c4_traffic"),
             "kernel": "approx (gather, 8 sweeps, scatter)",
-            "note": "algorithmic 12 B/px per frame + 18 B/px per sweep; the sweep is FP32-VALU "
-                    "bound at taps=96 (97 taps x 3 channels x multiply and add = 582 FP32 ops "
-                    "per px per sweep)"}
+            "note": "algorithmic 12 B/px per frame + 8 B/px per sweep (RGB16 in, index in "
+                    "and out); traffic is the measured HBM bytes of the same launch; the sweep "
+                    "is FP32-VALU bound at taps=96 (97 taps x 3 channels x multiply and add = "
+                    "582 FP32 ops per px per sweep)"}
     return _line(args, "Mpixel/s and frames/s, 3840x2160 RGB16, Jacobi 8 sweeps",
                  px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f32",
                  {"workload": f"{nf} 3840x2160 RGB16 radial-gradient frames with 5% salt and "
@@ -457,7 +480,128 @@ def run_approx4k(args, sa, torch, synth, dev):
                  {"fps": round(nf / t, 2), "approx_ms": round(ap_ms, 3),
                   "chain": "hipGraph replay of the 16-frame batch (sa_chain_run)", "tuned": tuned,
                   "eager_ms": round(t_eager * 1e3, 3),
-                  "tolerance": "0 differing indices against the CPU restatement (bit-exact)"})
+                  "tolerance": tolerance})
+
+
+def serving_mix(args, sa, torch, synth):
+    """The serving mix: Poisson arrivals at --mix-rate requests/s for
+    --mix-seconds through Server with a --mix-hbm-gb admission cap; 70% stills
+    (square side uniform in [256, 2048], RGB8 plasma + noise), 30% segments of
+    30 frames at 1280x720 whose palette (built from frame 0) stays on the
+    device for all 30; colours uniform in {4..64}; engines 30% sequential,
+    50% block (B=64), 20% Jacobi (8 sweeps, taps 96).  Inputs come from pools
+    generated on the device beforehand (24 stills, 6 segments), so arrivals
+    cost no host generation; outputs are RGB8.  One completed request of each
+    (engine, kind) is checked against the same request run alone."""
+    from stipple_amd.serve import Request, Server
+    rng = np.random.default_rng(args.mix_seed)
+    t0 = time.perf_counter()
+    sides = [int(v) for v in rng.integers(256, 2049, 24)]
+    still_pool = [synth.plasma_device(s_, s_, t=k, seed=11) for k, s_ in enumerate(sides)]
+    seg_pool = [[synth.plasma_device(1280, 720, t=f, seed=20 + k) for f in range(30)]
+                for k in range(6)]
+    torch.cuda.synchronize()
+    log(f"mix: input pools generated on the device in {time.perf_counter() - t0:.1f}s")
+    sched, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / args.mix_rate))
+        if t >= args.mix_seconds:
+            break
+        kind = "still" if rng.random() < 0.7 else "segment"
+        u = rng.random()
+        engine = "sequential" if u < 0.3 else ("blocks" if u < 0.8 else "approx")
+        colours = int(rng.integers(4, 65))
+        idx = int(rng.integers(0, len(still_pool) if kind == "still" else len(seg_pool)))
+        sched.append((t, kind, engine, colours, idx))
+    # Sampled for the alone check: the first request of each (engine, kind)
+    # after the first 10 s (the mix is then in full flow).
+    sample, seen = set(), set()
+    for k, (ta, kind, engine, _, _) in enumerate(sched):
+        if ta >= min(10.0, args.mix_seconds / 2) and (engine, kind) not in seen:
+            seen.add((engine, kind))
+            sample.add(k)
+    kept, stats = {}, []
+
+    def make(k):
+        ta, kind, engine, colours, idx = sched[k]
+        frames = [still_pool[idx]] if kind == "still" else seg_pool[idx]
+        h, w = frames[0].shape[:2]
+        pairs = [(f, torch.empty(w * h * 3, dtype=torch.uint8, device="cuda")) for f in frames]
+        req = Request(engine, pairs, w, h, sa.PIX_RGB8, sa.PIX_RGB8, colours, False, block=64,
+                      iterations=8, taps=96)
+        req.k, req.kind, req.px = k, kind, w * h * len(frames)
+        return req
+
+    def done(req):
+        stats.append((req.k, req.kind, req.engine, req.px, req.t_arrive, req.t_done, req.error))
+        if req.k in sample:
+            kept[req.k] = req
+        else:
+            req.frames = None          # the outputs are complete: release them
+
+    opts = sa.SeqOptions(memo_bits=args.mix_memo_bits)
+    srv = Server(hbm_budget=int(args.mix_hbm_gb * (1 << 30)), build_threads=args.build_threads,
+                 budget_us=args.budget_us, on_done=done, seq_options=opts)
+    arrivals = [(sched[k][0], (lambda k=k: make(k))) for k in range(len(sched))]
+    log(f"mix: {len(sched)} arrivals over {args.mix_seconds} s")
+    srv.run(arrivals, stop_after=args.mix_seconds * 2 + 120, feed_until=args.mix_seconds * 1.5)
+    torch.cuda.synchronize()
+    t_first = srv.t0
+    srv.close()
+    errs = [e for *_, e in stats if e]
+    if errs:
+        raise RuntimeError(f"mix: {len(errs)} requests failed, first: {errs[0]}")
+    lat = np.array([td - ta for _, _, _, _, ta, td, _ in stats])
+    t_end = max(td for _, _, _, _, _, td, _ in stats)
+    px = sum(p for _, _, _, p, _, _, _ in stats)
+
+    def pct(sel):
+        v = np.array([td - ta for _, kd, en, _, ta, td, _ in stats if sel(kd, en)])
+        if v.size == 0:
+            return None
+        return {"n": int(v.size), "p50": round(float(np.percentile(v, 50)) * 1e3, 3),
+                "p99": round(float(np.percentile(v, 99)) * 1e3, 3)}
+
+    per = {f"{en}/{kd}": pct(lambda a, b, kd=kd, en=en: a == kd and b == en)
+           for en in ("sequential", "blocks", "approx") for kd in ("still", "segment")}
+    # The sampled requests again, each alone (its own palette build from frame 0
+    # and one engine call): the mix must change when work runs, never what it
+    # computes.  (tests/test_gpu_serve.py checks mixed results against the oracle.)
+    checked, equal = 0, True
+    for k, req in sorted(kept.items()):
+        pal = sa.Palette.build(req.frames[0][0], req.width, req.height, sa.PIX_RGB8, req.colors,
+                               False, False)
+        pairs = [req.frames[0]] if req.kind == "still" else [req.frames[0], req.frames[-1]]
+        alone = [torch.empty_like(dst) for _, dst in pairs]
+        specs = [sa.image_spec(src, a, req.width, req.height, sa.PIX_RGB8, sa.PIX_RGB8, pal)
+                 for (src, _), a in zip(pairs, alone)]
+        if req.engine == "sequential":
+            sa.walk_seq_batch(specs, sa.SeqOptions(memo_bits=args.mix_memo_bits))
+        elif req.engine == "blocks":
+            sa.walk_blocks(specs, block=64)
+        else:
+            sa.walk_approx(specs, iterations=8, taps=96)
+        torch.cuda.synchronize()
+        for (_, dst), a in zip(pairs, alone):
+            equal = equal and bool(torch.equal(dst, a))
+            checked += 1
+        pal.close()
+    res = {"seconds": args.mix_seconds, "rate_per_s": args.mix_rate, "hbm_cap_gb": args.mix_hbm_gb,
+           "arrivals": len(sched), "completed": len(stats), "not_admitted": srv.dropped,
+           "value": round(px / (t_end - t_first) / 1e6, 3), "unit": "Mpixel/s",
+           "wall_s": round(t_end - t_first, 2), "pixels": int(px),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                          "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+           "per_engine_kind": per, "admission_wait_s": round(srv.admission_wait_s, 2),
+           "seq_memo_bits": args.mix_memo_bits or None,
+           "checked_frames": checked, "same_as_alone": equal,
+           "inputs": "pools generated on the device: 24 stills (side uniform in [256, 2048]) "
+                     "and 6 segments of 30 1280x720 frames, RGB8 plasma + noise"}
+    log(f"mix: {res['completed']} requests, {res['value']} Mpx/s, p50 {res['latency_ms']['p50']} "
+        f"ms, p99 {res['latency_ms']['p99']} ms, {checked} frames checked, equal: {equal}")
+    del still_pool, seg_pool, kept
+    torch.cuda.synchronize()
+    return res
 
 
 def main():
@@ -490,13 +634,20 @@ def main():
                          "rest are classified (0: one walk launch after all palettes)")
     ap.add_argument("--video-format", choices=["rgb8", "yuv420"], default="rgb8",
                     help="video workload: RGB8 in and out, or planar 4:2:0 in and 4:4:4 out")
-    ap.add_argument("--workload", choices=["stills", "ramp", "video", "approx4k", "screens"],
+    ap.add_argument("--workload", choices=["stills", "ramp", "video", "approx4k", "screens", "mix"],
                     default="stills",
                     help="stills: 64 uneven RGBA8 stills, sequential engine (the headline); "
                          "ramp: one 640x480 image; video: 1080p frames, block engine; "
                          "approx4k: 4K RGB16 frames, Jacobi engine")
     ap.add_argument("--frames", type=int, default=0,
                     help="frames per step (0: 32 for video, one batched build; 16 for approx4k)")
+    ap.add_argument("--mix-seconds", type=float, default=120.0,
+                    help="stills: seconds of Poisson arrivals for the serving-mix leg (0: skip)")
+    ap.add_argument("--mix-rate", type=float, default=200.0, help="serving mix: requests per second")
+    ap.add_argument("--mix-hbm-gb", type=float, default=64.0, help="serving mix: HBM admission cap")
+    ap.add_argument("--mix-memo-bits", type=int, default=8,
+                    help="serving mix: LDS memo size of the sequential walks (4..10; 0: default)")
+    ap.add_argument("--mix-seed", type=int, default=7)
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run a 256 MiB device copy (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -520,6 +671,14 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.workload == "mix":
+        mix = serving_mix(args, sa, torch, synth)
+        print(json.dumps(_line(args, "Mpixel/s and p50/p99 request latency, serving mix",
+                               mix["value"], "Mpixel/s", mix["wall_s"] * 1e3, "f64",
+                               {"workload": "serving mix: Poisson arrivals, stills and 30-frame "
+                                            "segments, sequential/block/Jacobi engines",
+                                "device": dev}, None, None, {"serving_mix": mix})), flush=True)
+        return
     other = {"ramp": run_ramp, "video": run_video, "approx4k": run_approx4k,
              "screens": run_screens}.get(args.workload)
     if other:
@@ -665,6 +824,8 @@ def main():
         del sdsts, stills
         torch.cuda.synchronize()
 
+    mix = serving_mix(args, sa, torch, synth) if args.mix_seconds > 0 else None
+
     # Engine alone, palettes resident, with the walk kernel timed by events.
     specs = specs_for(pals)
     sa.timing_enable(True)
@@ -768,6 +929,7 @@ def main():
         "chain": args.chain,
         "latency_ms": chain_lat,
         "served": served,
+        "serving_mix": mix,
         "palette_build_ms": round(t_pal * 1e3, 3),
         "palette_build_serial_ms": round(t_pal_serial * 1e3, 3),
         "plain": plain,

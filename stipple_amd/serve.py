@@ -294,15 +294,19 @@ class Server:
     own without waiting for each other; a finisher thread stamps each one done
     when its event completes."""
 
-    def __init__(self, hbm_budget=64 << 30, build_threads=2, budget_us=8000.0, slice_visits=0):
+    def __init__(self, hbm_budget=64 << 30, build_threads=2, budget_us=8000.0, slice_visits=0,
+                 on_done=None, seq_options=None):
         import torch
         self.budget = hbm_budget
         self.used = 0
         self.done = []
+        self.on_done = on_done           # called with each finished Request (any thread)
+        self.admission_wait_s = 0.0      # time arrivals spent waiting for the HBM budget
         self._cv = threading.Condition()
         self.builder = PaletteBuilder(threads=build_threads)
         self._seq = SeqPipeline(budget_us=budget_us, on_done=self._seq_done,
-                                slice_visits=slice_visits, builder=self.builder)
+                                slice_visits=slice_visits, builder=self.builder,
+                                options=seq_options)
         self._fast_stream = torch.cuda.Stream()
         self._fast = ThreadPoolExecutor(max_workers=1)
         self._events = queue.Queue()
@@ -310,6 +314,8 @@ class Server:
         self._finisher.start()
 
     def _release(self, req):
+        if self.on_done:
+            self.on_done(req)
         with self._cv:
             self.used -= req.bytes
             self.done.append(req)
@@ -390,29 +396,44 @@ class Server:
         self.builder.submit(src0, req.width, req.height, req.src_fmt, req.colors, req.assoc,
                             False, seg)
 
-    def run(self, arrivals, stop_after=None):
+    def run(self, arrivals, stop_after=None, feed_until=None):
         """Feeds (t_offset, Request) pairs at their times, admitting each when
         its footprint fits the budget (first come, first served); returns the
-        finished requests."""
+        finished requests.  The second element may instead be a callable that
+        returns the Request when it arrives (so a long arrival schedule does not
+        hold every request's buffers at once).  With `feed_until` (seconds),
+        arrivals not admitted by then are not started; `self.dropped` counts
+        them."""
         arrivals = sorted(arrivals, key=lambda a: a[0])
         stop = threading.Event()
         driver = threading.Thread(target=self._seq.run, kwargs={"until_idle": False, "stop": stop})
         driver.start()
         t0 = time.perf_counter()
+        self.t0 = t0
+        self.dropped = 0
+        started = 0
         try:
             for t_off, req in arrivals:
                 now = time.perf_counter() - t0
+                if feed_until is not None and now > feed_until:
+                    self.dropped = len(arrivals) - started
+                    break
                 if t_off > now:
                     time.sleep(t_off - now)
+                if callable(req):
+                    req = req()
                 req.t_arrive = t0 + t_off
                 req.bytes = footprint(req)
                 with self._cv:
+                    w0 = time.perf_counter()
                     while self.used + req.bytes > self.budget and self.used > 0:
                         self._cv.wait(0.05)
+                    self.admission_wait_s += time.perf_counter() - w0
                     self.used += req.bytes
                 self._start(req)
+                started += 1
             with self._cv:
-                while len(self.done) < len(arrivals):
+                while len(self.done) < started:
                     self._cv.wait(0.05)
                     if stop_after and time.perf_counter() - t0 > stop_after:
                         break

@@ -154,3 +154,28 @@ def rgb8_to_yuv420(img: np.ndarray) -> np.ndarray:
     v = ((112 * r2 - 94 * g2 - 18 * b2 + 128) >> 8) + 128
     planes = [np.clip(p, 0, 255).astype(np.uint8).reshape(-1) for p in (y, u, v)]
     return np.concatenate(planes)
+
+
+# -- serving mix: plasma + noise generated on the device ----------------------
+def plasma_device(w, h, t=0, seed=5, sigma=4.0, device="cuda"):
+    """An RGB8 plasma frame (the formula of plasma_frame: per channel 127.5 +
+    127.5 * mean of 4 sinusoids with seeded frequencies in [0.002, 0.02]
+    cycles/px, phases drifting 0.05 rad per frame t, plus N(0, sigma)), made
+    with torch on `device` and returned there as h*w*3 contiguous uint8 --
+    for input pools too large to generate on the host at serving rates."""
+    import torch
+    p = plasma_params(seed)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed * 100003 + t)
+    x = torch.arange(w, dtype=torch.float32, device=device)
+    y = torch.arange(h, dtype=torch.float32, device=device)
+    planes = torch.empty((h, w, 3), dtype=torch.float32, device=device)
+    for c in range(3):
+        acc = torch.zeros((h, w), dtype=torch.float32, device=device)
+        for k in range(4):
+            fx, fy = (float(v) for v in p["freq"][c, k])
+            ph = float(p["phase"][c, k]) + p["drift"] * t
+            acc += torch.sin(2 * math.pi * fy * y[:, None] + 2 * math.pi * fx * x[None, :] + ph)
+        planes[..., c] = 127.5 + 127.5 * acc / 4.0
+    planes += torch.randn((h, w, 3), generator=g, device=device) * sigma
+    return planes.round_().clamp_(0, 255).to(torch.uint8).contiguous()
