@@ -1,6 +1,11 @@
 // rd_engine.cpp -- the reference's engine API (include/rd_riemersma.h,
 // include/rd_octree.h, include/rd_plugin.h) over the C-ABI of
-// include/stipple_amd.h.
+// include/stipple_amd.h, as this repository's drop-in library (librd_amd.so).
+//
+// On top of the GPU engine entry points it shares with the reference-side
+// adapter (rd_gpu_engines.inc), it holds what only a stand-alone build needs:
+// ColorTree built on the device, the curve functions, RiemersmaWalkBatch and
+// the plugin registry with the device screen dithers.
 //
 // Host code only moves memory and arguments: every result is computed by the
 // HIP kernels behind the C-ABI.  Engines keep the reference's conventions:
@@ -21,97 +26,19 @@
 
 namespace rd {
 namespace {
-
-constexpr size_t kErr = 512;
-
-std::string prefixed(const char* what, const char* err) {
-  return std::string(what) + ": " + (err[0] ? err : "failed");
-}
-
-bool ensure_device(std::string* why) {
-  static std::once_flag once;
-  static int rc = SA_EDEVICE;
-  static char msg[kErr] = {0};
-  std::call_once(once, [] {
-    if (!sa_available()) {
-      std::snprintf(msg, sizeof(msg), "no MI355X (gfx950) device available");
-      return;
-    }
-    rc = sa_init(0, msg, sizeof(msg));
-  });
-  if (rc != SA_OK && why) *why = msg[0] ? msg : "device initialisation failed";
-  return rc == SA_OK;
-}
-
-void fill_name(std::string* device_name) {
-  if (!device_name) return;
-  char b[256] = {0};
-  sa_device_name(b, sizeof(b));
-  *device_name = b;
-}
-
-// Device memory owned by one call (or one state slot).
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  bool reserve(size_t n) {
-    if (n <= bytes) return true;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    if (hipMalloc(&p, std::max<size_t>(n, 16)) != hipSuccess) return false;
-    bytes = n;
-    return true;
-  }
-};
-
-std::string hip_fail(const char* what, hipError_t e) {
-  return std::string(what) + ": " + hipGetErrorString(e);
-}
-
-// The device search structure for (palette, tree): the tree's own when the
-// palette is its colormap, else uploaded from the tree's node arrays.
-struct PaletteRef {
-  sa_palette p = nullptr;
-  bool owned = false;
-  ~PaletteRef() {
-    if (owned && p) sa_palette_destroy(p);
-  }
-};
-
-std::string palette_for(const Palette& pal, const ColorTree& tree, PaletteRef* out) {
-  if (pal.count <= 0 || pal.count > kMaxColormapSize) return "palette is empty or too large";
-  if (tree.node_count() == 0) return "colour tree is empty (ColorTree::Build not run)";
+// The tree's own device search structure when `pal` is its colormap.
+sa_palette resident_palette(const Palette& pal, const ColorTree& tree) {
   if (tree.device_palette() && tree.color_count() == pal.count &&
-      std::memcmp(tree.colormap(), pal.entries, sizeof(PaletteEntry) * pal.count) == 0) {
-    out->p = tree.device_palette();
-    out->owned = false;
-    return "";
-  }
-  const std::vector<QNode>& nodes = tree.nodes();
-  const size_t n = nodes.size();
-  std::vector<int32_t> child(16 * n), parent(n);
-  std::vector<uint32_t> cn(n);
-  std::vector<uint64_t> nu(n);
-  for (size_t i = 0; i < n; ++i) {
-    for (int k = 0; k < 16; ++k) child[16 * i + k] = nodes[i].child[k];
-    parent[i] = nodes[i].parent;
-    cn[i] = nodes[i].color_number;
-    nu[i] = nodes[i].number_unique;
-  }
-  char err[kErr] = {0};
-  const int rc = sa_palette_create(&pal.entries[0].r, pal.count, tree.associate_alpha() ? 1 : 0,
-                                   child.data(), parent.data(), cn.data(), nu.data(),
-                                   (int64_t)n, &out->p, err, sizeof(err));
-  if (rc != SA_OK) return prefixed("palette upload", err);
-  out->owned = true;
-  return "";
+      std::memcmp(tree.colormap(), pal.entries, sizeof(PaletteEntry) * pal.count) == 0)
+    return tree.device_palette();
+  return nullptr;
 }
-
 }  // namespace
+
+// RiemersmaWalkCuda, RiemersmaBlocksCuda, RiemersmaApproxCuda, ApproxInverseFilter,
+// CudaAvailable, CudaMaxFrames, CudaAllocPinned/FreePinned: shared with the
+// adapter compiled inside the reference's build (rd_hip_engine.cpp).
+#include "rd_gpu_engines.inc"
 
 // ---- ColorTree ----------------------------------------------------------------
 
@@ -289,71 +216,7 @@ const std::vector<int>* BlockCurveOwner(std::size_t width, std::size_t height) {
   return c.valid ? &c.owner : nullptr;
 }
 
-// ---- runtime ---------------------------------------------------------------------
-
-bool CudaAvailable() { return ensure_device(nullptr); }
-
-int CudaMaxFrames(std::size_t width, std::size_t height, std::size_t vram_budget) {
-  if (!ensure_device(nullptr)) return 0;
-  const size_t px = width * height;
-  if (px == 0) return 0;
-  size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1;
-  const size_t budget = vram_budget ? std::min(vram_budget, free_b) : free_b;
-  // Per frame: the image (16 B/px), the staged inputs (32 B/px), the index
-  // per visit (2 B/px) and the memo table (2^24 entries of 2 B with alpha).
-  const size_t per_frame = px * (16 + 32 + 2) + ((size_t)2 << 24);
-  const size_t fixed = px * 4 + ((size_t)64 << 20);  // curve + slack
-  if (budget <= fixed) return 0;
-  return (int)std::min<size_t>((budget - fixed) / per_frame, 1 << 20);
-}
-
-void* CudaAllocPinned(std::size_t bytes) {
-  if (!ensure_device(nullptr)) return nullptr;
-  void* p = nullptr;
-  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
-  return p;
-}
-
-void CudaFreePinned(void* p) {
-  if (p) (void)hipHostFree(p);
-}
-
 // ---- engines -------------------------------------------------------------------------
-
-std::string RiemersmaWalkCuda(const Palette& palette, const DitherParams& params,
-                              const ColorTree& tree, std::size_t width, std::size_t height,
-                              RgbaF* pixels, int frames, std::string* device_name) {
-  std::string why;
-  if (!ensure_device(&why)) return why;
-  if (frames < 1) frames = 1;
-  fill_name(device_name);
-  const size_t px = width * height;
-  if (px == 0) return "";
-  PaletteRef pr;
-  std::string e = palette_for(palette, tree, &pr);
-  if (!e.empty()) return e;
-  // `pixels` holds ONE frame: the reference uploads frame 0 as the shared
-  // source of its `frames` walks and downloads one frame
-  // (src/rd_riemersma_cuda.cu:688, :747).  Those walks read the same source
-  // with the same palette and state, so they are identical, and one walk
-  // yields the frame it returns.  Distinct images walk concurrently through
-  // RiemersmaWalkBatch.
-  const size_t bytes = sizeof(RgbaF) * px;
-  DevBuf d;
-  if (!d.reserve(bytes)) return "device allocation failed";
-  hipError_t he = hipMemcpy(d.p, pixels, bytes, hipMemcpyHostToDevice);
-  if (he != hipSuccess) return hip_fail("upload", he);
-  const sa_image item{d.p, d.p, (int64_t)width, (int64_t)height, SA_PIX_RGBAF32,
-                      SA_PIX_RGBAF32, pr.p};
-  sa_seq_options o{params.diffusion, params.use_cache ? 1 : 0, 0, 0, 0};
-  char err[kErr] = {0};
-  if (sa_walk_seq_batch(&item, 1, &o, nullptr, err, sizeof(err)) != SA_OK)
-    return prefixed("RiemersmaWalkCuda", err);
-  he = hipMemcpy(pixels, d.p, bytes, hipMemcpyDeviceToHost);
-  if (he != hipSuccess) return hip_fail("download", he);
-  return "";
-}
 
 std::string RiemersmaWalkBatch(const std::vector<WalkItem>& items, const DitherParams& params,
                                std::string* device_name) {
@@ -393,127 +256,6 @@ std::string RiemersmaWalkBatch(const std::vector<WalkItem>& items, const DitherP
                               hipMemcpyDeviceToHost);
     if (he != hipSuccess) return hip_fail("download", he);
   }
-  return "";
-}
-
-std::vector<double> ApproxInverseFilter(int taps, double diffusion) {
-  std::vector<double> g;
-  if (taps < 1 || taps > 4096) return g;
-  g.resize((size_t)taps + 1);
-  char err[kErr] = {0};
-  if (sa_inverse_filter(taps, diffusion, g.data(), err, sizeof(err)) != SA_OK) g.clear();
-  return g;
-}
-
-std::string RiemersmaApproxCuda(const Palette& palette, const DitherParams& params,
-                                const ColorTree& tree, std::size_t width, std::size_t height,
-                                RgbaF* pixels, const ApproxOptions& options,
-                                std::string* device_name) {
-  std::string why;
-  if (!ensure_device(&why)) return why;
-  fill_name(device_name);
-  const size_t px = width * height;
-  if (px == 0) return "";
-  PaletteRef pr;
-  std::string e = palette_for(palette, tree, &pr);
-  if (!e.empty()) return e;
-  const size_t bytes = sizeof(RgbaF) * px;
-  DevBuf src, dst;
-  if (!src.reserve(bytes) || !dst.reserve(bytes)) return "device allocation failed";
-  hipError_t he = hipMemcpy(src.p, pixels, bytes, hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMemcpy(dst.p, src.p, bytes, hipMemcpyDeviceToDevice);
-  if (he != hipSuccess) return hip_fail("upload", he);
-  sa_image im{src.p, dst.p, (int64_t)width, (int64_t)height, SA_PIX_RGBAF32, SA_PIX_RGBAF32,
-              pr.p};
-  char err[kErr] = {0};
-  if (sa_walk_approx(&im, 1, options.iterations, options.taps, options.fp64 ? 1 : 0,
-                     options.use_tree ? 1 : 0, params.diffusion, nullptr, err,
-                     sizeof(err)) != SA_OK)
-    return prefixed("RiemersmaApproxCuda", err);
-  he = hipMemcpy(pixels, dst.p, bytes, hipMemcpyDeviceToHost);
-  if (he != hipSuccess) return hip_fail("download", he);
-  return "";
-}
-
-namespace {
-struct BlockSlot {
-  std::mutex mu;
-  DevBuf src, dst;
-};
-BlockSlot g_slots[kGpuStateSlots];
-}  // namespace
-
-std::string RiemersmaBlocksCuda(const Palette& palette, const DitherParams& params,
-                                const ColorTree& tree, std::size_t width, std::size_t height,
-                                RgbaF* batch, const BlockOptions& options,
-                                std::string* device_name, std::uint16_t* out_u16, int state_slot,
-                                const std::uint16_t* in_u16) {
-  std::string why;
-  if (!ensure_device(&why)) return why;
-  fill_name(device_name);
-  if (options.frames < 1) return "RiemersmaBlocksCuda: frames must be >= 1";
-  if (state_slot < 0 || state_slot >= kGpuStateSlots)
-    return "RiemersmaBlocksCuda: state_slot out of range";
-  // The planar modes pass 8-bit planes through the uint16 pointers, as the
-  // reference does (src/rd_blocks_cuda.cu:1066-1081, :1532-1545).
-  const bool in_planar = options.upload_u16 && options.in_mode != BlockOptions::InMode::Interleaved16;
-  if (options.upload_u16 && !in_u16) return "RiemersmaBlocksCuda: upload_u16 needs in_u16";
-  if (options.upload_u16 && !in_planar && options.in_channels != 3 && options.in_channels != 4)
-    return "RiemersmaBlocksCuda: upload_u16 needs in_u16 with 3 or 4 channels";
-  if (options.emit_u16 && !out_u16) return "RiemersmaBlocksCuda: emit_u16 needs out_u16";
-  if (!options.upload_u16 && !batch) return "RiemersmaBlocksCuda: null batch";
-  const size_t px = width * height;
-  if (px == 0) return "";
-  PaletteRef pr;
-  std::string e = palette_for(palette, tree, &pr);
-  if (!e.empty()) return e;
-  BlockSlot& slot = g_slots[state_slot];
-  std::lock_guard<std::mutex> lock(slot.mu);
-  const int frames = options.frames;
-  // Bytes per frame of the source and the result.
-  int in_fmt = SA_PIX_RGBAF32;
-  size_t in_frame = sizeof(RgbaF) * px;
-  if (options.upload_u16) {
-    switch (options.in_mode) {
-      case BlockOptions::InMode::PlanarYuv444:
-      case BlockOptions::InMode::PlanarYuv444Prepass:  // the same values, widened in one pass
-        in_fmt = SA_PIX_YUV444P8;
-        in_frame = 3 * px;
-        break;
-      case BlockOptions::InMode::PlanarYuv420:
-        in_fmt = SA_PIX_YUV420P8;
-        in_frame = px + 2 * (width / 2) * (height / 2);
-        break;
-      default:
-        in_fmt = options.in_channels == 3 ? SA_PIX_RGB16 : SA_PIX_RGBA16;
-        in_frame = 2 * (size_t)options.in_channels * px;
-        break;
-    }
-  }
-  int out_fmt = SA_PIX_RGBAF32;
-  size_t out_frame = sizeof(RgbaF) * px;
-  if (options.emit_u16) {
-    out_fmt = options.emit_yuv444 ? SA_PIX_YUV444P8 : SA_PIX_RGBA16;
-    out_frame = options.emit_yuv444 ? 3 * px : 8 * px;
-  }
-  if (!slot.src.reserve(in_frame * frames) || !slot.dst.reserve(out_frame * frames))
-    return "device allocation failed";
-  const void* host_in = options.upload_u16 ? (const void*)in_u16 : (const void*)batch;
-  hipError_t he = hipMemcpy(slot.src.p, host_in, in_frame * frames, hipMemcpyHostToDevice);
-  if (he != hipSuccess) return hip_fail("upload", he);
-  std::vector<sa_image> fr(frames);
-  for (int f = 0; f < frames; ++f)
-    fr[f] = sa_image{(unsigned char*)slot.src.p + in_frame * f,
-                     (unsigned char*)slot.dst.p + out_frame * f, (int64_t)width,
-                     (int64_t)height, in_fmt, out_fmt, pr.p};
-  char err[kErr] = {0};
-  if (sa_walk_blocks(fr.data(), frames, options.block, params.diffusion, nullptr, err,
-                     sizeof(err)) != SA_OK)
-    return prefixed("RiemersmaBlocksCuda", err);
-  void* host_out = options.emit_u16 ? (void*)out_u16 : (void*)batch;
-  if (!host_out) return "RiemersmaBlocksCuda: nowhere to write the result";
-  he = hipMemcpy(host_out, slot.dst.p, out_frame * frames, hipMemcpyDeviceToHost);
-  if (he != hipSuccess) return hip_fail("download", he);
   return "";
 }
 

@@ -61,3 +61,40 @@ def test_dropin_engines_match_the_oracle(gpu):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "OK: 0 failure(s)" in r.stdout
+
+
+# ---- the reference-side adapter (stipple_amd/host/rd_hip_engine.cpp) -------------------
+REFERENCE = "/root/reference"
+ADAPTER = os.path.join(ROOT, "oracle", "_ref", "adapter_test")
+# What the reference's CUDA translation units and their stub define for the rest
+# of the reference (src/rd_riemersma_cuda.cu:460-554, src/rd_blocks_cuda.cu:1006,
+# :1720-1726, src/rd_approx_cuda.cu:232, :386, src/rd_cuda_stub.cpp:50-95).
+ADAPTER_DEFINES = ["rd::CudaAvailable()", "rd::CudaMaxFrames(", "rd::RiemersmaWalkCuda(",
+                   "rd::RiemersmaBlocksCuda(", "rd::CudaAllocPinned(", "rd::CudaFreePinned(",
+                   "rd::RiemersmaApproxCuda(", "rd::ApproxInverseFilter("]
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "src")),
+                    reason="the reference checkout is not present")
+def test_reference_adapter_links_into_the_reference_build():
+    """The adapter compiles against the reference's own headers and links with its
+    ColorTree and CPU engines, with nothing else from the reference's CUDA files."""
+    subprocess.run(["make", "-s", "adapter"], cwd=os.path.join(ROOT, "oracle"), check=True)
+    assert os.path.exists(ADAPTER)
+    syms = subprocess.run(["nm", "-C", "--defined-only", ADAPTER], check=True,
+                          capture_output=True, text=True).stdout
+    missing = [s for s in ADAPTER_DEFINES if s not in syms]
+    assert not missing, missing
+    # ... and the reference's own CPU engines are the ones linked beside it
+    assert "rd::RiemersmaWalkCpu(" in syms and "rd::RiemersmaBlocksCpu(" in syms
+    assert "rd::ColorTree::Build(" in syms
+
+
+@pytest.mark.gpu
+def test_reference_adapter_matches_the_reference_cpu_engines(gpu):
+    if not os.path.exists(ADAPTER):
+        pytest.skip("oracle/_ref/adapter_test is not built (no reference checkout at build time)")
+    r = subprocess.run([ADAPTER], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "OK: 0 failure(s)" in r.stdout
