@@ -121,6 +121,52 @@ typedef struct {
 int sa_palette_build_batch(const sa_build* items, int n, void* stream, sa_palette* out,
                            int* status, char* err, size_t errlen);
 
+/* ---- video palette: montage of sampled frames -----------------------------
+ * VideoBuildPalette's montage path (src/rd_video.cpp:1816, :2080-2700): the
+ * `n` sampled frames (device pointers, one geometry width x height, src_fmt
+ * SA_PIX_RGB8 / RGBA8 / RGB16 / RGBA16 -- the decoder's rgb24 / rgba /
+ * rgba64le) are placed in sample order in the cells of a grid `cols` cells
+ * wide (0: ceil(sqrt(want)), the reference's default; `want` cells in all,
+ * rows = ceil(want / cols)).  A cell is the whole frame (tile <= 0) or a
+ * tile x tile nearest-neighbour sample of it; cells past the last sample repeat
+ * it.  8-bit samples widen x257, 3-channel sources get alpha 65535.
+ *
+ * sa_video_montage writes that montage (device, *mw x *mh RGBA16, exactly the
+ * reference's RgbaF montage) and sets *gray when every pixel has r == g == b;
+ * with montage == NULL it only reports the size.  Synchronises `stream` when
+ * gray is non-null.  Montages above 2^28 pixels are refused (the reference's
+ * own montage cap is 64 Mpixel: reduce `want` to fit before sampling). */
+int sa_video_montage(const void* const* frames, int n, int src_fmt, int64_t width,
+                     int64_t height, int want, int tile, int cols, void* montage, int64_t* mw,
+                     int64_t* mh, int* gray, void* stream, char* err, size_t errlen);
+
+/* The palette of that montage: ColorTree::Build at `colors`, alpha not
+ * associated, greyscale as detected on the montage -- the tree that
+ * ImBuildPaletteFromPixels builds (src/rd_im.cpp:535-656); bit-exact with
+ * rd::ColorTree::Build on the montage.  (HarvestColormap's removal of exact
+ * duplicate colormap entries, src/rd_im.cpp:555-571, is applied by the C++
+ * layer's ImBuildPaletteFromPixels, not here.) */
+int sa_video_palette(const void* const* frames, int n, int src_fmt, int64_t width,
+                     int64_t height, int want, int tile, int cols, int colors, int keep_tree,
+                     void* stream, sa_palette* out, char* err, size_t errlen);
+
+/* ---- palette text files -----------------------------------------------------
+ * The .txt / .text form of ImWritePalette / ImReadPalette (include/rd_im.h:
+ * 128-186; src/rd_im.cpp:994-1024, :1175-1330).  Host only.  Write: `count`
+ * entries (4 doubles each) as "index r g b a RRGGBB" rows at %.6f after two
+ * comment lines.  Read: `entries` needs room for 65536 x 4 doubles; rows may
+ * omit alpha (opaque); indices must be contiguous from 0; a positive
+ * expect_colors must equal the file's count (SA_EREFUSED otherwise); r, g, b
+ * come back narrowed through float and the entries sorted by (r, g, b, a);
+ * *assoc is 1 when some alpha is not 65535.  The reference's importer then
+ * builds its tree from those entries as a count x 1 strip at `count` colours
+ * and adopts the tree's colormap (the C++ layer's ImReadPalette does that on
+ * the device).  Other file types (image palettes) return SA_EUNSUPPORTED. */
+int sa_palette_write_txt(const double* entries, int count, const char* path, char* err,
+                         size_t errlen);
+int sa_palette_read_txt(const char* path, int expect_colors, double* entries, int* count,
+                        int* assoc, char* err, size_t errlen);
+
 /* Palette facts: colours, association, tree depth, node count (ColorTree::
  * color_count / depth / node_count, include/rd_octree.h:65-99) and whether the
  * root's quantize_error was summed serially (1) or bounded (0). */

@@ -1379,3 +1379,61 @@ extern "C" int orc_jacobi_sweep(const OrcTree* t, const void* cx, int64_t n, con
   }
   return 0;
 }
+
+// ===========================================================================
+// Video palette montage.  src/rd_video.cpp:2373-2380 (grid), :2404-2418
+// (widen), :2441-2470 (place_raw), :2636-2660 (cells past the last sample
+// repeat it), restated on the CPU.  Frames are the decoder's bytes: fmt 1
+// rgb24, 2 rgba, 3 rgb48, 4 rgba64le.  out: mw*mh RgbaF.
+// ===========================================================================
+namespace {
+void widen_px(int fmt, const void* frame, int64_t o, float* d) {
+  if (fmt == 1 || fmt == 2) {
+    const int ch = fmt == 1 ? 3 : 4;
+    const uint8_t* p = (const uint8_t*)frame + ch * o;
+    d[0] = (float)p[0] * 257.0f;
+    d[1] = (float)p[1] * 257.0f;
+    d[2] = (float)p[2] * 257.0f;
+    d[3] = ch == 4 ? (float)p[3] * 257.0f : 65535.0f;
+  } else {
+    const int ch = fmt == 3 ? 3 : 4;
+    const uint16_t* p = (const uint16_t*)frame + ch * o;
+    d[0] = (float)p[0];
+    d[1] = (float)p[1];
+    d[2] = (float)p[2];
+    d[3] = ch == 4 ? (float)p[3] : 65535.0f;
+  }
+}
+}  // namespace
+
+extern "C" int orc_video_montage(const void* const* frames, int sampled, int fmt, int64_t w,
+                                 int64_t h, int want, int tile, int cols, float* out,
+                                 int64_t* mw_out, int64_t* mh_out) {
+  if (sampled < 1 || want < sampled || w < 1 || h < 1) return -1;
+  const int64_t cell_w = tile > 0 ? tile : w, cell_h = tile > 0 ? tile : h;
+  const int c = cols > 0 ? cols : (int)std::ceil(std::sqrt((double)want));
+  const int rows = (want + c - 1) / c;
+  const int64_t mw = (int64_t)c * cell_w, mh = (int64_t)rows * cell_h;
+  if (mw_out) *mw_out = mw;
+  if (mh_out) *mh_out = mh;
+  if (!out) return 0;
+  for (int s = 0; s < sampled; ++s) {  // place_raw
+    const int64_t cx = s % c, cy = s / c;
+    for (int64_t ty = 0; ty < cell_h; ++ty) {
+      const int64_t sy = tile > 0 ? ty * h / cell_h : ty;
+      float* row = out + 4 * ((cy * cell_h + ty) * mw + cx * cell_w);
+      for (int64_t tx = 0; tx < cell_w; ++tx) {
+        const int64_t sx = tile > 0 ? tx * w / cell_w : tx;
+        widen_px(fmt, frames[s], sy * w + sx, row + 4 * tx);
+      }
+    }
+  }
+  const int64_t lx = (sampled - 1) % c, ly = (sampled - 1) / c;
+  for (int s = sampled; s < c * rows; ++s) {  // repeat the last sample
+    const int64_t cx = s % c, cy = s / c;
+    for (int64_t ty = 0; ty < cell_h; ++ty)
+      std::memcpy(out + 4 * ((cy * cell_h + ty) * mw + cx * cell_w),
+                  out + 4 * ((ly * cell_h + ty) * mw + lx * cell_w), 16 * (size_t)cell_w);
+  }
+  return 0;
+}

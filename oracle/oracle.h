@@ -129,6 +129,15 @@ int orc_jacobi_sweep(const OrcTree* t, const void* cx, int64_t n, const uint8_t*
                      uint8_t* q_out, double* t_out, int taps, int fp64, int use_tree,
                      double diffusion, int mutant);
 
+/* The palette montage of VideoBuildPalette (src/rd_video.cpp:2373-2470,
+ * :2636-2660): `sampled` frames of the decoder's bytes (fmt 1 rgb24, 2 rgba,
+ * 3 rgb48, 4 rgba64le) placed in a grid `cols` wide (0: ceil(sqrt(want))),
+ * whole (tile <= 0) or tile x tile nearest-neighbour cells, cells past the
+ * last sample repeating it.  out (mw*mh RgbaF) may be null to query the size.
+ * Returns 0 or -1. */
+int orc_video_montage(const void* const* frames, int sampled, int fmt, int64_t w, int64_t h,
+                      int want, int tile, int cols, float* out, int64_t* mw, int64_t* mh);
+
 /* Thread-pool CPU baseline: walks `n` independent images (each with its own
  * tree) with `threads` std::threads, largest first.  Returns 0. */
 int orc_walk_seq_many(OrcTree* const* trees, float* const* images,

@@ -105,6 +105,8 @@ class Oracle(_Lib):
         self._screen = f("dither_screen", _i32, _p, _i32, _p, _i64, _i64, _i32, _i32, _dbl)
         self._inv = f("inverse_filter", None, _i32, _dbl, _p)
         self._jac = f("jacobi", _i32, _p, _p, _i64, _i64, _i32, _i32, _i32, _i32, _dbl, _p)
+        self._montage = f("video_montage", _i32, _p, _i32, _i32, _i64, _i64, _i32, _i32, _i32,
+                          _p, _p, _p)
         self._jinit = f("jacobi_init", _i32, _p, _p, _i64, _i32, _i32, _p)
         self._jsweep = f("jacobi_sweep", _i32, _p, _p, _i64, _p, _p, _p, _i32, _i32, _i32, _dbl,
                          _i32)
@@ -250,6 +252,22 @@ class Oracle(_Lib):
         if rc != 0:
             raise ValueError("refused")
         return (a, q[:n]) if want_q else a
+
+    def video_montage(self, frames, want=None, tile=0, cols=0):
+        """The palette montage (RgbaF, mh x mw x 4) of decoder frames: uint8 or
+        uint16 arrays of h x w x 3 or 4."""
+        frames = [np.ascontiguousarray(f) for f in frames]
+        h, w, ch = frames[0].shape
+        fmt = (1 if ch == 3 else 2) if frames[0].dtype == np.uint8 else (3 if ch == 3 else 4)
+        want = want or len(frames)
+        ptrs = (C.c_void_p * len(frames))(*[f.ctypes.data for f in frames])
+        mw, mh = C.c_int64(), C.c_int64()
+        if self._montage(ptrs, len(frames), fmt, w, h, want, tile, cols, None, C.byref(mw),
+                         C.byref(mh)) != 0:
+            raise ValueError("refused")
+        out = np.zeros((mh.value, mw.value, 4), np.float32)
+        self._montage(ptrs, len(frames), fmt, w, h, want, tile, cols, _ptr(out), None, None)
+        return out
 
     def jacobi_gather(self, img, fp64=False):
         """GatherKernelT (src/rd_approx_cuda.cu:110-121): the pixels in block
@@ -435,6 +453,22 @@ class Ref(_Lib):
         out = np.zeros((frames, h, w, 4), np.uint16)
         if self._blocks16(tree.h, _ptr(a), w, h, frames, block, diffusion, _ptr(out)) != 0:
             raise ValueError("refused")
+        return out
+
+    def video_montage(self, frames, want=None, tile=0, cols=0):
+        """The palette montage (RgbaF, mh x mw x 4) of decoder frames: uint8 or
+        uint16 arrays of h x w x 3 or 4."""
+        frames = [np.ascontiguousarray(f) for f in frames]
+        h, w, ch = frames[0].shape
+        fmt = (1 if ch == 3 else 2) if frames[0].dtype == np.uint8 else (3 if ch == 3 else 4)
+        want = want or len(frames)
+        ptrs = (C.c_void_p * len(frames))(*[f.ctypes.data for f in frames])
+        mw, mh = C.c_int64(), C.c_int64()
+        if self._montage(ptrs, len(frames), fmt, w, h, want, tile, cols, None, C.byref(mw),
+                         C.byref(mh)) != 0:
+            raise ValueError("refused")
+        out = np.zeros((mh.value, mw.value, 4), np.float32)
+        self._montage(ptrs, len(frames), fmt, w, h, want, tile, cols, _ptr(out), None, None)
         return out
 
     def jacobi_gather(self, img, fp64=False):

@@ -52,6 +52,9 @@ from ._lib import (  # noqa: F401
     walk_blocks,
     walk_blocks_noalloc,
     walk_seq_batch,
+    video_montage,
+    write_palette_txt,
+    read_palette_txt,
 )
 
 __all__ = [
@@ -73,4 +76,7 @@ __all__ = [
     "walk_blocks",
     "walk_blocks_noalloc",
     "walk_seq_batch",
+    "video_montage",
+    "write_palette_txt",
+    "read_palette_txt",
 ]

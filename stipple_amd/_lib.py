@@ -89,6 +89,14 @@ def lib():
                 "sa_walk_seq_batch": (i32, [C.POINTER(_Image), i32, C.POINTER(SeqOptions), vp, cp, sz]),
                 "sa_walk_blocks": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_walk_blocks_noalloc": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
+                "sa_video_montage": (i32, [vp, i32, i32, i64, i64, i32, i32, i32, vp,
+                                           C.POINTER(i64), C.POINTER(i64), C.POINTER(i32), vp,
+                                           cp, sz]),
+                "sa_video_palette": (i32, [vp, i32, i32, i64, i64, i32, i32, i32, i32, i32, vp,
+                                           C.POINTER(vp), cp, sz]),
+                "sa_palette_write_txt": (i32, [vp, i32, cp, cp, sz]),
+                "sa_palette_read_txt": (i32, [cp, i32, vp, C.POINTER(i32), C.POINTER(i32), cp,
+                                              sz]),
                 "sa_walk_approx": (i32, [C.POINTER(_Image), i32, i32, i32, i32, i32, dbl, vp, cp, sz]),
                 "sa_dither_screen": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_seq_stats": (i32, [vp, cp, sz]),
@@ -225,6 +233,41 @@ class Palette:
             _check(rc, e)
         return pals
 
+    @classmethod
+    def from_video(cls, frames, width, height, src_fmt, colors, want=None, tile=0, cols=0,
+                   keep_tree=False, stream=None):
+        """VideoBuildPalette's montage palette (sa_video_palette): `frames` are
+        the sampled device frames in order, `want` the montage's cell count
+        (default: len(frames))."""
+        ptrs, keep = _frame_ptrs(frames)
+        h = C.c_void_p()
+        e = _err()
+        rc = lib().sa_video_palette(ptrs, len(frames), src_fmt, width, height,
+                                    want or len(frames), tile, cols, colors, int(bool(keep_tree)),
+                                    _stream_ptr(stream), C.byref(h), e, len(e))
+        _check(rc, e)
+        return cls._built(h.value, False)
+
+    @classmethod
+    def import_txt(cls, path, expect_colors=0, keep_tree=False):
+        """ImReadPalette's .txt form: the file's entries (narrowed, canonically
+        sorted) as a count x 1 strip, its tree built on the device at `count`
+        colours, the tree's colormap adopted.  Raises EngineError when the
+        tree reduces the entries (duplicate or unreachable colours)."""
+        import torch
+        ent, _assoc = read_palette_txt(path, expect_colors)
+        n = ent.shape[0]
+        strip = torch.from_numpy(ent.astype(np.float32).reshape(-1).view(np.uint8).copy()).cuda()
+        gray = bool(np.all((ent[:, 0] == ent[:, 1]) & (ent[:, 1] == ent[:, 2])))
+        pal = cls.build(strip, n, 1, PIX_RGBAF32, n, False, gray, keep_tree)
+        torch.cuda.synchronize()
+        if pal.count != n:
+            got = pal.count
+            pal.close()
+            raise EngineError(SA_EINVAL, f"{path}: the tree reduced {n} colours to {got}; the "
+                                         "palette holds duplicate or unreachable entries")
+        return pal
+
     @property
     def entries(self):
         if self._entries is None:
@@ -280,6 +323,47 @@ class Palette:
             self.close()
         except Exception:
             pass
+
+
+def _frame_ptrs(frames):
+    arr = (C.c_void_p * max(len(frames), 1))(*[_dev_ptr(f) for f in frames])
+    return arr, frames
+
+
+def video_montage(frames, width, height, src_fmt, want=None, tile=0, cols=0, stream=None):
+    """The palette montage of sampled device frames (sa_video_montage): returns
+    (montage as an RGBA16 device tensor of mh x mw x 4 uint16 bytes, mw, mh,
+    gray)."""
+    import torch
+    want = want or len(frames)
+    mw, mh = C.c_int64(), C.c_int64()
+    e = _err()
+    ptrs, _keep = _frame_ptrs(frames)
+    _check(lib().sa_video_montage(ptrs, len(frames), src_fmt, width, height, want, tile, cols,
+                                  None, C.byref(mw), C.byref(mh), None, None, e, len(e)), e)
+    out = torch.empty(8 * mw.value * mh.value, dtype=torch.uint8, device="cuda")
+    gray = C.c_int()
+    _check(lib().sa_video_montage(ptrs, len(frames), src_fmt, width, height, want, tile, cols,
+                                  _dev_ptr(out), C.byref(mw), C.byref(mh), C.byref(gray),
+                                  _stream_ptr(stream), e, len(e)), e)
+    return out, mw.value, mh.value, bool(gray.value)
+
+
+def write_palette_txt(entries, path):
+    """ImWritePalette's .txt form (sa_palette_write_txt)."""
+    ent = np.ascontiguousarray(entries, np.float64).reshape(-1, 4)
+    e = _err()
+    _check(lib().sa_palette_write_txt(_np_ptr(ent), ent.shape[0], os.fsencode(path), e, len(e)), e)
+
+
+def read_palette_txt(path, expect_colors=0):
+    """ImReadPalette's .txt parse (sa_palette_read_txt): (entries n x 4, assoc)."""
+    buf = np.zeros((65536, 4), np.float64)
+    n, a = C.c_int(), C.c_int()
+    e = _err()
+    _check(lib().sa_palette_read_txt(os.fsencode(path), expect_colors, _np_ptr(buf), C.byref(n),
+                                     C.byref(a), e, len(e)), e)
+    return buf[:n.value].copy(), bool(a.value)
 
 
 def _dev_ptr(t):

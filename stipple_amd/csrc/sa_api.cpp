@@ -525,6 +525,98 @@ extern "C" int sa_palette_build(const void* src, int src_fmt, int64_t w, int64_t
   return SA_OK;
 }
 
+// ---- video palette montage ----------------------------------------------------------
+namespace {
+// The montage geometry of src/rd_video.cpp:2373-2380 (cols: ceil(sqrt(want))
+// unless given).
+int montage_shape(int64_t w, int64_t h, int want, int tile, int cols, int64_t* cell_w,
+                  int64_t* cell_h, int* cols_out, int64_t* mw, int64_t* mh) {
+  if (w < 1 || h < 1 || want < 1) return SA_EINVAL;
+  *cell_w = tile > 0 ? tile : w;
+  *cell_h = tile > 0 ? tile : h;
+  int c = cols > 0 ? cols : (int)std::ceil(std::sqrt((double)want));
+  if (c < 1) c = 1;
+  const int rows = (want + c - 1) / c;
+  *cols_out = c;
+  *mw = (int64_t)c * *cell_w;
+  *mh = (int64_t)rows * *cell_h;
+  return SA_OK;
+}
+}  // namespace
+
+extern "C" int sa_video_montage(const void* const* frames, int n, int src_fmt, int64_t width,
+                                int64_t height, int want, int tile, int cols, void* montage,
+                                int64_t* mw, int64_t* mh, int* gray, void* stream, char* err,
+                                size_t errlen) {
+  int64_t cw = 0, ch = 0, W = 0, H = 0;
+  int c = 0;
+  if (montage_shape(width, height, want, tile, cols, &cw, &ch, &c, &W, &H) != SA_OK)
+    return fail(err, errlen, SA_EINVAL, "bad montage geometry (%lldx%lld, want %d)",
+                (long long)width, (long long)height, want);
+  if (mw) *mw = W;
+  if (mh) *mh = H;
+  if (!montage) return SA_OK;  // a size query
+  if (n < 1 || n > want || !frames)
+    return fail(err, errlen, SA_EINVAL, "montage: %d sampled frames for %d cells", n, want);
+  if (src_fmt != sa::kRgb8 && src_fmt != sa::kRgba8 && src_fmt != sa::kRgb16 &&
+      src_fmt != sa::kRgba16)
+    return fail(err, errlen, SA_EINVAL, "montage: frames must be RGB8, RGBA8, RGB16 or RGBA16");
+  if (W * H > ((int64_t)1 << 28))
+    return fail(err, errlen, SA_EUNSUPPORTED, "montage of %lld pixels: above 2^28",
+                (long long)(W * H));
+  for (int i = 0; i < n; ++i)
+    if (!frames[i]) return fail(err, errlen, SA_EINVAL, "montage: frame %d is null", i);
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char* d = nullptr;
+  const size_t ptr_bytes = (sizeof(void*) * (size_t)n + 255) & ~(size_t)255;
+  HIP_TRY(hipMallocAsync((void**)&d, ptr_bytes + 256, s));
+  HIP_TRY(staged_copy(d, frames, sizeof(void*) * (size_t)n, s));
+  int* d_gray = (int*)(d + ptr_bytes);
+  HIP_TRY(sa::launch_video_montage((const void* const*)d, n, src_fmt, width, height, cw, ch, c, W,
+                                   H, montage, d_gray, s));
+  if (gray) {
+    int* h_gray = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&h_gray, sizeof(int), hipHostMallocDefault));
+    hipError_t e = hipMemcpyAsync(h_gray, d_gray, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    *gray = *h_gray;
+    (void)hipHostFree(h_gray);
+    if (e != hipSuccess) {
+      (void)hipFreeAsync(d, s);
+      return fail(err, errlen, SA_EDEVICE, "montage: %s", hipGetErrorString(e));
+    }
+  }
+  HIP_TRY(hipFreeAsync(d, s));
+  return SA_OK;
+}
+
+extern "C" int sa_video_palette(const void* const* frames, int n, int src_fmt, int64_t width,
+                                int64_t height, int want, int tile, int cols, int colors,
+                                int keep_tree, void* stream, sa_palette* out, char* err,
+                                size_t errlen) {
+  if (!out) return fail(err, errlen, SA_EINVAL, "null output handle");
+  *out = nullptr;
+  int64_t W = 0, H = 0;
+  int rc = sa_video_montage(frames, n, src_fmt, width, height, want, tile, cols, nullptr, &W, &H,
+                            nullptr, stream, err, errlen);
+  if (rc) return rc;
+  rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  void* m = nullptr;
+  HIP_TRY(hipMallocAsync(&m, (size_t)(8 * W * H), s));
+  int gray = 0;
+  rc = sa_video_montage(frames, n, src_fmt, width, height, want, tile, cols, m, &W, &H, &gray,
+                        stream, err, errlen);
+  if (rc == SA_OK)
+    rc = sa_palette_build(m, sa::kRgba16, W, H, colors, 0, gray, keep_tree, stream, out, err,
+                          errlen);
+  HIP_TRY(hipFreeAsync(m, s));
+  return rc;
+}
+
 extern "C" int sa_palette_build_batch(const sa_build* items, int n, void* stream, sa_palette* out,
                                       int* status, char* err, size_t errlen) {
   if (n < 0 || (n > 0 && (!items || !out))) return fail(err, errlen, SA_EINVAL, "bad build list");

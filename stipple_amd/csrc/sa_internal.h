@@ -210,6 +210,12 @@ hipError_t launch_curve_emit(int level, int64_t w, int64_t h, const int32_t* chu
                              int64_t chunks, int32_t* order, hipStream_t s);
 int64_t curve_chunks(int level);
 
+// sa_video.hip: the palette montage of `sampled` frames (device pointer array)
+// as RGBA16, and whether every montage pixel is grey (*gray, device int).
+hipError_t launch_video_montage(const void* const* frames, int sampled, int src_fmt, int64_t w,
+                                int64_t h, int64_t cell_w, int64_t cell_h, int cols, int64_t mw,
+                                int64_t mh, void* out, int* gray, hipStream_t s);
+
 hipError_t launch_seq_gather(const SeqItem* items, int nitems, int64_t max_n, hipStream_t s);
 // Sets every item's state->t to 0.
 hipError_t launch_seq_state_init(const SeqItem* items, int nitems, hipStream_t s);

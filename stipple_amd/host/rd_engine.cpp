@@ -19,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "rd_im.h"
 #include "rd_octree.h"
 #include "rd_plugin.h"
 #include "rd_riemersma.h"
@@ -257,6 +258,79 @@ std::string RiemersmaWalkBatch(const std::vector<WalkItem>& items, const DitherP
     if (he != hipSuccess) return hip_fail("download", he);
   }
   return "";
+}
+
+// ---- palette files and the pixel-buffer palette ------------------------------------------
+
+bool ImWritePalette(const Palette& palette, const std::string& path, std::string* error) {
+  char err[kErr] = {0};
+  if (sa_palette_write_txt(&palette.entries[0].r, palette.count, path.c_str(), err,
+                           sizeof(err)) != SA_OK) {
+    if (error) *error = err;
+    return false;
+  }
+  return true;
+}
+
+bool ImReadPalette(const std::string& path, int expect_colors, Palette* palette,
+                   ColorTree* tree, std::string* error) {
+  char err[kErr] = {0};
+  int count = 0, assoc = 0;
+  if (sa_palette_read_txt(path.c_str(), expect_colors, &palette->entries[0].r, &count, &assoc,
+                          err, sizeof(err)) != SA_OK) {
+    if (error) *error = err;
+    return false;
+  }
+  palette->count = count;
+  palette->associate_alpha = assoc != 0;
+  // The tree of the palette itself: a count x 1 strip at `count` colours.
+  std::vector<RgbaF> strip((size_t)count);
+  bool gray = true;
+  for (int i = 0; i < count; ++i) {
+    const PaletteEntry& e = palette->entries[i];
+    strip[(size_t)i] = RgbaF{(float)e.r, (float)e.g, (float)e.b, (float)e.a};
+    if (e.r != e.g || e.g != e.b) gray = false;
+  }
+  tree->Build(strip.data(), (size_t)count, 1, count, false, gray);
+  if (!tree->last_error().empty()) {
+    if (error) *error = path + ": " + tree->last_error();
+    return false;
+  }
+  if (tree->color_count() != count) {
+    if (error)
+      *error = path + ": the tree reduced " + std::to_string(count) + " colours to " +
+               std::to_string(tree->color_count()) +
+               "; the palette holds duplicate or unreachable entries";
+    return false;
+  }
+  for (int i = 0; i < count; ++i) palette->entries[i] = tree->colormap()[i];
+  return true;
+}
+
+bool ImBuildPaletteFromPixels(const RgbaF* pixels, std::size_t width, std::size_t height,
+                              int colors, Palette* palette, ColorTree* tree, std::string* error) {
+  bool gray = true;
+  for (size_t i = 0; i < width * height && gray; ++i)
+    gray = pixels[i].r == pixels[i].g && pixels[i].g == pixels[i].b;
+  tree->Build(pixels, width, height, colors, false, gray);
+  if (!tree->last_error().empty() || tree->color_count() <= 0) {
+    if (error) *error = tree->last_error().empty() ? "empty colormap" : tree->last_error();
+    return false;
+  }
+  // HarvestColormap: the colormap with exact (r, g, b) duplicates dropped,
+  // first occurrence kept.
+  palette->associate_alpha = false;
+  int out = 0;
+  for (int i = 0; i < tree->color_count(); ++i) {
+    const PaletteEntry& e = tree->colormap()[i];
+    bool dup = false;
+    for (int k = 0; k < out && !dup; ++k)
+      dup = palette->entries[k].r == e.r && palette->entries[k].g == e.g &&
+            palette->entries[k].b == e.b;
+    if (!dup) palette->entries[out++] = e;
+  }
+  palette->count = out;
+  return true;
 }
 
 // ---- threshold-screen plugins on the device ----------------------------------------------

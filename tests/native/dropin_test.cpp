@@ -8,11 +8,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../oracle/oracle.h"
+#include "rd_im.h"
 #include "rd_plugin.h"
 #include "rd_riemersma.h"
 
@@ -343,6 +345,58 @@ int main() {
     check(rd::FindDither("clustered-dot")->run(empty, nullptr).error ==
               "clustered-dot: empty frame",
           "screen dither refuses an empty frame");
+  }
+
+  // ---- rd_im.h: palette files and the pixel-buffer palette --------------------------
+  {
+    // ImBuildPaletteFromPixels == ColorTree::Build's colormap without exact duplicates.
+    std::vector<rd::RgbaF> im = make_image(120, 70, 5, false);
+    rd::Palette* p2 = new rd::Palette();
+    rd::ColorTree t2;
+    std::string e;
+    const bool ok = rd::ImBuildPaletteFromPixels(im.data(), 120, 70, 12, p2, &t2, &e);
+    OrcTree* o2 = orc_tree_build(&im[0].r, 120, 70, 12, 0, 0);
+    std::vector<double> cm(4 * (size_t)orc_tree_color_count(o2));
+    orc_tree_colormap(o2, cm.data());
+    std::vector<double> uniq;
+    for (size_t i = 0; i < cm.size() / 4; ++i) {
+      bool dup = false;
+      for (size_t k = 0; k < uniq.size() / 4 && !dup; ++k)
+        dup = uniq[4 * k] == cm[4 * i] && uniq[4 * k + 1] == cm[4 * i + 1] &&
+              uniq[4 * k + 2] == cm[4 * i + 2];
+      if (!dup) uniq.insert(uniq.end(), cm.begin() + 4 * i, cm.begin() + 4 * i + 4);
+    }
+    check(ok && p2->count == (int)(uniq.size() / 4) &&
+              same_bytes(&p2->entries[0].r, uniq.data(), 8 * uniq.size()) &&
+              t2.color_count() == orc_tree_color_count(o2),
+          "ImBuildPaletteFromPixels == ColorTree::Build colormap, duplicates removed", e);
+    // ImWritePalette -> ImReadPalette: the tree of the file's entries.
+    const char* tmpdir = std::getenv("TMPDIR");
+    const std::string path = std::string(tmpdir && *tmpdir ? tmpdir : "/tmp") + "/dropin_palette_test.txt";
+    check(rd::ImWritePalette(*p2, path, &e), "ImWritePalette .txt", e);
+    rd::Palette* p3 = new rd::Palette();
+    rd::ColorTree t3;
+    const bool rok = rd::ImReadPalette(path, p2->count, p3, &t3, &e);
+    std::vector<rd::RgbaF> strip((size_t)p3->count);
+    for (int i = 0; i < p3->count; ++i)
+      strip[(size_t)i] = rd::RgbaF{(float)p3->entries[i].r, (float)p3->entries[i].g,
+                                   (float)p3->entries[i].b, (float)p3->entries[i].a};
+    OrcTree* o3 = orc_tree_build(&strip[0].r, p3->count, 1, p3->count, 0, 0);
+    std::vector<double> cm3(4 * (size_t)orc_tree_color_count(o3));
+    orc_tree_colormap(o3, cm3.data());
+    check(rok && p3->count == p2->count && orc_tree_color_count(o3) == p3->count &&
+              same_bytes(&p3->entries[0].r, cm3.data(), 8 * cm3.size()) &&
+              t3.color_count() == p3->count,
+          "ImReadPalette .txt: the strip's tree colormap adopted", e);
+    std::string e2;
+    check(!rd::ImReadPalette(path, p2->count + 1, p3, &t3, &e2) && !e2.empty(),
+          "ImReadPalette refuses a count that disagrees with --colors", e2);
+    check(!rd::ImWritePalette(*p2, "palette.png", &e2), "ImWritePalette refuses the image form");
+    std::remove(path.c_str());
+    orc_tree_free(o2);
+    orc_tree_free(o3);
+    delete p2;
+    delete p3;
   }
 
   orc_tree_free(ot);

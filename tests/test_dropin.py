@@ -25,6 +25,7 @@ EXPECTED = [
     "rd::BlockCurveOrder(", "rd::BlockCurveOwner(", "rd::HilbertPoint(", "rd::CudaAvailable()",
     "rd::CudaMaxFrames(", "rd::CudaAllocPinned(", "rd::CudaFreePinned(", "rd::ColorTree::Build(",
     "rd::RegisterDither(", "rd::FindDither(", "rd::RegisteredDithers()", "rd::DitherListText",
+    "rd::ImWritePalette(", "rd::ImReadPalette(", "rd::ImBuildPaletteFromPixels(",
 ]
 
 
