@@ -75,14 +75,19 @@ def cpu_baseline(batch, threads_list):
     return ("reference" if use_ref else "port"), pixels, out
 
 
-def _traffic(name):
+def _traffic(name, frames=None):
     """HBM bytes per launch measured by rocprofv3 --pmc on the same workload
-    (profiles/<name>.json), or None."""
+    (profiles/<name>.json), or None; with `frames`, scaled from the profiled
+    launch's frame count to this launch's (the bytes per frame are the same)."""
     path = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return round(json.load(fh)["traffic_bytes"])
+        d = json.load(fh)
+    t = d["traffic_bytes"]
+    if frames and d.get("frames"):
+        t = t * frames / d["frames"]
+    return round(t)
 
 
 def _ev_ms(fn, stream=None):
@@ -169,17 +174,24 @@ def run_video(args, sa, torch, synth, dev):
     planar 4:2:0 in and 4:4:4 out."""
     nf = args.frames or 32
     yuv = args.video_format == "yuv420"
+    on_device = getattr(args, "device_gen", False) and not yuv
     params = synth.plasma_params(3)
     t0 = time.perf_counter()
-    frames = [synth.plasma_frame(t, params=params) for t in range(nf)]
+    if on_device:
+        # the same plasma formula made on the device (many frames)
+        srcs = [synth.plasma_device(1920, 1080, t=t, seed=3).reshape(-1) for t in range(nf)]
+        frames = [srcs[0].cpu().numpy().reshape(1080, 1920, 3)]
+    else:
+        frames = [synth.plasma_frame(t, params=params) for t in range(nf)]
     if yuv:
         frames = [synth.rgb8_to_yuv420(f) for f in frames]
     log(f"generated {nf} frames in {time.perf_counter() - t0:.1f}s")
     h, w = 1080, 1920
     fin = sa.PIX_YUV420 if yuv else sa.PIX_RGB8
     fout = sa.PIX_YUV444 if yuv else sa.PIX_RGB8
-    srcs = [torch.from_numpy(f.reshape(-1).copy()).cuda() for f in frames]
-    dsts = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in frames]
+    if not on_device:
+        srcs = [torch.from_numpy(f.reshape(-1).copy()).cuda() for f in frames]
+    dsts = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in srcs]
     torch.cuda.synchronize()
 
     def build():
@@ -261,7 +273,7 @@ def run_video(args, sa, torch, synth, dev):
     achieved = bpp * px * nf / (walk_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": _traffic("blocks_yuv420_traffic" if yuv else "blocks_c3_traffic"),
+            "traffic": _traffic("blocks_yuv420_traffic" if yuv else "blocks_c3_traffic", nf),
             "kernel": "blocks_kernel",
             "note": (f"algorithmic 4.5 B/px (4:2:0 in + 4:4:4 out)" if yuv else
                      f"algorithmic 6 B/px (RGB8 in + out)") +
@@ -271,7 +283,8 @@ def run_video(args, sa, torch, synth, dev):
                 "frame built on the device from the planes, block walk B=64, planar 4:4:4 out"
                 if yuv else
                 f"{nf} 1920x1080 RGB8 plasma frames, a 16-colour octree palette per "
-                "frame built on the device, block walk B=64, RGB8 out")
+                "frame built on the device, block walk B=64, RGB8 out" +
+                (" (frames generated on the device)" if on_device else ""))
     return _line(args, "Mpixel/s and frames/s, 1920x1080 video, block engine B=64",
                  px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f64",
                  {"workload": workload, "frames": nf, "device": dev}, roof, cpu,
@@ -332,7 +345,7 @@ def run_screens(args, sa, torch, synth, dev):
     frames = [synth.plasma_frame(t, params=params) for t in range(nf)]
     h, w = frames[0].shape[:2]
     srcs = [torch.from_numpy(f.reshape(-1).copy()).cuda() for f in frames]
-    dsts = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in frames]
+    dsts = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for _ in srcs]
     pal = sa.Palette.build(srcs[0], w, h, sa.PIX_RGB8, 16, False)
     specs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB8, sa.PIX_RGB8, pal)
              for k in range(nf)]
@@ -381,11 +394,17 @@ def run_approx4k(args, sa, torch, synth, dev):
     frame 0 held on the device, Jacobi 8 sweeps, taps 96, fp32."""
     nf = args.frames or 16
     t0 = time.perf_counter()
-    frames = [synth.radial_frame16(t) for t in range(nf)]
+    if getattr(args, "device_gen", False):
+        # the same radial-gradient formula made on the device (many frames)
+        dev_frames = [synth.radial_device(t) for t in range(nf)]
+        srcs = [f.reshape(-1).view(torch.uint8) for f in dev_frames]
+        frames = [dev_frames[0].cpu().view(torch.int16).numpy().view(np.uint16)]
+    else:
+        frames = [synth.radial_frame16(t) for t in range(nf)]
+        srcs = [torch.from_numpy(f.reshape(-1).view(np.uint8).copy()).cuda() for f in frames]
     log(f"generated {nf} frames in {time.perf_counter() - t0:.1f}s")
     h, w = frames[0].shape[:2]
-    srcs = [torch.from_numpy(f.reshape(-1).view(np.uint8).copy()).cuda() for f in frames]
-    dsts = [torch.empty(w * h * 6, dtype=torch.uint8, device="cuda") for _ in frames]
+    dsts = [torch.empty(w * h * 6, dtype=torch.uint8, device="cuda") for _ in srcs]
     torch.cuda.synchronize()
     pal = sa.Palette.build(srcs[0], w, h, sa.PIX_RGB16, 256, False, False)
     specs = [sa.image_spec(srcs[k], dsts[k], w, h, sa.PIX_RGB16, sa.PIX_RGB16, pal)
@@ -465,7 +484,7 @@ def run_approx4k(args, sa, torch, synth, dev):
     algo = px * nf * (12 + 8 * 8)
     achieved = algo / (ap_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": _traffic("approx_c4_traffic"),
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": _traffic("approx_c4_traffic", nf),
             "kernel": "approx (gather, 8 sweeps, scatter)",
             "note": "algorithmic 12 B/px per frame + 8 B/px per sweep (RGB16 in, index in "
                     "and out); traffic is the measured HBM bytes of the same launch; the sweep "
@@ -475,7 +494,9 @@ def run_approx4k(args, sa, torch, synth, dev):
                  px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f32",
                  {"workload": f"{nf} 3840x2160 RGB16 radial-gradient frames with 5% salt and "
                               "pepper, one 256-colour palette from frame 0 resident on the device, "
-                              "Jacobi 8 sweeps, taps 96, fp32, octree lookup",
+                              "Jacobi 8 sweeps, taps 96, fp32, octree lookup" +
+                              (" (frames generated on the device)"
+                               if getattr(args, "device_gen", False) else ""),
                   "frames": nf, "device": dev}, roof, cpu,
                  {"fps": round(nf / t, 2), "approx_ms": round(ap_ms, 3),
                   "chain": "hipGraph replay of the 16-frame batch (sa_chain_run)", "tuned": tuned,
@@ -648,6 +669,12 @@ def main():
     ap.add_argument("--mix-memo-bits", type=int, default=8,
                     help="serving mix: LDS memo size of the sequential walks (4..10; 0: default)")
     ap.add_argument("--mix-seed", type=int, default=7)
+    ap.add_argument("--engines", type=int, default=1,
+                    help="stills: also measure the block engine on 300 1920x1080 frames and the "
+                         "Jacobi engine on 120 3840x2160 RGB16 frames (frames generated on the "
+                         "device) and report them in the same line (0: skip)")
+    ap.add_argument("--device-gen", action="store_true",
+                    help="video / approx4k: generate the frames on the device")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run a 256 MiB device copy (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -826,6 +853,26 @@ def main():
 
     mix = serving_mix(args, sa, torch, synth) if args.mix_seconds > 0 else None
 
+    # The block and Jacobi engines at their workloads' full sizes.
+    engines = None
+    if args.engines:
+        engines = {}
+        for name, fn, nf in (("block_video_1080p", run_video, 300),
+                             ("jacobi_4k_rgb16", run_approx4k, 120)):
+            sub = argparse.Namespace(**vars(args))
+            sub.frames, sub.steps, sub.warmup, sub.device_gen = nf, 3, 1, True
+            sub.video_format = "rgb8"
+            r = fn(sub, sa, torch, synth, dev)
+            keep = ("value", "unit", "ms_per_step", "fps", "roofline", "cpu_baseline",
+                    "tolerance", "plain", "walk_ms", "approx_ms", "palette_build_ms", "tuned",
+                    "config", "dtype", "steps", "warmup")
+            engines[name] = {k: r[k] for k in keep if k in r}
+            for k in r:
+                if k.startswith("gpu_vs_cpu"):
+                    engines[name][k] = r[k]
+            log(f"{name}: {r['value']} Mpx/s")
+            torch.cuda.empty_cache()
+
     # Engine alone, palettes resident, with the walk kernel timed by events.
     specs = specs_for(pals)
     sa.timing_enable(True)
@@ -930,6 +977,7 @@ def main():
         "latency_ms": chain_lat,
         "served": served,
         "serving_mix": mix,
+        "engines": engines,
         "palette_build_ms": round(t_pal * 1e3, 3),
         "palette_build_serial_ms": round(t_pal_serial * 1e3, 3),
         "plain": plain,

@@ -179,3 +179,27 @@ def plasma_device(w, h, t=0, seed=5, sigma=4.0, device="cuda"):
         planes[..., c] = 127.5 + 127.5 * acc / 4.0
     planes += torch.randn((h, w, 3), generator=g, device=device) * sigma
     return planes.round_().clamp_(0, 255).to(torch.uint8).contiguous()
+
+
+def radial_device(t, w=3840, h=2160, seed=4, salt=0.05, device="cuda"):
+    """An RGB16 frame of radial_frame16's formula (three radial gradients from
+    seeded centres drifting 3 px per frame t, then `salt` of the pixels set to
+    0 or 65535 per pixel), made with torch on `device` as h*w*3 contiguous
+    uint16 -- for frame counts too large to generate on the host."""
+    import torch
+    base = np.random.default_rng(seed)
+    centres = base.uniform(0, 1, (3, 2)) * [w, h]
+    g = torch.Generator(device=device)
+    g.manual_seed(seed * 7919 + t)
+    x = torch.arange(w, dtype=torch.float64, device=device)[None, :]
+    y = torch.arange(h, dtype=torch.float64, device=device)[:, None]
+    diag = math.hypot(w, h)
+    img = torch.empty((h, w, 3), dtype=torch.float64, device=device)
+    for c in range(3):
+        cx, cy = (float(v) for v in centres[c] + 3.0 * t)
+        r = torch.sqrt((x - cx) ** 2 + (y - cy) ** 2) / diag
+        img[..., c] = 65535.0 * torch.clamp(1.0 - r, 0, 1)
+    mask = torch.rand((h, w), generator=g, device=device) < salt
+    vals = torch.where(torch.rand((h, w), generator=g, device=device) < 0.5, 0.0, 65535.0)
+    img = torch.where(mask[..., None], vals[..., None].to(torch.float64), img)
+    return img.round_().clamp_(0, 65535).to(torch.int32).to(torch.uint16).contiguous()
