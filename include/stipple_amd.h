@@ -310,6 +310,22 @@ enum { SA_SCREEN_BAYER8 = 0, SA_SCREEN_VOID64 = 1, SA_SCREEN_VOID32 = 2, SA_SCRE
 int sa_dither_screen(const sa_image* frames, int n, int screen, double diffusion, void* stream,
                      char* err, size_t errlen);
 
+/* The error-diffusion dithers of the plugin seam, over frames of one geometry
+ * (each with its own palette, any entry count), one frame per wavefront:
+ *   SA_DIFFUSE_FLOYD    "floyd-steinberg" (src/rd_dithers.cc:74-110)
+ *   SA_DIFFUSE_ATKINSON "atkinson"        (:112-148)
+ *   SA_DIFFUSE_JARVIS   "jarvis"          (:150-220)
+ * Each replaces that DitherAlgorithm's run (include/rd_plugin.h:132-136) on
+ * device frames, bit-exact with the plugin: the same float arithmetic in the
+ * same order, the chosen entry's r, g, b with the source's alpha.  `diffusion`
+ * is DitherJob's.  dst may equal src when the formats match.  Frames keep
+ * their error rows in LDS: widths above about 6800 (floyd-steinberg,
+ * atkinson) or 4500 (jarvis) return SA_EUNSUPPORTED.  Refusals as
+ * sa_dither_screen ("<name>: no palette", "<name>: empty frame"). */
+enum { SA_DIFFUSE_FLOYD = 0, SA_DIFFUSE_ATKINSON = 1, SA_DIFFUSE_JARVIS = 2 };
+int sa_dither_diffuse(const sa_image* frames, int n, int kind, double diffusion, void* stream,
+                      char* err, size_t errlen);
+
 /* The Jacobi approximation: `iterations` fixed-point sweeps of
  * t = pal[q] + F (x - pal[q]) along the block curve, F = (I-H)^-1 truncated
  * to `taps`+1 terms, then an owner-filtered scatter.  Replaces

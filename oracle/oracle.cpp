@@ -1437,3 +1437,95 @@ extern "C" int orc_video_montage(const void* const* frames, int sampled, int fmt
   }
   return 0;
 }
+
+// ===========================================================================
+// Error-diffusion plugin dithers, restated from src/rd_dithers.cc:74-220 and
+// src/rd_dither_common.h (NearestEntry, ErrorPlane, Accumulate, WriteChosen):
+// kind 0 floyd-steinberg, 1 atkinson, 2 jarvis, in place on RgbaF frames.
+// ===========================================================================
+namespace {
+int nearest_entry_f(const double* ent, int count, float r, float g, float b) {
+  int best = 0;
+  float best_d = -1.0f;
+  for (int i = 0; i < count; ++i) {
+    const float dr = r - (float)ent[4 * i];
+    const float dg = g - (float)ent[4 * i + 1];
+    const float db = b - (float)ent[4 * i + 2];
+    const float d = dr * dr + dg * dg + db * db;
+    if (best_d < 0.0f || d < best_d) {
+      best_d = d;
+      best = i;
+    }
+  }
+  return best;
+}
+
+struct Plane {
+  std::vector<float> r, g, b;
+  size_t n = 0;
+  void add(size_t i, float w, float er, float eg, float eb) {
+    if (i >= n) return;
+    r[i] += w * er;
+    g[i] += w * eg;
+    b[i] += w * eb;
+  }
+};
+}  // namespace
+
+extern "C" int orc_dither_diffuse(const double* ent, int count, float* rgba, int64_t w64,
+                                  int64_t h64, int frames, int kind, double diffusion) {
+  if (count <= 0 || w64 <= 0 || h64 <= 0 || kind < 0 || kind > 2) return -1;
+  const size_t w = (size_t)w64, h = (size_t)h64, n = w * h;
+  const float d = (float)diffusion;
+  Plane e;
+  e.n = n;
+  for (int f = 0; f < frames; ++f) {
+    e.r.assign(n, 0.f);
+    e.g.assign(n, 0.f);
+    e.b.assign(n, 0.f);
+    float* fr = rgba + 4 * n * (size_t)f;
+    for (size_t y = 0; y < h; ++y)
+      for (size_t x = 0; x < w; ++x) {
+        const size_t i = y * w + x;
+        float* p = fr + 4 * i;
+        const float r = p[0] + e.r[i] * d;
+        const float g = p[1] + e.g[i] * d;
+        const float b = p[2] + e.b[i] * d;
+        const int k = nearest_entry_f(ent, count, r, g, b);
+        const double* c = ent + 4 * k;
+        const float er = r - (float)c[0], eg = g - (float)c[1], eb = b - (float)c[2];
+        if (kind == 0) {  // :98-105
+          if (x + 1 < w) e.add(i + 1, 7.0f / 16.0f, er, eg, eb);
+          e.add(i + w - (x > 0 ? 1 : 0), 3.0f / 16.0f, er, eg, eb);
+          e.add(i + w, 5.0f / 16.0f, er, eg, eb);
+          if (x + 1 < w) e.add(i + w + 1, 1.0f / 16.0f, er, eg, eb);
+        } else if (kind == 1) {  // :134-141
+          const float q = 1.0f / 8.0f;
+          if (x + 1 < w) e.add(i + 1, q, er, eg, eb);
+          if (x + 2 < w) e.add(i + 2, q, er, eg, eb);
+          if (x > 0) e.add(i + w - 1, q, er, eg, eb);
+          e.add(i + w, q, er, eg, eb);
+          if (x + 1 < w) e.add(i + w + 1, q, er, eg, eb);
+          if (y + 1 < h) e.add(i + 2 * w, q, er, eg, eb);
+        } else {  // :198-212: columns in size_t arithmetic, clamped only at the borders
+          static const int tdy[12] = {2, 2, 2, 2, 1, 1, 1, 1, 0, 0, 0, 0};
+          static const int tdx[12] = {-2, -1, 0, 1, -2, -1, 0, 1, -2, -1, 1, 2};
+          static const float tw[12] = {1.0f / 48.0f, 3.0f / 48.0f, 5.0f / 48.0f, 3.0f / 48.0f,
+                                       7.0f / 48.0f, 5.0f / 48.0f, 7.0f / 48.0f, 5.0f / 48.0f,
+                                       3.0f / 48.0f, 1.0f / 48.0f, 3.0f / 48.0f, 1.0f / 48.0f};
+          for (int t = 0; t < 12; ++t) {
+            size_t tx = x + (size_t)(ptrdiff_t)tdx[t];
+            size_t ty = y + (size_t)(ptrdiff_t)tdy[t];
+            if (tdx[t] < 0 && x == 0) tx = 0;
+            if (tdx[t] > 0 && x + 1 >= w) tx = w - 1;
+            if (tdy[t] > 0 && y + 1 >= h) ty = h - 1;
+            e.add(ty * w + tx, tw[t], er, eg, eb);
+          }
+        }
+        p[0] = (float)c[0];  // WriteChosen: r, g, b; alpha untouched
+        p[1] = (float)c[1];
+        p[2] = (float)c[2];
+      }
+  }
+  return 0;
+}

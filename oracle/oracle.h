@@ -106,6 +106,12 @@ int orc_walk_blocks_yuv444(const OrcTree* t, const float* batch, int64_t w, int6
 int orc_dither_screen(const double* ent, int count, float* rgba, int64_t w, int64_t h,
                       int frames, int screen, double diffusion);
 
+/* The error-diffusion plugin dithers in place on RgbaF frames (kind 0
+ * floyd-steinberg, 1 atkinson, 2 jarvis; src/rd_dithers.cc:74-220); ent is
+ * count x 4 doubles.  Returns 0 or -1. */
+int orc_dither_diffuse(const double* ent, int count, float* rgba, int64_t w, int64_t h,
+                       int frames, int kind, double diffusion);
+
 /* Inverse filter g[0..taps] (src/rd_approx_cuda.cu:73-88). */
 void orc_inverse_filter(int taps, double diffusion, double* g);
 /* The Jacobi engine restated on the CPU (src/rd_approx_cuda.cu:90-384):

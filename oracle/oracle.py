@@ -105,6 +105,7 @@ class Oracle(_Lib):
         self._screen = f("dither_screen", _i32, _p, _i32, _p, _i64, _i64, _i32, _i32, _dbl)
         self._inv = f("inverse_filter", None, _i32, _dbl, _p)
         self._jac = f("jacobi", _i32, _p, _p, _i64, _i64, _i32, _i32, _i32, _i32, _dbl, _p)
+        self._diffuse = f("dither_diffuse", _i32, _p, _i32, _p, _i64, _i64, _i32, _i32, _dbl)
         self._montage = f("video_montage", _i32, _p, _i32, _i32, _i64, _i64, _i32, _i32, _i32,
                           _p, _p, _p)
         self._jinit = f("jacobi_init", _i32, _p, _p, _i64, _i32, _i32, _p)
@@ -222,6 +223,18 @@ class Oracle(_Lib):
             a = a[None]
         n, h, w = a.shape[:3]
         if self._screen(_ptr(e), len(e), _ptr(a), w, h, n, screen, diffusion) != 0:
+            raise ValueError("refused")
+        return a
+
+    def dither_diffuse(self, entries, frames, kind, diffusion=1.0):
+        """An error-diffusion plugin dither (kind 0 floyd-steinberg, 1 atkinson,
+        2 jarvis) of (frames, h, w, 4) float32 RgbaF; returns the result."""
+        e = np.ascontiguousarray(entries, dtype=np.float64)
+        a = np.ascontiguousarray(frames, dtype=np.float32).copy()
+        if a.ndim == 3:
+            a = a[None]
+        n, h, w = a.shape[:3]
+        if self._diffuse(_ptr(e), len(e), _ptr(a), w, h, n, kind, diffusion) != 0:
             raise ValueError("refused")
         return a
 

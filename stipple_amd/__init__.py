@@ -37,6 +37,8 @@ from ._lib import (  # noqa: F401
     available,
     curve_order,
     dither_screen,
+    dither_diffuse,
+    DIFFUSERS,
     device_name,
     image_spec,
     init,
@@ -76,6 +78,7 @@ __all__ = [
     "walk_blocks",
     "walk_blocks_noalloc",
     "walk_seq_batch",
+    "dither_diffuse",
     "video_montage",
     "write_palette_txt",
     "read_palette_txt",

@@ -99,6 +99,7 @@ def lib():
                                               sz]),
                 "sa_walk_approx": (i32, [C.POINTER(_Image), i32, i32, i32, i32, i32, dbl, vp, cp, sz]),
                 "sa_dither_screen": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
+                "sa_dither_diffuse": (i32, [C.POINTER(_Image), i32, i32, dbl, vp, cp, sz]),
                 "sa_seq_stats": (i32, [vp, cp, sz]),
                 "sa_inverse_filter": (i32, [i32, dbl, vp, cp, sz]),
                 "sa_walk_job_create": (i32, [C.POINTER(SeqOptions), C.POINTER(vp), cp, sz]),
@@ -502,6 +503,22 @@ def walk_blocks_noalloc(frames, block=64, diffusion=1.0, stream=None):
 SCREEN_BAYER8, SCREEN_VOID64, SCREEN_VOID32, SCREEN_CLUSTERED8 = range(4)
 SCREENS = {"bayer-ordered": SCREEN_BAYER8, "void-and-cluster": SCREEN_VOID64,
            "void-and-cluster-fast": SCREEN_VOID32, "clustered-dot": SCREEN_CLUSTERED8}
+
+
+DIFFUSE_FLOYD, DIFFUSE_ATKINSON, DIFFUSE_JARVIS = range(3)
+DIFFUSERS = {"floyd-steinberg": DIFFUSE_FLOYD, "atkinson": DIFFUSE_ATKINSON,
+             "jarvis": DIFFUSE_JARVIS}
+
+
+def dither_diffuse(frames, kind, diffusion=1.0, stream=None):
+    """An error-diffusion plugin dither (sa_dither_diffuse) over frames of one
+    geometry; `kind` is DIFFUSE_* or a plugin name."""
+    if isinstance(kind, str):
+        kind = DIFFUSERS[kind]
+    arr = (_Image * len(frames))(*frames)
+    e = _err()
+    _check(lib().sa_dither_diffuse(arr, len(frames), kind, diffusion, _stream_ptr(stream), e,
+                                   len(e)), e)
 
 
 def dither_screen(frames, screen, diffusion=1.0, stream=None):

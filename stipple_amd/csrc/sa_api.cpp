@@ -1649,6 +1649,52 @@ extern "C" int sa_dither_screen(const sa_image* frames, int n, int screen, doubl
   return SA_OK;
 }
 
+// ---- error-diffusion dithers --------------------------------------------------------
+extern "C" int sa_dither_diffuse(const sa_image* frames, int n, int kind, double diffusion,
+                                 void* stream, char* err, size_t errlen) {
+  static const char* kWho[3] = {"floyd-steinberg", "atkinson", "jarvis"};
+  if (kind < 0 || kind > 2) return fail(err, errlen, SA_EINVAL, "unknown diffusion dither %d", kind);
+  if (n < 1 || !frames) return fail(err, errlen, SA_EINVAL, "%s: no frames", kWho[kind]);
+  const int64_t w = frames[0].width, h = frames[0].height;
+  for (int i = 0; i < n; ++i) {
+    const sa_image& f = frames[i];
+    if (!f.palette || f.palette->count <= 0)
+      return fail(err, errlen, SA_EREFUSED, "%s: no palette", kWho[kind]);
+    if (f.width <= 0 || f.height <= 0)
+      return fail(err, errlen, SA_EREFUSED, "%s: empty frame", kWho[kind]);
+    if (!f.src || !f.dst) return fail(err, errlen, SA_EINVAL, "frame %d: null pointer", i);
+    if (f.width != w || f.height != h)
+      return fail(err, errlen, SA_EINVAL, "frame %d: geometry differs", i);
+    if (!valid_blocks_in_fmt(f.src_fmt) || !valid_blocks_out_fmt(f.dst_fmt))
+      return fail(err, errlen, SA_EINVAL, "frame %d: bad pixel format", i);
+    if (f.dst == f.src && f.src_fmt != f.dst_fmt)
+      return fail(err, errlen, SA_EINVAL, "frame %d: in-place needs the same format", i);
+    if (f.src_fmt == sa::kYuv420 && ((w & 1) || (h & 1)))
+      return fail(err, errlen, SA_EUNSUPPORTED, "frame %d: 4:2:0 input needs even width and height",
+                  i);
+  }
+  if (sa::diffuse_lds_bytes(kind, w) > g_lds_per_cu)
+    return fail(err, errlen, SA_EUNSUPPORTED,
+                "%s: a frame %lld pixels wide needs %zu bytes of LDS for its error rows (at most "
+                "%zu)", kWho[kind], (long long)w, sa::diffuse_lds_bytes(kind, w), g_lds_per_cu);
+  int rc = ensure_init(err, errlen);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<sa::ScreenFrame> fr((size_t)n);
+  for (int i = 0; i < n; ++i)
+    fr[(size_t)i] = sa::ScreenFrame{frames[i].src, frames[i].dst, frames[i].palette->dev_hdr,
+                                    frames[i].palette->dev_entries, frames[i].src_fmt,
+                                    frames[i].dst_fmt};
+  sa::ScreenFrame* d_fr = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::ScreenFrame) * fr.size(), s));
+  HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::ScreenFrame) * fr.size(), s));
+  HIP_TRY(timed("diffuse_dither", s, [&] {
+    return sa::launch_diffuse(d_fr, n, w, h, kind, (float)diffusion, s);
+  }));
+  HIP_TRY(hipFreeAsync(d_fr, s));
+  return SA_OK;
+}
+
 // ---- Jacobi approximation ----------------------------------------------------------
 // BuildInverseFilter (src/rd_approx_cuda.cu:73-88) with the host libm weights.
 static void inverse_filter(int taps, double diffusion, std::vector<double>& g) {

@@ -6,6 +6,7 @@
 
 #include <vector>
 
+#include "sa_math.h"
 #include "sa_palette.h"
 
 namespace sa {
@@ -197,6 +198,47 @@ struct ScreenFrame {
   int32_t dst_fmt;
 };
 
+// Writes a plugin dither's choice at pixel `pix`: the entry's r, g, b (WriteChosen,
+// src/rd_dither_common.h) with the source's alpha `a`, in the frame's output
+// format (the index itself for the index formats).
+__device__ __forceinline__ void store_pick(const ScreenFrame& f, int64_t pix, int64_t pp, int idx,
+                                           const float4& e, float a) {
+  switch (f.dst_fmt) {
+    case kIndex8: ((uint8_t*)f.dst)[pix] = (uint8_t)idx; break;
+    case kIndex16: ((uint16_t*)f.dst)[pix] = (uint16_t)idx; break;
+    case kRgb8: {
+      uint8_t* d = (uint8_t*)f.dst + 3 * pix;
+      d[0] = (uint8_t)scale_char(e.x);
+      d[1] = (uint8_t)scale_char(e.y);
+      d[2] = (uint8_t)scale_char(e.z);
+      break;
+    }
+    case kRgba8:
+      ((uint32_t*)f.dst)[pix] = scale_char(e.x) | (scale_char(e.y) << 8) |
+                                (scale_char(e.z) << 16) | (scale_char(a) << 24);
+      break;
+    case kRgb16: {
+      uint16_t* d = (uint16_t*)f.dst + 3 * pix;
+      d[0] = (uint16_t)e.x;
+      d[1] = (uint16_t)e.y;
+      d[2] = (uint16_t)e.z;
+      break;
+    }
+    case kRgba16: {
+      ushort4 o;
+      o.x = (uint16_t)e.x;
+      o.y = (uint16_t)e.y;
+      o.z = (uint16_t)e.z;
+      o.w = (uint16_t)a;
+      ((ushort4*)f.dst)[pix] = o;
+      break;
+    }
+    case kYuv444: store_yuv444(f.dst, pix, pp, e.x, e.y, e.z); break;
+    default: ((float4*)f.dst)[pix] = make_float4(e.x, e.y, e.z, a); break;
+  }
+}
+
+
 struct Launch {
   hipStream_t stream;
   int cus;  // compute units on the device
@@ -258,6 +300,11 @@ hipError_t launch_screen(const ScreenFrame* frames, int nframes, int64_t w, int6
                          const uint16_t* table, int side, int screen, double diffusion,
                          ScreenRange* ranges, hipStream_t s);
 std::vector<uint16_t> void_cluster_ranks(int n);
+// sa_diffuse.hip: the error-diffusion dithers (kind 0 floyd-steinberg, 1
+// atkinson, 2 jarvis), one frame per wavefront; LDS bytes per workgroup.
+size_t diffuse_lds_bytes(int kind, int64_t w);
+hipError_t launch_diffuse(const ScreenFrame* frames, int nframes, int64_t w, int64_t h, int kind,
+                          float diffusion, hipStream_t s);
 extern const uint16_t kBayer8[64];
 extern const uint16_t kDot8[64];
 

@@ -93,45 +93,6 @@ __device__ __forceinline__ double luma(double r, double g, double b) {
   return 0.299 * r + 0.587 * g + 0.114 * b;
 }
 
-// The chosen entry at pixel `pix`: r, g, b of the entry, alpha the source's
-// (WriteChosen, src/rd_dither_common.h:100-104), in the output format.
-__device__ __forceinline__ void store_pick(const ScreenFrame& f, int64_t pix, int64_t pp, int idx,
-                                           const float4& e, float a) {
-  switch (f.dst_fmt) {
-    case kIndex8: ((uint8_t*)f.dst)[pix] = (uint8_t)idx; break;
-    case kIndex16: ((uint16_t*)f.dst)[pix] = (uint16_t)idx; break;
-    case kRgb8: {
-      uint8_t* d = (uint8_t*)f.dst + 3 * pix;
-      d[0] = (uint8_t)scale_char(e.x);
-      d[1] = (uint8_t)scale_char(e.y);
-      d[2] = (uint8_t)scale_char(e.z);
-      break;
-    }
-    case kRgba8:
-      ((uint32_t*)f.dst)[pix] = scale_char(e.x) | (scale_char(e.y) << 8) |
-                                (scale_char(e.z) << 16) | (scale_char(a) << 24);
-      break;
-    case kRgb16: {
-      uint16_t* d = (uint16_t*)f.dst + 3 * pix;
-      d[0] = (uint16_t)e.x;
-      d[1] = (uint16_t)e.y;
-      d[2] = (uint16_t)e.z;
-      break;
-    }
-    case kRgba16: {
-      ushort4 o;
-      o.x = (uint16_t)e.x;
-      o.y = (uint16_t)e.y;
-      o.z = (uint16_t)e.z;
-      o.w = (uint16_t)a;
-      ((ushort4*)f.dst)[pix] = o;
-      break;
-    }
-    case kYuv444: store_yuv444(f.dst, pix, pp, e.x, e.y, e.z); break;
-    default: ((float4*)f.dst)[pix] = make_float4(e.x, e.y, e.z, a); break;
-  }
-}
-
 // clustered-dot's two candidates per frame, the palette's darkest and
 // lightest entries by luma, first wins (src/rd_dithers.cc:317-328): one
 // wavefront per frame, lexicographic on (luma, index) through the shuffles.

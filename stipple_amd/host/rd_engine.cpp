@@ -333,13 +333,20 @@ bool ImBuildPaletteFromPixels(const RgbaF* pixels, std::size_t width, std::size_
   return true;
 }
 
-// ---- threshold-screen plugins on the device ----------------------------------------------
-// The reference's four threshold-screen dithers (src/rd_dithers_ordered.cc:488-502,
+// ---- plugin dithers on the device ---------------------------------------------------------
+// The reference's error-diffusion dithers (src/rd_dithers.cc:362-369) and its
+// four threshold-screen dithers (src/rd_dithers_ordered.cc:488-502,
 // src/rd_dithers.cc:370-371) registered under their own names: linking this
-// library gives a caller the same --dither names, run by sa_dither_screen.
+// library gives a caller the same --dither names, run by sa_dither_diffuse and
+// sa_dither_screen.
 namespace {
 
-DitherResult screen_run(DitherJob& job, RgbaF* px, int screen, const char* who) {
+// One registered device dither: `launch` is sa_dither_screen or sa_dither_diffuse
+// with its kind.
+using DeviceDither = int (*)(const sa_image*, int, int, double, void*, char*, size_t);
+
+DitherResult device_run(DitherJob& job, RgbaF* px, DeviceDither launch, int kind,
+                        const char* who) {
   DitherResult res;
   if (!job.palette || job.palette->count <= 0) {  // Prepare, src/rd_dither_common.h:108-120
     res.error = std::string(who) + ": no palette";
@@ -376,8 +383,7 @@ DitherResult screen_run(DitherJob& job, RgbaF* px, int screen, const char* who) 
       fr[(size_t)f] = sa_image{d, d, (int64_t)job.width, (int64_t)job.height, SA_PIX_RGBAF32,
                                SA_PIX_RGBAF32, pal};
     }
-    if (sa_dither_screen(fr.data(), frames, screen, job.diffusion, nullptr, err, sizeof(err)) !=
-        SA_OK)
+    if (launch(fr.data(), frames, kind, job.diffusion, nullptr, err, sizeof(err)) != SA_OK)
       res.error = prefixed(who, err);
     else
       he = hipMemcpy(px, buf.p, bytes, hipMemcpyDeviceToHost);
@@ -388,16 +394,25 @@ DitherResult screen_run(DitherJob& job, RgbaF* px, int screen, const char* who) 
 }
 
 DitherResult run_bayer(DitherJob& j, RgbaF* p) {
-  return screen_run(j, p, SA_SCREEN_BAYER8, "bayer-ordered");
+  return device_run(j, p, sa_dither_screen, SA_SCREEN_BAYER8, "bayer-ordered");
 }
 DitherResult run_void64(DitherJob& j, RgbaF* p) {
-  return screen_run(j, p, SA_SCREEN_VOID64, "void-and-cluster");
+  return device_run(j, p, sa_dither_screen, SA_SCREEN_VOID64, "void-and-cluster");
 }
 DitherResult run_void32(DitherJob& j, RgbaF* p) {
-  return screen_run(j, p, SA_SCREEN_VOID32, "void-and-cluster-fast");
+  return device_run(j, p, sa_dither_screen, SA_SCREEN_VOID32, "void-and-cluster-fast");
 }
 DitherResult run_cluster(DitherJob& j, RgbaF* p) {
-  return screen_run(j, p, SA_SCREEN_CLUSTERED8, "clustered-dot");
+  return device_run(j, p, sa_dither_screen, SA_SCREEN_CLUSTERED8, "clustered-dot");
+}
+DitherResult run_floyd(DitherJob& j, RgbaF* p) {
+  return device_run(j, p, sa_dither_diffuse, SA_DIFFUSE_FLOYD, "floyd-steinberg");
+}
+DitherResult run_atkinson(DitherJob& j, RgbaF* p) {
+  return device_run(j, p, sa_dither_diffuse, SA_DIFFUSE_ATKINSON, "atkinson");
+}
+DitherResult run_jarvis(DitherJob& j, RgbaF* p) {
+  return device_run(j, p, sa_dither_diffuse, SA_DIFFUSE_JARVIS, "jarvis");
 }
 
 }  // namespace
@@ -438,6 +453,12 @@ const std::vector<DitherAlgorithm>& RegisteredDithers() { return reg().algos; }
 
 namespace {
 const bool kScreensRegistered = [] {
+  RegisterDither({"floyd-steinberg", "error diffusion, 4 neighbours (7/16 3/16 5/16 1/16)",
+                  run_floyd});
+  RegisterDither({"atkinson", "error diffusion, 6 neighbours at 1/8; discards 1/4 of the error",
+                  run_atkinson});
+  RegisterDither({"jarvis", "Jarvis/Judice/Ninke, 12 neighbours, 48-denominator weights",
+                  run_jarvis});
   RegisterDither({"bayer-ordered", "ordered 8x8 Bayer, no error diffusion", run_bayer});
   RegisterDither({"void-and-cluster", "blue-noise thresholds, 64x64 matrix", run_void64});
   RegisterDither({"void-and-cluster-fast", "blue-noise thresholds, 32x32 matrix (quicker)",

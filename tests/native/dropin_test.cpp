@@ -310,10 +310,10 @@ int main() {
     const std::string list = rd::DitherListText();
     const std::string tail = "\n  probe           a test dither";
     check(r1 && !r2 && !r3 && rd::FindDither("probe") && !rd::FindDither("nope") &&
-              rd::RegisteredDithers().size() == 5 && list.size() > tail.size() &&
+              rd::RegisteredDithers().size() == 8 && list.size() > tail.size() &&
               list.compare(list.size() - tail.size(), tail.size(), tail) == 0 &&
-              list.rfind("  bayer-ordered   ordered 8x8 Bayer", 0) == 0,
-          "RegisterDither / FindDither / DitherListText (the four screen dithers registered)");
+              list.rfind("  floyd-steinberg error diffusion, 4 neighbours", 0) == 0,
+          "RegisterDither / FindDither / DitherListText (the seven device dithers registered)");
   }
 
   // ---- the screen dithers behind DitherAlgorithm, on the device ---------------------------
@@ -345,6 +345,27 @@ int main() {
     check(rd::FindDither("clustered-dot")->run(empty, nullptr).error ==
               "clustered-dot: empty frame",
           "screen dither refuses an empty frame");
+    // The error-diffusion dithers, two frames per job.
+    const char* dnames[3] = {"floyd-steinberg", "atkinson", "jarvis"};
+    std::vector<rd::RgbaF> two = img;
+    std::vector<rd::RgbaF> img2 = make_image(W, H, 9, false);
+    two.insert(two.end(), img2.begin(), img2.end());
+    for (int k = 0; k < 3; ++k) {
+      const rd::DitherAlgorithm* a = rd::FindDither(dnames[k]);
+      std::vector<rd::RgbaF> got = two, want = two;
+      rd::DitherJob job;
+      job.palette = pal;
+      job.width = W;
+      job.height = H;
+      job.frames = 2;
+      job.diffusion = 0.9;
+      const rd::DitherResult r = a ? a->run(job, got.data()) : rd::DitherResult{"missing"};
+      orc_dither_diffuse(ent.data(), pal->count, &want[0].r, W, H, 2, k, 0.9);
+      check(r.error.empty() && same_bytes(got.data(), want.data(), 16 * got.size()), dnames[k],
+            r.error);
+    }
+    check(rd::FindDither("jarvis")->run(empty, nullptr).error == "jarvis: empty frame",
+          "diffusion dither refuses an empty frame");
   }
 
   // ---- rd_im.h: palette files and the pixel-buffer palette --------------------------
