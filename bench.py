@@ -360,6 +360,33 @@ def run_screens(args, sa, torch, synth, dev):
                     stream) / args.steps
         per[name] = {"ms": round(ms, 4), "Mpixel_s": round(nf * w * h / ms / 1e3, 1),
                      "GB_s": round(6.0 * nf * w * h / ms / 1e6, 1)}
+    # The error-diffusion plugins over the same frames.  floyd-steinberg and
+    # atkinson pipeline a frame's rows in one wavefront; jarvis walks a frame
+    # in raster order, so it is given more frames (one wavefront each) to fill
+    # the device.
+    diffuse = {}
+    for kind, name in enumerate(["floyd-steinberg", "atkinson", "jarvis"]):
+        sa.dither_diffuse(specs, kind)
+        torch.cuda.synchronize()
+        reps = max(1, min(args.steps, 3))
+        ms = _ev_ms(lambda: [sa.dither_diffuse(specs, kind) for _ in range(reps)],
+                    stream) / reps
+        diffuse[name] = {"ms": round(ms, 3), "Mpixel_s": round(nf * w * h / ms / 1e3, 1),
+                         "frames": nf}
+        log(f"{name}: {ms:.2f} ms per {nf} frames")
+    cpu_diffuse = None
+    if not args.no_cpu:
+        from oracle.oracle import Ref, Oracle
+        f0 = synth.widen_q16(frames[0])
+        cpu_diffuse = {}
+        for kind, name in enumerate(["floyd-steinberg", "atkinson", "jarvis"]):
+            t1 = time.perf_counter()
+            if Ref.available():
+                Ref().dither(name, pal.entries, f0)
+            else:
+                Oracle().dither_diffuse(pal.entries, f0, kind)
+            cpu_diffuse[name] = round(w * h / (time.perf_counter() - t1) / 1e6, 3)
+            diffuse[name]["vs_cpu_1core"] = round(diffuse[name]["Mpixel_s"] / cpu_diffuse[name], 1)
     t = per["bayer-ordered"]["ms"] * 1e-3
     px = w * h
     cpu = None
@@ -386,7 +413,9 @@ def run_screens(args, sa, torch, synth, dev):
                  px * nf / t / 1e6, "Mpixel/s", t * 1e3, "f32",
                  {"workload": f"{nf} 1920x1080 RGB8 plasma frames, bayer-ordered with a resident "
                               "16-colour device palette (value); every screen in per_screen",
-                  "frames": nf, "device": dev}, roof, cpu, {"per_screen": per})
+                  "frames": nf, "device": dev}, roof, cpu,
+                 {"per_screen": per, "diffusion": diffuse,
+                  "cpu_diffusion_1core_Mpixel_s": cpu_diffuse})
 
 
 def run_approx4k(args, sa, torch, synth, dev):

@@ -1673,10 +1673,13 @@ extern "C" int sa_dither_diffuse(const sa_image* frames, int n, int kind, double
       return fail(err, errlen, SA_EUNSUPPORTED, "frame %d: 4:2:0 input needs even width and height",
                   i);
   }
-  if (sa::diffuse_lds_bytes(kind, w) > g_lds_per_cu)
+  int max_count = 0;
+  for (int i = 0; i < n; ++i) max_count = std::max(max_count, frames[i].palette->count);
+  if (sa::diffuse_lds_bytes(kind, w, max_count) > g_lds_per_cu)
     return fail(err, errlen, SA_EUNSUPPORTED,
-                "%s: a frame %lld pixels wide needs %zu bytes of LDS for its error rows (at most "
-                "%zu)", kWho[kind], (long long)w, sa::diffuse_lds_bytes(kind, w), g_lds_per_cu);
+                "%s: a frame %lld pixels wide with %d entries needs %zu bytes of LDS for its "
+                "error rows and entries (at most %zu)", kWho[kind], (long long)w, max_count,
+                sa::diffuse_lds_bytes(kind, w, max_count), g_lds_per_cu);
   int rc = ensure_init(err, errlen);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -1689,7 +1692,7 @@ extern "C" int sa_dither_diffuse(const sa_image* frames, int n, int kind, double
   HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::ScreenFrame) * fr.size(), s));
   HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::ScreenFrame) * fr.size(), s));
   HIP_TRY(timed("diffuse_dither", s, [&] {
-    return sa::launch_diffuse(d_fr, n, w, h, kind, (float)diffusion, s);
+    return sa::launch_diffuse(d_fr, n, w, h, kind, max_count, (float)diffusion, s);
   }));
   HIP_TRY(hipFreeAsync(d_fr, s));
   return SA_OK;

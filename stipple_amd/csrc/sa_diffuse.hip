@@ -49,12 +49,118 @@ __device__ __forceinline__ void acc_add(F3& acc, float w, const F3& v) {
   acc.b = acc.b + w * v.b;
 }
 
+// Global-address-space views: their loads and stores count only against the
+// vector-memory counter, so the LDS waits of a step (shuffles, carried rows,
+// error ring) never wait on a pixel load or store in flight.
+typedef const uint8_t __attribute__((address_space(1))) gcu8;
+typedef const uint16_t __attribute__((address_space(1))) gcu16;
+typedef const uint32_t __attribute__((address_space(1))) gcu32;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef const u32x2 __attribute__((address_space(1))) gcu2;
+typedef const u32x4 __attribute__((address_space(1))) gcu4;
+typedef uint8_t __attribute__((address_space(1))) gu8;
+typedef uint16_t __attribute__((address_space(1))) gu16;
+typedef uint32_t __attribute__((address_space(1))) gu32;
+typedef u16x4 __attribute__((address_space(1))) gus4;
+typedef f32x4 __attribute__((address_space(1))) gf4;
+
+// A source pixel's raw bits, loaded early and converted when used (load_pixel's
+// conversions: 8-bit x257, 16-bit as-is, alpha 65535 without an alpha
+// channel).  Planar YUV is converted at load (load_any).
+template <int FMT>
+__device__ __forceinline__ uint4 load_raw(const void* src, int64_t p, int64_t w, int64_t h) {
+  switch (FMT) {
+    case kRgb8: {
+      gcu8* q = (gcu8*)src + 3 * p;
+      return make_uint4(q[0], q[1], q[2], 0);
+    }
+    case kRgba8: return make_uint4(((gcu32*)src)[p], 0, 0, 0);
+    case kRgb16: {
+      gcu16* q = (gcu16*)src + 3 * p;
+      return make_uint4(q[0], q[1], q[2], 0);
+    }
+    case kRgba16: {
+      const u32x2 v = ((gcu2*)src)[p];
+      return make_uint4(v.x, v.y, 0, 0);
+    }
+    case kRgbaF32: {
+      const u32x4 v = ((gcu4*)src)[p];
+      return make_uint4(v.x, v.y, v.z, v.w);
+    }
+    default: {
+      const float4 v = load_any(FMT, src, p, w, h);
+      return make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z),
+                        __float_as_uint(v.w));
+    }
+  }
+}
+
+template <int FMT>
+__device__ __forceinline__ float4 raw_pixel(uint4 r) {
+  switch (FMT) {
+    case kRgb8:
+      return make_float4((float)r.x * 257.0f, (float)r.y * 257.0f, (float)r.z * 257.0f, 65535.0f);
+    case kRgba8:
+      return make_float4((float)(r.x & 255u) * 257.0f, (float)((r.x >> 8) & 255u) * 257.0f,
+                         (float)((r.x >> 16) & 255u) * 257.0f, (float)(r.x >> 24) * 257.0f);
+    case kRgb16: return make_float4((float)r.x, (float)r.y, (float)r.z, 65535.0f);
+    case kRgba16:
+      return make_float4((float)(r.x & 0xFFFFu), (float)(r.x >> 16), (float)(r.y & 0xFFFFu),
+                         (float)(r.y >> 16));
+    default:
+      return make_float4(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z),
+                         __uint_as_float(r.w));
+  }
+}
+
+// store_pick through global-space stores (planar 4:4:4 through store_pick).
+__device__ __forceinline__ void store_choice(const ScreenFrame& f, int64_t pix, int64_t pp, int idx,
+                                             const float4& e, float a) {
+  switch (f.dst_fmt) {
+    case kIndex8: ((gu8*)f.dst)[pix] = (uint8_t)idx; break;
+    case kIndex16: ((gu16*)f.dst)[pix] = (uint16_t)idx; break;
+    case kRgb8: {
+      gu8* d = (gu8*)f.dst + 3 * pix;
+      d[0] = (uint8_t)scale_char(e.x);
+      d[1] = (uint8_t)scale_char(e.y);
+      d[2] = (uint8_t)scale_char(e.z);
+      break;
+    }
+    case kRgba8:
+      ((gu32*)f.dst)[pix] = scale_char(e.x) | (scale_char(e.y) << 8) | (scale_char(e.z) << 16) |
+                            (scale_char(a) << 24);
+      break;
+    case kRgb16: {
+      gu16* d = (gu16*)f.dst + 3 * pix;
+      d[0] = (uint16_t)e.x;
+      d[1] = (uint16_t)e.y;
+      d[2] = (uint16_t)e.z;
+      break;
+    }
+    case kRgba16: {
+      const u16x4 o = {(uint16_t)e.x, (uint16_t)e.y, (uint16_t)e.z, (uint16_t)a};
+      ((gus4*)f.dst)[pix] = o;
+      break;
+    }
+    case kRgbaF32: {
+      const f32x4 o = {e.x, e.y, e.z, a};
+      ((gf4*)f.dst)[pix] = o;
+      break;
+    }
+    default: store_pick(f, pix, pp, idx, e, a); break;
+  }
+}
+
 __device__ __forceinline__ F3 shift_up(const F3& v) {
   return F3{__shfl_up(v.r, 1, kWave), __shfl_up(v.g, 1, kWave), __shfl_up(v.b, 1, kWave)};
 }
 
-// NearestEntry over the frame's entries (wave-uniform loads): the first entry
-// is taken unconditionally, then only a strictly smaller distance replaces it.
+// NearestEntry over the frame's entries, staged in LDS (every lane reads the
+// same entry: a broadcast): the first entry is taken unconditionally, then
+// only a strictly smaller distance replaces it.
 __device__ __forceinline__ int nearest(const float4* __restrict__ ent, int count, float r, float g,
                                        float b, float4& c) {
   float4 e = ent[0];
@@ -62,6 +168,7 @@ __device__ __forceinline__ int nearest(const float4* __restrict__ ent, int count
   float bd = dr * dr + dg * dg + db * db;
   int bi = 0;
   c = e;
+#pragma unroll 4
   for (int i = 1; i < count; ++i) {
     e = ent[i];
     dr = r - e.x;
@@ -78,13 +185,11 @@ __device__ __forceinline__ int nearest(const float4* __restrict__ ent, int count
 }
 
 // KIND 0: floyd-steinberg (src/rd_dithers.cc:74-110); 1: atkinson (:112-148).
-template <int KIND>
-__global__ __launch_bounds__(kWave) void k_diffuse_rows(const ScreenFrame* __restrict__ frames,
-                                                        int64_t w, int64_t h, float diffusion) {
-  extern __shared__ float carry[];  // r, g, b planes: row y0-1, then (atkinson) row y0-2
-  const ScreenFrame f = frames[blockIdx.x];
-  const float4* ent = (const float4*)((const unsigned char*)f.pal->image + f.pal->off_ent);
-  const int count = f.pal->count;
+// FMT: the frame's source format, fixed for the loop.
+template <int KIND, int FMT>
+__device__ __forceinline__ void rows_frame(const ScreenFrame& f, const float4* __restrict__ ent,
+                                           int count, float* carry, int64_t w, int64_t h,
+                                           float diffusion) {
   const int lane = threadIdx.x;
   const int64_t pp = w * h;
   float* c1r = carry;
@@ -111,7 +216,7 @@ __global__ __launch_bounds__(kWave) void k_diffuse_rows(const ScreenFrame* __res
     }
     // The row's first pixel (lane 0 has no step before its first pixel to
     // prefetch it in).
-    float4 pcur = live ? load_any(f.src_fmt, f.src, y * w, w, h) : make_float4(0.f, 0.f, 0.f, 0.f);
+    uint4 pcur = live ? load_raw<FMT>(f.src, y * w, w, h) : make_uint4(0, 0, 0, 0);
     const int64_t steps = w + 2 * (int64_t)(rows - 1);
     for (int64_t t = 0; t < steps; ++t) {
       const int64_t X = t - 2 * (int64_t)lane;
@@ -131,8 +236,8 @@ __global__ __launch_bounds__(kWave) void k_diffuse_rows(const ScreenFrame* __res
       const F3 v2 = vprev;  // atkinson: the residual two rows up at X
       vprev = in_u;
       // Prefetch the next step's pixel.
-      float4 pnext = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (live && X >= -1 && X + 1 < w) pnext = load_any(f.src_fmt, f.src, y * w + X + 1, w, h);
+      uint4 pnext = make_uint4(0, 0, 0, 0);
+      if (live && X >= -1 && X + 1 < w) pnext = load_raw<FMT>(f.src, y * w + X + 1, w, h);
       F3 e_out = zero;
       if (live && X >= 0 && X < w) {
         // This pixel's error, summed in the reference's source order.
@@ -156,14 +261,14 @@ __global__ __launch_bounds__(kWave) void k_diffuse_rows(const ScreenFrame* __res
           if (X >= 2) acc_add(acc, k, own2);
           if (X >= 1) acc_add(acc, k, own1);
         }
-        const float4 p = pcur;
+        const float4 p = raw_pixel<FMT>(pcur);
         const float r = p.x + acc.r * diffusion;
         const float g = p.y + acc.g * diffusion;
         const float b = p.z + acc.b * diffusion;
         float4 c;
         const int bi = nearest(ent, count, r, g, b, c);
         e_out = f3(r - c.x, g - c.y, b - c.z);
-        store_pick(f, y * w + X, pp, bi, c, p.w);
+        store_choice(f, y * w + X, pp, bi, c, p.w);
         own2 = own1;
         own1 = e_out;
         if (more && lane == rows - 1) {
@@ -179,23 +284,71 @@ __global__ __launch_bounds__(kWave) void k_diffuse_rows(const ScreenFrame* __res
       }
       pub = e_out;
       pub_u = u0;
-      if (X >= -1 && X + 1 < w) pcur = pnext;
+      // What the next step reads: pixel X+1 when this step loaded it, and a
+      // pixel no step of this lane reads otherwise.
+      pcur = pnext;
     }
     __syncthreads();  // the carried rows are complete before the next strip reads them
   }
 }
 
-// Wave-wide lexicographic minimum of (d, i) for d >= 0 (or +inf): the bits of
-// a non-negative float order as unsigned integers.
-__device__ __forceinline__ int wave_argmin(float d, int i) {
-  uint64_t key = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)i;
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint32_t lo = __shfl_xor((uint32_t)key, off, kWave);
-    const uint32_t hi = __shfl_xor((uint32_t)(key >> 32), off, kWave);
-    const uint64_t o = ((uint64_t)hi << 32) | lo;
-    key = o < key ? o : key;
+template <int KIND>
+__global__ __launch_bounds__(kWave) void k_diffuse_rows(const ScreenFrame* __restrict__ frames,
+                                                        int64_t w, int64_t h, float diffusion) {
+  // LDS: the entries (float4 each), then the carried rows' r, g, b planes:
+  // row y0-1, then (atkinson) row y0-2.
+  extern __shared__ float4 lds4[];
+  const ScreenFrame f = frames[blockIdx.x];
+  const int count = __builtin_amdgcn_readfirstlane(f.pal->count);
+  const int lane = threadIdx.x;
+  float4* ent = lds4;
+  {
+    const float4* g = (const float4*)((const unsigned char*)f.pal->image + f.pal->off_ent);
+    for (int k = lane; k < count; k += kWave) ent[k] = g[k];
+    __syncthreads();
   }
-  return (int)(uint32_t)key;
+  float* carry = (float*)(lds4 + count);
+  switch (f.src_fmt) {
+    case kRgb8: rows_frame<KIND, kRgb8>(f, ent, count, carry, w, h, diffusion); break;
+    case kRgba8: rows_frame<KIND, kRgba8>(f, ent, count, carry, w, h, diffusion); break;
+    case kRgb16: rows_frame<KIND, kRgb16>(f, ent, count, carry, w, h, diffusion); break;
+    case kRgba16: rows_frame<KIND, kRgba16>(f, ent, count, carry, w, h, diffusion); break;
+    case kRgbaF32: rows_frame<KIND, kRgbaF32>(f, ent, count, carry, w, h, diffusion); break;
+    case kYuv444: rows_frame<KIND, kYuv444>(f, ent, count, carry, w, h, diffusion); break;
+    default: rows_frame<KIND, kYuv420>(f, ent, count, carry, w, h, diffusion); break;
+  }
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v, float fill) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// The first index at the wave's smallest d (d >= 0 or +inf, never NaN; lane l
+// offers its own first minimum li): a row-wise DPP minimum, the four row
+// minima, then the lowest index among the lanes holding it.
+__device__ __forceinline__ int wave_argmin(float d, int li) {
+  const float inf = __builtin_inff();
+  float m = d;
+  m = fminf(m, dppf<0x111>(m, inf));  // row_shr:1
+  m = fminf(m, dppf<0x112>(m, inf));  // row_shr:2
+  m = fminf(m, dppf<0x114>(m, inf));  // row_shr:4
+  m = fminf(m, dppf<0x118>(m, inf));  // row_shr:8
+  const float m0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), 15));
+  const float m1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), 31));
+  const float m2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), 47));
+  const float m3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), 63));
+  const float mm = fminf(fminf(m0, m1), fminf(m2, m3));
+  uint64_t tied = __ballot(d == mm);
+  int best = 0x7FFFFFFF;
+  while (tied) {
+    const int l = __builtin_ctzll(tied);
+    tied &= tied - 1;
+    const int v = __builtin_amdgcn_readlane(li, l);
+    best = v < best ? v : best;
+  }
+  return best;
 }
 
 struct JTap {
@@ -224,25 +377,22 @@ __device__ __forceinline__ uint64_t jarvis_target(int k, uint64_t x, uint64_t y,
 }
 
 // jarvis (src/rd_dithers.cc:150-220), one frame per wavefront.
-__global__ __launch_bounds__(kWave) void k_diffuse_jarvis(const ScreenFrame* __restrict__ frames,
-                                                          int64_t w, int64_t h, float diffusion) {
-  extern __shared__ float ring[];  // 3 rows x (r, g, b planes) of w floats
-  const ScreenFrame f = frames[blockIdx.x];
-  const float4* ent = (const float4*)((const unsigned char*)f.pal->image + f.pal->off_ent);
-  const int count = f.pal->count;
+template <int FMT>
+__device__ __forceinline__ void jarvis_frame(const ScreenFrame& f, const float4* __restrict__ ent,
+                                             int count, float* ring, int64_t w, int64_t h,
+                                             float diffusion) {
   const int lane = threadIdx.x;
   const int64_t n = w * h;
   for (int64_t k = lane; k < 9 * w; k += kWave) ring[k] = 0.f;
   __syncthreads();
   // Pixels in chunks of 64 in raster order, one per lane; the next chunk is
   // loaded while this one is dithered, the choices stored after it.
-  float4 cur = lane < n ? load_any(f.src_fmt, f.src, lane, w, h) : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 cur = raw_pixel<FMT>(lane < n ? load_raw<FMT>(f.src, lane, w, h) : make_uint4(0, 0, 0, 0));
   int chosen = 0;
   float4 chosen_c = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int64_t base = 0; base < n; base += kWave) {
     const int64_t nb = base + kWave + lane;
-    const float4 next = nb < n ? load_any(f.src_fmt, f.src, nb, w, h)
-                               : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint4 next = nb < n ? load_raw<FMT>(f.src, nb, w, h) : make_uint4(0, 0, 0, 0);
     const int m = (int)(n - base < kWave ? n - base : kWave);
     for (int j = 0; j < m; ++j) {
       const int64_t i = base + j;
@@ -318,22 +468,47 @@ __global__ __launch_bounds__(kWave) void k_diffuse_jarvis(const ScreenFrame* __r
       }
       __syncthreads();
     }
-    if (lane < m) store_pick(f, base + lane, n, chosen, chosen_c, cur.w);
-    cur = next;
+    if (lane < m) store_choice(f, base + lane, n, chosen, chosen_c, cur.w);
+    cur = raw_pixel<FMT>(next);
+  }
+}
+
+__global__ __launch_bounds__(kWave) void k_diffuse_jarvis(const ScreenFrame* __restrict__ frames,
+                                                          int64_t w, int64_t h, float diffusion) {
+  // LDS: the entries (float4 each), then 3 rows x (r, g, b planes) of w floats.
+  extern __shared__ float4 lds4[];
+  const ScreenFrame f = frames[blockIdx.x];
+  const int count = __builtin_amdgcn_readfirstlane(f.pal->count);
+  const int lane = threadIdx.x;
+  float4* ent = lds4;
+  {
+    const float4* g = (const float4*)((const unsigned char*)f.pal->image + f.pal->off_ent);
+    for (int k = lane; k < count; k += kWave) ent[k] = g[k];
+  }
+  float* ring = (float*)(lds4 + count);
+  switch (f.src_fmt) {
+    case kRgb8: jarvis_frame<kRgb8>(f, ent, count, ring, w, h, diffusion); break;
+    case kRgba8: jarvis_frame<kRgba8>(f, ent, count, ring, w, h, diffusion); break;
+    case kRgb16: jarvis_frame<kRgb16>(f, ent, count, ring, w, h, diffusion); break;
+    case kRgba16: jarvis_frame<kRgba16>(f, ent, count, ring, w, h, diffusion); break;
+    case kRgbaF32: jarvis_frame<kRgbaF32>(f, ent, count, ring, w, h, diffusion); break;
+    case kYuv444: jarvis_frame<kYuv444>(f, ent, count, ring, w, h, diffusion); break;
+    default: jarvis_frame<kYuv420>(f, ent, count, ring, w, h, diffusion); break;
   }
 }
 
 }  // namespace
 
-size_t diffuse_lds_bytes(int kind, int64_t w) {
-  if (kind == 2) return sizeof(float) * 9 * (size_t)w;
-  return sizeof(float) * (kind == 1 ? 6 : 3) * (size_t)w;
+size_t diffuse_lds_bytes(int kind, int64_t w, int count) {
+  const size_t ent = 16 * (size_t)count;
+  if (kind == 2) return ent + sizeof(float) * 9 * (size_t)w;
+  return ent + sizeof(float) * (kind == 1 ? 6 : 3) * (size_t)w;
 }
 
 hipError_t launch_diffuse(const ScreenFrame* frames, int nframes, int64_t w, int64_t h, int kind,
-                          float diffusion, hipStream_t s) {
+                          int max_count, float diffusion, hipStream_t s) {
   if (nframes <= 0 || w * h <= 0) return hipSuccess;
-  const size_t lds = diffuse_lds_bytes(kind, w);
+  const size_t lds = diffuse_lds_bytes(kind, w, max_count);
   auto go = [&](auto kernel) -> hipError_t {
     hipError_t e = hipFuncSetAttribute((const void*)kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

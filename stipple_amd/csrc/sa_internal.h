@@ -302,9 +302,9 @@ hipError_t launch_screen(const ScreenFrame* frames, int nframes, int64_t w, int6
 std::vector<uint16_t> void_cluster_ranks(int n);
 // sa_diffuse.hip: the error-diffusion dithers (kind 0 floyd-steinberg, 1
 // atkinson, 2 jarvis), one frame per wavefront; LDS bytes per workgroup.
-size_t diffuse_lds_bytes(int kind, int64_t w);
+size_t diffuse_lds_bytes(int kind, int64_t w, int count);
 hipError_t launch_diffuse(const ScreenFrame* frames, int nframes, int64_t w, int64_t h, int kind,
-                          float diffusion, hipStream_t s);
+                          int max_count, float diffusion, hipStream_t s);
 extern const uint16_t kBayer8[64];
 extern const uint16_t kDot8[64];
 
