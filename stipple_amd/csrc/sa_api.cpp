@@ -281,6 +281,12 @@ hipError_t timed(const char* name, hipStream_t s, Fn&& launch) {
   return e;
 }
 
+// Staged inputs of one image: its pixels in visit order at the source's
+// width, and 16 bytes of padding (the walk loads 16 bytes from each start).
+size_t stage_bytes(int fmt, int64_t px) {
+  return (((size_t)px << sa::seq_stage_shift(fmt)) + 16 + 255) & ~(size_t)255;
+}
+
 bool valid_in_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kRgba16; }
 bool valid_out_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kIndex16; }
 // The block engine and the palette build also read planar YUV 4:4:4 / 4:2:0;
@@ -762,22 +768,22 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     int S = o.memo_bits;
     if (S <= 0) {
       S = 10;
-      while (S > 4 && (132u << S) + (size_t)max_img > 158 * 1024) --S;
+      while (S > 4 && (132u << S) + (size_t)max_img + sa::kSeqRingBytes > 158 * 1024) --S;
       const int tuned = g_knobs.seq_memo_bits.load();
       if (tuned > 0 && tuned < S) S = tuned;
     }
     if (S < 4 || S > 10) return fail(err, errlen, SA_EINVAL, "memo_bits %d outside 4..10", S);
-    const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)max_img;
+    const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)max_img + sa::kSeqRingBytes;
     if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "LDS %zu too large", lds);
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
     int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
     waves = std::min<int>(waves, (int)sel.size());
-    // Scratch: gathered inputs, indices, HBM memo tables.
+    // Scratch: staged inputs, indices, HBM memo tables.
     std::vector<sa::SeqItem> items(sel.size());
     size_t xs_bytes = 0, idx_bytes = 0, spill_bytes = 0;
     for (size_t k = 0; k < sel.size(); ++k) {
       const int64_t px = images[sel[k]].width * images[sel[k]].height;
-      xs_bytes += 32 * (size_t)px;
+      xs_bytes += stage_bytes(images[sel[k]].src_fmt, px);
       idx_bytes += (2 * (size_t)px + 255) & ~(size_t)255;
       if (o.use_cache) spill_bytes += (size_t)2 << K;
     }
@@ -799,8 +805,8 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       it.src = im.src;
       it.dst = im.dst;
       it.curve = curve;
-      it.xs = (double*)p;
-      p += 32 * (size_t)px;
+      it.stage = p;
+      p += stage_bytes(im.src_fmt, px);
       it.idx = (uint16_t*)p;
       p += (2 * (size_t)px + 255) & ~(size_t)255;
       it.memo = nullptr;
@@ -841,7 +847,7 @@ namespace {
 int memo_bits_for(const sa_seq_options& o, int img_bytes) {
   if (o.memo_bits > 0) return o.memo_bits;
   int S = 10;
-  while (S > 4 && (132u << S) + (size_t)img_bytes > 158 * 1024) --S;
+  while (S > 4 && (132u << S) + (size_t)img_bytes + sa::kSeqRingBytes > 158 * 1024) --S;
   const int tuned = g_knobs.seq_memo_bits.load();
   if (tuned > 0 && tuned < S) S = tuned;
   return S;
@@ -988,7 +994,8 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     if (im.palette->count > 65535)
       return fail(err, errlen, SA_EUNSUPPORTED, "image %d: the walk's memo holds indices below 65535", i);
     const int S = memo_bits_for(o, im.palette->hdr.bytes);
-    const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)im.palette->hdr.bytes;
+    const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)im.palette->hdr.bytes +
+                        sa::kSeqRingBytes;
     if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "image %d: LDS %zu too large", i, lds);
   }
   std::vector<JobItem> fresh;
@@ -1003,7 +1010,7 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     const int K = J.assoc ? 24 : 18;
     const int S = memo_bits_for(o, J.img_bytes);
     const int64_t px = im.width * im.height;
-    const size_t xs_b = 32 * (size_t)px, idx_b = (2 * (size_t)px + 255) & ~(size_t)255;
+    const size_t xs_b = stage_bytes(im.src_fmt, px), idx_b = (2 * (size_t)px + 255) & ~(size_t)255;
     const size_t memo_b = o.use_cache ? ((size_t)2 << K) : 0;
     const size_t st_b = (sizeof(sa::SeqState) + 255) & ~(size_t)255;
     const size_t snap_b = o.use_cache ? ((size_t)132 << S) : 0;
@@ -1027,7 +1034,7 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     it.src = im.src;
     it.dst = im.dst;
     it.curve = curve;
-    it.xs = (double*)p;
+    it.stage = p;
     p += xs_b;
     it.idx = (uint16_t*)p;
     p += idx_b;
@@ -1100,7 +1107,8 @@ int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int
     size_t lds = 0;
     for (size_t k : sel) {
       const JobItem& J = job->items[k];
-      lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes);
+      lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes +
+                                 sa::kSeqRingBytes);
     }
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
     int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
@@ -1266,7 +1274,8 @@ extern "C" int sa_walk_job_step_overlapped(sa_walk_job job, double budget_us, vo
       for (const JobItem& J : job->items) {
         if (J.assoc != assoc) continue;
         cand.push_back(&J);
-        lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes);
+        lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes +
+                                 sa::kSeqRingBytes);
       }
       if (cand.empty()) continue;
       const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));

@@ -160,12 +160,23 @@ struct SeqState {
   double e[64];
 };
 
+// The sequential walk stages each visit's source pixel in visit order, in the
+// source's own width: 8-bit RGB(A) as 4 bytes (alpha 255 for RGB), 16-bit as
+// 8 bytes (alpha 65535 for RGB), float RGBA as 16.  log2 of that width:
+__host__ __device__ inline int seq_stage_shift(int fmt) {
+  return (fmt == kRgb8 || fmt == kRgba8) ? 2 : (fmt == kRgb16 || fmt == kRgba16) ? 3 : 4;
+}
+// LDS the walk keeps beside the memo cache and the search image: the next
+// 32 visits' associated inputs (4 doubles each).
+constexpr int kSeqRingBytes = 32 * 4 * 8;
+
 // One image of the sequential engine (device array of these).
 struct SeqItem {
   const void* src;        // source pixels (raster)
   void* dst;              // output (raster), may alias src for kRgbaF32
   const int32_t* curve;   // visit order: w*h pixel ids
-  double* xs;             // scratch: 4 doubles per visit (associated input)
+  void* stage;            // scratch: the source pixel of each visit, in visit order
+                          // (1 << seq_stage_shift(src_fmt) bytes each, +16 B of padding)
   uint16_t* idx;          // scratch: palette index per visit
   uint16_t* memo;         // scratch: 1<<K memo entries (K = 18, or 24 with alpha)
   const PaletteDev* pal;  // device palette header

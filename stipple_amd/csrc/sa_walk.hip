@@ -16,11 +16,20 @@
 // summed left to right.  Only the last term depends on the visit just
 // finished, so when e_{t-1} is known every in-flight visit adds its own tap of
 // it -- lane j adds w_{15-j} -- in ONE v_mul_f64 + ONE v_add_f64 for all
-// channels.  Lane 0 is then complete; after the visit every row rotates left by
-// one lane (DPP row_ror:15) and lane 15 restarts with x_{t+16}, taken from a
-// register chunk of the next inputs that rotates in step (prefetched two
-// chunks ahead with coalesced loads).  The serial chain per visit is one mul,
-// one add, the clamp, the memo probe and the residual.
+// channels.  Lane 0 is then complete; after the visit every row shifts left by
+// one lane (DPP row_shl:1) and lane 15, which has no source, keeps the DPP's
+// fill value: x_{t+16}, read from an LDS ring of the next inputs.
+//
+// Inputs.  A gather kernel stages each visit's source pixel in visit order at
+// the source's own width (4 bytes for 8-bit RGBA); at the start of each
+// 16-visit chunk the walk associates the next chunk's pixels in registers
+// (loaded one chunk earlier) and writes them to the ring.
+//
+// The serial chain per visit is: tap (mul, add), the channel byte, the memo
+// key (four lane reads, scalar ORs and hash), the memo read, and the residual
+// against the palette colour -- which is read from LDS only when the index
+// differs from the previous visit's (55-70% of visits repeat it).  The clamped
+// target is formed in the shadow of the memo read.
 //
 // Memo (quantize.c's cache, src/rd_riemersma_cpu.cpp:75-94).  First-touch
 // semantics are observable, so it is reproduced exactly: a zeroed per-image
@@ -38,6 +47,9 @@
 // reference's `<=` rule).
 #include "sa_internal.h"
 #include "sa_math.h"
+
+// v_writelane_b32: `v` with lane `lane` (uniform) replaced by the uniform `s`.
+__device__ int llvm_writelane(int s, int lane, int v) __asm("llvm.amdgcn.writelane.i32");
 
 namespace sa {
 namespace {
@@ -58,7 +70,6 @@ __device__ __forceinline__ double dpp(double v, double fill) {
 }
 
 constexpr int kRowBcast0 = 0x150;  // row_newbcast:0
-constexpr int kRowRor15 = 0x12F;   // lane j <- lane (j+1) mod 16
 constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
 
 // Minimum over the wavefront of a per-lane double (lanes outside the
@@ -135,7 +146,24 @@ __device__ __forceinline__ int lookup(const Search& s, unsigned c0, unsigned c1,
 // global_load/global_store (vmcnt only), so they never hold up an LDS wait the
 // way a flat access would.
 typedef const double __attribute__((address_space(1))) gdouble;
+typedef const uint32_t __attribute__((address_space(1))) gcu32;
 typedef uint16_t __attribute__((address_space(1))) gu16;
+
+__device__ __forceinline__ int64_t rfl64(int64_t v) {
+  const uint32_t lo = (uint32_t)rfl((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)rfl((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// DPP move in which every lane has a source (no fill value needed).
+template <int CTRL>
+__device__ __forceinline__ double dpp_all(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+constexpr int kRowShl1 = 0x101;  // lane j <- lane j+1; lane 15 has no source
 
 // LDS cache of the HBM memo table's 64-entry lines.
 struct Memo {
@@ -144,18 +172,18 @@ struct Memo {
   uint16_t* table;  // HBM, 1 << K entries, zeroed before the walk
   int bits;         // log2(slots)
   // This lane's channel part of the key (memo_key_part for its row):
-  // ((c8 >> k_hi_shr) & k_hi_mask) << k_hi_shl | ((c8 >> 2) & k_lo_mask) << k_lo_shl.
-  uint32_t k_hi_shr, k_hi_mask, k_hi_shl, k_lo_mask, k_lo_shl;
+  // bfe(c8, k_hi_off, k_hi_w) << k_hi_shl | bfe(c8, 2, k_lo_w) << k_lo_shl.
+  uint32_t k_hi_off, k_hi_w, k_hi_shl, k_lo_w, k_lo_shl;
 };
 
 // Rows 0..2 (r, g, b) give their top 4 bits to the line and bits 2..3 to the
 // offset; row 3 (alpha) gives its top 6 bits to the line when associated.
 template <bool ASSOC>
 __device__ __forceinline__ void memo_key_consts(Memo& m, int ch) {
-  m.k_hi_shr = ch == 3 ? 2u : 4u;
-  m.k_hi_mask = ch == 3 ? (ASSOC ? 63u : 0u) : 15u;
+  m.k_hi_off = ch == 3 ? 2u : 4u;
+  m.k_hi_w = ch == 3 ? (ASSOC ? 6u : 0u) : 4u;
   m.k_hi_shl = ch == 3 ? 18u : (uint32_t)(6 + 4 * ch);
-  m.k_lo_mask = ch == 3 ? 0u : 3u;
+  m.k_lo_w = ch == 3 ? 0u : 2u;
   m.k_lo_shl = ch == 3 ? 0u : (uint32_t)(2 * ch);
 }
 
@@ -189,157 +217,220 @@ struct Counters {
   uint32_t misses = 0, fills = 0, loads = 0;
 };
 
-
-// Inputs of visit v (clamped into the image, so the load never branches and
-// the wave does not wait for it until the value is used).
-__device__ __forceinline__ double load_x(const double* xs, int64_t v, int64_t n, int ch) {
-  return ((gdouble*)xs)[4 * (v < n ? v : n - 1) + ch];
+// The staged source pixel of visit v (clamped into the image, so the loads
+// never branch): 16 bytes from its start, which the stage's padding covers.
+__device__ __forceinline__ uint4 load_stage(const unsigned char* stage, int shift, int64_t v,
+                                            int64_t n) {
+  const gcu32* p = (const gcu32*)(stage + ((v < n ? v : n - 1) << shift));
+  return make_uint4(p[0], p[1], p[2], p[3]);
 }
 
-struct WalkState {
-  double acc;     // running sum of this lane's in-flight visit
-  double e;       // residual of the visit completed last (lane 0 of each row)
-  uint32_t obuf;  // palette indices of the current 64-visit window
+// This lane's channel of a staged pixel as the walk's input: Q16 floats as
+// load_pixel reads them, associated as AssociateAlphaPixel does
+// (include/rd_types.h:179-192), +0.0 (the zero taps a visit near the start
+// adds turn -0.0 into +0.0, exactly as the reference's first adds do).
+template <bool ASSOC>
+__device__ __forceinline__ double stage_input(uint4 w, int shift, int ch) {
+  float c, a;
+  if (shift == 2) {
+    c = (float)((w.x >> (8 * ch)) & 255u) * 257.0f;
+    a = (float)(w.x >> 24) * 257.0f;
+  } else if (shift == 3) {
+    const uint32_t h = ch < 2 ? w.x : w.y;
+    c = (float)((h >> (16 * (ch & 1))) & 0xFFFFu);
+    a = (float)(w.y >> 16);
+  } else {
+    c = __uint_as_float(ch == 0 ? w.x : ch == 1 ? w.y : ch == 2 ? w.z : w.w);
+    a = __uint_as_float(w.w);
+  }
+  double x = (double)c;
+  if (ASSOC && ch < 3 && a != 65535.0f) x = (kScale * (double)a) * x;
+  return x + 0.0;
+}
+
+// The next inputs wait in an LDS ring of kRing visits, 4 doubles each; a lane
+// reads and writes its own channel (`ring` points at it).
+constexpr int kRing = kSeqRingBytes / 32;
+
+// The channel byte scale_char_clamped(clamp_pixel(v)) from the sum itself:
+// for v in [0, 65535) the float is clamp_pixel's; below 0 the sum s + 0.5 is
+// at most 0.5 (and a NaN gives 0 through the max), and at or above 65535 (or
+// inf) the final min gives 255, as the clamp would.  Two dependent operations
+// fewer than clamping first.
+__device__ __forceinline__ unsigned key_char(double v) {
+  const float s = (float)((double)(float)v * kInv257);
+  return (unsigned)fminf(fmaxf(s + 0.5f, 0.0f), 255.0f);
+}
+
+struct Visit {
+  double acc;      // running sum of this lane's in-flight visit
+  double eb;       // residual of the last visit, in every lane of its row
+  double col;      // this row's channel of palette colour `last`
+  uint32_t obuf;   // palette indices of the current 64-visit window
+  int last;        // index of the last visit (-1: none yet)
 };
 
-// One visit (visit t); on return lane j holds visit t+1+j.  `xq` is the
-// current chunk's input register, rotating in step with the accumulators.
+// One visit (visit t); on return lane j holds visit t+1+j.
 template <bool ASSOC, bool CACHE>
-__device__ __forceinline__ void visit(WalkState& w, double& xq, double W, int64_t t, int64_t n,
-                                      uint16_t* idx_out, const Memo& m, const Search& s,
-                                      Counters& cnt) {
+__device__ __forceinline__ void visit(Visit& w, double W, int64_t t, double* ring, const Memo& m,
+                                      const Search& s, Counters& cnt) {
   const int lane = threadIdx.x;
-  const int j = lane & 15;
   const int ch = lane >> 4;
-  // Every in-flight visit adds its tap of e_{t-1} (held by lane 0).
-  w.acc = w.acc + W * dpp<kRowBcast0>(w.e, 0.0);
-  const float pc = clamp_pixel(w.acc);
-  const double pd = (double)pc;
-  const unsigned c8 = scale_char_clamped(pc);
+  // x_{t+16}, which lane 15 restarts with at the end of the visit; read
+  // first, so its wait is long over when the memo reads complete (LDS reads
+  // return in order).
+  const double xin = ring[4 * ((t + 16) & (kRing - 1))];
+  __builtin_amdgcn_sched_barrier(0);
+  // Every in-flight visit adds its tap of e_{t-1}; lane 0 is then complete.
+  w.acc = w.acc + W * w.eb;
+  const unsigned c8 = key_char(w.acc);
+  // The clamped target itself (the residual's minuend and the search's
+  // target) is formed while the memo read is in flight.
+  double pd, pdb;
+  auto target = [&]() {
+    pd = (double)clamp_pixel(w.acc);
+    pdb = dpp_all<kRowBcast0>(pd);
+  };
   // The channel bytes of visit t sit in lanes 0, 16, 32, 48.
-  auto chan = [&]() {
-    struct C4 {
-      unsigned c0, c1, c2, c3;
-    };
-    return C4{(unsigned)rl((int)c8, 0), (unsigned)rl((int)c8, 16), (unsigned)rl((int)c8, 32),
-              ASSOC ? (unsigned)rl((int)c8, 48) : 0u};
+  auto search = [&]() {
+    ++cnt.misses;
+    return lookup<ASSOC>(s, (unsigned)rl((int)c8, 0), (unsigned)rl((int)c8, 16),
+                         (unsigned)rl((int)c8, 32), ASSOC ? (unsigned)rl((int)c8, 48) : 0u,
+                         rl(pd, 0), rl(pd, 16), rl(pd, 32), ASSOC ? rl(pd, 48) : 65535.0);
   };
   int idx;
   if (CACHE) {
     // Each row forms its channel's part of the memo key (memo_key_part with
     // this lane's constants: no branch); four lane reads and three ORs join
     // them (the channel bytes themselves are read only on a miss).
-    const uint32_t kc = (((c8 >> m.k_hi_shr) & m.k_hi_mask) << m.k_hi_shl) |
-                        (((c8 >> 2) & m.k_lo_mask) << m.k_lo_shl);
+    const uint32_t kc =
+        (__builtin_amdgcn_ubfe(c8, m.k_hi_off, m.k_hi_w) << m.k_hi_shl) |
+        (__builtin_amdgcn_ubfe(c8, 2u, m.k_lo_w) << m.k_lo_shl);
     uint32_t key = (uint32_t)rl((int)kc, 0) | (uint32_t)rl((int)kc, 16) | (uint32_t)rl((int)kc, 32);
     if (ASSOC) key |= (uint32_t)rl((int)kc, 48);
     const uint32_t line = key >> 6;
     const uint32_t slot = memo_slot(m, line);
-    const uint32_t tg = (uint32_t)rfl((int)m.tag[slot]);
-    uint32_t ent = (uint32_t)rfl((int)m.line[64 * slot + (key & 63)]);
-    if (__builtin_expect((tg & ~kDirty) != line, 0)) {
-      ++cnt.fills;
-      cnt.loads += memo_fill(m, slot, line, tg);
-      ent = (uint32_t)rfl((int)m.line[64 * slot + (key & 63)]);
-    }
-    if (__builtin_expect(ent != 0, 1)) {
-      idx = (int)ent - 1;
-    } else {
-      ++cnt.misses;
-      const auto c = chan();
-      idx = lookup<ASSOC>(s, c.c0, c.c1, c.c2, c.c3, rl(pd, 0), rl(pd, 16), rl(pd, 32),
-                          ASSOC ? rl(pd, 48) : 65535.0);
-      if (lane == 0) {
-        m.line[64 * slot + (key & 63)] = (uint16_t)(idx + 1);
-        m.tag[slot] = line | kDirty;
+    const uint32_t tgv = m.tag[slot];
+    const uint32_t entv = m.line[64 * slot + (key & 63)];
+    // Nothing else is scheduled ahead of these reads (in-order issue: work
+    // placed before them would delay the chain), and the target is formed
+    // behind them.
+    __builtin_amdgcn_sched_barrier(0);
+    target();
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t tg = (uint32_t)rfl((int)tgv);
+    uint32_t ent = (uint32_t)rfl((int)entv) & 0xFFFFu;
+    idx = (int)ent - 1;
+    // The hit (line resident, entry touched) falls through: (ent - 1) >> 16
+    // is nonzero only for ent == 0.
+    if (__builtin_expect((((tg & ~kDirty) ^ line) | ((ent - 1u) >> 16)) != 0, 0)) {
+      if ((tg & ~kDirty) != line) {
+        ++cnt.fills;
+        cnt.loads += memo_fill(m, slot, line, tg);
+        ent = (uint32_t)rfl((int)m.line[64 * slot + (key & 63)]);
+        idx = (int)ent - 1;
+      }
+      if (ent == 0) {
+        idx = search();
+        if (lane == 0) {
+          m.line[64 * slot + (key & 63)] = (uint16_t)(idx + 1);
+          m.tag[slot] = line | kDirty;
+        }
       }
     }
   } else {
-    ++cnt.misses;
-    const auto c = chan();
-    idx = lookup<ASSOC>(s, c.c0, c.c1, c.c2, c.c3, rl(pd, 0), rl(pd, 16), rl(pd, 32),
-                        ASSOC ? rl(pd, 48) : 65535.0);
+    target();
+    idx = search();
   }
-  // Residual against the associated palette colour, per row's channel.
-  w.e = pd - s.col[4 * idx + ch];
-  w.obuf = lane == (int)(t & 63) ? (uint32_t)idx : w.obuf;
-  if ((t & 63) == 63 || t + 1 == n) {
-    const int64_t v = (t & ~(int64_t)63) + lane;
-    if (v <= t) ((gu16*)idx_out)[v] = (uint16_t)w.obuf;
+  // Residual against the associated palette colour, per row's channel, formed
+  // in every lane from lane 0's target.  Consecutive visits often take the
+  // same entry: its colour is then already in a register.
+  if (__builtin_expect(idx != w.last, 0)) {
+    w.col = s.col[4 * idx + ch];
+    w.last = idx;
   }
-  // Rotate: lane j takes visit t+1+j; lane 15 restarts with x_{t+16}.  (+0.0
-  // is that visit's zero-error taps: it turns -0.0 into +0.0 as the
-  // reference's first adds do.)
-  w.acc = dpp<kRowRor15>(w.acc, 0.0);
-  xq = dpp<kRowRor15>(xq, 0.0);
-  w.acc = j == 15 ? xq + 0.0 : w.acc;
+  w.eb = pdb - w.col;
+  w.obuf = (uint32_t)llvm_writelane(idx, (int)(t & 63), (int)w.obuf);
+  // Shift: lane j takes visit t+1+j; lane 15 restarts with x_{t+16}.
+  w.acc = dpp<kRowShl1>(w.acc, xin);
 }
 
-// Chunk c (visits 16c .. 16c+15) restarts from its register `xq`, loaded two
-// chunks earlier; afterwards `xq` is refilled for chunk c+3.  Returns false
-// when `stop` (the image's end, or where this launch's slice ends) is reached.
-template <bool ASSOC, bool CACHE>
-__device__ __forceinline__ bool chunk(WalkState& w, double& xq, double W, int64_t base, int64_t n,
-                                      int64_t stop, const SeqItem& it, const Memo& m,
-                                      const Search& s, Counters& cnt) {
-  const int lane = threadIdx.x;
-  const int64_t end = base + 16 < stop ? base + 16 : stop;
-  for (int64_t t = base; t < end; ++t) visit<ASSOC, CACHE>(w, xq, W, t, n, it.idx, m, s, cnt);
-  xq = load_x(it.xs, base + 64 + (lane & 15), n, lane >> 4);
-  return end < stop;
+// Writes the indices of visits (t & ~63) .. t (the window's filled part).
+__device__ __forceinline__ void flush_window(uint16_t* idx_out, uint32_t obuf, int64_t t) {
+  const int64_t v = (t & ~(int64_t)63) + threadIdx.x;
+  if (v <= t) ((gu16*)idx_out)[v] = (uint16_t)obuf;
 }
 
 // Visits [t0, stop).  t0 and stop - t0 are multiples of 48 unless stop is the
-// end, so the three chunk registers restart in their usual roles.  A walk that
-// stops early leaves its queue (the in-flight sums and residuals) in it.state
-// and flushes the indices of its last, partial 64-visit window; resuming loads
-// them back, so a sliced walk computes exactly what an uninterrupted one does.
-// `deadline` (a s_memrealtime tick, 0: none) ends the walk at the first 48-visit
-// boundary past it.  Returns where the walk stopped.
+// end.  A walk that stops early leaves its queue (the in-flight sums and
+// residuals) in it.state and flushes the indices of its last, partial 64-visit
+// window; resuming loads them back, so a sliced walk computes exactly what an
+// uninterrupted one does.  `deadline` (a s_memrealtime tick, 0: none) ends the
+// walk at the first 48-visit boundary past it.  Returns where the walk stopped.
+//
+// Inputs: at the start of each 16-visit chunk the next chunk's staged pixels
+// (loaded one chunk earlier) are associated and written to the ring, and the
+// chunk after that is loaded; each visit reads its x_{t+16} from the ring.
 template <bool ASSOC, bool CACHE>
 __device__ int64_t walk_image(const SeqItem& it, double W, const Memo& m, const Search& s,
-                              Counters& cnt, int64_t t0, int64_t stop, uint64_t deadline) {
+                              double* ring_base, Counters& cnt, int64_t t0, int64_t stop,
+                              uint64_t deadline) {
   const int lane = threadIdx.x;
   const int j = lane & 15;
   const int ch = lane >> 4;
-  const int64_t n = it.n;
-  WalkState w;
+  const int64_t n = rfl64(it.n);
+  t0 = rfl64(t0);
+  stop = rfl64(stop);
+  const int shift = rfl(seq_stage_shift(it.src_fmt));
+  const unsigned char* stage = (const unsigned char*)it.stage;
+  double* ring = ring_base + ch;
+  Visit w;
+  w.last = -1;
+  w.col = 0.0;
   if (t0 == 0) {
-    w.e = 0.0;
+    w.eb = 0.0;
     w.obuf = 0;
-    w.acc = load_x(it.xs, j, n, ch) + 0.0;
+    w.acc = stage_input<ASSOC>(load_stage(stage, shift, j, n), shift, ch);
   } else {
     w.acc = ((gdouble*)it.state->acc)[lane];
-    w.e = ((gdouble*)it.state->e)[lane];
+    w.eb = ((gdouble*)it.state->e)[lane];
     const int64_t v = (t0 & ~(int64_t)63) + lane;
     w.obuf = v < t0 ? (uint32_t)((const gu16*)it.idx)[v] : 0u;
-    // Settle these loads here, before the prefetches below: left pending, the
-    // compiler waits for them at the top of every visit of the first chunk,
-    // which then also drains the prefetches once per 48 visits.
-    __builtin_amdgcn_s_waitcnt(0);
   }
-  double x0 = load_x(it.xs, t0 + 16 + j, n, ch);
-  double x1 = load_x(it.xs, t0 + 32 + j, n, ch);
-  double x2 = load_x(it.xs, t0 + 48 + j, n, ch);
-  // Three chunk registers used round-robin: no register copy ever waits on
-  // an outstanding load.
-  for (int64_t base = t0;; base += 48) {
-    if (deadline && base > t0 && __builtin_amdgcn_s_memrealtime() >= deadline) {
-      stop = base;
-      break;
+  ring[4 * ((t0 + 16 + j) & (kRing - 1))] =
+      stage_input<ASSOC>(load_stage(stage, shift, t0 + 16 + j, n), shift, ch);
+  // Settle these loads here, before the prefetch below: left pending, the
+  // compiler waits for them inside the first chunk instead.
+  __builtin_amdgcn_s_waitcnt(0);
+  uint4 raw = load_stage(stage, shift, t0 + 32 + j, n);
+  int third = 0;
+  for (int64_t base = t0;; base += 16) {
+    if (base != t0) {
+      if (++third == 3) {
+        third = 0;
+        if (deadline && __builtin_amdgcn_s_memrealtime() >= deadline) {
+          stop = base;
+          break;
+        }
+      }
+      ring[4 * ((base + 16 + j) & (kRing - 1))] = stage_input<ASSOC>(raw, shift, ch);
+      raw = load_stage(stage, shift, base + 32 + j, n);
     }
-    if (!chunk<ASSOC, CACHE>(w, x0, W, base, n, stop, it, m, s, cnt)) break;
-    if (!chunk<ASSOC, CACHE>(w, x1, W, base + 16, n, stop, it, m, s, cnt)) break;
-    if (!chunk<ASSOC, CACHE>(w, x2, W, base + 32, n, stop, it, m, s, cnt)) break;
+    // A 32-bit count keeps the loop test on the scalar unit (it has no
+    // 64-bit ordered compare).
+    const int len = rfl((int)(stop - base < 16 ? stop - base : 16));
+    for (int k = 0; k < len; ++k) visit<ASSOC, CACHE>(w, W, base + k, ring, m, s, cnt);
+    const int64_t last = base + len - 1;
+    if ((last & 63) == 63 || last + 1 == n) flush_window(it.idx, w.obuf, last);
+    if (len < 16 || base + 16 == stop) break;
   }
   if (stop < n) {
+    // The last window is partial unless stop - 1 ended it (then it was
+    // flushed in the loop); the queue stays for the next launch.
     const int64_t last = stop - 1;
-    if ((last & 63) != 63) {
-      const int64_t v = (last & ~(int64_t)63) + lane;
-      if (v <= last) ((gu16*)it.idx)[v] = (uint16_t)w.obuf;
-    }
+    if ((last & 63) != 63) flush_window(it.idx, w.obuf, last);
     it.state->acc[lane] = w.acc;
-    it.state->e[lane] = w.e;
+    it.state->e[lane] = w.eb;
   }
   return stop;
 }
@@ -414,7 +505,8 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     m.table = it.memo;
     const uint64_t deadline =
         (budget && it.state) ? __builtin_amdgcn_s_memrealtime() + budget : 0;
-    stop = walk_image<ASSOC, CACHE>(it, W, m, s, cnt, t0, stop, deadline);
+    double* ring = (double*)(img + pd.bytes);
+    stop = walk_image<ASSOC, CACHE>(it, W, m, s, ring, cnt, t0, stop, deadline);
     __syncthreads();
     if (it.state) {
       if (CACHE && stop < it.n) copy16((uint4*)it.snap, (const uint4*)lds, memo_bytes / 16);
@@ -451,22 +543,30 @@ __global__ __launch_bounds__(256) void memo_clear_kernel(const SeqItem* __restri
     t[k] = make_uint4(0, 0, 0, 0);
 }
 
-// Gather: visit order, associated, +0.0 (the zero taps a visit near the start
-// adds turn -0.0 into +0.0, exactly as the reference's first adds do).
+// Gather: each visit's source pixel in visit order, in the source's width
+// (seq_stage_shift); the walk associates it.
 __global__ __launch_bounds__(256) void seq_gather_kernel(const SeqItem* __restrict__ items) {
   const SeqItem it = items[blockIdx.y];
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= it.n) return;
-  const bool assoc = it.pal->assoc != 0;
-  const float4 p = load_pixel(it.src_fmt, it.src, it.curve[v]);
-  double x[4];
-  assoc_pixel(assoc, p.x, p.y, p.z, p.w, x);
-  double4 o;
-  o.x = x[0] + 0.0;
-  o.y = x[1] + 0.0;
-  o.z = x[2] + 0.0;
-  o.w = x[3] + 0.0;
-  ((double4*)it.xs)[v] = o;
+  const int64_t p = it.curve[v];
+  switch (it.src_fmt) {
+    case kRgb8: {
+      const uint8_t* s = (const uint8_t*)it.src + 3 * p;
+      ((uint32_t*)it.stage)[v] = (uint32_t)s[0] | ((uint32_t)s[1] << 8) | ((uint32_t)s[2] << 16) |
+                                 0xFF000000u;
+      break;
+    }
+    case kRgba8: ((uint32_t*)it.stage)[v] = ((const uint32_t*)it.src)[p]; break;
+    case kRgb16: {
+      const uint16_t* s = (const uint16_t*)it.src + 3 * p;
+      ((uint2*)it.stage)[v] = make_uint2((uint32_t)s[0] | ((uint32_t)s[1] << 16),
+                                         (uint32_t)s[2] | 0xFFFF0000u);
+      break;
+    }
+    case kRgba16: ((uint2*)it.stage)[v] = ((const uint2*)it.src)[p]; break;
+    default: ((float4*)it.stage)[v] = ((const float4*)it.src)[p]; break;
+  }
 }
 
 // Pack: palette index per visit -> output pixel at the visited position.

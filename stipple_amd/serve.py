@@ -279,7 +279,9 @@ def footprint(req: Request) -> int:
     px = req.width * req.height
     if req.engine == "sequential":
         memo = (2 << 24) if req.assoc else (2 << 18)
-        return len(req.frames) * (px * (32 + 2) + memo + (132 << 10))
+        # staged inputs: 4 B per pixel for 8-bit sources, 8 for 16-bit, 16 for float
+        stage = {1: 4, 2: 4, 3: 8, 4: 8}.get(req.src_fmt, 16)
+        return len(req.frames) * (px * (stage + 2) + memo + (132 << 10))
     if req.engine == "approx":
         return len(req.frames) * px * (16 + 2) + (1 << 20)
     return len(req.frames) * px * 16 + (1 << 20)
