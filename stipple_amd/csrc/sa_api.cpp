@@ -287,6 +287,21 @@ size_t stage_bytes(int fmt, int64_t px) {
   return (((size_t)px << sa::seq_stage_shift(fmt)) + 16 + 255) & ~(size_t)255;
 }
 
+// The walk's LDS memo cache for one image: byte entries when the palette has
+// at most 255 colours (index+1 fits), and as many line slots as the LDS left
+// beside the search image and the input ring holds (a multiple of 4, at most
+// 4096); `memo_bits` > 0 (or the seq.memo_bits knob, as a cap) asks for
+// 1 << memo_bits slots instead.
+struct MemoShape {
+  int slots = 0;
+  int narrow = 0;
+  size_t bytes = 0;   // LDS (and snapshot) bytes
+  int key_bits = 0;   // the HBM table holds 1 << key_bits entries
+  size_t table = 0;   // its bytes
+};
+
+MemoShape memo_shape(const sa_seq_options& o, int assoc, int img_bytes, int count);
+
 bool valid_in_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kRgba16; }
 bool valid_out_fmt(int f) { return f >= sa::kRgbaF32 && f <= sa::kIndex16; }
 // The block engine and the palette build also read planar YUV 4:4:4 / 4:2:0;
@@ -763,17 +778,15 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       max_n = std::max(max_n, px);
       total += px;
     }
-    const int K = assoc ? 24 : 18;
-    // memo_bits: log2 of the LDS line slots (132 B each: tag + 64 entries).
-    int S = o.memo_bits;
-    if (S <= 0) {
-      S = 10;
-      while (S > 4 && (132u << S) + (size_t)max_img + sa::kSeqRingBytes > 158 * 1024) --S;
-      const int tuned = g_knobs.seq_memo_bits.load();
-      if (tuned > 0 && tuned < S) S = tuned;
+    if (o.memo_bits != 0 && (o.memo_bits < 4 || o.memo_bits > 10))
+      return fail(err, errlen, SA_EINVAL, "memo_bits %d outside 4..10", o.memo_bits);
+    std::vector<MemoShape> shape(sel.size());
+    size_t lds = 0;
+    for (size_t k = 0; k < sel.size(); ++k) {
+      const sa_image& im = images[sel[k]];
+      shape[k] = memo_shape(o, assoc, im.palette->hdr.bytes, im.palette->count);
+      lds = std::max(lds, shape[k].bytes + (size_t)im.palette->hdr.bytes + sa::kSeqRingBytes);
     }
-    if (S < 4 || S > 10) return fail(err, errlen, SA_EINVAL, "memo_bits %d outside 4..10", S);
-    const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)max_img + sa::kSeqRingBytes;
     if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "LDS %zu too large", lds);
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
     int waves = o.waves > 0 ? o.waves : g_cus * per_cu;
@@ -785,7 +798,7 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       const int64_t px = images[sel[k]].width * images[sel[k]].height;
       xs_bytes += stage_bytes(images[sel[k]].src_fmt, px);
       idx_bytes += (2 * (size_t)px + 255) & ~(size_t)255;
-      if (o.use_cache) spill_bytes += (size_t)2 << K;
+      spill_bytes += shape[k].table;
     }
     unsigned char* scratch = nullptr;
     const size_t items_bytes = (sizeof(sa::SeqItem) * sel.size() + 255) & ~(size_t)255;
@@ -811,8 +824,8 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       p += (2 * (size_t)px + 255) & ~(size_t)255;
       it.memo = nullptr;
       if (o.use_cache) {
-        it.memo = (uint16_t*)p;
-        p += (size_t)2 << K;
+        it.memo = p;
+        p += shape[k].table;
       }
       it.pal = im.palette->dev_hdr;
       it.state = nullptr;
@@ -820,8 +833,8 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       it.n = px;
       it.src_fmt = im.src_fmt;
       it.dst_fmt = im.dst_fmt;
-      it.memo_bits = S;
-      it.pad_ = 0;
+      it.memo_slots = shape[k].slots;
+      it.memo_narrow = shape[k].narrow;
     }
     sa::SeqItem* d_items = (sa::SeqItem*)scratch;
     int32_t* d_queue = (int32_t*)(scratch + items_bytes);
@@ -829,7 +842,7 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     HIP_TRY(hipMemsetAsync(d_queue, 0, sizeof(int32_t), s));
     const int ni = (int)items.size();
     HIP_TRY(timed("seq_gather", s, [&] { return sa::launch_seq_gather(d_items, ni, max_n, s); }));
-    if (o.use_cache) HIP_TRY(sa::launch_memo_clear(d_items, ni, K, s));
+    if (o.use_cache) HIP_TRY(sa::launch_memo_clear(d_items, ni, assoc ? 24 : 18, s));
     HIP_TRY(timed("seq_walk", s, [&] {
       return sa::launch_seq_walk(d_items, ni, d_queue, sw, o.use_cache, lds, waves, assoc, 0, 0,
                                  g_stats, s);
@@ -844,14 +857,28 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
 // ---- resumable walks: continuous batching -----------------------------------
 namespace {
 
-int memo_bits_for(const sa_seq_options& o, int img_bytes) {
-  if (o.memo_bits > 0) return o.memo_bits;
-  int S = 10;
-  while (S > 4 && (132u << S) + (size_t)img_bytes + sa::kSeqRingBytes > 158 * 1024) --S;
-  const int tuned = g_knobs.seq_memo_bits.load();
-  if (tuned > 0 && tuned < S) S = tuned;
-  return S;
+MemoShape memo_shape(const sa_seq_options& o, int assoc, int img_bytes, int count) {
+  MemoShape m;
+  if (!o.use_cache) return m;
+  m.narrow = count <= 255 ? 1 : 0;
+  m.key_bits = assoc ? 24 : 18;
+  m.table = (size_t)(m.narrow ? 1 : 2) << m.key_bits;
+  const size_t per = m.narrow ? 68 : 132;
+  if (o.memo_bits > 0) {
+    m.slots = 1 << o.memo_bits;
+  } else {
+    const size_t budget = std::min<size_t>(g_lds_per_cu, 160 * 1024);
+    const size_t used = (size_t)img_bytes + sa::kSeqRingBytes;
+    const size_t room = budget > used ? budget - used : 0;
+    m.slots = (int)std::min<size_t>(4096, room / per) & ~3;
+    const int tuned = g_knobs.seq_memo_bits.load();
+    if (tuned > 0) m.slots = std::min(m.slots, 1 << tuned);
+    m.slots = std::max(m.slots, 16);
+  }
+  m.bytes = sa::seq_memo_bytes(m.slots, m.narrow);
+  return m;
 }
+
 
 struct JobItem {
   int64_t id = 0;
@@ -993,9 +1020,8 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
       return fail(err, errlen, SA_EUNSUPPORTED, "image %d: palette search image above 96 KiB", i);
     if (im.palette->count > 65535)
       return fail(err, errlen, SA_EUNSUPPORTED, "image %d: the walk's memo holds indices below 65535", i);
-    const int S = memo_bits_for(o, im.palette->hdr.bytes);
-    const size_t lds = (o.use_cache ? (132u << S) : 0u) + (size_t)im.palette->hdr.bytes +
-                        sa::kSeqRingBytes;
+    const MemoShape ms = memo_shape(o, im.palette->assoc, im.palette->hdr.bytes, im.palette->count);
+    const size_t lds = ms.bytes + (size_t)im.palette->hdr.bytes + sa::kSeqRingBytes;
     if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "image %d: LDS %zu too large", i, lds);
   }
   std::vector<JobItem> fresh;
@@ -1007,13 +1033,12 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     J.id = job->next_id + i;
     J.assoc = im.palette->assoc;
     J.img_bytes = im.palette->hdr.bytes;
-    const int K = J.assoc ? 24 : 18;
-    const int S = memo_bits_for(o, J.img_bytes);
+    const MemoShape ms = memo_shape(o, J.assoc, J.img_bytes, im.palette->count);
     const int64_t px = im.width * im.height;
     const size_t xs_b = stage_bytes(im.src_fmt, px), idx_b = (2 * (size_t)px + 255) & ~(size_t)255;
-    const size_t memo_b = o.use_cache ? ((size_t)2 << K) : 0;
+    const size_t memo_b = ms.table;
     const size_t st_b = (sizeof(sa::SeqState) + 255) & ~(size_t)255;
-    const size_t snap_b = o.use_cache ? ((size_t)132 << S) : 0;
+    const size_t snap_b = ms.bytes;
     int rc = SA_OK;
     const int32_t* curve = nullptr;
     rc = get_curve(im.width, im.height, s, &curve, err, errlen);
@@ -1038,7 +1063,7 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     p += xs_b;
     it.idx = (uint16_t*)p;
     p += idx_b;
-    it.memo = o.use_cache ? (uint16_t*)p : nullptr;
+    it.memo = o.use_cache ? p : nullptr;
     p += memo_b;
     it.state = (sa::SeqState*)p;
     p += st_b;
@@ -1047,8 +1072,8 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     it.n = px;
     it.src_fmt = im.src_fmt;
     it.dst_fmt = im.dst_fmt;
-    it.memo_bits = S;
-    it.pad_ = 0;
+    it.memo_slots = ms.slots;
+    it.memo_narrow = ms.narrow;
     max_n = std::max(max_n, px);
     dev[J.assoc].push_back(it);
     fresh.push_back(J);
@@ -1107,7 +1132,8 @@ int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int
     size_t lds = 0;
     for (size_t k : sel) {
       const JobItem& J = job->items[k];
-      lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes +
+      lds = std::max(lds, (o.use_cache ? sa::seq_memo_bytes(J.it.memo_slots, J.it.memo_narrow) : 0u) +
+                                 (size_t)J.img_bytes +
                                  sa::kSeqRingBytes);
     }
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
@@ -1274,7 +1300,8 @@ extern "C" int sa_walk_job_step_overlapped(sa_walk_job job, double budget_us, vo
       for (const JobItem& J : job->items) {
         if (J.assoc != assoc) continue;
         cand.push_back(&J);
-        lds = std::max(lds, (o.use_cache ? (132u << J.it.memo_bits) : 0u) + (size_t)J.img_bytes +
+        lds = std::max(lds, (o.use_cache ? sa::seq_memo_bytes(J.it.memo_slots, J.it.memo_narrow) : 0u) +
+                                 (size_t)J.img_bytes +
                                  sa::kSeqRingBytes);
       }
       if (cand.empty()) continue;

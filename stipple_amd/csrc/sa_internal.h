@@ -178,16 +178,23 @@ struct SeqItem {
   void* stage;            // scratch: the source pixel of each visit, in visit order
                           // (1 << seq_stage_shift(src_fmt) bytes each, +16 B of padding)
   uint16_t* idx;          // scratch: palette index per visit
-  uint16_t* memo;         // scratch: 1<<K memo entries (K = 18, or 24 with alpha)
+  void* memo;             // scratch: 1<<K memo entries of 1 or 2 bytes (K = 18, or 24
+                          // with alpha)
   const PaletteDev* pal;  // device palette header
   SeqState* state;        // null: walk to the end in one launch
-  unsigned char* snap;    // with state: the LDS memo cache between launches (132 << memo_bits B)
+  unsigned char* snap;    // with state: the LDS memo cache between launches
+                          // (seq_memo_bytes(memo_slots, memo_narrow) bytes)
   int64_t n;              // visits == w*h
   int32_t src_fmt;
   int32_t dst_fmt;
-  int32_t memo_bits;      // log2 of the LDS memo line slots
-  int32_t pad_;
+  int32_t memo_slots;     // LDS memo line slots (a multiple of 4)
+  int32_t memo_narrow;    // 1: memo entries are bytes (palettes of at most 255 colours)
 };
+
+// LDS bytes of the walk's memo cache: per slot a 4-byte tag and 64 entries.
+__host__ __device__ inline uint32_t seq_memo_bytes(int slots, int narrow) {
+  return (uint32_t)slots * (4u + (narrow ? 64u : 128u));
+}
 
 // One frame of the block engine.
 struct BlockFrame {
@@ -276,7 +283,7 @@ hipError_t launch_seq_state_init(const SeqItem* items, int nitems, hipStream_t s
 hipError_t launch_memo_clear(const SeqItem* items, int nitems, int key_bits, hipStream_t s);
 // state->t of every item (n for items without a state).
 hipError_t launch_seq_progress(const SeqItem* items, int nitems, int64_t* out, hipStream_t s);
-// `lds_bytes`: the largest (132 << memo_bits) + search image of the items.
+// `lds_bytes`: the largest memo cache + search image + input ring of the items.
 // Items with a state advance by at most `slice` visits when slice > 0 (a
 // multiple of 48), and stop at the first 48-visit boundary `budget`
 // s_memrealtime ticks (100 MHz) after they start when budget > 0; otherwise,

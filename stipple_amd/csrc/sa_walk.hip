@@ -48,6 +48,8 @@
 #include "sa_internal.h"
 #include "sa_math.h"
 
+#include <type_traits>
+
 // v_writelane_b32: `v` with lane `lane` (uniform) replaced by the uniform `s`.
 __device__ int llvm_writelane(int s, int lane, int v) __asm("llvm.amdgcn.writelane.i32");
 
@@ -165,12 +167,14 @@ __device__ __forceinline__ double dpp_all(double v) {
 
 constexpr int kRowShl1 = 0x101;  // lane j <- lane j+1; lane 15 has no source
 
-// LDS cache of the HBM memo table's 64-entry lines.
+// LDS cache of the HBM memo table's 64-entry lines.  Entries are index+1
+// (0 = untouched), bytes when the palette has at most 255 colours (twice the
+// lines in the same LDS), else 16-bit.
 struct Memo {
   uint32_t* tag;    // per slot: bit 31 empty, bit 30 dirty, bits 0..29 line id
-  uint16_t* line;   // 64 entries per slot: index+1, 0 = untouched
-  uint16_t* table;  // HBM, 1 << K entries, zeroed before the walk
-  int bits;         // log2(slots)
+  void* line;       // 64 entries per slot
+  void* table;      // HBM, 1 << K entries, zeroed before the walk
+  uint32_t slots;   // any count: the hash scales into it
   // This lane's channel part of the key (memo_key_part for its row):
   // bfe(c8, k_hi_off, k_hi_w) << k_hi_shl | bfe(c8, 2, k_lo_w) << k_lo_shl.
   uint32_t k_hi_off, k_hi_w, k_hi_shl, k_lo_w, k_lo_shl;
@@ -190,22 +194,29 @@ __device__ __forceinline__ void memo_key_consts(Memo& m, int ch) {
 constexpr uint32_t kEmptyTag = 0x80000000u;
 constexpr uint32_t kDirty = 0x40000000u;
 
+// Multiplicative hash of the line id, scaled into [0, slots).
 __device__ __forceinline__ uint32_t memo_slot(const Memo& m, uint32_t line) {
-  return (line * 0x9E3779B1u) >> (32 - m.bits);
+  return __umulhi(line * 0x9E3779B1u, m.slots);
 }
 
+template <bool NARROW>
+using MemoEntry = typename std::conditional<NARROW, uint8_t, uint16_t>::type;
+
 // Makes `line` resident in `slot` (uniform).  Returns 1 when it read HBM.
+template <bool NARROW>
 __device__ __forceinline__ uint32_t memo_fill(const Memo& m, uint32_t slot, uint32_t line,
                                               uint32_t old) {
+  typedef MemoEntry<NARROW> E;
+  typedef E __attribute__((address_space(1))) gE;
   const int lane = threadIdx.x;
-  uint16_t* dst = m.line + 64 * slot;
+  E* dst = (E*)m.line + 64 * slot;
   uint32_t loaded = 0;
   if (old & kEmptyTag) {
     // Never held a line: this line was never written, so it is all zero.
     dst[lane] = 0;
   } else {
-    if (old & kDirty) ((gu16*)m.table)[64 * (old & 0x3FFFFFFFu) + lane] = dst[lane];
-    dst[lane] = ((const gu16*)m.table)[64 * line + lane];
+    if (old & kDirty) ((gE*)m.table)[64 * (old & 0x3FFFFFFFu) + lane] = dst[lane];
+    dst[lane] = ((const gE*)m.table)[64 * line + lane];
     loaded = 1;
   }
   if (lane == 0) m.tag[slot] = line;
@@ -271,7 +282,7 @@ struct Visit {
 };
 
 // One visit (visit t); on return lane j holds visit t+1+j.
-template <bool ASSOC, bool CACHE>
+template <bool ASSOC, bool CACHE, bool NARROW>
 __device__ __forceinline__ void visit(Visit& w, double W, int64_t t, double* ring, const Memo& m,
                                       const Search& s, Counters& cnt) {
   const int lane = threadIdx.x;
@@ -310,8 +321,10 @@ __device__ __forceinline__ void visit(Visit& w, double W, int64_t t, double* rin
     if (ASSOC) key |= (uint32_t)rl((int)kc, 48);
     const uint32_t line = key >> 6;
     const uint32_t slot = memo_slot(m, line);
+    typedef MemoEntry<NARROW> E;
+    E* const entp = (E*)m.line + 64 * slot + (key & 63);
     const uint32_t tgv = m.tag[slot];
-    const uint32_t entv = m.line[64 * slot + (key & 63)];
+    const uint32_t entv = *entp;
     // Nothing else is scheduled ahead of these reads (in-order issue: work
     // placed before them would delay the chain), and the target is formed
     // behind them.
@@ -326,14 +339,14 @@ __device__ __forceinline__ void visit(Visit& w, double W, int64_t t, double* rin
     if (__builtin_expect((((tg & ~kDirty) ^ line) | ((ent - 1u) >> 16)) != 0, 0)) {
       if ((tg & ~kDirty) != line) {
         ++cnt.fills;
-        cnt.loads += memo_fill(m, slot, line, tg);
-        ent = (uint32_t)rfl((int)m.line[64 * slot + (key & 63)]);
+        cnt.loads += memo_fill<NARROW>(m, slot, line, tg);
+        ent = (uint32_t)rfl((int)*entp);
         idx = (int)ent - 1;
       }
       if (ent == 0) {
         idx = search();
         if (lane == 0) {
-          m.line[64 * slot + (key & 63)] = (uint16_t)(idx + 1);
+          *entp = (E)(idx + 1);
           m.tag[slot] = line | kDirty;
         }
       }
@@ -371,7 +384,7 @@ __device__ __forceinline__ void flush_window(uint16_t* idx_out, uint32_t obuf, i
 // Inputs: at the start of each 16-visit chunk the next chunk's staged pixels
 // (loaded one chunk earlier) are associated and written to the ring, and the
 // chunk after that is loaded; each visit reads its x_{t+16} from the ring.
-template <bool ASSOC, bool CACHE>
+template <bool ASSOC, bool CACHE, bool NARROW>
 __device__ int64_t walk_image(const SeqItem& it, double W, const Memo& m, const Search& s,
                               double* ring_base, Counters& cnt, int64_t t0, int64_t stop,
                               uint64_t deadline) {
@@ -419,7 +432,7 @@ __device__ int64_t walk_image(const SeqItem& it, double W, const Memo& m, const 
     // A 32-bit count keeps the loop test on the scalar unit (it has no
     // 64-bit ordered compare).
     const int len = rfl((int)(stop - base < 16 ? stop - base : 16));
-    for (int k = 0; k < len; ++k) visit<ASSOC, CACHE>(w, W, base + k, ring, m, s, cnt);
+    for (int k = 0; k < len; ++k) visit<ASSOC, CACHE, NARROW>(w, W, base + k, ring, m, s, cnt);
     const int64_t last = base + len - 1;
     if ((last & 63) == 63 || last + 1 == n) flush_window(it.idx, w.obuf, last);
     if (len < 16 || base + 16 == stop) break;
@@ -475,12 +488,13 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     if (t0 >= it.n) continue;
     int64_t stop = (slice > 0 && it.state && t0 + slice < it.n) ? t0 + slice : it.n;
     Memo m;
-    m.bits = it.memo_bits;
+    const uint32_t slots = (uint32_t)rfl(it.memo_slots);
+    const bool narrow = rfl(it.memo_narrow) != 0;
+    m.slots = slots;
     memo_key_consts<ASSOC>(m, lane >> 4);
-    const uint32_t slots = 1u << m.bits;
-    const uint32_t memo_bytes = CACHE ? 132u * slots : 0u;
+    const uint32_t memo_bytes = CACHE ? seq_memo_bytes((int)slots, narrow) : 0u;
     m.tag = (uint32_t*)lds;
-    m.line = (uint16_t*)(lds + 4 * slots);
+    m.line = lds + 4 * slots;
     unsigned char* img = lds + memo_bytes;
     const PaletteDev pd = *it.pal;
     {
@@ -506,7 +520,10 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     const uint64_t deadline =
         (budget && it.state) ? __builtin_amdgcn_s_memrealtime() + budget : 0;
     double* ring = (double*)(img + pd.bytes);
-    stop = walk_image<ASSOC, CACHE>(it, W, m, s, ring, cnt, t0, stop, deadline);
+    if (CACHE && narrow)
+      stop = walk_image<ASSOC, CACHE, true>(it, W, m, s, ring, cnt, t0, stop, deadline);
+    else
+      stop = walk_image<ASSOC, CACHE, false>(it, W, m, s, ring, cnt, t0, stop, deadline);
     __syncthreads();
     if (it.state) {
       if (CACHE && stop < it.n) copy16((uint4*)it.snap, (const uint4*)lds, memo_bytes / 16);
@@ -539,7 +556,8 @@ __global__ __launch_bounds__(256) void memo_clear_kernel(const SeqItem* __restri
                                                          int64_t words) {
   const SeqItem it = items[blockIdx.y];
   uint4* t = (uint4*)it.memo;
-  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < words; k += (int64_t)gridDim.x * 256)
+  const int64_t w = it.memo_narrow ? words / 2 : words;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < w; k += (int64_t)gridDim.x * 256)
     t[k] = make_uint4(0, 0, 0, 0);
 }
 
