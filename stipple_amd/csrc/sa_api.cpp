@@ -183,11 +183,11 @@ int build_curve(int64_t w, int64_t h, int32_t* order, hipStream_t s, char* err, 
   if (level > 15) return fail(err, errlen, SA_EUNSUPPORTED, "image side above 32768");
   const int64_t chunks = sa::curve_chunks(level);
   int32_t* counts = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&counts, sizeof(int32_t) * chunks, s));
+  HIP_TRY(sa::scratch_alloc((void**)&counts, sizeof(int32_t) * chunks, s));
   HIP_TRY(sa::launch_curve_count(level, w, h, counts, chunks, s));
   HIP_TRY(sa::launch_curve_scan(counts, chunks, s));
   HIP_TRY(sa::launch_curve_emit(level, w, h, counts, chunks, order, s));
-  HIP_TRY(hipFreeAsync(counts, s));
+  HIP_TRY(sa::scratch_free(counts, s));
   return SA_OK;
 }
 
@@ -592,7 +592,7 @@ extern "C" int sa_video_montage(const void* const* frames, int n, int src_fmt, i
   hipStream_t s = (hipStream_t)stream;
   unsigned char* d = nullptr;
   const size_t ptr_bytes = (sizeof(void*) * (size_t)n + 255) & ~(size_t)255;
-  HIP_TRY(hipMallocAsync((void**)&d, ptr_bytes + 256, s));
+  HIP_TRY(sa::scratch_alloc((void**)&d, ptr_bytes + 256, s));
   HIP_TRY(staged_copy(d, frames, sizeof(void*) * (size_t)n, s));
   int* d_gray = (int*)(d + ptr_bytes);
   HIP_TRY(sa::launch_video_montage((const void* const*)d, n, src_fmt, width, height, cw, ch, c, W,
@@ -605,11 +605,11 @@ extern "C" int sa_video_montage(const void* const* frames, int n, int src_fmt, i
     *gray = *h_gray;
     (void)hipHostFree(h_gray);
     if (e != hipSuccess) {
-      (void)hipFreeAsync(d, s);
+      (void)sa::scratch_free(d, s);
       return fail(err, errlen, SA_EDEVICE, "montage: %s", hipGetErrorString(e));
     }
   }
-  HIP_TRY(hipFreeAsync(d, s));
+  HIP_TRY(sa::scratch_free(d, s));
   return SA_OK;
 }
 
@@ -627,14 +627,14 @@ extern "C" int sa_video_palette(const void* const* frames, int n, int src_fmt, i
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   void* m = nullptr;
-  HIP_TRY(hipMallocAsync(&m, (size_t)(8 * W * H), s));
+  HIP_TRY(sa::scratch_alloc(&m, (size_t)(8 * W * H), s));
   int gray = 0;
   rc = sa_video_montage(frames, n, src_fmt, width, height, want, tile, cols, m, &W, &H, &gray,
                         stream, err, errlen);
   if (rc == SA_OK)
     rc = sa_palette_build(m, sa::kRgba16, W, H, colors, 0, gray, keep_tree, stream, out, err,
                           errlen);
-  HIP_TRY(hipFreeAsync(m, s));
+  HIP_TRY(sa::scratch_free(m, s));
   return rc;
 }
 
@@ -803,7 +803,7 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     unsigned char* scratch = nullptr;
     const size_t items_bytes = (sizeof(sa::SeqItem) * sel.size() + 255) & ~(size_t)255;
     const size_t total_bytes = items_bytes + 256 + xs_bytes + idx_bytes + spill_bytes;
-    HIP_TRY(hipMallocAsync((void**)&scratch, total_bytes, s));
+    HIP_TRY(sa::scratch_alloc((void**)&scratch, total_bytes, s));
     unsigned char* p = scratch + items_bytes + 256;
     for (size_t k = 0; k < sel.size(); ++k) {
       const sa_image& im = images[sel[k]];
@@ -812,7 +812,7 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       const int32_t* curve = nullptr;
       rc = get_curve(im.width, im.height, s, &curve, err, errlen);
       if (rc) {
-        (void)hipFreeAsync(scratch, s);
+        (void)sa::scratch_free(scratch, s);
         return rc;
       }
       it.src = im.src;
@@ -848,7 +848,7 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
                                  g_stats, s);
     }));
     HIP_TRY(timed("seq_pack", s, [&] { return sa::launch_seq_pack(d_items, ni, max_n, s); }));
-    HIP_TRY(hipFreeAsync(scratch, s));
+    HIP_TRY(sa::scratch_free(scratch, s));
     (void)total;
   }
   return SA_OK;
@@ -895,7 +895,7 @@ struct JobItem {
 int upload_items(const std::vector<sa::SeqItem>& items, hipStream_t s, unsigned char** out,
                  char* err, size_t errlen) {
   const size_t bytes = (sizeof(sa::SeqItem) * items.size() + 255) & ~(size_t)255;
-  HIP_TRY(hipMallocAsync((void**)out, bytes + 256, s));
+  HIP_TRY(sa::scratch_alloc((void**)out, bytes + 256, s));
   HIP_TRY(staged_copy(*out, items.data(), sizeof(sa::SeqItem) * items.size(), s));
   return SA_OK;
 }
@@ -951,7 +951,7 @@ int park_block(sa_walk_job job, JobItem& J, hipStream_t s, char* err, size_t err
     job->spare.erase(job->spare.begin());
     job->spare_bytes -= o.bytes;
     HIP_TRY(hipStreamWaitEvent(s, o.ev, 0));
-    HIP_TRY(hipFreeAsync(o.mem, s));
+    HIP_TRY(sa::scratch_free(o.mem, s));
     HIP_TRY(hipEventDestroy(o.ev));
   }
   return SA_OK;
@@ -1045,7 +1045,7 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     const size_t need = xs_b + idx_b + memo_b + st_b + snap_b;
     if (rc == SA_OK) rc = reuse_block(job, J, need, s, err, errlen);
     if (rc == SA_OK && !J.mem) {
-      hipError_t e = hipMallocAsync((void**)&J.mem, need, s);
+      hipError_t e = sa::scratch_alloc((void**)&J.mem, need, s);
       if (e != hipSuccess) rc = fail(err, errlen, SA_ENOMEM, "image %d: %s", i, hipGetErrorString(e));
       J.mem_bytes = need;
     }
@@ -1092,7 +1092,7 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     HIP_TRY(timed("seq_gather", s, [&] { return sa::launch_seq_gather(di, ni, max_n, s); }));
     if (o.use_cache) HIP_TRY(sa::launch_memo_clear(di, ni, assoc ? 24 : 18, s));
     HIP_TRY(sa::launch_seq_state_init(di, ni, s));
-    HIP_TRY(hipFreeAsync(d, s));
+    HIP_TRY(sa::scratch_free(d, s));
   }
   for (int i = 0; i < n; ++i) {
     if (ids) ids[i] = fresh[i].id;
@@ -1147,7 +1147,7 @@ int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int
     waves = std::min<int>(waves, (int)its.size());
     const size_t items_bytes = (sizeof(sa::SeqItem) * its.size() + 255) & ~(size_t)255;
     unsigned char* d = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&d, items_bytes + 256 + 8 * its.size(), s));
+    HIP_TRY(sa::scratch_alloc((void**)&d, items_bytes + 256 + 8 * its.size(), s));
     HIP_TRY(staged_copy(d, its.data(), sizeof(sa::SeqItem) * its.size(), s));
     int32_t* queue = (int32_t*)(d + items_bytes);
     int64_t* prog = (int64_t*)(d + items_bytes + 256);
@@ -1164,7 +1164,7 @@ int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int
       HIP_TRY(hipMemcpyAsync(t.data(), prog, 8 * its.size(), hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
     }
-    HIP_TRY(hipFreeAsync(d, s));
+    HIP_TRY(sa::scratch_free(d, s));
     std::vector<sa::SeqItem> fin;
     int64_t max_n = 0;
     for (size_t q = 0; q < sel.size(); ++q) {
@@ -1186,7 +1186,7 @@ int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int
       HIP_TRY(timed("seq_pack", s, [&] {
         return sa::launch_seq_pack((const sa::SeqItem*)f, nf, max_n, s);
       }));
-      HIP_TRY(hipFreeAsync(f, s));
+      HIP_TRY(sa::scratch_free(f, s));
     }
   }
   // Retire the finished items (their scratch is freed after this step's work).
@@ -1250,7 +1250,7 @@ int retire_finished(sa_walk_job job, hipStream_t s, int64_t* done, int cap, int*
     HIP_TRY(timed("seq_pack", s, [&] {
       return sa::launch_seq_pack((const sa::SeqItem*)f, nf, max_n, s);
     }));
-    HIP_TRY(hipFreeAsync(f, s));
+    HIP_TRY(sa::scratch_free(f, s));
   }
   std::vector<JobItem> keep;
   for (JobItem& J : job->items) {
@@ -1324,7 +1324,7 @@ extern "C" int sa_walk_job_step_overlapped(sa_walk_job job, double budget_us, vo
       waves = std::min<int>(waves, (int)its.size());
       const size_t items_bytes = (sizeof(sa::SeqItem) * its.size() + 255) & ~(size_t)255;
       unsigned char* d = nullptr;
-      HIP_TRY(hipMallocAsync((void**)&d, items_bytes + 256 + 8 * its.size(), s));
+      HIP_TRY(sa::scratch_alloc((void**)&d, items_bytes + 256 + 8 * its.size(), s));
       HIP_TRY(staged_copy(d, its.data(), sizeof(sa::SeqItem) * its.size(), s));
       int32_t* queue = (int32_t*)(d + items_bytes);
       int64_t* prog = (int64_t*)(d + items_bytes + 256);
@@ -1337,7 +1337,7 @@ extern "C" int sa_walk_job_step_overlapped(sa_walk_job job, double budget_us, vo
       }));
       HIP_TRY(sa::launch_seq_progress(di, ni, prog, s));
       HIP_TRY(hipMemcpyAsync(F.host + off, prog, 8 * its.size(), hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipFreeAsync(d, s));
+      HIP_TRY(sa::scratch_free(d, s));
       off += its.size();
     }
     HIP_TRY(hipEventRecord(F.ev, s));
@@ -1368,12 +1368,13 @@ extern "C" int sa_walk_job_destroy(sa_walk_job job) {
   {
     std::lock_guard<std::mutex> g(job->mu);
     (void)hipDeviceSynchronize();
+    // The blocks are the scratch cache's: they go back to it.
     for (JobItem& J : job->items) {
-      (void)hipFree(J.mem);
+      (void)sa::scratch_free(J.mem, nullptr);
       if (J.mem_ev) (void)hipEventDestroy(J.mem_ev);
     }
     for (auto& o : job->spare) {
-      (void)hipFree(o.mem);
+      (void)sa::scratch_free(o.mem, nullptr);
       (void)hipEventDestroy(o.ev);
     }
     for (auto& F : job->flight) {
@@ -1488,14 +1489,14 @@ extern "C" int sa_walk_blocks(const sa_image* frames, int n, int block, double d
   rc = get_curve(p.w, p.h, s, &curve, err, errlen);
   if (rc) return rc;
   sa::BlockFrame* d_fr = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::BlockFrame) * p.fr.size(), s));
+  HIP_TRY(sa::scratch_alloc((void**)&d_fr, sizeof(sa::BlockFrame) * p.fr.size(), s));
   HIP_TRY(staged_copy(d_fr, p.fr.data(), sizeof(sa::BlockFrame) * p.fr.size(), s));
   HIP_TRY(timed("blocks_walk", s, [&] {
     return sa::launch_blocks(d_fr, p.n, curve, p.w, p.h, p.nb, p.block, p.sw, p.assoc, p.max_img,
                              p.max_count, p.src_class, p.threads, s);
   }));
   if (p.gap >= 0) HIP_TRY(sa::launch_blocks_fill(d_fr, p.n, p.gap, p.w, p.h, s));
-  HIP_TRY(hipFreeAsync(d_fr, s));
+  HIP_TRY(sa::scratch_free(d_fr, s));
   return SA_OK;
 }
 
@@ -1674,14 +1675,14 @@ extern "C" int sa_dither_screen(const sa_image* frames, int n, int screen, doubl
                                     frames[i].dst_fmt};
   sa::ScreenFrame* d_fr = nullptr;
   const size_t fr_bytes = (sizeof(sa::ScreenFrame) * fr.size() + 255) & ~(size_t)255;
-  HIP_TRY(hipMallocAsync((void**)&d_fr, fr_bytes + sizeof(sa::ScreenRange) * fr.size(), s));
+  HIP_TRY(sa::scratch_alloc((void**)&d_fr, fr_bytes + sizeof(sa::ScreenRange) * fr.size(), s));
   sa::ScreenRange* d_rng = (sa::ScreenRange*)((unsigned char*)d_fr + fr_bytes);
   HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::ScreenFrame) * fr.size(), s));
   HIP_TRY(timed("screen_dither", s, [&] {
     return sa::launch_screen(d_fr, n, w, h, g_screens.dev + kScreenOff[screen], kScreenSide[screen],
                              screen, diffusion, d_rng, s);
   }));
-  HIP_TRY(hipFreeAsync(d_fr, s));
+  HIP_TRY(sa::scratch_free(d_fr, s));
   return SA_OK;
 }
 
@@ -1725,12 +1726,12 @@ extern "C" int sa_dither_diffuse(const sa_image* frames, int n, int kind, double
                                     frames[i].palette->dev_entries, frames[i].src_fmt,
                                     frames[i].dst_fmt};
   sa::ScreenFrame* d_fr = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&d_fr, sizeof(sa::ScreenFrame) * fr.size(), s));
+  HIP_TRY(sa::scratch_alloc((void**)&d_fr, sizeof(sa::ScreenFrame) * fr.size(), s));
   HIP_TRY(staged_copy(d_fr, fr.data(), sizeof(sa::ScreenFrame) * fr.size(), s));
   HIP_TRY(timed("diffuse_dither", s, [&] {
     return sa::launch_diffuse(d_fr, n, w, h, kind, max_count, (float)diffusion, s);
   }));
-  HIP_TRY(hipFreeAsync(d_fr, s));
+  HIP_TRY(sa::scratch_free(d_fr, s));
   return SA_OK;
 }
 
@@ -1871,11 +1872,11 @@ extern "C" int sa_walk_approx(const sa_image* frames, int n, int iterations, int
   rc = get_curve(p.w, p.h, s, &curve, err, errlen);
   if (rc) return rc;
   unsigned char* scratch = nullptr;
-  HIP_TRY(hipMallocAsync((void**)&scratch, p.scratch(), s));
+  HIP_TRY(sa::scratch_alloc((void**)&scratch, p.scratch(), s));
   HIP_TRY(staged_copy(scratch, p.fr.data(), sizeof(sa::BlockFrame) * p.fr.size(), s));
   HIP_TRY(staged_copy(scratch + p.b_fr, p.g_host(), p.g_bytes(), s));
   HIP_TRY(timed("approx", s, [&] { return launch_approx_plan(p, scratch, curve, s); }));
-  HIP_TRY(hipFreeAsync(scratch, s));
+  HIP_TRY(sa::scratch_free(scratch, s));
   return SA_OK;
 }
 

@@ -160,6 +160,13 @@ struct SeqState {
   double e[64];
 };
 
+// Device scratch for the duration of a call's work on stream `s`, reused
+// across calls (sa_scratch.cpp); scratch_free hands it back once the work
+// queued on `s` so far is done with it.  Same contract as hipMallocAsync /
+// hipFreeAsync, which a stream under graph capture gets.
+hipError_t scratch_alloc(void** out, size_t bytes, hipStream_t s);
+hipError_t scratch_free(void* p, hipStream_t s);
+
 // The sequential walk stages each visit's source pixel in visit order, in the
 // source's own width: 8-bit RGB(A) as 4 bytes (alpha 255 for RGB), 16-bit as
 // 8 bytes (alpha 65535 for RGB), float RGBA as 16.  log2 of that width:

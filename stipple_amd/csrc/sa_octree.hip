@@ -2214,7 +2214,7 @@ struct Carve {
     size_t total = 0;
     for (const Req& r : reqs) total += r.bytes;
     void* base = nullptr;
-    const hipError_t e = hipMallocAsync(&base, total, s);
+    const hipError_t e = scratch_alloc(&base, total, s);
     if (e != hipSuccess) return e;
     owned.push_back(base);
     size_t off = 0;
@@ -2268,7 +2268,7 @@ class SubBuild {
     }
     for (int f = 0; f < F_; ++f)
       if (res_[f].status != 0) drop_outputs(f);
-    for (void* q : owned_) (void)hipFreeAsync(q, s_);
+    for (void* q : owned_) (void)scratch_free(q, s_);
     owned_.clear();
   }
 
