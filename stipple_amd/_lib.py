@@ -374,8 +374,12 @@ def _dev_ptr(t):
 
 
 def image_spec(src, dst, width, height, src_fmt, dst_fmt, palette):
-    """One image for the engines; src/dst are contiguous torch device tensors."""
-    return _Image(_dev_ptr(src), _dev_ptr(dst), width, height, src_fmt, dst_fmt, palette.handle)
+    """One image for the engines; src/dst are contiguous torch device tensors.
+    The spec keeps them and the palette referenced (`_keep`), so a job or a
+    chain that holds the spec keeps what its device work reads and writes."""
+    spec = _Image(_dev_ptr(src), _dev_ptr(dst), width, height, src_fmt, dst_fmt, palette.handle)
+    spec._keep = (src, dst, palette)
+    return spec
 
 
 def _stream_ptr(stream):
@@ -407,6 +411,18 @@ class WalkJob:
         _check(lib().sa_walk_job_create(C.byref(opt), C.byref(h), e, len(e)), e)
         self.handle = h.value
         self.active = 0
+        # What each unfinished image's device work uses (its spec's tensors and
+        # palette), and what finished in the last step: released one call
+        # later, when the stream has passed that step.
+        self._refs = {}
+        self._done_refs = []
+
+    def _finished(self, ids, synced):
+        released = [self._refs.pop(i, None) for i in ids]
+        if synced:
+            self._done_refs = []
+        else:
+            self._done_refs = released
 
     def add(self, images, stream=None):
         """Adds image_spec()s; returns their ids."""
@@ -418,6 +434,8 @@ class WalkJob:
         _check(lib().sa_walk_job_add(self.handle, arr, len(images), _np_ptr(ids),
                                      _stream_ptr(stream), e, len(e)), e)
         self.active += len(images)
+        for i, img in zip(ids, images):
+            self._refs[int(i)] = getattr(img, "_keep", None)
         return [int(i) for i in ids]
 
     def step(self, visits, stream=None):
@@ -431,7 +449,9 @@ class WalkJob:
         _check(lib().sa_walk_job_step(self.handle, int(visits), _stream_ptr(stream), _np_ptr(done),
                                       cap, C.byref(nd), C.byref(act), e, len(e)), e)
         self.active = act.value
-        return [int(i) for i in done[:nd.value]]
+        ids = [int(i) for i in done[:nd.value]]
+        self._finished(ids, synced=False)
+        return ids
 
     def step_timed(self, budget_us, stream=None):
         """One launch in which every image walks for about `budget_us`
@@ -445,7 +465,9 @@ class WalkJob:
                                             _np_ptr(done), cap, C.byref(nd), C.byref(act), e,
                                             len(e)), e)
         self.active = act.value
-        return [int(i) for i in done[:nd.value]]
+        ids = [int(i) for i in done[:nd.value]]
+        self._finished(ids, synced=True)
+        return ids
 
     def step_overlapped(self, budget_us, stream=None):
         """Launches a timed step, then returns the ids that finished in the
@@ -460,12 +482,16 @@ class WalkJob:
                                                  _stream_ptr(stream), _np_ptr(done), cap,
                                                  C.byref(nd), C.byref(act), e, len(e)), e)
         self.active = act.value
-        return [int(i) for i in done[:nd.value]]
+        ids = [int(i) for i in done[:nd.value]]
+        self._finished(ids, synced=True)  # those finished in the step already waited for
+        return ids
 
     def close(self):
         if getattr(self, "handle", None):
-            lib().sa_walk_job_destroy(self.handle)
+            lib().sa_walk_job_destroy(self.handle)  # synchronises the device first
             self.handle = None
+        self._refs = {}
+        self._done_refs = []
 
     def __del__(self):
         try:
@@ -547,8 +573,10 @@ class FrameChain:
     into a hipGraph (sa_chain_*): refill the src buffers, then run() replays
     the batch with one launch."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, frames=()):
         self.handle = handle
+        # the frames' tensors and palettes, which every replay reads and writes
+        self._keep = [getattr(f, "_keep", None) for f in frames]
 
     @classmethod
     def blocks(cls, frames, block=64, diffusion=1.0, stream=None):
@@ -558,7 +586,7 @@ class FrameChain:
         rc = lib().sa_chain_create_blocks(arr, len(frames), block, diffusion, _stream_ptr(stream),
                                           C.byref(h), e, len(e))
         _check(rc, e)
-        return cls(h.value)
+        return cls(h.value, frames)
 
     @classmethod
     def approx(cls, frames, iterations=20, taps=96, fp64=False, use_tree=True, diffusion=1.0,
@@ -570,7 +598,7 @@ class FrameChain:
                                           int(bool(use_tree)), diffusion, _stream_ptr(stream),
                                           C.byref(h), e, len(e))
         _check(rc, e)
-        return cls(h.value)
+        return cls(h.value, frames)
 
     def run(self, stream=None):
         e = _err()
@@ -580,6 +608,7 @@ class FrameChain:
         if getattr(self, "handle", None):
             lib().sa_chain_destroy(self.handle)
             self.handle = None
+        self._keep = []
 
     def __del__(self):
         try:

@@ -229,3 +229,36 @@ def test_job_more_images_than_wavefronts(gpu, oracle, mode):
     for c in cases:
         _check(oracle, c)
     job.close()
+
+
+def test_job_keeps_what_its_images_use(gpu, oracle):
+    """A job holds its images' tensors and palettes (through their specs)
+    until they finish: with every other reference dropped and the freed sizes
+    allocated again and scribbled over, the outputs still equal the oracle's."""
+    import gc
+    import torch
+    say("job keeps its images' buffers")
+    rng = np.random.default_rng(21)
+    job = gpu.WalkJob()
+    expect = []
+    for w, h, col in [(96, 64, 16), (40, 31, 5)]:
+        img = synth.sinusoid_image(w, h, rng, channels=4)
+        tree, f = H.oracle_tree(oracle, img, col)
+        exp = oracle.walk_seq(tree, f)
+        ids = job.add([gpu.image_spec(H.to_device(img), H.out_alloc((h, w), gpu.PIX_RGBAF32),
+                                      w, h, H.fmt_of(img), gpu.PIX_RGBAF32,
+                                      gpu.Palette.from_tree(tree.data()))])
+        expect.append((ids[0], job._refs[ids[0]][1], exp, (h, w)))
+    outs = {i: d for i, d, _, _ in expect}
+    expect = [(i, e, hw) for i, _, e, hw in expect]
+    gc.collect()
+    junk = [torch.full((n,), 255, dtype=torch.uint8, device="cuda") for n in (96 * 64 * 4, 40 * 31 * 4)]
+    while job.active:
+        job.step(480)
+    torch.cuda.synchronize()
+    for i, exp, (h, w) in expect:
+        got = H.from_device(outs[i], (h, w, 4), np.float32)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    del junk
+    job.close()
+    assert not job._refs
