@@ -208,7 +208,9 @@ typedef struct {
 typedef struct {
   double diffusion; /* dither:diffusion-amount (rd::DitherParams::diffusion), 1.0 */
   int32_t use_cache;/* quantize.c's memo; 1 reproduces ImageMagick */
-  int32_t memo_bits;/* LDS memo line slots per wavefront = 2^memo_bits (4..10), 0 = auto */
+  int32_t memo_bits;/* LDS memo line slots per wavefront = 2^memo_bits (4..10); 0 = as many
+                       as the LDS left beside the palette holds (byte entries for palettes
+                       of at most 255 colours, so twice the lines) */
   int32_t waves;    /* persistent wavefronts, 0 = auto */
   int32_t reserved;
 } sa_seq_options;
@@ -376,8 +378,8 @@ int sa_chain_destroy(sa_chain chain);
  * no knob changes a result.  New, with no counterpart in the reference.
  *   "blocks.threads"  block walk: Hilbert blocks per workgroup (64..512; 256)
  *   "approx.per"      Jacobi sweep: outputs per thread (2, 4, 8; 4)
- *   "seq.memo_bits"   sequential walk: LDS memo slots, log2 (4..10; 0 = the
- *                     largest that fits, which leaves one image per CU)
+ *   "seq.memo_bits"   sequential walk: a cap of 2^value LDS memo slots (4..10;
+ *                     0 = no cap: the most that fit, which leaves one image per CU)
  * Calls made after a set use the new value; chains keep the value they were
  * created with. */
 int sa_tune_set(const char* knob, int value, char* err, size_t errlen);
