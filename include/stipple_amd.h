@@ -92,12 +92,22 @@ int sa_palette_count(sa_palette p);
 /* ColorTree::Build on the device (classify, reduce, colormap).  Replaces
  * rd::ColorTree::Build (include/rd_octree.h:62-63, src/rd_octree.cpp:219-409);
  * bit-exact with it: same colormap, same node numbering, the same doubles in
- * every node.  `src` is a device image in src_fmt (planar YUV included: the
+ * every node but one.  The root's quantize_error (a sum over every
+ * classification step) is summed in parallel within a rounding bound; its
+ * value can affect the reduction only if it comes within 1 of another node's
+ * error, and the serial chain runs exactly when the bound cannot rule that
+ * out.  So every decision and the colormap are exact, and the root's stored
+ * double may differ in its last bits (sa_palette_info's exact_root says
+ * whether it was summed serially).  `src` is a device image in src_fmt
+ * (planar YUV included: the
  * tree of the frame as the reader widens it).  Synchronises `stream`
  * (the build reads back a few scalars).  With keep_tree the node arrays stay
  * on the device for sa_palette_tree.  Trees that would exceed MaxQNodes
- * during classification (quantize.c then prunes whole levels mid-image)
- * return SA_EUNSUPPORTED. */
+ * (266817) during classification (quantize.c then prunes whole levels
+ * mid-image) return SA_EUNSUPPORTED; the test is on each loop's node total,
+ * so it also refuses a tree that crosses the limit only in its last row,
+ * which the reference builds unpruned.  Only palettes of about a thousand
+ * colours or more on busy images come near it. */
 int sa_palette_build(const void* src, int src_fmt, int64_t width, int64_t height, int colors,
                      int assoc, int gray, int keep_tree, void* stream, sa_palette* out,
                      char* err, size_t errlen);
