@@ -86,6 +86,10 @@ int sa_palette_create(const double* entries, int count, int assoc, const int32_t
                       const int32_t* parent, const uint32_t* color_number,
                       const uint64_t* number_unique, int64_t nodes, sa_palette* out,
                       char* err, size_t errlen);
+/* Releases a palette.  No work queued after the call may use it; work queued
+ * before it may still be running.  The call does not wait for the device: the
+ * palette's memory goes back to the runtime with those of later destroys,
+ * after one device synchronisation once they hold more than 256 MiB. */
 int sa_palette_destroy(sa_palette p);
 int sa_palette_count(sa_palette p);
 

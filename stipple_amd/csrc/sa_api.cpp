@@ -473,17 +473,48 @@ extern "C" int sa_palette_create(const double* entries, int count, int assoc,
   return SA_OK;
 }
 
+namespace {
+// Destroyed palettes' device blocks.  Work queued before a destroy may still
+// read the palette, so its blocks cannot go back at once; a device
+// synchronisation per destroy, though, stalls whichever thread drops a
+// palette (a serving loop's palette builder waited a whole walk step per
+// palette).  So the blocks wait here and go back after one synchronisation
+// once they hold more than kDeferBytes; no work queued after the destroy may
+// use the palette (the contract), so that synchronisation covers every use.
+std::mutex g_defer_mu;
+std::vector<void*> g_defer;
+size_t g_defer_bytes = 0;
+constexpr size_t kDeferBytes = (size_t)256 << 20;
+
+void defer_free(void* q, size_t bytes) {
+  if (!q) return;
+  std::vector<void*> drop;
+  {
+    std::lock_guard<std::mutex> g(g_defer_mu);
+    g_defer.push_back(q);
+    g_defer_bytes += bytes;
+    if (g_defer_bytes <= kDeferBytes) return;
+    drop.swap(g_defer);
+    g_defer_bytes = 0;
+  }
+  (void)hipDeviceSynchronize();
+  for (void* d : drop) (void)hipFree(d);
+}
+}  // namespace
+
 extern "C" int sa_palette_destroy(sa_palette p) {
   if (!p) return SA_OK;
-  (void)hipDeviceSynchronize();
+  // Sizes for the deferred list's budget (estimates: the blocks' exact sizes
+  // are not kept).
+  const size_t img = (size_t)p->hdr.bytes + 256 + sizeof(double) * 4 * (size_t)p->count;
   if (p->dev_block) {
-    (void)hipFree(p->dev_block);
+    defer_free(p->dev_block, img);
   } else {
-    if (p->dev_image) (void)hipFree(p->dev_image);
-    if (p->dev_hdr) (void)hipFree(p->dev_hdr);
-    if (p->dev_entries) (void)hipFree(p->dev_entries);
+    defer_free(p->dev_image, (size_t)p->hdr.bytes);
+    defer_free(p->dev_hdr, 256);
+    defer_free(p->dev_entries, sizeof(double) * 4 * (size_t)p->count);
   }
-  if (p->tree_block) (void)hipFree(p->tree_block);
+  if (p->tree_block) defer_free(p->tree_block, (size_t)std::max<int64_t>(p->nodes, 1) * 128);
   delete p;
   return SA_OK;
 }

@@ -8,10 +8,11 @@
 // what a serving loop or a batch builder of varying batch sizes does.  So the
 // library keeps its own cache: blocks come from hipMalloc, go back to the
 // cache when a call is done with them (an event recorded on the call's
-// stream marks the end of their last use), and are handed out again best-fit;
-// a block taken on another stream than its last user's waits for that event
-// on the device.  Free cached bytes above kKeep go back to the runtime, oldest
-// first.  A stream under graph capture allocates stream-ordered instead (a
+// stream marks the end of their last use), and are handed out again best-fit
+// to the same stream, or to another one once that event has completed (a
+// stream never waits for another stream's queued work through the cache; it
+// gets a new block instead).  Free cached bytes above kKeep go back to the
+// runtime, oldest first.  A stream under graph capture allocates stream-ordered instead (a
 // captured graph owns such memory; hipMalloc is not allowed during capture).
 //
 // Host code only: no device work.
@@ -74,18 +75,24 @@ hipError_t scratch_alloc(void** out, size_t bytes, hipStream_t s) {
   if (bytes == 0) bytes = 256;
   if (capturing(s)) return hipMallocAsync(out, bytes, s);
   std::lock_guard<std::mutex> g(g_mu);
+  // Best fit among the blocks this stream used last or whose last use is
+  // already over: a block still in use on another stream would make this
+  // stream wait for that stream's queued work (a palette build behind a walk
+  // step, say), so such blocks are left alone and a new one is allocated.
   size_t best = g_blocks.size();
   for (size_t i = 0; i < g_blocks.size(); ++i) {
     const Block& b = g_blocks[i];
     if (b.busy || b.bytes < bytes || b.bytes > 2 * bytes + ((size_t)64 << 20)) continue;
-    if (best == g_blocks.size() || b.bytes < g_blocks[best].bytes) best = i;
+    if (best != g_blocks.size() && b.bytes >= g_blocks[best].bytes) continue;
+    if (b.last != s) {
+      const hipError_t q = hipEventQuery(b.ev);
+      if (q == hipErrorNotReady) (void)hipGetLastError();  // not an error: keep it off the launch checks
+      if (q != hipSuccess) continue;
+    }
+    best = i;
   }
   if (best < g_blocks.size()) {
     Block& b = g_blocks[best];
-    if (b.last != s) {
-      const hipError_t e = hipStreamWaitEvent(s, b.ev, 0);
-      if (e != hipSuccess) return e;
-    }
     b.busy = true;
     b.last = s;
     g_free -= b.bytes;

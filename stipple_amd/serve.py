@@ -23,6 +23,7 @@ and in any mix (tests/test_gpu_serve.py).
 """
 from __future__ import annotations
 
+import collections
 import queue
 import threading
 import time
@@ -43,7 +44,10 @@ class PaletteBuilder:
         import torch
         self.max_px = int(max_batch_px)
         self.max_items = int(max_items)
-        self._q = queue.Queue()
+        # Requests in submission order; an item that does not fit a batch goes
+        # back to the front, so it leads the next batch.
+        self._q = collections.deque()
+        self._cv = threading.Condition()
         self._stop = False
         self.batches = 0
         self._threads = []
@@ -53,33 +57,30 @@ class PaletteBuilder:
             self._threads.append(th)
 
     def submit(self, src, width, height, src_fmt, colors, assoc, gray, callback):
-        self._q.put((src, width, height, src_fmt, colors, assoc, gray, callback))
+        with self._cv:
+            self._q.append((src, width, height, src_fmt, colors, assoc, gray, callback))
+            self._cv.notify()
 
     def _take(self):
-        first = self._q.get()
-        if first is None:
-            return None
-        got, px = [first], first[1] * first[2]
-        while len(got) < self.max_items:
-            try:
-                nxt = self._q.get_nowait()
-            except queue.Empty:
-                break
-            if nxt is None:
-                self._q.put(None)  # let the other threads see the stop too
-                break
-            if px + nxt[1] * nxt[2] > self.max_px:
-                self._q.put(nxt)
-                break
-            got.append(nxt)
-            px += nxt[1] * nxt[2]
-        return got
+        with self._cv:
+            while not self._q and not self._stop:
+                self._cv.wait()
+            if not self._q:
+                return None  # stopped and drained
+            first = self._q.popleft()
+            got, px = [first], first[1] * first[2]
+            while self._q and len(got) < self.max_items:
+                nxt = self._q[0]
+                if px + nxt[1] * nxt[2] > self.max_px:
+                    break
+                got.append(self._q.popleft())
+                px += nxt[1] * nxt[2]
+            return got
 
     def _loop(self, stream):
         while True:
             got = self._take()
             if got is None:
-                self._q.put(None)
                 return
             items = [g[:7] for g in got]
             try:
@@ -93,7 +94,9 @@ class PaletteBuilder:
                 g[7](pal, err)
 
     def close(self):
-        self._q.put(None)
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
         for th in self._threads:
             th.join()
 
@@ -216,7 +219,12 @@ class SeqPipeline:
         while True:
             if stop is not None and stop.is_set():
                 break
-            self._admit(block=not self._live)
+            # Wait for a palette only when nothing walks and something is still
+            # being classified (waiting otherwise would idle out the timeout at
+            # the end of every run).
+            with self._lock:
+                waiting = self._pending > 0
+            self._admit(block=not self._live and waiting)
             self._finish(block=False)
             if not self._live:
                 self._finish(block=True)
