@@ -10,8 +10,12 @@ other stream's work can wait behind the walk on a shared hardware queue.
 
 Palettes are classified by a `PaletteBuilder`: whatever has arrived is built
 as one batch (sa_palette_build_batch: one launch per kernel for all images),
-and requests that arrive meanwhile join the next batch.  A few builder
-threads, one stream each, keep batches in flight while another reads back.
+and requests that arrive meanwhile join the next batch.  A request that finds
+a builder thread free (fewer requests queued or building than threads) is
+built on its own, so the first images of a burst -- the largest, when a batch
+is submitted largest first -- start walking after their own builds rather
+than after a whole batch's.  A few builder threads, one stream each, keep batches in flight
+while another reads back.
 
 `Server` admits mixed requests -- stills and video segments, for the
 sequential, block and Jacobi engines -- against an HBM budget, runs the block
@@ -49,6 +53,8 @@ class PaletteBuilder:
         self._q = collections.deque()
         self._cv = threading.Condition()
         self._stop = False
+        self._building = 0               # batches taken and not yet built
+        self._nthreads = max(1, threads)
         self.batches = 0
         self._threads = []
         for _ in range(max(1, threads)):
@@ -58,7 +64,10 @@ class PaletteBuilder:
 
     def submit(self, src, width, height, src_fmt, colors, assoc, gray, callback):
         with self._cv:
-            self._q.append((src, width, height, src_fmt, colors, assoc, gray, callback))
+            # A free builder thread: this request is built alone (see the module
+            # docstring).
+            solo = len(self._q) + self._building < self._nthreads
+            self._q.append((src, width, height, src_fmt, colors, assoc, gray, callback, solo))
             self._cv.notify()
 
     def _take(self):
@@ -68,8 +77,9 @@ class PaletteBuilder:
             if not self._q:
                 return None  # stopped and drained
             first = self._q.popleft()
+            self._building += 1
             got, px = [first], first[1] * first[2]
-            while self._q and len(got) < self.max_items:
+            while self._q and len(got) < self.max_items and not first[8]:
                 nxt = self._q[0]
                 if px + nxt[1] * nxt[2] > self.max_px:
                     break
@@ -90,6 +100,9 @@ class PaletteBuilder:
                         for p, c in zip(pals, status)]
             except L.EngineError as e:
                 pals, errs = [None] * len(got), [str(e)] * len(got)
+            finally:
+                with self._cv:
+                    self._building -= 1
             for g, pal, err in zip(got, pals, errs):
                 g[7](pal, err)
 
