@@ -783,8 +783,6 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       return fail(err, errlen, SA_EINVAL, "image %d: bad pixel format", i);
     if (im.dst == im.src && (im.src_fmt != im.dst_fmt))
       return fail(err, errlen, SA_EINVAL, "image %d: in-place needs the same format", i);
-    if (im.palette->hdr.bytes > 96 * 1024)
-      return fail(err, errlen, SA_EUNSUPPORTED, "image %d: palette search image above 96 KiB", i);
     if (im.palette->count > 65535)
       return fail(err, errlen, SA_EUNSUPPORTED, "image %d: the walk's memo holds indices below 65535", i);
   }
@@ -815,8 +813,9 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
     size_t lds = 0;
     for (size_t k = 0; k < sel.size(); ++k) {
       const sa_image& im = images[sel[k]];
-      shape[k] = memo_shape(o, assoc, im.palette->hdr.bytes, im.palette->count);
-      lds = std::max(lds, shape[k].bytes + (size_t)im.palette->hdr.bytes + sa::kSeqRingBytes);
+      const int img_lds = sa::seq_img_lds_bytes(im.palette->hdr.bytes);
+      shape[k] = memo_shape(o, assoc, img_lds, im.palette->count);
+      lds = std::max(lds, shape[k].bytes + (size_t)img_lds + sa::kSeqRingBytes);
     }
     if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "LDS %zu too large", lds);
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, g_lds_per_cu / lds));
@@ -1047,12 +1046,11 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
       return fail(err, errlen, SA_EINVAL, "image %d: bad pixel format", i);
     if (im.dst == im.src && (im.src_fmt != im.dst_fmt))
       return fail(err, errlen, SA_EINVAL, "image %d: in-place needs the same format", i);
-    if (im.palette->hdr.bytes > 96 * 1024)
-      return fail(err, errlen, SA_EUNSUPPORTED, "image %d: palette search image above 96 KiB", i);
     if (im.palette->count > 65535)
       return fail(err, errlen, SA_EUNSUPPORTED, "image %d: the walk's memo holds indices below 65535", i);
-    const MemoShape ms = memo_shape(o, im.palette->assoc, im.palette->hdr.bytes, im.palette->count);
-    const size_t lds = ms.bytes + (size_t)im.palette->hdr.bytes + sa::kSeqRingBytes;
+    const int img_lds = sa::seq_img_lds_bytes(im.palette->hdr.bytes);
+    const MemoShape ms = memo_shape(o, im.palette->assoc, img_lds, im.palette->count);
+    const size_t lds = ms.bytes + (size_t)img_lds + sa::kSeqRingBytes;
     if (lds > g_lds_per_cu) return fail(err, errlen, SA_EUNSUPPORTED, "image %d: LDS %zu too large", i, lds);
   }
   std::vector<JobItem> fresh;
@@ -1063,7 +1061,7 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     JobItem J;
     J.id = job->next_id + i;
     J.assoc = im.palette->assoc;
-    J.img_bytes = im.palette->hdr.bytes;
+    J.img_bytes = sa::seq_img_lds_bytes(im.palette->hdr.bytes);  // its LDS share
     const MemoShape ms = memo_shape(o, J.assoc, J.img_bytes, im.palette->count);
     const int64_t px = im.width * im.height;
     const size_t xs_b = stage_bytes(im.src_fmt, px), idx_b = (2 * (size_t)px + 255) & ~(size_t)255;

@@ -198,6 +198,14 @@ struct SeqItem {
   int32_t memo_narrow;    // 1: memo entries are bytes (palettes of at most 255 colours)
 };
 
+// Palette search images up to this size are staged in LDS for the sequential
+// walk; larger ones (palettes of more than about 1400 colours) stay in HBM and
+// the walk reads them there, leaving the LDS to its memo cache.
+constexpr int kSeqImageLdsMax = 96 * 1024;
+__host__ __device__ inline int seq_img_lds_bytes(int bytes) {
+  return bytes <= kSeqImageLdsMax ? bytes : 0;
+}
+
 // LDS bytes of the walk's memo cache: per slot a 4-byte tag and 64 entries.
 __host__ __device__ inline uint32_t seq_memo_bytes(int slots, int narrow) {
   return (uint32_t)slots * (4u + (narrow ? 64u : 128u));

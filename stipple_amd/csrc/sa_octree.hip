@@ -1697,7 +1697,7 @@ __device__ void level_lists(const Tree& T, TreeShared& S) {
   __syncthreads();
 }
 
-enum PassMode { kPruneCube = 0, kReduce = 1, kShape = 2 };
+enum PassMode { kPruneCube = 0, kReduce = 1, kShape = 2, kPruneLevel = 3 };
 
 // Attachment, the S ancestor, subtree and group sizes, post-order starts and
 // pre-order indices of the current tree.
@@ -1715,6 +1715,9 @@ __device__ void tree_shape(const Tree& T, TreeShared& S, int mode, double thr) {
       }
       bool s = false;
       if (a && mode == kPruneCube) s = t.level[y] > T.depth;
+      // PruneLevel (rd_octree.cpp:95-102) enters PruneChild at every node of
+      // the cube depth; the nodes below it are attempted inside that call.
+      if (a && mode == kPruneLevel) s = t.level[y] == T.depth;
       if (a && mode == kReduce) s = t.qe[y] <= thr;
       W.att[y] = a;
       W.anc[y] = !a ? -1 : (s ? y : up);
@@ -2135,6 +2138,396 @@ __global__ void k_root_bound(const Frame* __restrict__ fr) {
     d.dinfo[2] = root_qe;
     d.info[14] = exact ? 1 : 0;
   }
+}
+
+// ---- 6. trees that pass MaxQNodes: classification row by row ----------------------
+// When the live node count has passed MaxQNodes at the start of a row, the
+// reference prunes the cube depth's level (PruneLevel, then depth - 1) and the
+// rows after it descend to the new depth (rd_octree.cpp:366-369, :380-383).
+// Node creation, the prunes and every sum then depend on the row at which the
+// count crossed, so such an image is classified again by one workgroup that
+// follows the reference row by row, each row in parallel:
+//
+//   * Descent (a chunk of up to kRowChunk consecutive runs at a time, level by
+//     level).  A run whose child is missing asks for it under (parent, child
+//     index); the first run that asks creates it, the others take it.  New
+//     nodes are staged while the chunk descends, then numbered in the
+//     reference's creation order -- by creating run, and by level within a
+//     run (a run creates consecutive levels down to its leaf).
+//   * Sums (level by level).  Every run's term is formed in parallel, the
+//     runs are sorted by node (then by run), and each node's chain is folded
+//     in run order onto its value from the rows before.  The root's chain of
+//     running totals is added serially, run by run and level by level, so the
+//     root's quantize_error is exact here.
+//   * At the start of a row PruneLevel runs as a tree pass (kPruneLevel); after
+//     a loop-1 row whose colours passed the palette, PruneToCubeDepth.
+//
+// The pass leaves the node arrays, the live and colour counts and the root's
+// error where k_reduce takes them.  A launch stops before a row whose new
+// nodes could pass the node capacity (status 1); the host grows the arrays and
+// launches again from that row.
+constexpr int kRowChunk = 2048;
+constexpr int kRowLevels = 8;
+constexpr int kStageMax = kRowChunk * kRowLevels;
+constexpr int kRowHash = 4096;
+
+struct RowsState {
+  int64_t y, r, M, live, colors;
+  double root_qe;
+  int32_t loop, depth, status, pad;
+};
+
+struct RowsArgs {
+  const float4* px;
+  const int32_t* count;
+  const uint32_t* pack;
+  const int32_t* row_first;  // first run of each row, row_first[h] = R
+  int64_t h;
+  int32_t assoc, kids, maxc, pad;
+  TreeDev t;
+  TreeWork W;
+  int64_t cap;               // node capacity of t and W
+  int32_t* at;               // node of each run of the chunk at each level, 9 x kRowChunk
+  double* rv;                // running value of each (level, run) pair, 8 x kRowChunk
+  int32_t* sp;               // staged nodes: parent, child index | level << 8, creating run
+  int32_t* sp_child;         // 16 per staged node
+  int32_t* fid;              // node id of each staged node
+  RowsState* st;
+  int32_t* info;
+  double* dinfo;
+};
+
+// A node reference during a chunk's descent: an id (>= 0), none (-1), or
+// staged node s (-2 - s).
+__device__ __forceinline__ int32_t rows_child(const RowsArgs& a, int32_t p, int cid) {
+  return p >= 0 ? a.t.child[16 * (int64_t)p + cid] : a.sp_child[16 * (-2 - p) + cid];
+}
+
+__device__ __forceinline__ void rows_pixel(const RowsArgs& a, int64_t r, double av[4]) {
+  const float4 f = a.px[r];
+  assoc_pixel(a.assoc != 0, f.x, f.y, f.z, f.w, av);
+}
+
+// One chunk of a row: runs [c0, c0 + n), levels 1..limit.  Everything uniform
+// is passed by reference and updated identically in every thread.
+__device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw, int64_t c0,
+                           int n, int limit, int colors_level, int64_t& M, int64_t& live,
+                           int64_t& colors, double& root_qe) {
+  const int tid = threadIdx.x;
+  const bool assoc = a.assoc != 0;
+  const TreeDev& t = a.t;
+  int32_t* cur = (int32_t*)raw;                               // kRowChunk
+  uint64_t* hkey = (uint64_t*)(raw + 4 * kRowChunk);          // kRowHash
+  uint32_t* hmin = (uint32_t*)(hkey + kRowHash);              // kRowHash
+  int32_t* hval = (int32_t*)(hmin + kRowHash);                // kRowHash
+  int32_t* nstage = (int32_t*)&S.scal[0];
+  constexpr int kPer = kRowChunk / kTreeThreads;
+  for (int i = tid; i < n; i += kTreeThreads) {
+    cur[i] = 0;
+    a.at[i] = 0;
+  }
+  if (tid == 0) *nstage = 0;
+  __syncthreads();
+  // -- descent ------------------------------------------------------------------
+  for (int l = 1; l <= limit; ++l) {
+    for (int k = tid; k < kRowHash; k += kTreeThreads) {
+      hkey[k] = kEmpty64;
+      hmin[k] = kEmpty32;
+    }
+    __syncthreads();
+    int32_t nxt[kPer], par[kPer];
+    uint32_t slot[kPer];
+    int cidv[kPer];
+    for (int q = 0; q < kPer; ++q) {
+      const int i = tid + q * kTreeThreads;
+      nxt[q] = -1;
+      slot[q] = kEmpty32;
+      if (i >= n) continue;
+      const int32_t p = cur[i];
+      const uint32_t pk = a.pack[c0 + i];
+      const int cid = node_id(assoc, pk & 0xFFu, (pk >> 8) & 0xFFu, (pk >> 16) & 0xFFu, pk >> 24,
+                              8 - l);
+      par[q] = p;
+      cidv[q] = cid;
+      const int32_t c = rows_child(a, p, cid);
+      nxt[q] = c;
+      if (c != -1) continue;
+      const uint64_t key = ((((uint64_t)(uint32_t)p) << 8) | (uint64_t)cid) + 1ull;
+      uint32_t sl = hash32(key) & (kRowHash - 1);
+      for (;;) {  // at most kRowChunk keys in kRowHash slots: a free slot exists
+        const unsigned long long prev =
+            atomicCAS((unsigned long long*)&hkey[sl], (unsigned long long)kEmpty64,
+                      (unsigned long long)key);
+        if (prev == kEmpty64 || prev == key) break;
+        sl = (sl + 1) & (kRowHash - 1);
+      }
+      atomicMin(&hmin[sl], (uint32_t)i);
+      slot[q] = sl;
+    }
+    __syncthreads();
+    for (int q = 0; q < kPer; ++q) {
+      const int i = tid + q * kTreeThreads;
+      if (slot[q] == kEmpty32 || hmin[slot[q]] != (uint32_t)i) continue;
+      const int32_t s = atomicAdd(nstage, 1);  // < kStageMax: one per (run, level)
+      a.sp[3 * s] = par[q];
+      a.sp[3 * s + 1] = cidv[q] | (l << 8);
+      a.sp[3 * s + 2] = i;
+      for (int k = 0; k < 16; ++k) a.sp_child[16 * s + k] = -1;
+      if (par[q] >= 0)
+        a.t.child[16 * (int64_t)par[q] + cidv[q]] = -2 - s;
+      else
+        a.sp_child[16 * (-2 - par[q]) + cidv[q]] = -2 - s;
+      hval[slot[q]] = -2 - s;
+    }
+    __syncthreads();
+    for (int q = 0; q < kPer; ++q) {
+      const int i = tid + q * kTreeThreads;
+      if (i >= n) continue;
+      const int32_t v = slot[q] == kEmpty32 ? nxt[q] : hval[slot[q]];
+      cur[i] = v;
+      a.at[l * kRowChunk + i] = v;
+    }
+    __syncthreads();
+  }
+  // -- number the staged nodes in creation order -----------------------------------
+  const int32_t ns = *nstage;
+  int32_t* made = (int32_t*)raw;                    // per run: nodes it created
+  int32_t* base = made + kRowChunk;                 // their exclusive scan
+  for (int i = tid; i < n; i += kTreeThreads) made[i] = 0;
+  __syncthreads();
+  for (int s = tid; s < ns; s += kTreeThreads) atomicAdd(&made[a.sp[3 * s + 2]], 1);
+  __syncthreads();
+  for (int i = tid; i < n; i += kTreeThreads) base[i] = made[i];
+  __syncthreads();
+  block_scan(base, n, S);
+  for (int s = tid; s < ns; s += kTreeThreads) {
+    const int i = a.sp[3 * s + 2];
+    const int lev = a.sp[3 * s + 1] >> 8;
+    a.fid[s] = (int32_t)(M + 1 + base[i] + (lev - (limit - made[i] + 1)));
+  }
+  __syncthreads();
+  int64_t ncol = 0;
+  for (int s = tid; s < ns; s += kTreeThreads) {
+    const int32_t f = a.fid[s];
+    const int32_t p = a.sp[3 * s];
+    const int cid = a.sp[3 * s + 1] & 0xFF, lev = a.sp[3 * s + 1] >> 8;
+    const int i = a.sp[3 * s + 2];
+    t.key[f] = node_key(a.pack[c0 + i], lev, assoc);
+    t.level[f] = lev;
+    t.id[f] = cid;
+    t.parent[f] = p >= 0 ? p : a.fid[-2 - p];
+    t.qe[f] = 0.0;
+    t.share[f] = 0.0;
+    for (int c = 0; c < 4; ++c) t.total[4 * (int64_t)f + c] = 0.0;
+    t.nu[f] = 0;
+    t.color_number[f] = 0;
+    t.row[f] = -1;
+    for (int k = 0; k < 16; ++k) {
+      const int32_t ch = a.sp_child[16 * s + k];
+      t.child[16 * (int64_t)f + k] = ch >= -1 ? ch : a.fid[-2 - ch];
+    }
+    if (p >= 0) t.child[16 * (int64_t)p + cid] = f;
+    if (lev == colors_level) ++ncol;
+  }
+  for (int l = 1; l <= limit; ++l)
+    for (int i = tid; i < n; i += kTreeThreads) {
+      const int32_t v = a.at[l * kRowChunk + i];
+      if (v < -1) a.at[l * kRowChunk + i] = a.fid[-2 - v];
+    }
+  ncol = block_sum(ncol, S);  // synchronises
+  M += ns;
+  live += ns;
+  colors += ncol;
+  // -- sums ---------------------------------------------------------------------------
+  uint64_t* keys = (uint64_t*)raw;                      // kRowChunk
+  double* tm = (double*)(raw + 8 * kRowChunk);          // kRowChunk
+  int n2 = 1;
+  while (n2 < n) n2 <<= 1;
+  for (int l = 1; l <= limit; ++l) {
+    for (int i = tid; i < n2; i += kTreeThreads) {
+      if (i >= n) {
+        keys[i] = ~0ull;
+        continue;
+      }
+      const int32_t node = a.at[l * kRowChunk + i];
+      double av[4], mid[4];
+      rows_pixel(a, c0 + i, av);
+      node_mid(t.key[node], mid);
+      const double e0 = kScale * (av[0] - mid[0]);
+      const double e1 = kScale * (av[1] - mid[1]);
+      const double e2 = kScale * (av[2] - mid[2]);
+      const double e3 = assoc ? kScale * (av[3] - mid[3]) : 0.0;
+      double dd = e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
+      if (dd != dd) dd = 0.0;
+      tm[i] = (double)a.count[c0 + i] * sqrt(dd);
+      keys[i] = ((uint64_t)(uint32_t)node << 32) | (uint32_t)i;
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1)  // bitonic sort, ascending (node, run)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < n2; i += kTreeThreads) {
+          const int x = i ^ j;
+          if (x > i) {
+            const uint64_t u = keys[i], v = keys[x];
+            if ((u > v) == ((i & k) == 0)) {
+              keys[i] = v;
+              keys[x] = u;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    const bool leaf = l == limit;
+    for (int k = tid; k < n; k += kTreeThreads) {
+      const uint32_t node = (uint32_t)(keys[k] >> 32);
+      if (k > 0 && (uint32_t)(keys[k - 1] >> 32) == node) continue;
+      double v = t.qe[node];
+      double tot[4];
+      uint64_t nu = 0;
+      if (leaf) {
+        for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * (int64_t)node + c];
+        nu = t.nu[node];
+      }
+      for (int j = k; j < n && (uint32_t)(keys[j] >> 32) == node; ++j) {
+        const int i = (int)(uint32_t)keys[j];
+        v += tm[i];
+        a.rv[(l - 1) * kRowChunk + i] = v;
+        if (leaf) {
+          double av[4];
+          rows_pixel(a, c0 + i, av);
+          const int32_t cnt = a.count[c0 + i];
+          tot[0] += tot_term(cnt, clamp_pixel(av[0]));
+          tot[1] += tot_term(cnt, clamp_pixel(av[1]));
+          tot[2] += tot_term(cnt, clamp_pixel(av[2]));
+          tot[3] += tot_term(cnt, clamp_pixel(assoc ? av[3] : kRange));
+          nu += (uint64_t)cnt;
+        }
+      }
+      t.qe[node] = v;
+      if (leaf) {
+        for (int c = 0; c < 4; ++c) t.total[4 * (int64_t)node + c] = tot[c];
+        t.nu[node] = nu;
+      }
+    }
+    __syncthreads();
+  }
+  // The root's chain: run by run, level by level (rd_octree.cpp:341).
+  double* sroot = (double*)&S.scal[1];
+  if (tid == 0) {
+    double q = root_qe;
+    for (int i = 0; i < n; ++i)
+      for (int l = 1; l <= limit; ++l) q += a.rv[(l - 1) * kRowChunk + i];
+    *sroot = q;
+  }
+  __syncthreads();
+  root_qe = *sroot;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kTreeThreads) void k_rows(RowsArgs a) {
+  __shared__ TreeShared S;
+  __shared__ __align__(16) unsigned char raw[4 * kRowChunk + 16 * kRowHash];
+  const int tid = threadIdx.x;
+  const RowsState st = *a.st;
+  int64_t y = st.y, r = st.r, M = st.M, live = st.live, colors = st.colors;
+  double root_qe = st.root_qe;
+  int loop = st.loop, depth = st.depth, status = 0;
+  const TreeDev& t = a.t;
+  if (y == 0 && M == 0 && tid == 0) {  // the root (rd_octree.cpp:282-285)
+    t.key[0] = 0;
+    t.level[0] = 0;
+    t.id[0] = 0;
+    t.parent[0] = 0;
+    t.qe[0] = 0.0;
+    t.share[0] = 0.0;
+    for (int c = 0; c < 4; ++c) t.total[c] = 0.0;
+    t.nu[0] = 0;
+    t.color_number[0] = 0;
+    t.row[0] = -1;
+    for (int k = 0; k < 16; ++k) t.child[k] = -1;
+  }
+  __syncthreads();
+  while (y < a.h) {
+    const int64_t re = a.row_first[y + 1];
+    int limit = loop == 1 ? kMaxDepth : depth;
+    if (M + 1 + (re - r) * limit > a.cap) {
+      status = 1;
+      break;
+    }
+    if (live > kMaxQNodes) {  // PruneLevel, then depth - 1
+      const Tree T{a.t, a.W, a.kids, a.maxc, depth, M};
+      level_lists(T, S);
+      int64_t cnt = 0;
+      double nx = 0.0;
+      tree_pass(T, S, kPruneLevel, 0.0, &live, &cnt, &nx);
+      --depth;
+      if (loop == 2) limit = depth;
+    }
+    for (int64_t c0 = r; c0 < re; c0 += kRowChunk) {
+      const int n = (int)(re - c0 < kRowChunk ? re - c0 : kRowChunk);
+      rows_chunk(a, S, raw, c0, n, limit, limit, M, live, colors, root_qe);
+    }
+    if (loop == 1 && colors > a.maxc) {  // PruneToCubeDepth, then loop 2
+      const Tree T{a.t, a.W, a.kids, a.maxc, depth, M};
+      level_lists(T, S);
+      int64_t cnt = 0;
+      double nx = 0.0;
+      tree_pass(T, S, kPruneCube, 0.0, &live, &cnt, &nx);
+      loop = 2;
+    }
+    ++y;
+    r = re;
+  }
+  if (status == 0) {
+    const Tree T{a.t, a.W, a.kids, a.maxc, depth, M};
+    level_lists(T, S);
+    if (tid < 10) a.W.lvl_off[tid] = S.off[tid];
+    if (tid == 0) {
+      a.info[2] = (int32_t)colors;
+      a.info[3] = 0;
+      a.info[6] = 0;
+      a.info[7] = (int32_t)live;
+      a.info[14] = 1;
+      a.dinfo[0] = root_qe;
+      a.dinfo[2] = root_qe;
+      t.qe[0] = root_qe;
+    }
+  }
+  if (tid == 0) {
+    RowsState o;
+    o.y = y;
+    o.r = r;
+    o.M = M;
+    o.live = live;
+    o.colors = colors;
+    o.root_qe = root_qe;
+    o.loop = loop;
+    o.depth = depth;
+    o.status = status == 0 ? 2 : status;
+    o.pad = 0;
+    *a.st = o;
+  }
+}
+
+// row_first[y] = the first run of row y (every row starts a run), row_first[h] = R.
+__global__ void k_row_first(const int32_t* __restrict__ row, int64_t R, int64_t h,
+                            int32_t* __restrict__ row_first) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) row_first[h] = (int32_t)R;
+  if (i >= R) return;
+  if (i == 0 || row[i] != row[i - 1]) row_first[row[i]] = (int32_t)i;
+}
+
+// k_descent for one image.
+__global__ void k_descent_one(const Frame* __restrict__ fr) {
+  const Frame& d = *fr;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (!d.ok || d.info[6] != 0) return;
+  const PaletteDev h = *d.pb.hdr;
+  if (e >= desc_entries(h.kids)) return;
+  unsigned char* img = (unsigned char*)h.image;
+  ((uint32_t*)(img + h.off_desc))[e] =
+      descent3((const int16_t*)(img + h.off_child), h.kids, (uint32_t)e);
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -2734,8 +3127,16 @@ class SubBuild {
     k_reduce<<<F_, kTreeThreads, 0, s_>>>(d_fr_);
     if (grid_[kMapDesc]) k_descent<<<grid_[kMapDesc], 256, 0, s_>>>(d_fr_, map(kMapDesc));
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = keep_trees()) != hipSuccess) return e;
+    for (int f = 0; f < F_; ++f)
+      if ((e = keep_tree(f)) != hipSuccess) return e;
     if ((e = readback()) != hipSuccess) return e;
+    // Trees that may pass MaxQNodes are classified again row by row.
+    std::vector<int> rows;
+    for (int f = 0; f < F_; ++f)
+      if (H_[f].ok && hinfo(f)[6] == 1) rows.push_back(f);
+    for (int f : rows)
+      if ((e = rows_build(f)) != hipSuccess) return e;
+    if (!rows.empty() && (e = readback()) != hipSuccess) return e;
     for (int f = 0; f < F_; ++f) {
       Frame& d = H_[f];
       if (!d.ok) continue;
@@ -2744,9 +3145,7 @@ class SubBuild {
       if (in[6] == 4) {
         fail(f, 3, "internal: node hash table overflow");
       } else if (in[6] == 1) {
-        fail(f, 5,
-             "the octree would exceed MaxQNodes (266817) during classification; the level "
-             "pruning quantize.c performs then is not implemented on the device");
+        fail(f, 3, "internal: the row-by-row classification did not finish");
       } else if (in[6] == 2) {
         fail(f, 5, "search image above its capacity (%d rows, %d bytes)", in[10], in[11]);
       } else if (in[6] == 3) {
@@ -2766,12 +3165,20 @@ class SubBuild {
     return hipSuccess;
   }
 
-  // Copies of the node arrays for the images that keep their tree.
-  hipError_t keep_trees() {
-    for (int f = 0; f < F_; ++f) {
+  // A copy of the node arrays for an image that keeps its tree (replacing an
+  // earlier copy).
+  hipError_t keep_tree(int f) {
+    {
       Frame& d = H_[f];
       OctreeOut& o = out_[f];
-      if (!d.ok || !o.keep_tree) continue;
+      if (!d.ok || !o.keep_tree) return hipSuccess;
+      if (o.tree_block) {
+        hipError_t e = hipStreamSynchronize(s_);
+        if (e == hipSuccess) e = hipFree(o.tree_block);
+        if (e != hipSuccess) return e;
+        o.tree_block = nullptr;
+        o.tree = TreeDev{};
+      }
       const size_t n = (size_t)d.M + 1;
       const size_t sz[11] = {8 * n, 64 * n, 4 * n, 4 * n, 4 * n, 8 * n,
                              8 * n, 32 * n, 8 * n, 4 * n, 4 * n};
@@ -2796,6 +3203,105 @@ class SubBuild {
       }
     }
     return hipSuccess;
+  }
+
+  // Node and sweep arrays of capacity `cap` for the row-by-row classification.
+  hipError_t rows_arrays(RowsArgs& a, int64_t cap) {
+    Carve c;
+    c.want(&a.t.key, cap);
+    c.want(&a.t.child, 16 * cap);
+    c.want(&a.t.parent, cap);
+    c.want(&a.t.id, cap);
+    c.want(&a.t.level, cap);
+    c.want(&a.t.qe, cap);
+    c.want(&a.t.share, cap);
+    c.want(&a.t.total, 4 * cap);
+    c.want(&a.t.nu, cap);
+    c.want(&a.t.color_number, cap);
+    c.want(&a.t.row, cap);
+    int32_t** wp[11] = {&a.W.lvl, &a.W.att, &a.W.anc, &a.W.size, &a.W.gsz, &a.W.start,
+                        &a.W.gstart, &a.W.pre, &a.W.merged, &a.W.arr, &a.W.arr2};
+    for (int k = 0; k < 11; ++k) c.want(wp[k], cap + 2);
+    a.cap = cap;
+    return c.take(owned_, s_);
+  }
+
+  // The classification of an image whose live node count may pass MaxQNodes,
+  // row by row (k_rows), then Reduce, the colormap and the search image.
+  hipError_t rows_build(int f) {
+    Frame& d = H_[f];
+    const int64_t R = d.R, h = d.h;
+    const int64_t worst = 1 + (int64_t)kMaxDepth * R;  // every (run, level) creating a node
+    int64_t cap = std::min<int64_t>(worst, std::max<int64_t>((int64_t)1 << 20, 4 * R));
+    RowsArgs a{};
+    a.px = d.px;
+    a.count = d.count;
+    a.pack = d.pack;
+    a.h = h;
+    a.assoc = d.assoc;
+    a.kids = d.kids;
+    a.maxc = d.maxc;
+    a.info = d.info;
+    a.dinfo = d.dinfo;
+    int32_t *row_first = nullptr, *lvl_off = nullptr;
+    Carve c;
+    c.want(&row_first, h + 1);
+    c.want(&a.at, (int64_t)(kRowLevels + 1) * kRowChunk);
+    c.want(&a.rv, (int64_t)kRowLevels * kRowChunk);
+    c.want(&a.sp, 3 * (int64_t)kStageMax);
+    c.want(&a.sp_child, 16 * (int64_t)kStageMax);
+    c.want(&a.fid, (int64_t)kStageMax);
+    c.want(&lvl_off, 10);
+    c.want(&a.st, 1);
+    hipError_t e = c.take(owned_, s_);
+    if (e != hipSuccess) return e;
+    if ((e = rows_arrays(a, cap)) != hipSuccess) return e;
+    a.row_first = row_first;
+    a.W.lvl_off = lvl_off;
+    RowsState st{};
+    st.live = 1;
+    st.loop = 1;
+    st.depth = d.depth;
+    if ((e = hipMemcpyAsync(a.st, &st, sizeof st, hipMemcpyHostToDevice, s_)) != hipSuccess) return e;
+    k_row_first<<<blocks(R, 256), 256, 0, s_>>>(d.row, R, h, row_first);
+    for (;;) {
+      k_rows<<<1, kTreeThreads, 0, s_>>>(a);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = hipMemcpyAsync(&st, a.st, sizeof st, hipMemcpyDeviceToHost, s_)) != hipSuccess)
+        return e;
+      if ((e = hipStreamSynchronize(s_)) != hipSuccess) return e;
+      if (st.status != 1 || cap >= worst) break;
+      // Out of node capacity before row st.y: grow and go on from there.
+      RowsArgs b = a;
+      if ((e = rows_arrays(b, std::min<int64_t>(worst, 2 * cap))) != hipSuccess) return e;
+      const size_t n = (size_t)st.M + 1;
+      const struct {
+        void* dst;
+        const void* src;
+        size_t bytes;
+      } mv[11] = {{b.t.key, a.t.key, 8 * n},          {b.t.child, a.t.child, 64 * n},
+                  {b.t.parent, a.t.parent, 4 * n},    {b.t.id, a.t.id, 4 * n},
+                  {b.t.level, a.t.level, 4 * n},      {b.t.qe, a.t.qe, 8 * n},
+                  {b.t.share, a.t.share, 8 * n},      {b.t.total, a.t.total, 32 * n},
+                  {b.t.nu, a.t.nu, 8 * n},            {b.t.color_number, a.t.color_number, 4 * n},
+                  {b.t.row, a.t.row, 4 * n}};
+      for (const auto& m : mv)
+        if ((e = hipMemcpyAsync(m.dst, m.src, m.bytes, hipMemcpyDeviceToDevice, s_)) != hipSuccess)
+          return e;
+      b.W.lvl_off = lvl_off;
+      a = b;
+      cap = a.cap;
+    }
+    if (st.status != 2) return hipSuccess;  // info[6] stays 1: reported as unfinished
+    d.t = a.t;
+    d.W = a.W;
+    d.M = st.M;
+    d.depth = st.depth;
+    if ((e = upload()) != hipSuccess) return e;
+    k_reduce<<<1, kTreeThreads, 0, s_>>>(d_fr_ + f);
+    k_descent_one<<<blocks(desc_entries(d.kids), 256), 256, 0, s_>>>(d_fr_ + f);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return keep_tree(f);
   }
 
   int64_t sumP_ = 0;

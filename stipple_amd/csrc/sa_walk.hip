@@ -41,8 +41,10 @@
 // all zero and is filled without touching HBM.
 //
 // Palette search on a miss: the search image (sa_palette.h) is staged in LDS
-// once per image; the first three descent levels are one table read, the
-// rest are uniform LDS reads, and the candidate range is scored across lanes
+// once per image (a search image above 96 KiB -- palettes of more than about
+// 1400 colours -- is read where it lies in HBM instead); the first three
+// descent levels are one table read, the rest are uniform reads, and the
+// candidate range is scored across lanes
 // with a DPP min-reduction (the last index at the minimum wins, which is the
 // reference's `<=` rule).
 #include "sa_internal.h"
@@ -504,10 +506,14 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     const uint32_t memo_bytes = CACHE ? seq_memo_bytes((int)slots, narrow) : 0u;
     m.tag = (uint32_t*)lds;
     m.line = lds + 4 * slots;
-    unsigned char* img = lds + memo_bytes;
     const PaletteDev pd = *it.pal;
+    // The search image in LDS, or read where it lies when it is too large
+    // (the memo cache then has the LDS).
+    const bool img_lds = pd.bytes <= kSeqImageLdsMax;
+    unsigned char* img = img_lds ? lds + memo_bytes : (unsigned char*)pd.image;
+    double* ring = (double*)(lds + memo_bytes + (img_lds ? pd.bytes : 0));
     {
-      copy16((uint4*)img, (const uint4*)pd.image, (uint32_t)pd.bytes / 16);
+      if (img_lds) copy16((uint4*)img, (const uint4*)pd.image, (uint32_t)pd.bytes / 16);
       if (CACHE) {
         if (t0 == 0) {
           for (uint32_t k = lane; k < slots; k += kWave) m.tag[k] = kEmptyTag;
@@ -528,7 +534,6 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     m.table = it.memo;
     const uint64_t deadline =
         (budget && it.state) ? __builtin_amdgcn_s_memrealtime() + budget : 0;
-    double* ring = (double*)(img + pd.bytes);
     if (CACHE && narrow)
       stop = walk_image<ASSOC, CACHE, true>(it, W, m, s, ring, cnt, t0, stop, deadline);
     else

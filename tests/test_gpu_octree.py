@@ -178,26 +178,70 @@ def test_octree_batch_splits_into_sub_batches(gpu):
 
 
 def test_octree_batch_item_failure_is_its_own(gpu, oracle):
-    """An image the device does not build (its tree would exceed MaxQNodes)
-    fails alone with SA_EUNSUPPORTED; the images beside it build exactly."""
-    rng = np.random.default_rng(12)
-    noise = rng.integers(0, 256, (700, 700, 3), dtype=np.uint8)
+    """An image the device does not build (above 2^28 pixels) fails alone with
+    SA_EUNSUPPORTED; the images beside it build exactly, one of them a tree
+    that passes MaxQNodes (classified row by row)."""
+    import torch
     cases = _batch_cases(13)[:4]
+    cases.insert(1, (_noise(512, 512, 3, 21), 4096))
     srcs = [H.to_device(img) for img, _ in cases]
-    nsrc = H.to_device(noise)
-    _KEEP.extend(srcs + [nsrc])
+    big_w = big_h = 16385  # 2^28 + 32769 pixels: refused before any kernel reads them
+    big = torch.empty(big_w * big_h * 3, dtype=torch.uint8, device="cuda")
+    _KEEP.extend(srcs + [big])
     items = [(s, img.shape[1], img.shape[0], H.fmt_of(img), c, img.shape[2] == 4, False, True)
              for s, (img, c) in zip(srcs, cases)]
-    items.insert(2, (nsrc, 700, 700, gpu.PIX_RGB8, 65536, False, False, True))
+    items.insert(3, (big, big_w, big_h, gpu.PIX_RGB8, 16, False, False, True))
     say("octree batch with a failing item")
     pals, status = gpu.Palette.build_batch(items, strict=False)
-    assert status[2] == gpu.SA_EUNSUPPORTED and pals[2] is None
-    assert [s for k, s in enumerate(status) if k != 2] == [0] * 4
-    with pytest.raises(gpu.EngineError, match="item 2"):
+    assert status[3] == gpu.SA_EUNSUPPORTED and pals[3] is None
+    assert [s for k, s in enumerate(status) if k != 3] == [0] * 5
+    with pytest.raises(gpu.EngineError, match="item 3"):
         gpu.Palette.build_batch(items)
+    del big
     for k, (img, c) in enumerate(cases):
-        pal = pals[k if k < 2 else k + 1]
+        pal = pals[k if k < 3 else k + 1]
         _check_built(oracle, pal, synth.widen_q16(img), c, img.shape[2] == 4, False)
+
+
+def _noise(w, h, ch, seed):
+    return np.random.default_rng(seed).integers(0, 256, (h, w, ch), dtype=np.uint8)
+
+
+def _initial_depth(colours, assoc):  # src/rd_octree.cpp:255-275 (not greyscale)
+    d, c = 1, colours
+    while c:
+        c >>= 2
+        d += 1
+    if d > 2:
+        d -= 1
+    if assoc and d > 5:
+        d -= 1
+    return max(2, min(d, 8))
+
+
+@pytest.mark.parametrize("kind,w,h,colours,seed", [
+    ("noise rgb", 512, 512, 4096, 21),      # loop 2 prunes once: depth 7 -> 6
+    ("noise rgba", 512, 512, 4096, 22),     # 6 -> 5, 16-ary tree
+    ("noise rgb", 512, 512, 16384, 23),     # two prunes: 8 -> 6
+    ("noise rgb", 384, 384, 65536, 24),     # 65536 colours: loop 1 runs long
+    ("sinusoid rgba", 1024, 1024, 16384, 27)])
+def test_octree_past_maxqnodes(gpu, oracle, kind, w, h, colours, seed):
+    """Trees whose live node count passes MaxQNodes during classification: the
+    reference prunes the cube depth's level at the next row's start and later
+    rows descend to the new depth (src/rd_octree.cpp:366-369, :380-383).  Every
+    node, the pruned ones included, equals the oracle's, the root's
+    quantize_error exactly."""
+    if kind.startswith("noise"):
+        img = _noise(w, h, 4 if "rgba" in kind else 3, seed)
+    else:
+        img = synth.sinusoid_image(w, h, np.random.default_rng(seed))
+    assoc = img.shape[2] == 4
+    src = H.to_device(img)
+    _KEEP.append(src)
+    pal, t = _compare(gpu, oracle, src, H.fmt_of(img), w, h, synth.widen_q16(img), colours,
+                      assoc, False)
+    assert t.depth < _initial_depth(colours, assoc)  # the reference did prune a level
+    assert pal.info()["exact_root"]
 
 
 def _random_mix(seed, n):

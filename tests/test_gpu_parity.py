@@ -116,6 +116,32 @@ def test_seq_small_memo_spills(gpu, oracle):
     assert np.array_equal(got, oracle.walk_seq(tree, f))
 
 
+@pytest.mark.parametrize("w,h,ch,colors,slice_visits", [
+    (320, 240, 3, 4096, 0), (256, 200, 4, 2048, 0), (320, 240, 3, 4096, 48 * 31)])
+def test_seq_large_palette(gpu, oracle, w, h, ch, colors, slice_visits):
+    """Palettes whose search image is above the walk's 96 KiB LDS share (more
+    than about 1400 colours) are read from HBM; every output is the oracle's,
+    in one launch and in resumable slices."""
+    import torch
+    img = np.random.default_rng(w + colors).integers(0, 256, (h, w, ch), dtype=np.uint8)
+    say("seq large palette", w, h, ch, colors, "slices", slice_visits)
+    tree, f, pal, spec, dst = _seq_case(gpu, oracle, img, colors)
+    assert pal.info()["count"] > 1400
+    if slice_visits:
+        job = gpu.WalkJob()
+        job.add([spec])
+        steps = 0
+        while job.active:
+            job.step(slice_visits)
+            steps += 1
+            assert steps < 100000
+    else:
+        gpu.walk_seq_batch([spec])
+    torch.cuda.synchronize()
+    got = H.from_device(dst, (h, w, 4), np.float32)
+    assert np.array_equal(got, oracle.walk_seq(tree, f))
+
+
 def test_seq_diffusion(gpu, oracle):
     rng = np.random.default_rng(4)
     img = synth.sinusoid_image(80, 60, rng, channels=3)
