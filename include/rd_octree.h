@@ -50,9 +50,8 @@ class ColorTree {
   ColorTree& operator=(ColorTree&&) noexcept;
 
   // rd_octree.cpp:219-409, on the device.  `pixels` is host memory (w*h RgbaF).
-  // The reference has no failure path; this one can fail (no device, a tree
-  // above MaxQNodes during classification): color_count() is then 0 and
-  // last_error() says why.
+  // The reference has no failure path; this one can fail (no device, device
+  // memory): color_count() is then 0 and last_error() says why.
   void Build(const RgbaF* pixels, std::size_t width, std::size_t height, int max_colors,
              bool associate_alpha, bool grayscale);
 

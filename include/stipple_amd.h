@@ -106,12 +106,14 @@ int sa_palette_count(sa_palette p);
  * (planar YUV included: the
  * tree of the frame as the reader widens it).  Synchronises `stream`
  * (the build reads back a few scalars).  With keep_tree the node arrays stay
- * on the device for sa_palette_tree.  Trees that would exceed MaxQNodes
- * (266817) during classification (quantize.c then prunes whole levels
- * mid-image) return SA_EUNSUPPORTED; the test is on each loop's node total,
- * so it also refuses a tree that crosses the limit only in its last row,
- * which the reference builds unpruned.  Only palettes of about a thousand
- * colours or more on busy images come near it. */
+ * on the device for sa_palette_tree.  A tree whose node total could pass
+ * MaxQNodes (266817) during classification -- palettes of about a thousand
+ * colours or more on busy images -- is classified again row by row, as
+ * quantize.c does: PruneLevel at the start of the row after the live count
+ * passed the limit, later rows one level shallower (src/rd_octree.cpp:366-369,
+ * :380-383).  Its every node, the pruned ones and the root included, is then
+ * the reference's exactly; such a build takes a few hundred milliseconds per
+ * megapixel. */
 int sa_palette_build(const void* src, int src_fmt, int64_t width, int64_t height, int colors,
                      int assoc, int gray, int keep_tree, void* stream, sa_palette* out,
                      char* err, size_t errlen);

@@ -287,14 +287,15 @@ size_t stage_bytes(int fmt, int64_t px) {
   return (((size_t)px << sa::seq_stage_shift(fmt)) + 16 + 255) & ~(size_t)255;
 }
 
-// The walk's LDS memo cache for one image: byte entries when the palette has
-// at most 255 colours (index+1 fits), and as many line slots as the LDS left
+// The walk's LDS memo cache for one image: entries of the fewest bytes that
+// hold index+1 (1 up to 255 colours, 2 up to 65535, 4 for 65536), and as many
+// line slots as the LDS left
 // beside the search image and the input ring holds (a multiple of 4, at most
 // 4096); `memo_bits` > 0 (or the seq.memo_bits knob, as a cap) asks for
 // 1 << memo_bits slots instead.
 struct MemoShape {
   int slots = 0;
-  int narrow = 0;
+  int eb = 2;         // bytes per entry
   size_t bytes = 0;   // LDS (and snapshot) bytes
   int key_bits = 0;   // the HBM table holds 1 << key_bits entries
   size_t table = 0;   // its bytes
@@ -783,8 +784,6 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       return fail(err, errlen, SA_EINVAL, "image %d: bad pixel format", i);
     if (im.dst == im.src && (im.src_fmt != im.dst_fmt))
       return fail(err, errlen, SA_EINVAL, "image %d: in-place needs the same format", i);
-    if (im.palette->count > 65535)
-      return fail(err, errlen, SA_EUNSUPPORTED, "image %d: the walk's memo holds indices below 65535", i);
   }
   const double* sw = nullptr;
   rc = get_weights(o.diffusion, s, &sw, err, errlen);
@@ -864,7 +863,7 @@ extern "C" int sa_walk_seq_batch(const sa_image* images, int n, const sa_seq_opt
       it.src_fmt = im.src_fmt;
       it.dst_fmt = im.dst_fmt;
       it.memo_slots = shape[k].slots;
-      it.memo_narrow = shape[k].narrow;
+      it.memo_eb = shape[k].eb;
     }
     sa::SeqItem* d_items = (sa::SeqItem*)scratch;
     int32_t* d_queue = (int32_t*)(scratch + items_bytes);
@@ -890,10 +889,10 @@ namespace {
 MemoShape memo_shape(const sa_seq_options& o, int assoc, int img_bytes, int count) {
   MemoShape m;
   if (!o.use_cache) return m;
-  m.narrow = count <= 255 ? 1 : 0;
+  m.eb = count <= 255 ? 1 : count <= 65535 ? 2 : 4;
   m.key_bits = assoc ? 24 : 18;
-  m.table = (size_t)(m.narrow ? 1 : 2) << m.key_bits;
-  const size_t per = m.narrow ? 68 : 132;
+  m.table = (size_t)m.eb << m.key_bits;
+  const size_t per = sa::seq_memo_bytes(1, m.eb);
   if (o.memo_bits > 0) {
     m.slots = 1 << o.memo_bits;
   } else {
@@ -905,7 +904,7 @@ MemoShape memo_shape(const sa_seq_options& o, int assoc, int img_bytes, int coun
     if (tuned > 0) m.slots = std::min(m.slots, 1 << tuned);
     m.slots = std::max(m.slots, 16);
   }
-  m.bytes = sa::seq_memo_bytes(m.slots, m.narrow);
+  m.bytes = sa::seq_memo_bytes(m.slots, m.eb);
   return m;
 }
 
@@ -1046,8 +1045,6 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
       return fail(err, errlen, SA_EINVAL, "image %d: bad pixel format", i);
     if (im.dst == im.src && (im.src_fmt != im.dst_fmt))
       return fail(err, errlen, SA_EINVAL, "image %d: in-place needs the same format", i);
-    if (im.palette->count > 65535)
-      return fail(err, errlen, SA_EUNSUPPORTED, "image %d: the walk's memo holds indices below 65535", i);
     const int img_lds = sa::seq_img_lds_bytes(im.palette->hdr.bytes);
     const MemoShape ms = memo_shape(o, im.palette->assoc, img_lds, im.palette->count);
     const size_t lds = ms.bytes + (size_t)img_lds + sa::kSeqRingBytes;
@@ -1102,7 +1099,7 @@ extern "C" int sa_walk_job_add(sa_walk_job job, const sa_image* images, int n, i
     it.src_fmt = im.src_fmt;
     it.dst_fmt = im.dst_fmt;
     it.memo_slots = ms.slots;
-    it.memo_narrow = ms.narrow;
+    it.memo_eb = ms.eb;
     max_n = std::max(max_n, px);
     dev[J.assoc].push_back(it);
     fresh.push_back(J);
@@ -1161,7 +1158,7 @@ int job_step(sa_walk_job job, int64_t slice, uint64_t budget, hipStream_t s, int
     size_t lds = 0;
     for (size_t k : sel) {
       const JobItem& J = job->items[k];
-      lds = std::max(lds, (o.use_cache ? sa::seq_memo_bytes(J.it.memo_slots, J.it.memo_narrow) : 0u) +
+      lds = std::max(lds, (o.use_cache ? sa::seq_memo_bytes(J.it.memo_slots, J.it.memo_eb) : 0u) +
                                  (size_t)J.img_bytes +
                                  sa::kSeqRingBytes);
     }
@@ -1329,7 +1326,7 @@ extern "C" int sa_walk_job_step_overlapped(sa_walk_job job, double budget_us, vo
       for (const JobItem& J : job->items) {
         if (J.assoc != assoc) continue;
         cand.push_back(&J);
-        lds = std::max(lds, (o.use_cache ? sa::seq_memo_bytes(J.it.memo_slots, J.it.memo_narrow) : 0u) +
+        lds = std::max(lds, (o.use_cache ? sa::seq_memo_bytes(J.it.memo_slots, J.it.memo_eb) : 0u) +
                                  (size_t)J.img_bytes +
                                  sa::kSeqRingBytes);
       }

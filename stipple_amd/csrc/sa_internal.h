@@ -185,17 +185,18 @@ struct SeqItem {
   void* stage;            // scratch: the source pixel of each visit, in visit order
                           // (1 << seq_stage_shift(src_fmt) bytes each, +16 B of padding)
   uint16_t* idx;          // scratch: palette index per visit
-  void* memo;             // scratch: 1<<K memo entries of 1 or 2 bytes (K = 18, or 24
+  void* memo;             // scratch: 1<<K memo entries of memo_eb bytes (K = 18, or 24
                           // with alpha)
   const PaletteDev* pal;  // device palette header
   SeqState* state;        // null: walk to the end in one launch
   unsigned char* snap;    // with state: the LDS memo cache between launches
-                          // (seq_memo_bytes(memo_slots, memo_narrow) bytes)
+                          // (seq_memo_bytes(memo_slots, memo_eb) bytes)
   int64_t n;              // visits == w*h
   int32_t src_fmt;
   int32_t dst_fmt;
   int32_t memo_slots;     // LDS memo line slots (a multiple of 4)
-  int32_t memo_narrow;    // 1: memo entries are bytes (palettes of at most 255 colours)
+  int32_t memo_eb;        // bytes per memo entry (index + 1): 1 for palettes of at most
+                          // 255 colours, 2 up to 65535, 4 for 65536
 };
 
 // Palette search images up to this size are staged in LDS for the sequential
@@ -207,8 +208,8 @@ __host__ __device__ inline int seq_img_lds_bytes(int bytes) {
 }
 
 // LDS bytes of the walk's memo cache: per slot a 4-byte tag and 64 entries.
-__host__ __device__ inline uint32_t seq_memo_bytes(int slots, int narrow) {
-  return (uint32_t)slots * (4u + (narrow ? 64u : 128u));
+__host__ __device__ inline uint32_t seq_memo_bytes(int slots, int eb) {
+  return (uint32_t)slots * (4u + 64u * (uint32_t)eb);
 }
 
 // One frame of the block engine.

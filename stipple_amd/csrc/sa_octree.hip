@@ -41,9 +41,10 @@
 // chain is summed in parallel, its rounding error bounded, and the exact serial
 // chain is run only when the bound cannot separate it (tiny images).
 //
-// Not handled here (the build reports SA_EUNSUPPORTED): trees whose live node
-// count would exceed MaxQNodes (266817) during classification, where the
-// reference starts pruning levels mid-image (rd_octree.cpp:366-369, 380-383).
+// Trees whose live node count could pass MaxQNodes (266817) during
+// classification, where the reference starts pruning levels mid-image
+// (rd_octree.cpp:366-369, 380-383), are classified again row by row (section
+// 6, k_rows); the tree logic and the colormap are then the same kernels'.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>

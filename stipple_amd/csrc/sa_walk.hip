@@ -171,7 +171,7 @@ constexpr int kRowShl1 = 0x101;  // lane j <- lane j+1; lane 15 has no source
 
 // LDS cache of the HBM memo table's 64-entry lines.  Entries are index+1
 // (0 = untouched), bytes when the palette has at most 255 colours (twice the
-// lines in the same LDS), else 16-bit.
+// lines in the same LDS), 16-bit up to 65535 colours, 32-bit for 65536.
 struct Memo {
   uint32_t* tag;    // per slot: bit 31 empty, bit 30 dirty, bits 0..29 line id
   void* line;       // 64 entries per slot
@@ -201,14 +201,15 @@ __device__ __forceinline__ uint32_t memo_slot(const Memo& m, uint32_t line) {
   return __umulhi(line * 0x9E3779B1u, m.slots);
 }
 
-template <bool NARROW>
-using MemoEntry = typename std::conditional<NARROW, uint8_t, uint16_t>::type;
+template <int EB>
+using MemoEntry = typename std::conditional<
+    EB == 1, uint8_t, typename std::conditional<EB == 2, uint16_t, uint32_t>::type>::type;
 
 // Makes `line` resident in `slot` (uniform).  Returns 1 when it read HBM.
-template <bool NARROW>
+template <int EB>
 __device__ __forceinline__ uint32_t memo_fill(const Memo& m, uint32_t slot, uint32_t line,
                                               uint32_t old) {
-  typedef MemoEntry<NARROW> E;
+  typedef MemoEntry<EB> E;
   typedef E __attribute__((address_space(1))) gE;
   const int lane = threadIdx.x;
   E* dst = (E*)m.line + 64 * slot;
@@ -284,7 +285,7 @@ struct Visit {
 };
 
 // One visit (visit t); on return lane j holds visit t+1+j.
-template <bool ASSOC, bool CACHE, bool NARROW>
+template <bool ASSOC, bool CACHE, int EB>
 __device__ __forceinline__ void visit(Visit& w, double W, int64_t t, double* ring, const Memo& m,
                                       const Search& s, Counters& cnt) {
   const int lane = threadIdx.x;
@@ -323,7 +324,7 @@ __device__ __forceinline__ void visit(Visit& w, double W, int64_t t, double* rin
     if (ASSOC) key |= (uint32_t)rl((int)kc, 48);
     const uint32_t line = key >> 6;
     const uint32_t slot = memo_slot(m, line);
-    typedef MemoEntry<NARROW> E;
+    typedef MemoEntry<EB> E;
     E* const entp = (E*)m.line + 64 * slot + (key & 63);
     const uint32_t tgv = m.tag[slot];
     const uint32_t entv = *entp;
@@ -334,14 +335,14 @@ __device__ __forceinline__ void visit(Visit& w, double W, int64_t t, double* rin
     target();
     __builtin_amdgcn_sched_barrier(0);
     const uint32_t tg = (uint32_t)rfl((int)tgv);
-    uint32_t ent = (uint32_t)rfl((int)entv) & 0xFFFFu;
+    uint32_t ent = (uint32_t)rfl((int)entv);
     idx = (int)ent - 1;
     // The hit (line resident, entry touched) falls through: (ent - 1) >> 16
-    // is nonzero only for ent == 0.
+    // is nonzero only for ent == 0 (entries are at most 65536).
     if (__builtin_expect((((tg & ~kDirty) ^ line) | ((ent - 1u) >> 16)) != 0, 0)) {
       if ((tg & ~kDirty) != line) {
         ++cnt.fills;
-        cnt.loads += memo_fill<NARROW>(m, slot, line, tg);
+        cnt.loads += memo_fill<EB>(m, slot, line, tg);
         ent = (uint32_t)rfl((int)*entp);
         idx = (int)ent - 1;
       }
@@ -386,7 +387,7 @@ __device__ __forceinline__ void flush_window(uint16_t* idx_out, uint32_t obuf, i
 // Inputs: at the start of each 16-visit chunk the next chunk's staged pixels
 // (loaded one chunk earlier) are associated and written to the ring, and the
 // chunk after that is loaded; each visit reads its x_{t+16} from the ring.
-template <bool ASSOC, bool CACHE, bool NARROW>
+template <bool ASSOC, bool CACHE, int EB, bool IMG_LDS>
 __device__ int64_t walk_image(const SeqItem& it, double W, const Memo& m, const Search& s,
                               double* ring_base, Counters& cnt, int64_t t0, int64_t stop,
                               uint64_t deadline) {
@@ -440,9 +441,9 @@ __device__ int64_t walk_image(const SeqItem& it, double W, const Memo& m, const 
       // (240 against 248 ns per visit on the largest still; a full unroll
       // outgrows the instruction cache and is slower again).
 #pragma unroll 4
-      for (int k = 0; k < 16; ++k) visit<ASSOC, CACHE, NARROW>(w, W, base + k, ring, m, s, cnt);
+      for (int k = 0; k < 16; ++k) visit<ASSOC, CACHE, EB>(w, W, base + k, ring, m, s, cnt);
     } else {
-      for (int k = 0; k < len; ++k) visit<ASSOC, CACHE, NARROW>(w, W, base + k, ring, m, s, cnt);
+      for (int k = 0; k < len; ++k) visit<ASSOC, CACHE, EB>(w, W, base + k, ring, m, s, cnt);
     }
     const int64_t last = base + len - 1;
     if ((last & 63) == 63 || last + 1 == n) flush_window(it.idx, w.obuf, last);
@@ -457,6 +458,19 @@ __device__ int64_t walk_image(const SeqItem& it, double W, const Memo& m, const 
     it.state->e[lane] = w.eb;
   }
   return stop;
+}
+
+// The walk of an image whose search image stays in HBM, out of line so the
+// common case's code stays as compact as before.
+template <bool ASSOC, bool CACHE>
+__device__ __attribute__((noinline)) int64_t walk_image_hbm(const SeqItem& it, double W,
+                                                            const Memo& m, const Search& s,
+                                                            double* ring, Counters& cnt,
+                                                            int64_t t0, int64_t stop,
+                                                            uint64_t deadline, int eb) {
+  if (CACHE && eb == 4)
+    return walk_image<ASSOC, CACHE, 4, false>(it, W, m, s, ring, cnt, t0, stop, deadline);
+  return walk_image<ASSOC, CACHE, 2, false>(it, W, m, s, ring, cnt, t0, stop, deadline);
 }
 
 // Wavefront copy of `words` 16-byte words with eight loads in flight per
@@ -500,20 +514,20 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
     int64_t stop = (slice > 0 && it.state && t0 + slice < it.n) ? t0 + slice : it.n;
     Memo m;
     const uint32_t slots = (uint32_t)rfl(it.memo_slots);
-    const bool narrow = rfl(it.memo_narrow) != 0;
+    const int eb = rfl(it.memo_eb);
     m.slots = slots;
     memo_key_consts<ASSOC>(m, lane >> 4);
-    const uint32_t memo_bytes = CACHE ? seq_memo_bytes((int)slots, narrow) : 0u;
+    const uint32_t memo_bytes = CACHE ? seq_memo_bytes((int)slots, eb) : 0u;
     m.tag = (uint32_t*)lds;
     m.line = lds + 4 * slots;
     const PaletteDev pd = *it.pal;
     // The search image in LDS, or read where it lies when it is too large
-    // (the memo cache then has the LDS).
+    // (the memo cache then has the LDS).  Each case is its own instantiation,
+    // so the LDS one keeps LDS reads (a pointer that may be either would make
+    // every search read a flat access).
     const bool img_lds = pd.bytes <= kSeqImageLdsMax;
-    unsigned char* img = img_lds ? lds + memo_bytes : (unsigned char*)pd.image;
-    double* ring = (double*)(lds + memo_bytes + (img_lds ? pd.bytes : 0));
     {
-      if (img_lds) copy16((uint4*)img, (const uint4*)pd.image, (uint32_t)pd.bytes / 16);
+      if (img_lds) copy16((uint4*)(lds + memo_bytes), (const uint4*)pd.image, (uint32_t)pd.bytes / 16);
       if (CACHE) {
         if (t0 == 0) {
           for (uint32_t k = lane; k < slots; k += kWave) m.tag[k] = kEmptyTag;
@@ -523,21 +537,32 @@ __global__ __launch_bounds__(kWave) void seq_walk_kernel(const SeqItem* __restri
       }
       __syncthreads();
     }
-    Search s;
-    s.col = (const double*)img;
-    s.cent = (const float4*)(img + pd.off_cent);
-    s.sub = (const int2*)(img + pd.off_sub);
-    s.desc = (const uint32_t*)(img + pd.off_desc);
-    s.child = (const int16_t*)(img + pd.off_child);
-    s.cand = (const uint16_t*)(img + pd.off_cand);
-    s.kids = pd.kids;
+    auto search_at = [&](const unsigned char* img) {
+      Search s;
+      s.col = (const double*)img;
+      s.cent = (const float4*)(img + pd.off_cent);
+      s.sub = (const int2*)(img + pd.off_sub);
+      s.desc = (const uint32_t*)(img + pd.off_desc);
+      s.child = (const int16_t*)(img + pd.off_child);
+      s.cand = (const uint16_t*)(img + pd.off_cand);
+      s.kids = pd.kids;
+      return s;
+    };
     m.table = it.memo;
     const uint64_t deadline =
         (budget && it.state) ? __builtin_amdgcn_s_memrealtime() + budget : 0;
-    if (CACHE && narrow)
-      stop = walk_image<ASSOC, CACHE, true>(it, W, m, s, ring, cnt, t0, stop, deadline);
-    else
-      stop = walk_image<ASSOC, CACHE, false>(it, W, m, s, ring, cnt, t0, stop, deadline);
+    if (img_lds) {
+      const Search s = search_at(lds + memo_bytes);
+      double* ring = (double*)(lds + memo_bytes + pd.bytes);
+      if (CACHE && eb == 1)
+        stop = walk_image<ASSOC, CACHE, 1, true>(it, W, m, s, ring, cnt, t0, stop, deadline);
+      else
+        stop = walk_image<ASSOC, CACHE, 2, true>(it, W, m, s, ring, cnt, t0, stop, deadline);
+    } else {  // more than about 1400 colours: 2- or 4-byte entries
+      const Search s = search_at((const unsigned char*)pd.image);
+      stop = walk_image_hbm<ASSOC, CACHE>(it, W, m, s, (double*)(lds + memo_bytes), cnt, t0,
+                                          stop, deadline, eb);
+    }
     __syncthreads();
     if (it.state) {
       if (CACHE && stop < it.n) copy16((uint4*)it.snap, (const uint4*)lds, memo_bytes / 16);
@@ -567,10 +592,10 @@ __global__ void seq_state_init_kernel(const SeqItem* __restrict__ items, int nit
 // Zeroes each image's HBM memo table (a fill kernel, so the sequence can be
 // captured in a graph).
 __global__ __launch_bounds__(256) void memo_clear_kernel(const SeqItem* __restrict__ items,
-                                                         int64_t words) {
+                                                         int key_bits) {
   const SeqItem it = items[blockIdx.y];
   uint4* t = (uint4*)it.memo;
-  const int64_t w = it.memo_narrow ? words / 2 : words;
+  const int64_t w = ((int64_t)it.memo_eb << key_bits) / 16;
   for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < w; k += (int64_t)gridDim.x * 256)
     t[k] = make_uint4(0, 0, 0, 0);
 }
@@ -685,8 +710,7 @@ hipError_t launch_seq_state_init(const SeqItem* items, int nitems, hipStream_t s
 
 hipError_t launch_memo_clear(const SeqItem* items, int nitems, int key_bits, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  const int64_t words = ((int64_t)2 << key_bits) / 16;
-  memo_clear_kernel<<<dim3(64, (unsigned)nitems), 256, 0, s>>>(items, words);
+  memo_clear_kernel<<<dim3(64, (unsigned)nitems), 256, 0, s>>>(items, key_bits);
   return hipGetLastError();
 }
 

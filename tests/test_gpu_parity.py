@@ -117,16 +117,20 @@ def test_seq_small_memo_spills(gpu, oracle):
 
 
 @pytest.mark.parametrize("w,h,ch,colors,slice_visits", [
-    (320, 240, 3, 4096, 0), (256, 200, 4, 2048, 0), (320, 240, 3, 4096, 48 * 31)])
+    (320, 240, 3, 4096, 0), (256, 200, 4, 2048, 0), (320, 240, 3, 4096, 48 * 31),
+    (384, 384, 3, 65536, 0), (384, 384, 3, 65536, 48 * 211)])
 def test_seq_large_palette(gpu, oracle, w, h, ch, colors, slice_visits):
     """Palettes whose search image is above the walk's 96 KiB LDS share (more
-    than about 1400 colours) are read from HBM; every output is the oracle's,
-    in one launch and in resumable slices."""
+    than about 1400 colours) are read from HBM, and a full 65536-colour palette
+    keeps index+1 in 32-bit memo entries; every output is the oracle's, in one
+    launch and in resumable slices."""
     import torch
     img = np.random.default_rng(w + colors).integers(0, 256, (h, w, ch), dtype=np.uint8)
     say("seq large palette", w, h, ch, colors, "slices", slice_visits)
     tree, f, pal, spec, dst = _seq_case(gpu, oracle, img, colors)
     assert pal.info()["count"] > 1400
+    if colors == 65536:
+        assert pal.info()["count"] == 65536
     if slice_visits:
         job = gpu.WalkJob()
         job.add([spec])
