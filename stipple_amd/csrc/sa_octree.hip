@@ -2189,7 +2189,11 @@ struct RowsArgs {
   TreeWork W;
   int64_t cap;               // node capacity of t and W
   int32_t* at;               // node of each run of the chunk at each level, 9 x kRowChunk
-  double* rv;                // running value of each (level, run) pair, 8 x kRowChunk
+  double* rv;                // running value of each (run, level) pair, kRowChunk x 8
+  double* lt;                // leaf terms of total_color of each run, kRowChunk x 4
+  double* tmg;               // quantize_error term of each (level, run), 8 x kRowChunk
+  uint64_t* skey;            // each level's runs sorted by (node, run), 8 x kRowChunk
+  int32_t* seg;              // chain starts: level << 16 | sorted position
   int32_t* sp;               // staged nodes: parent, child index | level << 8, creating run
   int32_t* sp_child;         // 16 per staged node
   int32_t* fid;              // node id of each staged node
@@ -2340,16 +2344,26 @@ __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw,
   live += ns;
   colors += ncol;
   // -- sums ---------------------------------------------------------------------------
+  // Level by level, every run's term is formed and the runs are sorted by node
+  // (then by run) in LDS; then the chains of every level are folded at once --
+  // the levels' nodes are distinct, so the longest chain sets the time, not the
+  // sum of the levels' longest.
   uint64_t* keys = (uint64_t*)raw;                      // kRowChunk
-  double* tm = (double*)(raw + 8 * kRowChunk);          // kRowChunk
-  int n2 = 1;
+  int32_t* nseg = (int32_t*)&S.scal[2];
+  int n2 = 4;  // at least the chain's padding to whole groups of four runs
   while (n2 < n) n2 <<= 1;
+  if (tid == 0) *nseg = 0;
   for (int l = 1; l <= limit; ++l) {
+    double* tm = a.tmg + (l - 1) * kRowChunk;
     for (int i = tid; i < n2; i += kTreeThreads) {
       if (i >= n) {
         keys[i] = ~0ull;
+        if (l == 1 && i < ((n + 3) & ~3))  // runs padding the chain to whole groups of four
+          for (int z = 0; z < kRowLevels; ++z) a.rv[(int64_t)kRowLevels * i + z] = 0.0;
         continue;
       }
+      if (l == 1)  // the slots of levels past the leaf add +0.0 to the root's chain
+        for (int z = limit; z < kRowLevels; ++z) a.rv[(int64_t)kRowLevels * i + z] = 0.0;
       const int32_t node = a.at[l * kRowChunk + i];
       double av[4], mid[4];
       rows_pixel(a, c0 + i, av);
@@ -2360,8 +2374,16 @@ __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw,
       const double e3 = assoc ? kScale * (av[3] - mid[3]) : 0.0;
       double dd = e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
       if (dd != dd) dd = 0.0;
-      tm[i] = (double)a.count[c0 + i] * sqrt(dd);
+      const int32_t cnt = a.count[c0 + i];
+      tm[i] = (double)cnt * sqrt(dd);
       keys[i] = ((uint64_t)(uint32_t)node << 32) | (uint32_t)i;
+      if (l == limit) {  // this run's leaf terms, folded below in run order
+        double* lt = a.lt + 4 * i;
+        lt[0] = tot_term(cnt, clamp_pixel(av[0]));
+        lt[1] = tot_term(cnt, clamp_pixel(av[1]));
+        lt[2] = tot_term(cnt, clamp_pixel(av[2]));
+        lt[3] = tot_term(cnt, clamp_pixel(assoc ? av[3] : kRange));
+      }
     }
     __syncthreads();
     for (int k = 2; k <= n2; k <<= 1)  // bitonic sort, ascending (node, run)
@@ -2378,46 +2400,88 @@ __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw,
         }
         __syncthreads();
       }
-    const bool leaf = l == limit;
+    uint64_t* sk = a.skey + (l - 1) * kRowChunk;
     for (int k = tid; k < n; k += kTreeThreads) {
-      const uint32_t node = (uint32_t)(keys[k] >> 32);
-      if (k > 0 && (uint32_t)(keys[k - 1] >> 32) == node) continue;
-      double v = t.qe[node];
-      double tot[4];
-      uint64_t nu = 0;
-      if (leaf) {
-        for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * (int64_t)node + c];
-        nu = t.nu[node];
-      }
-      for (int j = k; j < n && (uint32_t)(keys[j] >> 32) == node; ++j) {
-        const int i = (int)(uint32_t)keys[j];
-        v += tm[i];
-        a.rv[(l - 1) * kRowChunk + i] = v;
-        if (leaf) {
-          double av[4];
-          rows_pixel(a, c0 + i, av);
-          const int32_t cnt = a.count[c0 + i];
-          tot[0] += tot_term(cnt, clamp_pixel(av[0]));
-          tot[1] += tot_term(cnt, clamp_pixel(av[1]));
-          tot[2] += tot_term(cnt, clamp_pixel(av[2]));
-          tot[3] += tot_term(cnt, clamp_pixel(assoc ? av[3] : kRange));
-          nu += (uint64_t)cnt;
-        }
-      }
-      t.qe[node] = v;
-      if (leaf) {
-        for (int c = 0; c < 4; ++c) t.total[4 * (int64_t)node + c] = tot[c];
-        t.nu[node] = nu;
-      }
+      sk[k] = keys[k];
+      if (k == 0 || (keys[k - 1] >> 32) != (keys[k] >> 32))
+        a.seg[atomicAdd(nseg, 1)] = (l << 16) | k;  // a chain starts here
     }
     __syncthreads();
   }
-  // The root's chain: run by run, level by level (rd_octree.cpp:341).
+  const int segs = *nseg;
+  for (int e = tid; e < segs; e += kTreeThreads) {
+    const int l = a.seg[e] >> 16, k = a.seg[e] & 0xFFFF;
+    const uint64_t* sk = a.skey + (l - 1) * kRowChunk;
+    const double* tm = a.tmg + (l - 1) * kRowChunk;
+    const bool leaf = l == limit;
+    const uint32_t node = (uint32_t)(sk[k] >> 32);
+    double v = t.qe[node];
+    double tot[4];
+    uint64_t nu = 0;
+    if (leaf) {
+      for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * (int64_t)node + c];
+      nu = t.nu[node];
+    }
+    // The chain in batches of four: the batch's terms are loaded before the
+    // in-order additions, so the loads overlap.
+    for (int j = k; j < n && (uint32_t)(sk[j] >> 32) == node;) {
+      constexpr int kB = 4;
+      int ii[kB];
+      int m = 0;
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const bool in = m == q && j + q < n && (uint32_t)(sk[j + q] >> 32) == node;
+        ii[q] = in ? (int)(uint32_t)sk[j + q] : 0;
+        m += in ? 1 : 0;
+      }
+      double tv[kB], lv[kB][4];
+      int32_t cv[kB];
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        tv[q] = tm[ii[q]];
+        if (leaf) {
+          const double* lt = a.lt + 4 * ii[q];
+          for (int c = 0; c < 4; ++c) lv[q][c] = lt[c];
+          cv[q] = a.count[c0 + ii[q]];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        if (q >= m) break;
+        v += tv[q];
+        a.rv[(int64_t)kRowLevels * ii[q] + (l - 1)] = v;
+        if (leaf) {
+          for (int c = 0; c < 4; ++c) tot[c] += lv[q][c];
+          nu += (uint64_t)cv[q];
+        }
+      }
+      j += m;
+    }
+    t.qe[node] = v;
+    if (leaf) {
+      for (int c = 0; c < 4; ++c) t.total[4 * (int64_t)node + c] = tot[c];
+      t.nu[node] = nu;
+    }
+  }
+  __syncthreads();
+  // The root's chain: run by run, level by level (rd_octree.cpp:341).  Each
+  // run's 8 slots hold its levels' running values and +0.0 past the leaf (the
+  // chain only grows from +0.0, so adding +0.0 changes nothing); one thread adds
+  // them, four runs' slots loaded ahead of their additions.
   double* sroot = (double*)&S.scal[1];
   if (tid == 0) {
     double q = root_qe;
-    for (int i = 0; i < n; ++i)
-      for (int l = 1; l <= limit; ++l) q += a.rv[(l - 1) * kRowChunk + i];
+    const double2* rv2 = (const double2*)a.rv;
+    for (int i = 0; i < n; i += 4) {
+      double2 r[4 * kRowLevels / 2];
+#pragma unroll
+      for (int k = 0; k < 4 * kRowLevels / 2; ++k) r[k] = rv2[(int64_t)i * (kRowLevels / 2) + k];
+#pragma unroll
+      for (int k = 0; k < 4 * kRowLevels / 2; ++k) {
+        q += r[k].x;
+        q += r[k].y;
+      }
+    }
     *sroot = q;
   }
   __syncthreads();
@@ -3249,6 +3313,10 @@ class SubBuild {
     c.want(&row_first, h + 1);
     c.want(&a.at, (int64_t)(kRowLevels + 1) * kRowChunk);
     c.want(&a.rv, (int64_t)kRowLevels * kRowChunk);
+    c.want(&a.lt, 4 * (int64_t)kRowChunk);
+    c.want(&a.tmg, (int64_t)kRowLevels * kRowChunk);
+    c.want(&a.skey, (int64_t)kRowLevels * kRowChunk);
+    c.want(&a.seg, (int64_t)kRowLevels * kRowChunk);
     c.want(&a.sp, 3 * (int64_t)kStageMax);
     c.want(&a.sp_child, 16 * (int64_t)kStageMax);
     c.want(&a.fid, (int64_t)kStageMax);
