@@ -46,6 +46,7 @@
 // (rd_octree.cpp:366-369, 380-383), are classified again row by row (section
 // 6, k_rows); the tree logic and the colormap are then the same kernels'.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/block/block_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstdarg>
@@ -2189,11 +2190,16 @@ struct RowsArgs {
   TreeWork W;
   int64_t cap;               // node capacity of t and W
   int32_t* at;               // node of each run of the chunk at each level, 9 x kRowChunk
-  double* rv;                // running value of each (run, level) pair, kRowChunk x 8
+  double* rv;                // running value of each (run, level) pair, run-major, limit
+                             // per run (the root's chain order)
   double* lt;                // leaf terms of total_color of each run, kRowChunk x 4
   double* tmg;               // quantize_error term of each (level, run), 8 x kRowChunk
   uint64_t* skey;            // each level's runs sorted by (node, run), 8 x kRowChunk
   int32_t* seg;              // chain starts: level << 16 | sorted position
+  double* ts;                // the terms in sorted order, 8 x kRowChunk
+  double* rvs;               // running values in sorted order, 8 x kRowChunk
+  double* lts;               // leaf terms in sorted order, kRowChunk x 4
+  int32_t* scnt;             // run lengths in sorted order (leaf level)
   int32_t* sp;               // staged nodes: parent, child index | level << 8, creating run
   int32_t* sp_child;         // 16 per staged node
   int32_t* fid;              // node id of each staged node
@@ -2215,6 +2221,73 @@ __device__ __forceinline__ void rows_pixel(const RowsArgs& a, int64_t r, double 
 
 // One chunk of a row: runs [c0, c0 + n), levels 1..limit.  Everything uniform
 // is passed by reference and updated identically in every thread.
+// One node's chain of a level, from sorted position k: the terms in run order
+// onto the node's value from the rows before, B at a time (a batch's loads
+// overlap; the additions stay in order).  Leaf levels also fold the totals.
+template <bool LEAF, int B>
+struct FoldBatch {
+  uint32_t nd[B];
+  double tv[B];
+  double lv[LEAF ? B : 1][4];
+  int32_t cv[LEAF ? B : 1];
+};
+
+template <bool LEAF, int B>
+__device__ __forceinline__ void rows_load(const RowsArgs& a, const uint64_t* sk, const double* ts,
+                                          int j, int n, FoldBatch<LEAF, B>& f) {
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const bool in = j + q < n;
+    f.nd[q] = in ? (uint32_t)(sk[j + q] >> 32) : ~0u;
+    f.tv[q] = in ? ts[j + q] : 0.0;
+    if (LEAF) {
+      for (int c = 0; c < 4; ++c) f.lv[q][c] = in ? a.lts[4 * (j + q) + c] : 0.0;
+      f.cv[q] = in ? a.scnt[j + q] : 0;
+    }
+  }
+}
+
+template <bool LEAF, int B>
+__device__ __forceinline__ void rows_fold(const RowsArgs& a, int l, int k, int n) {
+  const TreeDev& t = a.t;
+  const uint64_t* sk = a.skey + (l - 1) * kRowChunk;
+  const double* ts = a.ts + (l - 1) * kRowChunk;
+  double* rvs = a.rvs + (l - 1) * kRowChunk;
+  const uint32_t node = (uint32_t)(sk[k] >> 32);
+  double v = t.qe[node];
+  double tot[4] = {0.0, 0.0, 0.0, 0.0};
+  uint64_t nu = 0;
+  if (LEAF) {
+    for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * (int64_t)node + c];
+    nu = t.nu[node];
+  }
+  FoldBatch<LEAF, B> cur;
+  rows_load<LEAF, B>(a, sk, ts, k, n, cur);
+  for (int j = k;; j += B) {
+    bool end = false;
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      if (end || cur.nd[q] != node) {
+        end = true;
+        continue;
+      }
+      v += cur.tv[q];
+      rvs[j + q] = v;
+      if (LEAF) {
+        for (int c = 0; c < 4; ++c) tot[c] += cur.lv[q][c];
+        nu += (uint64_t)cur.cv[q];
+      }
+    }
+    if (end) break;
+    rows_load<LEAF, B>(a, sk, ts, j + B, n, cur);
+  }
+  t.qe[node] = v;
+  if (LEAF) {
+    for (int c = 0; c < 4; ++c) t.total[4 * (int64_t)node + c] = tot[c];
+    t.nu[node] = nu;
+  }
+}
+
 __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw, int64_t c0,
                            int n, int limit, int colors_level, int64_t& M, int64_t& live,
                            int64_t& colors, double& root_qe) {
@@ -2345,25 +2418,31 @@ __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw,
   colors += ncol;
   // -- sums ---------------------------------------------------------------------------
   // Level by level, every run's term is formed and the runs are sorted by node
-  // (then by run) in LDS; then the chains of every level are folded at once --
-  // the levels' nodes are distinct, so the longest chain sets the time, not the
-  // sum of the levels' longest.
-  uint64_t* keys = (uint64_t*)raw;                      // kRowChunk
+  // -- a stable block radix sort, so each node's runs stay in run order -- into
+  // global arrays in sorted order; then the chains of every level are folded
+  // at once (the levels' nodes are distinct, so the longest chain sets the
+  // time, not the sum of the levels' longest), and the running values go back
+  // to run-major order for the root's chain.
+  using RowSort = rocprim::block_radix_sort<uint32_t, kTreeThreads, kRowChunk / kTreeThreads,
+                                            uint32_t>;
+  static_assert(sizeof(typename RowSort::storage_type) <= 4 * kRowChunk + 16 * kRowHash,
+                "the sort's storage fits the chunk's LDS");
+  typename RowSort::storage_type& sort_st = *reinterpret_cast<typename RowSort::storage_type*>(raw);
   int32_t* nseg = (int32_t*)&S.scal[2];
-  int n2 = 4;  // at least the chain's padding to whole groups of four runs
-  while (n2 < n) n2 <<= 1;
+  const int end_bit = 64 - __builtin_clzll((unsigned long long)(M + 2));  // every id + padding
+  const uint32_t pad = (uint32_t)(((uint64_t)1 << end_bit) - 1);          // above every id
   if (tid == 0) *nseg = 0;
   for (int l = 1; l <= limit; ++l) {
     double* tm = a.tmg + (l - 1) * kRowChunk;
-    for (int i = tid; i < n2; i += kTreeThreads) {
+    constexpr int kPer2 = kRowChunk / kTreeThreads;
+    uint32_t sk[kPer2], sv[kPer2];
+    for (int q = 0; q < kPer2; ++q) {  // blocked: thread t holds runs kPer2 * t + q
+      const int i = kPer2 * tid + q;
+      sv[q] = (uint32_t)i;
       if (i >= n) {
-        keys[i] = ~0ull;
-        if (l == 1 && i < ((n + 3) & ~3))  // runs padding the chain to whole groups of four
-          for (int z = 0; z < kRowLevels; ++z) a.rv[(int64_t)kRowLevels * i + z] = 0.0;
+        sk[q] = pad;
         continue;
       }
-      if (l == 1)  // the slots of levels past the leaf add +0.0 to the root's chain
-        for (int z = limit; z < kRowLevels; ++z) a.rv[(int64_t)kRowLevels * i + z] = 0.0;
       const int32_t node = a.at[l * kRowChunk + i];
       double av[4], mid[4];
       rows_pixel(a, c0 + i, av);
@@ -2376,7 +2455,7 @@ __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw,
       if (dd != dd) dd = 0.0;
       const int32_t cnt = a.count[c0 + i];
       tm[i] = (double)cnt * sqrt(dd);
-      keys[i] = ((uint64_t)(uint32_t)node << 32) | (uint32_t)i;
+      sk[q] = (uint32_t)node;
       if (l == limit) {  // this run's leaf terms, folded below in run order
         double* lt = a.lt + 4 * i;
         lt[0] = tot_term(cnt, clamp_pixel(av[0]));
@@ -2385,103 +2464,63 @@ __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw,
         lt[3] = tot_term(cnt, clamp_pixel(assoc ? av[3] : kRange));
       }
     }
+    __syncthreads();  // the terms are written; the sort reuses the chunk's LDS
+    RowSort().sort(sk, sv, sort_st, 0, (unsigned)end_bit);
     __syncthreads();
-    for (int k = 2; k <= n2; k <<= 1)  // bitonic sort, ascending (node, run)
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = tid; i < n2; i += kTreeThreads) {
-          const int x = i ^ j;
-          if (x > i) {
-            const uint64_t u = keys[i], v = keys[x];
-            if ((u > v) == ((i & k) == 0)) {
-              keys[i] = v;
-              keys[x] = u;
-            }
-          }
-        }
-        __syncthreads();
+    uint32_t* lnode = (uint32_t*)raw;  // the sorted nodes, to find where chains start
+    for (int q = 0; q < kPer2; ++q) lnode[kPer2 * tid + q] = sk[q];
+    __syncthreads();
+    uint64_t* skl = a.skey + (l - 1) * kRowChunk;
+    double* ts = a.ts + (l - 1) * kRowChunk;
+    for (int q = 0; q < kPer2; ++q) {
+      const int k = kPer2 * tid + q;
+      if (k >= n) continue;
+      const int i = (int)sv[q];
+      skl[k] = ((uint64_t)sk[q] << 32) | (uint32_t)i;
+      ts[k] = tm[i];
+      if (l == limit) {
+        for (int c = 0; c < 4; ++c) a.lts[4 * k + c] = a.lt[4 * i + c];
+        a.scnt[k] = a.count[c0 + i];
       }
-    uint64_t* sk = a.skey + (l - 1) * kRowChunk;
-    for (int k = tid; k < n; k += kTreeThreads) {
-      sk[k] = keys[k];
-      if (k == 0 || (keys[k - 1] >> 32) != (keys[k] >> 32))
-        a.seg[atomicAdd(nseg, 1)] = (l << 16) | k;  // a chain starts here
+      if (k == 0 || lnode[k - 1] != sk[q]) a.seg[atomicAdd(nseg, 1)] = (l << 16) | k;
     }
     __syncthreads();
   }
   const int segs = *nseg;
   for (int e = tid; e < segs; e += kTreeThreads) {
     const int l = a.seg[e] >> 16, k = a.seg[e] & 0xFFFF;
-    const uint64_t* sk = a.skey + (l - 1) * kRowChunk;
-    const double* tm = a.tmg + (l - 1) * kRowChunk;
-    const bool leaf = l == limit;
-    const uint32_t node = (uint32_t)(sk[k] >> 32);
-    double v = t.qe[node];
-    double tot[4];
-    uint64_t nu = 0;
-    if (leaf) {
-      for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * (int64_t)node + c];
-      nu = t.nu[node];
-    }
-    // The chain in batches of four: the batch's terms are loaded before the
-    // in-order additions, so the loads overlap.
-    for (int j = k; j < n && (uint32_t)(sk[j] >> 32) == node;) {
-      constexpr int kB = 4;
-      int ii[kB];
-      int m = 0;
-#pragma unroll
-      for (int q = 0; q < kB; ++q) {
-        const bool in = m == q && j + q < n && (uint32_t)(sk[j + q] >> 32) == node;
-        ii[q] = in ? (int)(uint32_t)sk[j + q] : 0;
-        m += in ? 1 : 0;
-      }
-      double tv[kB], lv[kB][4];
-      int32_t cv[kB];
-#pragma unroll
-      for (int q = 0; q < kB; ++q) {
-        tv[q] = tm[ii[q]];
-        if (leaf) {
-          const double* lt = a.lt + 4 * ii[q];
-          for (int c = 0; c < 4; ++c) lv[q][c] = lt[c];
-          cv[q] = a.count[c0 + ii[q]];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kB; ++q) {
-        if (q >= m) break;
-        v += tv[q];
-        a.rv[(int64_t)kRowLevels * ii[q] + (l - 1)] = v;
-        if (leaf) {
-          for (int c = 0; c < 4; ++c) tot[c] += lv[q][c];
-          nu += (uint64_t)cv[q];
-        }
-      }
-      j += m;
-    }
-    t.qe[node] = v;
-    if (leaf) {
-      for (int c = 0; c < 4; ++c) t.total[4 * (int64_t)node + c] = tot[c];
-      t.nu[node] = nu;
-    }
+    if (l == limit)
+      rows_fold<true, 4>(a, l, k, n);
+    else
+      rows_fold<false, 8>(a, l, k, n);
   }
   __syncthreads();
-  // The root's chain: run by run, level by level (rd_octree.cpp:341).  Each
-  // run's 8 slots hold its levels' running values and +0.0 past the leaf (the
-  // chain only grows from +0.0, so adding +0.0 changes nothing); one thread adds
-  // them, four runs' slots loaded ahead of their additions.
+  for (int e = tid; e < limit * n; e += kTreeThreads) {  // running values, run-major
+    const int l = 1 + e / n, k = e % n;
+    const int i = (int)(uint32_t)a.skey[(l - 1) * kRowChunk + k];
+    a.rv[(int64_t)limit * i + (l - 1)] = a.rvs[(l - 1) * kRowChunk + k];
+  }
+  __syncthreads();
+  // The root's chain: run by run, level by level (rd_octree.cpp:341): the
+  // running values lie in that order, limit per run; one thread adds them,
+  // 16 loaded ahead of their additions.
   double* sroot = (double*)&S.scal[1];
   if (tid == 0) {
     double q = root_qe;
+    const int total = n * limit;
     const double2* rv2 = (const double2*)a.rv;
-    for (int i = 0; i < n; i += 4) {
-      double2 r[4 * kRowLevels / 2];
+    int p = 0;
+    for (; p + 16 <= total; p += 16) {
+      double2 r[8];
 #pragma unroll
-      for (int k = 0; k < 4 * kRowLevels / 2; ++k) r[k] = rv2[(int64_t)i * (kRowLevels / 2) + k];
+      for (int k = 0; k < 8; ++k) r[k] = rv2[p / 2 + k];
 #pragma unroll
-      for (int k = 0; k < 4 * kRowLevels / 2; ++k) {
+      for (int k = 0; k < 8; ++k) {
         q += r[k].x;
         q += r[k].y;
       }
     }
+    for (; p < total; ++p) q += a.rv[p];
     *sroot = q;
   }
   __syncthreads();
@@ -3317,6 +3356,10 @@ class SubBuild {
     c.want(&a.tmg, (int64_t)kRowLevels * kRowChunk);
     c.want(&a.skey, (int64_t)kRowLevels * kRowChunk);
     c.want(&a.seg, (int64_t)kRowLevels * kRowChunk);
+    c.want(&a.ts, (int64_t)kRowLevels * kRowChunk);
+    c.want(&a.rvs, (int64_t)kRowLevels * kRowChunk);
+    c.want(&a.lts, 4 * (int64_t)kRowChunk);
+    c.want(&a.scnt, (int64_t)kRowChunk);
     c.want(&a.sp, 3 * (int64_t)kStageMax);
     c.want(&a.sp_child, 16 * (int64_t)kStageMax);
     c.want(&a.fid, (int64_t)kStageMax);
