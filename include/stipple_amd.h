@@ -396,6 +396,9 @@ int sa_chain_destroy(sa_chain chain);
  *   "approx.per"      Jacobi sweep: outputs per thread (2, 4, 8; 4)
  *   "seq.memo_bits"   sequential walk: a cap of 2^value LDS memo slots (4..10;
  *                     0 = no cap: the most that fit, which leaves one image per CU)
+ *   "octree.cells"    palette build: 1 places loop 2's pairs by a counting sort
+ *                     over cells when the cube depth leaves at most 4680 cells,
+ *                     0 sends every pair through the node hash (1)
  * Calls made after a set use the new value; chains keep the value they were
  * created with. */
 int sa_tune_set(const char* knob, int value, char* err, size_t errlen);

@@ -2043,6 +2043,9 @@ extern "C" int sa_tune_set(const char* knob, int value, char* err, size_t errlen
     if (value != 0 && (value < 4 || value > 10))
       return fail(err, errlen, SA_EINVAL, "seq.memo_bits must be 0 (largest fit) or 4..10");
     g_knobs.seq_memo_bits = value;
+  } else if (!std::strcmp(knob, "octree.cells")) {
+    if (value != 0 && value != 1) return fail(err, errlen, SA_EINVAL, "octree.cells must be 0 or 1");
+    sa::octree_set_cells(value);
   } else {
     return fail(err, errlen, SA_EINVAL, "unknown knob '%s'", knob);
   }
@@ -2054,6 +2057,7 @@ extern "C" int sa_tune_get(const char* knob, int* value) {
   if (!std::strcmp(knob, "blocks.threads")) *value = g_knobs.blocks_threads;
   else if (!std::strcmp(knob, "approx.per")) *value = g_knobs.approx_per;
   else if (!std::strcmp(knob, "seq.memo_bits")) *value = g_knobs.seq_memo_bits;
+  else if (!std::strcmp(knob, "octree.cells")) *value = sa::octree_cells();
   else return SA_EINVAL;
   return SA_OK;
 }

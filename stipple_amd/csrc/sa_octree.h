@@ -75,4 +75,10 @@ OctreeResult build_octree(const void* src, int fmt, int64_t w, int64_t h, int ma
 void build_octree_batch(const OctreeReq* req, int n, hipStream_t s, OctreeOut* out,
                         OctreeResult* res);
 
+// 1 (the default): loop 2's pairs of an image whose cube depth leaves few cells
+// are placed by a counting sort over cells; 0: every pair goes through the
+// node hash and the interval sort.  Both give the same trees.
+void octree_set_cells(int on);
+int octree_cells();
+
 }  // namespace sa

@@ -49,6 +49,7 @@
 #include <rocprim/block/block_radix_sort.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -187,6 +188,19 @@ struct Frame {
   Terms T;
   double* record;
   PaletteBuild pb;
+  // 3b. loop 2 placed by cell (images whose cube depth has at most
+  // kDenseCellsMax cells in all)
+  int32_t dense;
+  int32_t dcells;          // cells of levels 1..depth
+  int32_t dtshift;         // log2 of the loop-2 runs per tile
+  int32_t dtiles;
+  int64_t P_ins;           // pairs inserted through the hash: 8 R1 when dense, else P
+  int64_t dbase;           // this image's first cell in the batch's cell arrays
+  int32_t* dtab;           // [tile][cell] pair counts, then their exclusive prefix over tiles
+  uint32_t* dfirst;        // per cell: its first run
+  int32_t* dtot;           // per cell: its loop-2 pairs
+  uint32_t* dslot;         // per cell: the hash slot of its node
+  const uint32_t* dpos;    // per cell: first sorted position (batch-wide) of its loop-2 pairs
 };
 
 // Batch-wide arrays and counters.
@@ -430,7 +444,7 @@ __global__ __launch_bounds__(256) void k_node_insert(const Frame* __restrict__ f
   const At at = map_at(m, 256);
   const Frame& d = fr[at.f];
   const int64_t p = at.i;
-  const bool valid = p < d.P;
+  const bool valid = p < d.P_ins;
   const int64_t R1 = d.R1;
   const int depth = d.depth, assoc = d.assoc;
   int64_t r = 0;
@@ -484,14 +498,25 @@ __global__ __launch_bounds__(256) void k_node_insert(const Frame* __restrict__ f
 
 // Node entries (first pair, slot) of every image into one list; sorting it by
 // pair_base + first pair orders the batch by image, then by creation.
+// A workgroup's threads belong to one image, so each wavefront takes its
+// entries' places with one atomic.
 __global__ void k_node_collect(const Frame* __restrict__ fr, Map m, Batch B) {
   const At at = map_at(m, 256);
   const Frame& d = fr[at.f];
   const int64_t s = at.i;
-  if (s >= d.ncap) return;
-  if (d.nkeys[s] == kEmpty64) return;
-  const int k = atomicAdd(&B.ctl[0], 1);
-  atomicAdd(&d.info[13], 1);
+  const bool used = s < d.ncap && d.nkeys[s] != kEmpty64;
+  const uint64_t bal = __ballot(used);
+  if (!bal) return;
+  const int lane = threadIdx.x & 63;
+  const int lead = __ffsll((unsigned long long)bal) - 1;
+  int base = 0;
+  if (lane == lead) {
+    base = atomicAdd(&B.ctl[0], __popcll(bal));
+    atomicAdd(&d.info[13], __popcll(bal));
+  }
+  base = __shfl(base, lead, 64);
+  if (!used) return;
+  const int k = base + __popcll(bal & ((1ull << lane) - 1));
   B.ekey[k] = (uint32_t)d.pair_base + d.nfirst[s];
   B.eslot[k] = (uint32_t)s;
 }
@@ -501,19 +526,28 @@ __global__ void k_node_assign(const Frame* __restrict__ fr, Map m, Batch B) {
   const At at = map_at(m, 256);
   const Frame& d = fr[at.f];
   const int64_t k = at.i;
-  if (k >= d.M || !d.ok) return;
-  int32_t* info = d.info;
-  const int32_t node = (int32_t)(k + 1);
-  const uint32_t slot = B.sslot[d.entry_base + k];
-  const uint32_t first = B.skey[d.entry_base + k] - (uint32_t)d.pair_base;
-  d.slot_node[slot] = node;
-  const uint64_t key = d.nkeys[slot];
-  d.t.key[node] = key;
-  const int level = (int)(key >> 40);
-  const bool loop1 = first < 8ull * (uint64_t)d.R1;
-  if (loop1 && level == 8) atomicAdd(&info[2], 1);          // leaves counted in loop 1
-  if (!loop1 && level == d.depth) atomicAdd(&info[3], 1);   // level-depth nodes new in loop 2
-  if (loop1) atomicAdd(&info[4], 1);                        // nodes created by loop 1
+  const bool live = k < d.M && d.ok;
+  bool leaf1 = false, new2 = false, loop1 = false;
+  if (live) {
+    const int32_t node = (int32_t)(k + 1);
+    const uint32_t slot = B.sslot[d.entry_base + k];
+    const uint32_t first = B.skey[d.entry_base + k] - (uint32_t)d.pair_base;
+    d.slot_node[slot] = node;
+    const uint64_t key = d.nkeys[slot];
+    d.t.key[node] = key;
+    const int level = (int)(key >> 40);
+    loop1 = first < 8ull * (uint64_t)d.R1;
+    leaf1 = loop1 && level == 8;            // leaves counted in loop 1
+    new2 = !loop1 && level == d.depth;      // level-depth nodes new in loop 2
+  }
+  // One workgroup, one image: a count per wavefront.
+  const uint64_t b2 = __ballot(leaf1), b3 = __ballot(new2), b4 = __ballot(loop1);
+  if ((threadIdx.x & 63) == 0) {
+    int32_t* info = d.info;
+    if (b2) atomicAdd(&info[2], __popcll(b2));
+    if (b3) atomicAdd(&info[3], __popcll(b3));
+    if (b4) atomicAdd(&info[4], __popcll(b4));  // nodes created by loop 1
+  }
 }
 
 // The intervals in pair order: the workgroups of k_node_insert's grid again,
@@ -534,7 +568,7 @@ __global__ __launch_bounds__(256) void k_compact(const Frame* __restrict__ fr, M
   const Frame& d = fr[at.f];
   const int64_t p = at.i;
   const int64_t g = d.pair_base + p;
-  const bool start = p < d.P && pair_slot[g] != kNoSlot;
+  const bool start = p < d.P_ins && pair_slot[g] != kNoSlot;
   const unsigned long long bits = __ballot(start);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) s_wave[wave] = __popcll(bits);
@@ -545,17 +579,6 @@ __global__ __launch_bounds__(256) void k_compact(const Frame* __restrict__ fr, M
   istart[k] = (uint32_t)g;
   ikey[k] = (int32_t)d.node_base + (d.ok ? d.slot_node[pair_slot[g]] : 0);
   ival[k] = (uint32_t)k;
-}
-
-// Lengths in sorted order (then an exclusive scan gives each sorted
-// interval's first position).
-__global__ void k_interval_len(const uint32_t* __restrict__ sival,
-                               const uint32_t* __restrict__ istart, int64_t ni, int64_t sum_p,
-                               int32_t* __restrict__ slen) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ni) return;
-  const int64_t k = sival[j];
-  slen[j] = (int32_t)((k + 1 < ni ? (int64_t)istart[k + 1] : sum_p) - (int64_t)istart[k]);
 }
 
 // The image whose pair range holds batch position i: the last with
@@ -573,24 +596,57 @@ __device__ __forceinline__ int image_of_pair(const Frame* __restrict__ fr, int F
   return lo;
 }
 
+// Intervals [0, ni) come from k_compact; [ni, nt) stand for the loop-2 pairs
+// of one cell of an image placed by cell (k_dense_virtual), their length in
+// vlen.  An interval of k_compact runs up to the next one's first pair, and at
+// most to the end of its image's inserted pairs.
+constexpr uint32_t kVirtual = 0xFFFFFFFFu;
+
+// Lengths in sorted order (then an exclusive scan gives each sorted
+// interval's first position).
+__global__ void k_interval_len(const Frame* __restrict__ fr, int F,
+                               const uint32_t* __restrict__ sival,
+                               const uint32_t* __restrict__ istart, int64_t ni, int64_t nt,
+                               int64_t sum_p, const int32_t* __restrict__ vlen,
+                               int32_t* __restrict__ slen) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nt) return;
+  const int64_t k = sival[j];
+  if (k >= ni) {
+    slen[j] = vlen[k - ni];
+    return;
+  }
+  const int64_t b = istart[k];
+  int64_t e = k + 1 < ni ? (int64_t)istart[k + 1] : sum_p;
+  const Frame& d = fr[image_of_pair(fr, F, b)];
+  e = min(e, d.pair_base + d.P_ins);
+  slen[j] = (int32_t)(e - b);
+}
+
 // Sorted positions are expanded a tile of kExpandTile at a time.  Sorted
 // interval j covers positions [spos[j], spos[j] + slen[j]); it is the first
 // interval of every tile whose first position it covers.  A node's segment
 // (its pairs' sorted positions, as positions of its image) begins with its
-// first interval and ends with its last.
+// first interval and ends with its last.  A cell's interval (sival >= ni)
+// records its first position in dpos; the empty ones carry the key sum_nodes,
+// sort after every node and are skipped.
 constexpr int kExpandTile = 1024;
 
 __global__ void k_interval_ends(const Frame* __restrict__ fr, int F,
                                 const int32_t* __restrict__ sikey,
-                                const int32_t* __restrict__ spos, const int32_t* __restrict__ slen,
-                                int64_t ni, int32_t* __restrict__ tile_first) {
+                                const uint32_t* __restrict__ sival, const int32_t* __restrict__ spos,
+                                const int32_t* __restrict__ slen, int64_t ni, int64_t nt,
+                                int32_t sum_nodes, uint32_t* __restrict__ dpos,
+                                int32_t* __restrict__ tile_first) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ni) return;
+  if (j >= nt) return;
+  const int32_t n = sikey[j];
+  if (n >= sum_nodes) return;
   const int64_t b = spos[j], e = b + slen[j];
   for (int64_t t = (b + kExpandTile - 1) / kExpandTile; t * kExpandTile < e; ++t)
     tile_first[t] = (int32_t)j;
-  const int32_t n = sikey[j];
-  const bool first = j == 0 || sikey[j - 1] != n, last = j + 1 == ni || sikey[j + 1] != n;
+  if (sival[j] >= (uint64_t)ni) dpos[sival[j] - ni] = (uint32_t)b;
+  const bool first = j == 0 || sikey[j - 1] != n, last = j + 1 == nt || sikey[j + 1] != n;
   if (!first && !last) return;
   const Frame& d = fr[image_of_pair(fr, F, b)];
   if (!d.ok) return;
@@ -655,13 +711,18 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
   const int64_t i0 = (int64_t)blockIdx.x * kExpandTile;
   const int64_t i1 = i0 + kExpandTile < sum_p ? i0 + kExpandTile : sum_p;
   const int64_t j0 = tile_first[blockIdx.x];
+  // A tile inside one cell's interval has nothing to expand.
+  if (istart[sival[j0]] == kVirtual && (j0 + 1 >= ni || spos[j0 + 1] >= i1)) return;
   if (threadIdx.x < kWords) s_mark[threadIdx.x] = 0;
   __syncthreads();
-  for (int k = threadIdx.x; k <= kExpandTile; k += blockDim.x) {
+  for (int k0 = 0; k0 <= kExpandTile; k0 += blockDim.x) {
+    const int k = k0 + threadIdx.x;
     const int64_t j = j0 + k;
-    if (j < ni) {
+    bool past = k > kExpandTile || j >= ni;
+    if (!past) {
       const int32_t pos = spos[j];
-      if (pos < i1) {
+      past = pos >= i1;
+      if (!past) {
         l_pos[k] = pos;
         l_node[k] = sikey[j];
         l_first[k] = istart[sival[j]];
@@ -671,6 +732,7 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
         }
       }
     }
+    if (__syncthreads_or(past)) break;  // intervals start in ascending order
   }
   if (threadIdx.x == 0) {
     const int f = image_of_pair(fr, F, i0);
@@ -695,6 +757,7 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
     const int o = (int)(i - i0);
     const uint32_t below = (o & 31) == 31 ? 0xFFFFFFFFu : ((2u << (o & 31)) - 1u);
     const int k = s_before[o >> 5] + __popc(s_mark[o >> 5] & below);
+    if (l_first[k] == kVirtual) continue;  // placed by k_dense_scatter
     const int64_t g = (int64_t)l_first[k] + (i - l_pos[k]);
     if (one_image) {
       const Frame& d = fr[f0];
@@ -703,6 +766,371 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
       const Frame& d = fr[image_of_pair(fr, F, i)];
       expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[k], spair);
     }
+  }
+}
+
+// ---- 3b. loop 2 placed by cell -------------------------------------------------------
+// When the cube depth leaves few cells (kids + kids^2 + ... + kids^depth at most
+// kDenseCellsMax: depth <= 4 without alpha, <= 3 with), loop 2's pairs skip the
+// node hash and the interval sort.  A cell (level l, the top l bits of each
+// channel byte) is a node, and its pairs in run order are a counting sort of
+// loop 2's runs by cell:
+//   k_dense_count    one wavefront per tile of runs counts every cell's pairs
+//                    and notes each cell's first run;
+//   k_dense_scan     an exclusive scan over tiles: each tile's offset in a cell;
+//   k_dense_insert   each cell enters the node hash with its first pair, so
+//                    node numbering is unchanged;
+//   k_dense_virtual  one interval per cell stands for its pairs in the interval
+//                    sort, after the cell's loop-1 intervals;
+//   k_dense_scatter  a second pass over the tiles writes every pair, its term
+//                    and leaf value at its sorted position.
+// Inside a tile a wavefront takes 64 consecutive runs at a time and ranks each
+// lane among the lanes with its cell (a lane mask refined by one ballot per key
+// bit and level); the lowest such lane then advances the cell's count, so a
+// cell's pairs keep run order.
+constexpr int kDenseCellsMax = 4680;  // 8 + 64 + 512 + 4096
+
+struct TileAt {
+  int32_t f;
+  int64_t t;
+};
+
+// Tile of block b (one block per tile).
+__device__ __forceinline__ TileAt map_tile(const Map& m, int64_t b) {
+  int32_t lo = 0, hi = m.F - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (m.first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return {lo, b - m.first[lo]};
+}
+
+// Workgroups are dealt to the 8 XCDs in turn; this numbering gives each XCD a
+// contiguous range of tiles, so the segments that consecutive tiles extend are
+// filled in one L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t q = nb / 8, rem = nb % 8, x = b % 8;
+  return x * q + (x < rem ? x : rem) + b / 8;
+}
+
+__global__ __launch_bounds__(64) void k_dense_count(const Frame* __restrict__ fr, Map m) {
+  extern __shared__ int32_t s_dyn[];
+  const TileAt tl = map_tile(m, xcd_block(blockIdx.x, gridDim.x));
+  const Frame& d = fr[tl.f];
+  const int dcells = d.dcells;
+  int32_t* s_cnt = s_dyn;
+  uint32_t* s_first = (uint32_t*)(s_dyn + dcells);
+  const int lane = threadIdx.x;
+  for (int c = lane; c < dcells; c += 64) s_cnt[c] = 0;
+  __syncthreads();
+  const int64_t r0 = d.R1 + (tl.t << d.dtshift);
+  const int64_t r1 = min(d.R, r0 + ((int64_t)1 << d.dtshift));
+  const int depth = d.depth, kids = d.kids, nch = d.assoc ? 4 : 3;
+  const uint64_t below = (1ull << lane) - 1;
+  const uint32_t* pack = d.pack;
+  for (int64_t rb = r0; rb < r1; rb += 64) {
+    const int64_t r = rb + lane;
+    const bool valid = r < r1;
+    const uint32_t pk = valid ? pack[r] : 0u;
+    uint64_t msk = __ballot(valid);
+    int off = 0, pw = 1;
+    for (int l = 1; l <= depth; ++l) {
+      uint32_t cell = 0;
+      for (int c = 0; c < nch; ++c) {
+        const uint32_t top = ((pk >> (8 * c)) & 0xFFu) >> (8 - l);
+        const uint64_t v = __ballot((top & 1u) != 0);
+        msk &= (top & 1u) ? v : ~v;
+        cell |= top << (l * c);
+      }
+      if (valid && (msk & below) == 0) {  // the lowest lane of its cell
+        const int k = off + (int)cell;
+        const int32_t n = s_cnt[k];
+        if (n == 0) s_first[k] = (uint32_t)r;
+        s_cnt[k] = n + __popcll(msk);
+      }
+      pw *= kids;
+      off += pw;
+    }
+  }
+  __syncthreads();
+  int32_t* tab = d.dtab + tl.t * (int64_t)dcells;
+  for (int c = lane; c < dcells; c += 64) {
+    const int32_t n = s_cnt[c];
+    tab[c] = n;
+    if (n) atomicMin(&d.dfirst[c], s_first[c]);
+  }
+}
+
+// One thread per cell: the tiles' counts become their exclusive prefix, and
+// dtot the cell's loop-2 pairs.
+__global__ void k_dense_scan(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t c = at.i;
+  if (c >= d.dcells) return;
+  int32_t* p = d.dtab + c;
+  const int64_t st = d.dcells, nt = d.dtiles;
+  int32_t acc = 0;
+  int64_t t = 0;
+  for (; t + 8 <= nt; t += 8) {
+    int32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p[(t + k) * st];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      p[(t + k) * st] = acc;
+      acc += v[k];
+    }
+  }
+  for (; t < nt; ++t) {
+    const int32_t v = p[t * st];
+    p[t * st] = acc;
+    acc += v;
+  }
+  d.dtot[c] = acc;
+}
+
+// Level of cell c (cells are numbered level by level).
+__device__ __forceinline__ int dense_level(int64_t c, int kids, int* off) {
+  int l = 1;
+  int64_t o = 0, pw = kids;
+  while (c >= o + pw) {
+    o += pw;
+    pw *= kids;
+    ++l;
+  }
+  *off = (int)o;
+  return l;
+}
+
+// Each cell with pairs enters the node hash (k_node_insert's table and
+// protocol) with the run-major index of its first pair; a cell that loop 1
+// already created keeps loop 1's first pair.
+__global__ void k_dense_insert(const Frame* __restrict__ fr, Map m) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t c = at.i;
+  if (c >= d.dcells || !d.ok) return;
+  const uint32_t r = d.dfirst[c];
+  if (r == kEmpty32) return;
+  int off;
+  const int l = dense_level(c, d.kids, &off);
+  const uint64_t key = node_key(d.pack[r], l, d.assoc != 0);
+  const uint32_t first = pair_rmajor(r, l, d.R1, d.depth);
+  uint64_t* hkeys = d.nkeys;
+  uint32_t* hfirst = d.nfirst;
+  const uint32_t cap = d.ncap;
+  uint32_t slot = hash32(key) & (cap - 1);
+  for (uint32_t probe = 0;; ++probe) {
+    if (probe >= cap) {  // table full: report, never spin
+      d.info[6] = 4;
+      d.dslot[c] = 0;
+      return;
+    }
+    const uint64_t cur = hkeys[slot];
+    if (cur == key) break;
+    if (cur == kEmpty64) {
+      const unsigned long long prev =
+          atomicCAS((unsigned long long*)&hkeys[slot], (unsigned long long)kEmpty64,
+                    (unsigned long long)key);
+      if (prev == kEmpty64 || prev == key) break;
+    }
+    slot = (slot + 1) & (cap - 1);
+  }
+  atomicMin(&hfirst[slot], first);
+  d.dslot[c] = slot;
+}
+
+// Interval ni + dbase + c stands for cell c's loop-2 pairs: no first pair of
+// its own (kVirtual), its node as the sort key -- the empty ones sort after
+// every node -- and, since the sort is stable and these follow k_compact's
+// intervals, it lands after the node's loop-1 intervals.
+__global__ void k_dense_virtual(const Frame* __restrict__ fr, Map m, int64_t ni,
+                                int32_t sum_nodes, uint32_t* __restrict__ istart,
+                                int32_t* __restrict__ ikey, uint32_t* __restrict__ ival,
+                                int32_t* __restrict__ vlen) {
+  const At at = map_at(m, 256);
+  const Frame& d = fr[at.f];
+  const int64_t c = at.i;
+  if (c >= d.dcells) return;
+  const int64_t v = d.dbase + c, k = ni + v;
+  const int32_t n = d.dtot[c];
+  istart[k] = kVirtual;
+  ival[k] = (uint32_t)k;
+  vlen[v] = n;
+  ikey[k] = n == 0 ? sum_nodes
+                   : (int32_t)d.node_base + (d.ok ? d.slot_node[d.dslot[c]] : 0);
+}
+
+// Exclusive prefix of v over a 256-thread workgroup (s_part: 4 ints).
+__device__ __forceinline__ int block_excl_256(int v, int* s_part) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) s_part[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += s_part[w];
+  __syncthreads();
+  return before + incl - v;
+}
+
+// The cell of a run at level l and the lanes of the wavefront whose run has
+// the same cell (msk: the candidate lanes on entry).
+__device__ __forceinline__ uint32_t dense_cell(uint32_t pk, int l, int nch, uint64_t* msk) {
+  uint64_t mk = *msk;
+  uint32_t cell = 0;
+  for (int k = 1; k <= l; ++k) {
+    for (int c = 0; c < nch; ++c) {
+      const uint32_t top = ((pk >> (8 * c)) & 0xFFu) >> (8 - k);
+      const uint64_t v = __ballot((top & 1u) != 0);
+      mk &= (top & 1u) ? v : ~v;
+      if (k == l) cell |= top << (l * c);
+    }
+  }
+  *msk = mk;
+  return cell;
+}
+
+// Every loop-2 pair of a tile at its sorted position, one level at a time.
+// The tile's runs are first put in the level's order inside the tile (cells
+// ascending, runs ascending within a cell): each wavefront counts the cells of
+// its quarter of the tile, a scan over cells gives every cell's first slot and
+// each wavefront's first slot in it, and a second walk ranks the runs into a
+// slot -> run map.  The pairs are then written slot by slot, so consecutive
+// threads write consecutive positions of one cell's segment (the cell's first
+// global position for this tile minus its first slot, plus the slot).  The
+// values are expand_one's, the cube midpoints formed level by level as the
+// classification descends.  LDS: kDenseLds(T, cells of the deepest level).
+constexpr int kScatterThreads = 256;
+
+__host__ __device__ constexpr int64_t dense_scatter_lds(int64_t tile, int64_t cmax) {
+  return 4 * cmax + 2 * 4 * cmax + 2 * tile;
+}
+
+__global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* __restrict__ fr,
+                                                                   Map m,
+                                                                   uint32_t* __restrict__ spair_all) {
+  extern __shared__ int32_t s_dyn[];
+  __shared__ int s_part[4];
+  const TileAt tl = map_tile(m, xcd_block(blockIdx.x, gridDim.x));
+  const Frame& d = fr[tl.f];
+  const int depth = d.depth, kids = d.kids, assoc = d.assoc, nch = assoc ? 4 : 3;
+  int cmax = 1;
+  for (int l = 1; l <= depth; ++l) cmax *= kids;
+  const int64_t tile = (int64_t)1 << d.dtshift;
+  const int64_t R1 = d.R1, n2 = d.R - d.R1;
+  const int64_t r0 = R1 + tl.t * tile, r1 = min(d.R, r0 + tile);
+  const int n = (int)(r1 - r0);
+  int32_t* s_delta = s_dyn;                          // cmax
+  uint16_t* s_cnt = (uint16_t*)(s_dyn + cmax);       // 4 x cmax
+  uint16_t* s_run = s_cnt + 4 * cmax;                // tile
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t quarter = tile / 4;
+  const int64_t wr0 = r0 + wave * quarter, wr1 = min(r1, wr0 + quarter);
+  const uint64_t below = (1ull << lane) - 1;
+  const uint32_t* pack = d.pack;
+  const int32_t* tab = d.dtab + tl.t * (int64_t)d.dcells;
+  const uint32_t* dpos = d.dpos + d.dbase;
+  const uint32_t l1 = (uint32_t)(8 * R1);
+  const Terms T = d.T;
+  uint32_t* spair = spair_all + d.pair_base;
+  const bool bytes = !assoc && (d.fmt == kRgb8 || d.fmt == kRgba8);
+  int off = 0, C = 1;
+  for (int l = 1; l <= depth; ++l) {
+    C *= kids;
+    uint16_t* cw = s_cnt + wave * C;
+    for (int c = tid; c < 4 * C; c += kScatterThreads) s_cnt[c] = 0;
+    __syncthreads();
+    for (int64_t rb = wr0; rb < wr1; rb += 64) {  // count this quarter's cells
+      const int64_t r = rb + lane;
+      const bool valid = r < wr1;
+      uint64_t msk = __ballot(valid);
+      const uint32_t cell = dense_cell(valid ? pack[r] : 0u, l, nch, &msk);
+      if (valid && (msk & below) == 0) cw[cell] += (uint16_t)__popcll(msk);
+    }
+    __syncthreads();
+    {  // first slots: per cell, per wavefront in it; the slot -> position offsets
+      const int per = (C + kScatterThreads - 1) / kScatterThreads;
+      const int cb = min(C, tid * per), ce = min(C, cb + per);
+      int sum = 0;
+      for (int c = cb; c < ce; ++c)
+        sum += s_cnt[c] + s_cnt[C + c] + s_cnt[2 * C + c] + s_cnt[3 * C + c];
+      int run = block_excl_256(sum, s_part);
+      for (int c = cb; c < ce; ++c) {
+        int at = run;
+        for (int w = 0; w < 4; ++w) {
+          const int v = s_cnt[w * C + c];
+          s_cnt[w * C + c] = (uint16_t)at;
+          at += v;
+        }
+        if (at > run) s_delta[c] = (int32_t)((int64_t)dpos[off + c] - d.pair_base) + tab[off + c] - run;
+        run = at;
+      }
+    }
+    __syncthreads();
+    for (int64_t rb = wr0; rb < wr1; rb += 64) {  // rank: the slot -> run map
+      const int64_t r = rb + lane;
+      const bool valid = r < wr1;
+      uint64_t msk = __ballot(valid);
+      const uint32_t cell = dense_cell(valid ? pack[r] : 0u, l, nch, &msk);
+      if (valid) {
+        const uint64_t lo = msk & below;
+        const int base = cw[cell];
+        s_run[base + __popcll(lo)] = (uint16_t)(r - r0);
+        if (lo == 0) cw[cell] = (uint16_t)(base + __popcll(msk));
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kScatterThreads) {  // write in slot order
+      const int64_t r = r0 + s_run[i];
+      const uint32_t pk = pack[r];
+      const int32_t cnt = d.count[r];
+      // An 8-bit source read without association: its channels are the packed
+      // bytes times 257 (load_pixel), so only the pack is gathered.
+      const float4 f = bytes ? make_float4((float)(pk & 0xFFu) * 257.0f,
+                                           (float)((pk >> 8) & 0xFFu) * 257.0f,
+                                           (float)((pk >> 16) & 0xFFu) * 257.0f, 65535.0f)
+                             : d.px[r];
+      double a[4];
+      assoc_pixel(assoc != 0, f.x, f.y, f.z, f.w, a);
+      double mid[4] = {kRange / 2.0, kRange / 2.0, kRange / 2.0, kRange / 2.0};
+      double bisect = (kRange + 1.0) / 2.0;
+      uint32_t cell = 0;
+      for (int k = 1; k <= l; ++k) {
+        bisect *= 0.5;
+        for (int c = 0; c < 4; ++c) {
+          uint32_t bit = 0;
+          if (c < nch) {
+            const uint32_t top = ((pk >> (8 * c)) & 0xFFu) >> (8 - k);
+            bit = top & 1u;
+            if (k == l) cell |= top << (l * c);
+          }
+          mid[c] += bit ? bisect : -bisect;
+        }
+      }
+      const int64_t li = s_delta[cell] + i;
+      const double e0 = kScale * (a[0] - mid[0]);
+      const double e1 = kScale * (a[1] - mid[1]);
+      const double e2 = kScale * (a[2] - mid[2]);
+      const double e3 = assoc ? kScale * (a[3] - mid[3]) : 0.0;
+      double dd = e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
+      if (dd != dd) dd = 0.0;
+      spair[li] = l1 + (uint32_t)((int64_t)(l - 1) * n2 + (r - R1));
+      T.term[li] = (double)cnt * sqrt(dd);
+      T.lcnt[li] = cnt;
+      if (l == depth)
+        T.lval[li] = make_float4(clamp_pixel(a[0]), clamp_pixel(a[1]), clamp_pixel(a[2]),
+                                 clamp_pixel(assoc ? a[3] : kRange));
+    }
+    __syncthreads();
+    off += C;
   }
 }
 
@@ -2662,6 +3090,13 @@ inline unsigned blocks(int64_t n, int per) { return (unsigned)((n + per - 1) / p
 // arrays are one stream-ordered allocation; descriptors go up and results come
 // back through a pinned buffer kept per host thread.
 // ---------------------------------------------------------------------------------
+namespace {
+std::atomic<int> g_cells{1};  // the octree.cells knob
+}  // namespace
+
+void octree_set_cells(int on) { g_cells.store(on ? 1 : 0); }
+int octree_cells() { return g_cells.load(); }
+
 int octree_depth(int maxc, bool assoc, bool gray) {  // rd_octree.cpp:255-275
   int depth = 1;
   for (int64_t c = maxc; c != 0; ++depth) c >>= 2;
@@ -2680,7 +3115,7 @@ namespace {
 // keeps every batch-wide index within 32 bits.
 constexpr int64_t kBatchPixels = (int64_t)1 << 26;
 
-enum MapKind { kMapN, kMapR, kMapP, kMapCap, kMapM256, kMapM64, kMapDesc, kMaps };
+enum MapKind { kMapN, kMapR, kMapP, kMapCap, kMapM256, kMapM64, kMapDesc, kMapDC, kMapDT, kMaps };
 
 uint32_t pow2_at_least(int64_t v) {
   uint32_t c = 1;
@@ -2818,7 +3253,7 @@ class SubBuild {
   void fill_maps(int64_t* tab) {
     for (int k = 0; k < kMaps; ++k) {
       int64_t* first = tab + (size_t)k * (F_ + 1);
-      const int per = k == kMapM64 ? 64 : 256;
+      const int per = k == kMapM64 ? 64 : k == kMapDT ? 1 : 256;
       int64_t run = 0;
       for (int f = 0; f < F_; ++f) {
         const Frame& d = H_[f];
@@ -2826,7 +3261,9 @@ class SubBuild {
         switch (k) {
           case kMapN: n = d.ok ? d.N : 0; break;
           case kMapR: n = d.ok ? d.R : 0; break;
-          case kMapP: n = d.P; break;  // failed images keep their pairs in the sort
+          case kMapP: n = d.P_ins; break;  // failed images keep their pairs in the sort
+          case kMapDC: n = d.dense ? d.dcells : 0; break;
+          case kMapDT: n = d.dense && d.ok ? d.dtiles : 0; break;
           case kMapCap: n = d.ok ? d.ncap : 0; break;
           case kMapM256:
           case kMapM64: n = d.ok ? d.M : 0; break;
@@ -2982,6 +3419,23 @@ class SubBuild {
       bound_nodes = std::min<int64_t>(bound_nodes, P + 1);
       bound_[f] = bound_nodes;
       d.P = P;
+      d.P_ins = P;
+      int64_t cells = 0;
+      for (int64_t l = 1, pw = 1; l <= d.depth; ++l) cells += (pw *= d.kids);
+      int64_t cmax = 1;
+      for (int l = 1; l <= d.depth; ++l) cmax *= d.kids;
+      // Tiles of 8192 runs (4096 when the deepest level has 4096 cells), so
+      // the scatter's LDS stays within 64 KiB; at most 2^22 table entries.
+      const int sh = cmax <= 512 ? 13 : 12;
+      const int64_t tiles = (d.R - d.R1 + ((int64_t)1 << sh) - 1) >> sh;
+      if (g_cells.load() && cells <= kDenseCellsMax && d.R > d.R1 &&
+          tiles * cells <= ((int64_t)1 << 22)) {
+        d.dense = 1;
+        d.dcells = (int32_t)cells;
+        d.dtshift = sh;
+        d.dtiles = (int32_t)((d.R - d.R1 + ((int64_t)1 << sh) - 1) >> sh);
+        d.P_ins = 8 * d.R1;
+      }
       d.ncap = pow2_at_least(2 * bound_nodes + 64);
       d.pair_base = sumP;
       sumP += P;
@@ -2999,8 +3453,24 @@ class SubBuild {
     for (int64_t f = 0, run = 0; f < F_; ++f) {
       H_[f].pair_base = run;
       run += H_[f].P;
-      nwg_ += blocks(H_[f].P, 256);  // k_node_insert's workgroups (the kMapP grid)
+      nwg_ += blocks(H_[f].P_ins, 256);  // k_node_insert's workgroups (the kMapP grid)
     }
+    int64_t sumDC = 0, sumDT = 0;
+    max_dc_ = 0;
+    max_scatter_lds_ = 0;
+    for (int f = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (!d.dense) continue;
+      d.dbase = sumDC;
+      sumDC += d.dcells;
+      sumDT += (int64_t)d.dtiles * d.dcells;
+      max_dc_ = std::max<int64_t>(max_dc_, d.dcells);
+      int64_t cmax = 1;
+      for (int l = 1; l <= d.depth; ++l) cmax *= d.kids;
+      max_scatter_lds_ = std::max<int64_t>(max_scatter_lds_,
+                                           dense_scatter_lds((int64_t)1 << d.dtshift, cmax));
+    }
+    sumDC_ = sumDC;
     uint64_t* nkeys = nullptr;
     uint32_t *nfirst = nullptr, *pair_slot = nullptr;
     Carve c;
@@ -3017,8 +3487,23 @@ class SubBuild {
     c.want(&B_.ekey, sumC);
     c.want(&B_.eslot, sumC);
     c.want(&slot_node_, sumC);
+    int32_t *dtab = nullptr, *dtot = nullptr;
+    uint32_t *dfirst = nullptr, *dslot = nullptr;
+    c.want(&dtab, sumDT);
+    c.want(&dfirst, sumDC);
+    c.want(&dtot, sumDC);
+    c.want(&dslot, sumDC);
     if ((e = c.take(owned_, s_)) != hipSuccess) return e;
     pair_slot_ = pair_slot;
+    for (int64_t f = 0, to = 0; f < F_; ++f) {
+      Frame& d = H_[f];
+      if (!d.dense) continue;
+      d.dtab = dtab + to;
+      d.dfirst = dfirst + d.dbase;
+      d.dtot = dtot + d.dbase;
+      d.dslot = dslot + d.dbase;
+      to += (int64_t)d.dtiles * d.dcells;
+    }
     int64_t co = 0;
     for (int f = 0; f < F_; ++f) {
       Frame& d = H_[f];
@@ -3038,8 +3523,14 @@ class SubBuild {
       if ((e = hipcub::DeviceReduce::Sum(red_tmp, tmp_red, wgc_, B_.ctl + 12, (int)nwg_, s_)) !=
           hipSuccess)
         return e;
-      k_node_collect<<<grid_[kMapCap], 256, 0, s_>>>(d_fr_, map(kMapCap), B_);
     }
+    if (grid_[kMapDT]) {  // loop 2 of the images placed by cell
+      fill(dfirst, sumDC, kEmpty32);
+      k_dense_count<<<grid_[kMapDT], 64, 8 * max_dc_, s_>>>(d_fr_, map(kMapDT));
+      k_dense_scan<<<grid_[kMapDC], 256, 0, s_>>>(d_fr_, map(kMapDC));
+      k_dense_insert<<<grid_[kMapDC], 256, 0, s_>>>(d_fr_, map(kMapDC));
+    }
+    if (grid_[kMapP]) k_node_collect<<<grid_[kMapCap], 256, 0, s_>>>(d_fr_, map(kMapCap), B_);
     // The outputs outlive the batch (plain allocations owned by the caller):
     // allocated while the node kernels run, sized by the node bound.
     for (int f = 0; f < F_; ++f) {
@@ -3083,21 +3574,22 @@ class SubBuild {
     for (int f = 0; f < F_; ++f)
       if (H_[f].ok) sumW += H_[f].M + 2;
     const int64_t NI = hctl()[12];  // intervals (k_node_insert)
-    const int bitsA = bits_for(sumP), bitsB = bits_for(sumNodes);
+    const int64_t NT = NI + sumDC_;  // and one per cell of the images placed by cell
+    const int bitsA = bits_for(sumP), bitsB = bits_for(sumNodes + 1);
     size_t tmpA = 0, tmpB = 0, tmpSel = 0, tmpSum = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpA, B_.ekey, (uint32_t*)nullptr,
                                                       B_.eslot, (uint32_t*)nullptr, (int)E, 0,
                                                       bitsA, s_);
     if (e != hipSuccess) return e;
     e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpB, (int32_t*)nullptr, (int32_t*)nullptr,
-                                           (uint32_t*)nullptr, (uint32_t*)nullptr, (int)NI, 0,
+                                           (uint32_t*)nullptr, (uint32_t*)nullptr, (int)NT, 0,
                                            bitsB, s_);
     if (e != hipSuccess) return e;
     e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmpSel, (int32_t*)nullptr, (int32_t*)nullptr,
                                          (int)nwg_, s_);
     if (e != hipSuccess) return e;
     e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmpSum, (int32_t*)nullptr, (int32_t*)nullptr,
-                                         (int)NI, s_);
+                                         (int)NT, s_);
     if (e != hipSuccess) return e;
     const size_t tmpBytes = std::max({tmpA, tmpB, tmpSel, tmpSum});
     TreeDev t{};
@@ -3126,14 +3618,18 @@ class SubBuild {
     c.want(&spair, sumP);
     uint32_t *istart = nullptr, *ival = nullptr, *sival = nullptr;
     int32_t *ikey = nullptr, *sikey = nullptr, *slen = nullptr, *spos = nullptr, *wgo = nullptr;
-    c.want(&istart, NI);
-    c.want(&ival, NI);
-    c.want(&sival, NI);
-    c.want(&ikey, NI);
-    c.want(&sikey, NI);
-    c.want(&slen, NI);
-    c.want(&spos, NI);
+    c.want(&istart, NT);
+    c.want(&ival, NT);
+    c.want(&sival, NT);
+    c.want(&ikey, NT);
+    c.want(&sikey, NT);
+    c.want(&slen, NT);
+    c.want(&spos, NT);
     c.want(&wgo, nwg_);
+    int32_t* vlen = nullptr;
+    uint32_t* dpos = nullptr;
+    c.want(&vlen, sumDC_);
+    c.want(&dpos, sumDC_);
     const int64_t ntiles = (sumP + kExpandTile - 1) / kExpandTile;
     int32_t* tile_first = nullptr;
     c.want(&tile_first, ntiles);
@@ -3175,6 +3671,7 @@ class SubBuild {
       d.T.lval = lval + pb;
       d.T.lcnt = lcnt + pb;
       d.record = (double*)pp + pb;
+      d.dpos = dpos;
       if (d.ok) {
         int32_t** wp[11] = {&d.W.lvl, &d.W.att, &d.W.anc, &d.W.size, &d.W.gsz, &d.W.start,
                             &d.W.gstart, &d.W.pre, &d.W.merged, &d.W.arr, &d.W.arr2};
@@ -3203,16 +3700,23 @@ class SubBuild {
     if (e != hipSuccess) return e;
     k_compact<<<grid_[kMapP], 256, 0, s_>>>(d_fr_, map(kMapP), wgo, pair_slot_, istart, ikey,
                                             ival);
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpB, ikey, sikey, ival, sival, (int)NI, 0, bitsB,
+    if (grid_[kMapDC])
+      k_dense_virtual<<<grid_[kMapDC], 256, 0, s_>>>(d_fr_, map(kMapDC), NI, (int32_t)sumNodes,
+                                                    istart, ikey, ival, vlen);
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpB, ikey, sikey, ival, sival, (int)NT, 0, bitsB,
                                            s_);
     if (e != hipSuccess) return e;
-    const unsigned gi = (unsigned)blocks(NI, 256);
-    k_interval_len<<<gi, 256, 0, s_>>>(sival, istart, NI, sumP, slen);
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmpSum, slen, spos, (int)NI, s_);
+    const unsigned gi = (unsigned)blocks(NT, 256);
+    k_interval_len<<<gi, 256, 0, s_>>>(d_fr_, F_, sival, istart, NI, NT, sumP, vlen, slen);
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmpSum, slen, spos, (int)NT, s_);
     if (e != hipSuccess) return e;
-    k_interval_ends<<<gi, 256, 0, s_>>>(d_fr_, F_, sikey, spos, slen, NI, tile_first);
-    k_expand<<<(unsigned)ntiles, 256, 0, s_>>>(d_fr_, F_, sumP, NI, tile_first, sikey, sival,
+    k_interval_ends<<<gi, 256, 0, s_>>>(d_fr_, F_, sikey, sival, spos, slen, NI, NT,
+                                        (int32_t)sumNodes, dpos, tile_first);
+    k_expand<<<(unsigned)ntiles, 256, 0, s_>>>(d_fr_, F_, sumP, NT, tile_first, sikey, sival,
                                                istart, spos, spair);
+    if (grid_[kMapDT])
+      k_dense_scatter<<<grid_[kMapDT], kScatterThreads, max_scatter_lds_, s_>>>(d_fr_, map(kMapDT),
+                                                                            spair);
     if (grid_[kMapM256]) k_links<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256));
     // At most sumP / kShortSeg segments are medium, sumP / kMidSeg long.
     const int mid_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 8192);
@@ -3421,6 +3925,9 @@ class SubBuild {
   uint32_t* pair_slot_ = nullptr;  // batch pair array: slot of an interval's first pair
   int32_t* wgc_ = nullptr;         // intervals per workgroup of k_node_insert
   int64_t nwg_ = 0;
+  int64_t sumDC_ = 0;              // cells of the images placed by cell
+  int64_t max_dc_ = 0;
+  int64_t max_scatter_lds_ = 0;
   std::vector<int64_t> bound_ = std::vector<int64_t>(F_, 0);  // node bound per image
 };
 

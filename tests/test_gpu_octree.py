@@ -296,3 +296,51 @@ def test_octree_batch_random_mix(gpu, oracle, seed):
     for pal, (f, colours, assoc, gray, w, h) in zip(pals, meta):
         say("  item", w, h, colours, assoc, gray)
         _check_built(oracle, pal, f, colours, assoc, gray)
+
+
+def _build_mix(gpu, cases):
+    items = []
+    for img, colours in cases:
+        h, w = img.shape[:2]
+        src = H.to_device(img)
+        _KEEP.append(src)
+        assoc = img.shape[2] == 4 and img.dtype != np.float32
+        items.append((src, w, h, H.fmt_of(img), colours, assoc, synth.is_gray(img), True))
+    pals = gpu.Palette.build_batch(items)
+    _KEEP.extend(pals)
+    return pals
+
+
+def test_octree_cells_knob_same_trees(gpu):
+    """Loop 2 placed by a counting sort over cells (octree.cells = 1) and every
+    pair sent through the node hash (0) build the same trees, bit for bit."""
+    cases = _random_mix(23, 32)
+    rng = np.random.default_rng(23)
+    for colours, ch in [(16, 3), (64, 3), (16, 4), (4, 4), (2, 3)]:
+        img = synth.sinusoid_image(700, 400, rng, channels=ch)
+        noisy = np.clip(img.astype(np.int16) + rng.integers(-6, 7, img.shape), 0, 255)
+        cases.append((noisy.astype(np.uint8), colours))
+    say("octree cells knob", len(cases))
+    assert gpu.tune_get("octree.cells") == 1
+    pals1 = _build_mix(gpu, cases)
+    try:
+        gpu.tune_set("octree.cells", 0)
+        pals0 = _build_mix(gpu, cases)
+    finally:
+        gpu.tune_set("octree.cells", 1)
+    for (img, colours), a, b in zip(cases, pals1, pals0):
+        say("  item", img.shape, colours)
+        _same_palette(a, b)
+
+
+@pytest.mark.parametrize("colours,ch", [(64, 3), (16, 4)])
+def test_octree_cells_large_tiles(gpu, oracle, colours, ch):
+    """A noisy 1920x1080 frame whose cells need tiles above the smallest size:
+    cube depth 4 without alpha (4680 cells), depth 3 with alpha (4368)."""
+    rng = np.random.default_rng(colours + ch)
+    img = synth.sinusoid_image(1920, 1080, rng, channels=ch)
+    img = np.clip(img.astype(np.int16) + rng.integers(-5, 6, img.shape), 0, 255).astype(np.uint8)
+    src = H.to_device(img)
+    _KEEP.append(src)
+    _compare(gpu, oracle, src, H.fmt_of(img), 1920, 1080, synth.widen_q16(img), colours, ch == 4,
+             False)
