@@ -156,8 +156,7 @@ struct Frame {
   int32_t* info;           // 16 ints, layout in sa_octree.h
   double* dinfo;           // [0] parallel root chain, [1] max non-root qe, [2] root qe used
   // 1. runs
-  int32_t* flags;
-  int32_t* rid;            // exclusive scan of flags over the batch
+  int32_t px_pack;         // 1: no pixel copy, run_px rebuilds it from the pack
   int64_t R;
   int32_t* start;
   int32_t* count;
@@ -241,6 +240,24 @@ __device__ __forceinline__ At map_at(const Map& m, int per) {
   return {lo, (b - m.first[lo]) * per + (int64_t)threadIdx.x};
 }
 
+struct TileAt {
+  int32_t f;
+  int64_t t;
+};
+
+// Tile of block b (one block per tile).
+__device__ __forceinline__ TileAt map_tile(const Map& m, int64_t b) {
+  int32_t lo = 0, hi = m.F - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (m.first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return {lo, b - m.first[lo]};
+}
+
 // ---- 1. runs ------------------------------------------------------------------
 // info zeroed, info[5] = h, the double results zeroed (one workgroup per image).
 __global__ void k_frame_init(const Frame* __restrict__ fr) {
@@ -249,50 +266,104 @@ __global__ void k_frame_init(const Frame* __restrict__ fr) {
   if (threadIdx.x < 4) d.dinfo[threadIdx.x] = 0.0;
 }
 
-__global__ void k_flags(const Frame* __restrict__ fr, Map m) {
-  const At a = map_at(m, 256);
-  const Frame& d = fr[a.f];
-  const int64_t p = a.i;
-  if (p >= d.N) return;
-  int f = 1;
-  if (p % d.w != 0)
-    f = same_px(load_any(d.fmt, d.src, p, d.w, d.h), load_any(d.fmt, d.src, p - 1, d.w, d.h)) ? 0
-                                                                                             : 1;
-  d.flags[p] = f;
+// Runs are found a tile of kRunTile pixels at a time (one 256-thread workgroup
+// per tile; a tile never spans two images): k_run_tiles counts the run starts
+// of every tile, an exclusive scan over the batch's tiles gives each tile's
+// first run, and k_run_emit writes every run's record; k_run_len takes the
+// lengths from consecutive starts.
+constexpr int kRunTile = 4096;
+
+// The run's pixel as loaded.  An 8-bit source read without association keeps
+// no copy: its channels are the packed bytes times 257 (load_pixel), and its
+// alpha is never read.
+__device__ __forceinline__ float4 run_px(const float4* px, const uint32_t* pack, int px_pack,
+                                         int64_t r) {
+  if (!px_pack) return px[r];
+  const uint32_t pk = pack[r];
+  return make_float4((float)(pk & 0xFFu) * 257.0f, (float)((pk >> 8) & 0xFFu) * 257.0f,
+                     (float)((pk >> 16) & 0xFFu) * 257.0f, 65535.0f);
 }
 
-// info[12] <- R.  An image's first pixel always starts a run, so its run ids
-// are the batch scan minus the scan at that pixel.
-__global__ void k_run_count(const Frame* __restrict__ fr) {
+// Pixel p starts a run: the first of its row, or unlike the pixel before it
+// (rd_octree.cpp:305-313).
+__device__ __forceinline__ bool run_start(const Frame& d, int64_t p, float4* v) {
+  *v = load_any(d.fmt, d.src, p, d.w, d.h);
+  return p % d.w == 0 || !same_px(*v, load_any(d.fmt, d.src, p - 1, d.w, d.h));
+}
+
+__global__ __launch_bounds__(256) void k_run_tiles(const Frame* __restrict__ fr, Map m,
+                                                   int32_t* __restrict__ tcnt) {
+  __shared__ int s_w[4];
+  const TileAt tl = map_tile(m, blockIdx.x);
+  const Frame& d = fr[tl.f];
+  const int64_t p0 = tl.t * kRunTile;
+  int c = 0;
+  for (int k = 0; k < kRunTile / 256; ++k) {
+    const int64_t p = p0 + k * 256 + threadIdx.x;
+    float4 v;
+    if (p < d.N && run_start(d, p, &v)) ++c;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) tcnt[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// info[12] <- R, from the scan over the image's tiles.
+__global__ void k_run_count(const Frame* __restrict__ fr, Map m, const int32_t* __restrict__ tcnt,
+                            const int32_t* __restrict__ toff) {
   const Frame& d = fr[blockIdx.x];
-  if (threadIdx.x == 0) d.info[12] = d.rid[d.N - 1] + d.flags[d.N - 1] - d.rid[0];
+  if (threadIdx.x != 0) return;
+  const int64_t t0 = m.first[blockIdx.x], t1 = m.first[blockIdx.x + 1];
+  d.info[12] = t1 > t0 ? toff[t1 - 1] + tcnt[t1 - 1] - toff[t0] : 0;
 }
 
-__global__ void k_run_starts(const Frame* __restrict__ fr, Map m) {
-  const At a = map_at(m, 256);
-  const Frame& d = fr[a.f];
-  const int64_t p = a.i;
-  if (p >= d.N) return;
-  if (d.flags[p]) d.start[d.rid[p] - d.rid[0]] = (int32_t)p;
+// Every run start of the tile, in pixel order: its start, row, pack and (when
+// kept) pixel.  The starts are ranked 256 pixels at a time.
+__global__ __launch_bounds__(256) void k_run_emit(const Frame* __restrict__ fr, Map m,
+                                                  const int32_t* __restrict__ toff) {
+  __shared__ int s_w[2][4];
+  const TileAt tl = map_tile(m, blockIdx.x);
+  const Frame& d = fr[tl.f];
+  const int64_t p0 = tl.t * kRunTile;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int64_t base = toff[blockIdx.x] - toff[m.first[tl.f]];
+  for (int k = 0; k < kRunTile / 256; ++k) {
+    const int64_t p = p0 + k * 256 + threadIdx.x;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool st = p < d.N && run_start(d, p, &v);
+    const uint64_t bal = __ballot(st);
+    if (lane == 0) s_w[k & 1][wave] = __popcll(bal);
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < 4; ++w) {
+      const int c = s_w[k & 1][w];
+      before += w < wave ? c : 0;
+      all += c;
+    }
+    if (st) {
+      const int64_t r = base + before + __popcll(bal & ((1ull << lane) - 1));
+      double a[4];
+      assoc_pixel(d.assoc != 0, v.x, v.y, v.z, v.w, a);
+      if (!d.px_pack) d.px[r] = v;
+      const uint32_t c0 = scale_char(clamp_pixel(a[0])), c1 = scale_char(clamp_pixel(a[1]));
+      const uint32_t c2 = scale_char(clamp_pixel(a[2])), c3 = scale_char(clamp_pixel(a[3]));
+      d.pack[r] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+      d.start[r] = (int32_t)p;
+      d.row[r] = (int32_t)(p / d.w);
+    }
+    base += all;
+  }
 }
 
-__global__ void k_run_data(const Frame* __restrict__ fr, Map m) {
+__global__ void k_run_len(const Frame* __restrict__ fr, Map m) {
   const At at = map_at(m, 256);
   const Frame& d = fr[at.f];
   const int64_t r = at.i;
   const int64_t R = d.R;
   if (r >= R) return;
-  const int64_t p = d.start[r];
   const int64_t e = (r + 1 < R) ? d.start[r + 1] : d.N;
-  d.count[r] = (int32_t)(e - p);
-  const float4 f = load_any(d.fmt, d.src, p, d.w, d.h);
-  double a[4];
-  assoc_pixel(d.assoc != 0, f.x, f.y, f.z, f.w, a);
-  d.px[r] = f;
-  const uint32_t c0 = scale_char(clamp_pixel(a[0])), c1 = scale_char(clamp_pixel(a[1]));
-  const uint32_t c2 = scale_char(clamp_pixel(a[2])), c3 = scale_char(clamp_pixel(a[3]));
-  d.pack[r] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
-  d.row[r] = (int32_t)(p / d.w);
+  d.count[r] = (int32_t)(e - d.start[r]);
 }
 
 // ---- 2. loop-1 extent -----------------------------------------------------------
@@ -668,7 +739,7 @@ __device__ __forceinline__ void expand_one(const Frame& d, int64_t li, uint32_t 
   pair_decode(p, R1, d.R, depth, &r, &l);
   double mid[4];
   node_mid(d.t.key[node - d.node_base], mid);
-  const float4 f = d.px[r];
+  const float4 f = run_px(d.px, d.pack, d.px_pack, r);
   double a[4];
   assoc_pixel(assoc != 0, f.x, f.y, f.z, f.w, a);
   const double4 v = make_double4(a[0], a[1], a[2], a[3]);
@@ -789,24 +860,6 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
 // bit and level); the lowest such lane then advances the cell's count, so a
 // cell's pairs keep run order.
 constexpr int kDenseCellsMax = 4680;  // 8 + 64 + 512 + 4096
-
-struct TileAt {
-  int32_t f;
-  int64_t t;
-};
-
-// Tile of block b (one block per tile).
-__device__ __forceinline__ TileAt map_tile(const Map& m, int64_t b) {
-  int32_t lo = 0, hi = m.F - 1;
-  while (lo < hi) {
-    const int32_t mid = (lo + hi + 1) >> 1;
-    if (m.first[mid] <= b)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return {lo, b - m.first[lo]};
-}
 
 // Workgroups are dealt to the 8 XCDs in turn; this numbering gives each XCD a
 // contiguous range of tiles, so the segments that consecutive tiles extend are
@@ -1041,7 +1094,6 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
   const uint32_t l1 = (uint32_t)(8 * R1);
   const Terms T = d.T;
   uint32_t* spair = spair_all + d.pair_base;
-  const bool bytes = !assoc && (d.fmt == kRgb8 || d.fmt == kRgba8);
   int off = 0, C = 1;
   for (int l = 1; l <= depth; ++l) {
     C *= kids;
@@ -1092,12 +1144,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
       const int64_t r = r0 + s_run[i];
       const uint32_t pk = pack[r];
       const int32_t cnt = d.count[r];
-      // An 8-bit source read without association: its channels are the packed
-      // bytes times 257 (load_pixel), so only the pack is gathered.
-      const float4 f = bytes ? make_float4((float)(pk & 0xFFu) * 257.0f,
-                                           (float)((pk >> 8) & 0xFFu) * 257.0f,
-                                           (float)((pk >> 16) & 0xFFu) * 257.0f, 65535.0f)
-                             : d.px[r];
+      const float4 f = d.px_pack ? make_float4((float)(pk & 0xFFu) * 257.0f,
+                                               (float)((pk >> 8) & 0xFFu) * 257.0f,
+                                               (float)((pk >> 16) & 0xFFu) * 257.0f, 65535.0f)
+                                 : d.px[r];
       double a[4];
       assoc_pixel(assoc != 0, f.x, f.y, f.z, f.w, a);
       double mid[4] = {kRange / 2.0, kRange / 2.0, kRange / 2.0, kRange / 2.0};
@@ -2613,7 +2663,7 @@ struct RowsArgs {
   const uint32_t* pack;
   const int32_t* row_first;  // first run of each row, row_first[h] = R
   int64_t h;
-  int32_t assoc, kids, maxc, pad;
+  int32_t assoc, kids, maxc, px_pack;  // px_pack: px is rebuilt from the pack (run_px)
   TreeDev t;
   TreeWork W;
   int64_t cap;               // node capacity of t and W
@@ -2643,7 +2693,7 @@ __device__ __forceinline__ int32_t rows_child(const RowsArgs& a, int32_t p, int 
 }
 
 __device__ __forceinline__ void rows_pixel(const RowsArgs& a, int64_t r, double av[4]) {
-  const float4 f = a.px[r];
+  const float4 f = run_px(a.px, a.pack, a.px_pack, r);
   assoc_pixel(a.assoc != 0, f.x, f.y, f.z, f.w, av);
 }
 
@@ -3115,7 +3165,7 @@ namespace {
 // keeps every batch-wide index within 32 bits.
 constexpr int64_t kBatchPixels = (int64_t)1 << 26;
 
-enum MapKind { kMapN, kMapR, kMapP, kMapCap, kMapM256, kMapM64, kMapDesc, kMapDC, kMapDT, kMaps };
+enum MapKind { kMapNT, kMapR, kMapP, kMapCap, kMapM256, kMapM64, kMapDesc, kMapDC, kMapDT, kMaps };
 
 uint32_t pow2_at_least(int64_t v) {
   uint32_t c = 1;
@@ -3253,13 +3303,13 @@ class SubBuild {
   void fill_maps(int64_t* tab) {
     for (int k = 0; k < kMaps; ++k) {
       int64_t* first = tab + (size_t)k * (F_ + 1);
-      const int per = k == kMapM64 ? 64 : k == kMapDT ? 1 : 256;
+      const int per = k == kMapM64 ? 64 : (k == kMapDT || k == kMapNT) ? 1 : 256;
       int64_t run = 0;
       for (int f = 0; f < F_; ++f) {
         const Frame& d = H_[f];
         int64_t n = 0;
         switch (k) {
-          case kMapN: n = d.ok ? d.N : 0; break;
+          case kMapNT: n = d.ok ? (d.N + kRunTile - 1) / kRunTile : 0; break;
           case kMapR: n = d.ok ? d.R : 0; break;
           case kMapP: n = d.P_ins; break;  // failed images keep their pairs in the sort
           case kMapDC: n = d.dense ? d.dcells : 0; break;
@@ -3294,23 +3344,22 @@ class SubBuild {
   template <typename T>
   void fill(T* p, int64_t n, T v);
 
-  // 1. runs: flags, one scan over the batch, runs per image.
+  // 1. runs: run starts per tile, one scan over the batch's tiles, runs per image.
   hipError_t phase_runs() {
     hipError_t e = staging(up_bytes() + rb_bytes(), &pin_);
     if (e != hipSuccess) return e;
-    int64_t sumN = 0;
-    for (int f = 0; f < F_; ++f) sumN += rq_[f].w * rq_[f].h;
-    int32_t *flags = nullptr, *rid = nullptr;
+    int64_t sumT = 0;
+    for (int f = 0; f < F_; ++f) sumT += (rq_[f].w * rq_[f].h + kRunTile - 1) / kRunTile;
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
-    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, flags, rid, (int)sumN, s_);
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, tcnt_, toff_, (int)sumT, s_);
     if (e != hipSuccess) return e;
     char* up = nullptr;
     Carve c;
     c.want(&up, (int64_t)up_bytes());  // descriptors, then the block tables
     c.want(&d_rb_, (int64_t)rb_bytes());
-    c.want(&flags, sumN);
-    c.want(&rid, sumN);
+    c.want(&tcnt_, sumT);
+    c.want(&toff_, sumT);
     c.want((char**)&tmp, (int64_t)tmp_bytes);
     if ((e = c.take(owned_, s_)) != hipSuccess) return e;
     d_fr_ = (Frame*)up;
@@ -3318,7 +3367,6 @@ class SubBuild {
     int32_t* info = (int32_t*)d_rb_;
     double* dinfo = (double*)(d_rb_ + 64 * (size_t)F_);
     B_.ctl = (int32_t*)(d_rb_ + 96 * (size_t)F_);
-    int64_t off = 0;
     for (int f = 0; f < F_; ++f) {
       const OctreeReq& q = rq_[f];
       Frame& d = H_[f];
@@ -3326,6 +3374,7 @@ class SubBuild {
       d.src = q.src;
       d.fmt = q.fmt;
       d.assoc = q.assoc ? 1 : 0;
+      d.px_pack = !d.assoc && (q.fmt == kRgb8 || q.fmt == kRgba8);
       d.maxc = q.maxc <= 0 ? 65536 : std::min(q.maxc, 65536);
       d.depth = octree_depth(d.maxc, d.assoc != 0, q.gray != 0);
       d.kids = d.assoc ? 16 : 8;
@@ -3335,27 +3384,27 @@ class SubBuild {
       d.N = q.w * q.h;
       d.info = info + 16 * f;
       d.dinfo = dinfo + 4 * f;
-      d.flags = flags + off;
-      d.rid = rid + off;
-      off += d.N;
     }
     if ((e = upload()) != hipSuccess) return e;
     k_frame_init<<<F_, 64, 0, s_>>>(d_fr_);
-    k_flags<<<grid_[kMapN], 256, 0, s_>>>(d_fr_, map(kMapN));
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, flags, rid, (int)sumN, s_);
-    if (e != hipSuccess) return e;
-    k_run_count<<<F_, 64, 0, s_>>>(d_fr_);
+    if (grid_[kMapNT]) {
+      k_run_tiles<<<grid_[kMapNT], 256, 0, s_>>>(d_fr_, map(kMapNT), tcnt_);
+      e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, tcnt_, toff_, (int)sumT, s_);
+      if (e != hipSuccess) return e;
+    }
+    k_run_count<<<F_, 64, 0, s_>>>(d_fr_, map(kMapNT), tcnt_, toff_);
     return readback();
   }
 
   // 2. run records and loop 1's extent (one workgroup per image).
   hipError_t phase_extent() {
-    int64_t sumR = 0, sumB = 0;
+    int64_t sumR = 0, sumPx = 0, sumB = 0;
     for (int f = 0; f < F_; ++f) {
       Frame& d = H_[f];
       d.R = hinfo(f)[12];
       d.bcap = pow2_at_least(2 * ((int64_t)d.maxc + 1 + 1024));
       sumR += d.R;
+      sumPx += d.px_pack ? 0 : d.R;
       sumB += d.bcap;
     }
     int32_t *start = nullptr, *count = nullptr, *row = nullptr;
@@ -3366,28 +3415,29 @@ class SubBuild {
     c.want(&start, sumR);
     c.want(&count, sumR);
     c.want(&row, sumR);
-    c.want(&px, sumR);
+    c.want(&px, sumPx);
     c.want(&pack, sumR);
     c.want(&bkeys, sumB);
     c.want(&bfirst, sumB);
     hipError_t e = c.take(owned_, s_);
     if (e != hipSuccess) return e;
-    int64_t ro = 0, bo = 0;
+    int64_t ro = 0, po = 0, bo = 0;
     for (int f = 0; f < F_; ++f) {
       Frame& d = H_[f];
       d.start = start + ro;
       d.count = count + ro;
       d.row = row + ro;
-      d.px = px + ro;
+      d.px = d.px_pack ? nullptr : px + po;
       d.pack = pack + ro;
       d.bkeys = bkeys + bo;
       d.bfirst = bfirst + bo;
       ro += d.R;
+      po += d.px_pack ? 0 : d.R;
       bo += d.bcap;
     }
     if ((e = upload()) != hipSuccess) return e;
-    k_run_starts<<<grid_[kMapN], 256, 0, s_>>>(d_fr_, map(kMapN));  // every image has pixels
-    k_run_data<<<grid_[kMapR], 256, 0, s_>>>(d_fr_, map(kMapR));    // and runs
+    if (grid_[kMapNT]) k_run_emit<<<grid_[kMapNT], 256, 0, s_>>>(d_fr_, map(kMapNT), toff_);
+    if (grid_[kMapR]) k_run_len<<<grid_[kMapR], 256, 0, s_>>>(d_fr_, map(kMapR));
     fill(bkeys, sumB, kEmpty64);
     fill(bfirst, sumB, kEmpty32);
     k_break<<<F_, 1024, 0, s_>>>(d_fr_);
@@ -3849,6 +3899,7 @@ class SubBuild {
     a.assoc = d.assoc;
     a.kids = d.kids;
     a.maxc = d.maxc;
+    a.px_pack = d.px_pack;
     a.info = d.info;
     a.dinfo = d.dinfo;
     int32_t *row_first = nullptr, *lvl_off = nullptr;
@@ -3921,6 +3972,8 @@ class SubBuild {
   }
 
   int64_t sumP_ = 0;
+  int32_t* tcnt_ = nullptr;        // run starts per tile of kRunTile pixels
+  int32_t* toff_ = nullptr;        // their exclusive scan over the batch
   int32_t* slot_node_ = nullptr;
   uint32_t* pair_slot_ = nullptr;  // batch pair array: slot of an interval's first pair
   int32_t* wgc_ = nullptr;         // intervals per workgroup of k_node_insert
