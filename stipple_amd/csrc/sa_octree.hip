@@ -1051,16 +1051,19 @@ __device__ __forceinline__ uint32_t dense_cell(uint32_t pk, int l, int nch, uint
   return cell;
 }
 
-// Every loop-2 pair of a tile at its sorted position, one level at a time.
-// The tile's runs are first put in the level's order inside the tile (cells
-// ascending, runs ascending within a cell): each wavefront counts the cells of
-// its quarter of the tile, a scan over cells gives every cell's first slot and
-// each wavefront's first slot in it, and a second walk ranks the runs into a
+// Every loop-2 pair of a tile at its sorted position.  The tile's runs are
+// first put in each level's order inside the tile (cells ascending, runs
+// ascending within a cell): each wavefront counts the cells of its quarter of
+// the tile, a scan over cells gives every cell's first slot and each
+// wavefront's first slot in it, and a second walk ranks the runs into a
 // slot -> run map.  The pairs are then written slot by slot, so consecutive
 // threads write consecutive positions of one cell's segment (the cell's first
 // global position for this tile minus its first slot, plus the slot).  The
 // values are expand_one's, the cube midpoints formed level by level as the
-// classification descends.  LDS: kDenseLds(T, cells of the deepest level).
+// classification descends; a run's count is written for leaf pairs only (the
+// only ones whose count is read).
+// Levels are taken one at a time, so the LDS holds the counters of one
+// level's cells and one map: dense_scatter_lds.
 constexpr int kScatterThreads = 256;
 
 __host__ __device__ constexpr int64_t dense_scatter_lds(int64_t tile, int64_t cmax) {
@@ -1077,73 +1080,54 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
   const int depth = d.depth, kids = d.kids, assoc = d.assoc, nch = assoc ? 4 : 3;
   int cmax = 1;
   for (int l = 1; l <= depth; ++l) cmax *= kids;
+  const int dcells = d.dcells;
   const int64_t tile = (int64_t)1 << d.dtshift;
   const int64_t R1 = d.R1, n2 = d.R - d.R1;
   const int64_t r0 = R1 + tl.t * tile, r1 = min(d.R, r0 + tile);
   const int n = (int)(r1 - r0);
-  int32_t* s_delta = s_dyn;                          // cmax
-  uint16_t* s_cnt = (uint16_t*)(s_dyn + cmax);       // 4 x cmax
-  uint16_t* s_run = s_cnt + 4 * cmax;                // tile
+  const int ncell = cmax;                             // counters of one level
+  int32_t* s_delta = s_dyn;                           // ncell
+  uint16_t* s_cnt = (uint16_t*)(s_dyn + ncell);       // 4 x ncell
+  uint16_t* s_run = s_cnt + 4 * ncell;                // tile
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t quarter = tile / 4;
   const int64_t wr0 = r0 + wave * quarter, wr1 = min(r1, wr0 + quarter);
   const uint64_t below = (1ull << lane) - 1;
   const uint32_t* pack = d.pack;
-  const int32_t* tab = d.dtab + tl.t * (int64_t)d.dcells;
+  const int32_t* tab = d.dtab + tl.t * (int64_t)dcells;
   const uint32_t* dpos = d.dpos + d.dbase;
   const uint32_t l1 = (uint32_t)(8 * R1);
   const Terms T = d.T;
   uint32_t* spair = spair_all + d.pair_base;
-  int off = 0, C = 1;
-  for (int l = 1; l <= depth; ++l) {
-    C *= kids;
-    uint16_t* cw = s_cnt + wave * C;
-    for (int c = tid; c < 4 * C; c += kScatterThreads) s_cnt[c] = 0;
-    __syncthreads();
-    for (int64_t rb = wr0; rb < wr1; rb += 64) {  // count this quarter's cells
-      const int64_t r = rb + lane;
-      const bool valid = r < wr1;
-      uint64_t msk = __ballot(valid);
-      const uint32_t cell = dense_cell(valid ? pack[r] : 0u, l, nch, &msk);
-      if (valid && (msk & below) == 0) cw[cell] += (uint16_t)__popcll(msk);
-    }
-    __syncthreads();
-    {  // first slots: per cell, per wavefront in it; the slot -> position offsets
-      const int per = (C + kScatterThreads - 1) / kScatterThreads;
-      const int cb = min(C, tid * per), ce = min(C, cb + per);
-      int sum = 0;
-      for (int c = cb; c < ce; ++c)
-        sum += s_cnt[c] + s_cnt[C + c] + s_cnt[2 * C + c] + s_cnt[3 * C + c];
-      int run = block_excl_256(sum, s_part);
-      for (int c = cb; c < ce; ++c) {
-        int at = run;
-        for (int w = 0; w < 4; ++w) {
-          const int v = s_cnt[w * C + c];
-          s_cnt[w * C + c] = (uint16_t)at;
-          at += v;
-        }
-        if (at > run) s_delta[c] = (int32_t)((int64_t)dpos[off + c] - d.pair_base) + tab[off + c] - run;
-        run = at;
+  uint16_t* cw = s_cnt + wave * ncell;
+
+  // Counters of cells [0, nc) (flat offset foff): per wavefront, then first
+  // slots.
+  auto starts = [&](int nc, int foff) {
+    const int per = (nc + kScatterThreads - 1) / kScatterThreads;
+    const int cb = min(nc, tid * per), ce = min(nc, cb + per);
+    int sum = 0;
+    for (int c = cb; c < ce; ++c)
+      sum += s_cnt[c] + s_cnt[ncell + c] + s_cnt[2 * ncell + c] + s_cnt[3 * ncell + c];
+    int run = block_excl_256(sum, s_part);
+    for (int c = cb; c < ce; ++c) {
+      int at = run;
+      for (int w = 0; w < 4; ++w) {
+        const int v = s_cnt[w * ncell + c];
+        s_cnt[w * ncell + c] = (uint16_t)at;
+        at += v;
       }
-    }
-    __syncthreads();
-    for (int64_t rb = wr0; rb < wr1; rb += 64) {  // rank: the slot -> run map
-      const int64_t r = rb + lane;
-      const bool valid = r < wr1;
-      uint64_t msk = __ballot(valid);
-      const uint32_t cell = dense_cell(valid ? pack[r] : 0u, l, nch, &msk);
-      if (valid) {
-        const uint64_t lo = msk & below;
-        const int base = cw[cell];
-        s_run[base + __popcll(lo)] = (uint16_t)(r - r0);
-        if (lo == 0) cw[cell] = (uint16_t)(base + __popcll(msk));
+      if (at > run) {
+        s_delta[c] = (int32_t)((int64_t)dpos[foff + c] - d.pair_base) + tab[foff + c] - run;
       }
+      run = at;
     }
-    __syncthreads();
-    for (int i = tid; i < n; i += kScatterThreads) {  // write in slot order
-      const int64_t r = r0 + s_run[i];
+  };
+  // The pairs of level l in slot order from the map.
+  auto emit = [&](int l, const uint16_t* map) {
+    for (int i = tid; i < n; i += kScatterThreads) {
+      const int64_t r = r0 + map[i];
       const uint32_t pk = pack[r];
-      const int32_t cnt = d.count[r];
       const float4 f = d.px_pack ? make_float4((float)(pk & 0xFFu) * 257.0f,
                                                (float)((pk >> 8) & 0xFFu) * 257.0f,
                                                (float)((pk >> 16) & 0xFFu) * 257.0f, 65535.0f)
@@ -1165,6 +1149,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
           mid[c] += bit ? bisect : -bisect;
         }
       }
+      const int32_t cnt = d.count[r];
       const int64_t li = s_delta[cell] + i;
       const double e0 = kScale * (a[0] - mid[0]);
       const double e1 = kScale * (a[1] - mid[1]);
@@ -1174,11 +1159,43 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
       if (dd != dd) dd = 0.0;
       spair[li] = l1 + (uint32_t)((int64_t)(l - 1) * n2 + (r - R1));
       T.term[li] = (double)cnt * sqrt(dd);
-      T.lcnt[li] = cnt;
-      if (l == depth)
+      if (l == depth) {
+        T.lcnt[li] = cnt;
         T.lval[li] = make_float4(clamp_pixel(a[0]), clamp_pixel(a[1]), clamp_pixel(a[2]),
                                  clamp_pixel(assoc ? a[3] : kRange));
+      }
     }
+  };
+
+  int off = 0, C = 1;
+  for (int l = 1; l <= depth; ++l) {  // one level at a time
+    C *= kids;
+    for (int c = tid; c < 4 * ncell; c += kScatterThreads) s_cnt[c] = 0;
+    __syncthreads();
+    for (int64_t rb = wr0; rb < wr1; rb += 64) {  // count this quarter's cells
+      const int64_t r = rb + lane;
+      const bool valid = r < wr1;
+      uint64_t msk = __ballot(valid);
+      const uint32_t cell = dense_cell(valid ? pack[r] : 0u, l, nch, &msk);
+      if (valid && (msk & below) == 0) cw[cell] += (uint16_t)__popcll(msk);
+    }
+    __syncthreads();
+    starts(C, off);
+    __syncthreads();
+    for (int64_t rb = wr0; rb < wr1; rb += 64) {  // rank: the slot -> run map
+      const int64_t r = rb + lane;
+      const bool valid = r < wr1;
+      uint64_t msk = __ballot(valid);
+      const uint32_t cell = dense_cell(valid ? pack[r] : 0u, l, nch, &msk);
+      if (valid) {
+        const uint64_t lo = msk & below;
+        const int base = cw[cell];
+        s_run[base + __popcll(lo)] = (uint16_t)(r - r0);
+        if (lo == 0) cw[cell] = (uint16_t)(base + __popcll(msk));
+      }
+    }
+    __syncthreads();
+    emit(l, s_run);
     __syncthreads();
     off += C;
   }
