@@ -1055,19 +1055,21 @@ __device__ __forceinline__ uint32_t dense_cell(uint32_t pk, int l, int nch, uint
 // first put in each level's order inside the tile (cells ascending, runs
 // ascending within a cell): each wavefront counts the cells of its quarter of
 // the tile, a scan over cells gives every cell's first slot and each
-// wavefront's first slot in it, and a second walk ranks the runs into a
-// slot -> run map.  The pairs are then written slot by slot, so consecutive
+// wavefront's first slot in it, and a second walk places the runs into a
+// slot -> run map.  The first walk keeps each run's rank among the lanes of its
+// round with its cell, and whether it is the last of them, so the second walk
+// needs no ballots.  The pairs are then written slot by slot, so consecutive
 // threads write consecutive positions of one cell's segment (the cell's first
 // global position for this tile minus its first slot, plus the slot).  The
 // values are expand_one's, the cube midpoints formed level by level as the
 // classification descends; a run's count is written for leaf pairs only (the
 // only ones whose count is read).
 // Levels are taken one at a time, so the LDS holds the counters of one
-// level's cells and one map: dense_scatter_lds.
+// level's cells, one map and one rank byte per run: dense_scatter_lds.
 constexpr int kScatterThreads = 256;
 
 __host__ __device__ constexpr int64_t dense_scatter_lds(int64_t tile, int64_t cmax) {
-  return 4 * cmax + 2 * 4 * cmax + 2 * tile;
+  return 4 * cmax + 2 * 4 * cmax + 2 * tile + tile;
 }
 
 __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* __restrict__ fr,
@@ -1089,6 +1091,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
   int32_t* s_delta = s_dyn;                           // ncell
   uint16_t* s_cnt = (uint16_t*)(s_dyn + ncell);       // 4 x ncell
   uint16_t* s_run = s_cnt + 4 * ncell;                // tile
+  uint8_t* s_rk = (uint8_t*)(s_run + tile);           // tile: rank in round | last << 6
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t quarter = tile / 4;
   const int64_t wr0 = r0 + wave * quarter, wr1 = min(r1, wr0 + quarter);
@@ -1100,6 +1103,14 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
   const Terms T = d.T;
   uint32_t* spair = spair_all + d.pair_base;
   uint16_t* cw = s_cnt + wave * ncell;
+  // This wavefront's runs, loaded once: run wr0 + 64 k + lane in pk[k].
+  constexpr int kMaxRounds = 32;  // a quarter of the largest tile (8192) over 64 lanes
+  uint32_t pk[kMaxRounds];
+#pragma unroll
+  for (int k = 0; k < kMaxRounds; ++k) {
+    const int64_t r = wr0 + 64 * k + lane;
+    pk[k] = r < wr1 ? pack[r] : 0u;
+  }
 
   // Counters of cells [0, nc) (flat offset foff): per wavefront, then first
   // slots.
@@ -1134,20 +1145,16 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
                                  : d.px[r];
       double a[4];
       assoc_pixel(assoc != 0, f.x, f.y, f.z, f.w, a);
-      double mid[4] = {kRange / 2.0, kRange / 2.0, kRange / 2.0, kRange / 2.0};
-      double bisect = (kRange + 1.0) / 2.0;
+      // The cube midpoint, 32767.5 +- 2^14 +- ... +- 2^(15-l) by the channel's
+      // top l bits B, is (B + 1/2) 2^(16-l) - 1/2: every partial sum of the
+      // reference's loop is a half-integer below 2^17, so both are exact.
+      const double step = (double)(1 << (16 - l));
+      double mid[4];
       uint32_t cell = 0;
-      for (int k = 1; k <= l; ++k) {
-        bisect *= 0.5;
-        for (int c = 0; c < 4; ++c) {
-          uint32_t bit = 0;
-          if (c < nch) {
-            const uint32_t top = ((pk >> (8 * c)) & 0xFFu) >> (8 - k);
-            bit = top & 1u;
-            if (k == l) cell |= top << (l * c);
-          }
-          mid[c] += bit ? bisect : -bisect;
-        }
+      for (int c = 0; c < 4; ++c) {
+        const uint32_t top = c < nch ? ((pk >> (8 * c)) & 0xFFu) >> (8 - l) : 0u;
+        cell |= c < nch ? top << (l * c) : 0u;
+        mid[c] = ((double)top + 0.5) * step - 0.5;
       }
       const int32_t cnt = d.count[r];
       const int64_t li = s_delta[cell] + i;
@@ -1172,26 +1179,37 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
     C *= kids;
     for (int c = tid; c < 4 * ncell; c += kScatterThreads) s_cnt[c] = 0;
     __syncthreads();
-    for (int64_t rb = wr0; rb < wr1; rb += 64) {  // count this quarter's cells
+#pragma unroll
+    for (int k = 0; k < kMaxRounds; ++k) {  // count this quarter's cells
+      const int64_t rb = wr0 + 64 * k;
+      if (rb >= wr1) break;
       const int64_t r = rb + lane;
       const bool valid = r < wr1;
       uint64_t msk = __ballot(valid);
-      const uint32_t cell = dense_cell(valid ? pack[r] : 0u, l, nch, &msk);
-      if (valid && (msk & below) == 0) cw[cell] += (uint16_t)__popcll(msk);
+      const uint32_t cell = dense_cell(pk[k], l, nch, &msk);
+      if (valid) {
+        const uint64_t lo = msk & below;
+        const bool last = (msk >> lane) == 1ull;
+        s_rk[r - r0] = (uint8_t)(__popcll(lo) | (last ? 64 : 0));
+        if (lo == 0) cw[cell] += (uint16_t)__popcll(msk);
+      }
     }
     __syncthreads();
     starts(C, off);
     __syncthreads();
-    for (int64_t rb = wr0; rb < wr1; rb += 64) {  // rank: the slot -> run map
+#pragma unroll
+    for (int k = 0; k < kMaxRounds; ++k) {  // the slot -> run map
+      const int64_t rb = wr0 + 64 * k;
+      if (rb >= wr1) break;
       const int64_t r = rb + lane;
-      const bool valid = r < wr1;
-      uint64_t msk = __ballot(valid);
-      const uint32_t cell = dense_cell(valid ? pack[r] : 0u, l, nch, &msk);
-      if (valid) {
-        const uint64_t lo = msk & below;
-        const int base = cw[cell];
-        s_run[base + __popcll(lo)] = (uint16_t)(r - r0);
-        if (lo == 0) cw[cell] = (uint16_t)(base + __popcll(msk));
+      if (r < wr1) {
+        uint32_t cell = 0;
+        for (int c = 0; c < nch; ++c)
+          cell |= (((pk[k] >> (8 * c)) & 0xFFu) >> (8 - l)) << (l * c);
+        const int rk = s_rk[r - r0];
+        const int slot = cw[cell] + (rk & 63);
+        s_run[slot] = (uint16_t)(r - r0);
+        if (rk & 64) cw[cell] = (uint16_t)(slot + 1);  // the round's last of its cell
       }
     }
     __syncthreads();

@@ -76,3 +76,19 @@ def test_qsort_with_quantize_error_compare_sorts_ascending():
             a = np.ascontiguousarray(v.copy())
             libc.qsort(a.ctypes.data, n, 8, cmp)
             assert np.array_equal(a.view(np.uint64), np.sort(v).view(np.uint64)), (n, trial)
+
+
+def test_cube_midpoint_closed_form_equals_the_descent():
+    """The cube midpoint the classification forms level by level
+    (src/rd_octree.cpp:318-329: mid = 32767.5, bisect = 32768, then per level
+    bisect *= 0.5 and mid +-= bisect) equals (B + 1/2) 2^(16-l) - 1/2 for the
+    channel's top l bits B, bit for bit, at every level and every B (the
+    device's cell placement uses the closed form)."""
+    for level in range(1, 9):
+        for b in range(1 << level):
+            mid, bisect = 65535.0 / 2.0, (65535.0 + 1.0) / 2.0
+            for k in range(1, level + 1):
+                bisect *= 0.5
+                mid += bisect if (b >> (level - k)) & 1 else -bisect
+            closed = (float(b) + 0.5) * float(1 << (16 - level)) - 0.5
+            assert mid == closed, (level, b)
