@@ -445,10 +445,16 @@ __global__ __launch_bounds__(1024) void k_break(const Frame* __restrict__ fr) {
     } else {
       const int32_t y1 = row[s_hit];
       info[0] = y1;
-      // First run of row y1+1.
-      int64_t r = s_hit;
-      while (r < R && row[r] == y1) ++r;
-      info[1] = (int32_t)r;
+      // First run of row y1+1: row[] does not decrease, so a binary search.
+      int64_t lo = s_hit, hi = R;
+      while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (row[mid] <= y1)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      info[1] = (int32_t)lo;
     }
   }
 }
@@ -573,25 +579,28 @@ __global__ __launch_bounds__(256) void k_node_insert(const Frame* __restrict__ f
 
 // Node entries (first pair, slot) of every image into one list; sorting it by
 // pair_base + first pair orders the batch by image, then by creation.
-// A workgroup's threads belong to one image, so each wavefront takes its
+// A workgroup's threads belong to one image, so the workgroup takes its
 // entries' places with one atomic.
-__global__ void k_node_collect(const Frame* __restrict__ fr, Map m, Batch B) {
+__global__ __launch_bounds__(256) void k_node_collect(const Frame* __restrict__ fr, Map m, Batch B) {
+  __shared__ int s_w[4];
+  __shared__ int s_base;
   const At at = map_at(m, 256);
   const Frame& d = fr[at.f];
   const int64_t s = at.i;
   const bool used = s < d.ncap && d.nkeys[s] != kEmpty64;
   const uint64_t bal = __ballot(used);
-  if (!bal) return;
-  const int lane = threadIdx.x & 63;
-  const int lead = __ffsll((unsigned long long)bal) - 1;
-  int base = 0;
-  if (lane == lead) {
-    base = atomicAdd(&B.ctl[0], __popcll(bal));
-    atomicAdd(&d.info[13], __popcll(bal));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_w[wave] = __popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int all = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    s_base = all ? atomicAdd(&B.ctl[0], all) : 0;
+    if (all) atomicAdd(&d.info[13], all);
   }
-  base = __shfl(base, lead, 64);
+  __syncthreads();
   if (!used) return;
-  const int k = base + __popcll(bal & ((1ull << lane) - 1));
+  int k = s_base + __popcll(bal & ((1ull << lane) - 1));
+  for (int w = 0; w < wave; ++w) k += s_w[w];
   B.ekey[k] = (uint32_t)d.pair_base + d.nfirst[s];
   B.eslot[k] = (uint32_t)s;
 }
