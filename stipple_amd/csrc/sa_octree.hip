@@ -273,8 +273,8 @@ __global__ void k_frame_init(const Frame* __restrict__ fr) {
 // Runs are found a tile of kRunTile pixels at a time (one 256-thread workgroup
 // per tile; a tile never spans two images): k_run_tiles counts the run starts
 // of every tile, an exclusive scan over the batch's tiles gives each tile's
-// first run, and k_run_emit writes every run's record; k_run_len takes the
-// lengths from consecutive starts.
+// first run, and k_run_emit writes every run's record, its length too unless
+// it is the tile's last run, whose length k_run_tail takes from the next start.
 constexpr int kRunTile = 4096;
 
 // The run's pixel as loaded.  An 8-bit source read without association keeps
@@ -288,11 +288,19 @@ __device__ __forceinline__ float4 run_px(const float4* px, const uint32_t* pack,
                      (float)((pk >> 16) & 0xFFu) * 257.0f, 65535.0f);
 }
 
-// Pixel p starts a run: the first of its row, or unlike the pixel before it
-// (rd_octree.cpp:305-313).
-__device__ __forceinline__ bool run_start(const Frame& d, int64_t p, float4* v) {
-  *v = load_any(d.fmt, d.src, p, d.w, d.h);
-  return p % d.w == 0 || !same_px(*v, load_any(d.fmt, d.src, p - 1, d.w, d.h));
+// Pixel p (consecutive in the wavefront's lanes) starts a run: the first of
+// its row, or unlike the pixel before it (rd_octree.cpp:305-313), which the
+// lane below holds (lane 0 loads it).  Every lane of the wavefront calls it.
+__device__ __forceinline__ bool run_start(const Frame& d, int64_t p, bool valid, float4* v) {
+  *v = valid ? load_any(d.fmt, d.src, p, d.w, d.h) : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 u;
+  u.x = __shfl_up(v->x, 1, 64);
+  u.y = __shfl_up(v->y, 1, 64);
+  u.z = __shfl_up(v->z, 1, 64);
+  u.w = __shfl_up(v->w, 1, 64);
+  const bool row0 = !valid || p % d.w == 0;
+  if ((threadIdx.x & 63) == 0 && !row0) u = load_any(d.fmt, d.src, p - 1, d.w, d.h);
+  return valid && (row0 || !same_px(*v, u));
 }
 
 __global__ __launch_bounds__(256) void k_run_tiles(const Frame* __restrict__ fr, Map m,
@@ -305,7 +313,7 @@ __global__ __launch_bounds__(256) void k_run_tiles(const Frame* __restrict__ fr,
   for (int k = 0; k < kRunTile / 256; ++k) {
     const int64_t p = p0 + k * 256 + threadIdx.x;
     float4 v;
-    if (p < d.N && run_start(d, p, &v)) ++c;
+    if (run_start(d, p, p < d.N, &v)) ++c;
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
   if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
@@ -322,28 +330,41 @@ __global__ void k_run_count(const Frame* __restrict__ fr, Map m, const int32_t* 
   d.info[12] = t1 > t0 ? toff[t1 - 1] + tcnt[t1 - 1] - toff[t0] : 0;
 }
 
-// Every run start of the tile, in pixel order: its start, row, pack and (when
-// kept) pixel.  The starts are ranked 256 pixels at a time.
+// Every run start of the tile, in pixel order: its start, row, pack, (when
+// kept) pixel, and its length when the next start is in the tile.  The starts
+// are ranked 256 pixels at a time; a start whose successor is not in its
+// chunk is resolved by the next chunk with a start (thread 0 holds it).
 __global__ __launch_bounds__(256) void k_run_emit(const Frame* __restrict__ fr, Map m,
                                                   const int32_t* __restrict__ toff) {
-  __shared__ int s_w[2][4];
+  __shared__ int s_w[2][4];   // starts per wavefront
+  __shared__ int s_fl[2][4];  // first start's lane (64: none)
+  __shared__ int s_ll[2][4];  // last start's lane (-1: none)
   const TileAt tl = map_tile(m, blockIdx.x);
   const Frame& d = fr[tl.f];
   const int64_t p0 = tl.t * kRunTile;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int64_t base = toff[blockIdx.x] - toff[m.first[tl.f]];
+  int64_t pend_r = -1, pend_p = 0;  // thread 0: the start waiting for its successor
   for (int k = 0; k < kRunTile / 256; ++k) {
-    const int64_t p = p0 + k * 256 + threadIdx.x;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    const bool st = p < d.N && run_start(d, p, &v);
+    const int64_t pc = p0 + k * 256;
+    const int64_t p = pc + threadIdx.x;
+    float4 v;
+    const bool st = run_start(d, p, p < d.N, &v);
     const uint64_t bal = __ballot(st);
-    if (lane == 0) s_w[k & 1][wave] = __popcll(bal);
+    const int b = k & 1;
+    if (lane == 0) {
+      s_w[b][wave] = __popcll(bal);
+      s_fl[b][wave] = bal ? __ffsll((unsigned long long)bal) - 1 : 64;
+      s_ll[b][wave] = bal ? 63 - __clzll((long long)bal) : -1;
+    }
     __syncthreads();
-    int before = 0, all = 0;
+    int before = 0, all = 0, first = 256, last = -1;
     for (int w = 0; w < 4; ++w) {
-      const int c = s_w[k & 1][w];
+      const int c = s_w[b][w];
       before += w < wave ? c : 0;
       all += c;
+      if (first == 256 && s_fl[b][w] < 64) first = 64 * w + s_fl[b][w];
+      if (s_ll[b][w] >= 0) last = 64 * w + s_ll[b][w];
     }
     if (st) {
       const int64_t r = base + before + __popcll(bal & ((1ull << lane) - 1));
@@ -355,18 +376,38 @@ __global__ __launch_bounds__(256) void k_run_emit(const Frame* __restrict__ fr, 
       d.pack[r] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
       d.start[r] = (int32_t)p;
       d.row[r] = (int32_t)(p / d.w);
+      int nxt = 256;  // the next start in this chunk
+      const uint64_t above = bal & ~((2ull << lane) - 1);
+      if (above) {
+        nxt = 64 * wave + __ffsll((unsigned long long)above) - 1;
+      } else {
+        for (int w = wave + 1; w < 4; ++w)
+          if (s_fl[b][w] < 64) {
+            nxt = 64 * w + s_fl[b][w];
+            break;
+          }
+      }
+      if (nxt < 256) d.count[r] = nxt - (int)threadIdx.x;
+    }
+    if (threadIdx.x == 0 && first < 256) {
+      if (pend_r >= 0) d.count[pend_r] = (int32_t)(pc + first - pend_p);
+      pend_r = base + all - 1;
+      pend_p = pc + last;
     }
     base += all;
   }
 }
 
-__global__ void k_run_len(const Frame* __restrict__ fr, Map m) {
-  const At at = map_at(m, 256);
-  const Frame& d = fr[at.f];
-  const int64_t r = at.i;
-  const int64_t R = d.R;
-  if (r >= R) return;
-  const int64_t e = (r + 1 < R) ? d.start[r + 1] : d.N;
+// The length of each tile's last run: up to the next run's start, or to the
+// end of the image.
+__global__ void k_run_tail(const Frame* __restrict__ fr, Map m, const int32_t* __restrict__ tcnt,
+                           const int32_t* __restrict__ toff, int64_t nt) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nt || tcnt[b] == 0) return;
+  const TileAt tl = map_tile(m, b);
+  const Frame& d = fr[tl.f];
+  const int64_t r = toff[b] + tcnt[b] - 1 - toff[m.first[tl.f]];
+  const int64_t e = r + 1 < d.R ? d.start[r + 1] : d.N;
   d.count[r] = (int32_t)(e - d.start[r]);
 }
 
@@ -3484,8 +3525,11 @@ class SubBuild {
       bo += d.bcap;
     }
     if ((e = upload()) != hipSuccess) return e;
-    if (grid_[kMapNT]) k_run_emit<<<grid_[kMapNT], 256, 0, s_>>>(d_fr_, map(kMapNT), toff_);
-    if (grid_[kMapR]) k_run_len<<<grid_[kMapR], 256, 0, s_>>>(d_fr_, map(kMapR));
+    if (grid_[kMapNT]) {
+      k_run_emit<<<grid_[kMapNT], 256, 0, s_>>>(d_fr_, map(kMapNT), toff_);
+      k_run_tail<<<blocks(grid_[kMapNT], 256), 256, 0, s_>>>(d_fr_, map(kMapNT), tcnt_, toff_,
+                                                            grid_[kMapNT]);
+    }
     fill(bkeys, sumB, kEmpty64);
     fill(bfirst, sumB, kEmpty32);
     k_break<<<F_, 1024, 0, s_>>>(d_fr_);
