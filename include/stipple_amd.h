@@ -390,8 +390,9 @@ int sa_chain_run(sa_chain chain, void* stream, char* err, size_t errlen);
 int sa_chain_destroy(sa_chain chain);
 
 /* ---- tuning knobs -------------------------------------------------------
- * Launch shapes an autotuner may set (stipple_amd/tune.py measures them);
- * no knob changes a result.  New, with no counterpart in the reference.
+ * Launch shapes and code paths an autotuner or a test may set
+ * (stipple_amd/tune.py measures the first three); no knob changes a palette or
+ * an output.  New, with no counterpart in the reference.
  *   "blocks.threads"  block walk: Hilbert blocks per workgroup (64..512; 256)
  *   "approx.per"      Jacobi sweep: outputs per thread (2, 4, 8; 4)
  *   "seq.memo_bits"   sequential walk: a cap of 2^value LDS memo slots (4..10;
@@ -399,6 +400,10 @@ int sa_chain_destroy(sa_chain chain);
  *   "octree.cells"    palette build: 1 places loop 2's pairs by a counting sort
  *                     over cells when the cube depth leaves at most 4680 cells,
  *                     0 sends every pair through the node hash (1)
+ *   "octree.exact_root" palette build: 1 sums the root's quantize_error serially
+ *                     always (the palette is the same either way; only the
+ *                     tree's root value becomes exact), 0 only when it could
+ *                     matter (0)
  * Calls made after a set use the new value; chains keep the value they were
  * created with. */
 int sa_tune_set(const char* knob, int value, char* err, size_t errlen);

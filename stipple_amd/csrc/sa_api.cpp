@@ -2046,6 +2046,10 @@ extern "C" int sa_tune_set(const char* knob, int value, char* err, size_t errlen
   } else if (!std::strcmp(knob, "octree.cells")) {
     if (value != 0 && value != 1) return fail(err, errlen, SA_EINVAL, "octree.cells must be 0 or 1");
     sa::octree_set_cells(value);
+  } else if (!std::strcmp(knob, "octree.exact_root")) {
+    if (value != 0 && value != 1)
+      return fail(err, errlen, SA_EINVAL, "octree.exact_root must be 0 or 1");
+    sa::octree_set_exact_root(value);
   } else {
     return fail(err, errlen, SA_EINVAL, "unknown knob '%s'", knob);
   }
@@ -2058,6 +2062,7 @@ extern "C" int sa_tune_get(const char* knob, int* value) {
   else if (!std::strcmp(knob, "approx.per")) *value = g_knobs.approx_per;
   else if (!std::strcmp(knob, "seq.memo_bits")) *value = g_knobs.seq_memo_bits;
   else if (!std::strcmp(knob, "octree.cells")) *value = sa::octree_cells();
+  else if (!std::strcmp(knob, "octree.exact_root")) *value = sa::octree_exact_root();
   else return SA_EINVAL;
   return SA_OK;
 }

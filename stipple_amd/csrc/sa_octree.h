@@ -80,5 +80,10 @@ void build_octree_batch(const OctreeReq* req, int n, hipStream_t s, OctreeOut* o
 // node hash and the interval sort.  Both give the same trees.
 void octree_set_cells(int on);
 int octree_cells();
+// 1: the root's quantize_error is always summed serially (exact); 0 (the
+// default): only when the rounding bound of the parallel sum cannot tell it
+// from the other nodes' errors, which is when its exact value could matter.
+void octree_set_exact_root(int on);
+int octree_exact_root();
 
 }  // namespace sa

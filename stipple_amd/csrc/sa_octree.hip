@@ -156,6 +156,7 @@ struct Frame {
   int32_t fmt, assoc, depth, kids;
   int32_t maxc;
   int32_t ok;              // 0 once the image has failed: later phases skip it
+  int32_t exact_root;      // 1: the root's chain is summed serially whatever the bound
   int64_t w, h, N;
   int32_t* info;           // 16 ints, layout in sa_octree.h
   double* dinfo;           // [0] parallel root chain, [1] max non-root qe, [2] root qe used
@@ -204,6 +205,7 @@ struct Frame {
   int32_t* dtot;           // per cell: its loop-2 pairs
   uint32_t* dslot;         // per cell: the hash slot of its node
   const uint32_t* dpos;    // per cell: first sorted position (batch-wide) of its loop-2 pairs
+  int32_t* l2s;            // per node: first sorted position of its loop-2 pairs (INT32_MAX: none)
 };
 
 // Batch-wide arrays and counters.
@@ -770,11 +772,13 @@ __global__ void k_interval_ends(const Frame* __restrict__ fr, int F,
   const int64_t b = spos[j], e = b + slen[j];
   for (int64_t t = (b + kExpandTile - 1) / kExpandTile; t * kExpandTile < e; ++t)
     tile_first[t] = (int32_t)j;
-  if (sival[j] >= (uint64_t)ni) dpos[sival[j] - ni] = (uint32_t)b;
+  const bool cell = sival[j] >= (uint64_t)ni;
+  if (cell) dpos[sival[j] - ni] = (uint32_t)b;
   const bool first = j == 0 || sikey[j - 1] != n, last = j + 1 == nt || sikey[j + 1] != n;
-  if (!first && !last) return;
+  if (!first && !last && !cell) return;
   const Frame& d = fr[image_of_pair(fr, F, b)];
   if (!d.ok) return;
+  if (cell) d.l2s[n - d.node_base] = (int32_t)(b - d.pair_base);
   if (first) d.seg_b[n - d.node_base] = (int32_t)(b - d.pair_base);
   if (last) d.seg_e[n - d.node_base] = (int32_t)(e - d.pair_base);
 }
@@ -938,28 +942,40 @@ __global__ __launch_bounds__(64) void k_dense_count(const Frame* __restrict__ fr
   const int depth = d.depth, kids = d.kids, nch = d.assoc ? 4 : 3;
   const uint64_t below = (1ull << lane) - 1;
   const uint32_t* pack = d.pack;
-  for (int64_t rb = r0; rb < r1; rb += 64) {
-    const int64_t r = rb + lane;
-    const bool valid = r < r1;
-    const uint32_t pk = valid ? pack[r] : 0u;
-    uint64_t msk = __ballot(valid);
-    int off = 0, pw = 1;
-    for (int l = 1; l <= depth; ++l) {
-      uint32_t cell = 0;
-      for (int c = 0; c < nch; ++c) {
-        const uint32_t top = ((pk >> (8 * c)) & 0xFFu) >> (8 - l);
-        const uint64_t v = __ballot((top & 1u) != 0);
-        msk &= (top & 1u) ? v : ~v;
-        cell |= top << (l * c);
+  constexpr int kAhead = 8;  // rounds whose runs are loaded together
+  for (int64_t rg = r0; rg < r1; rg += 64 * kAhead) {
+    uint32_t pkg[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int64_t r = rg + 64 * k + lane;
+      pkg[k] = r < r1 ? pack[r] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int64_t rb = rg + 64 * k;
+      if (rb >= r1) break;
+      const int64_t r = rb + lane;
+      const bool valid = r < r1;
+      const uint32_t pk = pkg[k];
+      uint64_t msk = __ballot(valid);
+      int off = 0, pw = 1;
+      for (int l = 1; l <= depth; ++l) {
+        uint32_t cell = 0;
+        for (int c = 0; c < nch; ++c) {
+          const uint32_t top = ((pk >> (8 * c)) & 0xFFu) >> (8 - l);
+          const uint64_t v = __ballot((top & 1u) != 0);
+          msk &= (top & 1u) ? v : ~v;
+          cell |= top << (l * c);
+        }
+        if (valid && (msk & below) == 0) {  // the lowest lane of its cell
+          const int q = off + (int)cell;
+          const int32_t n = s_cnt[q];
+          if (n == 0) s_first[q] = (uint32_t)r;
+          s_cnt[q] = n + __popcll(msk);
+        }
+        pw *= kids;
+        off += pw;
       }
-      if (valid && (msk & below) == 0) {  // the lowest lane of its cell
-        const int k = off + (int)cell;
-        const int32_t n = s_cnt[k];
-        if (n == 0) s_first[k] = (uint32_t)r;
-        s_cnt[k] = n + __popcll(msk);
-      }
-      pw *= kids;
-      off += pw;
     }
   }
   __syncthreads();
@@ -1126,6 +1142,9 @@ __host__ __device__ constexpr int64_t dense_scatter_lds(int64_t tile, int64_t cm
   return 4 * cmax + 2 * 4 * cmax + 2 * tile + tile;
 }
 
+// SP: the pair indices only, for an image whose root chain is summed serially
+// (k_fold_record reads them); the first pass leaves them out.
+template <bool SP>
 __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* __restrict__ fr,
                                                                    Map m,
                                                                    uint32_t* __restrict__ spair_all) {
@@ -1133,6 +1152,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
   __shared__ int s_part[4];
   const TileAt tl = map_tile(m, xcd_block(blockIdx.x, gridDim.x));
   const Frame& d = fr[tl.f];
+  if (SP && (d.info[6] != 0 || !d.info[14])) return;
   const int depth = d.depth, kids = d.kids, assoc = d.assoc, nch = assoc ? 4 : 3;
   int cmax = 1;
   for (int l = 1; l <= depth; ++l) cmax *= kids;
@@ -1192,6 +1212,13 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
   auto emit = [&](int l, const uint16_t* map) {
     for (int i = tid; i < n; i += kScatterThreads) {
       const int64_t r = r0 + map[i];
+      if (SP) {
+        uint32_t cell = 0;
+        const uint32_t pk = pack[r];
+        for (int c = 0; c < nch; ++c) cell |= (((pk >> (8 * c)) & 0xFFu) >> (8 - l)) << (l * c);
+        spair[s_delta[cell] + i] = l1 + (uint32_t)((int64_t)(l - 1) * n2 + (r - R1));
+        continue;
+      }
       const uint32_t pk = pack[r];
       const float4 f = d.px_pack ? make_float4((float)(pk & 0xFFu) * 257.0f,
                                                (float)((pk >> 8) & 0xFFu) * 257.0f,
@@ -1218,7 +1245,6 @@ __global__ __launch_bounds__(kScatterThreads) void k_dense_scatter(const Frame* 
       const double e3 = assoc ? kScale * (a[3] - mid[3]) : 0.0;
       double dd = e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3;
       if (dd != dd) dd = 0.0;
-      spair[li] = l1 + (uint32_t)((int64_t)(l - 1) * n2 + (r - R1));
       T.term[li] = (double)cnt * sqrt(dd);
       if (l == depth) {
         T.lcnt[li] = cnt;
@@ -1346,6 +1372,12 @@ __device__ __forceinline__ int32_t loop2_start(const uint32_t* spair, int32_t b,
   return lo;
 }
 
+// The same for node n of an image: one placed by cell knows it per node (its
+// loop-2 pairs carry no pair index unless the exact root chain needs one).
+__device__ __forceinline__ int32_t seg_loop2(const Frame& d, int64_t n, int32_t b, int32_t e) {
+  return d.dense ? min(d.l2s[n], e) : loop2_start(d.spair, b, e, (uint32_t)(8 * d.R1));
+}
+
 // One leaf term of total_color, as the reference forms it: count * scale * value.
 __device__ __forceinline__ double tot_term(int32_t cnt, float v) {
   return (double)cnt * kScale * (double)v;
@@ -1370,8 +1402,6 @@ __global__ void k_fold_short(const Frame* __restrict__ fr, Map m, Batch B) {
   if (n > d.M || !d.ok) return;
   const TreeDev& t = d.t;
   const Terms& T = d.T;
-  const uint32_t* spair = d.spair;
-  const int64_t R1 = d.R1;
   const int depth = d.depth;
   const int32_t b = d.seg_b[n], e = d.seg_e[n];
   const int level = t.level[n];
@@ -1386,8 +1416,7 @@ __global__ void k_fold_short(const Frame* __restrict__ fr, Map m, Batch B) {
       B.mid_list[atomicAdd(&B.ctl[L2 ? 2 : 1], 1)] = ent;
     return;
   }
-  const uint32_t l1 = (uint32_t)(8 * R1);
-  const int32_t s = loop2_start(spair, b, e, l1);
+  const int32_t s = seg_loop2(d, n, b, e);
   if (!L2) {
     double qe = 0.0, share = 0.0;
     for (int32_t i = b; i < e; ++i) {
@@ -1558,11 +1587,9 @@ __global__ __launch_bounds__(64) void k_fold_mid(const Frame* __restrict__ fr, B
     const int32_t n = (int32_t)(ent & 0xFFFFFFFFu);
     const TreeDev& t = d.t;
     const Terms& T = d.T;
-    const uint32_t* spair = d.spair;
-    const uint32_t l1 = (uint32_t)(8 * d.R1);
     const int32_t b = d.seg_b[n], e = d.seg_e[n];
     const int level = t.level[n];
-    const int32_t s = loop2_start(spair, b, e, l1);
+    const int32_t s = seg_loop2(d, n, b, e);
     if (!L2) {
       double qe, share;
       fold_qe_wave(T.term, b, e, s_buf, &qe, &share);
@@ -2089,10 +2116,9 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_long(const Frame* __restr
     const int32_t n = (int32_t)(ent & 0xFFFFFFFFu);
     const TreeDev& t = d.t;
     const Terms& T = d.T;
-    const uint32_t l1 = (uint32_t)(8 * d.R1);
     const int32_t b = d.seg_b[n], e = d.seg_e[n];
     const int level = t.level[n];
-    const int32_t s = loop2_start(d.spair, b, e, l1);
+    const int32_t s = seg_loop2(d, n, b, e);
     if (Mode == kFoldQe) {
       const double* term = T.term;
       double qe = 0.0;
@@ -2701,7 +2727,8 @@ __global__ void k_root_bound(const Frame* __restrict__ fr) {
   if (threadIdx.x == 0) {
     const double root_qe = ssum[0], mx = smax[0];
     const double err = (double)d.P * 2.3e-16 * fabs(root_qe) * 4.0;
-    const bool exact = !(root_qe - 1.0 - err > mx + 1e-9 * mx + 2e-12) || M == 0;
+    const bool exact =
+        d.exact_root || !(root_qe - 1.0 - err > mx + 1e-9 * mx + 2e-12) || M == 0;
     d.dinfo[0] = root_qe;
     d.dinfo[1] = mx;
     d.dinfo[2] = root_qe;
@@ -3230,11 +3257,14 @@ inline unsigned blocks(int64_t n, int per) { return (unsigned)((n + per - 1) / p
 // back through a pinned buffer kept per host thread.
 // ---------------------------------------------------------------------------------
 namespace {
-std::atomic<int> g_cells{1};  // the octree.cells knob
+std::atomic<int> g_cells{1};       // the octree.cells knob
+std::atomic<int> g_exact_root{0};  // the octree.exact_root knob
 }  // namespace
 
 void octree_set_cells(int on) { g_cells.store(on ? 1 : 0); }
 int octree_cells() { return g_cells.load(); }
+void octree_set_exact_root(int on) { g_exact_root.store(on ? 1 : 0); }
+int octree_exact_root() { return g_exact_root.load(); }
 
 int octree_depth(int maxc, bool assoc, bool gray) {  // rd_octree.cpp:255-275
   int depth = 1;
@@ -3464,6 +3494,7 @@ class SubBuild {
       d.fmt = q.fmt;
       d.assoc = q.assoc ? 1 : 0;
       d.px_pack = !d.assoc && (q.fmt == kRgb8 || q.fmt == kRgba8);
+      d.exact_root = g_exact_root.load();
       d.maxc = q.maxc <= 0 ? 65536 : std::min(q.maxc, 65536);
       d.depth = octree_depth(d.maxc, d.assoc != 0, q.gray != 0);
       d.kids = d.assoc ? 16 : 8;
@@ -3772,6 +3803,8 @@ class SubBuild {
     uint32_t* dpos = nullptr;
     c.want(&vlen, sumDC_);
     c.want(&dpos, sumDC_);
+    int32_t* l2s = nullptr;
+    c.want(&l2s, sumNodes);
     const int64_t ntiles = (sumP + kExpandTile - 1) / kExpandTile;
     int32_t* tile_first = nullptr;
     c.want(&tile_first, ntiles);
@@ -3814,6 +3847,7 @@ class SubBuild {
       d.T.lcnt = lcnt + pb;
       d.record = (double*)pp + pb;
       d.dpos = dpos;
+      d.l2s = l2s + nb;
       if (d.ok) {
         int32_t** wp[11] = {&d.W.lvl, &d.W.att, &d.W.anc, &d.W.size, &d.W.gsz, &d.W.start,
                             &d.W.gstart, &d.W.pre, &d.W.merged, &d.W.arr, &d.W.arr2};
@@ -3833,6 +3867,7 @@ class SubBuild {
     fill(t.color_number, sumNodes, 0u);
     fill(seg_b, sumNodes, 0);
     fill(seg_e, sumNodes, 0);
+    if (sumDC_) fill(l2s, sumNodes, (int32_t)INT32_MAX);
     k_root_init<<<F_, 64, 0, s_>>>(d_fr_);
     if (grid_[kMapM256]) k_node_assign<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256), B_);
     // Pairs into node order through their intervals: the intervals' first pairs
@@ -3857,8 +3892,8 @@ class SubBuild {
     k_expand<<<(unsigned)ntiles, 256, 0, s_>>>(d_fr_, F_, sumP, NT, tile_first, sikey, sival,
                                                istart, spos, spair);
     if (grid_[kMapDT])
-      k_dense_scatter<<<grid_[kMapDT], kScatterThreads, max_scatter_lds_, s_>>>(d_fr_, map(kMapDT),
-                                                                            spair);
+      k_dense_scatter<false><<<grid_[kMapDT], kScatterThreads, max_scatter_lds_, s_>>>(
+          d_fr_, map(kMapDT), spair);
     if (grid_[kMapM256]) k_links<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256));
     // At most sumP / kShortSeg segments are medium, sumP / kMidSeg long.
     const int mid_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 8192);
@@ -3872,6 +3907,9 @@ class SubBuild {
     k_fold_long<kFoldTot2><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
     k_fold_mid<true><<<mid_grid, 64, 0, s_>>>(d_fr_, B_);
     k_root_bound<<<F_, 256, 0, s_>>>(d_fr_);
+    if (grid_[kMapDT])  // pair indices of the cell-placed images whose root is summed serially
+      k_dense_scatter<true><<<grid_[kMapDT], kScatterThreads, max_scatter_lds_, s_>>>(
+          d_fr_, map(kMapDT), spair);
     if (grid_[kMapM64]) k_fold_record<<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64));
     k_root_exact<<<F_, 64, 0, s_>>>(d_fr_);
     k_reduce<<<F_, kTreeThreads, 0, s_>>>(d_fr_);

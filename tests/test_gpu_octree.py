@@ -344,3 +344,27 @@ def test_octree_cells_large_tiles(gpu, oracle, colours, ch):
     _KEEP.append(src)
     _compare(gpu, oracle, src, H.fmt_of(img), 1920, 1080, synth.widen_q16(img), colours, ch == 4,
              False)
+
+
+def test_octree_cells_exact_root(gpu, oracle):
+    """With the root's chain always summed serially (octree.exact_root), images
+    whose loop 2 is placed by cell need their pairs' indices after all (they
+    are written late, for those images only): every node, the root included,
+    equals the oracle's bit for bit."""
+    rng = np.random.default_rng(31)
+    cases = []
+    for w, h, colours, ch in [(64, 48, 16, 3), (200, 150, 16, 3), (37, 91, 32, 4), (120, 80, 8, 3),
+                              (300, 200, 64, 3), (96, 64, 4, 4)]:
+        img = synth.sinusoid_image(w, h, rng, channels=ch)
+        img = np.clip(img.astype(np.int16) + rng.integers(-6, 7, img.shape), 0, 255)
+        cases.append((img.astype(np.uint8), colours))
+    assert gpu.tune_get("octree.exact_root") == 0
+    try:
+        gpu.tune_set("octree.exact_root", 1)
+        pals = _build_mix(gpu, cases)
+    finally:
+        gpu.tune_set("octree.exact_root", 0)
+    for (img, colours), pal in zip(cases, pals):
+        say("  item", img.shape, colours)
+        assert pal.info()["exact_root"] == 1
+        _check_built(oracle, pal, synth.widen_q16(img), colours, img.shape[2] == 4, False)
