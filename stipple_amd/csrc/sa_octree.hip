@@ -212,6 +212,7 @@ struct Frame {
 struct Batch {
   int32_t* ctl;            // [0] node entries; medium [1] [2], long [3] [4], huge [5] [6]
                            // segments of the loop-1 / loop-2 folds; [8 + FoldMode] queue heads
+                           // of the long folds, [13] [14] of the medium ones
   uint32_t* ekey;          // node entries: pair_base + first pair
   uint32_t* eslot;         // node entries: hash slot
   const uint32_t* skey;    // the same, sorted
@@ -1459,7 +1460,8 @@ __global__ void k_fold_short(const Frame* __restrict__ fr, Map m, Batch B) {
   }
 }
 
-constexpr int kChunk = 512;           // terms staged per step
+constexpr int kChunk = 256;           // terms staged per step (a smaller stage keeps more
+                                      // wavefronts resident: 512 was 0.3 ms slower on C3)
 constexpr int kPerLane = kChunk / 64;
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1576,12 +1578,19 @@ __device__ void fold_tot_wave(const Terms& T, int32_t b, int32_t e, double* s_p,
   *nu += nsum;
 }
 
-// Medium segments: one wavefront each, the chain folded serially.
+// Medium segments: one wavefront each, the chain folded serially; taken from a
+// queue, so the long ones do not gather on a few workgroups.
 template <bool L2>
 __global__ __launch_bounds__(64) void k_fold_mid(const Frame* __restrict__ fr, Batch B) {
   __shared__ double s_buf[2 * 4 * kChunk];
+  __shared__ int32_t s_item;
   const int32_t items = B.ctl[L2 ? 2 : 1];
-  for (int32_t it = blockIdx.x; it < items; it += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&B.ctl[L2 ? 14 : 13], 1);
+    __syncthreads();
+    const int32_t it = s_item;
+    __syncthreads();
+    if (it >= items) break;
     const uint64_t ent = B.mid_list[it];
     const Frame& d = fr[(int32_t)(ent >> 32)];
     const int32_t n = (int32_t)(ent & 0xFFFFFFFFu);
