@@ -1639,7 +1639,11 @@ __global__ __launch_bounds__(64) void k_fold_mid(const Frame* __restrict__ fr, B
 // that step is done as the plain IEEE addition it is.  The result equals the
 // serial loop bit for bit; the serial work left is one addition per binade
 // crossing or tie, and a short serial prologue where crossings are dense.
-constexpr int kFoldThreads = 512;
+// Workgroups of 256 threads with 4 terms each per window (1024-term windows):
+// smaller and more numerous than 512 x 14, which ran the 32 x 1080p folds
+// 0.8 ms slower -- half the windows hold a binade crossing, and a small
+// window's resolution rounds are cheap.
+constexpr int kFoldThreads = 256;
 constexpr int kFoldWaves = kFoldThreads / 64;
 constexpr int kFoldPrologue = 512;                 // terms added serially first
 constexpr int kFoldRounds = 8;                     // special steps resolved per window in
@@ -2089,7 +2093,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
 
 // LDS stage of one window (plus one record of padding per K): kQeK records
 // per thread of quantize_error terms, kTotK per thread of leaf records.
-constexpr int kQeK = 14;
+constexpr int kQeK = 4;
 constexpr int kTotK = 2;
 constexpr int kQeStage = kFoldThreads * (kQeK + 1) * (int)sizeof(double);
 constexpr int kTotStage = kFoldThreads * (kTotK + 1) * (int)sizeof(TotRaw);
@@ -3906,7 +3910,7 @@ class SubBuild {
     if (grid_[kMapM256]) k_links<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256));
     // At most sumP / kShortSeg segments are medium, sumP / kMidSeg long.
     const int mid_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 8192);
-    const int long_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kMidSeg, 1), 512);
+    const int long_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kMidSeg, 1), 2048);
     if (grid_[kMapM64]) k_fold_short<false><<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64), B_);
     k_fold_long<kFoldQe><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
     k_fold_long<kFoldTot1><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
