@@ -2250,6 +2250,22 @@ __device__ int64_t block_sum(int64_t v, TreeShared& S) {
   return s;
 }
 
+// Exclusive prefix of v over the workgroup, in thread order.
+__device__ int64_t block_excl_i64(int64_t v, TreeShared& S) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int64_t incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) S.part_i[wave] = incl;
+  __syncthreads();
+  int64_t before = 0;
+  for (int k = 0; k < wave; ++k) before += S.part_i[k];
+  __syncthreads();
+  return before + incl - v;
+}
+
 __device__ double block_min(double v, TreeShared& S) {
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
   if ((threadIdx.x & 63) == 0) S.part_d[threadIdx.x >> 6] = v;
@@ -2492,15 +2508,33 @@ __device__ double select_kth(const Tree& T, TreeShared& S, int64_t L, int64_t k,
     }
     if (threadIdx.x == 0 && prefix == 0 && zeros > 0) atomicAdd(&S.hist[0], (uint32_t)zeros);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      uint64_t kk = S.sel[2], cum = 0;
-      uint64_t d = 0;
-      for (; d < dm; ++d) {
-        if (cum + S.hist[d] > kk) break;
-        cum += S.hist[d];
-      }
+    // The first digit d whose cumulative count passes kk (the last digit, dm,
+    // when none before it does), by a prefix over the bins: thread t holds
+    // bins 2t and 2t + 1.
+    static_assert(kTreeThreads * 2 == 2048, "two bins per thread");
+    const uint64_t kk = S.sel[2];
+    const int b0 = 2 * (int)threadIdx.x;
+    const uint64_t h0 = S.hist[b0], h1 = S.hist[b0 + 1];
+    const int64_t before = block_excl_i64(h0 + h1, S);
+    const uint64_t c0 = (uint64_t)before, c1 = c0 + h0;
+    // Bin b below dm is the one when its range [lo, lo + h) holds kk; dm is
+    // when kk lies past every bin before it.  Exactly one bin qualifies.
+    const int idm = (int)dm;
+    auto hit = [&](int b, uint64_t lo, uint64_t h) {
+      return b < idm ? (kk >= lo && kk < lo + h) : (b == idm && kk >= lo);
+    };
+    int64_t d = -1;
+    uint64_t cum = 0;
+    if (hit(b0, c0, h0)) {
+      d = b0;
+      cum = c0;
+    } else if (hit(b0 + 1, c1, h1)) {
+      d = b0 + 1;
+      cum = c1;
+    }
+    if (d >= 0) {
       S.sel[2] = kk - cum;
-      S.sel[0] = prefix | (d << sh);
+      S.sel[0] = prefix | ((uint64_t)d << sh);
       S.sel[1] = mask | (dm << sh);
     }
     __syncthreads();
