@@ -206,6 +206,7 @@ struct Frame {
   uint32_t* dslot;         // per cell: the hash slot of its node
   const uint32_t* dpos;    // per cell: first sorted position (batch-wide) of its loop-2 pairs
   int32_t* l2s;            // per node: first sorted position of its loop-2 pairs (INT32_MAX: none)
+  int32_t exp1;            // its k_compact intervals are expanded one per thread (k_expand_short)
 };
 
 // Batch-wide arrays and counters.
@@ -824,6 +825,7 @@ __device__ __forceinline__ void expand_one(const Frame& d, int64_t li, uint32_t 
 // to its offset -- that interval's node, and the pair its first pair plus the
 // offset into it.
 __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, int F, int64_t sum_p,
+                                                const int32_t* __restrict__ tiles,
                                                 int64_t ni, const int32_t* __restrict__ tile_first,
                                                 const int32_t* __restrict__ sikey,
                                                 const uint32_t* __restrict__ sival,
@@ -838,9 +840,10 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
   __shared__ int32_t s_before[kWords];  // marks in the words before
   __shared__ int s_f;
   __shared__ int64_t s_next;
-  const int64_t i0 = (int64_t)blockIdx.x * kExpandTile;
+  const int64_t tile = tiles ? tiles[blockIdx.x] : blockIdx.x;  // null: every tile
+  const int64_t i0 = tile * kExpandTile;
   const int64_t i1 = i0 + kExpandTile < sum_p ? i0 + kExpandTile : sum_p;
-  const int64_t j0 = tile_first[blockIdx.x];
+  const int64_t j0 = tile_first[tile];
   // A tile inside one cell's interval has nothing to expand.
   if (istart[sival[j0]] == kVirtual && (j0 + 1 >= ni || spos[j0 + 1] >= i1)) return;
   if (threadIdx.x < kWords) s_mark[threadIdx.x] = 0;
@@ -889,14 +892,32 @@ __global__ __launch_bounds__(256) void k_expand(const Frame* __restrict__ fr, in
     const int k = s_before[o >> 5] + __popc(s_mark[o >> 5] & below);
     if (l_first[k] == kVirtual) continue;  // placed by k_dense_scatter
     const int64_t g = (int64_t)l_first[k] + (i - l_pos[k]);
-    if (one_image) {
-      const Frame& d = fr[f0];
-      expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[k], spair);
-    } else {
-      const Frame& d = fr[image_of_pair(fr, F, i)];
-      expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[k], spair);
-    }
+    const Frame& d = fr[one_image ? f0 : image_of_pair(fr, F, i)];
+    if (d.exp1) continue;  // expanded by k_expand_short
+    expand_one(d, i - d.pair_base, (uint32_t)(g - d.pair_base), l_node[k], spair);
   }
+}
+
+// The k_compact intervals of an image whose loop 1 is short (placed by cell,
+// a few rows at most): one thread per sorted interval writes its pairs, so
+// the sorted range of such images needs no tile at all.
+__global__ void k_expand_short(const Frame* __restrict__ fr, int F, int64_t ni, int64_t nt,
+                               const int32_t* __restrict__ sikey,
+                               const uint32_t* __restrict__ sival,
+                               const uint32_t* __restrict__ istart,
+                               const int32_t* __restrict__ spos,
+                               const int32_t* __restrict__ slen, uint32_t* __restrict__ spair) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nt) return;
+  const int64_t k = sival[j];
+  if (k >= ni) return;  // a cell's interval (k_dense_scatter)
+  const int64_t b = spos[j];
+  const Frame& d = fr[image_of_pair(fr, F, b)];
+  if (!d.exp1) return;
+  const int64_t g = istart[k];
+  const int32_t node = sikey[j];
+  for (int32_t q = 0; q < slen[j]; ++q)
+    expand_one(d, b - d.pair_base + q, (uint32_t)(g - d.pair_base + q), node, spair);
 }
 
 // ---- 3b. loop 2 placed by cell -------------------------------------------------------
@@ -3655,6 +3676,7 @@ class SubBuild {
         d.dtshift = sh;
         d.dtiles = (int32_t)((d.R - d.R1 + ((int64_t)1 << sh) - 1) >> sh);
         d.P_ins = 8 * d.R1;
+        d.exp1 = d.P_ins <= 65536;  // loop 1 of a few rows: its intervals one per thread
       }
       d.ncap = pow2_at_least(2 * bound_nodes + 64);
       d.pair_base = sumP;
@@ -3855,6 +3877,10 @@ class SubBuild {
     const int64_t ntiles = (sumP + kExpandTile - 1) / kExpandTile;
     int32_t* tile_first = nullptr;
     c.want(&tile_first, ntiles);
+    int32_t* tiles = nullptr;  // the tiles k_expand takes
+    c.want(&tiles, ntiles);
+    bool any_exp1 = false;
+    for (int f = 0; f < F_; ++f) any_exp1 |= H_[f].P > 0 && H_[f].exp1;
     c.want(&seg_b, sumNodes);
     c.want(&seg_e, sumNodes);
     c.want(&term, sumP);
@@ -3936,8 +3962,29 @@ class SubBuild {
     if (e != hipSuccess) return e;
     k_interval_ends<<<gi, 256, 0, s_>>>(d_fr_, F_, sikey, sival, spos, slen, NI, NT,
                                         (int32_t)sumNodes, dpos, tile_first);
-    k_expand<<<(unsigned)ntiles, 256, 0, s_>>>(d_fr_, F_, sumP, NT, tile_first, sikey, sival,
-                                               istart, spos, spair);
+    // Tiles of the sorted range that touch an image not expanded per interval
+    // (every tile when no image is).
+    std::vector<int32_t> tl;
+    for (int f = 0; f < F_ && any_exp1; ++f) {
+      const Frame& d = H_[f];
+      if (d.P == 0 || d.exp1) continue;
+      const int64_t t0 = d.pair_base / kExpandTile, t1 = (d.pair_base + d.P - 1) / kExpandTile;
+      for (int64_t t = std::max<int64_t>(t0, tl.empty() ? 0 : tl.back() + 1); t <= t1; ++t)
+        tl.push_back((int32_t)t);
+    }
+    if (!any_exp1) {
+      k_expand<<<(unsigned)ntiles, 256, 0, s_>>>(d_fr_, F_, sumP, nullptr, NT, tile_first, sikey,
+                                                 sival, istart, spos, spair);
+    } else if (!tl.empty()) {
+      if ((e = hipMemcpyAsync(tiles, tl.data(), sizeof(int32_t) * tl.size(), hipMemcpyHostToDevice,
+                              s_)) != hipSuccess)
+        return e;
+      k_expand<<<(unsigned)tl.size(), 256, 0, s_>>>(d_fr_, F_, sumP, tiles, NT, tile_first, sikey,
+                                                    sival, istart, spos, spair);
+    }
+    if (any_exp1)
+      k_expand_short<<<gi, 256, 0, s_>>>(d_fr_, F_, NI, NT, sikey, sival, istart, spos, slen,
+                                          spair);
     if (grid_[kMapDT])
       k_dense_scatter<false><<<grid_[kMapDT], kScatterThreads, max_scatter_lds_, s_>>>(
           d_fr_, map(kMapDT), spair);
