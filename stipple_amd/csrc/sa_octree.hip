@@ -47,6 +47,7 @@
 // 6, k_rows); the tree logic and the colormap are then the same kernels'.
 #include <hipcub/hipcub.hpp>
 #include <rocprim/block/block_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -3347,6 +3348,13 @@ int octree_depth(int maxc, bool assoc, bool gray) {  // rd_octree.cpp:255-275
 
 namespace {
 
+// The two sorts of the build (node entries by first pair, intervals by node)
+// hold a few hundred thousand keys on a video batch; rocprim's default sorts
+// that many by a block sort and eleven merge passes (22 launches each).
+// Onesweep at every size above one block: a pass per 8 key bits.
+using SortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                           rocprim::default_config, 0>;
+
 // Images are grouped into sub-batches of at most this many pixels (a larger
 // image is a sub-batch of its own), which bounds the scratch of one phase and
 // keeps every batch-wide index within 32 bits.
@@ -3819,11 +3827,11 @@ class SubBuild {
     const int64_t NT = NI + sumDC_;  // and one per cell of the images placed by cell
     const int bitsA = bits_for(sumP), bitsB = bits_for(sumNodes + 1);
     size_t tmpA = 0, tmpB = 0, tmpSel = 0, tmpSum = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpA, B_.ekey, (uint32_t*)nullptr,
-                                                      B_.eslot, (uint32_t*)nullptr, (int)E, 0,
-                                                      bitsA, s_);
+    hipError_t e = rocprim::radix_sort_pairs<SortCfg>(nullptr, tmpA, B_.ekey, (uint32_t*)nullptr,
+                                                       B_.eslot, (uint32_t*)nullptr, (int)E, 0,
+                                                       bitsA, s_);
     if (e != hipSuccess) return e;
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpB, (int32_t*)nullptr, (int32_t*)nullptr,
+    e = rocprim::radix_sort_pairs<SortCfg>(nullptr, tmpB, (int32_t*)nullptr, (int32_t*)nullptr,
                                            (uint32_t*)nullptr, (uint32_t*)nullptr, (int)NT, 0,
                                            bitsB, s_);
     if (e != hipSuccess) return e;
@@ -3932,7 +3940,7 @@ class SubBuild {
     if ((e = upload()) != hipSuccess) return e;
     fill(B_.ctl, kCtl, 0);
     if (E > 0) {
-      e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpA, B_.ekey, skey, B_.eslot, sslot, (int)E, 0,
+      e = rocprim::radix_sort_pairs<SortCfg>(tmp, tmpA, B_.ekey, skey, B_.eslot, sslot, (int)E, 0,
                                              bitsA, s_);
       if (e != hipSuccess) return e;
     }
@@ -3953,7 +3961,7 @@ class SubBuild {
     if (grid_[kMapDC])
       k_dense_virtual<<<grid_[kMapDC], 256, 0, s_>>>(d_fr_, map(kMapDC), NI, (int32_t)sumNodes,
                                                     istart, ikey, ival, vlen);
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmpB, ikey, sikey, ival, sival, (int)NT, 0, bitsB,
+    e = rocprim::radix_sort_pairs<SortCfg>(tmp, tmpB, ikey, sikey, ival, sival, (int)NT, 0, bitsB,
                                            s_);
     if (e != hipSuccess) return e;
     const unsigned gi = (unsigned)blocks(NT, 256);
