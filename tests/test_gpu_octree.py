@@ -320,6 +320,14 @@ def test_octree_cells_knob_same_trees(gpu):
         img = synth.sinusoid_image(700, 400, rng, channels=ch)
         noisy = np.clip(img.astype(np.int16) + rng.integers(-6, 7, img.shape), 0, 255)
         cases.append((noisy.astype(np.uint8), colours))
+    # Loop 1 long (three colours over the first 240 rows of single-pixel runs),
+    # then noise: placed by cell, with loop 1's intervals expanded in tiles.
+    for colours, ch in [(16, 3), (8, 4)]:
+        img = synth.sinusoid_image(700, 400, rng, channels=ch)
+        img = np.clip(img.astype(np.int16) + rng.integers(-6, 7, img.shape), 0, 255)
+        lut = rng.integers(0, 256, (3, ch))
+        img[:240] = lut[(np.arange(700)[None, :] + np.arange(240)[:, None]) % 3]
+        cases.append((img.astype(np.uint8), colours))
     say("octree cells knob", len(cases))
     assert gpu.tune_get("octree.cells") == 1
     pals1 = _build_mix(gpu, cases)
@@ -368,3 +376,20 @@ def test_octree_cells_exact_root(gpu, oracle):
         say("  item", img.shape, colours)
         assert pal.info()["exact_root"] == 1
         _check_built(oracle, pal, synth.widen_q16(img), colours, img.shape[2] == 4, False)
+
+
+@pytest.mark.parametrize("colours,ch", [(16, 3), (8, 4)])
+def test_octree_cells_long_loop1(gpu, oracle, colours, ch):
+    """Loop 1 covers 240 rows of three colours in single-pixel runs (over 8192
+    of its pairs), loop 2 noise: loop 2 placed by cell, loop 1's intervals
+    expanded in tiles; every node equals the oracle's."""
+    rng = np.random.default_rng(40 + colours)
+    img = synth.sinusoid_image(700, 400, rng, channels=ch)
+    img = np.clip(img.astype(np.int16) + rng.integers(-6, 7, img.shape), 0, 255)
+    lut = rng.integers(0, 256, (3, ch))
+    img[:240] = lut[(np.arange(700)[None, :] + np.arange(240)[:, None]) % 3]
+    img = img.astype(np.uint8)
+    src = H.to_device(img)
+    _KEEP.append(src)
+    _compare(gpu, oracle, src, H.fmt_of(img), 700, 400, synth.widen_q16(img), colours, ch == 4,
+             False)
