@@ -934,12 +934,14 @@ __global__ void k_expand_short(const Frame* __restrict__ fr, int F, int64_t ni, 
 //                    node numbering is unchanged;
 //   k_dense_virtual  one interval per cell stands for its pairs in the interval
 //                    sort, after the cell's loop-1 intervals;
-//   k_dense_scatter  a second pass over the tiles writes every pair, its term
-//                    and leaf value at its sorted position.
+//   k_dense_scatter  a second pass over the tiles writes every pair's term and
+//                    leaf value at its sorted position (its pair index only
+//                    when the exact root chain needs it: a late pass).
 // Inside a tile a wavefront takes 64 consecutive runs at a time and ranks each
 // lane among the lanes with its cell (a lane mask refined by one ballot per key
 // bit and level); the lowest such lane then advances the cell's count, so a
-// cell's pairs keep run order.
+// cell's pairs keep run order.  The folds find a node's first loop-2 pair from
+// l2s rather than from pair indices.
 constexpr int kDenseCellsMax = 4680;  // 8 + 64 + 512 + 4096
 
 // Workgroups are dealt to the 8 XCDs in turn; this numbering gives each XCD a
@@ -1154,9 +1156,9 @@ __device__ __forceinline__ uint32_t dense_cell(uint32_t pk, int l, int nch, uint
 // needs no ballots.  The pairs are then written slot by slot, so consecutive
 // threads write consecutive positions of one cell's segment (the cell's first
 // global position for this tile minus its first slot, plus the slot).  The
-// values are expand_one's, the cube midpoints formed level by level as the
-// classification descends; a run's count is written for leaf pairs only (the
-// only ones whose count is read).
+// values are expand_one's (the cube midpoints in closed form, equal to the
+// classification's level-by-level sums); a run's count is written for leaf
+// pairs only (the only ones whose count is read).
 // Levels are taken one at a time, so the LDS holds the counters of one
 // level's cells, one map and one rank byte per run: dense_scatter_lds.
 constexpr int kScatterThreads = 256;
