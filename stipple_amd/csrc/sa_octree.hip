@@ -7,8 +7,9 @@
 // reference's order.
 //
 //   1. Runs.  quantize.c collapses runs of identical adjacent pixels per row and
-//      weights them by their length (rd_octree.cpp:305-313).  A flag per pixel,
-//      an exclusive scan, and one record per run.
+//      weights them by their length (rd_octree.cpp:305-313).  Run starts are
+//      counted per tile of pixels, a scan over tiles numbers them, and a second
+//      pass writes one record per run.
 //   2. Loop-1 extent.  Loop 1 classifies rows to depth 8 until the number of
 //      distinct depth-8 leaves exceeds the palette size; the row where that
 //      happens is found by one workgroup inserting runs in order into a hash set
@@ -24,7 +25,9 @@
 //      share a node at a level form an interval of consecutive pairs; a stable
 //      radix sort of the intervals by node, expanded, puts each node's pairs in
 //      run order (at low levels an interval spans many runs, so there are far
-//      fewer intervals than pairs); every pair's term is computed in parallel,
+//      fewer intervals than pairs) -- or, when the cube depth leaves few cells,
+//      loop 2's pairs are placed by a counting sort over cells (section 3b);
+//      every pair's term is computed in parallel,
 //      then each node's chain is folded in order over contiguous memory.  The
 //      totals of loop 2 are folded after
 //      PruneToCubeDepth has merged loop 1's deep leaves, exactly when the
