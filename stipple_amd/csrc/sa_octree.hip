@@ -279,10 +279,12 @@ __global__ void k_frame_init(const Frame* __restrict__ fr) {
 }
 
 // Runs are found a tile of kRunTile pixels at a time (one 256-thread workgroup
-// per tile; a tile never spans two images): k_run_tiles counts the run starts
-// of every tile, an exclusive scan over the batch's tiles gives each tile's
-// first run, and k_run_emit writes every run's record, its length too unless
-// it is the tile's last run, whose length k_run_tail takes from the next start.
+// per tile, each wavefront a quarter of it; a tile never spans two images):
+// k_run_tiles counts the run starts of every quarter, an exclusive scan over
+// the batch's quarters gives each quarter's first run, and k_run_emit writes
+// every run's record, its length too unless it is its quarter's last run,
+// whose length k_run_tail takes from the next start.  The wavefronts need no
+// workgroup synchronisation.
 constexpr int kRunTile = 4096;
 
 // The run's pixel as loaded.  An 8-bit source read without association keeps
@@ -311,110 +313,91 @@ __device__ __forceinline__ bool run_start(const Frame& d, int64_t p, bool valid,
   return valid && (row0 || !same_px(*v, u));
 }
 
+constexpr int kRunQuarter = kRunTile / 4;  // pixels of one wavefront
+
 __global__ __launch_bounds__(256) void k_run_tiles(const Frame* __restrict__ fr, Map m,
                                                    int32_t* __restrict__ tcnt) {
-  __shared__ int s_w[4];
   const TileAt tl = map_tile(m, blockIdx.x);
   const Frame& d = fr[tl.f];
-  const int64_t p0 = tl.t * kRunTile;
+  const int wave = threadIdx.x >> 6;
+  const int64_t p0 = tl.t * kRunTile + wave * kRunQuarter;
   int c = 0;
-  for (int k = 0; k < kRunTile / 256; ++k) {
-    const int64_t p = p0 + k * 256 + threadIdx.x;
+  for (int k = 0; k < kRunQuarter / 64; ++k) {
+    const int64_t p = p0 + k * 64 + (threadIdx.x & 63);
     float4 v;
-    if (run_start(d, p, p < d.N, &v)) ++c;
+    c += __popcll(__ballot(run_start(d, p, p < d.N, &v)));
   }
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) tcnt[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  if ((threadIdx.x & 63) == 0) tcnt[4 * (int64_t)blockIdx.x + wave] = c;
 }
 
-// info[12] <- R, from the scan over the image's tiles.
+// info[12] <- R, from the scan over the image's quarters.
 __global__ void k_run_count(const Frame* __restrict__ fr, Map m, const int32_t* __restrict__ tcnt,
                             const int32_t* __restrict__ toff) {
   const Frame& d = fr[blockIdx.x];
   if (threadIdx.x != 0) return;
-  const int64_t t0 = m.first[blockIdx.x], t1 = m.first[blockIdx.x + 1];
-  d.info[12] = t1 > t0 ? toff[t1 - 1] + tcnt[t1 - 1] - toff[t0] : 0;
+  const int64_t q0 = 4 * m.first[blockIdx.x], q1 = 4 * m.first[blockIdx.x + 1];
+  d.info[12] = q1 > q0 ? toff[q1 - 1] + tcnt[q1 - 1] - toff[q0] : 0;
 }
 
-// Every run start of the tile, in pixel order: its start, row, pack, (when
-// kept) pixel, and its length when the next start is in the tile.  The starts
-// are ranked 256 pixels at a time; a start whose successor is not in its
-// chunk is resolved by the next chunk with a start (thread 0 holds it).
+// Every run start of the wavefront's quarter, in pixel order: its start, row,
+// pack, (when kept) pixel, and its length when the next start is in the
+// quarter.  64 pixels at a time; a start whose successor is not among them is
+// resolved by the next round with a start.  For an 8-bit source read without
+// association the pack is the source bytes (scale_char of a byte times 257 is
+// the byte), alpha 255 for RGB.
 __global__ __launch_bounds__(256) void k_run_emit(const Frame* __restrict__ fr, Map m,
                                                   const int32_t* __restrict__ toff) {
-  __shared__ int s_w[2][4];   // starts per wavefront
-  __shared__ int s_fl[2][4];  // first start's lane (64: none)
-  __shared__ int s_ll[2][4];  // last start's lane (-1: none)
   const TileAt tl = map_tile(m, blockIdx.x);
   const Frame& d = fr[tl.f];
-  const int64_t p0 = tl.t * kRunTile;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int64_t base = toff[blockIdx.x] - toff[m.first[tl.f]];
-  int64_t pend_r = -1, pend_p = 0;  // thread 0: the start waiting for its successor
-  for (int k = 0; k < kRunTile / 256; ++k) {
-    const int64_t pc = p0 + k * 256;
-    const int64_t p = pc + threadIdx.x;
+  const int64_t q = 4 * (int64_t)blockIdx.x + wave;
+  const int64_t p0 = tl.t * kRunTile + wave * kRunQuarter;
+  int64_t base = toff[q] - toff[4 * m.first[tl.f]];
+  int64_t pend_r = -1, pend_p = 0;  // the start waiting for its successor (uniform)
+  for (int k = 0; k < kRunQuarter / 64; ++k) {
+    const int64_t pc = p0 + 64 * k;
+    const int64_t p = pc + lane;
     float4 v;
     const bool st = run_start(d, p, p < d.N, &v);
     const uint64_t bal = __ballot(st);
-    const int b = k & 1;
-    if (lane == 0) {
-      s_w[b][wave] = __popcll(bal);
-      s_fl[b][wave] = bal ? __ffsll((unsigned long long)bal) - 1 : 64;
-      s_ll[b][wave] = bal ? 63 - __clzll((long long)bal) : -1;
-    }
-    __syncthreads();
-    int before = 0, all = 0, first = 256, last = -1;
-    for (int w = 0; w < 4; ++w) {
-      const int c = s_w[b][w];
-      before += w < wave ? c : 0;
-      all += c;
-      if (first == 256 && s_fl[b][w] < 64) first = 64 * w + s_fl[b][w];
-      if (s_ll[b][w] >= 0) last = 64 * w + s_ll[b][w];
-    }
+    if (!bal) continue;
     if (st) {
-      const int64_t r = base + before + __popcll(bal & ((1ull << lane) - 1));
-      double a[4];
-      assoc_pixel(d.assoc != 0, v.x, v.y, v.z, v.w, a);
-      if (!d.px_pack) d.px[r] = v;
-      const uint32_t c0 = scale_char(clamp_pixel(a[0])), c1 = scale_char(clamp_pixel(a[1]));
-      const uint32_t c2 = scale_char(clamp_pixel(a[2])), c3 = scale_char(clamp_pixel(a[3]));
-      d.pack[r] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+      const int64_t r = base + __popcll(bal & ((1ull << lane) - 1));
+      uint32_t pk;
+      if (d.px_pack) {
+        pk = (uint32_t)v.x / 257u | ((uint32_t)v.y / 257u) << 8 | ((uint32_t)v.z / 257u) << 16 |
+             ((uint32_t)v.w / 257u) << 24;
+      } else {
+        double a[4];
+        assoc_pixel(d.assoc != 0, v.x, v.y, v.z, v.w, a);
+        d.px[r] = v;
+        const uint32_t c0 = scale_char(clamp_pixel(a[0])), c1 = scale_char(clamp_pixel(a[1]));
+        const uint32_t c2 = scale_char(clamp_pixel(a[2])), c3 = scale_char(clamp_pixel(a[3]));
+        pk = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+      }
+      d.pack[r] = pk;
       d.start[r] = (int32_t)p;
       d.row[r] = (int32_t)(p / d.w);
-      int nxt = 256;  // the next start in this chunk
       const uint64_t above = bal & ~((2ull << lane) - 1);
-      if (above) {
-        nxt = 64 * wave + __ffsll((unsigned long long)above) - 1;
-      } else {
-        for (int w = wave + 1; w < 4; ++w)
-          if (s_fl[b][w] < 64) {
-            nxt = 64 * w + s_fl[b][w];
-            break;
-          }
-      }
-      if (nxt < 256) d.count[r] = nxt - (int)threadIdx.x;
+      if (above) d.count[r] = __ffsll((unsigned long long)above) - 1 - lane;
     }
-    if (threadIdx.x == 0 && first < 256) {
-      if (pend_r >= 0) d.count[pend_r] = (int32_t)(pc + first - pend_p);
-      pend_r = base + all - 1;
-      pend_p = pc + last;
-    }
-    base += all;
+    const int first = __ffsll((unsigned long long)bal) - 1, last = 63 - __clzll((long long)bal);
+    if (lane == 0 && pend_r >= 0) d.count[pend_r] = (int32_t)(pc + first - pend_p);
+    pend_r = base + __popcll(bal) - 1;
+    pend_p = pc + last;
+    base += __popcll(bal);
   }
 }
 
-// The length of each tile's last run: up to the next run's start, or to the
-// end of the image.
+// The length of each quarter's last run: up to the next run's start, or to
+// the end of the image.
 __global__ void k_run_tail(const Frame* __restrict__ fr, Map m, const int32_t* __restrict__ tcnt,
-                           const int32_t* __restrict__ toff, int64_t nt) {
+                           const int32_t* __restrict__ toff, int64_t nq) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nt || tcnt[b] == 0) return;
-  const TileAt tl = map_tile(m, b);
+  if (b >= nq || tcnt[b] == 0) return;
+  const TileAt tl = map_tile(m, b / 4);
   const Frame& d = fr[tl.f];
-  const int64_t r = toff[b] + tcnt[b] - 1 - toff[m.first[tl.f]];
+  const int64_t r = toff[b] + tcnt[b] - 1 - toff[4 * m.first[tl.f]];
   const int64_t e = r + 1 < d.R ? d.start[r + 1] : d.N;
   d.count[r] = (int32_t)(e - d.start[r]);
 }
@@ -3548,8 +3531,8 @@ class SubBuild {
   hipError_t phase_runs() {
     hipError_t e = staging(up_bytes() + rb_bytes(), &pin_);
     if (e != hipSuccess) return e;
-    int64_t sumT = 0;
-    for (int f = 0; f < F_; ++f) sumT += (rq_[f].w * rq_[f].h + kRunTile - 1) / kRunTile;
+    int64_t sumT = 0;  // quarters of run tiles
+    for (int f = 0; f < F_; ++f) sumT += 4 * ((rq_[f].w * rq_[f].h + kRunTile - 1) / kRunTile);
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
     e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, tcnt_, toff_, (int)sumT, s_);
@@ -3639,8 +3622,8 @@ class SubBuild {
     if ((e = upload()) != hipSuccess) return e;
     if (grid_[kMapNT]) {
       k_run_emit<<<grid_[kMapNT], 256, 0, s_>>>(d_fr_, map(kMapNT), toff_);
-      k_run_tail<<<blocks(grid_[kMapNT], 256), 256, 0, s_>>>(d_fr_, map(kMapNT), tcnt_, toff_,
-                                                            grid_[kMapNT]);
+      k_run_tail<<<blocks(4 * grid_[kMapNT], 256), 256, 0, s_>>>(d_fr_, map(kMapNT), tcnt_, toff_,
+                                                                4 * grid_[kMapNT]);
     }
     fill(bkeys, sumB, kEmpty64);
     fill(bfirst, sumB, kEmpty32);
@@ -4209,7 +4192,7 @@ class SubBuild {
   }
 
   int64_t sumP_ = 0;
-  int32_t* tcnt_ = nullptr;        // run starts per tile of kRunTile pixels
+  int32_t* tcnt_ = nullptr;        // run starts per quarter of a tile of kRunTile pixels
   int32_t* toff_ = nullptr;        // their exclusive scan over the batch
   int32_t* slot_node_ = nullptr;
   uint32_t* pair_slot_ = nullptr;  // batch pair array: slot of an interval's first pair
