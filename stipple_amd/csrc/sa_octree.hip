@@ -913,7 +913,7 @@ __global__ void k_expand_short(const Frame* __restrict__ fr, int F, int64_t ni, 
 // node hash and the interval sort.  A cell (level l, the top l bits of each
 // channel byte) is a node, and its pairs in run order are a counting sort of
 // loop 2's runs by cell:
-//   k_dense_count    one wavefront per tile of runs counts every cell's pairs
+//   k_dense_count    one workgroup per tile of runs counts every cell's pairs
 //                    and notes each cell's first run;
 //   k_dense_scan     an exclusive scan over tiles: each tile's offset in a cell;
 //   k_dense_insert   each cell enters the node hash with its first pair, so
@@ -938,61 +938,67 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
   return x * q + (x < rem ? x : rem) + b / 8;
 }
 
-__global__ __launch_bounds__(64) void k_dense_count(const Frame* __restrict__ fr, Map m) {
+// Four wavefronts per tile, each a quarter of its runs (held in registers);
+// their counts are summed at the end, the first runs kept by an LDS minimum.
+// LDS: dense_count_lds.
+__host__ __device__ constexpr int64_t dense_count_lds(int64_t dcells) {
+  return 2 * 4 * dcells + 4 * dcells;
+}
+
+__global__ __launch_bounds__(256) void k_dense_count(const Frame* __restrict__ fr, Map m) {
   extern __shared__ int32_t s_dyn[];
   const TileAt tl = map_tile(m, xcd_block(blockIdx.x, gridDim.x));
   const Frame& d = fr[tl.f];
   const int dcells = d.dcells;
-  int32_t* s_cnt = s_dyn;
-  uint32_t* s_first = (uint32_t*)(s_dyn + dcells);
-  const int lane = threadIdx.x;
-  for (int c = lane; c < dcells; c += 64) s_cnt[c] = 0;
+  uint32_t* s_first = (uint32_t*)s_dyn;             // dcells
+  uint16_t* s_cnt = (uint16_t*)(s_dyn + dcells);    // 4 x dcells
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = threadIdx.x; c < 4 * dcells; c += 256) s_cnt[c] = 0;
+  for (int c = threadIdx.x; c < dcells; c += 256) s_first[c] = kEmpty32;
   __syncthreads();
-  const int64_t r0 = d.R1 + (tl.t << d.dtshift);
-  const int64_t r1 = min(d.R, r0 + ((int64_t)1 << d.dtshift));
+  const int64_t tile = (int64_t)1 << d.dtshift;
+  const int64_t r0 = d.R1 + tl.t * tile, r1 = min(d.R, r0 + tile);
+  const int64_t wr0 = r0 + wave * (tile / 4), wr1 = min(r1, wr0 + tile / 4);
   const int depth = d.depth, kids = d.kids, nch = d.assoc ? 4 : 3;
   const uint64_t below = (1ull << lane) - 1;
-  const uint32_t* pack = d.pack;
-  constexpr int kAhead = 8;  // rounds whose runs are loaded together
-  for (int64_t rg = r0; rg < r1; rg += 64 * kAhead) {
-    uint32_t pkg[kAhead];
+  uint16_t* cw = s_cnt + wave * dcells;
+  constexpr int kMaxRounds = 32;  // a quarter of the largest tile (8192) over 64 lanes
+  uint32_t pk[kMaxRounds];
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      const int64_t r = rg + 64 * k + lane;
-      pkg[k] = r < r1 ? pack[r] : 0u;
-    }
+  for (int k = 0; k < kMaxRounds; ++k) {
+    const int64_t r = wr0 + 64 * k + lane;
+    pk[k] = r < wr1 ? d.pack[r] : 0u;
+  }
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      const int64_t rb = rg + 64 * k;
-      if (rb >= r1) break;
-      const int64_t r = rb + lane;
-      const bool valid = r < r1;
-      const uint32_t pk = pkg[k];
-      uint64_t msk = __ballot(valid);
-      int off = 0, pw = 1;
-      for (int l = 1; l <= depth; ++l) {
-        uint32_t cell = 0;
-        for (int c = 0; c < nch; ++c) {
-          const uint32_t top = ((pk >> (8 * c)) & 0xFFu) >> (8 - l);
-          const uint64_t v = __ballot((top & 1u) != 0);
-          msk &= (top & 1u) ? v : ~v;
-          cell |= top << (l * c);
-        }
-        if (valid && (msk & below) == 0) {  // the lowest lane of its cell
-          const int q = off + (int)cell;
-          const int32_t n = s_cnt[q];
-          if (n == 0) s_first[q] = (uint32_t)r;
-          s_cnt[q] = n + __popcll(msk);
-        }
-        pw *= kids;
-        off += pw;
+  for (int k = 0; k < kMaxRounds; ++k) {
+    const int64_t rb = wr0 + 64 * k;
+    if (rb >= wr1) break;
+    const int64_t r = rb + lane;
+    const bool valid = r < wr1;
+    uint64_t msk = __ballot(valid);
+    int off = 0, pw = 1;
+    for (int l = 1; l <= depth; ++l) {
+      uint32_t cell = 0;
+      for (int c = 0; c < nch; ++c) {
+        const uint32_t top = ((pk[k] >> (8 * c)) & 0xFFu) >> (8 - l);
+        const uint64_t v = __ballot((top & 1u) != 0);
+        msk &= (top & 1u) ? v : ~v;
+        cell |= top << (l * c);
       }
+      if (valid && (msk & below) == 0) {  // the lowest lane of its cell
+        const int q = off + (int)cell;
+        const int n = cw[q];
+        if (n == 0) atomicMin(&s_first[q], (uint32_t)r);
+        cw[q] = (uint16_t)(n + __popcll(msk));
+      }
+      pw *= kids;
+      off += pw;
     }
   }
   __syncthreads();
   int32_t* tab = d.dtab + tl.t * (int64_t)dcells;
-  for (int c = lane; c < dcells; c += 64) {
-    const int32_t n = s_cnt[c];
+  for (int c = threadIdx.x; c < dcells; c += 256) {
+    const int32_t n = s_cnt[c] + s_cnt[dcells + c] + s_cnt[2 * dcells + c] + s_cnt[3 * dcells + c];
     tab[c] = n;
     if (n) atomicMin(&d.dfirst[c], s_first[c]);
   }
@@ -3764,7 +3770,7 @@ class SubBuild {
     }
     if (grid_[kMapDT]) {  // loop 2 of the images placed by cell
       fill(dfirst, sumDC, kEmpty32);
-      k_dense_count<<<grid_[kMapDT], 64, 8 * max_dc_, s_>>>(d_fr_, map(kMapDT));
+      k_dense_count<<<grid_[kMapDT], 256, dense_count_lds(max_dc_), s_>>>(d_fr_, map(kMapDT));
       k_dense_scan<<<grid_[kMapDC], 256, 0, s_>>>(d_fr_, map(kMapDC));
       k_dense_insert<<<grid_[kMapDC], 256, 0, s_>>>(d_fr_, map(kMapDC));
     }
