@@ -1666,15 +1666,17 @@ constexpr int kFoldRounds = 8;                     // special steps resolved per
                                                    // parallel; the rest of it serially
 constexpr double kFoldSat = 0x1p54;                // saturation of the integer scan
 
-struct FoldShared {
-  double part[2][kFoldWaves][4];                   // scan partials (alternating passes)
-  int32_t first[2][kFoldWaves][4];                 // min partials
-  double dpart[kFoldWaves];
-  uint64_t upart[kFoldWaves];
+template <int NW>  // wavefronts of the workgroup
+struct FoldSharedT {
+  double part[2][NW][4];                           // scan partials (alternating passes)
+  int32_t first[2][NW][4];                         // min partials
+  double dpart[NW];
+  uint64_t upart[NW];
   double Sk[4];
   double tk[4];
   double ser[4];
 };
+using FoldShared = FoldSharedT<kFoldWaves>;
 
 // Wavefront sums through DPP: quad permutes and row rotates give every lane
 // its 16-lane row's sum in four steps; two shuffles join the rows.
@@ -1712,22 +1714,24 @@ __device__ __forceinline__ int wave_or(int v) {
   return v | __shfl_xor(v, 32, 64);
 }
 
-__device__ double fold_sum_d(double v, FoldShared& sh) {  // any order (approximate sums)
+template <int NW>
+__device__ double fold_sum_d(double v, FoldSharedT<NW>& sh) {  // any order (approximate sums)
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if ((threadIdx.x & 63) == 0) sh.dpart[threadIdx.x >> 6] = v;
   __syncthreads();
   double m = 0.0;
-  for (int k = 0; k < kFoldWaves; ++k) m += sh.dpart[k];
+  for (int k = 0; k < NW; ++k) m += sh.dpart[k];
   __syncthreads();
   return m;
 }
 
-__device__ uint64_t fold_sum_u64(uint64_t v, FoldShared& sh) {
+template <int NW>
+__device__ uint64_t fold_sum_u64(uint64_t v, FoldSharedT<NW>& sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if ((threadIdx.x & 63) == 0) sh.upart[threadIdx.x >> 6] = v;
   __syncthreads();
   uint64_t m = 0;
-  for (int k = 0; k < kFoldWaves; ++k) m += sh.upart[k];
+  for (int k = 0; k < NW; ++k) m += sh.upart[k];
   __syncthreads();
   return m;
 }
@@ -1767,18 +1771,18 @@ __device__ __forceinline__ int32_t stage_slot(int32_t p) {
   return p + p / K;
 }
 
-template <int K, class Src>
+template <int K, class Src, int NT = kFoldThreads>
 __device__ void stage_in(const Src& src, int32_t from, int32_t to, typename Src::Raw* stage) {
   typename Src::Raw r[K];
-  const int32_t n = to - from;  // <= K * kFoldThreads
+  const int32_t n = to - from;  // <= K * NT
 #pragma unroll
   for (int j = 0; j < K; ++j) {  // every load in flight before the first store
-    const int32_t p = (int32_t)threadIdx.x + j * kFoldThreads;
+    const int32_t p = (int32_t)threadIdx.x + j * NT;
     if (p < n) r[j] = src.load(from + p);
   }
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    const int32_t p = (int32_t)threadIdx.x + j * kFoldThreads;
+    const int32_t p = (int32_t)threadIdx.x + j * NT;
     if (p < n) stage[stage_slot<K, Src>(p)] = r[j];
   }
 }
@@ -1813,11 +1817,11 @@ __device__ double serial_lds(double v, const Src& src, const typename Src::Raw* 
 //
 // It also returns, summed in any order, the side sum src.side over [b, e)
 // (each index counted once: `seen` marks how far it has been taken).
-template <int NC, int K, class Src>
-__device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, FoldShared& sh,
+template <int NC, int K, class Src, int NT = kFoldThreads>
+__device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, FoldSharedT<NT / 64>& sh,
                              typename Src::Raw* stage) {
   using Raw = typename Src::Raw;
-  constexpr int32_t kWin = kFoldThreads * K;
+  constexpr int32_t kWin = NT * K;
   static_assert(kFoldPrologue <= kWin, "the prologue fits the stage");
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t me = (int32_t)threadIdx.x;
@@ -1826,7 +1830,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
   int32_t seen = pro;
   if (pro > b) {  // the first terms serially: binade crossings come densely there
     __syncthreads();
-    stage_in<K>(src, b, pro, stage);
+    stage_in<K, Src, NT>(src, b, pro, stage);
     __syncthreads();
     if (me < NC) {
       double v = 0.0;
@@ -1835,7 +1839,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
         if (k == me) v = s[k];
       sh.ser[me] = serial_lds<K>(v, src, stage, 0, pro - b, me);
     }
-    for (int32_t p = me; p < pro - b; p += kFoldThreads)
+    for (int32_t p = me; p < pro - b; p += NT)
       side += src.side(stage[stage_slot<K, Src>(p)], (double)(e - (b + p)));
     __syncthreads();
 #pragma unroll
@@ -1846,13 +1850,13 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
   for (int c = 0; c < NC; ++c) pos[c] = pro;
   int par = 0;
   // Window records, two buffers in turn: one summed while the next loads.
-  // Thread t holds records W + t + j * kFoldThreads (coalesced loads; the
+  // Thread t holds records W + t + j * NT (coalesced loads; the
   // index is clamped so that every load is in bounds and none waits on a branch).
   Raw bufA[K], bufB[K];
   int32_t wA = -1, wB = -1;
   auto fetch = [&](Raw (&buf)[K], int32_t& wb, int32_t w) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) buf[j] = src.load(min(w + me + j * kFoldThreads, e - 1));
+    for (int j = 0; j < K; ++j) buf[j] = src.load(min(w + me + j * NT, e - 1));
     wb = w;
   };
   auto pass = [&](Raw (&cur)[K], int32_t& wc, Raw (&nxt)[K], int32_t& wn) -> bool {
@@ -1865,7 +1869,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
     if (W >= e) return false;
     const int32_t wend = min(e, W + kWin);
     if (W > seen) {  // serial stretches passed [seen, W): their side sums, in parallel
-      for (int32_t i = seen + me; i < W; i += kFoldThreads)
+      for (int32_t i = seen + me; i < W; i += NT)
         side += src.side(src.load(i), (double)(e - i));
       seen = W;
     }
@@ -1900,7 +1904,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
       const double ei0 = (double)(e - (W + me));  // e - i, exact integers in a double
 #pragma unroll
       for (int j = 0; j < K; ++j) {
-        const int32_t i = W + me + j * kFoldThreads;
+        const int32_t i = W + me + j * NT;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           const bool act = i < e && i >= pos[c];
@@ -1909,7 +1913,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
           tie |= fabs(q - r) == 0.5 ? 1 << c : 0;  // inf or NaN: the total fails below
           acc[c][j & 1] += r;
         }
-        if (i < wend && i >= seen) side += src.side(cur[j], ei0 - (double)(j * kFoldThreads));
+        if (i < wend && i >= seen) side += src.side(cur[j], ei0 - (double)(j * NT));
       }
       seen = max(seen, wend);
       // Integers of at most 2^54 each (or inf / NaN, which fail the test
@@ -1927,11 +1931,11 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
     __syncthreads();
     bool clean = true;
     int ties = 0;
-    for (int k = 0; k < kFoldWaves; ++k) ties |= sh.first[par][k][0];
+    for (int k = 0; k < (NT / 64); ++k) ties |= sh.first[par][k][0];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       double tot = 0.0;
-      for (int k = 0; k < kFoldWaves; ++k) tot += sh.part[par][k][c];
+      for (int k = 0; k < (NT / 64); ++k) tot += sh.part[par][k][c];
       total[c] = fmin(tot, kFoldSat);
       if ((ties >> c) & 1 || !(S[c] + tot < 0x1p53 - 1.0)) clean = false;
     }
@@ -1951,7 +1955,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
     // as the IEEE addition it is, go on from there on the new grid).  The next
     // window is then the one already loading.
 #pragma unroll
-    for (int j = 0; j < K; ++j) stage[stage_slot<K, Src>(me + j * kFoldThreads)] = cur[j];
+    for (int j = 0; j < K; ++j) stage[stage_slot<K, Src>(me + j * NT)] = cur[j];
     __syncthreads();
     const int32_t base = W + me * K;
     for (int round = 0;; ++round) {
@@ -2035,7 +2039,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         double before = 0.0, all = 0.0;
-        for (int k = 0; k < kFoldWaves; ++k) {
+        for (int k = 0; k < (NT / 64); ++k) {
           const double v = sh.part[par][k][c];
           if (k < wv) before = fmin(before + v, kFoldSat);
           all = fmin(all + v, kFoldSat);
@@ -2072,7 +2076,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         int32_t m = sh.first[par][0][c];
-        for (int k = 1; k < kFoldWaves; ++k) m = min(m, sh.first[par][k][c]);
+        for (int k = 1; k < (NT / 64); ++k) m = min(m, sh.first[par][k][c]);
         first[c] = m;
         if (m == INT32_MAX) {
           if (pos[c] < wend) {  // the rest of the window, none special
@@ -2102,7 +2106,7 @@ __device__ double exact_sums(double* s, int32_t b, int32_t e, const Src& src, Fo
     if (!pass(bufB, wB, bufA, wA)) break;
   }
   if (seen < e)  // the last serial stretch reached e
-    for (int32_t i = seen + me; i < e; i += kFoldThreads)
+    for (int32_t i = seen + me; i < e; i += NT)
       side += src.side(src.load(i), (double)(e - i));
   return fold_sum_d(side, sh);
 }
@@ -2945,6 +2949,8 @@ __device__ __forceinline__ void rows_fold(const RowsArgs& a, int l, int k, int n
   }
 }
 
+constexpr int kRootK = 8;  // root-chain terms per thread and window (8192 per window)
+
 __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw, int64_t c0,
                            int n, int limit, int colors_level, int64_t& M, int64_t& live,
                            int64_t& colors, double& root_qe) {
@@ -3159,30 +3165,16 @@ __device__ void rows_chunk(const RowsArgs& a, TreeShared& S, unsigned char* raw,
   }
   __syncthreads();
   // The root's chain: run by run, level by level (rd_octree.cpp:341): the
-  // running values lie in that order, limit per run; one thread adds them,
-  // 16 loaded ahead of their additions.
-  double* sroot = (double*)&S.scal[1];
-  if (tid == 0) {
-    double q = root_qe;
-    const int total = n * limit;
-    const double2* rv2 = (const double2*)a.rv;
-    int p = 0;
-    for (; p + 16 <= total; p += 16) {
-      double2 r[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) r[k] = rv2[p / 2 + k];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        q += r[k].x;
-        q += r[k].y;
-      }
-    }
-    for (; p < total; ++p) q += a.rv[p];
-    *sroot = q;
-  }
-  __syncthreads();
-  root_qe = *sroot;
-  __syncthreads();
+  // running values lie in that order, limit per run.  Summed exactly in
+  // parallel (exact_sums: the running values are finite and non-negative),
+  // its windows staged in the chunk's LDS, which is free by now.
+  __shared__ FoldSharedT<kTreeWaves> fsh;
+  static_assert(kTreeThreads * (kRootK + 1) * sizeof(double) <= 4 * kRowChunk + 16 * kRowHash,
+                "the root chain's stage fits the chunk's LDS");
+  double q = root_qe;
+  (void)exact_sums<1, kRootK, QeSrc, kTreeThreads>(&q, 0, n * limit, QeSrc{a.rv}, fsh,
+                                                   (double*)raw);
+  root_qe = q;
 }
 
 __global__ __launch_bounds__(kTreeThreads) void k_rows(RowsArgs a) {
