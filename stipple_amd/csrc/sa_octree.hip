@@ -2416,17 +2416,26 @@ __device__ __forceinline__ int32_t post_of(const TreeWork& W, int32_t y) {
   return W.start[y] + W.size[y] - 1;
 }
 
-// One PruneToCubeDepth or Reduce walk.  Updates *live; for Reduce returns the
-// colour count and the next threshold (seeded with root qe - 1).
-__device__ void tree_pass(const Tree& T, TreeShared& S, int mode, double thr, int64_t* live,
-                          int64_t* count, double* next) {
-  const TreeDev& t = T.t;
+// Merges a walk makes with `live` nodes: every attempt when they fit within
+// live - maxc, else that many.
+__device__ int64_t tree_merges(const Tree& T, TreeShared& S, int64_t live, int64_t* attempts) {
   const TreeWork& W = T.W;
-  tree_shape(T, S, mode, thr);
   int64_t mine = 0;
   for (int64_t y = threadIdx.x; y <= T.M; y += kTreeThreads) mine += (W.att[y] && W.anc[y] >= 0);
-  const int64_t attempts = block_sum(mine, S);
-  const int64_t K = *live - T.maxc;
+  *attempts = block_sum(mine, S);
+  const int64_t K = live - T.maxc;
+  return K > 0 ? min(*attempts, K) : 0;
+}
+
+// The first half of a PruneToCubeDepth or Reduce walk: the shape of the tree and
+// which nodes merge (W.merged).  Reads only the links and levels (and qe for
+// Reduce); returns the number of merges.
+__device__ int64_t tree_mark(const Tree& T, TreeShared& S, int mode, double thr, int64_t live) {
+  const TreeWork& W = T.W;
+  tree_shape(T, S, mode, thr);
+  int64_t attempts = 0;
+  const int64_t merged = tree_merges(T, S, live, &attempts);
+  const int64_t K = live - T.maxc;
   const bool cut = K > 0 && attempts > K;
   if (cut) {
     const int32_t A = W.size[0];
@@ -2450,7 +2459,16 @@ __device__ void tree_pass(const Tree& T, TreeShared& S, int mode, double thr, in
     W.merged[y] = m;
   }
   __syncthreads();
-  const int64_t merged = K > 0 ? min(attempts, K) : 0;
+  return merged;
+}
+
+// The second half: the merges W.merged marks, bottom-up.  Updates *live; for
+// Reduce returns the colour count and the next threshold (seeded with root
+// qe - 1).
+__device__ void tree_merge(const Tree& T, TreeShared& S, int mode, int64_t merged, int64_t* live,
+                           int64_t* count, double* next) {
+  const TreeDev& t = T.t;
+  const TreeWork& W = T.W;
   int64_t cnt = 0;
   double mn = INFINITY;
   for (int l = 8; l >= 0; --l) {
@@ -2500,6 +2518,13 @@ __device__ void tree_pass(const Tree& T, TreeShared& S, int mode, double thr, in
     *count = block_sum(cnt, S);
     *next = fmin(block_min(mn, S), t.qe[0] - 1.0);
   }
+}
+
+// One PruneToCubeDepth or Reduce walk.
+__device__ void tree_pass(const Tree& T, TreeShared& S, int mode, double thr, int64_t* live,
+                          int64_t* count, double* next) {
+  const int64_t merged = tree_mark(T, S, mode, thr, *live);
+  tree_merge(T, S, mode, merged, live, count, next);
 }
 
 // The k-th smallest (0-based) of {qe of attached nodes with pre-order < L}
@@ -2562,22 +2587,37 @@ __device__ double select_kth(const Tree& T, TreeShared& S, int64_t L, int64_t k,
   return __longlong_as_double((long long)S.sel[0]);
 }
 
+// PruneToCubeDepth's first half (rd_octree.cpp:370-372): which nodes merge
+// depends on the links and levels only, so it runs beside the folds that sum
+// loop 1's values.  Also the level lists every later walk reads.
+__global__ __launch_bounds__(kTreeThreads) void k_prune_mark(const Frame* __restrict__ fr) {
+  const Frame& d = fr[blockIdx.x];
+  if (!d.ok || d.info[6] != 0) return;
+  const Tree T{d.t, d.W, d.kids, d.maxc, d.depth, d.M};
+  __shared__ TreeShared S;
+  level_lists(T, S);
+  if (threadIdx.x < 10) T.W.lvl_off[threadIdx.x] = S.off[threadIdx.x];
+  if (d.info[0] < d.info[5]) (void)tree_mark(T, S, kPruneCube, 0.0, 1 + (int64_t)d.info[4]);
+}
+
+// The second half, once loop 1's sums are folded: the merges themselves.
 __global__ __launch_bounds__(kTreeThreads) void k_prune_cube(const Frame* __restrict__ fr) {
   const Frame& d = fr[blockIdx.x];
   if (!d.ok || d.info[6] != 0) return;
   int32_t* info = d.info;
   const Tree T{d.t, d.W, d.kids, d.maxc, d.depth, d.M};
   __shared__ TreeShared S;
-  level_lists(T, S);
-  if (threadIdx.x < 10) T.W.lvl_off[threadIdx.x] = S.off[threadIdx.x];
+  if (threadIdx.x < 10) S.off[threadIdx.x] = T.W.lvl_off[threadIdx.x];
+  __syncthreads();
   int64_t live = 1 + (int64_t)info[4];
   // Loop 1's MaxQNodes test (rd_octree.cpp:366): the nodes loop 1 creates before
   // its last row are bounded by its node total.
   if (threadIdx.x == 0 && live > kMaxQNodes) info[6] = 1;
   if (info[0] < info[5]) {  // loop 1 broke: PruneToCubeDepth (rd_octree.cpp:370-372)
-    int64_t cnt = 0;
+    int64_t attempts = 0, cnt = 0;
     double nx = 0.0;
-    tree_pass(T, S, kPruneCube, 0.0, &live, &cnt, &nx);
+    const int64_t merged = tree_merges(T, S, live, &attempts);
+    tree_merge(T, S, kPruneCube, merged, &live, &cnt, &nx);
   }
   // Loop 2's nodes join the live count; its MaxQNodes test (rd_octree.cpp:380)
   // is bounded by the total.
@@ -3412,6 +3452,57 @@ hipError_t staging(size_t bytes, char** out) {
   return hipSuccess;
 }
 
+// A fork of a stream onto a side stream (one per device, default priority,
+// kept for the process) and the join back: work launched on side() after the
+// fork sees everything before it on the origin; the origin's work after wait()
+// sees the side's work launched before join().  Without a side stream or
+// events, side() is the origin and both are no-ops.
+class Fork {
+ public:
+  explicit Fork(hipStream_t s) : s_(s) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevs) return;
+    {
+      std::lock_guard<std::mutex> lk(mu());
+      hipStream_t& st = sides()[dev];
+      if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+      side_ = st;
+    }
+    if (!side_) return;
+    if (hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&join_, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(fork_, s_) != hipSuccess || hipStreamWaitEvent(side_, fork_, 0) != hipSuccess) {
+      release();
+      side_ = nullptr;
+    }
+  }
+  ~Fork() { release(); }
+  Fork(const Fork&) = delete;
+  Fork& operator=(const Fork&) = delete;
+  hipStream_t side() const { return side_ ? side_ : s_; }
+  hipError_t join() { return side_ ? hipEventRecord(join_, side_) : hipGetLastError(); }
+  hipError_t wait() { return side_ ? hipStreamWaitEvent(s_, join_, 0) : hipSuccess; }
+
+ private:
+  static constexpr int kDevs = 64;
+  static std::mutex& mu() {
+    static std::mutex m;
+    return m;
+  }
+  static hipStream_t* sides() {
+    static hipStream_t st[kDevs] = {};
+    return st;
+  }
+  void release() {
+    if (fork_) (void)hipEventDestroy(fork_);
+    if (join_) (void)hipEventDestroy(join_);
+    fork_ = join_ = nullptr;
+  }
+  hipStream_t s_;
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
+};
+
 class SubBuild {
  public:
   SubBuild(const OctreeReq* rq, int F, hipStream_t s, OctreeOut* outs, OctreeResult* res)
@@ -3983,6 +4074,11 @@ class SubBuild {
       k_dense_scatter<false><<<grid_[kMapDT], kScatterThreads, max_scatter_lds_, s_>>>(
           d_fr_, map(kMapDT), spair);
     if (grid_[kMapM256]) k_links<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256));
+    // PruneToCubeDepth's marks need the links only: a side stream takes them
+    // while this one folds loop 1's sums (the same stream when none is free).
+    Fork fork(s_);
+    k_prune_mark<<<F_, kTreeThreads, 0, fork.side()>>>(d_fr_);
+    if ((e = fork.join()) != hipSuccess) return e;
     // At most sumP / kShortSeg segments are medium, sumP / kMidSeg long.
     const int mid_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 8192);
     const int long_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kMidSeg, 1), 2048);
@@ -3990,6 +4086,7 @@ class SubBuild {
     k_fold_long<kFoldQe><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
     k_fold_long<kFoldTot1><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
     k_fold_mid<false><<<mid_grid, 64, 0, s_>>>(d_fr_, B_);
+    if ((e = fork.wait()) != hipSuccess) return e;
     k_prune_cube<<<F_, kTreeThreads, 0, s_>>>(d_fr_);
     if (grid_[kMapM64]) k_fold_short<true><<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64), B_);
     k_fold_long<kFoldTot2><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
