@@ -2520,6 +2520,92 @@ __device__ void tree_merge(const Tree& T, TreeShared& S, int mode, int64_t merge
   }
 }
 
+// tree_merge with each node's children read together and their sums loaded
+// four at a time (more registers; the row-by-row kernel keeps tree_merge).
+__device__ void tree_merge_batched(const Tree& T, TreeShared& S, int mode, int64_t merged, int64_t* live,
+                                   int64_t* count, double* next) {
+  const TreeDev& t = T.t;
+  const TreeWork& W = T.W;
+  int64_t cnt = 0;
+  double mn = INFINITY;
+  for (int l = 8; l >= 0; --l) {
+    for_level(S, W, l, [&](int32_t y) {
+      if (!W.att[y]) return;
+      int32_t* cl = t.child + 16 * y;
+      // The children's links and marks are read together, then each merged
+      // child's sums four slots at a time: the walk's dependent loads per node
+      // are a few batches, not a chain per child.  The additions keep the
+      // reference's order (S children by slot, then the rest by slot).
+      uint32_t mS = 0, mR = 0;  // merged children by slot: S nodes, the rest
+      {
+        int32_t cs[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) cs[k] = k < T.kids ? cl[k] : -1;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const bool v = cs[k] >= 0;
+          const int32_t mg = v ? W.merged[cs[k]] : 0;
+          const int32_t an = v ? W.anc[cs[k]] : -1;
+          if (mg) (an == cs[k] ? mS : mR) |= 1u << k;
+        }
+      }
+      double tot[4];
+      for (int c = 0; c < 4; ++c) tot[c] = t.total[4 * y + c];
+      uint64_t nu = t.nu[y];
+      auto add4 = [&](uint32_t mask, int k0) {
+        uint64_t cn[4];
+        double ct[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const bool on = (mask >> (k0 + j)) & 1u;
+          const int32_t ch = on ? cl[k0 + j] : 0;
+          cn[j] = on ? t.nu[ch] : 0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) ct[j][c] = on ? t.total[4 * ch + c] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (!((mask >> (k0 + j)) & 1u)) continue;
+          nu += cn[j];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) tot[c] += ct[j][c];
+        }
+      };
+#pragma unroll 1
+      for (int k0 = 0; k0 < 16; k0 += 4)  // S children: merged at their own visit
+        if (mS >> k0 & 15u) add4(mS, k0);
+      const uint64_t nu_visit = nu;
+#pragma unroll 1
+      for (int k0 = 0; k0 < 16; k0 += 4)  // the rest: merged inside an ancestor's PruneChild
+        if (mR >> k0 & 15u) add4(mR, k0);
+      bool any = (mS | mR) != 0;
+      if (y == 0 && W.merged[0]) {  // the root's parent is the root itself
+        nu += nu;
+        for (int c = 0; c < 4; ++c) tot[c] += tot[c];
+        any = true;
+      }
+      if (any) {
+        for (int c = 0; c < 4; ++c) t.total[4 * y + c] = tot[c];
+        t.nu[y] = nu;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          if ((mS | mR) >> k & 1u) cl[k] = -1;
+        if (y == 0 && W.merged[0]) cl[t.id[0]] = -1;
+      }
+      if (mode == kReduce && W.anc[y] != y) {
+        if (nu_visit > 0) ++cnt;
+        mn = fmin(mn, t.qe[y]);
+      }
+    });
+    __syncthreads();
+  }
+  *live -= merged;
+  if (mode == kReduce) {
+    *count = block_sum(cnt, S);
+    *next = fmin(block_min(mn, S), t.qe[0] - 1.0);
+  }
+}
+
 // One PruneToCubeDepth or Reduce walk.
 __device__ void tree_pass(const Tree& T, TreeShared& S, int mode, double thr, int64_t* live,
                           int64_t* count, double* next) {
@@ -2617,7 +2703,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_prune_cube(const Frame* __rest
     int64_t attempts = 0, cnt = 0;
     double nx = 0.0;
     const int64_t merged = tree_merges(T, S, live, &attempts);
-    tree_merge(T, S, kPruneCube, merged, &live, &cnt, &nx);
+    tree_merge_batched(T, S, kPruneCube, merged, &live, &cnt, &nx);
   }
   // Loop 2's nodes join the live count; its MaxQNodes test (rd_octree.cpp:380)
   // is bounded by the total.
