@@ -2673,36 +2673,22 @@ __device__ double select_kth(const Tree& T, TreeShared& S, int64_t L, int64_t k,
   return __longlong_as_double((long long)S.sel[0]);
 }
 
-// PruneToCubeDepth's first half (rd_octree.cpp:370-372): which nodes merge
-// depends on the links and levels only, so it runs beside the folds that sum
-// loop 1's values.  Also the level lists every later walk reads.
-__global__ __launch_bounds__(kTreeThreads) void k_prune_mark(const Frame* __restrict__ fr) {
-  const Frame& d = fr[blockIdx.x];
-  if (!d.ok || d.info[6] != 0) return;
-  const Tree T{d.t, d.W, d.kids, d.maxc, d.depth, d.M};
-  __shared__ TreeShared S;
-  level_lists(T, S);
-  if (threadIdx.x < 10) T.W.lvl_off[threadIdx.x] = S.off[threadIdx.x];
-  if (d.info[0] < d.info[5]) (void)tree_mark(T, S, kPruneCube, 0.0, 1 + (int64_t)d.info[4]);
-}
-
-// The second half, once loop 1's sums are folded: the merges themselves.
 __global__ __launch_bounds__(kTreeThreads) void k_prune_cube(const Frame* __restrict__ fr) {
   const Frame& d = fr[blockIdx.x];
   if (!d.ok || d.info[6] != 0) return;
   int32_t* info = d.info;
   const Tree T{d.t, d.W, d.kids, d.maxc, d.depth, d.M};
   __shared__ TreeShared S;
-  if (threadIdx.x < 10) S.off[threadIdx.x] = T.W.lvl_off[threadIdx.x];
-  __syncthreads();
+  level_lists(T, S);
+  if (threadIdx.x < 10) T.W.lvl_off[threadIdx.x] = S.off[threadIdx.x];
   int64_t live = 1 + (int64_t)info[4];
   // Loop 1's MaxQNodes test (rd_octree.cpp:366): the nodes loop 1 creates before
   // its last row are bounded by its node total.
   if (threadIdx.x == 0 && live > kMaxQNodes) info[6] = 1;
   if (info[0] < info[5]) {  // loop 1 broke: PruneToCubeDepth (rd_octree.cpp:370-372)
-    int64_t attempts = 0, cnt = 0;
+    int64_t cnt = 0;
     double nx = 0.0;
-    const int64_t merged = tree_merges(T, S, live, &attempts);
+    const int64_t merged = tree_mark(T, S, kPruneCube, 0.0, live);
     tree_merge_batched(T, S, kPruneCube, merged, &live, &cnt, &nx);
   }
   // Loop 2's nodes join the live count; its MaxQNodes test (rd_octree.cpp:380)
@@ -3538,57 +3524,6 @@ hipError_t staging(size_t bytes, char** out) {
   return hipSuccess;
 }
 
-// A fork of a stream onto a side stream (one per device, default priority,
-// kept for the process) and the join back: work launched on side() after the
-// fork sees everything before it on the origin; the origin's work after wait()
-// sees the side's work launched before join().  Without a side stream or
-// events, side() is the origin and both are no-ops.
-class Fork {
- public:
-  explicit Fork(hipStream_t s) : s_(s) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevs) return;
-    {
-      std::lock_guard<std::mutex> lk(mu());
-      hipStream_t& st = sides()[dev];
-      if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
-      side_ = st;
-    }
-    if (!side_) return;
-    if (hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&join_, hipEventDisableTiming) != hipSuccess ||
-        hipEventRecord(fork_, s_) != hipSuccess || hipStreamWaitEvent(side_, fork_, 0) != hipSuccess) {
-      release();
-      side_ = nullptr;
-    }
-  }
-  ~Fork() { release(); }
-  Fork(const Fork&) = delete;
-  Fork& operator=(const Fork&) = delete;
-  hipStream_t side() const { return side_ ? side_ : s_; }
-  hipError_t join() { return side_ ? hipEventRecord(join_, side_) : hipGetLastError(); }
-  hipError_t wait() { return side_ ? hipStreamWaitEvent(s_, join_, 0) : hipSuccess; }
-
- private:
-  static constexpr int kDevs = 64;
-  static std::mutex& mu() {
-    static std::mutex m;
-    return m;
-  }
-  static hipStream_t* sides() {
-    static hipStream_t st[kDevs] = {};
-    return st;
-  }
-  void release() {
-    if (fork_) (void)hipEventDestroy(fork_);
-    if (join_) (void)hipEventDestroy(join_);
-    fork_ = join_ = nullptr;
-  }
-  hipStream_t s_;
-  hipStream_t side_ = nullptr;
-  hipEvent_t fork_ = nullptr, join_ = nullptr;
-};
-
 class SubBuild {
  public:
   SubBuild(const OctreeReq* rq, int F, hipStream_t s, OctreeOut* outs, OctreeResult* res)
@@ -4160,11 +4095,6 @@ class SubBuild {
       k_dense_scatter<false><<<grid_[kMapDT], kScatterThreads, max_scatter_lds_, s_>>>(
           d_fr_, map(kMapDT), spair);
     if (grid_[kMapM256]) k_links<<<grid_[kMapM256], 256, 0, s_>>>(d_fr_, map(kMapM256));
-    // PruneToCubeDepth's marks need the links only: a side stream takes them
-    // while this one folds loop 1's sums (the same stream when none is free).
-    Fork fork(s_);
-    k_prune_mark<<<F_, kTreeThreads, 0, fork.side()>>>(d_fr_);
-    if ((e = fork.join()) != hipSuccess) return e;
     // At most sumP / kShortSeg segments are medium, sumP / kMidSeg long.
     const int mid_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kShortSeg, 1), 8192);
     const int long_grid = (int)std::min<int64_t>(std::max<int64_t>(sumP / kMidSeg, 1), 2048);
@@ -4172,7 +4102,6 @@ class SubBuild {
     k_fold_long<kFoldQe><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
     k_fold_long<kFoldTot1><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
     k_fold_mid<false><<<mid_grid, 64, 0, s_>>>(d_fr_, B_);
-    if ((e = fork.wait()) != hipSuccess) return e;
     k_prune_cube<<<F_, kTreeThreads, 0, s_>>>(d_fr_);
     if (grid_[kMapM64]) k_fold_short<true><<<grid_[kMapM64], 64, 0, s_>>>(d_fr_, map(kMapM64), B_);
     k_fold_long<kFoldTot2><<<long_grid, kFoldThreads, 0, s_>>>(d_fr_, B_);
