@@ -2522,8 +2522,8 @@ __device__ void tree_merge(const Tree& T, TreeShared& S, int mode, int64_t merge
 
 // tree_merge with each node's children read together and their sums loaded
 // four at a time (more registers; the row-by-row kernel keeps tree_merge).
-__device__ void tree_merge_batched(const Tree& T, TreeShared& S, int mode, int64_t merged, int64_t* live,
-                                   int64_t* count, double* next) {
+__device__ void tree_merge_batched(const Tree& T, TreeShared& S, int mode, int64_t merged,
+                                   int64_t* live, int64_t* count, double* next) {
   const TreeDev& t = T.t;
   const TreeWork& W = T.W;
   int64_t cnt = 0;
